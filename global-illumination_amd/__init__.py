@@ -1,0 +1,330 @@
+"""Python host layer over libvgi.so (include/vgi.h): the MI355X voxel GI
+lighting hot path.
+
+`GpuTracer` mirrors the reference's transport seam `GpuTracer`
+(crates/bevy_voxel_gi/src/gpu.rs:309-1236): `new`, `enable_resident`,
+`submit_compact`, `snapshot_ready`, `complete_readback`, plus the scene input
+of `VoxelGiInput` (input.rs:54-112) and the optics of `VoxelGiMaterials`
+(world.rs:90-174). Errors raise `VoxelGiError` carrying the library's
+message, as the reference returns `Err(String)`.
+
+Everything runs in the HIP library; there is no CPU fallback. Loading fails
+loudly when libvgi.so is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvgi.so")
+
+AIR = 256
+NO_FACE = 0xFFFFFFFF
+RECORD_HITS = 1
+COUNT_RAYS = 2
+
+FACE_TASK = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("sequence", "<u4"), ("padding", "<u4", (3,))])
+assert FACE_TASK.itemsize == 32
+
+
+class VoxelGiError(RuntimeError):
+    pass
+
+
+class Light(ctypes.Structure):
+    _fields_ = [("position_range", ctypes.c_float * 4), ("energy", ctypes.c_float * 4), ("extra", ctypes.c_float * 4)]
+
+
+class Transport(ctypes.Structure):
+    _fields_ = [
+        ("bounces", ctypes.c_uint32),
+        ("samples", ctypes.c_uint32),
+        ("lanes", ctypes.c_uint32),
+        ("limited_reach", ctypes.c_uint32),
+        ("reach", ctypes.c_float),
+        ("max_distance", ctypes.c_float),
+        ("environment", ctypes.c_float * 3),
+        ("history_limit", ctypes.c_uint32),
+        ("history_recency", ctypes.c_float),
+        ("measured_retention", ctypes.c_float),
+    ]
+
+
+_lib = None
+
+
+def library():
+    """The loaded libvgi.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VoxelGiError("libvgi.so is not built: run __graft_entry__.build() (or global-illumination_amd/build.py)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32, i32, u64, i64, f32 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_float
+    sig = {
+        "vgi_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "vgi_destroy": (None, [P]),
+        "vgi_enable_resident": (ctypes.c_int, [P]),
+        "vgi_upload_pages": (ctypes.c_int, [P, P, u32, P, P, u32]),
+        "vgi_edit_voxels": (ctypes.c_int, [P, P, P, u32]),
+        "vgi_set_materials": (ctypes.c_int, [P, P, P]),
+        "vgi_set_lights": (ctypes.c_int, [P, P, u32, u32]),
+        "vgi_set_transport": (ctypes.c_int, [P, ctypes.POINTER(Transport)]),
+        "vgi_submit_faces": (ctypes.c_int, [P, P, u32, u32, ctypes.POINTER(u64)]),
+        "vgi_poll": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_int)]),
+        "vgi_read_faces": (ctypes.c_int, [P, u64, P, u32]),
+        "vgi_read_hits": (ctypes.c_int, [P, u64, P, u64]),
+        "vgi_read_counters": (ctypes.c_int, [P, u64, P, u32]),
+        "vgi_timings": (ctypes.c_int, [P, P, u32]),
+        "vgi_read_slots": (ctypes.c_int, [P, P, u32]),
+        "vgi_bind_faces": (ctypes.c_int, [P, P, u32]),
+        "vgi_run_frame": (ctypes.c_int, [P, u32]),
+        "vgi_sync": (ctypes.c_int, [P]),
+        "vgi_last_trace_ms": (f32, [P]),
+        "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
+        "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
+        "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
+        "vgi_scene_bounds": (ctypes.c_int, [P, P]),
+        "vgi_scene_region_canonical": (ctypes.c_int, [P, i32, i32, i32, P]),
+        "vgi_scene_stats": (i64, [P, P]),
+        "vgi_emitter_words": (i64, [P, P, i64]),
+        "vgi_last_error": (ctypes.c_char_p, []),
+        "vgi_scene_generate": (ctypes.c_int, [ctypes.c_int, u32, u32, P, P]),
+        "vgi_scene_optics": (ctypes.c_int, [ctypes.c_int, u32, P, P, P, P, P]),
+        "vgi_scene_faces": (i64, [P, P, ctypes.c_int, P, i64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names the C ABI declares (include/vgi.h)."""
+    return [
+        "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_upload_pages", "vgi_edit_voxels",
+        "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
+        "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
+        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_read_bound",
+        "vgi_cache_state", "vgi_apply_events", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
+        "vgi_scene_optics", "vgi_scene_faces",
+    ]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(rc):
+    if rc < 0:
+        raise VoxelGiError(library().vgi_last_error().decode())
+    return rc
+
+
+# ---------------------------------------------------------------- scenes
+def generate(workload, seed, edge=0):
+    """Dense u16 voxel grid (z, y, x) of a seeded workload; AIR = 256."""
+    lib = library()
+    dims = np.zeros(3, np.int32)
+    _check(lib.vgi_scene_generate(workload, seed, edge, None, _ptr(dims)))
+    grid = np.empty((dims[2], dims[1], dims[0]), np.uint16)
+    _check(lib.vgi_scene_generate(workload, seed, edge, _ptr(grid), _ptr(dims)))
+    return grid
+
+
+def optics(workload, seed):
+    """(diffuse[256,3], emission[256,3], lights (n,12) float32, environment[3])."""
+    lib = library()
+    diffuse = np.zeros((256, 3), np.float32)
+    emission = np.zeros((256, 3), np.float32)
+    lights = np.zeros((16, 12), np.float32)
+    count = np.zeros(1, np.uint32)
+    env = np.zeros(3, np.float32)
+    _check(lib.vgi_scene_optics(workload, seed, _ptr(diffuse), _ptr(emission), _ptr(lights), _ptr(count), _ptr(env)))
+    return diffuse, emission, lights[: int(count[0])].copy(), env
+
+
+def exposed_faces(grid, interior_only=False):
+    """Exposed faces (solid voxel, air neighbour inside the grid) as FACE_TASK records."""
+    lib = library()
+    grid = np.ascontiguousarray(grid, np.uint16)
+    dims = np.array([grid.shape[2], grid.shape[1], grid.shape[0]], np.int32)
+    n = _check(lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), None, 0))
+    out = np.zeros(n, FACE_TASK)
+    lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), _ptr(out), n)
+    return out
+
+
+def pages_of(grid, origin=(0, 0, 0)):
+    """Split a dense (z, y, x) grid with edges multiple of 32 into pages:
+    (page xyz int32 [n,3], voxels uint16 [n, 32768] with x fastest)."""
+    z, y, x = grid.shape
+    assert z % 32 == 0 and y % 32 == 0 and x % 32 == 0
+    g = grid.reshape(z // 32, 32, y // 32, 32, x // 32, 32).transpose(0, 2, 4, 1, 3, 5)
+    pz, py, px = np.meshgrid(np.arange(z // 32), np.arange(y // 32), np.arange(x // 32), indexing="ij")
+    coords = np.stack([px.ravel(), py.ravel(), pz.ravel()], axis=1).astype(np.int32) + np.asarray(origin, np.int32) // 32
+    return coords, np.ascontiguousarray(g.reshape(-1, 32768))
+
+
+# ---------------------------------------------------------------- tracer
+class GpuTracer:
+    """The reference's GpuTracer seam over the HIP library (gpu.rs:309-1236)."""
+
+    def __init__(self, device=0):
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.vgi_create(device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    @classmethod
+    def new(cls, device=0):
+        return cls(device)
+
+    def close(self):
+        if self._h:
+            self._lib.vgi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def enable_resident(self):
+        _check(self._lib.vgi_enable_resident(self._h))
+
+    # ---- scene input (VoxelGiInput page_loaded / page_unloaded / edit_voxel)
+    def upload_pages(self, page_xyz, voxels, removed=None):
+        page_xyz = np.ascontiguousarray(page_xyz, np.int32).reshape(-1, 3)
+        voxels = np.ascontiguousarray(voxels, np.uint16).reshape(-1, 32768)
+        removed = np.zeros((0, 3), np.int32) if removed is None else np.ascontiguousarray(removed, np.int32).reshape(-1, 3)
+        _check(self._lib.vgi_upload_pages(self._h, _ptr(page_xyz), len(page_xyz), _ptr(voxels), _ptr(removed), len(removed)))
+
+    def upload_grid(self, grid, origin=(0, 0, 0)):
+        coords, pages = pages_of(grid, origin)
+        self.upload_pages(coords, pages)
+
+    def edit_voxels(self, xyz, values):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        values = np.ascontiguousarray(values, np.uint16).reshape(-1)
+        _check(self._lib.vgi_edit_voxels(self._h, _ptr(xyz), _ptr(values), len(values)))
+
+    # ---- optics and sources
+    def set_materials(self, diffuse, emission):
+        d = np.ascontiguousarray(diffuse, np.float32).reshape(256, 3)
+        e = np.ascontiguousarray(emission, np.float32).reshape(256, 3)
+        _check(self._lib.vgi_set_materials(self._h, _ptr(d), _ptr(e)))
+
+    def set_lights(self, lights, directional=None):
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 12)
+        if directional is None:
+            directional = int((lights[:, 3] < 0).sum()) if len(lights) else 0
+        _check(self._lib.vgi_set_lights(self._h, _ptr(lights) if len(lights) else None, len(lights), directional))
+
+    def set_transport(self, bounces=3, samples=16, lanes=0, environment=(0.0, 0.0, 0.0), history_limit=0,
+                      history_recency=1.0, measured_retention=1.0, limited_reach=False, reach=768.0, max_distance=768.0):
+        t = Transport()
+        t.bounces, t.samples, t.lanes, t.limited_reach = bounces, samples, lanes, int(limited_reach)
+        t.reach, t.max_distance = reach, max_distance
+        for i in range(3):
+            t.environment[i] = environment[i]
+        t.history_limit, t.history_recency, t.measured_retention = history_limit, history_recency, measured_retention
+        self.transport = t
+        _check(self._lib.vgi_set_transport(self._h, ctypes.byref(t)))
+
+    # ---- batches (submit_compact / snapshot_ready / complete_readback)
+    def submit_compact(self, tasks, flags=0):
+        tasks = np.ascontiguousarray(tasks, FACE_TASK)
+        ticket = ctypes.c_uint64()
+        _check(self._lib.vgi_submit_faces(self._h, _ptr(tasks), len(tasks), flags, ctypes.byref(ticket)))
+        self._last_count = len(tasks)
+        return ticket.value
+
+    def snapshot_ready(self, ticket):
+        ready = ctypes.c_int()
+        _check(self._lib.vgi_poll(self._h, ticket, ctypes.byref(ready)))
+        return bool(ready.value)
+
+    def complete_readback(self, ticket, count):
+        out = np.zeros((count, 24), np.float32)
+        _check(self._lib.vgi_read_faces(self._h, ticket, _ptr(out), count))
+        return out
+
+    def read_hits(self, ticket, count, samples):
+        out = np.zeros((count * samples, 8), np.uint32)
+        _check(self._lib.vgi_read_hits(self._h, ticket, _ptr(out), out.size))
+        return out
+
+    def read_counters(self, ticket):
+        out = np.zeros(8, np.uint64)
+        _check(self._lib.vgi_read_counters(self._h, ticket, _ptr(out), 8))
+        return out
+
+    def timings(self):
+        out = np.zeros(4, np.float32)
+        _check(self._lib.vgi_timings(self._h, _ptr(out), 4))
+        return out
+
+    def slots(self, count):
+        out = np.zeros(count, np.uint32)
+        _check(self._lib.vgi_read_slots(self._h, _ptr(out), count))
+        return out
+
+    # ---- device-resident frames
+    def bind_faces(self, tasks):
+        tasks = np.ascontiguousarray(tasks, FACE_TASK)
+        _check(self._lib.vgi_bind_faces(self._h, _ptr(tasks), len(tasks)))
+        self._bound = len(tasks)
+
+    def run_frame(self, flags=0):
+        _check(self._lib.vgi_run_frame(self._h, flags))
+
+    def sync(self):
+        _check(self._lib.vgi_sync(self._h))
+
+    def last_trace_ms(self):
+        return float(self._lib.vgi_last_trace_ms(self._h))
+
+    def read_bound(self):
+        out = np.zeros((self._bound, 24), np.float32)
+        counters = np.zeros(2, np.uint64)
+        _check(self._lib.vgi_read_bound(self._h, _ptr(out), self._bound, _ptr(counters)))
+        return out, counters
+
+    def cache_state(self, first, count):
+        out = np.zeros((count, 36), np.uint32)
+        _check(self._lib.vgi_cache_state(self._h, first, count, _ptr(out)))
+        return out
+
+    def apply_events(self, events, event_count):
+        events = np.ascontiguousarray(events, np.uint32).reshape(-1)
+        _check(self._lib.vgi_apply_events(self._h, _ptr(events), event_count, events.size))
+
+    # ---- inspection
+    def bounds(self):
+        out = np.zeros(6, np.int32)
+        return out if _check(self._lib.vgi_scene_bounds(self._h, _ptr(out))) else None
+
+    def region_canonical(self, rx, ry, rz):
+        out = np.zeros(4 + 64 + 64 * 32, np.uint32)
+        _check(self._lib.vgi_scene_region_canonical(self._h, rx, ry, rz, _ptr(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(8, np.int64)
+        _check(self._lib.vgi_scene_stats(self._h, _ptr(out)))
+        return out
+
+    def emitter_words(self):
+        n = _check(self._lib.vgi_emitter_words(self._h, None, 0))
+        out = np.zeros(n, np.uint32)
+        _check(self._lib.vgi_emitter_words(self._h, _ptr(out), n))
+        return out

@@ -1,0 +1,1006 @@
+// The C ABI of include/vgi.h: scene upload and edits (GPU NAADF build),
+// optics, batches, the resident world cache and device-resident frames.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/vgi.h"
+#include "vgi_host_scene.h"
+#include "vgi_kernels.h"
+#include "vgi_trace.h"
+
+using vgi_host::Int3;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string &message) {
+    g_error = message;
+    return -1;
+}
+
+#define HIP_OK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// A device allocation that grows to a power of two (gpu.rs:254-276).
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes && ptr) return hipSuccess;
+        size_t cap = 256;
+        while (cap < need) cap <<= 1;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+        }
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, cap);
+        if (e == hipSuccess) bytes = cap;
+        return e;
+    }
+    hipError_t release() {
+        hipError_t e = hipSuccess;
+        if (ptr) e = hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        return e;
+    }
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(ptr);
+    }
+};
+
+inline size_t lin3(int x, int y, int z, int sx, int sy) { return (size_t)x + (size_t)sx * ((size_t)y + (size_t)sy * (size_t)z); }
+inline int floor_div(int a, int b) { int q = a / b; if ((a % b) != 0 && ((a < 0) != (b < 0))) --q; return q; }
+inline int floor_mod(int a, int b) { int m = a % b; if (m < 0) m += b; return m; }
+inline uint64_t page_key(int x, int y, int z) {
+    return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) ^ ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) ^ (uint64_t)(uint32_t)(z + (1 << 20));
+}
+
+struct Ticket {
+    DeviceBuffer results, hits, counters;
+    uint32_t count = 0, samples = 0;
+    bool with_hits = false, with_counters = false;
+    hipEvent_t done = nullptr;
+};
+
+}  // namespace
+
+struct vgi_tracer {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    float timings[4] = {0, 0, 0, 0};
+    // ---- scene (host authority for page contents, as the reference's RegionMap)
+    std::unordered_map<uint64_t, Int3> pages;
+    Int3 vmin{0, 0, 0};           // voxel origin of the dense box
+    int gx = 0, gy = 0, gz = 0;   // dense box size in voxels
+    std::vector<uint16_t> grid;   // host copy, x fastest
+    Int3 rmin{0, 0, 0};           // region box (loaded bounds when non-empty)
+    int rsx = 0, rsy = 0, rsz = 0;
+    std::vector<uint8_t> loaded;  // per region of the box
+    std::vector<uint8_t> solid;
+    std::vector<uint64_t> masks;  // 4 words per region: materials present
+    bool have_regions = false;
+    Int3 bmin{0, 0, 0}, bmax{0, 0, 0};  // loaded region bounds
+    vgi_host::TopLevelField top;
+    vgi_host::EmitterTable emitters;
+    bool emitters_dirty = true;
+    // ---- device scene
+    DeviceBuffer d_grid, d_headers, d_cells, d_info, d_list, d_top, d_totals, d_scatter_idx, d_scatter_val;
+    uint32_t cells_used = 0;
+    uint32_t cells_live = 0;
+    // ---- optics
+    float diffuse[768], emission[768];
+    std::vector<float> lights;  // 12 floats per light
+    uint32_t light_count = 0, directional = 0;
+    DeviceBuffer d_materials, d_lights, d_emitters;
+    vgi_transport transport{};
+    // ---- resident cache
+    bool resident = false;
+    vgi_host::FaceSlots face_slots;
+    std::vector<uint32_t> free_slots;
+    uint32_t high_water = 0;
+    DeviceBuffer d_index, d_state, d_checkpoint, d_meta;
+    size_t slot_capacity = 0;
+    std::vector<uint32_t> last_slots;
+    std::chrono::steady_clock::time_point origin = std::chrono::steady_clock::now();
+    // ---- batches
+    DeviceBuffer d_tasks, d_slots;
+    std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
+    uint64_t next_ticket = 1;
+    // ---- bound frames
+    DeviceBuffer b_tasks, b_slots, b_results, b_counters;
+    uint32_t bound = 0;
+    float last_trace_ms = 0.0f;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- scene build
+// The material presence mask and solidity of one region from the host grid.
+void scan_region(vgi_tracer &t, int ri) {
+    int rx = ri % t.rsx, ry = (ri / t.rsx) % t.rsy, rz = ri / (t.rsx * t.rsy);
+    uint64_t *m = &t.masks[(size_t)ri * 4];
+    m[0] = m[1] = m[2] = m[3] = 0;
+    bool any = false;
+    if (t.loaded[ri]) {
+        int ox = (t.rmin.x + rx) * 16 - t.vmin.x, oy = (t.rmin.y + ry) * 16 - t.vmin.y, oz = (t.rmin.z + rz) * 16 - t.vmin.z;
+        for (int z = 0; z < 16; ++z)
+            for (int y = 0; y < 16; ++y) {
+                const uint16_t *row = &t.grid[lin3(ox, oy + y, oz + z, t.gx, t.gy)];
+                for (int x = 0; x < 16; ++x) {
+                    uint16_t v = row[x];
+                    if (v < vgi_host::AIR) {
+                        any = true;
+                        m[v >> 6] |= 1ull << (v & 63);
+                    }
+                }
+            }
+    }
+    t.solid[ri] = any ? 1 : 0;
+}
+
+uint32_t *host_top_words(vgi_tracer &t, std::vector<uint32_t> &words) {
+    size_t cells = (size_t)t.rsx * t.rsy * t.rsz;
+    words.assign(cells * 2, 0u);
+    for (int z = 0; z < t.rsz; ++z)
+        for (int y = 0; y < t.rsy; ++y)
+            for (int x = 0; x < t.rsx; ++x) {
+                size_t i = lin3(x, y, z, t.rsx, t.rsy);
+                if (!t.loaded[i]) continue;
+                vgi_host::Extents e = t.top.at(Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z});
+                // gpu_scene.rs:452-460: negative bytes, positive bytes (stride 0).
+                words[i * 2] = (uint32_t)e[0] | ((uint32_t)e[2] << 8) | ((uint32_t)e[4] << 16);
+                words[i * 2 + 1] = (uint32_t)e[1] | ((uint32_t)e[3] << 8) | ((uint32_t)e[5] << 16);
+            }
+    return words.data();
+}
+
+// Classify and pack the listed regions on the GPU (appending their tables and
+// bricks to the cells buffer), then write their headers.
+int gpu_pack(vgi_tracer &t, const std::vector<int32_t> &list4, bool restart) {
+    uint32_t n = (uint32_t)(list4.size() / 4);
+    if (n == 0) return 0;
+    if (restart) t.cells_used = 0;
+    HIP_OK(t.d_list.reserve(list4.size() * 4));
+    HIP_OK(t.d_info.reserve((size_t)n * sizeof(vgi::RegionInfo)));
+    HIP_OK(t.d_totals.reserve(16));
+    HIP_OK(hipMemcpyAsync(t.d_list.ptr, list4.data(), list4.size() * 4, hipMemcpyHostToDevice, t.stream));
+    vgi::launch_classify(t.d_grid.as<uint16_t>(), t.gx, t.gy, t.gz, t.vmin.x, t.vmin.y, t.vmin.z, t.d_list.as<int4>(), n,
+                         t.d_info.as<vgi::RegionInfo>(), t.stream);
+    HIP_OK(hipGetLastError());
+    vgi::launch_scan(t.d_info.as<vgi::RegionInfo>(), n, t.cells_used, t.d_totals.as<uint32_t>(), t.stream);
+    HIP_OK(hipGetLastError());
+    uint32_t totals[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(totals, t.d_totals.ptr, 8, hipMemcpyDeviceToHost, t.stream));
+    HIP_OK(hipStreamSynchronize(t.stream));
+    uint64_t need = (uint64_t)t.cells_used + (uint64_t)totals[0] * 64u + (uint64_t)totals[1] * 32u;
+    if (need >= (1ull << 30)) return fail("packed scene exceeds 2^30 cell words (brick pointers are 30 bits)");
+    if (need * 4 > t.d_cells.bytes) {
+        // Grow, keeping the live words (tables of unchanged regions point into them).
+        DeviceBuffer grown;
+        HIP_OK(grown.reserve(need * 4 + (need * 4) / 2));
+        if (t.cells_used) HIP_OK(hipMemcpyAsync(grown.ptr, t.d_cells.ptr, (size_t)t.cells_used * 4, hipMemcpyDeviceToDevice, t.stream));
+        HIP_OK(hipStreamSynchronize(t.stream));
+        HIP_OK(t.d_cells.release());
+        t.d_cells = grown;
+    }
+    vgi::launch_pack(t.d_grid.as<uint16_t>(), t.gx, t.gy, t.gz, t.vmin.x, t.vmin.y, t.vmin.z, t.d_list.as<int4>(), n,
+                     t.d_info.as<vgi::RegionInfo>(), t.d_cells.as<uint32_t>(), t.stream);
+    HIP_OK(hipGetLastError());
+    t.cells_used = (uint32_t)need;
+    // Header words z, w come from the top level; written below for all regions.
+    std::vector<uint32_t> zero((size_t)n * 2, 0u);
+    HIP_OK(t.d_top.reserve(std::max<size_t>(zero.size() * 4, (size_t)t.rsx * t.rsy * t.rsz * 8)));
+    HIP_OK(hipMemcpyAsync(t.d_top.ptr, zero.data(), zero.size() * 4, hipMemcpyHostToDevice, t.stream));
+    vgi::launch_headers(t.d_list.as<int4>(), n, t.d_info.as<vgi::RegionInfo>(), t.d_top.as<uint32_t>(), t.d_headers.as<uint4>(),
+                        t.rmin.x, t.rmin.y, t.rmin.z, t.rsx, t.rsy, t.stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t.stream));
+    return 0;
+}
+
+int upload_top(vgi_tracer &t) {
+    std::vector<uint32_t> words;
+    host_top_words(t, words);
+    size_t cells = (size_t)t.rsx * t.rsy * t.rsz;
+    HIP_OK(t.d_top.reserve(words.size() * 4));
+    HIP_OK(hipMemcpyAsync(t.d_top.ptr, words.data(), words.size() * 4, hipMemcpyHostToDevice, t.stream));
+    vgi::launch_top(t.d_top.as<uint32_t>(), (uint32_t)cells, t.d_headers.as<uint4>(), t.stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t.stream));
+    return 0;
+}
+
+std::vector<uint8_t> barrier_map(const vgi_tracer &t) {
+    // world.rs:497: unloaded or solid regions are barriers.
+    std::vector<uint8_t> b((size_t)t.rsx * t.rsy * t.rsz);
+    for (size_t i = 0; i < b.size(); ++i) b[i] = (!t.loaded[i] || t.solid[i]) ? 1 : 0;
+    return b;
+}
+
+std::vector<int32_t> region_list(const vgi_tracer &t, const std::vector<int> &indices) {
+    std::vector<int32_t> list;
+    list.reserve(indices.size() * 4);
+    for (int i : indices) {
+        int x = i % t.rsx, y = (i / t.rsx) % t.rsy, z = i / (t.rsx * t.rsy);
+        list.push_back(t.rmin.x + x); list.push_back(t.rmin.y + y); list.push_back(t.rmin.z + z); list.push_back(0);
+    }
+    return list;
+}
+
+// Pages changed: re-lay the dense box if the page bounds moved, re-pack the
+// affected regions on the GPU and update the top level for the regions whose
+// barrier state flipped (TransportScene::with_page_contents, world.rs:982-1046).
+int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t *>> &incoming, const std::vector<Int3> &removed) {
+    // Previous state per region coordinate (for the flip list).
+    std::unordered_map<uint64_t, uint8_t> before;  // bit0 loaded, bit1 solid
+    auto note = [&](int rx, int ry, int rz) {
+        uint64_t k = page_key(rx, ry, rz);
+        if (before.count(k)) return;
+        uint8_t st = 0;
+        if (t.have_regions && rx >= t.rmin.x && ry >= t.rmin.y && rz >= t.rmin.z && rx < t.rmin.x + t.rsx && ry < t.rmin.y + t.rsy &&
+            rz < t.rmin.z + t.rsz) {
+            size_t i = lin3(rx - t.rmin.x, ry - t.rmin.y, rz - t.rmin.z, t.rsx, t.rsy);
+            st = (uint8_t)((t.loaded[i] ? 1 : 0) | (t.solid[i] ? 2 : 0));
+        }
+        before[k] = st;
+    };
+    std::vector<Int3> touched_pages;
+    for (auto &p : incoming) touched_pages.push_back(p.first);
+    for (auto &p : removed) touched_pages.push_back(p);
+    for (const Int3 &p : touched_pages)
+        for (int o = 0; o < 8; ++o) note(p.x * 2 + (o & 1), p.y * 2 + ((o >> 1) & 1), p.z * 2 + (o >> 2));
+    // New page set and box.
+    Int3 old_vmin = t.vmin;
+    int ogx = t.gx, ogy = t.gy, ogz = t.gz;
+    std::vector<Int3> old_pages;
+    for (auto &kv : t.pages) old_pages.push_back(kv.second);
+    for (const Int3 &p : removed) t.pages.erase(page_key(p.x, p.y, p.z));
+    for (auto &p : incoming) t.pages[page_key(p.first.x, p.first.y, p.first.z)] = p.first;
+    if (t.pages.empty()) {
+        t.have_regions = false;
+        t.rsx = t.rsy = t.rsz = 0;
+        t.grid.clear();
+        t.top = vgi_host::TopLevelField();
+        t.emitters_dirty = true;
+        return 0;
+    }
+    Int3 pmn{INT32_MAX, INT32_MAX, INT32_MAX}, pmx{INT32_MIN, INT32_MIN, INT32_MIN};
+    for (auto &kv : t.pages) {
+        const Int3 &p = kv.second;
+        pmn = Int3{std::min(pmn.x, p.x), std::min(pmn.y, p.y), std::min(pmn.z, p.z)};
+        pmx = Int3{std::max(pmx.x, p.x), std::max(pmx.y, p.y), std::max(pmx.z, p.z)};
+    }
+    Int3 nvmin{pmn.x * 32, pmn.y * 32, pmn.z * 32};
+    int ngx = (pmx.x - pmn.x + 1) * 32, ngy = (pmx.y - pmn.y + 1) * 32, ngz = (pmx.z - pmn.z + 1) * 32;
+    bool relayout = !t.have_regions || !(nvmin == old_vmin) || ngx != ogx || ngy != ogy || ngz != ogz;
+    if (relayout) {
+        std::vector<uint16_t> grid((size_t)ngx * ngy * ngz, vgi_host::AIR);
+        for (const Int3 &p : old_pages) {
+            if (!t.pages.count(page_key(p.x, p.y, p.z))) continue;
+            for (int z = 0; z < 32; ++z)
+                for (int y = 0; y < 32; ++y) {
+                    const uint16_t *src = &t.grid[lin3(p.x * 32 - old_vmin.x, p.y * 32 - old_vmin.y + y, p.z * 32 - old_vmin.z + z, ogx, ogy)];
+                    uint16_t *dst = &grid[lin3(p.x * 32 - nvmin.x, p.y * 32 - nvmin.y + y, p.z * 32 - nvmin.z + z, ngx, ngy)];
+                    std::memcpy(dst, src, 64);
+                }
+        }
+        t.grid.swap(grid);
+        t.vmin = nvmin; t.gx = ngx; t.gy = ngy; t.gz = ngz;
+        t.rmin = Int3{pmn.x * 2, pmn.y * 2, pmn.z * 2};
+        t.rsx = ngx / 16; t.rsy = ngy / 16; t.rsz = ngz / 16;
+    }
+    for (const Int3 &p : removed)
+        for (int z = 0; z < 32; ++z)
+            for (int y = 0; y < 32; ++y) {
+                int px = p.x * 32 - t.vmin.x, py = p.y * 32 - t.vmin.y + y, pz = p.z * 32 - t.vmin.z + z;
+                if (px < 0 || py < 0 || pz < 0 || px >= t.gx || py >= t.gy || pz >= t.gz) continue;
+                std::fill_n(&t.grid[lin3(px, py, pz, t.gx, t.gy)], 32, vgi_host::AIR);
+            }
+    for (auto &p : incoming)
+        for (int z = 0; z < 32; ++z)
+            for (int y = 0; y < 32; ++y)
+                std::memcpy(&t.grid[lin3(p.first.x * 32 - t.vmin.x, p.first.y * 32 - t.vmin.y + y, p.first.z * 32 - t.vmin.z + z, t.gx, t.gy)],
+                            p.second + (size_t)32 * (y + 32 * z), 64);
+    size_t rcells = (size_t)t.rsx * t.rsy * t.rsz;
+    std::vector<int> repack;
+    if (relayout) {
+        t.loaded.assign(rcells, 0);
+        t.solid.assign(rcells, 0);
+        t.masks.assign(rcells * 4, 0);
+        for (auto &kv : t.pages) {
+            const Int3 &p = kv.second;
+            for (int o = 0; o < 8; ++o)
+                t.loaded[lin3(p.x * 2 + (o & 1) - t.rmin.x, p.y * 2 + ((o >> 1) & 1) - t.rmin.y, p.z * 2 + (o >> 2) - t.rmin.z, t.rsx, t.rsy)] = 1;
+        }
+        for (size_t i = 0; i < rcells; ++i) {
+            scan_region(t, (int)i);
+            if (t.loaded[i]) repack.push_back((int)i);
+        }
+    } else {
+        for (const Int3 &p : touched_pages)
+            for (int o = 0; o < 8; ++o) {
+                int i = (int)lin3(p.x * 2 + (o & 1) - t.rmin.x, p.y * 2 + ((o >> 1) & 1) - t.rmin.y, p.z * 2 + (o >> 2) - t.rmin.z, t.rsx, t.rsy);
+                t.loaded[i] = t.pages.count(page_key(p.x, p.y, p.z)) ? 1 : 0;
+                scan_region(t, i);
+                if (t.loaded[i]) repack.push_back(i);
+            }
+        std::sort(repack.begin(), repack.end());
+        repack.erase(std::unique(repack.begin(), repack.end()), repack.end());
+    }
+    t.bmin = t.rmin;
+    t.bmax = Int3{t.rmin.x + t.rsx - 1, t.rmin.y + t.rsy - 1, t.rmin.z + t.rsz - 1};
+    t.have_regions = true;
+    HIP_OK(t.d_grid.reserve(t.grid.size() * 2));
+    HIP_OK(hipMemcpyAsync(t.d_grid.ptr, t.grid.data(), t.grid.size() * 2, hipMemcpyHostToDevice, t.stream));
+    if (relayout) {
+        HIP_OK(t.d_headers.reserve(rcells * 16));
+        HIP_OK(hipMemsetAsync(t.d_headers.ptr, 0, rcells * 16, t.stream));
+    } else {
+        // Unloaded regions get kind 0 headers.
+        for (const Int3 &p : removed)
+            for (int o = 0; o < 8; ++o) {
+                size_t i = lin3(p.x * 2 + (o & 1) - t.rmin.x, p.y * 2 + ((o >> 1) & 1) - t.rmin.y, p.z * 2 + (o >> 2) - t.rmin.z, t.rsx, t.rsy);
+                if (t.loaded[i]) continue;
+                HIP_OK(hipMemsetAsync(t.d_headers.as<uint4>() + i, 0, 16, t.stream));
+            }
+    }
+    hipEventRecord(t.ev[6], t.stream);
+    if (gpu_pack(t, region_list(t, repack), relayout) != 0) return -1;
+    hipEventRecord(t.ev[7], t.stream);
+    // Regions whose barrier state flipped, in z, y, x order (world.rs:595-601).
+    std::vector<std::pair<std::tuple<int, int, int>, Int3>> flips;
+    for (auto &kv : before) {
+        (void)kv;
+    }
+    for (const Int3 &p : touched_pages)
+        for (int o = 0; o < 8; ++o) {
+            Int3 r{p.x * 2 + (o & 1), p.y * 2 + ((o >> 1) & 1), p.z * 2 + (o >> 2)};
+            size_t i = lin3(r.x - t.rmin.x, r.y - t.rmin.y, r.z - t.rmin.z, t.rsx, t.rsy);
+            uint8_t now = (uint8_t)((t.loaded[i] ? 1 : 0) | (t.solid[i] ? 2 : 0));
+            if (now != before[page_key(r.x, r.y, r.z)]) flips.push_back({std::make_tuple(r.z, r.y, r.x), r});
+        }
+    std::sort(flips.begin(), flips.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    flips.erase(std::unique(flips.begin(), flips.end(), [](const auto &a, const auto &b) { return a.first == b.first; }), flips.end());
+    if (!flips.empty() || relayout) {
+        std::vector<Int3> keys;
+        for (auto &f : flips) keys.push_back(f.second);
+        t.top.update(t.bmin, t.bmax, keys, barrier_map(t));
+    }
+    if (upload_top(t) != 0) return -1;
+    hipEventSynchronize(t.ev[7]);
+    hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    t.emitters_dirty = true;
+    return 0;
+}
+
+// Voxel edits (pages.rs:498 page overlay): scatter into the GPU grid, re-pack
+// the edited regions and update the top level if any region's emptiness flipped.
+int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
+    if (!t.have_regions) return 0;
+    std::vector<uint32_t> idx;
+    std::vector<uint16_t> val;
+    std::vector<int> dirty;
+    idx.reserve(count);
+    val.reserve(count);
+    for (uint32_t e = 0; e < count; ++e) {
+        int x = xyz[e * 3], y = xyz[e * 3 + 1], z = xyz[e * 3 + 2];
+        int rx = floor_div(x, 16) - t.rmin.x, ry = floor_div(y, 16) - t.rmin.y, rz = floor_div(z, 16) - t.rmin.z;
+        if (rx < 0 || ry < 0 || rz < 0 || rx >= t.rsx || ry >= t.rsy || rz >= t.rsz) continue;
+        size_t ri = lin3(rx, ry, rz, t.rsx, t.rsy);
+        if (!t.loaded[ri]) continue;  // edits outside loaded pages wait for the page (dropped here)
+        size_t gi = lin3(x - t.vmin.x, y - t.vmin.y, z - t.vmin.z, t.gx, t.gy);
+        t.grid[gi] = values[e];
+        idx.push_back((uint32_t)gi);
+        val.push_back(values[e]);
+        dirty.push_back((int)ri);
+    }
+    if (idx.empty()) return 0;
+    std::sort(dirty.begin(), dirty.end());
+    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
+    HIP_OK(t.d_scatter_idx.reserve(idx.size() * 4));
+    HIP_OK(t.d_scatter_val.reserve(val.size() * 2));
+    HIP_OK(hipMemcpyAsync(t.d_scatter_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipMemcpyAsync(t.d_scatter_val.ptr, val.data(), val.size() * 2, hipMemcpyHostToDevice, t.stream));
+    vgi::launch_scatter_voxels(t.d_grid.as<uint16_t>(), t.d_scatter_idx.as<uint32_t>(), t.d_scatter_val.as<uint16_t>(),
+                               (uint32_t)idx.size(), t.stream);
+    HIP_OK(hipGetLastError());
+    // Emitting materials near an edit change the emitter distribution.
+    uint64_t emitting[4] = {0, 0, 0, 0};
+    for (int m = 0; m < 256; ++m) {
+        float w = std::max(std::max(t.emission[m * 3], t.emission[m * 3 + 1]), t.emission[m * 3 + 2]);
+        if (w > 0.0f) emitting[m >> 6] |= 1ull << (m & 63);
+    }
+    auto emits = [&](int ri) {
+        const uint64_t *m = &t.masks[(size_t)ri * 4];
+        return (m[0] & emitting[0]) | (m[1] & emitting[1]) | (m[2] & emitting[2]) | (m[3] & emitting[3]);
+    };
+    std::vector<std::pair<std::tuple<int, int, int>, Int3>> flips;
+    for (int ri : dirty) {
+        int x = ri % t.rsx, y = (ri / t.rsx) % t.rsy, z = ri / (t.rsx * t.rsy);
+        bool near_emitter = false;
+        const int nb[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        for (auto &o : nb) {
+            int nx = x + o[0], ny = y + o[1], nz = z + o[2];
+            if (nx < 0 || ny < 0 || nz < 0 || nx >= t.rsx || ny >= t.rsy || nz >= t.rsz) continue;
+            if (emits((int)lin3(nx, ny, nz, t.rsx, t.rsy))) near_emitter = true;
+        }
+        uint8_t was = t.solid[ri];
+        scan_region(t, ri);
+        if (emits(ri)) near_emitter = true;
+        if (near_emitter) t.emitters_dirty = true;
+        if (was != t.solid[ri]) flips.push_back({std::make_tuple(t.rmin.z + z, t.rmin.y + y, t.rmin.x + x), Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z}});
+    }
+    // Keep the cells buffer bounded: re-pack everything once appended tables
+    // and bricks outgrow four times the last full pack.
+    bool full = t.cells_used > 4u * std::max<uint32_t>(t.cells_live, 1u << 16);
+    std::vector<int> repack = dirty;
+    if (full) {
+        repack.clear();
+        for (size_t i = 0; i < t.loaded.size(); ++i)
+            if (t.loaded[i]) repack.push_back((int)i);
+    }
+    hipEventRecord(t.ev[6], t.stream);
+    if (gpu_pack(t, region_list(t, repack), full) != 0) return -1;
+    if (full) t.cells_live = t.cells_used;
+    hipEventRecord(t.ev[7], t.stream);
+    if (!flips.empty()) {
+        std::sort(flips.begin(), flips.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::vector<Int3> keys;
+        for (auto &f : flips) keys.push_back(f.second);
+        t.top.update(t.bmin, t.bmax, keys, barrier_map(t));
+    }
+    if (upload_top(t) != 0) return -1;
+    hipEventSynchronize(t.ev[7]);
+    hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    return 0;
+}
+
+// ---------------------------------------------------------------- optics / emitters
+int sync_emitters(vgi_tracer &t) {
+    if (!t.emitters_dirty) return 0;
+    vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
+    uint64_t emitting[4] = {0, 0, 0, 0};
+    for (int m = 0; m < 256; ++m) {
+        float w = std::max(std::max(t.emission[m * 3], t.emission[m * 3 + 1]), t.emission[m * 3 + 2]);
+        if (w > 0.0f) emitting[m >> 6] |= 1ull << (m & 63);
+    }
+    std::vector<Int3> regions;
+    if (t.have_regions)
+        for (int z = 0; z < t.rsz; ++z)
+            for (int y = 0; y < t.rsy; ++y)
+                for (int x = 0; x < t.rsx; ++x) {
+                    size_t i = lin3(x, y, z, t.rsx, t.rsy);
+                    const uint64_t *m = &t.masks[i * 4];
+                    if (!t.loaded[i]) continue;
+                    if ((m[0] & emitting[0]) | (m[1] & emitting[1]) | (m[2] & emitting[2]) | (m[3] & emitting[3]))
+                        regions.push_back(Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z});
+                }
+    vgi_host::build_emitters(src, regions, t.emission, t.emitters);
+    HIP_OK(t.d_emitters.reserve(t.emitters.words.size() * 4));
+    HIP_OK(hipMemcpyAsync(t.d_emitters.ptr, t.emitters.words.data(), t.emitters.words.size() * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipStreamSynchronize(t.stream));
+    t.emitters_dirty = false;
+    return 0;
+}
+
+uint32_t lanes_for(const vgi_transport &tr) {
+    if (tr.lanes) return tr.lanes;
+    return tr.samples < 16u ? 4u : 8u;
+}
+
+void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
+    s.headers = t.d_headers.as<uint4>();
+    s.cells = t.d_cells.as<uint32_t>();
+    s.minx = t.bmin.x; s.miny = t.bmin.y; s.minz = t.bmin.z;
+    s.maxx = t.bmax.x; s.maxy = t.bmax.y; s.maxz = t.bmax.z;
+    s.sx = t.rsx; s.sy = t.rsy;
+    o.materials = t.d_materials.as<float4>();
+    o.lights = t.d_lights.as<float4>();
+    o.emitters = t.d_emitters.as<uint4>();
+    o.light_count = t.light_count;
+    o.emitter_faces = t.emitters.faces;
+    o.total_weight = t.emitters.total_weight;
+    o.has_tree = t.emitters.nodes > 0 ? 1u : 0u;
+    o.env_r = t.transport.environment[0]; o.env_g = t.transport.environment[1]; o.env_b = t.transport.environment[2];
+    o.max_distance = t.transport.max_distance > 0.0f ? t.transport.max_distance : 768.0f;
+    o.reach = t.transport.reach;
+    o.limited_reach = t.transport.limited_reach;
+    o.bounces = t.transport.bounces;
+    o.cache_index = t.d_index.as<uint4>();
+    o.cache_state = t.d_state.as<uint32_t>();
+    o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
+    o.cache_on = t.resident ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------- resident slots
+// resident.rs:770-812 (allocate_slot): a free slot (LIFO) or the high-water mark.
+int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, std::vector<uint32_t> &slots) {
+    slots.resize(count);
+    uint32_t first_new = t.high_water;
+    std::vector<uint32_t> identities;
+    vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
+    for (uint32_t i = 0; i < count; ++i) {
+        const vgi_face_task &f = tasks[i];
+        if (f.face == VGI_NO_FACE) { slots[i] = vgi_host::NONE; continue; }
+        long at = t.face_slots.find(f.voxel[0], f.voxel[1], f.voxel[2], f.face);
+        if (at >= 0) { slots[i] = t.face_slots.entries[at][3] & 0x1fffffffu; continue; }
+        uint32_t slot;
+        if (!t.free_slots.empty()) { slot = t.free_slots.back(); t.free_slots.pop_back(); }
+        else slot = t.high_water++;
+        if (slot >= (1u << 29)) return fail("resident slot exceeds the face index");
+        t.face_slots.insert(f.voxel[0], f.voxel[1], f.voxel[2], f.face, slot);
+        uint32_t m = t.have_regions ? src.material_at(f.voxel[0], f.voxel[1], f.voxel[2]) : 0xffffu;
+        uint32_t material = m < vgi_host::AIR ? m : 0u;
+        identities.push_back(slot);
+        identities.push_back((uint32_t)f.voxel[0]); identities.push_back((uint32_t)f.voxel[1]); identities.push_back((uint32_t)f.voxel[2]);
+        identities.push_back(f.face | 8u | (material << 16));
+        slots[i] = slot;
+    }
+    // Grow the resident arrays, keeping existing slots.
+    if (t.high_water > t.slot_capacity) {
+        size_t cap = std::max<size_t>(1024, t.slot_capacity);
+        while (cap < t.high_water) cap *= 2;
+        DeviceBuffer ns, nc, nm;
+        HIP_OK(ns.reserve(cap * 144));
+        HIP_OK(nc.reserve(cap * 48));
+        HIP_OK(nm.reserve(cap * 48));
+        HIP_OK(hipMemsetAsync(ns.ptr, 0, cap * 144, t.stream));
+        HIP_OK(hipMemsetAsync(nc.ptr, 0, cap * 48, t.stream));
+        HIP_OK(hipMemsetAsync(nm.ptr, 0, cap * 48, t.stream));
+        if (t.slot_capacity) {
+            HIP_OK(hipMemcpyAsync(ns.ptr, t.d_state.ptr, t.slot_capacity * 144, hipMemcpyDeviceToDevice, t.stream));
+            HIP_OK(hipMemcpyAsync(nc.ptr, t.d_checkpoint.ptr, t.slot_capacity * 48, hipMemcpyDeviceToDevice, t.stream));
+            HIP_OK(hipMemcpyAsync(nm.ptr, t.d_meta.ptr, t.slot_capacity * 48, hipMemcpyDeviceToDevice, t.stream));
+        }
+        HIP_OK(hipStreamSynchronize(t.stream));
+        t.d_state.release(); t.d_checkpoint.release(); t.d_meta.release();
+        t.d_state = ns; t.d_checkpoint = nc; t.d_meta = nm;
+        t.slot_capacity = cap;
+    }
+    // Seed new slots (resident.rs:1426-1470 from an empty FaceWork: all zero)
+    // and write their identity records. Slots come from the high-water mark
+    // (nothing is freed by this build), so new slots form one range.
+    if (!identities.empty()) {
+        uint32_t lo = identities[0], hi = identities[0];
+        for (size_t k = 0; k < identities.size(); k += 5) { lo = std::min(lo, identities[k]); hi = std::max(hi, identities[k]); }
+        size_t n = (size_t)(hi - lo + 1);
+        std::vector<uint32_t> meta(n * 12, 0u);
+        for (size_t k = 0; k < identities.size(); k += 5) {
+            uint32_t *m = &meta[(size_t)(identities[k] - lo) * 12];
+            m[0] = identities[k + 1]; m[1] = identities[k + 2]; m[2] = identities[k + 3]; m[3] = identities[k + 4];
+        }
+        if (n * 5 != identities.size()) return fail("resident slots were not allocated as one range");
+        HIP_OK(hipMemsetAsync(t.d_state.as<uint32_t>() + (size_t)lo * 36, 0, n * 144, t.stream));
+        HIP_OK(hipMemsetAsync(t.d_checkpoint.as<uint32_t>() + (size_t)lo * 12, 0, n * 48, t.stream));
+        HIP_OK(hipMemcpyAsync(t.d_meta.as<uint32_t>() + (size_t)lo * 12, meta.data(), n * 48, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipStreamSynchronize(t.stream));
+    }
+    (void)first_new;
+    if (t.face_slots.dirty) {
+        size_t n = t.face_slots.entries.size();
+        HIP_OK(t.d_index.reserve(n * 16));
+        HIP_OK(hipMemcpyAsync(t.d_index.ptr, t.face_slots.entries.data(), n * 16, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipStreamSynchronize(t.stream));
+        t.face_slots.dirty = false;
+    }
+    return 0;
+}
+
+uint32_t clock_ms(const vgi_tracer &t) {
+    return (uint32_t)std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t.origin).count();
+}
+
+vgi::AccumulateParams accumulate_params(const vgi_tracer &t, uint32_t now) {
+    vgi::AccumulateParams p;
+    uint32_t limit = t.transport.history_limit;
+    p.history_limit = limit == 0 ? 65536u : std::min(limit, 65536u);
+    p.samples = t.transport.samples;
+    p.clock_ms = now;
+    p.recency = t.transport.history_recency >= 1.0f ? t.transport.history_recency : 1.0f;
+    p.physical = std::max(t.transport.measured_retention, 0.0f);
+    p.response = 0.075f;  // settings.rs:266 display_response 75 ms
+    return p;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char *vgi_last_error(void) { return g_error.c_str(); }
+
+int vgi_create(int device_ordinal, vgi_tracer **out) {
+    if (!out) return fail("null output");
+    int count = 0;
+    HIP_OK(hipGetDeviceCount(&count));
+    if (device_ordinal < 0 || device_ordinal >= count) return fail("no such HIP device");
+    HIP_OK(hipSetDevice(device_ordinal));
+    std::unique_ptr<vgi_tracer> t(new vgi_tracer());
+    t->device = device_ordinal;
+    HIP_OK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    for (auto &e : t->ev) HIP_OK(hipEventCreate(&e));
+    for (int m = 0; m < 256; ++m)
+        for (int c = 0; c < 3; ++c) { t->diffuse[m * 3 + c] = 0.45f; t->emission[m * 3 + c] = 0.0f; }  // world.rs:72-79
+    t->transport.bounces = 3;
+    t->transport.samples = 16;
+    t->transport.max_distance = 768.0f;
+    t->transport.reach = 768.0f;
+    t->transport.history_recency = 1.0f;
+    t->transport.measured_retention = 1.0f;
+    if (vgi_set_materials(t.get(), t->diffuse, t->emission) != 0) return -1;
+    if (vgi_set_lights(t.get(), nullptr, 0, 0) != 0) return -1;
+    *out = t.release();
+    return 0;
+}
+
+void vgi_destroy(vgi_tracer *t) {
+    if (!t) return;
+    hipSetDevice(t->device);
+    hipStreamSynchronize(t->stream);
+    for (auto &kv : t->tickets) {
+        kv.second->results.release(); kv.second->hits.release(); kv.second->counters.release();
+        if (kv.second->done) hipEventDestroy(kv.second->done);
+    }
+    DeviceBuffer *bufs[] = {&t->d_grid, &t->d_headers, &t->d_cells, &t->d_info, &t->d_list, &t->d_top, &t->d_totals,
+                            &t->d_scatter_idx, &t->d_scatter_val, &t->d_materials, &t->d_lights, &t->d_emitters,
+                            &t->d_index, &t->d_state, &t->d_checkpoint, &t->d_meta, &t->d_tasks, &t->d_slots,
+                            &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters};
+    for (DeviceBuffer *b : bufs) b->release();
+    for (auto &e : t->ev) hipEventDestroy(e);
+    hipStreamDestroy(t->stream);
+    delete t;
+}
+
+int vgi_enable_resident(vgi_tracer *t) {
+    if (!t) return fail("null tracer");
+    t->resident = true;
+    return 0;
+}
+
+int vgi_upload_pages(vgi_tracer *t, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels, const int32_t *removed_xyz,
+                     uint32_t removed) {
+    if (!t) return fail("null tracer");
+    if (pages && (!page_xyz || !voxels)) return fail("null page data");
+    HIP_OK(hipSetDevice(t->device));
+    std::vector<std::pair<Int3, const uint16_t *>> incoming;
+    for (uint32_t i = 0; i < pages; ++i)
+        incoming.push_back({Int3{page_xyz[i * 3], page_xyz[i * 3 + 1], page_xyz[i * 3 + 2]}, voxels + (size_t)i * 32768});
+    std::vector<Int3> rem;
+    for (uint32_t i = 0; i < removed; ++i) rem.push_back(Int3{removed_xyz[i * 3], removed_xyz[i * 3 + 1], removed_xyz[i * 3 + 2]});
+    int rc = apply_pages(*t, incoming, rem);
+    if (rc == 0) t->cells_live = t->cells_used;
+    return rc;
+}
+
+int vgi_edit_voxels(vgi_tracer *t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
+    if (!t) return fail("null tracer");
+    if (count && (!xyz || !values)) return fail("null edit data");
+    HIP_OK(hipSetDevice(t->device));
+    return apply_edits(*t, xyz, values, count);
+}
+
+int vgi_set_materials(vgi_tracer *t, const float *diffuse_rgb, const float *emission_rgb) {
+    if (!t || !diffuse_rgb || !emission_rgb) return fail("null materials");
+    HIP_OK(hipSetDevice(t->device));
+    std::vector<float> packed(256 * 8);
+    for (int m = 0; m < 256; ++m) {
+        float d[3], e[3];
+        for (int c = 0; c < 3; ++c) {
+            // world.rs:166-173 (bounded): diffuse to [0, 1], emission to [0, 1e12]; non-finite -> 0.
+            float dv = diffuse_rgb[m * 3 + c], ev = emission_rgb[m * 3 + c];
+            dv = std::isfinite(dv) ? std::min(std::max(dv, 0.0f), 1.0f) : 0.0f;
+            ev = std::isfinite(ev) ? std::min(std::max(ev, 0.0f), 1e12f) : 0.0f;
+            d[c] = dv; e[c] = ev;
+        }
+        float power = std::max(std::max(e[0], e[1]), e[2]);
+        float rec[8] = {d[0], d[1], d[2], power, e[0], e[1], e[2], 0.0f};
+        std::memcpy(&packed[m * 8], rec, sizeof(rec));
+        for (int c = 0; c < 3; ++c) {
+            if (t->emission[m * 3 + c] != e[c]) t->emitters_dirty = true;
+            t->diffuse[m * 3 + c] = d[c];
+            t->emission[m * 3 + c] = e[c];
+        }
+    }
+    HIP_OK(t->d_materials.reserve(packed.size() * 4));
+    HIP_OK(hipMemcpyAsync(t->d_materials.ptr, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int vgi_set_lights(vgi_tracer *t, const vgi_light *lights, uint32_t count, uint32_t directional) {
+    if (!t) return fail("null tracer");
+    if (count && !lights) return fail("null lights");
+    HIP_OK(hipSetDevice(t->device));
+    t->lights.assign((size_t)std::max<uint32_t>(count, 1) * 12, 0.0f);
+    for (uint32_t i = 0; i < count; ++i) std::memcpy(&t->lights[i * 12], &lights[i], 48);
+    t->light_count = count;
+    t->directional = directional;
+    HIP_OK(t->d_lights.reserve(t->lights.size() * 4));
+    HIP_OK(hipMemcpyAsync(t->d_lights.ptr, t->lights.data(), t->lights.size() * 4, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    return 0;
+}
+
+int vgi_set_transport(vgi_tracer *t, const vgi_transport *tr) {
+    if (!t || !tr) return fail("null transport");
+    if (tr->bounces < 1 || tr->bounces > 8) return fail("bounces must be 1..8");
+    if (tr->samples < 1) return fail("samples must be >= 1");
+    if (tr->lanes != 0 && (tr->lanes > 64 || (64 % tr->lanes) != 0)) return fail("lanes must divide 64");
+    t->transport = *tr;
+    if (!(t->transport.max_distance > 0.0f)) t->transport.max_distance = 768.0f;
+    return 0;
+}
+
+int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, uint32_t flags, uint64_t *ticket) {
+    if (!t || !ticket) return fail("null argument");
+    if (count && !tasks) return fail("null tasks");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    if (sync_emitters(*t) != 0) return -1;
+    std::unique_ptr<Ticket> k(new Ticket());
+    k->count = count;
+    k->samples = t->transport.samples;
+    k->with_hits = (flags & VGI_RECORD_HITS) != 0;
+    k->with_counters = (flags & VGI_COUNT_RAYS) != 0;
+    HIP_OK(hipEventCreate(&k->done));
+    HIP_OK(k->results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
+    if (k->with_hits) HIP_OK(k->hits.reserve((size_t)std::max<uint32_t>(count, 1) * k->samples * 32));
+    HIP_OK(k->counters.reserve(64));
+    HIP_OK(hipMemsetAsync(k->counters.ptr, 0, 64, t->stream));
+    HIP_OK(hipMemsetAsync(k->results.ptr, 0, (size_t)std::max<uint32_t>(count, 1) * 96, t->stream));
+    HIP_OK(t->d_tasks.reserve((size_t)std::max<uint32_t>(count, 1) * 32));
+    HIP_OK(hipMemcpyAsync(t->d_tasks.ptr, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
+    vgi::SceneView s;
+    vgi::Optics o;
+    std::vector<uint32_t> slots;
+    if (t->resident) {
+        if (register_faces(*t, tasks, count, slots) != 0) return -1;
+        HIP_OK(t->d_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+        HIP_OK(hipMemcpyAsync(t->d_slots.ptr, slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
+        t->last_slots = slots;
+    }
+    views(*t, s, o);
+    uint32_t lanes = lanes_for(t->transport);
+    hipEventRecord(t->ev[0], t->stream);
+    if (t->resident)
+        vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
+                            std::max(t->transport.measured_retention, 0.0f), t->stream);
+    hipEventRecord(t->ev[1], t->stream);
+    vgi::launch_trace(s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, k->results.as<float4>(),
+                      k->with_hits ? k->hits.as<uint32_t>() : nullptr,
+                      k->with_counters ? k->counters.as<unsigned long long>() : nullptr, t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->ev[2], t->stream);
+    if (t->resident)
+        vgi::launch_accumulate(t->d_slots.as<uint32_t>(), count, k->results.as<float4>(), t->d_state.as<uint32_t>(),
+                               t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->ev[3], t->stream);
+    HIP_OK(hipEventRecord(k->done, t->stream));
+    uint64_t id = t->next_ticket++;
+    t->tickets[id] = std::move(k);
+    *ticket = id;
+    return 0;
+}
+
+int vgi_poll(vgi_tracer *t, uint64_t ticket, int *ready) {
+    if (!t || !ready) return fail("null argument");
+    auto it = t->tickets.find(ticket);
+    if (it == t->tickets.end()) return fail("unknown ticket");
+    hipError_t e = hipEventQuery(it->second->done);
+    if (e == hipSuccess) { *ready = 1; return 0; }
+    if (e == hipErrorNotReady) { *ready = 0; return 0; }
+    return fail(std::string("hipEventQuery: ") + hipGetErrorString(e));
+}
+
+int vgi_read_faces(vgi_tracer *t, uint64_t ticket, vgi_face_result *out, uint32_t count) {
+    if (!t || !out) return fail("null argument");
+    auto it = t->tickets.find(ticket);
+    if (it == t->tickets.end()) return fail("unknown ticket");
+    Ticket &k = *it->second;
+    if (count > k.count) return fail("more records than the ticket holds");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipEventSynchronize(k.done));
+    HIP_OK(hipMemcpy(out, k.results.ptr, (size_t)count * 96, hipMemcpyDeviceToHost));
+    hipEventElapsedTime(&t->timings[0], t->ev[0], t->ev[1]);
+    hipEventElapsedTime(&t->timings[1], t->ev[1], t->ev[2]);
+    hipEventElapsedTime(&t->timings[2], t->ev[2], t->ev[3]);
+    // A ticket is complete once its faces are read unless it holds hits/counters.
+    if (!k.with_hits && !k.with_counters) {
+        k.results.release(); k.hits.release(); k.counters.release();
+        hipEventDestroy(k.done);
+        t->tickets.erase(it);
+    }
+    return 0;
+}
+
+int vgi_read_hits(vgi_tracer *t, uint64_t ticket, uint32_t *out, uint64_t words) {
+    if (!t || !out) return fail("null argument");
+    auto it = t->tickets.find(ticket);
+    if (it == t->tickets.end()) return fail("unknown ticket");
+    Ticket &k = *it->second;
+    if (!k.with_hits) return fail("ticket has no hit records (VGI_RECORD_HITS)");
+    uint64_t have = (uint64_t)k.count * k.samples * 8;
+    if (words > have) return fail("more words than the ticket holds");
+    HIP_OK(hipEventSynchronize(k.done));
+    HIP_OK(hipMemcpy(out, k.hits.ptr, words * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_read_counters(vgi_tracer *t, uint64_t ticket, uint64_t *out, uint32_t count) {
+    if (!t || !out) return fail("null argument");
+    auto it = t->tickets.find(ticket);
+    if (it == t->tickets.end()) return fail("unknown ticket");
+    Ticket &k = *it->second;
+    HIP_OK(hipEventSynchronize(k.done));
+    uint64_t c[8];
+    HIP_OK(hipMemcpy(c, k.counters.ptr, 64, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < count && i < 8; ++i) out[i] = c[i];
+    return 0;
+}
+
+int vgi_timings(vgi_tracer *t, float *out, uint32_t count) {
+    if (!t || !out) return fail("null argument");
+    for (uint32_t i = 0; i < count && i < 4; ++i) out[i] = t->timings[i];
+    return 0;
+}
+
+int vgi_read_slots(vgi_tracer *t, uint32_t *out, uint32_t count) {
+    if (!t || !out) return fail("null argument");
+    if (count > t->last_slots.size()) return fail("more slots than the last batch");
+    std::memcpy(out, t->last_slots.data(), (size_t)count * 4);
+    return 0;
+}
+
+int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
+    if (!t || (count && !tasks)) return fail("null argument");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(t->b_tasks.reserve((size_t)std::max<uint32_t>(count, 1) * 32));
+    HIP_OK(hipMemcpyAsync(t->b_tasks.ptr, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(t->b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
+    HIP_OK(t->b_counters.reserve(64));
+    if (t->resident) {
+        std::vector<uint32_t> slots;
+        if (register_faces(*t, tasks, count, slots) != 0) return -1;
+        HIP_OK(t->b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+        HIP_OK(hipMemcpyAsync(t->b_slots.ptr, slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
+        t->last_slots = slots;
+    }
+    HIP_OK(hipStreamSynchronize(t->stream));
+    t->bound = count;
+    return 0;
+}
+
+int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
+    if (!t) return fail("null tracer");
+    if (!t->have_regions) return fail("no scene loaded");
+    if (sync_emitters(*t) != 0) return -1;
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    uint32_t count = t->bound;
+    if (flags & VGI_COUNT_RAYS) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
+    if (t->resident)
+        vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
+                            std::max(t->transport.measured_retention, 0.0f), t->stream);
+    hipEventRecord(t->ev[4], t->stream);
+    vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), t->b_results.as<float4>(),
+                      nullptr, (flags & VGI_COUNT_RAYS) ? t->b_counters.as<unsigned long long>() : nullptr, t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->ev[5], t->stream);
+    if (t->resident)
+        vgi::launch_accumulate(t->b_slots.as<uint32_t>(), count, t->b_results.as<float4>(), t->d_state.as<uint32_t>(),
+                               t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int vgi_sync(vgi_tracer *t) {
+    if (!t) return fail("null tracer");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    hipEventElapsedTime(&t->last_trace_ms, t->ev[4], t->ev[5]);
+    return 0;
+}
+
+float vgi_last_trace_ms(vgi_tracer *t) { return t ? t->last_trace_ms : 0.0f; }
+
+int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
+    if (!t || !out) return fail("null argument");
+    if (count > t->bound) return fail("more records than bound faces");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, t->b_results.ptr, (size_t)count * 96, hipMemcpyDeviceToHost));
+    if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_apply_events(vgi_tracer *t, const uint32_t *events, uint32_t event_count, uint32_t words) {
+    if (!t || (words && !events)) return fail("null argument");
+    if (!t->resident) return fail("retention events need the resident cache");
+    if ((uint64_t)event_count * 12 > words) return fail("event headers exceed the buffer");
+    HIP_OK(hipSetDevice(t->device));
+    DeviceBuffer ev;
+    HIP_OK(ev.reserve((size_t)std::max<uint32_t>(words, 4) * 4));
+    HIP_OK(hipMemcpyAsync(ev.ptr, events, (size_t)words * 4, hipMemcpyHostToDevice, t->stream));
+    vgi::launch_apply_events(ev.as<uint4>(), event_count, t->high_water, t->d_state.as<uint32_t>(), t->d_checkpoint.as<uint32_t>(),
+                             t->d_meta.as<uint32_t>(), std::max(t->transport.measured_retention, 0.0f), t->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t->stream));
+    ev.release();
+    return 0;
+}
+
+int vgi_cache_state(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t *out) {
+    if (!t || !out) return fail("null argument");
+    if ((size_t)first + count > t->high_water) return fail("slot range beyond the high-water mark");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, t->d_state.as<uint32_t>() + (size_t)first * 36, (size_t)count * 144, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_scene_bounds(vgi_tracer *t, int32_t out[6]) {
+    if (!t || !out) return fail("null argument");
+    if (!t->have_regions) return 0;
+    out[0] = t->bmin.x; out[1] = t->bmin.y; out[2] = t->bmin.z; out[3] = t->bmax.x; out[4] = t->bmax.y; out[5] = t->bmax.z;
+    return 1;
+}
+
+int vgi_scene_region_canonical(vgi_tracer *t, int32_t rx, int32_t ry, int32_t rz, uint32_t *out) {
+    if (!t || !out) return fail("null argument");
+    std::memset(out, 0, (4 + 64 + 64 * 32) * 4);
+    if (!t->have_regions || rx < t->bmin.x || ry < t->bmin.y || rz < t->bmin.z || rx > t->bmax.x || ry > t->bmax.y || rz > t->bmax.z)
+        return 0;
+    HIP_OK(hipStreamSynchronize(t->stream));
+    size_t i = lin3(rx - t->rmin.x, ry - t->rmin.y, rz - t->rmin.z, t->rsx, t->rsy);
+    uint32_t h[4];
+    HIP_OK(hipMemcpy(h, t->d_headers.as<uint4>() + i, 16, hipMemcpyDeviceToHost));
+    if (h[0] == 0) return 0;
+    out[0] = h[0]; out[1] = h[0] == 3 ? 0u : h[1]; out[2] = h[2]; out[3] = h[3];
+    if (h[0] != 3) return 1;
+    uint32_t table[64];
+    HIP_OK(hipMemcpy(table, t->d_cells.as<uint32_t>() + h[1], 256, hipMemcpyDeviceToHost));
+    for (int b = 0; b < 64; ++b) {
+        if ((table[b] & 3u) == 2u) {
+            out[4 + b] = 2u;
+            HIP_OK(hipMemcpy(&out[68 + b * 32], t->d_cells.as<uint32_t>() + (table[b] >> 2), 128, hipMemcpyDeviceToHost));
+        } else {
+            out[4 + b] = table[b];
+        }
+    }
+    return 1;
+}
+
+int64_t vgi_scene_stats(vgi_tracer *t, int64_t out[8]) {
+    if (!t || !out) return fail("null argument");
+    int64_t loaded = 0, solid = 0;
+    for (size_t i = 0; i < t->loaded.size(); ++i) { loaded += t->loaded[i]; solid += t->loaded[i] && t->solid[i]; }
+    out[0] = loaded; out[1] = solid; out[2] = t->cells_used; out[3] = (int64_t)t->top.fields.size();
+    out[4] = t->high_water; out[5] = (int64_t)t->emitters.faces; out[6] = (int64_t)t->emitters.nodes; out[7] = 0;
+    return loaded;
+}
+
+int64_t vgi_emitter_words(vgi_tracer *t, uint32_t *out, int64_t capacity) {
+    if (!t) return fail("null tracer");
+    if (sync_emitters(*t) != 0) return -1;
+    int64_t n = (int64_t)t->emitters.words.size();
+    if (out && capacity >= n) std::memcpy(out, t->emitters.words.data(), (size_t)n * 4);
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Host-side scene bookkeeping of the tracer: the top-level region field grown
+// in the reference's FIFO order, the emitter distribution and light tree, and
+// the (voxel, face) -> slot index. These run on the CPU in the reference too
+// (world.rs scene worker, emission.rs, gpu/face_index.rs); the brick and voxel
+// NAADF fields and all tracing run on the GPU.
+#pragma once
+#include <stdint.h>
+
+#include <array>
+#include <cstddef>
+#include <deque>
+#include <vector>
+
+namespace vgi_host {
+
+constexpr uint16_t AIR = 256;
+constexpr uint32_t NONE = 0xffffffffu;
+
+struct Int3 {
+    int32_t x, y, z;
+};
+inline bool operator==(const Int3 &a, const Int3 &b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+inline uint32_t pcg(uint32_t value) {
+    uint32_t state = value * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+inline uint32_t hash_cell(int32_t x, int32_t y, int32_t z) {
+    return pcg(pcg(pcg((uint32_t)x) ^ (uint32_t)y) ^ (uint32_t)z);
+}
+
+// Six u8 empty-box extents: -x, +x, -y, +y, -z, +z (naadf.rs:3-15).
+using Extents = std::array<uint8_t, 6>;
+
+// The top-level field over the loaded regions' bounding box
+// (TopLevelAadf, world.rs:428-713): barriers are unloaded or solid regions.
+struct TopLevelField {
+    Int3 minimum{0, 0, 0}, maximum{0, 0, 0};
+    int sx = 0, sy = 0, sz = 0;
+    bool enabled = false;
+    std::vector<Extents> fields;
+    std::vector<uint8_t> blocked;
+
+    // Rebuild or update after the `changed` regions (sorted z, y, x) flipped
+    // residency or emptiness. `barrier` is the new state over [mn, mx]
+    // (x fastest). Follows TopLevelAadf::updated (world.rs:477-583).
+    void update(Int3 mn, Int3 mx, const std::vector<Int3> &changed, const std::vector<uint8_t> &barrier);
+    Extents at(Int3 c) const;
+};
+
+// Emissive faces with their power CDF and light tree, packed as the words the
+// kernels read (gpu.rs:686-713, tree.rs:170-192).
+struct EmitterTable {
+    std::vector<uint32_t> words;
+    uint32_t faces = 0;
+    uint32_t nodes = 0;
+    float total_weight = 0.0f;
+};
+
+// Material lookup over the loaded scene: AIR for air, 0xffff outside it.
+struct VoxelSource {
+    const uint16_t *grid;  // dense box, x fastest
+    Int3 origin;           // voxel coordinate of grid[0]
+    int gx, gy, gz;
+    const uint8_t *loaded; // per region of the box
+    Int3 region_min;
+    int rsx, rsy, rsz;
+    uint32_t material_at(int32_t x, int32_t y, int32_t z) const;
+};
+
+// emission.rs:91-165 + tree.rs:28-74 over the regions listed in z, y, x
+// order (those holding an emitting material).
+void build_emitters(const VoxelSource &scene, const std::vector<Int3> &regions, const float *emission_rgb, EmitterTable &out);
+
+// gpu/face_index.rs: open addressing, linear probing, home =
+// stream_id(voxel, face) & mask, capacity doubling at load 1/2.
+struct FaceSlots {
+    std::vector<std::array<uint32_t, 4>> entries;
+    size_t live = 0;
+    bool dirty = true;
+    FaceSlots();
+    long find(int32_t x, int32_t y, int32_t z, uint32_t face) const;
+    void insert(int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t slot);
+};
+
+}  // namespace vgi_host

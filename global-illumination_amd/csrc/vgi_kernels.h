@@ -1,0 +1,52 @@
+// Kernel launch interface shared by the host library and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vgi {
+
+struct SceneView;
+struct Optics;
+
+// Per listed region: the classification of world.rs:243-319 plus the packing
+// offsets the scan assigns (words into the cells buffer).
+struct RegionInfo {
+    uint32_t storage;  // 0 empty, 1 uniform, 2 dense, 3 all bricks uniform
+    uint32_t uniform;
+    uint32_t any_solid;
+    uint32_t table;    // 1 when the region packs a 64-word brick table
+    uint32_t mixed_lo, mixed_hi;        // bricks that need a 32-word payload
+    uint32_t occupied_lo, occupied_hi;  // bricks holding a solid voxel
+    uint32_t mixed_count, table_rank, brick_rank, table_offset;
+    uint32_t brick_offset, pad0, pad1, pad2;
+    uint16_t brick_material[64];  // first voxel's material per brick
+};
+
+struct AccumulateParams {
+    uint32_t history_limit;
+    uint32_t samples;
+    uint32_t clock_ms;
+    float recency;
+    float physical;  // measured retention c
+    float response;  // display response seconds
+};
+
+void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
+                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, hipStream_t stream);
+void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
+                     RegionInfo *info, hipStream_t stream);
+void launch_scan(RegionInfo *info, uint32_t n, uint32_t cells_base, uint32_t *totals, hipStream_t stream);
+void launch_pack(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
+                 const RegionInfo *info, uint32_t *cells, hipStream_t stream);
+void launch_headers(const int4 *list, uint32_t n, const RegionInfo *info, const uint32_t *top, uint4 *headers, int rx, int ry,
+                    int rz, int sx, int sy, hipStream_t stream);
+void launch_top(const uint32_t *top, uint32_t cells, uint4 *headers, hipStream_t stream);
+void launch_scatter_voxels(uint16_t *grid, const uint32_t *index, const uint16_t *values, uint32_t n, hipStream_t stream);
+void launch_prepare(uint4 *tasks, const uint32_t *slots, uint32_t n, uint32_t *state, uint32_t *meta, float physical,
+                    hipStream_t stream);
+void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint32_t *state, uint32_t *checkpoint, uint32_t *meta,
+                       const AccumulateParams &p, hipStream_t stream);
+void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t high_water, uint32_t *state, uint32_t *checkpoint,
+                         uint32_t *meta, float physical, hipStream_t stream);
+
+}  // namespace vgi

@@ -1,0 +1,617 @@
+// HIP kernels for gfx950 (CDNA4, wave64): face tracing, GPU NAADF build and
+// packing, and the resident world cache passes.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vgi_kernels.h"
+#include "vgi_trace.h"
+
+namespace vgi {
+
+// ============================================================== face tracing
+// voxel_trace.wgsl:44-92 (trace_moments + trace_faces_parallel): `lanes` lanes
+// share a face; lane l traces samples l, l+lanes, ...; lane 0 adds the lanes'
+// moments in lane order (zero-initialised, as the reference's workgroup sum)
+// and writes the 96-byte record after one more analytic direct estimate
+// (write_result, voxel_trace.wgsl:25-37). lanes divides 64, so a face's lanes
+// share a wavefront and the sum uses cross-lane reads instead of LDS.
+__global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optics optics, const uint4 *__restrict__ tasks,
+                                                          uint32_t count, uint32_t samples, uint32_t lanes,
+                                                          float4 *__restrict__ results, uint32_t *__restrict__ hits,
+                                                          unsigned long long *__restrict__ counters) {
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t face_index = tid / lanes;
+    uint32_t lane = tid % lanes;
+    bool valid = face_index < count;
+    uint4 t0 = make_uint4(0u, 0u, 0u, NONE), t1 = make_uint4(0u, 0u, 0u, 0u);
+    if (valid) {
+        t0 = tasks[face_index * 2u];
+        t1 = tasks[face_index * 2u + 1u];
+        valid = t0.w != NONE;
+    }
+    Tally tally{0u, 0u};
+    f3 mean = mk(0.0f, 0.0f, 0.0f), second = mk(0.0f, 0.0f, 0.0f);
+    uint32_t known = 0u;
+    int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
+    uint32_t face = t0.w;
+    f3 normal = mk(0.0f, 0.0f, 0.0f);
+    if (valid) {
+        normal = face_normal(face);
+        f3 surface = mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f;
+        uint32_t stream = hash_voxel(vx, vy, vz) ^ pcg(face + 1u);
+        for (uint32_t i = lane; i < samples; i += lanes) {
+            bool k;
+            FirstHit first;
+            uint32_t rays_before = tally.rays;
+            f3 value = trace_receiver(scene, optics, surface, normal, stream, t1.x + i, k, first, tally);
+            if (hits) {
+                uint4 *h = reinterpret_cast<uint4 *>(hits + ((size_t)face_index * samples + i) * 8u);
+                h[0] = make_uint4((uint32_t)first.vx, (uint32_t)first.vy, (uint32_t)first.vz, first.face | (first.status << 8));
+                h[1] = make_uint4(first.material, first.slot, tally.rays - rays_before, k ? 1u : 0u);
+            }
+            if (!k) continue;
+            mean = mean + value;
+            second = second + value * value;
+            known += 1u;
+        }
+    }
+    // Lane 0 gathers the face's lanes in order.
+    uint32_t base = (threadIdx.x & 63u) - lane;
+    f3 tmean = mk(0.0f, 0.0f, 0.0f), tsecond = mk(0.0f, 0.0f, 0.0f);
+    uint32_t tknown = 0u;
+    for (uint32_t l = 0u; l < lanes; ++l) {
+        int src = (int)(base + l);
+        f3 m = mk(__shfl(mean.x, src, 64), __shfl(mean.y, src, 64), __shfl(mean.z, src, 64));
+        f3 q = mk(__shfl(second.x, src, 64), __shfl(second.y, src, 64), __shfl(second.z, src, 64));
+        uint32_t kn = (uint32_t)__shfl((int)known, src, 64);
+        tmean = tmean + m;
+        tsecond = tsecond + q;
+        tknown += kn;
+    }
+    if (valid && lane == 0u) {
+        f3 dv, dc;
+        direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc, tally);
+        float inverse = 1.0f / (float)(tknown > 1u ? tknown : 1u);
+        float4 *r = results + (size_t)face_index * 6u;
+        r[0] = make_float4(tmean.x * inverse, tmean.y * inverse, tmean.z * inverse, 0.0f);
+        r[1] = make_float4(0.0f, (float)tknown, 0.0f, 0.0f);
+        r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
+        r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+        r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
+    }
+    if (counters) {
+        unsigned long long rays = tally.rays, steps = tally.steps;
+        for (int off = 32; off > 0; off >>= 1) {
+            rays += __shfl_xor(rays, off, 64);
+            steps += __shfl_xor(steps, off, 64);
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicAdd(&counters[0], rays);
+            atomicAdd(&counters[1], steps);
+        }
+    }
+}
+
+// ============================================================== NAADF build
+// One wavefront computes the NAADF of a 4^3 grid, one cell per lane (lane =
+// x + 4y + 16z): 3 rounds of x, y, z Jacobi sweeps (naadf.rs:65-112, rounds =
+// max(size) - 1). A cell's six u8 extents travel as two u32 (bytes 0-3, 4-5);
+// neighbours along x, y, z are lanes +-1, +-4, +-16 of the same wavefront, read
+// with cross-lane shuffles. The sweep from zero reaches the closed forms of
+// naadf.rs:42-63 for uniform grids (naadf/construction_tests.rs:110-145).
+__device__ __forceinline__ uint32_t ext_get(uint32_t lo, uint32_t hi, int d) {
+    return d < 4 ? (lo >> (d * 8)) & 255u : (hi >> ((d - 4) * 8)) & 255u;
+}
+__device__ __forceinline__ void aadf_4cubed(bool blocked, uint32_t lane, uint32_t &lo, uint32_t &hi) {
+    lo = 0u;
+    hi = 0u;
+    int x = (int)(lane & 3u), y = (int)((lane >> 2) & 3u), z = (int)(lane >> 4);
+    for (int round = 0; round < 3; ++round) {
+        for (int axis = 0; axis < 3; ++axis) {
+            int stride = axis == 0 ? 1 : (axis == 1 ? 4 : 16);
+            int coord = axis == 0 ? x : (axis == 1 ? y : z);
+            // Neighbour data (source = previous sweep), fetched by every lane.
+            int nlane = (int)lane - stride, plane = (int)lane + stride;
+            uint32_t nlo = (uint32_t)__shfl((int)lo, nlane < 0 ? 0 : nlane, 64);
+            uint32_t nhi = (uint32_t)__shfl((int)hi, nlane < 0 ? 0 : nlane, 64);
+            uint32_t nbl = (uint32_t)__shfl((int)blocked, nlane < 0 ? 0 : nlane, 64);
+            uint32_t plo = (uint32_t)__shfl((int)lo, plane > 63 ? 63 : plane, 64);
+            uint32_t phi = (uint32_t)__shfl((int)hi, plane > 63 ? 63 : plane, 64);
+            uint32_t pbl = (uint32_t)__shfl((int)blocked, plane > 63 ? 63 : plane, 64);
+            uint32_t cur[6];
+            for (int d = 0; d < 6; ++d) cur[d] = ext_get(lo, hi, d);
+            if (!blocked) {
+                // try_grow (naadf.rs:117-148) toward -axis, then +axis, reading
+                // the neighbour's previous-sweep extents.
+                for (int side = 0; side < 2; ++side) {
+                    int direction = axis * 2 + side;
+                    bool inside = side == 0 ? coord > 0 : coord < 3;
+                    uint32_t ol = side == 0 ? nlo : plo, oh = side == 0 ? nhi : phi;
+                    bool ob = side == 0 ? nbl != 0u : pbl != 0u;
+                    if (!inside || cur[direction] == 255u || ob) continue;
+                    int opposite = direction ^ 1;
+                    bool grow = true;
+                    for (int d = 0; d < 6; ++d)
+                        if (d != opposite && ext_get(ol, oh, d) < cur[d]) grow = false;
+                    if (grow) cur[direction] += 1u;
+                }
+                lo = cur[0] | (cur[1] << 8) | (cur[2] << 16) | (cur[3] << 24);
+                hi = cur[4] | (cur[5] << 8);
+            } else {
+                lo = 0u;
+                hi = 0u;
+            }
+        }
+    }
+}
+// gpu_scene.rs:462-467: six 2-bit extents, NEG_X in bits 0-1 ... POS_Z in 10-11.
+__device__ __forceinline__ uint32_t pack_extents2(uint32_t lo, uint32_t hi) {
+    uint32_t bits = 0u;
+    for (int d = 0; d < 6; ++d) bits |= ext_get(lo, hi, d) << (d * 2);
+    return bits;
+}
+
+// world.rs:243-319 (NaadfRegion::build) per region, one 256-thread block per
+// listed region: storage class, per-brick uniformity, brick materials and
+// occupancy. Voxels come from the dense u16 grid (x fastest).
+__global__ __launch_bounds__(256) void classify_regions_kernel(const uint16_t *__restrict__ grid, int gx, int gy, int gz,
+                                                               int vminx, int vminy, int vminz, const int4 *__restrict__ list,
+                                                               uint32_t list_count, RegionInfo *__restrict__ info) {
+    __shared__ uint16_t mats[4096];
+    __shared__ uint32_t flags[4];
+    uint32_t r = blockIdx.x;
+    if (r >= list_count) return;
+    int4 reg = list[r];
+    int ox = reg.x * 16 - vminx, oy = reg.y * 16 - vminy, oz = reg.z * 16 - vminz;
+    // 256 rows of 16 voxels: row = y + 16 z.
+    {
+        int row = (int)threadIdx.x;
+        int y = row & 15, z = row >> 4;
+        const uint16_t *src = grid + (size_t)(ox) + (size_t)gx * ((size_t)(oy + y) + (size_t)gy * (size_t)(oz + z));
+        for (int x = 0; x < 16; ++x) mats[row * 16 + x] = src[x];
+    }
+    if (threadIdx.x < 4) flags[threadIdx.x] = 0u;
+    __syncthreads();
+    uint16_t first = mats[0];
+    bool any_solid_local = false, all_same_local = true;
+    for (int i = (int)threadIdx.x; i < 4096; i += 256) {
+        any_solid_local |= mats[i] != AIR;
+        all_same_local &= mats[i] == first;
+    }
+    if (any_solid_local) atomicOr(&flags[0], 1u);
+    if (!all_same_local) atomicOr(&flags[1], 1u);
+    // Bricks: thread b < 64 scans brick b (x fastest brick order).
+    uint32_t brick_uniform = 1u, brick_solid = 0u;
+    uint16_t brick_first = AIR;
+    if (threadIdx.x < 64) {
+        int b = (int)threadIdx.x;
+        int bx = b & 3, by = (b >> 2) & 3, bz = b >> 4;
+        brick_first = mats[(bz * 4) * 256 + (by * 4) * 16 + bx * 4];
+        for (int i = 0; i < 64; ++i) {
+            int lx = i & 3, ly = (i >> 2) & 3, lz = i >> 4;
+            uint16_t m = mats[(bz * 4 + lz) * 256 + (by * 4 + ly) * 16 + bx * 4 + lx];
+            if (m != brick_first) brick_uniform = 0u;
+            if (m != AIR) brick_solid = 1u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // wave 0 holds the 64 bricks: ballots give the masks.
+        uint64_t uniform_mask = __ballot(brick_uniform != 0u);
+        uint64_t solid_mask = __ballot(brick_solid != 0u);
+        if (threadIdx.x < 64) info[r].brick_material[threadIdx.x] = brick_first;
+        if (threadIdx.x == 0) {
+            RegionInfo &ri = info[r];
+            bool any_solid = flags[0] != 0u;
+            bool all_same = flags[1] == 0u;
+            uint32_t storage;
+            if (!any_solid) storage = 0u;                 // Empty
+            else if (all_same) storage = 1u;              // Uniform
+            else if (uniform_mask == ~0ull) storage = 3u; // Bricks
+            else storage = 2u;                            // Dense
+            ri.storage = storage;
+            ri.uniform = first;
+            uint64_t mixed = storage == 2u ? ~uniform_mask : 0ull;
+            ri.mixed_lo = (uint32_t)mixed;
+            ri.mixed_hi = (uint32_t)(mixed >> 32);
+            // any_solid (world.rs:308): Dense from brick occupancy, Bricks from
+            // any non-air brick; both equal "some voxel is solid" here.
+            ri.any_solid = any_solid ? 1u : 0u;
+            ri.occupied_lo = (uint32_t)solid_mask;
+            ri.occupied_hi = (uint32_t)(solid_mask >> 32);
+            ri.mixed_count = (uint32_t)__popcll(mixed);
+            ri.table = storage >= 2u ? 1u : 0u;
+        }
+    }
+}
+
+// Exclusive scans of table and mixed-brick counts over the listed regions, in
+// list order, one block of 1024 threads looping over chunks. Offsets start at
+// `cells_base` (words): tables first, then bricks.
+__global__ __launch_bounds__(1024) void scan_regions_kernel(RegionInfo *__restrict__ info, uint32_t count, uint32_t cells_base,
+                                                            uint32_t *__restrict__ totals) {
+    __shared__ uint32_t sums_t[1024], sums_b[1024];
+    __shared__ uint32_t carry_t, carry_b;
+    if (threadIdx.x == 0) { carry_t = 0u; carry_b = 0u; }
+    __syncthreads();
+    // Pass 1: tables.
+    uint32_t total_tables = 0u;
+    for (uint32_t start = 0; start < count; start += 1024u) {
+        uint32_t i = start + threadIdx.x;
+        uint32_t vt = i < count ? info[i].table : 0u;
+        uint32_t vb = i < count ? info[i].mixed_count : 0u;
+        sums_t[threadIdx.x] = vt;
+        sums_b[threadIdx.x] = vb;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024u; off <<= 1) {
+            uint32_t at = threadIdx.x >= off ? sums_t[threadIdx.x - off] : 0u;
+            uint32_t ab = threadIdx.x >= off ? sums_b[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sums_t[threadIdx.x] += at;
+            sums_b[threadIdx.x] += ab;
+            __syncthreads();
+        }
+        if (i < count) {
+            info[i].table_rank = carry_t + sums_t[threadIdx.x] - vt;
+            info[i].brick_rank = carry_b + sums_b[threadIdx.x] - vb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023u) { carry_t += sums_t[1023]; carry_b += sums_b[1023]; }
+        __syncthreads();
+    }
+    total_tables = carry_t;
+    if (threadIdx.x == 0) {
+        totals[0] = carry_t;
+        totals[1] = carry_b;
+    }
+    // Offsets in words.
+    for (uint32_t i = threadIdx.x; i < count; i += 1024u) {
+        info[i].table_offset = cells_base + info[i].table_rank * 64u;
+        info[i].brick_offset = cells_base + total_tables * 64u + info[i].brick_rank * 32u;
+    }
+}
+
+// gpu_scene.rs:186-239 (brick table and mixed-brick payloads), one 256-thread
+// block per listed region with a table: wavefront w handles bricks w, w+4, ...
+// Brick bounds (world.rs:372-384) come from the occupancy mask, voxel bounds
+// (world.rs:352-370) from each mixed brick's own voxels.
+__global__ __launch_bounds__(256) void pack_regions_kernel(const uint16_t *__restrict__ grid, int gx, int gy, int gz, int vminx,
+                                                           int vminy, int vminz, const int4 *__restrict__ list,
+                                                           uint32_t list_count, const RegionInfo *__restrict__ info,
+                                                           uint32_t *__restrict__ cells) {
+    __shared__ uint16_t mats[4096];
+    __shared__ uint32_t brick_extents[64];
+    uint32_t r = blockIdx.x;
+    if (r >= list_count) return;
+    const RegionInfo &ri = info[r];
+    if (ri.table == 0u) return;
+    int4 reg = list[r];
+    int ox = reg.x * 16 - vminx, oy = reg.y * 16 - vminy, oz = reg.z * 16 - vminz;
+    {
+        int row = (int)threadIdx.x;
+        int y = row & 15, z = row >> 4;
+        const uint16_t *src = grid + (size_t)(ox) + (size_t)gx * ((size_t)(oy + y) + (size_t)gy * (size_t)(oz + z));
+        for (int x = 0; x < 16; ++x) mats[row * 16 + x] = src[x];
+    }
+    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    uint64_t occupied = ((uint64_t)ri.occupied_hi << 32) | ri.occupied_lo;
+    uint64_t mixed = ((uint64_t)ri.mixed_hi << 32) | ri.mixed_lo;
+    if (wave == 0) {
+        uint32_t lo, hi;
+        aadf_4cubed(((occupied >> lane) & 1ull) != 0ull, lane, lo, hi);
+        brick_extents[lane] = pack_extents2(lo, hi);
+    }
+    __syncthreads();
+    for (uint32_t b = wave; b < 64u; b += 4u) {
+        uint32_t word;
+        uint16_t bm = ri.brick_material[b];
+        bool is_mixed = ((mixed >> b) & 1ull) != 0ull;
+        if (!is_mixed) {
+            word = bm == AIR ? (brick_extents[b] << 2) : (((uint32_t)bm << 2) | 1u);
+        } else {
+            uint32_t bx = b & 3u, by = (b >> 2) & 3u, bz = b >> 4;
+            uint32_t lx = lane & 3u, ly = (lane >> 2) & 3u, lz = lane >> 4;
+            uint16_t m = mats[(bz * 4u + lz) * 256u + (by * 4u + ly) * 16u + bx * 4u + lx];
+            bool blocked = m != AIR;
+            uint32_t lo, hi;
+            aadf_4cubed(blocked, lane, lo, hi);
+            uint32_t cell = blocked ? (((uint32_t)m << 1) | 1u) : (pack_extents2(lo, hi) << 1);
+            uint32_t other = (uint32_t)__shfl((int)cell, (int)(lane | 1u), 64);
+            uint32_t rank = (uint32_t)__popcll(mixed & ((1ull << b) - 1ull));
+            uint32_t pointer = ri.brick_offset + rank * 32u;
+            if ((lane & 1u) == 0u) cells[pointer + (lane >> 1)] = cell | (other << 16);
+            word = (pointer << 2) | 2u;
+        }
+        if (lane == 0u) cells[ri.table_offset + b] = word;
+    }
+}
+
+// Region headers (gpu_scene.rs:174-195): kind 1 empty / 2 uniform / 3 table,
+// with the top-level extents supplied by the host (2 words per listed region).
+__global__ void write_headers_kernel(const int4 *__restrict__ list, uint32_t list_count, const RegionInfo *__restrict__ info,
+                                     const uint32_t *__restrict__ top, uint4 *__restrict__ headers, int rminx, int rminy, int rminz,
+                                     int rsx, int rsy) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= list_count) return;
+    int4 reg = list[i];
+    const RegionInfo &ri = info[i];
+    uint4 h;
+    if (ri.storage == 0u) h = make_uint4(1u, AIR, top[i * 2], top[i * 2 + 1]);
+    else if (ri.storage == 1u) h = make_uint4(2u, ri.uniform, top[i * 2], top[i * 2 + 1]);
+    else h = make_uint4(3u, ri.table_offset, top[i * 2], top[i * 2 + 1]);
+    size_t at = (size_t)(reg.x - rminx) + (size_t)rsx * ((size_t)(reg.y - rminy) + (size_t)rsy * (size_t)(reg.z - rminz));
+    headers[at] = h;
+}
+
+// Top-level extents of every loaded region (header words z, w).
+__global__ void write_top_kernel(const uint32_t *__restrict__ top, uint32_t cells, uint4 *__restrict__ headers) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    uint4 h = headers[i];
+    if (h.x == 0u) return;
+    h.z = top[i * 2];
+    h.w = top[i * 2 + 1];
+    headers[i] = h;
+}
+
+// Voxel edits into the dense grid.
+__global__ void scatter_voxels_kernel(uint16_t *__restrict__ grid, const uint32_t *__restrict__ index,
+                                      const uint16_t *__restrict__ values, uint32_t count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) grid[index[i]] = values[i];
+}
+
+// ============================================================== resident cache
+// prepare (voxel_accumulate.wgsl:83-133) for listed faces: a traced face takes
+// its slot's sequence (begin_trace); then mark_changed (:432-446).
+__device__ __forceinline__ f3 load3(const uint32_t *w) { return mk(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2])); }
+__device__ __forceinline__ bool binds(const uint32_t *st, float change, float physical) {
+    f3 w = mk(0.2126f, 0.7152f, 0.0722f);
+    f3 mean = load3(st);
+    f3 sec = load3(st + 8);
+    f3 var = mk(fmx(sec.x - mean.x * mean.x, 0.0f), fmx(sec.y - mean.y * mean.y, 0.0f), fmx(sec.z - mean.z * mean.z, 0.0f));
+    bool lit = dot3(w, mean) >= 0.05f * change;
+    float keep = physical * dot3(w, var) / fmx(change * change, 1e-30f);
+    return !lit || keep < (float)(st[5] & 0x7fffffffu);
+}
+
+__global__ void prepare_kernel(uint4 *__restrict__ tasks, const uint32_t *__restrict__ slots, uint32_t count,
+                               uint32_t *__restrict__ state, uint32_t *__restrict__ meta, float physical) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t s = slots[i];
+    if (s == NONE) return;
+    uint32_t *st = state + (size_t)s * 36u;
+    tasks[i * 2u + 1u].x = st[6];
+    uint32_t *pw = meta + (size_t)s * 12u + 4u;
+    float pending = __uint_as_float(*pw);
+    if (pending > 0.0f) {
+        if (binds(st, pending, physical)) st[5] |= 0x80000000u;
+        else *pw = 0u;
+    }
+}
+
+// accumulate (voxel_accumulate.wgsl:135-227): merge the batch's raw moments
+// into the slot with the history cap, recency and measured-retention keep,
+// then the display EMA. record_stability (selection ranking) is not part of
+// this path and leaves the settled flag clear.
+__global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t count, float4 *__restrict__ results,
+                                  uint32_t *__restrict__ state, uint32_t *__restrict__ checkpoint, uint32_t *__restrict__ meta,
+                                  AccumulateParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t s = slots[i];
+    if (s == NONE) return;
+    uint32_t *st = state + (size_t)s * 36u;
+    uint32_t *cp = checkpoint + (size_t)s * 12u;
+    uint32_t *out = reinterpret_cast<uint32_t *>(results + (size_t)i * 6u);
+    cp[0] = st[0]; cp[1] = st[1]; cp[2] = st[2]; cp[3] = st[3];
+    cp[4] = st[8]; cp[5] = st[9]; cp[6] = st[10]; cp[7] = st[11];
+    cp[8] = st[4]; cp[9] = st[5] & 0x7fffffffu; cp[10] = st[12]; cp[11] = 0u;
+    uint32_t fresh_count = (uint32_t)__uint_as_float(out[5]);
+    uint32_t *pw = meta + (size_t)s * 12u + 4u;
+    float pending = __uint_as_float(*pw);
+    if (fresh_count > 0u) *pw = 0u;
+    uint32_t previous_count = st[5] & 0x7fffffffu;
+    float physical = p.physical;
+    float change = (st[5] & 0x80000000u) != 0u ? pending : 0.0f;
+    if (physical > 0.0f && fresh_count > 0u && change > 0.0f) {
+        f3 w = mk(0.2126f, 0.7152f, 0.0722f);
+        f3 om = load3(st), os = load3(st + 8);
+        f3 ov = mk(fmx(os.x - om.x * om.x, 0.0f), fmx(os.y - om.y * om.y, 0.0f), fmx(os.z - om.z * om.z, 0.0f));
+        f3 fm = load3(out), fs = load3(out + 8);
+        f3 fv = mk(fmx(fs.x - fm.x * fm.x, 0.0f), fmx(fs.y - fm.y * fm.y, 0.0f), fmx(fs.z - fm.z * fm.z, 0.0f));
+        float variance = dot3(w, om) < 0.05f * change ? dot3(w, fv) : dot3(w, ov);
+        float keep = physical * variance / (change * change);
+        uint32_t kept = (uint32_t)fmn(keep, 65536.0f);
+        previous_count = previous_count < kept ? previous_count : kept;
+        cp[9] = cp[9] < previous_count ? cp[9] : previous_count;
+    }
+    uint32_t total = previous_count + fresh_count;
+    uint32_t cnt = total < p.history_limit ? total : p.history_limit;
+    float alpha = fresh_count > 0u ? fmn(p.recency * (float)fresh_count / (float)(cnt > 1u ? cnt : 1u), 1.0f) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float pm = __uint_as_float(st[k]), m = __uint_as_float(out[k]);
+        st[k] = __float_as_uint(pm + (m - pm) * alpha);
+        float ps = __uint_as_float(st[8 + k]), q = __uint_as_float(out[8 + k]);
+        st[8 + k] = __float_as_uint(ps + (q - ps) * alpha);
+    }
+    st[5] = cnt;
+    st[6] = st[6] + p.samples;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) st[16 + k] = out[16 + k];
+    f3 radiance = load3(st), moment = load3(st + 8);
+    float fc = fmx((float)cnt, 1.0f);
+    f3 error = mk(fsqrt(fmx(moment.x - radiance.x * radiance.x, 0.0f) / fc), fsqrt(fmx(moment.y - radiance.y * radiance.y, 0.0f) / fc),
+                  fsqrt(fmx(moment.z - radiance.z * radiance.z, 0.0f) / fc));
+    float uncertainty = cnt < 2u ? 1.0f
+                                 : fmn(8.0f, max3(mk(error.x / (0.02f + fabsf(radiance.x)), error.y / (0.02f + fabsf(radiance.y)),
+                                                     error.z / (0.02f + fabsf(radiance.z)))));
+    float noise = cnt < 2u ? 1.0f
+                           : fmn(8.0f, max3(mk(error.x / (0.002f + fabsf(radiance.x)), error.y / (0.002f + fabsf(radiance.y)),
+                                               error.z / (0.002f + fabsf(radiance.z)))));
+    float elapsed = st[30] != 0u ? (float)(p.clock_ms - st[29]) * 0.001f : 1.0f / 60.0f;
+    float time_alpha = 1.0f - exp_own(-elapsed / p.response);
+    float evidence = (float)cnt / ((float)cnt + 32.0f);
+    float blend = fmn(time_alpha, evidence / (1.0f + 16.0f * noise * noise));
+    f3 prev = load3(st + 24);
+    float prev_a = __uint_as_float(st[27]);
+    float old_weight = (1.0f - blend) / (0.25f + max3(prev));
+    float new_weight = blend / (0.25f + max3(radiance));
+    float alpha_display = fmn(blend, new_weight / fmx(old_weight + new_weight, 0.0000001f));
+    float confidence = fclamp((float)cnt / ((float)cnt + 8.0f) / (1.0f + uncertainty), 0.125f, 1.0f);
+    f3 disp = prev * (1.0f - alpha_display) + radiance * alpha_display;
+    float disp_a = prev_a * (1.0f - blend) + confidence * blend;
+    st[24] = __float_as_uint(disp.x); st[25] = __float_as_uint(disp.y); st[26] = __float_as_uint(disp.z);
+    st[27] = __float_as_uint(disp_a);
+    st[29] = p.clock_ms;
+    st[30] = 0xffffffffu;
+    st[7] = st[24];
+    st[13] = st[25]; st[14] = st[26]; st[15] = st[27];
+    uint4 *o4 = reinterpret_cast<uint4 *>(out);
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(st);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o4[k] = s4[k];
+}
+
+// voxel_retention.wgsl:14-18
+__device__ __forceinline__ uint32_t box_distance2(int lx, int ly, int lz, int hx, int hy, int hz, int px, int py, int pz) {
+    int sx = max(max(lx - px, px - hx), 0), sy = max(max(ly - py, py - hy), 0), sz = max(max(lz - pz, pz - hz), 0);
+    uint32_t ux = (uint32_t)min(sx, 4096), uy = (uint32_t)min(sy, 4096), uz = (uint32_t)min(sz, 4096);
+    return ux * ux + uy * uy + uz * uz;
+}
+__device__ __forceinline__ uint32_t curve_limit(uint32_t d2) {
+    if (d2 >= 36864u) return NONE;
+    return (d2 * d2 + 10368u) / 20736u;
+}
+
+// apply_events (voxel_accumulate.wgsl:295-332) over every slot below the
+// high-water mark, with resolve_retention (voxel_retention.wgsl:29-96).
+__global__ void apply_events_kernel(const uint4 *__restrict__ events, uint32_t event_count, uint32_t high_water,
+                                    uint32_t *__restrict__ state, uint32_t *__restrict__ checkpoint, uint32_t *__restrict__ meta,
+                                    float physical, uint32_t reach, uint32_t urgent_measured, uint32_t urgent_curve) {
+    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= high_water) return;
+    uint32_t *id = meta + (size_t)slot * 12u;
+    if ((id[3] & 8u) == 0u) return;
+    int px = (int)id[0], py = (int)id[1], pz = (int)id[2];
+    bool measured_mode = physical > 0.0f;
+    uint32_t cap = NONE, source_cap = NONE;
+    float change = 0.0f;
+    bool known = true, woken = false, urgent_if_binding = false, urgent = false;
+    for (uint32_t e = 0u; e < event_count; ++e) {
+        uint4 h0 = events[e * 3u], h1 = events[e * 3u + 1u], h2 = events[e * 3u + 2u];
+        if ((h0.w & 2u) != 0u) {
+            uint32_t limit = h2.x;
+            for (uint32_t b = 0u; b < h2.w; ++b) {
+                uint4 sp = events[event_count * 3u + (h2.z + b) * 2u];
+                f3 d = mk((float)px, (float)py, (float)pz) - mk(__uint_as_float(sp.x), __uint_as_float(sp.y), __uint_as_float(sp.z));
+                if (dot3(d, d) <= __uint_as_float(sp.w)) limit = min(limit, 4u);
+            }
+            source_cap = min(source_cap, limit);
+            woken = true;
+            continue;
+        }
+        uint32_t outside = box_distance2((int)h0.x, (int)h0.y, (int)h0.z, (int)h1.x, (int)h1.y, (int)h1.z, px, py, pz);
+        if (outside > reach) continue;
+        woken = true;
+        bool exact = known;
+        uint32_t nearest = NONE;
+        for (uint32_t b = 0u; b < h2.w; ++b) {
+            uint32_t record = event_count * 3u + (h2.z + b) * 2u;
+            uint4 b0 = events[record], b1 = events[record + 1u];
+            uint32_t d = box_distance2((int)b0.x, (int)b0.y, (int)b0.z, (int)b1.x, (int)b1.y, (int)b1.z, px, py, pz);
+            if (d > reach) continue;
+            nearest = min(nearest, d);
+            uint32_t edit_kind = b1.w & 7u;
+            if ((b1.w & 8u) != 0u && (edit_kind == 0u || edit_kind == 3u)) {
+                if (edit_kind == 3u && measured_mode) urgent_if_binding = urgent_if_binding || d <= urgent_measured;
+                else urgent = urgent || d <= urgent_curve;
+            }
+            if (exact) {
+                uint32_t kind = b1.w & 7u;
+                if (kind == 0u) { known = false; exact = false; }
+                else if (kind != 1u) {
+                    float area = 1.0f;
+                    if (kind == 2u) area = (float)min(max(((int)b1.w >> 3) + 1 - py, 0), 16);
+                    change = change + __uint_as_float(b0.w) * area / (3.14159265358979f * (float)max(d, 1u));
+                }
+            }
+        }
+        if (nearest != NONE) cap = min(cap, curve_limit(nearest));
+    }
+    bool measured = measured_mode && known;
+    uint32_t limit = min(source_cap, measured ? NONE : cap);
+    uint32_t *st = state + (size_t)slot * 36u;
+    if (limit != NONE) {
+        st[5] = min(st[5], limit);
+        uint32_t *c = checkpoint + (size_t)slot * 12u + 9u;
+        *c = min(*c, limit);
+    }
+    uint32_t *pw = meta + (size_t)slot * 12u + 4u;
+    if (measured && change > 0.0f) *pw = __float_as_uint(__uint_as_float(*pw) + change);
+    if (urgent_if_binding) {
+        float pending = __uint_as_float(*pw);
+        urgent = urgent || !measured || (pending > 0.0f && binds(st, pending, physical));
+    }
+    if (woken || urgent) meta[(size_t)slot * 12u + 5u] &= 0xfffffff0u;
+    if (urgent) meta[(size_t)slot * 12u + 11u] |= 0x40000000u;
+}
+
+// ============================================================== launchers
+static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
+                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, hipStream_t stream) {
+    uint64_t threads = (uint64_t)count * lanes;
+    if (threads == 0) return;
+    hipLaunchKernelGGL(trace_faces_kernel, dim3(blocks(threads, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples,
+                       lanes, results, hits, counters);
+}
+void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
+                     RegionInfo *info, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(classify_regions_kernel, dim3(n), dim3(256), 0, stream, grid, gx, gy, gz, vx, vy, vz, list, n, info);
+}
+void launch_scan(RegionInfo *info, uint32_t n, uint32_t cells_base, uint32_t *totals, hipStream_t stream) {
+    hipLaunchKernelGGL(scan_regions_kernel, dim3(1), dim3(1024), 0, stream, info, n, cells_base, totals);
+}
+void launch_pack(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
+                 const RegionInfo *info, uint32_t *cells, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(pack_regions_kernel, dim3(n), dim3(256), 0, stream, grid, gx, gy, gz, vx, vy, vz, list, n, info, cells);
+}
+void launch_headers(const int4 *list, uint32_t n, const RegionInfo *info, const uint32_t *top, uint4 *headers, int rx, int ry,
+                    int rz, int sx, int sy, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(write_headers_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, list, n, info, top, headers, rx, ry, rz, sx,
+                       sy);
+}
+void launch_top(const uint32_t *top, uint32_t cells, uint4 *headers, hipStream_t stream) {
+    if (cells == 0) return;
+    hipLaunchKernelGGL(write_top_kernel, dim3(blocks(cells, 256)), dim3(256), 0, stream, top, cells, headers);
+}
+void launch_scatter_voxels(uint16_t *grid, const uint32_t *index, const uint16_t *values, uint32_t n, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(scatter_voxels_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, grid, index, values, n);
+}
+void launch_prepare(uint4 *tasks, const uint32_t *slots, uint32_t n, uint32_t *state, uint32_t *meta, float physical,
+                    hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(prepare_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, tasks, slots, n, state, meta, physical);
+}
+void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint32_t *state, uint32_t *checkpoint, uint32_t *meta,
+                       const AccumulateParams &p, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(accumulate_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, slots, n, results, state, checkpoint, meta, p);
+}
+void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t high_water, uint32_t *state, uint32_t *checkpoint,
+                         uint32_t *meta, float physical, hipStream_t stream) {
+    if (high_water == 0) return;
+    hipLaunchKernelGGL(apply_events_kernel, dim3(blocks(high_water, 256)), dim3(256), 0, stream, events, event_count, high_water,
+                       state, checkpoint, meta, physical, 192u * 192u, 64u * 64u, 48u * 48u);
+}
+
+}  // namespace vgi

@@ -1,0 +1,563 @@
+// Voxel traversal and the path estimator on the MI355X scene layout.
+//
+// Scene layout in HBM (one allocation each, all read-only while tracing):
+//   headers  uint4 per region cell of the loaded regions' bounding box, dense,
+//            x fastest: (kind, material-or-table-offset, top-level negative
+//            bytes, top-level positive bytes). kind 0 = not loaded, 1 empty,
+//            2 uniform solid, 3 brick table. One load replaces the
+//            reference's hashed directory probe + header load
+//            (voxel_traversal.wgsl:32-41 + :170).
+//   cells    u32: 64-word brick tables (word: extents<<2|0 air brick,
+//            mat<<2|1 uniform brick, ptr<<2|2 mixed brick), then 32-word
+//            mixed bricks of two 16-bit cells (mat<<1|1 solid, extents<<1
+//            air) -- the reference's cell encoding (gpu_scene.rs:199-236).
+// The traversal reads exactly the values the reference's directory/header
+// path yields, so boxes, steps and hits are identical.
+#pragma once
+#include "vgi_math.h"
+
+namespace vgi {
+
+struct SceneView {
+    const uint4 *__restrict__ headers;
+    const uint32_t *__restrict__ cells;
+    int minx, miny, minz;  // region bounds, inclusive
+    int maxx, maxy, maxz;
+    int sx, sy;            // region box size x, y
+};
+
+struct Optics {
+    const float4 *__restrict__ materials;   // 2 per material: (diffuse rgb, emission power), (emission rgb, 0)
+    const float4 *__restrict__ lights;      // 3 per light (gpu.rs:136-141)
+    const uint4 *__restrict__ emitters;     // 2 uint4 per emissive face, then 4 per tree node
+    uint32_t light_count;
+    uint32_t emitter_faces;                 // params.extra.w
+    float total_weight;                     // params.extra.z
+    uint32_t has_tree;
+    float env_r, env_g, env_b;
+    float max_distance;                     // params.extra.y
+    float reach;                            // params.transport.x
+    uint32_t limited_reach;
+    uint32_t bounces;                       // DIFFUSE_HITS
+    // world cache (voxel_path.wgsl:51-87)
+    const uint4 *__restrict__ cache_index;  // open-addressed (voxel, face<<29|slot), no header record
+    const uint32_t *__restrict__ cache_state;  // 36 words per slot
+    uint32_t cache_mask;                    // capacity - 1, 0 with no cache
+    uint32_t cache_on;
+};
+
+struct RayResult {
+    int vx, vy, vz;
+    uint32_t material;
+    f3 position;
+    uint32_t status;  // 0 escaped, 1 hit, 2 unknown, 3 cut at reach
+    f3 normal;
+};
+
+struct Tally {
+    uint32_t rays, steps;
+};
+
+__device__ __forceinline__ bool in_bounds(const SceneView &s, int rx, int ry, int rz) {
+    return rx >= s.minx && ry >= s.miny && rz >= s.minz && rx <= s.maxx && ry <= s.maxy && rz <= s.maxz;
+}
+// The region's header, kind 0 when not loaded (callers check bounds first).
+__device__ __forceinline__ uint4 region_header(const SceneView &s, int rx, int ry, int rz) {
+    size_t i = (size_t)(rx - s.minx) + (size_t)s.sx * ((size_t)(ry - s.miny) + (size_t)s.sy * (size_t)(rz - s.minz));
+    return s.headers[i];
+}
+
+// voxel_traversal.wgsl:90-107
+__device__ __forceinline__ void advance_box(f3 position, f3 direction, int cx, int cy, int cz, int nx, int ny, int nz,
+                                            int px, int py, int pz, float size, f3 &normal_out, float &dist_out) {
+    float mnx = (float)(cx - nx) * size, mny = (float)(cy - ny) * size, mnz = (float)(cz - nz) * size;
+    float mxx = (float)(cx + px + 1) * size, mxy = (float)(cy + py + 1) * size, mxz = (float)(cz + pz + 1) * size;
+    float tr[3];
+    const float dv[3] = {direction.x, direction.y, direction.z};
+    const float pv[3] = {position.x, position.y, position.z};
+    const float lo[3] = {mnx, mny, mnz}, hi[3] = {mxx, mxy, mxz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bool moving = fabsf(dv[a]) > 1e-8f;
+        float plane = dv[a] > 0.0f ? hi[a] : lo[a];
+        float raw = (plane - pv[a]) / (moving ? dv[a] : 1.0f);
+        tr[a] = (moving && raw >= -0.001f && raw < 1e30f) ? raw : 1e30f;
+    }
+    bool choose_y = tr[1] < fmx(tr[0], 0.0f);
+    float distance_xy = fmx(choose_y ? tr[1] : tr[0], 0.0f);
+    bool choose_z = tr[2] < distance_xy;
+    float distance = fmx(choose_z ? tr[2] : distance_xy, 0.0f);
+    f3 axis = choose_z ? mk(0.0f, 0.0f, 1.0f) : (choose_y ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f));
+    f3 sgn = mk(direction.x > 0.0f ? -1.0f : 1.0f, direction.y > 0.0f ? -1.0f : 1.0f, direction.z > 0.0f ? -1.0f : 1.0f);
+    normal_out = distance < 1e30f ? axis * sgn : mk(0.0f, 0.0f, 0.0f);
+    dist_out = fmx(distance + 0.001f, 0.001f);
+}
+
+// voxel_traversal.wgsl:108-121
+__device__ __noinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 direction, float max_distance) {
+    f3 position = start;
+    float remaining = max_distance;
+    for (uint32_t step = 0; step < 2048u; ++step) {
+        int rx = (int)floorf(position.x / 16.0f), ry = (int)floorf(position.y / 16.0f), rz = (int)floorf(position.z / 16.0f);
+        if (remaining < 0.0f || !in_bounds(s, rx, ry, rz)) return false;
+        uint4 h = region_header(s, rx, ry, rz);
+        if (h.x != 0u && h.x != 1u) return true;
+        f3 n;
+        float travel;
+        advance_box(position, direction, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, travel);
+        position = position + direction * travel;
+        remaining = remaining - travel;
+    }
+    return true;
+}
+
+// voxel_traversal.wgsl:126-219 with visible_geometry = false, max_steps = 2048
+// (the path tracer's `trace`, voxel_path.wgsl:239-243).
+__device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3 direction, float max_distance, Tally &tally) {
+    RayResult result;
+    result.vx = result.vy = result.vz = 0;
+    result.material = 0u;
+    result.position = mk(0.0f, 0.0f, 0.0f);
+    result.status = 0u;
+    result.normal = mk(0.0f, 0.0f, 0.0f);
+    tally.rays += 1u;
+    float near_ = -1e30f, far_ = 1e30f;
+    f3 normal = mk(0.0f, 0.0f, 0.0f);
+    const float dv[3] = {direction.x, direction.y, direction.z};
+    const float ov[3] = {origin.x, origin.y, origin.z};
+    const float lo[3] = {(float)s.minx * 16.0f, (float)s.miny * 16.0f, (float)s.minz * 16.0f};
+    const float hi[3] = {(float)(s.maxx + 1) * 16.0f, (float)(s.maxy + 1) * 16.0f, (float)(s.maxz + 1) * 16.0f};
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        if (fabsf(dv[axis]) <= 1e-8f) {
+            if (ov[axis] < lo[axis] || ov[axis] > hi[axis]) return result;
+            continue;
+        }
+        float first = (lo[axis] - ov[axis]) / dv[axis];
+        float second = (hi[axis] - ov[axis]) / dv[axis];
+        if (fmn(first, second) > near_) {
+            near_ = fmn(first, second);
+            float sg = first < second ? -1.0f : 1.0f;
+            normal = mk(axis == 0 ? sg : 0.0f, axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f);
+        }
+        far_ = fmn(far_, fmx(first, second));
+    }
+    if (far_ < fmx(near_, 0.0f)) return result;
+    float distance = fmx(near_, 0.0f) + 0.001f;
+    f3 position = origin + direction * distance;
+    int lrx = 0x7fffffff, lry = 0x7fffffff, lrz = 0x7fffffff;
+    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t step = 0; step < 2048u; ++step) {
+        tally.steps += 1u;
+        if (distance > max_distance) return result;
+        int vx = (int)floorf(position.x), vy = (int)floorf(position.y), vz = (int)floorf(position.z);
+        int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
+        if (!in_bounds(s, rx, ry, rz)) return result;
+        if (rx != lrx || ry != lry || rz != lrz) {
+            uint4 h = region_header(s, rx, ry, rz);
+            if (h.x == 0u) {
+                result.status = geometry_ahead(s, position, direction, max_distance - distance) ? 2u : 0u;
+                return result;
+            }
+            meta = h;
+            lrx = rx; lry = ry; lrz = rz;
+        }
+        f3 adv_n = mk(0.0f, 0.0f, 0.0f);
+        float adv_w = 0.0f;
+        if (meta.x == 1u) {
+            advance_box(position, direction, rx, ry, rz, (int)(meta.z & 255u), (int)((meta.z >> 8) & 255u),
+                        (int)((meta.z >> 16) & 255u), (int)(meta.w & 255u), (int)((meta.w >> 8) & 255u),
+                        (int)((meta.w >> 16) & 255u), 16.0f, adv_n, adv_w);
+        } else {
+            uint32_t material = meta.y;
+            if (meta.x == 3u) {
+                uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
+                uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
+                uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
+                uint32_t st = brick & 3u;
+                if (st == 0u) {
+                    uint32_t e = brick >> 2u;
+                    advance_box(position, direction, rx * 4 + (int)bx, ry * 4 + (int)by, rz * 4 + (int)bz, (int)(e & 3u),
+                                (int)((e >> 4) & 3u), (int)((e >> 8) & 3u), (int)((e >> 2) & 3u), (int)((e >> 6) & 3u),
+                                (int)((e >> 10) & 3u), 4.0f, adv_n, adv_w);
+                    material = AIR;
+                } else if (st == 1u) {
+                    material = brick >> 2u;
+                } else {
+                    uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
+                    uint32_t word = s.cells[(brick >> 2u) + (idx >> 1)];
+                    uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
+                    material = (cell & 1u) != 0u ? cell >> 1u : AIR;
+                    if (material == AIR) {
+                        uint32_t e = cell >> 1u;
+                        advance_box(position, direction, vx, vy, vz, (int)(e & 3u), (int)((e >> 4) & 3u), (int)((e >> 8) & 3u),
+                                    (int)((e >> 2) & 3u), (int)((e >> 6) & 3u), (int)((e >> 10) & 3u), 1.0f, adv_n, adv_w);
+                    }
+                }
+            }
+            if (material != AIR) {
+                result.vx = vx; result.vy = vy; result.vz = vz;
+                result.material = material;
+                result.position = position;
+                result.normal = normal;
+                result.status = 1u;
+                return result;
+            }
+        }
+        distance = distance + adv_w;
+        position = position + direction * adv_w;
+        normal = adv_n;
+    }
+    result.status = 2u;
+    result.material = NONE;
+    return result;
+}
+
+// ---------------------------------------------------------------- estimator (voxel_path.wgsl)
+struct Emitter {
+    int vx, vy, vz;
+    uint32_t face;
+    float cumulative, probability, material, tree;
+};
+__device__ __forceinline__ Emitter load_emitter(const Optics &o, uint32_t index) {
+    uint4 a = o.emitters[index * 2u];
+    uint4 b = o.emitters[index * 2u + 1u];
+    return Emitter{(int)a.x, (int)a.y, (int)a.z, a.w, __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
+                   __uint_as_float(b.w)};
+}
+struct Node {
+    f3 minimum;
+    float power;
+    f3 maximum;
+    uint32_t normals;
+    int px, py, pz;
+    uint32_t face;
+    uint4 children;
+};
+__device__ __forceinline__ Node load_node(const Optics &o, uint32_t index) {
+    uint32_t off = o.emitter_faces * 2u + index * 4u;
+    uint4 a = o.emitters[off], b = o.emitters[off + 1u], c = o.emitters[off + 2u], d = o.emitters[off + 3u];
+    Node n;
+    n.minimum = mk(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+    n.power = __uint_as_float(a.w);
+    n.maximum = mk(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
+    n.normals = b.w;
+    n.px = (int)c.x; n.py = (int)c.y; n.pz = (int)c.z;
+    n.face = c.w;
+    n.children = d;
+    return n;
+}
+// voxel_path.wgsl:108-122
+__device__ __forceinline__ float node_importance(const Node &node, f3 position, f3 normal) {
+    f3 delta = (node.minimum + node.maximum) * 0.5f - position;
+    float radius = len3(node.maximum - node.minimum) * 0.5f;
+    float distance = fmx(len3(delta), 1e-5f);
+    f3 direction = delta / distance;
+    float spread = fmn(radius / distance, 1.0f);
+    float receiver = fclamp(dot3(normal, direction) + spread, 0.0f, 1.0f);
+    float ppx = (node.normals & 1u) ? -direction.x : -1.0f, ppy = (node.normals & 4u) ? -direction.y : -1.0f,
+          ppz = (node.normals & 16u) ? -direction.z : -1.0f;
+    float pnx = (node.normals & 2u) ? direction.x : -1.0f, pny = (node.normals & 8u) ? direction.y : -1.0f,
+          pnz = (node.normals & 32u) ? direction.z : -1.0f;
+    float qx = fmx(ppx, pnx), qy = fmx(ppy, pny), qz = fmx(ppz, pnz);
+    float emitter = fclamp(fmx(fmx(qx, qy), qz) + spread, 0.0f, 1.0f);
+    return node.power * receiver * emitter / (distance * distance + radius * radius + 0.25f);
+}
+// voxel_path.wgsl:123-131
+__device__ __forceinline__ float node_left_probability(const Optics &o, const Node &node, f3 position, f3 normal) {
+    Node left = load_node(o, node.children.x);
+    Node right = load_node(o, node.children.y);
+    float power = left.power / (left.power + right.power);
+    float a = node_importance(left, position, normal);
+    float b = node_importance(right, position, normal);
+    if (a + b <= 1e-20f) return fclamp(power, 1e-6f, 1.0f - 1e-6f);
+    return fclamp(0.05f * power + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
+}
+// voxel_path.wgsl:132-149
+__device__ __noinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 normal, float choice) {
+    uint32_t index = 0u;
+    float uniform = choice;
+    float probability = 1.0f;
+    for (;;) {
+        Node node = load_node(o, index);
+        if (node.children.x == NONE) {
+            Emitter e = load_emitter(o, node.children.y);
+            e.probability = probability;
+            return e;
+        }
+        float p = node_left_probability(o, node, position, normal);
+        if (uniform < p) { probability = probability * p; uniform = uniform / p; index = node.children.x; }
+        else { probability = probability * (1.0f - p); uniform = (uniform - p) / (1.0f - p); index = node.children.y; }
+        uniform = fmn(uniform, __uint_as_float(0x3f7fffffu));
+    }
+}
+// voxel_path.wgsl:158-174 (key_less :150-157)
+__device__ __noinline__ float tree_probability(const Optics &o, f3 position, f3 normal, int vx, int vy, int vz, f3 hit_normal) {
+    uint32_t face = 0u;
+    for (uint32_t i = 0u; i < 6u; ++i) {
+        f3 fn = face_normal(i);
+        if (fn.x == hit_normal.x && fn.y == hit_normal.y && fn.z == hit_normal.z) face = i;
+    }
+    uint32_t index = 0u;
+    float probability = 1.0f;
+    for (;;) {
+        Node node = load_node(o, index);
+        if (node.children.x == NONE) {
+            Emitter f = load_emitter(o, node.children.y);
+            return (f.vx == vx && f.vy == vy && f.vz == vz && f.face == face) ? probability : 0.0f;
+        }
+        float p = node_left_probability(o, node, position, normal);
+        uint32_t axis = node.children.z;
+        const int v[3] = {vx, vy, vz};
+        const int pv[3] = {node.px, node.py, node.pz};
+        bool less = face < node.face;
+        for (uint32_t j = 0u; j < 3u; ++j) {
+            uint32_t i = (axis + j) % 3u;
+            if (v[i] != pv[i]) { less = v[i] < pv[i]; break; }
+        }
+        if (less) { probability = probability * p; index = node.children.x; }
+        else { probability = probability * (1.0f - p); index = node.children.y; }
+    }
+}
+// voxel_path.wgsl:309-324
+__device__ __forceinline__ Emitter choose_emitter(const Optics &o, f3 position, f3 normal, float choice) {
+    if (o.has_tree) return tree_emitter(o, position, normal, choice);
+    uint32_t low = 0u, high = o.emitter_faces;
+    while (low < high) {
+        uint32_t middle = (low + high) / 2u;
+        if (__uint_as_float(o.emitters[middle * 2u + 1u].x) <= choice) low = middle + 1u;
+        else high = middle;
+    }
+    return load_emitter(o, min(low, o.emitter_faces - 1u));
+}
+__device__ __forceinline__ float connection_reach(const Optics &o) { return o.limited_reach ? o.reach : o.max_distance; }
+
+// voxel_path.wgsl:332-364
+__device__ __noinline__ f3 sample_emitter(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t source_stream,
+                                          uint32_t sequence, uint32_t bounce, Tally &tally) {
+    f3 zero = mk(0.0f, 0.0f, 0.0f);
+    if (o.emitter_faces == 0u) return zero;
+    uint32_t stream = source_stream ^ pcg(bounce + 0xa341316cu);
+    Emitter e = choose_emitter(o, position, normal, sample_1d(stream, sequence, 12u));
+    f3 en = face_normal(e.face);
+    uint32_t axis = e.face / 2u;
+    f3 tangent = axis == 0u ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    f3 bitangent = axis == 2u ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f);
+    float u13 = sample_1d(stream, sequence, 13u);
+    float u14 = sample_1d(stream, sequence, 14u);
+    f3 light_position = mk((float)e.vx, (float)e.vy, (float)e.vz) + mk(0.5f, 0.5f, 0.5f) + en * 0.5f +
+                        tangent * (u13 - 0.5f) + bitangent * (u14 - 0.5f);
+    f3 delta = light_position - position;
+    float d2 = dot3(delta, delta);
+    float reach = connection_reach(o);
+    if (d2 < 0.0001f || d2 > reach * reach) return zero;
+    float distance = fsqrt(d2);
+    f3 direction = delta / distance;
+    float receiver_cosine = fmx(dot3(normal, direction), 0.0f);
+    float emitter_cosine = fmx(dot3(en, -direction), 0.0f);
+    if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || e.probability <= 0.0f) return zero;
+    float light_pdf = e.probability * d2 / emitter_cosine;
+    float directional_pdf = receiver_cosine / 3.14159265359f;
+    f3 shadow_origin = position + normal * 0.002f;
+    f3 shadow_delta = light_position + en * 0.002f - shadow_origin;
+    float shadow_distance = len3(shadow_delta);
+    if (trace_ray(s, shadow_origin, shadow_delta / shadow_distance, shadow_distance, tally).status != 0u) return zero;
+    float4 em = o.materials[(uint32_t)e.material * 2u + 1u];
+    float factor = directional_pdf / (light_pdf + directional_pdf);
+    return mk(em.x, em.y, em.z) * factor;
+}
+
+// voxel_path.wgsl:283-307
+__device__ __noinline__ void direct_at(const SceneView &s, const Optics &o, f3 position, f3 normal, f3 &value, f3 &custom,
+                                       Tally &tally) {
+    value = mk(0.0f, 0.0f, 0.0f);
+    custom = mk(0.0f, 0.0f, 0.0f);
+    f3 origin = position + normal * 0.002f;
+    for (uint32_t i = 0u; i < o.light_count; ++i) {
+        float4 pr = o.lights[i * 3u], en = o.lights[i * 3u + 1u], ex = o.lights[i * 3u + 2u];
+        f3 lp = mk(pr.x, pr.y, pr.z);
+        f3 direction = normalize3(lp);
+        float distance = o.max_distance;
+        float attenuation = 1.0f;
+        if (pr.w >= 0.0f) {
+            f3 delta = lp - position;
+            float d2 = dot3(delta, delta);
+            if (d2 <= 0.0000001f || d2 > pr.w * pr.w) continue;
+            distance = fsqrt(d2);
+            direction = delta / distance;
+            distance = distance - 0.004f;
+            attenuation = 1.0f / fmx(d2, 1.0f);
+        }
+        float cosine = fmx(dot3(normal, direction), 0.0f);
+        if (cosine > 0.0f && trace_ray(s, origin, direction, distance, tally).status == 0u) {
+            float sc = cosine * attenuation;
+            value = value + mk(en.x, en.y, en.z) * sc;
+            custom = custom + mk(ex.y, ex.z, ex.w) * sc;
+        }
+    }
+}
+
+// voxel_path.wgsl:66-87: a cached face ends the path once it holds 16 samples
+// and no pending-change bit.
+__device__ __forceinline__ bool cached_energy(const Optics &o, int vx, int vy, int vz, uint32_t face, f3 &value, uint32_t &slot_out) {
+    slot_out = NONE;
+    if (!o.cache_on || face > 5u) return false;
+    uint32_t mask = o.cache_mask;
+    uint32_t slot = (hash_voxel(vx, vy, vz) ^ pcg(face + 1u)) & mask;
+    for (uint32_t attempt = 0u; attempt < 64u; ++attempt) {
+        uint4 e = o.cache_index[slot];
+        if (e.w == NONE) return false;
+        if ((int)e.x == vx && (int)e.y == vy && (int)e.z == vz && (e.w >> 29u) == face) {
+            uint32_t s = e.w & 0x1fffffffu;
+            slot_out = s;
+            const uint32_t *st = o.cache_state + (size_t)s * 36u;
+            uint32_t count = st[5];
+            if (count < 16u || (count & 0x80000000u) != 0u) return false;
+            value = mk(__uint_as_float(st[0]), __uint_as_float(st[1]), __uint_as_float(st[2]));
+            return true;
+        }
+        slot = (slot + 1u) & mask;
+    }
+    return false;
+}
+
+// voxel_path.wgsl:271-282
+__device__ __forceinline__ RayResult trace_diffuse(const SceneView &s, const Optics &o, f3 origin, f3 direction, Tally &tally) {
+    float reach = connection_reach(o);
+    RayResult r = trace_ray(s, origin, direction, reach, tally);
+    if (o.limited_reach && r.status == 0u &&
+        geometry_ahead(s, origin + direction * reach, direction, o.max_distance - reach))
+        r.status = 3u;
+    return r;
+}
+
+// voxel_path.wgsl:368-394
+__device__ __noinline__ f3 sample_terminal_emission(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t stream,
+                                                    uint32_t sequence, uint32_t bounce, Tally &tally) {
+    f3 zero = mk(0.0f, 0.0f, 0.0f);
+    if (o.emitter_faces == 0u) return zero;
+    if (continuation_sample(stream, sequence, bounce, 2u) < 0.5f) {
+        f3 r = sample_emitter(s, o, position, normal, stream, sequence, bounce + 1u, tally);
+        return r * 2.0f;
+    }
+    f3 direction = hemisphere(normal, continuation_sample(stream, sequence, bounce, 0u),
+                              continuation_sample(stream, sequence, bounce, 1u));
+    if (dot3(normal, direction) <= 0.0f) return zero;
+    float reach = connection_reach(o);
+    RayResult hit = trace_ray(s, position + normal * 0.002f, direction, reach, tally);
+    if (hit.status != 1u) return zero;
+    float4 md = o.materials[hit.material * 2u];
+    float4 me = o.materials[hit.material * 2u + 1u];
+    float power = md.w;
+    f3 delta = hit.position - position;
+    float d2 = dot3(delta, delta);
+    if (power == 0.0f || d2 < 0.0001f || d2 > reach * reach) return zero;
+    float selection_pdf = power / fmx(o.total_weight, 1e-20f);
+    if (o.has_tree) selection_pdf = tree_probability(o, position, normal, hit.vx, hit.vy, hit.vz, hit.normal);
+    float light_pdf = selection_pdf * d2 / fmx(dot3(hit.normal, -direction), 1e-8f);
+    float directional_pdf = cosine_pdf(normal, direction);
+    float factor = 2.0f * directional_pdf / (directional_pdf + light_pdf);
+    return mk(me.x, me.y, me.z) * factor;
+}
+
+struct FirstHit {
+    int vx, vy, vz;
+    uint32_t face, status, material, slot;
+};
+
+// voxel_path.wgsl:418-496 (begin_path + advance_path with terminal = true)
+__device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3 start, f3 first_direction, f3 initial_normal,
+                                         uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally) {
+    f3 energy = mk(0.0f, 0.0f, 0.0f);
+    unknown = false;
+    f3 throughput = mk(1.0f, 1.0f, 1.0f);
+    f3 origin = start;
+    f3 direction = first_direction;
+    f3 previous_normal = initial_normal;
+    float previous_pdf = cosine_pdf(initial_normal, first_direction);
+    f3 previous_position = start - initial_normal * 0.001f;
+    if (previous_pdf <= 0.0f) return energy;
+    uint32_t steps = o.bounces;
+    uint32_t bounce = 0u;
+    for (uint32_t step = 0u; step < steps; ++step) {
+        RayResult hit = trace_diffuse(s, o, origin, direction, tally);
+        if (bounce == 0u) {
+            first.vx = hit.vx; first.vy = hit.vy; first.vz = hit.vz;
+            first.status = hit.status;
+            first.material = hit.material;
+            first.face = hit.status == 1u ? face_index_of(hit.normal) : 6u;
+        }
+        if (hit.status == 0u) {
+            float horizon = fclamp(direction.y * 0.35f + 0.65f, 0.15f, 1.0f);
+            energy = energy + throughput * mk(o.env_r, o.env_g, o.env_b) * horizon;
+            break;
+        }
+        if (hit.status == 2u) { unknown = true; break; }
+        if (hit.status == 3u) break;
+        float4 md = o.materials[hit.material * 2u];
+        float4 me = o.materials[hit.material * 2u + 1u];
+        f3 diffuse = mk(md.x, md.y, md.z);
+        f3 emission = mk(me.x, me.y, me.z);
+        f3 dv, dc;
+        direct_at(s, o, hit.position, hit.normal, dv, dc, tally);
+        float emission_weight = md.w;
+        f3 delta = hit.position - origin;
+        float selection_pdf = emission_weight / fmx(o.total_weight, 1e-20f);
+        if (emission_weight > 0.0f && o.emitter_faces != 0u && o.has_tree)
+            selection_pdf = tree_probability(o, previous_position, previous_normal, hit.vx, hit.vy, hit.vz, hit.normal);
+        float light_pdf = selection_pdf * dot3(delta, delta) / fmx(dot3(hit.normal, -direction), 1e-8f);
+        float mis = 1.0f;
+        if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
+        f3 cached;
+        uint32_t cslot;
+        bool found = cached_energy(o, hit.vx, hit.vy, hit.vz, face_index_of(hit.normal), cached, cslot);
+        if (bounce == 0u) first.slot = cslot;
+        if (found) {
+            energy = energy + throughput * (emission * mis + diffuse * (dv + cached));
+            break;
+        }
+        bool last = step + 1u == steps;
+        f3 nee = last ? sample_terminal_emission(s, o, hit.position, hit.normal, stream, sequence, bounce, tally)
+                      : sample_emitter(s, o, hit.position, hit.normal, stream, sequence, bounce + 1u, tally);
+        energy = energy + throughput * (emission * mis + diffuse * (dv + nee));
+        if (last) break;
+        throughput = throughput * diffuse;
+        float maximum = fmx(throughput.x, fmx(throughput.y, throughput.z));
+        if (maximum == 0.0f) break;
+        if (bounce >= 1u) {
+            float survival = fclamp(maximum, 0.1f, 0.95f);
+            if (continuation_sample(stream, sequence, bounce, 2u) > survival) break;
+            throughput = throughput / survival;
+        }
+        origin = hit.position + hit.normal * 0.002f;
+        previous_position = hit.position;
+        previous_normal = hit.normal;
+        direction = hemisphere(hit.normal, continuation_sample(stream, sequence, bounce, 0u),
+                               continuation_sample(stream, sequence, bounce, 1u));
+        previous_pdf = cosine_pdf(hit.normal, direction);
+        if (previous_pdf <= 0.0f) break;
+        bounce += 1u;
+    }
+    return energy;
+}
+
+// voxel_path.wgsl:504-520
+__device__ __forceinline__ f3 trace_receiver(const SceneView &s, const Optics &o, f3 surface, f3 normal, uint32_t stream,
+                                             uint32_t sequence, bool &known, FirstHit &first, Tally &tally) {
+    f3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));
+    f3 value = sample_emitter(s, o, surface, normal, stream, sequence, 0u, tally);
+    bool unknown = false;
+    first.vx = first.vy = first.vz = 0;
+    first.face = 6u;
+    first.status = 4u;
+    first.material = 0u;
+    first.slot = NONE;
+    if (dot3(normal, direction) > 0.0f) {
+        f3 c = trace_path(s, o, surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first, tally);
+        value = value + c;
+    }
+    known = !unknown;
+    return value;
+}
+
+}  // namespace vgi

@@ -1,0 +1,164 @@
+/*
+ * vgi.h -- C ABI of the MI355X-native voxel GI lighting hot path.
+ *
+ * The library replaces the reference's GPU transport seam, `GpuTracer`
+ * (crates/bevy_voxel_gi/src/gpu.rs:309-1236), together with the scene
+ * construction it consumes (`TransportScene::with_page_contents`,
+ * world.rs:982-1046; `PackedScene::update`, gpu_scene.rs:72-309) and the
+ * resident world cache (`ResidentAccumulator`, gpu/resident.rs:319-1213).
+ * Each entry point below names the reference interface it stands in for.
+ *
+ * Conventions: integer status codes (0 ok, negative error) and no exceptions
+ * across the ABI; `vgi_last_error()` describes the last failure on the calling
+ * thread. Host memory is caller-owned, device memory library-owned. One
+ * thread at a time per tracer (the reference's `&mut self` contract,
+ * dispatch.rs:167-271). No torch types, no HIP types in signatures.
+ */
+#ifndef VGI_H
+#define VGI_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGI_AIR 256u            /* world.rs:19 AIR: u16 material sentinel */
+#define VGI_PAGE_EDGE 32u       /* world.rs:22 */
+#define VGI_NO_FACE 0xffffffffu /* voxel_trace.wgsl:64 unused pick capacity */
+
+typedef struct vgi_tracer vgi_tracer;
+
+/* gpu.rs:136-141 GpuLight: position_range.w < 0 marks a directional source
+ * whose xyz is the unit direction toward the light (sources.rs:130). */
+typedef struct vgi_light {
+    float position_range[4];
+    float energy[4];
+    float extra[4]; /* (0, custom visible rgb) */
+} vgi_light;
+
+/* gpu.rs:144-177 / voxel_trace.wgsl:5-12 FaceTask, 32 bytes. */
+typedef struct vgi_face_task {
+    int32_t voxel[3];
+    uint32_t face;     /* 0..5 = +x,-x,+y,-y,+z,-z (world.rs:50-57); VGI_NO_FACE skips */
+    uint32_t sequence; /* first QMC sequence index of this batch */
+    uint32_t padding[3];
+} vgi_face_task;
+
+/* voxel_trace.wgsl:13-20 FaceResult, 96 bytes (gpu.rs:1072-1106 decode). */
+typedef struct vgi_face_result {
+    float mean[4];
+    float mean_extra[4]; /* y: known sample count (f32); w: presentation (resident) */
+    float second[4];
+    float second_extra[4];
+    float direct[4];
+    float direct_extra[4]; /* yzw: custom visible direct */
+} vgi_face_result;
+
+/* TransportSettings (settings.rs:391-426) + TraceWork history fields
+ * (work.rs:11-55) + the WGSL override constants (voxel_path.wgsl:4-14). */
+typedef struct vgi_transport {
+    uint32_t bounces;        /* DIFFUSE_HITS; the reference clamps its public setting to 1..3, 1..8 here */
+    uint32_t samples;        /* samples per face per batch */
+    uint32_t lanes;          /* PATH_LANES (0: 4 if samples < 16 else 8, gpu.rs:1179-1181) */
+    uint32_t limited_reach;  /* LIMITED_REACH */
+    float reach;             /* transport.x */
+    float max_distance;      /* extra.y = MAX_PATH_DISTANCE (tracer.rs:3), 0 -> 768 */
+    float environment[3];
+    uint32_t history_limit;  /* 0 -> HISTORY_LIMIT 65536 (resident.rs:715-719) */
+    float history_recency;   /* >= 1 (resident.rs:721-725) */
+    float measured_retention;/* settings.rs:262 */
+} vgi_transport;
+
+/* ---- lifecycle: GpuTracer::new / enable_resident (gpu.rs:309-366) ---- */
+int vgi_create(int device_ordinal, vgi_tracer **out);
+void vgi_destroy(vgi_tracer *tracer);
+int vgi_enable_resident(vgi_tracer *tracer);
+
+/* ---- scene: VoxelGiInput::{page_loaded, page_unloaded} (input.rs:54-112) feeding
+ * TransportScene::with_page_contents (world.rs:982-1046). Pages are 32^3 u16
+ * voxels, x fastest, VGI_AIR for air. The NAADF brick/voxel fields are built on
+ * the GPU; the top level is grown in exact FIFO order on the host. ---- */
+int vgi_upload_pages(vgi_tracer *tracer, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels,
+                     const int32_t *removed_xyz, uint32_t removed);
+/* VoxelGiInput::edit_voxel (input.rs:90-93): set voxels; edited regions are
+ * re-packed on the GPU and the top level is updated incrementally
+ * (TopLevelAadf::updated, world.rs:477-583). */
+int vgi_edit_voxels(vgi_tracer *tracer, const int32_t *xyz, const uint16_t *values, uint32_t count);
+
+/* ---- optics and sources: VoxelGiMaterials (world.rs:90-174) as packed by
+ * gpu.rs:714-727; TransportSources (sources.rs:105-160) as packed by
+ * gpu.rs:728-760 ---- */
+int vgi_set_materials(vgi_tracer *tracer, const float *diffuse_rgb, const float *emission_rgb);
+int vgi_set_lights(vgi_tracer *tracer, const vgi_light *lights, uint32_t count, uint32_t directional);
+int vgi_set_transport(vgi_tracer *tracer, const vgi_transport *transport);
+
+/* ---- batches: GpuTracer::submit_compact (gpu.rs:647-1022). Non-resident:
+ * trace `count` tasks as given. With the resident cache on, each face gets
+ * its slot (allocating as resident.rs:770-812), takes the slot's sequence
+ * (prepare, voxel_accumulate.wgsl:83-133) and is merged (accumulate,
+ * voxel_accumulate.wgsl:135-227). Flags: VGI_RECORD_HITS keeps the per
+ * face-sample first-hit records; VGI_COUNT_RAYS counts rays and steps. ---- */
+#define VGI_RECORD_HITS 1u
+#define VGI_COUNT_RAYS 2u
+int vgi_submit_faces(vgi_tracer *tracer, const vgi_face_task *tasks, uint32_t count, uint32_t flags,
+                     uint64_t *ticket);
+/* GpuTracer::snapshot_ready (gpu.rs:1024-1049) */
+int vgi_poll(vgi_tracer *tracer, uint64_t ticket, int *ready);
+/* GpuTracer::complete_readback (gpu.rs:1050-1170): 96-byte records. */
+int vgi_read_faces(vgi_tracer *tracer, uint64_t ticket, vgi_face_result *out, uint32_t count);
+/* First-hit record per face-sample (cf. voxel_view_trace.wgsl:21-38): 8 u32:
+ * voxel xyz, face | status << 8, material, cache slot, rays of the sample,
+ * known. Requires VGI_RECORD_HITS. */
+int vgi_read_hits(vgi_tracer *tracer, uint64_t ticket, uint32_t *out, uint64_t words);
+/* [rays, steps, 0...] of the ticket (VGI_COUNT_RAYS). */
+int vgi_read_counters(vgi_tracer *tracer, uint64_t ticket, uint64_t *out, uint32_t count);
+/* GPU timestamps (gpu.rs:486-497,1050-1070): ms of [prepare, trace, accumulate, build]. */
+int vgi_timings(vgi_tracer *tracer, float *out, uint32_t count);
+/* Resident slots of the last batch's faces (face_index.rs:71-97 lookups). */
+int vgi_read_slots(vgi_tracer *tracer, uint32_t *out, uint32_t count);
+
+/* ---- device-resident frames (no host transfer): the face set is bound once
+ * and each frame runs prepare -> trace -> accumulate on it, as the reference's
+ * worker does batch after batch. ---- */
+int vgi_bind_faces(vgi_tracer *tracer, const vgi_face_task *tasks, uint32_t count);
+int vgi_run_frame(vgi_tracer *tracer, uint32_t flags);
+int vgi_sync(vgi_tracer *tracer);
+/* Time of the last frame's trace launch on its stream (ms), valid after vgi_sync. */
+float vgi_last_trace_ms(vgi_tracer *tracer);
+/* The bound faces' 96-byte records after the last frame; counters (optional)
+ * receive [rays, steps] when the frame ran with VGI_COUNT_RAYS. */
+int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uint64_t *counters);
+/* Raw resident state (9 uint4 per slot, voxel_accumulate.wgsl:1-4) of slots [first, first+count). */
+int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
+
+/* ---- retention events (resident.rs:1326-1417 encode_events layout;
+ * voxel_retention.wgsl:29-96, apply_events voxel_accumulate.wgsl:295-332) ---- */
+int vgi_apply_events(vgi_tracer *tracer, const uint32_t *events, uint32_t event_count, uint32_t words);
+
+/* ---- scene inspection for parity checks ---- */
+int vgi_scene_bounds(vgi_tracer *tracer, int32_t out[6]);
+/* The canonical per-region form (see oracle): kind, material, top-level
+ * bytes, 64 brick words (pointer replaced by tag 2), 64x32 brick cell words. */
+int vgi_scene_region_canonical(vgi_tracer *tracer, int32_t rx, int32_t ry, int32_t rz, uint32_t *out);
+int64_t vgi_scene_stats(vgi_tracer *tracer, int64_t out[8]);
+int64_t vgi_emitter_words(vgi_tracer *tracer, uint32_t *out, int64_t capacity);
+
+const char *vgi_last_error(void);
+
+/* ---- seeded synthetic scenes (not part of the reference seam): dense u16
+ * grids of `dims` voxels, dims multiples of 32. ---- */
+/* Workloads: 1 hollow box 64^3, 2 noise terrain 256^3, 4 edit-stream terrain
+ * 512^3, 5 worm cave 512^3, 6 furnace (reference research.rs:71-115). */
+int vgi_scene_generate(int workload, uint32_t seed, uint32_t edge, uint16_t *voxels, int32_t dims[3]);
+/* Materials and lights of a workload (256x3 diffuse, 256x3 emission, lights). */
+int vgi_scene_optics(int workload, uint32_t seed, float *diffuse_rgb, float *emission_rgb, vgi_light *lights,
+                     uint32_t *light_count, float environment[3]);
+/* Exposed faces of a dense grid (solid voxel, air or out-of-grid neighbour),
+ * optionally limited to the box interior; returns the count. */
+int64_t vgi_scene_faces(const uint16_t *voxels, const int32_t dims[3], int interior_only, vgi_face_task *out,
+                        int64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

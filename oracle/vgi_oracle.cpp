@@ -1,0 +1,2132 @@
+// CPU oracle for the voxel-GI lighting hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, the smoke entry and the bench's cpu_baseline leg load this
+// library, and only as the checker. It is a scalar restatement of the
+// reference passes (paths relative to crates/bevy_voxel_gi/ of the
+// reference), compiled with -O2 -ffp-contract=off: every float operation is
+// one IEEE binary32 operation, in the order the WGSL writes it.
+//
+// Parity pins (see tests/): build_aadf against the reference's frozen
+// full-sweep construction (naadf/construction_tests.rs) and closed forms
+// (naadf.rs:201-208); the top level against the fixed-point invariant
+// (world.rs:596-684); packing against the byte KATs of gpu_scene.rs tests;
+// traversal against a brute-force voxel walk; the estimator against the
+// furnace closed forms E*sum(r^k) and E/(1-r) (gpu/view_tests.rs:387-459,
+// gpu/cache_tests.rs:43-101); emitter tables against emission.rs:401-432;
+// retention against retention.rs:346-376.
+#include "vgi_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr uint16_t AIR = 256;          // world.rs:19
+constexpr int BRICK_EDGE = 4;          // world.rs:20
+constexpr int REGION_EDGE = 16;        // world.rs:21
+constexpr int PAGE_EDGE = 32;          // world.rs:22
+constexpr uint64_t TOP_LEVEL_MAX_CELLS = 4u << 20;  // world.rs:426
+constexpr uint32_t NONE = 0xffffffffu;
+
+struct I3 {
+    int32_t x, y, z;
+    bool operator==(const I3 &o) const { return x == o.x && y == o.y && z == o.z; }
+    bool operator!=(const I3 &o) const { return !(*this == o); }
+    int32_t operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+struct F3 {
+    float x, y, z;
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+static inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline F3 operator*(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+static inline F3 operator/(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+static inline F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+static inline float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline F3 cross3(F3 a, F3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// WGSL min/max on non-NaN operands.
+static inline float fmin_(float a, float b) { return b < a ? b : a; }
+static inline float fmax_(float a, float b) { return a < b ? b : a; }
+static inline float clampf_(float v, float lo, float hi) { return fmin_(fmax_(v, lo), hi); }
+static inline float length3(F3 a) { return std::sqrt(dot3(a, a)); }
+// normalize(v) = v * (1 / |v|), the glam/this project's convention on both sides.
+static inline F3 normalize3(F3 a) {
+    float inv = 1.0f / std::sqrt(dot3(a, a));
+    return a * inv;
+}
+static inline F3 to_f3(I3 v) { return f3((float)v.x, (float)v.y, (float)v.z); }
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static inline uint64_t key_of(I3 v) {
+    return ((uint64_t)(uint32_t)(v.x + (1 << 20)) << 42) ^ ((uint64_t)(uint32_t)(v.y + (1 << 20)) << 21) ^
+           (uint64_t)(uint32_t)(v.z + (1 << 20));
+}
+static inline int floordiv(int a, int b) { int q = a / b; if ((a % b) != 0 && ((a < 0) != (b < 0))) --q; return q; }
+static inline int floormod(int a, int b) { int m = a % b; if (m < 0) m += b; return m; }
+
+// ---------------------------------------------------------------- hashes
+// sampling.rs:5-9, voxel_traversal.wgsl:24-28
+static inline uint32_t pcg(uint32_t value) {
+    uint32_t state = value * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+// voxel_traversal.wgsl:29-31, gpu_scene.rs:469-471
+static inline uint32_t hash_voxel(I3 v) {
+    return pcg(pcg(pcg((uint32_t)v.x) ^ (uint32_t)v.y) ^ (uint32_t)v.z);
+}
+// sampling.rs:11-13
+static inline uint32_t stream_id(I3 v, uint32_t face) { return hash_voxel(v) ^ pcg(face + 1u); }
+
+// ---------------------------------------------------------------- own transcendentals
+// sin/cos: Cody-Waite reduction by pi/2 in three parts, then Taylor series
+// on [-pi/4, pi/4] (sin to r^9, cos to r^10; truncation below 3e-9 relative).
+// The constants are pi/2 split so that k * PIO2_A and k * PIO2_B are exact for
+// |k| < 2^12 (PIO2_A has 8 significant bits, PIO2_B 12).
+constexpr float TWO_OVER_PI = 0.636619772367581343f;
+constexpr float PIO2_A = 1.5703125f;
+constexpr float PIO2_B = 4.8387050628662109375e-4f;
+constexpr float PIO2_C = -4.37113882867379e-8f;
+static inline float reduce_pio2(float x, int *quadrant) {
+    float q = std::floor(x * TWO_OVER_PI + 0.5f);
+    *quadrant = (int)q;
+    float r = x - q * PIO2_A;
+    r = r - q * PIO2_B;
+    r = r - q * PIO2_C;
+    return r;
+}
+static inline float sin_poly(float r) {
+    float r2 = r * r;
+    float p = 2.75573192239858906526e-6f;            // 1/9!
+    p = p * r2 + -1.98412698412698412698e-4f;         // -1/7!
+    p = p * r2 + 8.33333333333333333333e-3f;          // 1/5!
+    p = p * r2 + -1.66666666666666666667e-1f;         // -1/3!
+    return r + (r * r2) * p;
+}
+static inline float cos_poly(float r) {
+    float r2 = r * r;
+    float p = -2.75573192239858906526e-7f;            // -1/10!
+    p = p * r2 + 2.48015873015873015873e-5f;          // 1/8!
+    p = p * r2 + -1.38888888888888888889e-3f;         // -1/6!
+    p = p * r2 + 4.16666666666666666667e-2f;          // 1/4!
+    p = p * r2 + -0.5f;                               // -1/2!
+    return 1.0f + r2 * p;
+}
+static float vgi_sin(float x) {
+    int k;
+    float r = reduce_pio2(x, &k);
+    switch (k & 3) {
+    case 0: return sin_poly(r);
+    case 1: return cos_poly(r);
+    case 2: return -sin_poly(r);
+    default: return -cos_poly(r);
+    }
+}
+static float vgi_cos(float x) {
+    int k;
+    float r = reduce_pio2(x, &k);
+    switch (k & 3) {
+    case 0: return cos_poly(r);
+    case 1: return -sin_poly(r);
+    case 2: return -cos_poly(r);
+    default: return sin_poly(r);
+    }
+}
+// exp: x = k ln2 + r, |r| <= ln2/2, Taylor to r^7, scaled by 2^k through the
+// exponent bits. Used only by the display EMA (voxel_accumulate.wgsl:198).
+static float vgi_exp(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return bitsf(0x7f800000u);
+    float k = std::floor(x * 1.44269504088896341f + 0.5f);
+    // Cody & Waite (1980) split of ln2: the 9-bit high part keeps k * C1 exact.
+    float r = x - k * 0.693359375f;
+    r = r - k * -2.12194440054690583e-4f;
+    float p = 1.98412698412698412698e-4f;             // 1/7!
+    p = p * r + 1.38888888888888888889e-3f;
+    p = p * r + 8.33333333333333333333e-3f;
+    p = p * r + 4.16666666666666666667e-2f;
+    p = p * r + 1.66666666666666666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int e = (int)k;
+    // 2^e in two steps so that e down to -126 and up to 127 stay normal.
+    int e1 = e / 2, e2 = e - e1;
+    return (p * bitsf((uint32_t)(127 + e1) << 23)) * bitsf((uint32_t)(127 + e2) << 23);
+}
+
+// ---------------------------------------------------------------- QMC (voxel_path.wgsl:178-224)
+static inline uint32_t reverse_bits(uint32_t v) {
+    v = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+    v = ((v >> 2) & 0x33333333u) | ((v & 0x33333333u) << 2);
+    v = ((v >> 4) & 0x0f0f0f0fu) | ((v & 0x0f0f0f0fu) << 4);
+    v = ((v >> 8) & 0x00ff00ffu) | ((v & 0x00ff00ffu) << 8);
+    return (v >> 16) | (v << 16);
+}
+static inline uint32_t scramble_reverse(uint32_t input, uint32_t seed) {
+    uint32_t value = input;
+    value ^= value * 0x3d20adeau;
+    value += seed;
+    value *= (seed >> 16u) | 1u;
+    value ^= value * 0x05526c56u;
+    value ^= value * 0x53a22864u;
+    return value;
+}
+static const uint32_t PRIMES[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+static float sample_1d(uint32_t stream, uint32_t sequence, uint32_t dimension) {
+    if (dimension < 12u) {
+        uint32_t base = PRIMES[dimension];
+        uint32_t index = (sequence & 0x00ffffffu) + 1u;
+        float scale = 1.0f;
+        float value = 0.0f;
+        float fbase = (float)base;
+        while (index != 0u) {
+            scale = scale / fbase;
+            value = value + (float)(index % base) * scale;
+            index = index / base;
+        }
+        float rotation = (float)(pcg(stream ^ pcg(dimension)) >> 9u) / 8388608.0f;
+        float x = value + rotation;
+        return x - std::floor(x);  // fract
+    }
+    uint32_t seed = pcg(stream ^ pcg(dimension / 2u));
+    uint32_t index = reverse_bits(scramble_reverse(reverse_bits(sequence), pcg(seed ^ 0x79c68e4au)));
+    if ((dimension & 1u) != 0u) {
+        index ^= (index >> 1u) & 0x55555555u;
+        index ^= (index >> 2u) & 0x33333333u;
+        index ^= (index >> 4u) & 0x0f0f0f0fu;
+        index ^= (index >> 8u) & 0x00ff00ffu;
+        index ^= (index >> 16u) & 0x0000ffffu;
+    }
+    uint32_t value = reverse_bits(scramble_reverse(index, pcg(seed ^ pcg(dimension))));
+    return (float)(value >> 9u) / 8388608.0f;
+}
+// voxel_path.wgsl:218-224
+static float continuation_sample(uint32_t stream, uint32_t sequence, uint32_t bounce, uint32_t dimension) {
+    if (bounce < 3u) {
+        uint32_t legacy = dimension == 2u ? 8u + bounce : 2u + bounce * 2u + dimension;
+        return sample_1d(stream, sequence, legacy);
+    }
+    return sample_1d(stream, sequence, 16u + (bounce - 3u) * 3u + dimension);
+}
+
+// ---------------------------------------------------------------- geometry helpers
+// voxel_path.wgsl:226-230
+static inline F3 face_normal(uint32_t face) {
+    F3 n = f3(0.0f, 0.0f, 0.0f);
+    n[(int)(face / 2u)] = (face & 1u) != 0u ? -1.0f : 1.0f;
+    return n;
+}
+// voxel_traversal.wgsl:17-22
+static inline uint32_t face_index_of(F3 n) {
+    if (n.x != 0.0f) return n.x < 0.0f ? 1u : 0u;
+    if (n.y != 0.0f) return n.y < 0.0f ? 3u : 2u;
+    if (n.z != 0.0f) return n.z < 0.0f ? 5u : 4u;
+    return 6u;
+}
+// voxel_path.wgsl:231-238
+static F3 hemisphere(F3 n, float u, float v) {
+    F3 tangent;
+    if (std::fabs(n.z) < 0.999f) tangent = normalize3(cross3(n, f3(0.0f, 0.0f, 1.0f)));
+    else tangent = normalize3(cross3(n, f3(0.0f, 1.0f, 0.0f)));
+    F3 bitangent = cross3(n, tangent);
+    float phi = 6.28318530718f * u;
+    float sv = std::sqrt(v);
+    float s1 = std::sqrt(1.0f - v);
+    return tangent * (vgi_cos(phi) * sv) + bitangent * (vgi_sin(phi) * sv) + n * s1;
+}
+// voxel_path.wgsl:326-328
+static inline float cosine_pdf(F3 normal, F3 direction) {
+    return fmax_(dot3(normal, direction), 0.0f) / 3.14159265359f;
+}
+
+// ---------------------------------------------------------------- NAADF (naadf.rs:37-156)
+struct Aadf {
+    uint8_t b[6];
+    bool operator==(const Aadf &o) const { return memcmp(b, o.b, 6) == 0; }
+    bool operator!=(const Aadf &o) const { return !(*this == o); }
+};
+static const Aadf ZERO_AADF = {{0, 0, 0, 0, 0, 0}};
+
+static inline size_t lin(int x, int y, int z, int sx, int sy) { return (size_t)x + (size_t)sx * ((size_t)y + (size_t)sy * (size_t)z); }
+
+// naadf.rs:117-148
+static inline void try_grow(Aadf &current, int nx, int ny, int nz, int direction, int sx, int sy, int sz,
+                            const uint8_t *blocked, const Aadf *source) {
+    if (nx < 0 || ny < 0 || nz < 0 || nx >= sx || ny >= sy || nz >= sz) return;
+    if (current.b[direction] == 255) return;
+    size_t ni = lin(nx, ny, nz, sx, sy);
+    if (blocked[ni]) return;
+    const Aadf &n = source[ni];
+    int opposite = direction ^ 1;
+    for (int d = 0; d < 6; ++d)
+        if (d != opposite && n.b[d] < current.b[d]) return;
+    current.b[direction] += 1;
+}
+
+static void sweep_aadf(int sx, int sy, int sz, const uint8_t *blocked, std::vector<Aadf> &source) {
+    std::vector<Aadf> destination = source;
+    int rounds = std::max(std::max(sx, sy), sz) - 1;
+    if (rounds < 0) rounds = 0;
+    for (int r = 0; r < rounds; ++r) {
+        for (int axis = 0; axis < 3; ++axis) {
+            for (int z = 0; z < sz; ++z)
+                for (int y = 0; y < sy; ++y)
+                    for (int x = 0; x < sx; ++x) {
+                        size_t i = lin(x, y, z, sx, sy);
+                        if (blocked[i]) { destination[i] = ZERO_AADF; continue; }
+                        Aadf current = source[i];
+                        int dx = axis == 0, dy = axis == 1, dz = axis == 2;
+                        try_grow(current, x - dx, y - dy, z - dz, axis * 2, sx, sy, sz, blocked, source.data());
+                        try_grow(current, x + dx, y + dy, z + dz, axis * 2 + 1, sx, sy, sz, blocked, source.data());
+                        destination[i] = current;
+                    }
+            std::swap(source, destination);
+        }
+    }
+}
+
+// naadf.rs:37-115 (closed forms for all-blocked and all-empty volumes)
+static std::vector<Aadf> build_aadf(int sx, int sy, int sz, const uint8_t *blocked) {
+    size_t n = (size_t)sx * sy * sz;
+    std::vector<Aadf> source(n, ZERO_AADF);
+    bool all_blocked = true, all_empty = true;
+    for (size_t i = 0; i < n; ++i) { if (blocked[i]) all_empty = false; else all_blocked = false; }
+    if (all_blocked) return source;
+    if (all_empty) {
+        for (int z = 0; z < sz; ++z)
+            for (int y = 0; y < sy; ++y)
+                for (int x = 0; x < sx; ++x) {
+                    Aadf &a = source[lin(x, y, z, sx, sy)];
+                    a.b[0] = (uint8_t)x; a.b[1] = (uint8_t)(sx - x - 1);
+                    a.b[2] = (uint8_t)y; a.b[3] = (uint8_t)(sy - y - 1);
+                    a.b[4] = (uint8_t)z; a.b[5] = (uint8_t)(sz - z - 1);
+                }
+        return source;
+    }
+    sweep_aadf(sx, sy, sz, blocked, source);
+    return source;
+}
+
+// ---------------------------------------------------------------- regions (world.rs:176-419)
+enum Storage { ST_EMPTY = 0, ST_UNIFORM = 1, ST_DENSE = 2, ST_BRICKS = 3 };
+
+struct Region {
+    I3 coord;
+    int storage = ST_EMPTY;
+    uint16_t uniform = AIR;
+    std::vector<uint16_t> materials;   // dense: 16^3
+    uint16_t bricks[64];               // bricks storage
+    uint8_t brick_occupied[64];
+    bool any_solid = false;
+    uint64_t mask[4] = {0, 0, 0, 0};
+
+    uint16_t material(int x, int y, int z) const {  // world.rs:386-395
+        switch (storage) {
+        case ST_EMPTY: return AIR;
+        case ST_UNIFORM: return uniform;
+        case ST_DENSE: return materials[lin(x, y, z, 16, 16)];
+        default: return bricks[lin(x / 4, y / 4, z / 4, 4, 4)];
+        }
+    }
+    bool matches(const uint16_t *incoming) const {  // world.rs:220-230
+        for (int i = 0; i < 4096; ++i) {
+            int x = i % 16, y = (i / 16) % 16, z = i / 256;
+            if (material(x, y, z) != incoming[i]) return false;
+        }
+        return true;
+    }
+    // world.rs:372-384
+    std::vector<Aadf> brick_bounds() const {
+        uint8_t occ[64];
+        if (storage == ST_DENSE) memcpy(occ, brick_occupied, 64);
+        else if (storage == ST_BRICKS) for (int i = 0; i < 64; ++i) occ[i] = bricks[i] != AIR;
+        else return std::vector<Aadf>(64, ZERO_AADF);
+        return build_aadf(4, 4, 4, occ);
+    }
+    // world.rs:352-370
+    std::vector<Aadf> brick_voxel_bounds(int bx, int by, int bz) const {
+        uint8_t blocked[64];
+        if (storage == ST_BRICKS) {
+            bool solid = bricks[lin(bx, by, bz, 4, 4)] != AIR;
+            memset(blocked, solid ? 1 : 0, 64);
+            return build_aadf(4, 4, 4, blocked);
+        }
+        if (storage != ST_DENSE) return std::vector<Aadf>(64, ZERO_AADF);
+        for (int i = 0; i < 64; ++i) {
+            int lx = i & 3, ly = (i >> 2) & 3, lz = i >> 4;
+            blocked[i] = materials[lin(bx * 4 + lx, by * 4 + ly, bz * 4 + lz, 16, 16)] != AIR;
+        }
+        return build_aadf(4, 4, 4, blocked);
+    }
+};
+
+static inline int brick_of_index(int index) {
+    int x = index % 16, y = (index / 16) % 16, z = index / 256;
+    return (int)lin(x / 4, y / 4, z / 4, 4, 4);
+}
+
+// world.rs:243-319
+static std::shared_ptr<Region> build_region(I3 coord, const uint16_t *mats) {
+    auto r = std::make_shared<Region>();
+    r->coord = coord;
+    bool all_air = true, all_same = true;
+    for (int i = 0; i < 4096; ++i) {
+        if (mats[i] != AIR) all_air = false;
+        if (mats[i] != mats[0]) all_same = false;
+    }
+    if (all_air) { r->storage = ST_EMPTY; r->any_solid = false; return r; }
+    if (all_same) {
+        r->storage = ST_UNIFORM; r->uniform = mats[0]; r->any_solid = true;
+        r->mask[mats[0] / 64] |= 1ull << (mats[0] % 64);
+        return r;
+    }
+    uint16_t bricks[64];
+    bool uni[64], seen[64];
+    for (int b = 0; b < 64; ++b) { bricks[b] = AIR; uni[b] = true; seen[b] = false; }
+    for (int i = 0; i < 4096; ++i) {
+        int b = brick_of_index(i);
+        if (!seen[b]) { seen[b] = true; bricks[b] = mats[i]; }
+        else if (bricks[b] != mats[i]) uni[b] = false;
+    }
+    bool all_uni = true;
+    for (int b = 0; b < 64; ++b) all_uni = all_uni && uni[b];
+    if (all_uni) {
+        r->storage = ST_BRICKS;
+        memcpy(r->bricks, bricks, sizeof(bricks));
+        r->any_solid = false;
+        for (int b = 0; b < 64; ++b)
+            if (bricks[b] != AIR) { r->any_solid = true; r->mask[bricks[b] / 64] |= 1ull << (bricks[b] % 64); }
+        return r;
+    }
+    r->storage = ST_DENSE;
+    r->materials.assign(mats, mats + 4096);
+    memset(r->brick_occupied, 0, 64);
+    for (int i = 0; i < 4096; ++i) {
+        if (mats[i] == AIR) continue;
+        r->mask[mats[i] / 64] |= 1ull << (mats[i] % 64);
+        r->brick_occupied[brick_of_index(i)] = 1;
+    }
+    r->any_solid = false;
+    for (int b = 0; b < 64; ++b) r->any_solid = r->any_solid || r->brick_occupied[b];
+    return r;
+}
+
+// ---------------------------------------------------------------- top level (world.rs:428-913)
+struct TopLevel {
+    I3 minimum{0, 0, 0}, maximum{0, 0, 0};
+    int sx = 0, sy = 0, sz = 0;
+    std::vector<Aadf> fields;     // empty: disabled
+    std::vector<uint8_t> blocked;
+    Aadf field(I3 c) const {      // world.rs:706-712
+        if (fields.empty()) return ZERO_AADF;
+        return fields[lin(c.x - minimum.x, c.y - minimum.y, c.z - minimum.z, sx, sy)];
+    }
+};
+
+// world.rs:838-912
+struct Growth {
+    int sx, sy, sz;
+    std::deque<size_t> queue;
+    std::vector<uint8_t> queued;
+    Growth(int x, int y, int z) : sx(x), sy(y), sz(z), queued((size_t)x * y * z, 0) {}
+    I3 cell(size_t i) const { return I3{(int)(i % sx), (int)((i / sx) % sy), (int)(i / ((size_t)sx * sy))}; }
+    void push(size_t i) { if (!queued[i]) { queued[i] = 1; queue.push_back(i); } }
+    void touch(size_t i) {
+        push(i);
+        I3 c = cell(i);
+        for (int d = 0; d < 6; ++d) {
+            I3 n = c;
+            int s = (d % 2 == 0) ? -1 : 1;
+            if (d / 2 == 0) n.x += s; else if (d / 2 == 1) n.y += s; else n.z += s;
+            if (n.x >= 0 && n.y >= 0 && n.z >= 0 && n.x < sx && n.y < sy && n.z < sz) push(lin(n.x, n.y, n.z, sx, sy));
+        }
+    }
+    void run(std::vector<Aadf> &fields, const std::vector<uint8_t> &blocked) {
+        while (!queue.empty()) {
+            size_t i = queue.front();
+            queue.pop_front();
+            queued[i] = 0;
+            if (blocked[i]) continue;
+            I3 c = cell(i);
+            Aadf current = fields[i];
+            for (int d = 0; d < 6; ++d) {
+                int s = (d % 2 == 0) ? -1 : 1;
+                int nx = c.x + (d / 2 == 0 ? s : 0), ny = c.y + (d / 2 == 1 ? s : 0), nz = c.z + (d / 2 == 2 ? s : 0);
+                try_grow(current, nx, ny, nz, d, sx, sy, sz, blocked.data(), fields.data());
+            }
+            if (current != fields[i]) { fields[i] = current; touch(i); }
+        }
+    }
+};
+
+// world.rs:715-722
+static Aadf clip_field(Aadf f, I3 below, I3 above) {
+    for (int a = 0; a < 3; ++a) {
+        f.b[a * 2] = (uint8_t)std::min<int>(f.b[a * 2], std::min(below[a], 255));
+        f.b[a * 2 + 1] = (uint8_t)std::min<int>(f.b[a * 2 + 1], std::min(above[a], 255));
+    }
+    return f;
+}
+
+// world.rs:733-833
+static void cut_boxes(std::vector<Aadf> &fields, const std::vector<uint8_t> &blocked, int sx, int sy, int sz,
+                      const std::vector<I3> &newly, Growth &growth) {
+    I3 low = newly[0], high = newly[0];
+    for (const I3 &c : newly) {
+        low = I3{std::min(low.x, c.x), std::min(low.y, c.y), std::min(low.z, c.z)};
+        high = I3{std::max(high.x, c.x), std::max(high.y, c.y), std::max(high.z, c.z)};
+    }
+    int ex = high.x - low.x + 1, ey = high.y - low.y + 1, ez = high.z - low.z + 1;
+    int tx = ex + 1, ty = ey + 1, tz = ez + 1;
+    std::vector<uint32_t> table((size_t)tx * ty * tz, 0);
+    for (const I3 &c : newly) table[lin(c.x - low.x + 1, c.y - low.y + 1, c.z - low.z + 1, tx, ty)] += 1;
+    size_t steps[3] = {1, (size_t)tx, (size_t)tx * ty};
+    for (int axis = 0; axis < 3; ++axis)
+        for (size_t i = 0; i < table.size(); ++i) {
+            size_t co[3] = {i % tx, (i / tx) % ty, i / ((size_t)tx * ty)};
+            if (co[axis] > 0) table[i] += table[i - steps[axis]];
+        }
+    auto count = [&](I3 mn, I3 mx) -> uint32_t {
+        I3 a{std::max(mn.x, low.x), std::max(mn.y, low.y), std::max(mn.z, low.z)};
+        I3 b{std::min(mx.x, high.x), std::min(mx.y, high.y), std::min(mx.z, high.z)};
+        if (a.x > b.x || a.y > b.y || a.z > b.z) return 0;
+        a = I3{a.x - low.x, a.y - low.y, a.z - low.z};
+        b = I3{b.x - low.x + 1, b.y - low.y + 1, b.z - low.z + 1};
+        auto at = [&](int x, int y, int z) { return (int64_t)table[lin(x, y, z, tx, ty)]; };
+        return (uint32_t)(at(b.x, b.y, b.z) - at(a.x, b.y, b.z) - at(b.x, a.y, b.z) - at(b.x, b.y, a.z) +
+                          at(a.x, a.y, b.z) + at(a.x, b.y, a.z) + at(b.x, a.y, a.z) - at(a.x, a.y, a.z));
+    };
+    auto fbox = [](Aadf f, I3 c, I3 &mn, I3 &mx) {
+        mn = I3{c.x - f.b[0], c.y - f.b[2], c.z - f.b[4]};
+        mx = I3{c.x + f.b[1], c.y + f.b[3], c.z + f.b[5]};
+    };
+    for (int z = 0; z < sz; ++z)
+        for (int y = 0; y < sy; ++y)
+            for (int x = 0; x < sx; ++x) {
+                size_t i = lin(x, y, z, sx, sy);
+                if (blocked[i] || fields[i] == ZERO_AADF) continue;
+                I3 c{x, y, z}, mn, mx;
+                fbox(fields[i], c, mn, mx);
+                if (count(mn, mx) == 0) continue;
+                bool have = false;
+                uint64_t most = 0;
+                Aadf best = ZERO_AADF;
+                for (int d = 0; d < 6; ++d) {
+                    auto with = [&](uint8_t e) { Aadf f = fields[i]; f.b[d] = e; return f; };
+                    auto clear = [&](uint8_t e) { I3 a, b; fbox(with(e), c, a, b); return count(a, b) == 0; };
+                    if (!clear(0)) continue;
+                    uint8_t good = 0, bad = fields[i].b[d];
+                    while (bad - good > 1) {
+                        uint8_t middle = good + (bad - good) / 2;
+                        if (clear(middle)) good = middle; else bad = middle;
+                    }
+                    Aadf f = with(good);
+                    uint64_t volume = (uint64_t)(f.b[0] + f.b[1] + 1) * (f.b[2] + f.b[3] + 1) * (f.b[4] + f.b[5] + 1);
+                    if (!have || volume > most) { have = true; most = volume; best = f; }
+                }
+                fields[i] = have ? best : ZERO_AADF;
+                growth.push(i);
+            }
+}
+
+// ---------------------------------------------------------------- scene
+struct Scene {
+    std::unordered_map<uint64_t, std::shared_ptr<Region>> regions;
+    std::unordered_map<uint64_t, std::vector<uint16_t>> pages;  // host page contents (pages.rs overlay)
+    std::unordered_map<uint64_t, I3> page_keys;
+    TopLevel top;
+    uint64_t revision = 0;
+
+    const Region *region(I3 c) const {
+        auto it = regions.find(key_of(c));
+        return it == regions.end() ? nullptr : it->second.get();
+    }
+    bool is_blocked(I3 c) const {  // world.rs:497
+        const Region *r = region(c);
+        return r == nullptr || r->any_solid;
+    }
+    bool bounds(I3 &mn, I3 &mx) const {
+        if (regions.empty()) return false;
+        bool first = true;
+        for (auto &kv : regions) {
+            I3 c = kv.second->coord;
+            if (first) { mn = mx = c; first = false; continue; }
+            mn = I3{std::min(mn.x, c.x), std::min(mn.y, c.y), std::min(mn.z, c.z)};
+            mx = I3{std::max(mx.x, c.x), std::max(mx.y, c.y), std::max(mx.z, c.z)};
+        }
+        return true;
+    }
+    // world.rs:1050-1056; returns 0xffff when not loaded
+    uint32_t material_at(I3 v) const {
+        I3 rc{floordiv(v.x, 16), floordiv(v.y, 16), floordiv(v.z, 16)};
+        const Region *r = region(rc);
+        if (!r) return 0xffffu;
+        return r->material(floormod(v.x, 16), floormod(v.y, 16), floormod(v.z, 16));
+    }
+
+    // world.rs:477-583 (TopLevelAadf::updated)
+    void update_top(const std::vector<I3> &changed) {
+        I3 mn, mx;
+        if (!bounds(mn, mx)) { top = TopLevel(); return; }
+        int sx = mx.x - mn.x + 1, sy = mx.y - mn.y + 1, sz = mx.z - mn.z + 1;
+        uint64_t cells = (uint64_t)sx * sy * sz;
+        if (std::max(std::max(sx, sy), sz) > 255 || cells > TOP_LEVEL_MAX_CELLS) {
+            TopLevel t;
+            t.minimum = mn; t.maximum = mx; t.sx = sx; t.sy = sy; t.sz = sz;
+            top = t;
+            return;
+        }
+        Growth growth(sx, sy, sz);
+        std::vector<Aadf> fields;
+        std::vector<uint8_t> blocked;
+        bool in_place = !top.fields.empty() && top.minimum == mn && top.sx == sx && top.sy == sy && top.sz == sz;
+        if (in_place) {
+            fields = top.fields;
+            blocked = top.blocked;
+        } else {
+            fields.assign(cells, ZERO_AADF);
+            blocked.assign(cells, 1);
+            bool kept = !top.fields.empty();
+            for (int z = mn.z; z <= mx.z; ++z)
+                for (int y = mn.y; y <= mx.y; ++y)
+                    for (int x = mn.x; x <= mx.x; ++x) {
+                        I3 c{x, y, z};
+                        size_t i = lin(x - mn.x, y - mn.y, z - mn.z, sx, sy);
+                        if (kept && x >= top.minimum.x && y >= top.minimum.y && z >= top.minimum.z &&
+                            x <= top.maximum.x && y <= top.maximum.y && z <= top.maximum.z) {
+                            size_t old = lin(x - top.minimum.x, y - top.minimum.y, z - top.minimum.z, top.sx, top.sy);
+                            blocked[i] = top.blocked[old];
+                            fields[i] = clip_field(top.fields[old], I3{x - mn.x, y - mn.y, z - mn.z},
+                                                   I3{mx.x - x, mx.y - y, mx.z - z});
+                        } else {
+                            blocked[i] = is_blocked(c);
+                            if (!blocked[i]) growth.touch(i);
+                        }
+                    }
+        }
+        std::vector<I3> newly;
+        for (const I3 &c : changed) {
+            if (c.x < mn.x || c.y < mn.y || c.z < mn.z || c.x > mx.x || c.y > mx.y || c.z > mx.z) continue;
+            size_t i = lin(c.x - mn.x, c.y - mn.y, c.z - mn.z, sx, sy);
+            uint8_t now = is_blocked(c);
+            if (now == blocked[i]) continue;
+            blocked[i] = now;
+            if (now) { fields[i] = ZERO_AADF; newly.push_back(I3{c.x - mn.x, c.y - mn.y, c.z - mn.z}); }
+            else growth.touch(i);
+        }
+        if (!newly.empty()) cut_boxes(fields, blocked, sx, sy, sz, newly, growth);
+        growth.run(fields, blocked);
+        TopLevel t;
+        t.minimum = mn; t.maximum = mx; t.sx = sx; t.sy = sy; t.sz = sz;
+        t.fields = std::move(fields);
+        t.blocked = std::move(blocked);
+        top = std::move(t);
+    }
+
+    // world.rs:982-1046 (with_page_contents) over dense pages.
+    void apply_pages(const std::vector<std::pair<I3, const uint16_t *>> &changed_pages, const std::vector<I3> &removed) {
+        std::map<std::tuple<int, int, int>, I3> touched;  // ordered (z,y,x) below
+        std::unordered_map<uint64_t, std::shared_ptr<Region>> previous = regions;
+        for (const I3 &p : removed) {
+            pages.erase(key_of(p)); page_keys.erase(key_of(p));
+            for (int o = 0; o < 8; ++o) {
+                I3 k{p.x * 2 + (o & 1), p.y * 2 + ((o >> 1) & 1), p.z * 2 + (o >> 2)};
+                if (regions.erase(key_of(k))) touched[std::make_tuple(k.z, k.y, k.x)] = k;
+            }
+        }
+        std::vector<uint16_t> mats(4096);
+        for (auto &pp : changed_pages) {
+            I3 p = pp.first;
+            const uint16_t *page = pp.second;
+            for (int o = 0; o < 8; ++o) {  // world.rs:1506-1508: x fastest
+                I3 off{o & 1, (o >> 1) & 1, o >> 2};
+                for (int z = 0; z < 16; ++z)
+                    for (int y = 0; y < 16; ++y)
+                        for (int x = 0; x < 16; ++x)
+                            mats[lin(x, y, z, 16, 16)] = page[lin(off.x * 16 + x, off.y * 16 + y, off.z * 16 + z, 32, 32)];
+                I3 k{p.x * 2 + off.x, p.y * 2 + off.y, p.z * 2 + off.z};
+                auto it = previous.find(key_of(k));
+                std::shared_ptr<Region> r;
+                if (it != previous.end() && it->second->matches(mats.data())) r = it->second;
+                else r = build_region(k, mats.data());
+                if (it == previous.end() || it->second != r) touched[std::make_tuple(k.z, k.y, k.x)] = k;
+                regions[key_of(k)] = r;
+            }
+        }
+        std::vector<I3> keys;
+        bool topology_unchanged = true;
+        for (auto &kv : touched) {
+            I3 k = kv.second;
+            auto a = previous.find(key_of(k));
+            auto b = regions.find(key_of(k));
+            bool differ;
+            if (a != previous.end() && b != regions.end()) differ = a->second != b->second;
+            else differ = (a != previous.end()) != (b != regions.end());
+            if (!differ) continue;
+            keys.push_back(k);
+            if (!(a != previous.end() && b != regions.end() && a->second->any_solid == b->second->any_solid))
+                topology_unchanged = false;
+        }
+        if (!topology_unchanged) update_top(keys);
+        ++revision;
+    }
+};
+
+// ---------------------------------------------------------------- packed scene (gpu_scene.rs)
+struct Packed {
+    std::vector<int32_t> directory;  // 4 per slot
+    std::vector<uint32_t> headers;   // 4 per region
+    std::vector<uint32_t> cells;
+    uint32_t mask = 0;
+    I3 minimum{0, 0, 0}, maximum{0, 0, 0};
+};
+
+static inline uint32_t pack_bytes(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 8) | (z << 16); }
+static inline uint32_t pack_extents(const Aadf &a) {  // gpu_scene.rs:462-467
+    uint32_t bits = 0;
+    for (int i = 0; i < 6; ++i) bits |= (uint32_t)a.b[i] << (i * 2);
+    return bits;
+}
+
+struct BrickKeyHash {
+    size_t operator()(const std::array<uint32_t, 32> &a) const {
+        uint64_t h = 1469598103934665603ull;
+        for (uint32_t w : a) { h ^= w; h *= 1099511628211ull; }
+        return (size_t)h;
+    }
+};
+
+// gpu_scene.rs:72-309 as a full build, 352-434 (directory).
+static void pack_scene(const Scene &scene, Packed &out) {
+    out = Packed();
+    std::vector<const Region *> order;
+    for (auto &kv : scene.regions) order.push_back(kv.second.get());
+    std::sort(order.begin(), order.end(), [](const Region *a, const Region *b) {
+        if (a->coord.z != b->coord.z) return a->coord.z < b->coord.z;
+        if (a->coord.y != b->coord.y) return a->coord.y < b->coord.y;
+        return a->coord.x < b->coord.x;
+    });
+    std::unordered_map<std::array<uint32_t, 32>, uint32_t, BrickKeyHash> interned;
+    std::vector<std::pair<I3, uint32_t>> entries;
+    for (const Region *r : order) {
+        uint32_t header = (uint32_t)(out.headers.size() / 4);
+        Aadf top = scene.top.field(r->coord);
+        uint32_t negative = pack_bytes(top.b[0], top.b[2], top.b[4]);
+        uint32_t positive = pack_bytes(top.b[1], top.b[3], top.b[5]);
+        uint16_t um = AIR;
+        bool uniform = r->storage == ST_EMPTY || r->storage == ST_UNIFORM;
+        if (r->storage == ST_UNIFORM) um = r->uniform;
+        if (uniform) {
+            uint32_t h[4] = {um == AIR ? 1u : 2u, (uint32_t)um, negative, positive};
+            out.headers.insert(out.headers.end(), h, h + 4);
+        } else {
+            uint32_t offset = (uint32_t)out.cells.size();
+            out.cells.resize(offset + 64, 0);
+            uint32_t h[4] = {3u, offset, negative, positive};
+            out.headers.insert(out.headers.end(), h, h + 4);
+            std::vector<Aadf> bb = r->brick_bounds();
+            uint32_t table[64];
+            for (int index = 0; index < 64; ++index) {
+                int bx = index & 3, by = (index >> 2) & 3, bz = index >> 4;
+                uint16_t first = r->material(bx * 4, by * 4, bz * 4);
+                std::vector<Aadf> vb;
+                bool have_vb = false, uni = true;
+                std::array<uint32_t, 32> data{};
+                for (int i = 0; i < 64; ++i) {
+                    int lx = i & 3, ly = (i >> 2) & 3, lz = i >> 4;
+                    uint16_t m = r->material(bx * 4 + lx, by * 4 + ly, bz * 4 + lz);
+                    uni = uni && m == first;
+                    uint32_t cell;
+                    if (m == AIR) {
+                        if (!have_vb) { vb = r->brick_voxel_bounds(bx, by, bz); have_vb = true; }
+                        cell = pack_extents(vb[i]) << 1;
+                    } else {
+                        cell = ((uint32_t)m << 1) | 1u;
+                    }
+                    data[i / 2] |= cell << ((i & 1) * 16);
+                }
+                if (uni && first == AIR) table[index] = pack_extents(bb[index]) << 2;
+                else if (uni) table[index] = ((uint32_t)first << 2) | 1u;
+                else {
+                    auto it = interned.find(data);
+                    uint32_t pointer;
+                    if (it != interned.end()) pointer = it->second;
+                    else {
+                        pointer = (uint32_t)out.cells.size();
+                        out.cells.insert(out.cells.end(), data.begin(), data.end());
+                        interned[data] = pointer;
+                    }
+                    table[index] = (pointer << 2) | 2u;
+                }
+            }
+            memcpy(&out.cells[offset], table, sizeof(table));
+        }
+        entries.push_back({r->coord, header});
+    }
+    // gpu_scene.rs:411-434
+    size_t capacity = 1;
+    while (capacity < entries.size() * 2) capacity <<= 1;
+    for (;;) {
+        out.directory.assign(capacity * 4, 0);
+        for (size_t s = 0; s < capacity; ++s) out.directory[s * 4 + 3] = -1;
+        bool fits = true;
+        for (auto &e : entries) {
+            size_t slot = hash_voxel(e.first) & (capacity - 1);
+            int distance = 0;
+            while (out.directory[slot * 4 + 3] != -1) { slot = (slot + 1) & (capacity - 1); ++distance; }
+            if (distance >= 32) { fits = false; break; }
+            out.directory[slot * 4 + 0] = e.first.x;
+            out.directory[slot * 4 + 1] = e.first.y;
+            out.directory[slot * 4 + 2] = e.first.z;
+            out.directory[slot * 4 + 3] = (int32_t)e.second;
+        }
+        if (fits) break;
+        capacity *= 2;
+    }
+    out.mask = (uint32_t)(capacity - 1);
+    I3 mn{0, 0, 0}, mx{0, 0, 0};
+    scene.bounds(mn, mx);
+    // gpu.rs:808-812: the top level's bounds, which are the loaded regions' bounds.
+    out.minimum = mn;
+    out.maximum = mx;
+}
+
+// ---------------------------------------------------------------- emitters (emission.rs:91-165, tree.rs:28-74)
+struct Emitter {
+    I3 voxel;
+    uint32_t face;
+    uint32_t material;
+    float cumulative, probability;
+};
+struct LightNode {
+    F3 minimum, maximum;
+    float power = 0.0f;
+    uint32_t normals = 0;
+    I3 pivot{0, 0, 0};
+    uint32_t pivot_face = 0, left = 0, right = 0, axis = 0;
+};
+static const I3 FACE_OFFSETS[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+// emission.rs:206-216
+static F3 face_point(const Emitter &f, float u, float v) {
+    F3 normal = to_f3(FACE_OFFSETS[f.face]);
+    uint32_t axis = f.face / 2;
+    F3 tangent = axis == 0 ? f3(0, 1, 0) : f3(1, 0, 0);
+    F3 bitangent = axis == 2 ? f3(0, 1, 0) : f3(0, 0, 1);
+    return to_f3(f.voxel) + f3(0.5f, 0.5f, 0.5f) + normal * 0.5f + tangent * (u - 0.5f) + bitangent * (v - 0.5f);
+}
+
+struct TreeKey { int32_t a, b, c; uint32_t face; };
+static inline bool key_lt(const TreeKey &p, const TreeKey &q) {
+    if (p.a != q.a) return p.a < q.a;
+    if (p.b != q.b) return p.b < q.b;
+    if (p.c != q.c) return p.c < q.c;
+    return p.face < q.face;
+}
+static TreeKey tree_key(I3 v, uint32_t face, uint32_t axis) {
+    return TreeKey{v[(int)axis], v[(int)((axis + 1) % 3)], v[(int)((axis + 2) % 3)], face};
+}
+
+static uint32_t tree_split(std::vector<LightNode> &nodes, size_t *idx, size_t n, const std::vector<Emitter> &faces) {
+    uint32_t index = (uint32_t)nodes.size();
+    LightNode node;
+    float inf = bitsf(0x7f800000u);
+    node.minimum = f3(inf, inf, inf);
+    node.maximum = f3(-inf, -inf, -inf);
+    for (size_t k = 0; k < n; ++k) {
+        const Emitter &f = faces[idx[k]];
+        F3 center = face_point(f, 0.5f, 0.5f);
+        F3 an = to_f3(FACE_OFFSETS[f.face]);
+        F3 extent = (f3(1, 1, 1) - f3(std::fabs(an.x), std::fabs(an.y), std::fabs(an.z))) * 0.5f;
+        F3 lo = center - extent, hi = center + extent;
+        node.minimum = f3(fmin_(node.minimum.x, lo.x), fmin_(node.minimum.y, lo.y), fmin_(node.minimum.z, lo.z));
+        node.maximum = f3(fmax_(node.maximum.x, hi.x), fmax_(node.maximum.y, hi.y), fmax_(node.maximum.z, hi.z));
+        node.power = node.power + f.probability;
+        node.normals |= 1u << f.face;
+    }
+    nodes.push_back(node);
+    if (n == 1) {
+        node.left = NONE;
+        node.right = (uint32_t)idx[0];
+    } else {
+        F3 e = node.maximum - node.minimum;
+        node.axis = (e.x >= e.y && e.x >= e.z) ? 0 : (e.y >= e.z ? 1 : 2);
+        uint32_t axis = node.axis;
+        std::sort(idx, idx + n, [&](size_t a, size_t b) {
+            return key_lt(tree_key(faces[a].voxel, faces[a].face, axis), tree_key(faces[b].voxel, faces[b].face, axis));
+        });
+        size_t mid = n / 2;
+        node.pivot = faces[idx[mid]].voxel;
+        node.pivot_face = faces[idx[mid]].face;
+        node.left = tree_split(nodes, idx, mid, faces);
+        node.right = tree_split(nodes, idx + mid, n - mid, faces);
+    }
+    nodes[index] = node;
+    return index;
+}
+
+struct Emission {
+    std::vector<Emitter> faces;
+    float total_weight = 0.0f;
+    std::vector<LightNode> tree;
+    std::vector<uint32_t> words;  // gpu.rs:686-713 layout
+};
+
+static void build_emission(const Scene &scene, const float *emission_rgb, Emission &out) {
+    out = Emission();
+    float weights[256];
+    uint64_t emitting[4] = {0, 0, 0, 0};
+    for (int m = 0; m < 256; ++m) {
+        weights[m] = fmax_(fmax_(emission_rgb[m * 3], emission_rgb[m * 3 + 1]), emission_rgb[m * 3 + 2]);
+        if (weights[m] > 0.0f) emitting[m / 64] |= 1ull << (m % 64);
+    }
+    if ((emitting[0] | emitting[1] | emitting[2] | emitting[3]) != 0) {
+        std::vector<const Region *> rs;
+        for (auto &kv : scene.regions) {
+            const Region *r = kv.second.get();
+            bool any = false;
+            for (int w = 0; w < 4; ++w) any = any || (r->mask[w] & emitting[w]);
+            if (any) rs.push_back(r);
+        }
+        std::sort(rs.begin(), rs.end(), [](const Region *a, const Region *b) {
+            if (a->coord.z != b->coord.z) return a->coord.z < b->coord.z;
+            if (a->coord.y != b->coord.y) return a->coord.y < b->coord.y;
+            return a->coord.x < b->coord.x;
+        });
+        double total = 0.0;
+        for (const Region *r : rs)
+            for (int z = 0; z < 16; ++z)
+                for (int y = 0; y < 16; ++y)
+                    for (int x = 0; x < 16; ++x) {
+                        uint16_t m = r->material(x, y, z);
+                        if (m == AIR || weights[m] <= 0.0f) continue;
+                        I3 v{r->coord.x * 16 + x, r->coord.y * 16 + y, r->coord.z * 16 + z};
+                        for (uint32_t f = 0; f < 6; ++f) {
+                            uint32_t nm = scene.material_at(I3{v.x + FACE_OFFSETS[f].x, v.y + FACE_OFFSETS[f].y, v.z + FACE_OFFSETS[f].z});
+                            if (nm != 0xffffu && nm != AIR) continue;
+                            total += (double)weights[m];
+                            out.faces.push_back(Emitter{v, f, m, (float)total, weights[m]});
+                        }
+                    }
+        if (total != 0.0) {
+            out.total_weight = (float)total;
+            for (auto &f : out.faces) { f.cumulative = f.cumulative / out.total_weight; f.probability = f.probability / out.total_weight; }
+            out.faces.back().cumulative = 1.0f;
+            if (out.faces.size() >= 32) {
+                std::vector<size_t> idx(out.faces.size());
+                for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+                out.tree.reserve(out.faces.size() * 2);
+                tree_split(out.tree, idx.data(), idx.size(), out.faces);
+            }
+        } else {
+            out.faces.clear();
+        }
+    }
+    // Words: two uvec4 per face, then four per node (tree.rs:173-192).
+    size_t nf = out.faces.size();
+    out.words.assign((std::max<size_t>(nf, 1)) * 8 + out.tree.size() * 16, 0);
+    for (size_t i = 0; i < nf; ++i) {
+        const Emitter &f = out.faces[i];
+        uint32_t *w = &out.words[i * 8];
+        w[0] = (uint32_t)f.voxel.x; w[1] = (uint32_t)f.voxel.y; w[2] = (uint32_t)f.voxel.z; w[3] = f.face;
+        w[4] = fbits(f.cumulative); w[5] = fbits(f.probability); w[6] = fbits((float)f.material);
+        w[7] = fbits((float)out.tree.size());
+    }
+    size_t base = std::max<size_t>(nf, 1) * 8;
+    // load_node offsets from u32(extra.w) * 2 uvec4s, i.e. nf * 8 words: with no
+    // faces there is no tree, so the one padding record never shifts a node.
+    for (size_t i = 0; i < out.tree.size(); ++i) {
+        const LightNode &n = out.tree[i];
+        uint32_t *w = &out.words[base + i * 16];
+        w[0] = fbits(n.minimum.x); w[1] = fbits(n.minimum.y); w[2] = fbits(n.minimum.z); w[3] = fbits(n.power);
+        w[4] = fbits(n.maximum.x); w[5] = fbits(n.maximum.y); w[6] = fbits(n.maximum.z); w[7] = n.normals;
+        w[8] = (uint32_t)n.pivot.x; w[9] = (uint32_t)n.pivot.y; w[10] = (uint32_t)n.pivot.z; w[11] = n.pivot_face;
+        w[12] = n.left; w[13] = n.right; w[14] = n.axis; w[15] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- tracer state
+struct Material { F3 diffuse, emission; float power; };
+struct Light { float pr[4]; float energy[4]; float extra[4]; };
+
+struct FaceIndex {  // gpu/face_index.rs:9-188
+    std::vector<std::array<uint32_t, 4>> entries;
+    size_t len = 0;
+    FaceIndex() { entries.assign(1024, {0, 0, 0, NONE}); }
+    size_t mask() const { return entries.size() - 1; }
+    static bool live(const std::array<uint32_t, 4> &e) { return e[3] != NONE; }
+    long find(I3 v, uint32_t face) const {
+        size_t m = mask();
+        size_t i = stream_id(v, face) & m;
+        for (;;) {
+            const auto &e = entries[i];
+            if (!live(e)) return -1;
+            if ((int32_t)e[0] == v.x && (int32_t)e[1] == v.y && (int32_t)e[2] == v.z && (e[3] >> 29) == face) return (long)i;
+            i = (i + 1) & m;
+        }
+    }
+    void raw_insert(const std::array<uint32_t, 4> &e) {
+        size_t m = mask();
+        I3 v{(int32_t)e[0], (int32_t)e[1], (int32_t)e[2]};
+        size_t i = stream_id(v, e[3] >> 29) & m;
+        while (live(entries[i])) i = (i + 1) & m;
+        entries[i] = e;
+    }
+    void grow() {  // face_index.rs grow: double and reinsert in table order
+        std::vector<std::array<uint32_t, 4>> old;
+        old.swap(entries);
+        entries.assign(old.size() * 2, {0, 0, 0, NONE});
+        for (auto &e : old) if (live(e)) raw_insert(e);
+    }
+    void insert(I3 v, uint32_t face, uint32_t slot) {
+        long f = find(v, face);
+        if (f >= 0) { entries[f][3] = (face << 29) | slot; return; }
+        if ((len + 1) * 2 > entries.size()) grow();
+        raw_insert({(uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (face << 29) | slot});
+        ++len;
+    }
+};
+
+struct Tracer {
+    Scene *scene;
+    Packed packed;
+    Emission emission;
+    Material materials[256];
+    float diffuse_rgb[768], emission_rgb[768];
+    std::vector<Light> lights;
+    uint32_t directional = 0;
+    orc_transport transport{};
+    uint64_t synced_revision = ~0ull;
+    bool materials_changed = true;
+    // resident cache
+    bool cache_on = false;
+    FaceIndex face_index;
+    std::vector<uint32_t> state;       // 36 words per slot
+    std::vector<uint32_t> checkpoint;  // 12 words per slot
+    std::vector<uint32_t> meta;        // 12 words per slot
+    std::vector<uint32_t> free_slots;
+    uint32_t high_water = 0;
+    uint32_t history_limit = 65536;
+    float recency = 1.0f;
+    float measured_retention = 1.0f;
+    float response = 0.075f;  // settings.rs:266 display_response 75 ms
+    std::unordered_map<uint64_t, uint32_t> slot_of;
+};
+
+// ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
+struct RayResult {
+    I3 voxel{0, 0, 0};
+    uint32_t material = 0;
+    F3 position{0, 0, 0};
+    uint32_t status = 0;
+    F3 normal{0, 0, 0};
+};
+
+struct Counters {
+    uint64_t rays = 0, steps = 0, region_loads = 0, brick_loads = 0, cell_loads = 0, paths = 0;
+};
+
+struct Ctx {
+    const Tracer *t;
+    Counters *c;
+};
+
+static inline uint32_t lookup_region(const Packed &p, I3 c, Counters *cn) {  // voxel_traversal.wgsl:32-41
+    uint32_t slot = hash_voxel(c) & p.mask;
+    for (uint32_t attempt = 0; attempt < 32u; ++attempt) {
+        const int32_t *e = &p.directory[slot * 4];
+        if (cn) cn->region_loads++;
+        if (e[3] == -1) return NONE;
+        if (e[0] == c.x && e[1] == c.y && e[2] == c.z) return (uint32_t)e[3];
+        slot = (slot + 1u) & p.mask;
+    }
+    return NONE;
+}
+static inline bool in_bounds(const Packed &p, I3 r) {
+    return r.x >= p.minimum.x && r.y >= p.minimum.y && r.z >= p.minimum.z && r.x <= p.maximum.x && r.y <= p.maximum.y &&
+           r.z <= p.maximum.z;
+}
+static inline I3 unpack_bytes(uint32_t v) { return I3{(int)(v & 255u), (int)((v >> 8) & 255u), (int)((v >> 16) & 255u)}; }
+static inline I3 negative_extents(uint32_t b) { return I3{(int)(b & 3u), (int)((b >> 4) & 3u), (int)((b >> 8) & 3u)}; }
+static inline I3 positive_extents(uint32_t b) { return I3{(int)((b >> 2) & 3u), (int)((b >> 6) & 3u), (int)((b >> 10) & 3u)}; }
+
+// voxel_traversal.wgsl:90-107
+static void advance_box(F3 position, F3 direction, I3 cell, I3 negative, I3 positive, float size, F3 &normal_out, float &dist_out) {
+    F3 minimum = f3((float)(cell.x - negative.x), (float)(cell.y - negative.y), (float)(cell.z - negative.z)) * size;
+    F3 maximum = f3((float)(cell.x + positive.x + 1), (float)(cell.y + positive.y + 1), (float)(cell.z + positive.z + 1)) * size;
+    F3 travel;
+    for (int a = 0; a < 3; ++a) {
+        bool moving = std::fabs(direction[a]) > 1e-8f;
+        float plane = direction[a] > 0.0f ? maximum[a] : minimum[a];
+        float raw = (plane - position[a]) / (moving ? direction[a] : 1.0f);
+        travel[a] = (moving && raw >= -0.001f && raw < 1e30f) ? raw : 1e30f;
+    }
+    bool choose_y = travel.y < fmax_(travel.x, 0.0f);
+    float distance_xy = fmax_(choose_y ? travel.y : travel.x, 0.0f);
+    bool choose_z = travel.z < distance_xy;
+    float distance = fmax_(choose_z ? travel.z : distance_xy, 0.0f);
+    F3 axis = choose_z ? f3(0, 0, 1) : (choose_y ? f3(0, 1, 0) : f3(1, 0, 0));
+    F3 sgn = f3(direction.x > 0.0f ? -1.0f : 1.0f, direction.y > 0.0f ? -1.0f : 1.0f, direction.z > 0.0f ? -1.0f : 1.0f);
+    normal_out = distance < 1e30f ? axis * sgn : f3(0, 0, 0);
+    dist_out = fmax_(distance + 0.001f, 0.001f);
+}
+
+// voxel_traversal.wgsl:108-121
+static bool geometry_ahead(const Packed &p, F3 start, F3 direction, float max_distance, Counters *cn) {
+    F3 position = start;
+    float remaining = max_distance;
+    for (uint32_t step = 0; step < 2048u; ++step) {
+        I3 region{(int)std::floor(position.x / 16.0f), (int)std::floor(position.y / 16.0f), (int)std::floor(position.z / 16.0f)};
+        if (remaining < 0.0f || !in_bounds(p, region)) return false;
+        uint32_t index = lookup_region(p, region, cn);
+        if (index != NONE && p.headers[index * 4] != 1u) return true;
+        F3 nrm;
+        float travel;
+        advance_box(position, direction, region, I3{0, 0, 0}, I3{0, 0, 0}, 16.0f, nrm, travel);
+        position = position + direction * travel;
+        remaining = remaining - travel;
+    }
+    return true;
+}
+
+// voxel_traversal.wgsl:126-219
+static RayResult trace_domain_limited(const Packed &p, F3 origin, F3 direction, float max_distance, bool visible_geometry,
+                                      uint32_t max_steps, Counters *cn) {
+    RayResult result;
+    if (cn) cn->rays++;
+    float near_ = -1e30f, far_ = 1e30f;
+    F3 normal = f3(0, 0, 0);
+    F3 minimum = to_f3(p.minimum) * 16.0f;
+    F3 maximum = f3((float)(p.maximum.x + 1), (float)(p.maximum.y + 1), (float)(p.maximum.z + 1)) * 16.0f;
+    for (int axis = 0; axis < 3; ++axis) {
+        if (std::fabs(direction[axis]) <= 1e-8f) {
+            if (origin[axis] < minimum[axis] || origin[axis] > maximum[axis]) return result;
+            continue;
+        }
+        float first = (minimum[axis] - origin[axis]) / direction[axis];
+        float second = (maximum[axis] - origin[axis]) / direction[axis];
+        if (fmin_(first, second) > near_) {
+            near_ = fmin_(first, second);
+            normal = f3(0, 0, 0);
+            normal[axis] = first < second ? -1.0f : 1.0f;
+        }
+        far_ = fmin_(far_, fmax_(first, second));
+    }
+    if (far_ < fmax_(near_, 0.0f)) return result;
+    float distance = fmax_(near_, 0.0f) + 0.001f;
+    F3 position = origin + direction * distance;
+    I3 last_region{0x7fffffff, 0x7fffffff, 0x7fffffff};
+    uint32_t meta[4] = {0, 0, 0, 0};
+    for (uint32_t step = 0; step < max_steps; ++step) {
+        if (cn) cn->steps++;
+        if (distance > max_distance) return result;
+        I3 voxel{(int)std::floor(position.x), (int)std::floor(position.y), (int)std::floor(position.z)};
+        I3 region{voxel.x >> 4, voxel.y >> 4, voxel.z >> 4};
+        if (!in_bounds(p, region)) return result;
+        if (last_region != region) {
+            uint32_t index = lookup_region(p, region, cn);
+            if (index == NONE) {
+                if (visible_geometry) {
+                    F3 nrm;
+                    float w;
+                    advance_box(position, direction, region, I3{0, 0, 0}, I3{0, 0, 0}, 16.0f, nrm, w);
+                    distance = distance + w;
+                    position = position + direction * w;
+                    normal = nrm;
+                    continue;
+                }
+                result.status = geometry_ahead(p, position, direction, max_distance - distance, cn) ? 2u : 0u;
+                return result;
+            }
+            memcpy(meta, &p.headers[index * 4], 16);
+            last_region = region;
+        }
+        F3 adv_n = f3(0, 0, 0);
+        float adv_w = 0.0f;
+        if (meta[0] == 1u) {
+            advance_box(position, direction, region, unpack_bytes(meta[2]), unpack_bytes(meta[3]), 16.0f, adv_n, adv_w);
+        } else {
+            uint32_t material = meta[1];
+            if (meta[0] == 3u) {
+                uint32_t lx = (uint32_t)(voxel.x & 15), ly = (uint32_t)(voxel.y & 15), lz = (uint32_t)(voxel.z & 15);
+                uint32_t bx = lx / 4u, by = ly / 4u, bz = lz / 4u;
+                uint32_t brick_data = p.cells[meta[1] + (bz * 4u + by) * 4u + bx];
+                if (cn) cn->brick_loads++;
+                uint32_t st = brick_data & 3u;
+                if (st == 0u) {
+                    uint32_t ext = brick_data >> 2u;
+                    advance_box(position, direction, I3{region.x * 4 + (int)bx, region.y * 4 + (int)by, region.z * 4 + (int)bz},
+                                negative_extents(ext), positive_extents(ext), 4.0f, adv_n, adv_w);
+                    material = 256u;
+                } else if (st == 1u) {
+                    material = brick_data >> 2u;
+                } else {
+                    uint32_t ix = lx & 3u, iy = ly & 3u, iz = lz & 3u;
+                    uint32_t idx = (iz * 4u + iy) * 4u + ix;
+                    uint32_t word = p.cells[(brick_data >> 2u) + idx / 2u];
+                    if (cn) cn->cell_loads++;
+                    uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
+                    material = (cell & 1u) != 0u ? cell >> 1u : 256u;
+                    if (material == 256u) {
+                        uint32_t ext = cell >> 1u;
+                        advance_box(position, direction, voxel, negative_extents(ext), positive_extents(ext), 1.0f, adv_n, adv_w);
+                    }
+                }
+            }
+            if (material != 256u) {
+                result.voxel = voxel;
+                result.material = material;
+                result.position = position;
+                result.normal = normal;
+                result.status = 1u;
+                return result;
+            }
+        }
+        distance = distance + adv_w;
+        position = position + direction * adv_w;
+        normal = adv_n;
+    }
+    result.status = 2u;
+    result.material = NONE;
+    return result;
+}
+
+// ---------------------------------------------------------------- estimator (voxel_path.wgsl)
+struct Estimator {
+    const Tracer &t;
+    Counters *cn;
+    const Packed &p;
+    uint32_t nfaces;
+    bool has_tree;
+
+    explicit Estimator(const Tracer &tr, Counters *c) : t(tr), cn(c), p(tr.packed) {
+        nfaces = (uint32_t)t.emission.faces.size();
+        has_tree = bitsf(t.emission.words[7]) > 0.0f;
+    }
+    float extra_w() const { return (float)nfaces; }
+
+    // voxel_path.wgsl:239-243 (BUDGETED_RAYS false, PATH_STEPS 2048)
+    RayResult trace(F3 origin, F3 direction, float max_distance) const {
+        return trace_domain_limited(p, origin, direction, max_distance, false, 2048u, cn);
+    }
+    float connection_reach() const { return t.transport.limited_reach ? t.transport.reach : t.transport.max_distance; }
+    // voxel_path.wgsl:271-282
+    RayResult trace_diffuse(F3 origin, F3 direction) const {
+        float reach = connection_reach();
+        RayResult r = trace(origin, direction, reach);
+        if (t.transport.limited_reach && r.status == 0u &&
+            geometry_ahead(p, origin + direction * reach, direction, t.transport.max_distance - reach, cn))
+            r.status = 3u;
+        return r;
+    }
+    // voxel_path.wgsl:283-307
+    void direct_at(F3 position, F3 normal, F3 &value, F3 &custom) const {
+        value = f3(0, 0, 0);
+        custom = f3(0, 0, 0);
+        F3 origin = position + normal * 0.002f;
+        for (size_t i = 0; i < t.lights.size(); ++i) {
+            const Light &l = t.lights[i];
+            F3 lp = f3(l.pr[0], l.pr[1], l.pr[2]);
+            F3 direction = normalize3(lp);
+            float distance = t.transport.max_distance;
+            float attenuation = 1.0f;
+            if (l.pr[3] >= 0.0f) {
+                F3 delta = lp - position;
+                float d2 = dot3(delta, delta);
+                if (d2 <= 0.0000001f || d2 > l.pr[3] * l.pr[3]) continue;
+                distance = std::sqrt(d2);
+                direction = delta / distance;
+                distance = distance - 0.004f;
+                attenuation = 1.0f / fmax_(d2, 1.0f);
+            }
+            float cosine = fmax_(dot3(normal, direction), 0.0f);
+            if (cosine > 0.0f && trace(origin, direction, distance).status == 0u) {
+                float s = cosine * attenuation;
+                value = value + f3(l.energy[0], l.energy[1], l.energy[2]) * s;
+                custom = custom + f3(l.extra[1], l.extra[2], l.extra[3]) * s;
+            }
+        }
+    }
+    // voxel_path.wgsl:66-87
+    bool cached_energy(I3 voxel, uint32_t face, F3 &value, uint32_t *slot_out) const {
+        if (slot_out) *slot_out = NONE;
+        if (!t.cache_on || face > 5u) return false;
+        uint32_t mask = (uint32_t)t.face_index.mask();
+        uint32_t slot = (hash_voxel(voxel) ^ pcg(face + 1u)) & mask;
+        for (uint32_t attempt = 0; attempt < 64u; ++attempt) {
+            const auto &e = t.face_index.entries[slot];
+            if (e[3] == NONE) return false;
+            if ((int32_t)e[0] == voxel.x && (int32_t)e[1] == voxel.y && (int32_t)e[2] == voxel.z && (e[3] >> 29) == face) {
+                uint32_t s = e[3] & 0x1fffffffu;
+                if (slot_out) *slot_out = s;
+                const uint32_t *st = &t.state[s * 36];
+                uint32_t count = st[5];
+                if (count < 16u || (count & 0x80000000u) != 0u) return false;
+                value = f3(bitsf(st[0]), bitsf(st[1]), bitsf(st[2]));
+                return true;
+            }
+            slot = (slot + 1u) & mask;
+        }
+        return false;
+    }
+    // emitter records (voxel_path.wgsl:89-107)
+    void load_emitter(uint32_t index, I3 &voxel, uint32_t &face, float sampling[4]) const {
+        const uint32_t *w = &t.emission.words[index * 8];
+        voxel = I3{(int32_t)w[0], (int32_t)w[1], (int32_t)w[2]};
+        face = w[3];
+        for (int k = 0; k < 4; ++k) sampling[k] = bitsf(w[4 + k]);
+    }
+    struct Node { F3 minimum; float power; F3 maximum; uint32_t normals; I3 pivot; uint32_t face; uint32_t children[4]; };
+    Node load_node(uint32_t index) const {
+        const uint32_t *w = &t.emission.words[(size_t)nfaces * 8 + (size_t)index * 16];
+        Node n;
+        n.minimum = f3(bitsf(w[0]), bitsf(w[1]), bitsf(w[2])); n.power = bitsf(w[3]);
+        n.maximum = f3(bitsf(w[4]), bitsf(w[5]), bitsf(w[6])); n.normals = w[7];
+        n.pivot = I3{(int32_t)w[8], (int32_t)w[9], (int32_t)w[10]}; n.face = w[11];
+        for (int k = 0; k < 4; ++k) n.children[k] = w[12 + k];
+        return n;
+    }
+    // voxel_path.wgsl:108-122
+    static float node_importance(const Node &node, F3 position, F3 normal) {
+        F3 delta = (node.minimum + node.maximum) * 0.5f - position;
+        float radius = length3(node.maximum - node.minimum) * 0.5f;
+        float distance = fmax_(length3(delta), 1e-5f);
+        F3 direction = delta / distance;
+        float spread = fmin_(radius / distance, 1.0f);
+        float receiver = clampf_(dot3(normal, direction) + spread, 0.0f, 1.0f);
+        F3 positive = f3((node.normals & 1u) ? -direction.x : -1.0f, (node.normals & 4u) ? -direction.y : -1.0f,
+                         (node.normals & 16u) ? -direction.z : -1.0f);
+        F3 negative = f3((node.normals & 2u) ? direction.x : -1.0f, (node.normals & 8u) ? direction.y : -1.0f,
+                         (node.normals & 32u) ? direction.z : -1.0f);
+        F3 projected = f3(fmax_(positive.x, negative.x), fmax_(positive.y, negative.y), fmax_(positive.z, negative.z));
+        float emitter = clampf_(fmax_(fmax_(projected.x, projected.y), projected.z) + spread, 0.0f, 1.0f);
+        return node.power * receiver * emitter / (distance * distance + radius * radius + 0.25f);
+    }
+    // voxel_path.wgsl:123-131
+    float node_left_probability(const Node &node, F3 position, F3 normal) const {
+        Node left = load_node(node.children[0]);
+        Node right = load_node(node.children[1]);
+        float power = left.power / (left.power + right.power);
+        float a = node_importance(left, position, normal);
+        float b = node_importance(right, position, normal);
+        if (a + b <= 1e-20f) return clampf_(power, 1e-6f, 1.0f - 1e-6f);
+        return clampf_(0.05f * power + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
+    }
+    // voxel_path.wgsl:132-149
+    void tree_emitter(F3 position, F3 normal, float choice, I3 &voxel, uint32_t &face, float sampling[4]) const {
+        uint32_t index = 0;
+        float uniform = choice;
+        float probability = 1.0f;
+        for (;;) {
+            Node node = load_node(index);
+            if (node.children[0] == NONE) {
+                load_emitter(node.children[1], voxel, face, sampling);
+                sampling[1] = probability;
+                return;
+            }
+            float pl = node_left_probability(node, position, normal);
+            if (uniform < pl) { probability = probability * pl; uniform = uniform / pl; index = node.children[0]; }
+            else { probability = probability * (1.0f - pl); uniform = (uniform - pl) / (1.0f - pl); index = node.children[1]; }
+            uniform = fmin_(uniform, bitsf(0x3f7fffffu));
+        }
+    }
+    // voxel_path.wgsl:150-174
+    float tree_probability(F3 position, F3 normal, I3 voxel, F3 hit_normal) const {
+        uint32_t face = 0;
+        for (uint32_t i = 0; i < 6u; ++i) {
+            F3 fn = face_normal(i);
+            if (fn.x == hit_normal.x && fn.y == hit_normal.y && fn.z == hit_normal.z) face = i;
+        }
+        uint32_t index = 0;
+        float probability = 1.0f;
+        for (;;) {
+            Node node = load_node(index);
+            if (node.children[0] == NONE) {
+                I3 fv; uint32_t ff; float s[4];
+                load_emitter(node.children[1], fv, ff, s);
+                return (fv == voxel && ff == face) ? probability : 0.0f;
+            }
+            float pl = node_left_probability(node, position, normal);
+            uint32_t axis = node.children[2];
+            bool less = false, decided = false;
+            for (uint32_t j = 0; j < 3u; ++j) {
+                uint32_t i = (axis + j) % 3u;
+                if (voxel[(int)i] != node.pivot[(int)i]) { less = voxel[(int)i] < node.pivot[(int)i]; decided = true; break; }
+            }
+            if (!decided) less = face < node.face;
+            if (less) { probability = probability * pl; index = node.children[0]; }
+            else { probability = probability * (1.0f - pl); index = node.children[1]; }
+        }
+    }
+    // voxel_path.wgsl:309-324
+    void choose_emitter(F3 position, F3 normal, float choice, I3 &voxel, uint32_t &face, float sampling[4]) const {
+        uint32_t count = nfaces;
+        if (has_tree) { tree_emitter(position, normal, choice, voxel, face, sampling); return; }
+        uint32_t low = 0, high = count;
+        while (low < high) {
+            uint32_t middle = (low + high) / 2u;
+            I3 v; uint32_t f; float s[4];
+            load_emitter(middle, v, f, s);
+            if (s[0] <= choice) low = middle + 1u; else high = middle;
+        }
+        load_emitter(std::min(low, count - 1u), voxel, face, sampling);
+    }
+    // voxel_path.wgsl:332-364
+    F3 sample_emitter(F3 position, F3 normal, uint32_t source_stream, uint32_t sequence, uint32_t bounce) const {
+        F3 zero = f3(0, 0, 0);
+        if (extra_w() == 0.0f) return zero;
+        uint32_t stream = source_stream ^ pcg(bounce + 0xa341316cu);
+        I3 ev; uint32_t ef; float es[4];
+        choose_emitter(position, normal, sample_1d(stream, sequence, 12u), ev, ef, es);
+        F3 en = face_normal(ef);
+        uint32_t axis = ef / 2u;
+        F3 tangent = axis == 0u ? f3(0, 1, 0) : f3(1, 0, 0);
+        F3 bitangent = axis == 2u ? f3(0, 1, 0) : f3(0, 0, 1);
+        F3 light_position = to_f3(ev) + f3(0.5f, 0.5f, 0.5f) + en * 0.5f + tangent * (sample_1d(stream, sequence, 13u) - 0.5f) +
+                            bitangent * (sample_1d(stream, sequence, 14u) - 0.5f);
+        F3 delta = light_position - position;
+        float d2 = dot3(delta, delta);
+        float reach = connection_reach();
+        if (d2 < 0.0001f || d2 > reach * reach) return zero;
+        float distance = std::sqrt(d2);
+        F3 direction = delta / distance;
+        float receiver_cosine = fmax_(dot3(normal, direction), 0.0f);
+        float emitter_cosine = fmax_(dot3(en, -direction), 0.0f);
+        if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || es[1] <= 0.0f) return zero;
+        float light_pdf = es[1] * d2 / emitter_cosine;
+        float directional_pdf = receiver_cosine / 3.14159265359f;
+        F3 shadow_origin = position + normal * 0.002f;
+        F3 shadow_delta = light_position + en * 0.002f - shadow_origin;
+        float shadow_distance = length3(shadow_delta);
+        if (trace(shadow_origin, shadow_delta / shadow_distance, shadow_distance).status != 0u) return zero;
+        const Material &m = t.materials[(uint32_t)es[2]];
+        float factor = directional_pdf / (light_pdf + directional_pdf);
+        return m.emission * factor;
+    }
+    // voxel_path.wgsl:368-394
+    F3 sample_terminal_emission(F3 position, F3 normal, uint32_t stream, uint32_t sequence, uint32_t bounce) const {
+        F3 zero = f3(0, 0, 0);
+        if (extra_w() == 0.0f) return zero;
+        if (continuation_sample(stream, sequence, bounce, 2u) < 0.5f) {
+            F3 r = sample_emitter(position, normal, stream, sequence, bounce + 1u);
+            return r * 2.0f;
+        }
+        F3 direction = hemisphere(normal, continuation_sample(stream, sequence, bounce, 0u), continuation_sample(stream, sequence, bounce, 1u));
+        if (dot3(normal, direction) <= 0.0f) return zero;
+        float reach = connection_reach();
+        RayResult hit = trace(position + normal * 0.002f, direction, reach);
+        if (hit.status != 1u) return zero;
+        const Material &m = t.materials[hit.material];
+        float power = m.power;
+        F3 delta = hit.position - position;
+        float d2 = dot3(delta, delta);
+        if (power == 0.0f || d2 < 0.0001f || d2 > reach * reach) return zero;
+        float selection_pdf = power / fmax_(t.emission.total_weight, 1e-20f);
+        if (has_tree) selection_pdf = tree_probability(position, normal, hit.voxel, hit.normal);
+        float light_pdf = selection_pdf * d2 / fmax_(dot3(hit.normal, -direction), 1e-8f);
+        float directional_pdf = cosine_pdf(normal, direction);
+        float factor = 2.0f * directional_pdf / (directional_pdf + light_pdf);
+        return m.emission * factor;
+    }
+
+    struct Hit { I3 voxel; uint32_t face, status, material, slot; };
+
+    // voxel_path.wgsl:418-492 (begin_path + advance_path, terminal = true)
+    F3 trace_path(F3 start, F3 first_direction, F3 initial_normal, uint32_t stream, uint32_t sequence, bool &unknown, Hit *first) const {
+        F3 energy = f3(0, 0, 0);
+        unknown = false;
+        F3 throughput = f3(1, 1, 1);
+        F3 origin = start;
+        F3 direction = first_direction;
+        F3 previous_normal = initial_normal;
+        float previous_pdf = cosine_pdf(initial_normal, first_direction);
+        F3 previous_position = start - initial_normal * 0.001f;
+        if (previous_pdf <= 0.0f) return energy;
+        uint32_t steps = t.transport.bounces;
+        uint32_t bounce = 0;
+        for (uint32_t step = 0; step < steps; ++step) {
+            RayResult hit = trace_diffuse(origin, direction);
+            if (bounce == 0u && first) {
+                first->voxel = hit.voxel; first->status = hit.status; first->material = hit.material;
+                first->face = hit.status == 1u ? face_index_of(hit.normal) : 6u;
+                first->slot = NONE;
+            }
+            if (hit.status == 0u) {
+                float horizon = clampf_(direction.y * 0.35f + 0.65f, 0.15f, 1.0f);
+                F3 env = f3(t.transport.environment[0], t.transport.environment[1], t.transport.environment[2]);
+                energy = energy + throughput * env * horizon;
+                break;
+            }
+            if (hit.status == 2u) { unknown = true; break; }
+            if (hit.status == 3u) break;
+            const Material &m = t.materials[hit.material];
+            F3 dv, dc;
+            direct_at(hit.position, hit.normal, dv, dc);
+            float emission_weight = m.power;
+            F3 delta = hit.position - origin;
+            float selection_pdf = emission_weight / fmax_(t.emission.total_weight, 1e-20f);
+            if (emission_weight > 0.0f && extra_w() > 0.0f && has_tree)
+                selection_pdf = tree_probability(previous_position, previous_normal, hit.voxel, hit.normal);
+            float light_pdf = selection_pdf * dot3(delta, delta) / fmax_(dot3(hit.normal, -direction), 1e-8f);
+            float mis = 1.0f;
+            if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
+            F3 cached;
+            uint32_t cslot = NONE;
+            bool found = cached_energy(hit.voxel, face_index_of(hit.normal), cached, &cslot);
+            if (bounce == 0u && first) first->slot = cslot;
+            if (found) {
+                energy = energy + throughput * (m.emission * mis + m.diffuse * (dv + cached));
+                break;
+            }
+            bool last = step + 1u == steps;
+            F3 nee = last ? sample_terminal_emission(hit.position, hit.normal, stream, sequence, bounce)
+                          : sample_emitter(hit.position, hit.normal, stream, sequence, bounce + 1u);
+            energy = energy + throughput * (m.emission * mis + m.diffuse * (dv + nee));
+            if (last) break;
+            throughput = throughput * m.diffuse;
+            float maximum = fmax_(throughput.x, fmax_(throughput.y, throughput.z));
+            if (maximum == 0.0f) break;
+            if (bounce >= 1u) {
+                float survival = clampf_(maximum, 0.1f, 0.95f);
+                if (continuation_sample(stream, sequence, bounce, 2u) > survival) break;
+                throughput = throughput / survival;
+            }
+            origin = hit.position + hit.normal * 0.002f;
+            previous_position = hit.position;
+            previous_normal = hit.normal;
+            direction = hemisphere(hit.normal, continuation_sample(stream, sequence, bounce, 0u), continuation_sample(stream, sequence, bounce, 1u));
+            previous_pdf = cosine_pdf(hit.normal, direction);
+            if (previous_pdf <= 0.0f) break;
+            bounce += 1u;
+        }
+        return energy;
+    }
+    // voxel_path.wgsl:504-520
+    F3 trace_receiver(F3 surface, F3 normal, uint32_t stream, uint32_t sequence, bool &known, Hit *first) const {
+        if (cn) cn->paths++;
+        F3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));
+        F3 value = sample_emitter(surface, normal, stream, sequence, 0u);
+        bool unknown = false;
+        if (first) { first->voxel = I3{0, 0, 0}; first->face = 6u; first->status = 4u; first->material = 0; first->slot = NONE; }
+        if (dot3(normal, direction) > 0.0f) {
+            F3 c = trace_path(surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first);
+            value = value + c;
+        }
+        known = !unknown;
+        return value;
+    }
+};
+
+static uint32_t path_lanes(const orc_transport &tr) {
+    if (tr.lanes) return tr.lanes;
+    return tr.samples < 16u ? 4u : 8u;  // gpu.rs:1179-1181
+}
+
+// voxel_trace.wgsl:25-92 (trace_faces_parallel: lane l traces samples l, l+L, ...;
+// lane 0 sums the lanes' moments in lane order).
+static void trace_one_face(const Tracer &t, const uint32_t *task, float *out, uint32_t *hits, Counters *cn) {
+    Estimator est(t, cn);
+    I3 voxel{(int32_t)task[0], (int32_t)task[1], (int32_t)task[2]};
+    uint32_t face = task[3];
+    uint32_t sequence = task[4];
+    uint32_t spp = t.transport.samples;
+    if (face == NONE) return;
+    F3 normal = face_normal(face);
+    F3 surface = to_f3(voxel) + f3(0.5f, 0.5f, 0.5f) + normal * 0.5f;
+    uint32_t stream = hash_voxel(voxel) ^ pcg(face + 1u);
+    uint32_t L = path_lanes(t.transport);
+    F3 tmean = f3(0, 0, 0), tsecond = f3(0, 0, 0);
+    uint32_t tknown = 0;
+    for (uint32_t lane = 0; lane < L; ++lane) {
+        F3 mean = f3(0, 0, 0), second = f3(0, 0, 0);
+        uint32_t known = 0;
+        for (uint32_t i = lane; i < spp; i += L) {
+            Estimator::Hit h;
+            uint64_t rays_before = cn ? cn->rays : 0;
+            bool k;
+            F3 value = est.trace_receiver(surface, normal, stream, sequence + i, k, hits ? &h : nullptr);
+            if (hits) {
+                uint32_t *r = &hits[(size_t)i * 8];
+                r[0] = (uint32_t)h.voxel.x; r[1] = (uint32_t)h.voxel.y; r[2] = (uint32_t)h.voxel.z;
+                r[3] = h.face | (h.status << 8); r[4] = h.material; r[5] = h.slot;
+                r[6] = cn ? (uint32_t)(cn->rays - rays_before) : 0u; r[7] = k ? 1u : 0u;
+            }
+            if (!k) continue;
+            mean = mean + value;
+            second = second + value * value;
+            known += 1u;
+        }
+        tmean = tmean + mean;
+        tsecond = tsecond + second;
+        tknown += known;
+    }
+    // write_result (voxel_trace.wgsl:25-37): one more analytic direct estimate.
+    F3 dv, dc;
+    est.direct_at(to_f3(voxel) + f3(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc);
+    float inverse = 1.0f / (float)std::max(tknown, 1u);
+    F3 m = tmean * inverse, s = tsecond * inverse;
+    float rec[24] = {m.x, m.y, m.z, 0.0f, 0.0f, (float)tknown, 0.0f, 0.0f, s.x, s.y, s.z, 0.0f,
+                     0.0f, 0.0f, 0.0f, 0.0f, dv.x, dv.y, dv.z, 0.0f, 0.0f, dc.x, dc.y, dc.z};
+    memcpy(out, rec, sizeof(rec));
+}
+
+static void tracer_sync(Tracer &t) {
+    if (t.synced_revision != t.scene->revision || t.materials_changed) {
+        pack_scene(*t.scene, t.packed);
+        build_emission(*t.scene, t.emission_rgb, t.emission);
+        t.synced_revision = t.scene->revision;
+        t.materials_changed = false;
+    }
+}
+
+static void parallel_for(uint32_t count, int threads, const std::function<void(uint32_t, uint32_t, int)> &fn) {
+    if (threads <= 1 || count < 64) { fn(0, count, 0); return; }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> pool;
+    const uint32_t chunk = 16;
+    for (int w = 0; w < threads; ++w)
+        pool.emplace_back([&, w]() {
+            for (;;) {
+                uint32_t b = next.fetch_add(chunk);
+                if (b >= count) break;
+                fn(b, std::min(count, b + chunk), w);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+static int trace_faces(Tracer &t, const uint32_t *tasks, uint32_t count, float *results, uint32_t *hits, uint64_t *counters, int threads) {
+    tracer_sync(t);
+    if (threads < 1) threads = 1;
+    std::vector<Counters> per(threads);
+    uint32_t spp = t.transport.samples;
+    parallel_for(count, threads, [&](uint32_t b, uint32_t e, int w) {
+        for (uint32_t i = b; i < e; ++i)
+            trace_one_face(t, &tasks[(size_t)i * 8], &results[(size_t)i * 24], hits ? &hits[(size_t)i * spp * 8] : nullptr,
+                           counters ? &per[w] : nullptr);
+    });
+    if (counters) {
+        Counters s;
+        for (auto &c : per) {
+            s.rays += c.rays; s.steps += c.steps; s.region_loads += c.region_loads;
+            s.brick_loads += c.brick_loads; s.cell_loads += c.cell_loads; s.paths += c.paths;
+        }
+        counters[0] = s.rays; counters[1] = s.steps; counters[2] = s.region_loads;
+        counters[3] = s.brick_loads; counters[4] = s.cell_loads; counters[5] = s.paths;
+    }
+    return 0;
+}
+
+
+// ---------------------------------------------------------------- resident cache (voxel_accumulate.wgsl)
+constexpr uint32_t META_LIVE = 8u;             // resident.rs:21
+constexpr uint32_t URGENT = 0x40000000u;       // voxel_accumulate.wgsl:460
+
+static inline float max3f(F3 v) { return fmax_(v.x, fmax_(v.y, v.z)); }
+static inline F3 load3(const uint32_t *w) { return f3(bitsf(w[0]), bitsf(w[1]), bitsf(w[2])); }
+static inline void store4(uint32_t *w, F3 v, float a) { w[0] = fbits(v.x); w[1] = fbits(v.y); w[2] = fbits(v.z); w[3] = fbits(a); }
+
+// resident.rs:770-812 (allocate_slot): LIFO free list, then the high-water mark.
+static uint32_t allocate_slot(Tracer &t, I3 voxel, uint32_t face, uint32_t material) {
+    uint32_t index;
+    if (!t.free_slots.empty()) { index = t.free_slots.back(); t.free_slots.pop_back(); }
+    else index = t.high_water++;
+    t.face_index.insert(voxel, face, index);
+    if (t.state.size() < (size_t)(index + 1) * 36) {
+        t.state.resize((size_t)(index + 1) * 36, 0);
+        t.checkpoint.resize((size_t)(index + 1) * 12, 0);
+        t.meta.resize((size_t)(index + 1) * 12, 0);
+    }
+    uint32_t *m = &t.meta[(size_t)index * 12];
+    uint32_t words[12] = {(uint32_t)voxel.x, (uint32_t)voxel.y, (uint32_t)voxel.z, face | META_LIVE | (material << 16), 0, 0, 0, 0, 0, 0, 0, 0};
+    memcpy(m, words, sizeof(words));
+    // A new face's seed (resident.rs:1426-1470) from an empty FaceWork: zero
+    // statistics, sequence 0, no presentation yet.
+    memset(&t.state[(size_t)index * 36], 0, 36 * 4);
+    memset(&t.checkpoint[(size_t)index * 12], 0, 12 * 4);
+    return index;
+}
+
+// voxel_accumulate.wgsl:336-344
+static bool binds(const Tracer &t, uint32_t slot, float change) {
+    const uint32_t *st = &t.state[(size_t)slot * 36];
+    F3 w = f3(0.2126f, 0.7152f, 0.0722f);
+    F3 mean = load3(&st[0]);
+    F3 sec = load3(&st[8]);
+    F3 var = f3(fmax_(sec.x - mean.x * mean.x, 0.0f), fmax_(sec.y - mean.y * mean.y, 0.0f), fmax_(sec.z - mean.z * mean.z, 0.0f));
+    bool lit = dot3(w, mean) >= 0.05f * change;
+    float keep = t.measured_retention * dot3(w, var) / fmax_(change * change, 1e-30f);
+    return !lit || keep < (float)(st[5] & 0x7fffffffu);
+}
+
+// voxel_accumulate.wgsl:135-227 (record_stability and the settled flag belong
+// to GPU work selection and are not restated: the flag stays clear).
+static void accumulate_one(Tracer &t, uint32_t slot, float *result, uint32_t samples, uint32_t clock_ms) {
+    uint32_t *st = &t.state[(size_t)slot * 36];
+    uint32_t *cp = &t.checkpoint[(size_t)slot * 12];
+    uint32_t *out = (uint32_t *)result;
+    memcpy(&cp[0], &st[0], 16);
+    memcpy(&cp[4], &st[8], 16);
+    cp[8] = st[4]; cp[9] = st[5] & 0x7fffffffu; cp[10] = st[12]; cp[11] = 0;
+    uint32_t fresh_count = (uint32_t)bitsf(out[5]);
+    uint32_t &pending_word = t.meta[(size_t)slot * 12 + 4];
+    float pending = bitsf(pending_word);
+    if (fresh_count > 0u) pending_word = 0u;
+    uint32_t previous_count = st[5] & 0x7fffffffu;
+    float physical = t.measured_retention;
+    float change = (st[5] & 0x80000000u) != 0u ? pending : 0.0f;
+    if (physical > 0.0f && fresh_count > 0u && change > 0.0f) {
+        F3 w = f3(0.2126f, 0.7152f, 0.0722f);
+        F3 om = load3(&st[0]);
+        F3 os = load3(&st[8]);
+        F3 ov = f3(fmax_(os.x - om.x * om.x, 0.0f), fmax_(os.y - om.y * om.y, 0.0f), fmax_(os.z - om.z * om.z, 0.0f));
+        F3 fm = load3(&out[0]);
+        F3 fs = load3(&out[8]);
+        F3 fv = f3(fmax_(fs.x - fm.x * fm.x, 0.0f), fmax_(fs.y - fm.y * fm.y, 0.0f), fmax_(fs.z - fm.z * fm.z, 0.0f));
+        float variance = dot3(w, om) < 0.05f * change ? dot3(w, fv) : dot3(w, ov);
+        float keep = physical * variance / (change * change);
+        previous_count = std::min(previous_count, (uint32_t)fmin_(keep, 65536.0f));
+        cp[9] = std::min(cp[9], previous_count);
+    }
+    uint32_t count = std::min(previous_count + fresh_count, t.history_limit);
+    float alpha = fresh_count > 0u ? fmin_(t.recency * (float)fresh_count / (float)std::max(count, 1u), 1.0f) : 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float pm = bitsf(st[k]), m = bitsf(out[k]);
+        st[k] = fbits(pm + (m - pm) * alpha);
+        float ps = bitsf(st[8 + k]), s = bitsf(out[8 + k]);
+        st[8 + k] = fbits(ps + (s - ps) * alpha);
+    }
+    st[5] = count;
+    st[6] = st[6] + samples;
+    memcpy(&st[16], &out[16], 16);
+    memcpy(&st[20], &out[20], 16);
+    F3 radiance = load3(&st[0]);
+    F3 moment = load3(&st[8]);
+    float fc = fmax_((float)count, 1.0f);
+    F3 error = f3(std::sqrt(fmax_(moment.x - radiance.x * radiance.x, 0.0f) / fc),
+                  std::sqrt(fmax_(moment.y - radiance.y * radiance.y, 0.0f) / fc),
+                  std::sqrt(fmax_(moment.z - radiance.z * radiance.z, 0.0f) / fc));
+    float uncertainty = count < 2u ? 1.0f : fmin_(8.0f, max3f(f3(error.x / (0.02f + std::fabs(radiance.x)), error.y / (0.02f + std::fabs(radiance.y)), error.z / (0.02f + std::fabs(radiance.z)))));
+    float noise = count < 2u ? 1.0f : fmin_(8.0f, max3f(f3(error.x / (0.002f + std::fabs(radiance.x)), error.y / (0.002f + std::fabs(radiance.y)), error.z / (0.002f + std::fabs(radiance.z)))));
+    float elapsed = st[30] != 0u ? (float)(clock_ms - st[29]) * 0.001f : 1.0f / 60.0f;
+    float response = t.response;
+    float time_alpha = 1.0f - vgi_exp(-elapsed / response);
+    float evidence = (float)count / ((float)count + 32.0f);
+    float blend = fmin_(time_alpha, evidence / (1.0f + 16.0f * noise * noise));
+    F3 prev = load3(&st[24]);
+    float prev_a = bitsf(st[27]);
+    float old_weight = (1.0f - blend) / (0.25f + max3f(prev));
+    float new_weight = blend / (0.25f + max3f(radiance));
+    float alpha_display = fmin_(blend, new_weight / fmax_(old_weight + new_weight, 0.0000001f));
+    float confidence = clampf_((float)count / ((float)count + 8.0f) / (1.0f + uncertainty), 0.125f, 1.0f);
+    F3 disp = prev * (1.0f - alpha_display) + radiance * alpha_display;
+    float disp_a = prev_a * (1.0f - blend) + confidence * blend;
+    store4(&st[24], disp, disp_a);
+    st[29] = clock_ms;
+    st[30] = 0xffffffffu;
+    st[7] = st[24];
+    st[13] = st[25]; st[14] = st[26]; st[15] = st[27];
+    memcpy(out, st, 24 * 4);
+}
+
+static int cache_batch(Tracer &t, uint32_t *tasks, const uint32_t *slots, uint32_t count, float *results, uint32_t *hits,
+                       uint64_t *counters, int threads, uint32_t clock_ms) {
+    tracer_sync(t);
+    // prepare (voxel_accumulate.wgsl:83-133): each traced face takes its slot's sequence.
+    for (uint32_t i = 0; i < count; ++i) tasks[(size_t)i * 8 + 4] = t.state[(size_t)slots[i] * 36 + 6];
+    // mark_changed (voxel_accumulate.wgsl:432-446)
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t s = slots[i];
+        uint32_t &pw = t.meta[(size_t)s * 12 + 4];
+        float pending = bitsf(pw);
+        if (pending <= 0.0f) continue;
+        if (binds(t, s, pending)) t.state[(size_t)s * 36 + 5] |= 0x80000000u;
+        else pw = 0u;
+    }
+    trace_faces(t, tasks, count, results, hits, counters, threads);
+    for (uint32_t i = 0; i < count; ++i) accumulate_one(t, slots[i], &results[(size_t)i * 24], t.transport.samples, clock_ms);
+    return 0;
+}
+
+// voxel_retention.wgsl:14-18
+static inline uint32_t box_distance2(I3 lo, I3 hi, I3 p) {
+    uint32_t s[3];
+    for (int a = 0; a < 3; ++a) {
+        int v = std::max(std::max(lo[a] - p[a], p[a] - hi[a]), 0);
+        s[a] = (uint32_t)std::min(v, 4096);
+    }
+    return s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+}
+static inline uint32_t curve_limit(uint32_t d2) {
+    if (d2 >= 36864u) return NONE;
+    return (d2 * d2 + 10368u) / 20736u;
+}
+
+// voxel_retention.wgsl:29-96 + voxel_accumulate.wgsl:295-332 (apply_events)
+static void apply_events(Tracer &t, const uint32_t *events, uint32_t event_count, uint32_t reach, uint32_t urgent_measured,
+                         uint32_t urgent_curve) {
+    bool measured_mode = t.measured_retention > 0.0f;
+    for (uint32_t slot = 0; slot < t.high_water; ++slot) {
+        uint32_t *id = &t.meta[(size_t)slot * 12];
+        if ((id[3] & META_LIVE) == 0u) continue;
+        I3 p{(int32_t)id[0], (int32_t)id[1], (int32_t)id[2]};
+        uint32_t cap = NONE, source_cap = NONE;
+        float change = 0.0f;
+        bool known = true, woken = false, urgent_if_binding = false, urgent = false;
+        for (uint32_t e = 0; e < event_count; ++e) {
+            const uint32_t *h0 = &events[(e * 3u) * 4], *h1 = &events[(e * 3u + 1u) * 4], *h2 = &events[(e * 3u + 2u) * 4];
+            if ((h0[3] & 2u) != 0u) {
+                uint32_t limit = h2[0];
+                for (uint32_t b = 0; b < h2[3]; ++b) {
+                    const uint32_t *sp = &events[(event_count * 3u + (h2[2] + b) * 2u) * 4];
+                    F3 d = to_f3(p) - f3(bitsf(sp[0]), bitsf(sp[1]), bitsf(sp[2]));
+                    if (dot3(d, d) <= bitsf(sp[3])) limit = std::min(limit, 4u);
+                }
+                source_cap = std::min(source_cap, limit);
+                woken = true;
+                continue;
+            }
+            uint32_t outside = box_distance2(I3{(int32_t)h0[0], (int32_t)h0[1], (int32_t)h0[2]}, I3{(int32_t)h1[0], (int32_t)h1[1], (int32_t)h1[2]}, p);
+            if (outside > reach) continue;
+            woken = true;
+            bool exact = known;
+            uint32_t nearest = NONE;
+            for (uint32_t b = 0; b < h2[3]; ++b) {
+                uint32_t record = event_count * 3u + (h2[2] + b) * 2u;
+                const uint32_t *b0 = &events[record * 4], *b1 = &events[(record + 1u) * 4];
+                uint32_t d = box_distance2(I3{(int32_t)b0[0], (int32_t)b0[1], (int32_t)b0[2]}, I3{(int32_t)b1[0], (int32_t)b1[1], (int32_t)b1[2]}, p);
+                if (d > reach) continue;
+                nearest = std::min(nearest, d);
+                uint32_t edit_kind = b1[3] & 7u;
+                if ((b1[3] & 8u) != 0u && (edit_kind == 0u || edit_kind == 3u)) {
+                    if (edit_kind == 3u && measured_mode) urgent_if_binding = urgent_if_binding || d <= urgent_measured;
+                    else urgent = urgent || d <= urgent_curve;
+                }
+                if (exact) {
+                    uint32_t kind = b1[3] & 7u;
+                    if (kind == 0u) { known = false; exact = false; }
+                    else if (kind != 1u) {
+                        float area = 1.0f;
+                        if (kind == 2u) area = (float)std::min(std::max(((int32_t)b1[3] >> 3) + 1 - p.y, 0), 16);
+                        change = change + bitsf(b0[3]) * area / (3.14159265358979f * (float)std::max(d, 1u));
+                    }
+                }
+            }
+            if (nearest != NONE) cap = std::min(cap, curve_limit(nearest));
+        }
+        bool measured = measured_mode && known;
+        uint32_t limit = std::min(source_cap, measured ? NONE : cap);
+        uint32_t *st = &t.state[(size_t)slot * 36];
+        if (limit != NONE) {
+            st[5] = std::min(st[5], limit);
+            t.checkpoint[(size_t)slot * 12 + 9] = std::min(t.checkpoint[(size_t)slot * 12 + 9], limit);
+        }
+        if (measured && change > 0.0f) t.meta[(size_t)slot * 12 + 4] = fbits(bitsf(t.meta[(size_t)slot * 12 + 4]) + change);
+        if (urgent_if_binding) {
+            float pending = bitsf(t.meta[(size_t)slot * 12 + 4]);
+            urgent = urgent || !measured || (pending > 0.0f && binds(t, slot, pending));
+        }
+        if (woken || urgent) t.meta[(size_t)slot * 12 + 5] &= 0xfffffff0u;
+        if (urgent) t.meta[(size_t)slot * 12 + 11] |= URGENT;
+    }
+}
+}  // namespace
+
+// ================================================================ C API
+extern "C" {
+
+uint32_t orc_pcg(uint32_t v) { return pcg(v); }
+uint32_t orc_hash_voxel(int32_t x, int32_t y, int32_t z) { return hash_voxel(I3{x, y, z}); }
+uint32_t orc_stream_id(int32_t x, int32_t y, int32_t z, uint32_t face) { return stream_id(I3{x, y, z}, face); }
+float orc_sample_1d(uint32_t s, uint32_t q, uint32_t d) { return sample_1d(s, q, d); }
+float orc_sinf(float x) { return vgi_sin(x); }
+float orc_cosf(float x) { return vgi_cos(x); }
+float orc_expf(float x) { return vgi_exp(x); }
+void orc_hemisphere(const float n[3], float u, float v, float out[3]) {
+    F3 r = hemisphere(f3(n[0], n[1], n[2]), u, v);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_advance_box(const float position[3], const float direction[3], const int32_t cell[3], const int32_t negative[3],
+                     const int32_t positive[3], float size, float out[4]) {
+    F3 n;
+    float w;
+    advance_box(f3(position[0], position[1], position[2]), f3(direction[0], direction[1], direction[2]),
+                I3{cell[0], cell[1], cell[2]}, I3{negative[0], negative[1], negative[2]}, I3{positive[0], positive[1], positive[2]},
+                size, n, w);
+    out[0] = n.x; out[1] = n.y; out[2] = n.z; out[3] = w;
+}
+
+int orc_build_aadf(uint32_t sx, uint32_t sy, uint32_t sz, const uint8_t *blocked, uint8_t *out) {
+    if (std::max(std::max(sx, sy), sz) > 255u) return -1;
+    std::vector<Aadf> r = build_aadf((int)sx, (int)sy, (int)sz, blocked);
+    if (!r.empty()) memcpy(out, r.data(), r.size() * 6);
+    return 0;
+}
+// naadf/construction_tests.rs:6-62: the frozen full sweep, no closed forms.
+int orc_full_sweep_aadf(uint32_t sx, uint32_t sy, uint32_t sz, const uint8_t *blocked, uint8_t *out) {
+    if (std::max(std::max(sx, sy), sz) > 255u) return -1;
+    std::vector<Aadf> source((size_t)sx * sy * sz, ZERO_AADF);
+    sweep_aadf((int)sx, (int)sy, (int)sz, blocked, source);
+    if (!source.empty()) memcpy(out, source.data(), source.size() * 6);
+    return 0;
+}
+
+void *orc_scene_new(void) { return new Scene(); }
+void orc_scene_free(void *s) { delete (Scene *)s; }
+
+int orc_scene_load_pages(void *sp, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels, const int32_t *removed_xyz,
+                         uint32_t removed) {
+    Scene &s = *(Scene *)sp;
+    std::vector<std::pair<I3, const uint16_t *>> changed;
+    for (uint32_t i = 0; i < pages; ++i) {
+        I3 p{page_xyz[i * 3], page_xyz[i * 3 + 1], page_xyz[i * 3 + 2]};
+        auto &store = s.pages[key_of(p)];
+        store.assign(voxels + (size_t)i * 32768, voxels + (size_t)(i + 1) * 32768);
+        s.page_keys[key_of(p)] = p;
+    }
+    for (uint32_t i = 0; i < pages; ++i) {
+        I3 p{page_xyz[i * 3], page_xyz[i * 3 + 1], page_xyz[i * 3 + 2]};
+        changed.push_back({p, s.pages[key_of(p)].data()});
+    }
+    std::vector<I3> rem;
+    for (uint32_t i = 0; i < removed; ++i) rem.push_back(I3{removed_xyz[i * 3], removed_xyz[i * 3 + 1], removed_xyz[i * 3 + 2]});
+    s.apply_pages(changed, rem);
+    return 0;
+}
+
+int orc_scene_edit(void *sp, const int32_t *xyz, const uint16_t *values, uint32_t count) {
+    Scene &s = *(Scene *)sp;
+    std::map<std::tuple<int, int, int>, I3> edited;
+    for (uint32_t i = 0; i < count; ++i) {
+        I3 v{xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]};
+        I3 p{floordiv(v.x, 32), floordiv(v.y, 32), floordiv(v.z, 32)};
+        auto it = s.pages.find(key_of(p));
+        if (it == s.pages.end()) continue;  // edits outside loaded pages are dropped
+        it->second[lin(floormod(v.x, 32), floormod(v.y, 32), floormod(v.z, 32), 32, 32)] = values[i];
+        edited[std::make_tuple(p.z, p.y, p.x)] = p;
+    }
+    std::vector<std::pair<I3, const uint16_t *>> changed;
+    for (auto &kv : edited) changed.push_back({kv.second, s.pages[key_of(kv.second)].data()});
+    s.apply_pages(changed, {});
+    return 0;
+}
+
+int orc_scene_bounds(void *sp, int32_t out[6]) {
+    Scene &s = *(Scene *)sp;
+    I3 mn{0, 0, 0}, mx{0, 0, 0};
+    if (!s.bounds(mn, mx)) return 0;
+    out[0] = mn.x; out[1] = mn.y; out[2] = mn.z; out[3] = mx.x; out[4] = mx.y; out[5] = mx.z;
+    return 1;
+}
+
+int64_t orc_scene_top_level(void *sp, int32_t out_size[3], uint8_t *out) {
+    Scene &s = *(Scene *)sp;
+    out_size[0] = s.top.sx; out_size[1] = s.top.sy; out_size[2] = s.top.sz;
+    if (s.top.fields.empty()) return 0;
+    if (out) memcpy(out, s.top.fields.data(), s.top.fields.size() * 6);
+    return (int64_t)s.top.fields.size();
+}
+
+uint32_t orc_scene_material_at(void *sp, int32_t x, int32_t y, int32_t z) { return ((Scene *)sp)->material_at(I3{x, y, z}); }
+
+int orc_scene_region_solid(void *sp, int32_t rx, int32_t ry, int32_t rz) {
+    const Region *r = ((Scene *)sp)->region(I3{rx, ry, rz});
+    return r ? (r->any_solid ? 1 : 0) : -1;
+}
+
+int orc_scene_region_canonical(void *sp, int32_t rx, int32_t ry, int32_t rz, uint32_t *out) {
+    Scene &s = *(Scene *)sp;
+    memset(out, 0, (4 + 64 + 64 * 32) * 4);
+    const Region *r = s.region(I3{rx, ry, rz});
+    if (!r) return 0;
+    Aadf top = s.top.field(r->coord);
+    out[2] = pack_bytes(top.b[0], top.b[2], top.b[4]);
+    out[3] = pack_bytes(top.b[1], top.b[3], top.b[5]);
+    if (r->storage == ST_EMPTY) { out[0] = 1; out[1] = AIR; return 1; }
+    if (r->storage == ST_UNIFORM) { out[0] = 2; out[1] = r->uniform; return 1; }
+    out[0] = 3;
+    // Same construction as pack_scene, with the pointer replaced by tag 2 and
+    // the brick payload written at its brick ordinal.
+    std::vector<Aadf> bb = r->brick_bounds();
+    for (int index = 0; index < 64; ++index) {
+        int bx = index & 3, by = (index >> 2) & 3, bz = index >> 4;
+        uint16_t first = r->material(bx * 4, by * 4, bz * 4);
+        bool uni = true;
+        std::vector<Aadf> vb;
+        bool have = false;
+        uint32_t *data = &out[68 + index * 32];
+        for (int i = 0; i < 64; ++i) {
+            uint16_t m = r->material(bx * 4 + (i & 3), by * 4 + ((i >> 2) & 3), bz * 4 + (i >> 4));
+            uni = uni && m == first;
+            uint32_t cell;
+            if (m == AIR) {
+                if (!have) { vb = r->brick_voxel_bounds(bx, by, bz); have = true; }
+                cell = pack_extents(vb[i]) << 1;
+            } else cell = ((uint32_t)m << 1) | 1u;
+            data[i / 2] |= cell << ((i & 1) * 16);
+        }
+        if (uni) memset(data, 0, 32 * 4);
+        if (uni && first == AIR) out[4 + index] = pack_extents(bb[index]) << 2;
+        else if (uni) out[4 + index] = ((uint32_t)first << 2) | 1u;
+        else out[4 + index] = 2u;
+    }
+    return 1;
+}
+
+int64_t orc_scene_stats(void *sp, int64_t out[8]) {
+    Scene &s = *(Scene *)sp;
+    int64_t counts[4] = {0, 0, 0, 0};
+    int64_t mixed_bricks = 0;
+    for (auto &kv : s.regions) {
+        const Region *r = kv.second.get();
+        counts[r->storage]++;
+        if (r->storage == ST_DENSE)
+            for (int b = 0; b < 64; ++b) {
+                int bx = b & 3, by = (b >> 2) & 3, bz = b >> 4;
+                uint16_t first = r->material(bx * 4, by * 4, bz * 4);
+                for (int i = 1; i < 64; ++i)
+                    if (r->material(bx * 4 + (i & 3), by * 4 + ((i >> 2) & 3), bz * 4 + (i >> 4)) != first) { ++mixed_bricks; break; }
+            }
+    }
+    out[0] = (int64_t)s.regions.size();
+    for (int i = 0; i < 4; ++i) out[1 + i] = counts[i];
+    out[5] = mixed_bricks;
+    out[6] = (int64_t)s.top.fields.size();
+    out[7] = (int64_t)s.revision;
+    return out[0];
+}
+
+void *orc_tracer_new(void *scene) {
+    Tracer *t = new Tracer();
+    t->scene = (Scene *)scene;
+    for (int m = 0; m < 256; ++m) {  // world.rs:72-79 defaults
+        for (int c = 0; c < 3; ++c) { t->diffuse_rgb[m * 3 + c] = 0.45f; t->emission_rgb[m * 3 + c] = 0.0f; }
+        t->materials[m] = Material{f3(0.45f, 0.45f, 0.45f), f3(0, 0, 0), 0.0f};
+    }
+    t->transport.bounces = 3;
+    t->transport.samples = 16;
+    t->transport.max_distance = 768.0f;
+    t->transport.reach = 768.0f;
+    return t;
+}
+void orc_tracer_free(void *t) { delete (Tracer *)t; }
+
+int orc_tracer_set_materials(void *tp, const float *diffuse, const float *emission) {
+    Tracer &t = *(Tracer *)tp;
+    memcpy(t.diffuse_rgb, diffuse, sizeof(t.diffuse_rgb));
+    memcpy(t.emission_rgb, emission, sizeof(t.emission_rgb));
+    for (int m = 0; m < 256; ++m) {
+        t.materials[m].diffuse = f3(diffuse[m * 3], diffuse[m * 3 + 1], diffuse[m * 3 + 2]);
+        t.materials[m].emission = f3(emission[m * 3], emission[m * 3 + 1], emission[m * 3 + 2]);
+        t.materials[m].power = fmax_(fmax_(emission[m * 3], emission[m * 3 + 1]), emission[m * 3 + 2]);  // gpu.rs:719
+    }
+    t.materials_changed = true;
+    return 0;
+}
+
+int orc_tracer_set_lights(void *tp, const orc_light *lights, uint32_t count, uint32_t directional) {
+    Tracer &t = *(Tracer *)tp;
+    t.lights.resize(count);
+    for (uint32_t i = 0; i < count; ++i) memcpy(&t.lights[i], &lights[i], sizeof(Light));
+    t.directional = directional;
+    return 0;
+}
+
+int orc_tracer_set_transport(void *tp, const orc_transport *tr) {
+    Tracer &t = *(Tracer *)tp;
+    t.transport = *tr;
+    if (t.transport.max_distance <= 0.0f) t.transport.max_distance = 768.0f;
+    return 0;
+}
+
+int orc_tracer_sync(void *tp) { tracer_sync(*(Tracer *)tp); return 0; }
+
+int64_t orc_tracer_emitters(void *tp, uint32_t *out, int64_t capacity) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    int64_t n = (int64_t)t.emission.words.size();
+    if (out && capacity >= n) memcpy(out, t.emission.words.data(), (size_t)n * 4);
+    return n;
+}
+
+int orc_trace_faces(void *tp, const uint32_t *tasks, uint32_t count, float *results, uint32_t *hits, uint64_t *counters,
+                    int threads) {
+    return trace_faces(*(Tracer *)tp, tasks, count, results, hits, counters, threads);
+}
+
+
+int orc_cache_enable(void *tp, uint32_t history_limit, float recency, float measured_retention) {
+    Tracer &t = *(Tracer *)tp;
+    t.cache_on = true;
+    t.history_limit = history_limit == 0 ? 65536u : std::min(history_limit, 65536u);  // resident.rs:715-719
+    t.recency = recency >= 1.0f ? recency : 1.0f;
+    t.measured_retention = fmax_(measured_retention, 0.0f);
+    return 0;
+}
+
+int orc_cache_register(void *tp, const uint32_t *tasks, uint32_t count, uint32_t *slots) {
+    Tracer &t = *(Tracer *)tp;
+    for (uint32_t i = 0; i < count; ++i) {
+        I3 v{(int32_t)tasks[i * 8], (int32_t)tasks[i * 8 + 1], (int32_t)tasks[i * 8 + 2]};
+        uint32_t face = tasks[i * 8 + 3];
+        long f = t.face_index.find(v, face);
+        if (f >= 0) { slots[i] = t.face_index.entries[f][3] & 0x1fffffffu; continue; }
+        uint32_t m = t.scene->material_at(v);
+        slots[i] = allocate_slot(t, v, face, m == 0xffffu || m == AIR ? 0u : m);
+    }
+    return 0;
+}
+
+int orc_cache_batch(void *tp, uint32_t *tasks, const uint32_t *slots, uint32_t count, float *results, uint32_t *hits,
+                    uint64_t *counters, int threads, uint32_t clock_ms) {
+    return cache_batch(*(Tracer *)tp, tasks, slots, count, results, hits, counters, threads, clock_ms);
+}
+
+int orc_cache_state(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)(first + count) * 36 > t.state.size()) return -1;
+    memcpy(out, &t.state[(size_t)first * 36], (size_t)count * 36 * 4);
+    return 0;
+}
+
+int orc_cache_apply_events(void *tp, const uint32_t *events, uint32_t event_count, uint32_t words) {
+    (void)words;
+    apply_events(*(Tracer *)tp, events, event_count, 192u * 192u, 64u * 64u, 48u * 48u);
+    return 0;
+}
+
+uint32_t orc_curve_limit(uint32_t d2) { return curve_limit(d2); }
+
+// One trace_domain_limited call (voxel_traversal.wgsl:126-219, visible_geometry
+// false, 2048 steps): out_i = voxel xyz, material, status, face; out_f = position, normal.
+int orc_trace_ray(void *tp, const float origin[3], const float direction[3], float max_distance, int32_t *out_i, float *out_f) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    RayResult r = trace_domain_limited(t.packed, f3(origin[0], origin[1], origin[2]), f3(direction[0], direction[1], direction[2]),
+                                       max_distance, false, 2048u, nullptr);
+    out_i[0] = r.voxel.x; out_i[1] = r.voxel.y; out_i[2] = r.voxel.z; out_i[3] = (int32_t)r.material;
+    out_i[4] = (int32_t)r.status; out_i[5] = (int32_t)face_index_of(r.normal);
+    out_f[0] = r.position.x; out_f[1] = r.position.y; out_f[2] = r.position.z;
+    out_f[3] = r.normal.x; out_f[4] = r.normal.y; out_f[5] = r.normal.z;
+    return 0;
+}
+
+}  // extern "C"

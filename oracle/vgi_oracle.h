@@ -1,0 +1,134 @@
+/*
+ * CPU oracle for the voxel-GI lighting hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker, never the product: only tests/, the smoke
+ * entry and the bench's cpu_baseline leg may load it. It restates, in scalar
+ * C++ with IEEE single precision (-ffp-contract=off, no fast-math), the
+ * reference passes of bevy_voxel_gi (crates/bevy_voxel_gi/...):
+ *   naadf.rs, world.rs (NaadfRegion, TopLevelAadf, Growth, cut_boxes),
+ *   gpu_scene.rs (directory, headers, cells), emission.rs + emission/tree.rs,
+ *   sampling.rs, shaders/voxel_traversal.wgsl, voxel_path.wgsl,
+ *   voxel_trace.wgsl, voxel_accumulate.wgsl, voxel_retention.wgsl,
+ *   gpu/face_index.rs, gpu/resident.rs (slot allocation).
+ * Each function cites the file:line it follows.
+ *
+ * Transcendentals: the reference leaves cos/sin/exp to the shader compiler
+ * and driver (not correctly rounded anywhere). The oracle and the HIP kernels
+ * both use this project's own sin/cos/exp (Cody-Waite reduction + Taylor
+ * polynomials, see vgi_oracle.cpp), so both sides are bit-identical to each
+ * other; exactness against a driver's own cos/sin is not claimed.
+ */
+#ifndef VGI_ORACLE_H
+#define VGI_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- hashes and samplers (sampling.rs:5-13, voxel_path.wgsl:178-238) ---- */
+uint32_t orc_pcg(uint32_t value);
+uint32_t orc_hash_voxel(int32_t x, int32_t y, int32_t z);
+uint32_t orc_stream_id(int32_t x, int32_t y, int32_t z, uint32_t face);
+float orc_sample_1d(uint32_t stream, uint32_t sequence, uint32_t dimension);
+float orc_sinf(float x);
+float orc_cosf(float x);
+float orc_expf(float x);
+void orc_hemisphere(const float n[3], float u, float v, float out[3]);
+void orc_advance_box(const float position[3], const float direction[3], const int32_t cell[3],
+                     const int32_t negative[3], const int32_t positive[3], float size,
+                     float out[4]);
+
+/* ---- NAADF (naadf.rs:37-148, naadf/construction_tests.rs:6-95) ---- */
+/* blocked: size.x*size.y*size.z bytes (x fastest); out: 6 bytes per cell. */
+int orc_build_aadf(uint32_t sx, uint32_t sy, uint32_t sz, const uint8_t *blocked, uint8_t *out);
+int orc_full_sweep_aadf(uint32_t sx, uint32_t sy, uint32_t sz, const uint8_t *blocked,
+                        uint8_t *out);
+
+/* ---- scene (world.rs, gpu_scene.rs) ---- */
+void *orc_scene_new(void);
+void orc_scene_free(void *scene);
+/* pages: 32^3 u16 voxels each (x fastest), AIR = 256. */
+int orc_scene_load_pages(void *scene, const int32_t *page_xyz, uint32_t pages,
+                         const uint16_t *voxels, const int32_t *removed_xyz, uint32_t removed);
+/* Voxel edits: each edit sets one voxel (value AIR clears). Edited pages are
+ * rebuilt as the reference's page overlay does (pages.rs:498). */
+int orc_scene_edit(void *scene, const int32_t *xyz, const uint16_t *values, uint32_t count);
+/* Region bounds (min xyz, max xyz) of loaded regions; returns 0 if empty. */
+int orc_scene_bounds(void *scene, int32_t out[6]);
+/* The top-level field: size xyz in out_size, 6 bytes per cell in out (may be
+ * NULL to query the size). Returns cell count, 0 when the field is disabled. */
+int64_t orc_scene_top_level(void *scene, int32_t out_size[3], uint8_t *out);
+/* 0xffff outside the loaded scene, AIR (256) for air. */
+uint32_t orc_scene_material_at(void *scene, int32_t x, int32_t y, int32_t z);
+/* Canonical (layout-free) packed form of one region, as the traversal sees it:
+ * out[0]=kind (0 unloaded,1 empty,2 uniform,3 mixed), out[1]=material,
+ * out[2]=top-level negative bytes, out[3]=top-level positive bytes,
+ * out[4..68)=brick words with the pointer replaced by brick ordinal tag 2,
+ * out[68..68+64*32)=32 cell words per brick (zero for non-mixed bricks). */
+int orc_scene_region_canonical(void *scene, int32_t rx, int32_t ry, int32_t rz, uint32_t *out);
+/* Region any_solid flag (world.rs:183), -1 when not loaded. */
+int orc_scene_region_solid(void *scene, int32_t rx, int32_t ry, int32_t rz);
+/* Number of loaded regions / packed statistics for the bench. */
+int64_t orc_scene_stats(void *scene, int64_t out[8]);
+
+/* ---- transport (voxel_path.wgsl, voxel_trace.wgsl) ---- */
+typedef struct orc_light {
+    float position_range[4]; /* xyz direction toward the light (w < 0) or position, w range */
+    float energy[4];
+    float extra[4]; /* (0, custom visible rgb) */
+} orc_light;
+
+typedef struct orc_transport {
+    uint32_t bounces;      /* DIFFUSE_HITS (1..8) */
+    uint32_t samples;      /* samples per face per dispatch */
+    uint32_t lanes;        /* PATH_LANES: 0 = the reference's choice (4 if spp<16 else 8) */
+    uint32_t limited_reach;/* LIMITED_REACH */
+    float reach;           /* transport.x when limited */
+    float max_distance;    /* extra.y, MAX_PATH_DISTANCE 768 */
+    float environment[3];
+} orc_transport;
+
+void *orc_tracer_new(void *scene);
+void orc_tracer_free(void *tracer);
+/* 256 materials: diffuse rgb, emission rgb (already bounded by the caller). */
+int orc_tracer_set_materials(void *tracer, const float *diffuse, const float *emission);
+int orc_tracer_set_lights(void *tracer, const orc_light *lights, uint32_t count,
+                          uint32_t directional);
+int orc_tracer_set_transport(void *tracer, const orc_transport *transport);
+/* Rebuild packed scene + emitter distribution after scene changes. */
+int orc_tracer_sync(void *tracer);
+/* Emitter table: faces, tree nodes. */
+int64_t orc_tracer_emitters(void *tracer, uint32_t *out_words, int64_t capacity);
+/* tasks: 8 u32 per face (voxel xyz, face, sequence, 0,0,0).
+ * results: 24 floats per face (voxel_trace.wgsl:25-37 layout).
+ * hits (optional): per face-sample 8 u32 (first-hit voxel xyz, face|status<<8,
+ *   material, cache slot, rays traced by this sample, 0).
+ * counters (optional): [rays, steps, region loads, brick loads, cell loads, paths, 0, 0]. */
+int orc_trace_faces(void *tracer, const uint32_t *tasks, uint32_t count, float *results,
+                    uint32_t *hits, uint64_t *counters, int threads);
+
+/* ---- resident world cache (voxel_accumulate.wgsl, face_index.rs, resident.rs) ---- */
+int orc_cache_enable(void *tracer, uint32_t history_limit, float recency, float measured_retention);
+/* Registers faces (allocating slots as resident.rs:770-812) and writes their slots. */
+int orc_cache_register(void *tracer, const uint32_t *tasks, uint32_t count, uint32_t *slots);
+/* One batch: sequence from each slot's state, trace, accumulate. results as above
+ * (the accumulated state's first six records after the merge). */
+int orc_cache_batch(void *tracer, uint32_t *tasks, const uint32_t *slots, uint32_t count,
+                    float *results, uint32_t *hits, uint64_t *counters, int threads,
+                    uint32_t clock_ms);
+/* Raw state words (9 uvec4 per slot) of `count` slots from `first`. */
+int orc_cache_state(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
+/* Retention events: see voxel_retention.wgsl:29-96 for the layout. */
+int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_count,
+                           uint32_t words);
+/* voxel_retention.wgsl:9-12 */
+uint32_t orc_curve_limit(uint32_t d2);
+/* One traversal (voxel_traversal.wgsl:126-219): out_i = voxel xyz, material,
+ * status, face; out_f = position xyz, normal xyz. */
+int orc_trace_ray(void *tracer, const float origin[3], const float direction[3], float max_distance, int32_t *out_i,
+                  float *out_f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) for tests and the
+bench's CPU baseline. The oracle is the checker, never the product."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+class OrcTransport(ctypes.Structure):
+    _fields_ = [
+        ("bounces", ctypes.c_uint32),
+        ("samples", ctypes.c_uint32),
+        ("lanes", ctypes.c_uint32),
+        ("limited_reach", ctypes.c_uint32),
+        ("reach", ctypes.c_float),
+        ("max_distance", ctypes.c_float),
+        ("environment", ctypes.c_float * 3),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_LIB):
+        build()
+    L = ctypes.CDLL(ORACLE_LIB)
+    P = ctypes.c_void_p
+    u32, i32, u64, i64, f32 = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_float
+    sig = {
+        "orc_pcg": (u32, [u32]),
+        "orc_hash_voxel": (u32, [i32, i32, i32]),
+        "orc_stream_id": (u32, [i32, i32, i32, u32]),
+        "orc_sample_1d": (f32, [u32, u32, u32]),
+        "orc_sinf": (f32, [f32]),
+        "orc_cosf": (f32, [f32]),
+        "orc_expf": (f32, [f32]),
+        "orc_hemisphere": (None, [P, f32, f32, P]),
+        "orc_advance_box": (None, [P, P, P, P, P, f32, P]),
+        "orc_build_aadf": (ctypes.c_int, [u32, u32, u32, P, P]),
+        "orc_full_sweep_aadf": (ctypes.c_int, [u32, u32, u32, P, P]),
+        "orc_scene_new": (P, []),
+        "orc_scene_free": (None, [P]),
+        "orc_scene_load_pages": (ctypes.c_int, [P, P, u32, P, P, u32]),
+        "orc_scene_edit": (ctypes.c_int, [P, P, P, u32]),
+        "orc_scene_bounds": (ctypes.c_int, [P, P]),
+        "orc_scene_top_level": (i64, [P, P, P]),
+        "orc_scene_material_at": (u32, [P, i32, i32, i32]),
+        "orc_scene_region_canonical": (ctypes.c_int, [P, i32, i32, i32, P]),
+        "orc_scene_region_solid": (ctypes.c_int, [P, i32, i32, i32]),
+        "orc_scene_stats": (i64, [P, P]),
+        "orc_tracer_new": (P, [P]),
+        "orc_tracer_free": (None, [P]),
+        "orc_tracer_set_materials": (ctypes.c_int, [P, P, P]),
+        "orc_tracer_set_lights": (ctypes.c_int, [P, P, u32, u32]),
+        "orc_tracer_set_transport": (ctypes.c_int, [P, ctypes.POINTER(OrcTransport)]),
+        "orc_tracer_sync": (ctypes.c_int, [P]),
+        "orc_tracer_emitters": (i64, [P, P, i64]),
+        "orc_trace_faces": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
+        "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32]),
+        "orc_cache_register": (ctypes.c_int, [P, P, u32, P]),
+        "orc_cache_batch": (ctypes.c_int, [P, P, P, u32, P, P, P, ctypes.c_int, u32]),
+        "orc_cache_state": (ctypes.c_int, [P, u32, u32, P]),
+        "orc_cache_apply_events": (ctypes.c_int, [P, P, u32, u32]),
+        "orc_curve_limit": (u32, [u32]),
+        "orc_trace_ray": (ctypes.c_int, [P, P, P, f32, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def build_aadf(blocked, full_sweep=False):
+    """blocked: bool array (z, y, x). Returns uint8 (z, y, x, 6)."""
+    blocked = np.ascontiguousarray(blocked, np.uint8)
+    sz, sy, sx = blocked.shape
+    out = np.zeros((sz, sy, sx, 6), np.uint8)
+    fn = lib().orc_full_sweep_aadf if full_sweep else lib().orc_build_aadf
+    assert fn(sx, sy, sz, ptr(blocked), ptr(out)) == 0
+    return out
+
+
+class Scene:
+    """An oracle scene built from 32^3 pages."""
+
+    def __init__(self):
+        self.h = lib().orc_scene_new()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def load_pages(self, coords, pages, removed=None):
+        coords = np.ascontiguousarray(coords, np.int32)
+        pages = np.ascontiguousarray(pages, np.uint16)
+        removed = np.zeros((0, 3), np.int32) if removed is None else np.ascontiguousarray(removed, np.int32)
+        assert lib().orc_scene_load_pages(self.h, ptr(coords), len(coords), ptr(pages), ptr(removed), len(removed)) == 0
+
+    def edit(self, xyz, values):
+        xyz = np.ascontiguousarray(xyz, np.int32)
+        values = np.ascontiguousarray(values, np.uint16)
+        assert lib().orc_scene_edit(self.h, ptr(xyz), ptr(values), len(values)) == 0
+
+    def bounds(self):
+        out = np.zeros(6, np.int32)
+        return out if lib().orc_scene_bounds(self.h, ptr(out)) else None
+
+    def top_level(self):
+        size = np.zeros(3, np.int32)
+        n = lib().orc_scene_top_level(self.h, ptr(size), None)
+        if n == 0:
+            return None, size
+        out = np.zeros((size[2], size[1], size[0], 6), np.uint8)
+        lib().orc_scene_top_level(self.h, ptr(size), ptr(out))
+        return out, size
+
+    def region_canonical(self, rx, ry, rz):
+        out = np.zeros(4 + 64 + 64 * 32, np.uint32)
+        lib().orc_scene_region_canonical(self.h, rx, ry, rz, ptr(out))
+        return out
+
+    def material_at(self, x, y, z):
+        return lib().orc_scene_material_at(self.h, x, y, z)
+
+
+class Tracer:
+    def __init__(self, scene):
+        self.scene = scene
+        self.h = lib().orc_tracer_new(scene.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tracer_free(self.h)
+            self.h = None
+
+    def set_materials(self, diffuse, emission):
+        d = np.ascontiguousarray(diffuse, np.float32)
+        e = np.ascontiguousarray(emission, np.float32)
+        lib().orc_tracer_set_materials(self.h, ptr(d), ptr(e))
+
+    def set_lights(self, lights):
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 12)
+        directional = int((lights[:, 3] < 0).sum()) if len(lights) else 0
+        lib().orc_tracer_set_lights(self.h, ptr(lights) if len(lights) else None, len(lights), directional)
+
+    def set_transport(self, bounces=3, samples=16, lanes=0, environment=(0.0, 0.0, 0.0), limited_reach=False, reach=768.0,
+                      max_distance=768.0):
+        t = OrcTransport()
+        t.bounces, t.samples, t.lanes, t.limited_reach = bounces, samples, lanes, int(limited_reach)
+        t.reach, t.max_distance = reach, max_distance
+        for i in range(3):
+            t.environment[i] = environment[i]
+        self.samples = samples
+        lib().orc_tracer_set_transport(self.h, ctypes.byref(t))
+
+    def emitter_words(self):
+        n = lib().orc_tracer_emitters(self.h, None, 0)
+        out = np.zeros(n, np.uint32)
+        lib().orc_tracer_emitters(self.h, ptr(out), n)
+        return out
+
+    def trace_faces(self, tasks, hits=False, counters=False, threads=8):
+        tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
+        n = len(tasks)
+        results = np.zeros((n, 24), np.float32)
+        h = np.zeros((n * self.samples, 8), np.uint32) if hits else None
+        c = np.zeros(8, np.uint64) if counters else None
+        lib().orc_trace_faces(self.h, ptr(tasks), n, ptr(results), ptr(h), ptr(c), threads)
+        return results, h, c
+
+    def trace_ray(self, origin, direction, max_distance=768.0):
+        o = np.asarray(origin, np.float32)
+        d = np.asarray(direction, np.float32)
+        oi = np.zeros(6, np.int32)
+        of = np.zeros(6, np.float32)
+        lib().orc_trace_ray(self.h, ptr(o), ptr(d), max_distance, ptr(oi), ptr(of))
+        return oi, of
+
+    # resident cache
+    def cache_enable(self, history_limit=0, recency=1.0, measured_retention=1.0):
+        lib().orc_cache_enable(self.h, history_limit, recency, measured_retention)
+
+    def cache_register(self, tasks):
+        tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
+        slots = np.zeros(len(tasks), np.uint32)
+        lib().orc_cache_register(self.h, ptr(tasks), len(tasks), ptr(slots))
+        return slots
+
+    def cache_batch(self, tasks, slots, hits=False, threads=8, clock_ms=0):
+        tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
+        slots = np.ascontiguousarray(slots, np.uint32)
+        n = len(tasks)
+        results = np.zeros((n, 24), np.float32)
+        h = np.zeros((n * self.samples, 8), np.uint32) if hits else None
+        lib().orc_cache_batch(self.h, ptr(tasks), ptr(slots), n, ptr(results), ptr(h), None, threads, clock_ms)
+        return results, h
+
+    def cache_state(self, first, count):
+        out = np.zeros((count, 36), np.uint32)
+        assert lib().orc_cache_state(self.h, first, count, ptr(out)) == 0
+        return out
+
+    def apply_events(self, events, event_count):
+        events = np.ascontiguousarray(events, np.uint32).reshape(-1)
+        lib().orc_cache_apply_events(self.h, ptr(events), event_count, events.size)

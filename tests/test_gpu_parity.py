@@ -1,0 +1,89 @@
+"""HIP path vs the CPU oracle on the same seeded inputs.
+
+Bar: hit voxel, face, status, material, cache slot and per-sample ray counts
+bit-exact; per-face moments identical to 1e-4 relative (the kernels and the
+oracle evaluate the same IEEE operations in the same order, so they are
+expected and checked to be bit-identical first, with the tolerance only as
+the stated bound).
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_tracer, make_scene
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-4
+
+
+def compare_results(gpu, cpu, rel=REL):
+    assert gpu.shape == cpu.shape
+    bad = ~np.isclose(gpu, cpu, rtol=rel, atol=1e-6)
+    assert not bad.any(), "first mismatches: %s" % (np.argwhere(bad)[:5].tolist(),)
+
+
+def compare_hits(gpu, cpu):
+    assert gpu.shape == cpu.shape
+    diff = np.any(gpu != cpu, axis=1)
+    assert not diff.any(), "hit mismatch at samples %s: gpu %s cpu %s" % (
+        np.nonzero(diff)[0][:3].tolist(), gpu[diff][:3].tolist(), cpu[diff][:3].tolist())
+
+
+def trace_both(vgi, setup, tasks, bounces, samples, lanes=0):
+    print("parity: %d faces, bounces=%d, samples=%d" % (len(tasks), bounces, samples), flush=True)
+    t = setup["tracer"]
+    t.set_transport(bounces=bounces, samples=samples, lanes=lanes, environment=tuple(setup["env"]))
+    cpu, cpu_hits, cpu_counters = t.trace_faces(tasks, hits=True, counters=True)
+    g = gpu_tracer(vgi, setup)
+    g.set_transport(bounces=bounces, samples=samples, lanes=lanes, environment=tuple(setup["env"]))
+    ticket = g.submit_compact(tasks, vgi.RECORD_HITS | vgi.COUNT_RAYS)
+    gpu = g.complete_readback(ticket, len(tasks))
+    gpu_hits = g.read_hits(ticket, len(tasks), samples)
+    counters = g.read_counters(ticket)
+    g.close()
+    return gpu, cpu, gpu_hits, cpu_hits, counters, cpu_counters
+
+
+def test_box_one_bounce_bit_exact(vgi, orc):
+    setup = make_scene(vgi, orc, 1, 0x5EED0001)
+    tasks = vgi.exposed_faces(setup["grid"], interior_only=True)
+    assert 22000 < len(tasks) < 24000
+    gpu, cpu, gh, ch, gc, cc = trace_both(vgi, setup, tasks, bounces=1, samples=1)
+    compare_hits(gh, ch)
+    assert int(gc[0]) == int(cc[0]), "ray counts differ"
+    assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), "not bit-identical"
+    compare_results(gpu, cpu)
+
+
+def test_noise_multibounce_tree_bit_exact(vgi, orc):
+    setup = make_scene(vgi, orc, 3, 0x5EED0003, edge=64)
+    tasks = vgi.exposed_faces(setup["grid"])[::7]
+    words = setup["tracer"].emitter_words()
+    gpu, cpu, gh, ch, gc, cc = trace_both(vgi, setup, tasks, bounces=3, samples=4)
+    compare_hits(gh, ch)
+    assert int(gc[0]) == int(cc[0])
+    assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), "not bit-identical"
+    assert len(words) > 8 * 32  # light tree in use (>= 32 emissive faces)
+
+
+def test_emitter_table_matches(vgi, orc):
+    setup = make_scene(vgi, orc, 3, 0x5EED0003, edge=64)
+    g = gpu_tracer(vgi, setup)
+    assert np.array_equal(g.emitter_words(), setup["tracer"].emitter_words())
+    g.close()
+
+
+@pytest.mark.parametrize("workload,edge", [(1, 0), (3, 64), (3, 128), (6, 0)])
+def test_naadf_build_matches(vgi, orc, workload, edge):
+    print("naadf: workload %d edge %d" % (workload, edge), flush=True)
+    setup = make_scene(vgi, orc, workload, 0x5EED0000 + workload, edge)
+    g = gpu_tracer(vgi, setup)
+    b = setup["scene"].bounds()
+    assert np.array_equal(g.bounds(), b)
+    for rz in range(b[2], b[5] + 1):
+        for ry in range(b[1], b[4] + 1):
+            for rx in range(b[0], b[3] + 1):
+                a = g.region_canonical(rx, ry, rz)
+                c = setup["scene"].region_canonical(rx, ry, rz)
+                assert np.array_equal(a, c), "region %s: kind %d/%d" % ((rx, ry, rz), a[0], c[0])
+    g.close()
