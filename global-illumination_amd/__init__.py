@@ -48,6 +48,7 @@ class Transport(ctypes.Structure):
         ("history_limit", ctypes.c_uint32),
         ("history_recency", ctypes.c_float),
         ("measured_retention", ctypes.c_float),
+        ("plain", ctypes.c_uint32),
     ]
 
 
@@ -68,6 +69,7 @@ def library():
         "vgi_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
         "vgi_destroy": (None, [P]),
         "vgi_enable_resident": (ctypes.c_int, [P]),
+        "vgi_set_clock": (ctypes.c_int, [P, i64]),
         "vgi_upload_pages": (ctypes.c_int, [P, P, u32, P, P, u32]),
         "vgi_edit_voxels": (ctypes.c_int, [P, P, P, u32]),
         "vgi_set_materials": (ctypes.c_int, [P, P, P]),
@@ -87,6 +89,7 @@ def library():
         "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
+        "vgi_probe_copy": (ctypes.c_int, [P, u64, u32, P]),
         "vgi_scene_bounds": (ctypes.c_int, [P, P]),
         "vgi_scene_region_canonical": (ctypes.c_int, [P, i32, i32, i32, P]),
         "vgi_scene_stats": (i64, [P, P]),
@@ -107,11 +110,11 @@ def library():
 def exported_symbols():
     """Names the C ABI declares (include/vgi.h)."""
     return [
-        "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_upload_pages", "vgi_edit_voxels",
+        "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_set_clock", "vgi_upload_pages", "vgi_edit_voxels",
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_read_bound",
-        "vgi_cache_state", "vgi_apply_events", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces",
     ]
@@ -201,6 +204,9 @@ class GpuTracer:
     def enable_resident(self):
         _check(self._lib.vgi_enable_resident(self._h))
 
+    def set_clock(self, milliseconds):
+        _check(self._lib.vgi_set_clock(self._h, milliseconds))
+
     # ---- scene input (VoxelGiInput page_loaded / page_unloaded / edit_voxel)
     def upload_pages(self, page_xyz, voxels, removed=None):
         page_xyz = np.ascontiguousarray(page_xyz, np.int32).reshape(-1, 3)
@@ -230,13 +236,15 @@ class GpuTracer:
         _check(self._lib.vgi_set_lights(self._h, _ptr(lights) if len(lights) else None, len(lights), directional))
 
     def set_transport(self, bounces=3, samples=16, lanes=0, environment=(0.0, 0.0, 0.0), history_limit=0,
-                      history_recency=1.0, measured_retention=1.0, limited_reach=False, reach=768.0, max_distance=768.0):
+                      history_recency=1.0, measured_retention=1.0, limited_reach=False, reach=768.0, max_distance=768.0,
+                      plain=False):
         t = Transport()
         t.bounces, t.samples, t.lanes, t.limited_reach = bounces, samples, lanes, int(limited_reach)
         t.reach, t.max_distance = reach, max_distance
         for i in range(3):
             t.environment[i] = environment[i]
         t.history_limit, t.history_recency, t.measured_retention = history_limit, history_recency, measured_retention
+        t.plain = int(plain)
         self.transport = t
         _check(self._lib.vgi_set_transport(self._h, ctypes.byref(t)))
 
@@ -295,9 +303,14 @@ class GpuTracer:
 
     def read_bound(self):
         out = np.zeros((self._bound, 24), np.float32)
-        counters = np.zeros(2, np.uint64)
+        counters = np.zeros(5, np.uint64)
         _check(self._lib.vgi_read_bound(self._h, _ptr(out), self._bound, _ptr(counters)))
         return out, counters
+
+    def read_counters_bound(self):
+        counters = np.zeros(5, np.uint64)
+        _check(self._lib.vgi_read_bound(self._h, None, 0, _ptr(counters)))
+        return counters
 
     def cache_state(self, first, count):
         out = np.zeros((count, 36), np.uint32)
@@ -307,6 +320,11 @@ class GpuTracer:
     def apply_events(self, events, event_count):
         events = np.ascontiguousarray(events, np.uint32).reshape(-1)
         _check(self._lib.vgi_apply_events(self._h, _ptr(events), event_count, events.size))
+
+    def probe_copy(self, nbytes, repeats=5):
+        ms = np.zeros(1, np.float32)
+        _check(self._lib.vgi_probe_copy(self._h, nbytes, repeats, _ptr(ms)))
+        return float(ms[0])
 
     # ---- inspection
     def bounds(self):

@@ -119,6 +119,7 @@ struct vgi_tracer {
     size_t slot_capacity = 0;
     std::vector<uint32_t> last_slots;
     std::chrono::steady_clock::time_point origin = std::chrono::steady_clock::now();
+    int64_t fixed_clock = -1;  // resident.rs:709-710 test clock; < 0: wall clock
     // ---- batches
     DeviceBuffer d_tasks, d_slots;
     std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
@@ -526,7 +527,7 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.cache_index = t.d_index.as<uint4>();
     o.cache_state = t.d_state.as<uint32_t>();
     o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
-    o.cache_on = t.resident ? 1u : 0u;
+    o.cache_on = (t.resident && !t.transport.plain) ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------- resident slots
@@ -604,6 +605,7 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
 }
 
 uint32_t clock_ms(const vgi_tracer &t) {
+    if (t.fixed_clock >= 0) return (uint32_t)t.fixed_clock;
     return (uint32_t)std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t.origin).count();
 }
 
@@ -666,6 +668,12 @@ void vgi_destroy(vgi_tracer *t) {
     for (auto &e : t->ev) hipEventDestroy(e);
     hipStreamDestroy(t->stream);
     delete t;
+}
+
+int vgi_set_clock(vgi_tracer *t, int64_t milliseconds) {
+    if (!t) return fail("null tracer");
+    t->fixed_clock = milliseconds;
+    return 0;
 }
 
 int vgi_enable_resident(vgi_tracer *t) {
@@ -922,11 +930,11 @@ int vgi_sync(vgi_tracer *t) {
 float vgi_last_trace_ms(vgi_tracer *t) { return t ? t->last_trace_ms : 0.0f; }
 
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
-    if (!t || !out) return fail("null argument");
+    if (!t || (count && !out)) return fail("null argument");
     if (count > t->bound) return fail("more records than bound faces");
     HIP_OK(hipStreamSynchronize(t->stream));
-    HIP_OK(hipMemcpy(out, t->b_results.ptr, (size_t)count * 96, hipMemcpyDeviceToHost));
-    if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 16, hipMemcpyDeviceToHost));
+    if (count) HIP_OK(hipMemcpy(out, t->b_results.ptr, (size_t)count * 96, hipMemcpyDeviceToHost));
+    if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 40, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -951,6 +959,26 @@ int vgi_cache_state(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t *out
     if ((size_t)first + count > t->high_water) return fail("slot range beyond the high-water mark");
     HIP_OK(hipStreamSynchronize(t->stream));
     HIP_OK(hipMemcpy(out, t->d_state.as<uint32_t>() + (size_t)first * 36, (size_t)count * 144, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_probe_copy(vgi_tracer *t, uint64_t bytes, uint32_t repeats, float *ms) {
+    if (!t) return fail("null tracer");
+    HIP_OK(hipSetDevice(t->device));
+    DeviceBuffer a, b;
+    HIP_OK(a.reserve(bytes));
+    HIP_OK(b.reserve(bytes));
+    HIP_OK(hipMemsetAsync(a.ptr, 1, bytes, t->stream));
+    hipEventRecord(t->ev[6], t->stream);
+    for (uint32_t r = 0; r < repeats; ++r) vgi::launch_copy(a.ptr, b.ptr, bytes, t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->ev[7], t->stream);
+    HIP_OK(hipStreamSynchronize(t->stream));
+    float total = 0.0f;
+    hipEventElapsedTime(&total, t->ev[6], t->ev[7]);
+    if (ms) *ms = repeats ? total / (float)repeats : 0.0f;
+    a.release();
+    b.release();
     return 0;
 }
 

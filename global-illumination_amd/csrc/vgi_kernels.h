@@ -49,4 +49,6 @@ void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint3
 void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t high_water, uint32_t *state, uint32_t *checkpoint,
                          uint32_t *meta, float physical, hipStream_t stream);
 
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
+
 }  // namespace vgi

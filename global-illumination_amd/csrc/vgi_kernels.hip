@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optic
         t1 = tasks[face_index * 2u + 1u];
         valid = t0.w != NONE;
     }
-    Tally tally{0u, 0u};
+    Tally tally{0u, 0u, 0u, 0u, 0u};
     f3 mean = mk(0.0f, 0.0f, 0.0f), second = mk(0.0f, 0.0f, 0.0f);
     uint32_t known = 0u;
     int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
@@ -81,15 +81,12 @@ __global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optic
         r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
     }
     if (counters) {
-        unsigned long long rays = tally.rays, steps = tally.steps;
-        for (int off = 32; off > 0; off >>= 1) {
-            rays += __shfl_xor(rays, off, 64);
-            steps += __shfl_xor(steps, off, 64);
-        }
-        if ((threadIdx.x & 63u) == 0u) {
-            atomicAdd(&counters[0], rays);
-            atomicAdd(&counters[1], steps);
-        }
+        unsigned long long c[5] = {tally.rays, tally.steps, tally.headers, tally.bricks, tally.cells};
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            for (int off = 32; off > 0; off >>= 1) c[k] += __shfl_xor(c[k], off, 64);
+        if ((threadIdx.x & 63u) == 0u)
+            for (int k = 0; k < 5; ++k) atomicAdd(&counters[k], c[k]);
     }
 }
 
@@ -560,6 +557,14 @@ __global__ void apply_events_kernel(const uint4 *__restrict__ events, uint32_t e
     if (urgent) meta[(size_t)slot * 12u + 11u] |= 0x40000000u;
 }
 
+// A plain streaming copy (16 B per lane), the known-size reference against
+// which hardware byte counters are scaled.
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // ============================================================== launchers
 static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
@@ -612,6 +617,12 @@ void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t hig
     if (high_water == 0) return;
     hipLaunchKernelGGL(apply_events_kernel, dim3(blocks(high_water, 256)), dim3(256), 0, stream, events, event_count, high_water,
                        state, checkpoint, meta, physical, 192u * 192u, 64u * 64u, 48u * 48u);
+}
+
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream) {
+    size_t n = bytes / 16;
+    if (n == 0) return;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
 }
 
 }  // namespace vgi

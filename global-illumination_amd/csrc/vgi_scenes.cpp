@@ -104,8 +104,9 @@ bool exposed(const uint16_t *v, int n, int x, int y, int z) {
     return false;
 }
 
-// `count` emissive voxels on exposed solid voxels, materials 5.., drawn by
-// rejection from the PCG stream.
+// `count` emissive voxels on exposed solid voxels, drawn by rejection from the
+// PCG stream. Emitter i takes material 5 + (i mod 128) (materials are u8).
+constexpr int EMITTER_MATERIALS = 128;
 void place_emitters(uint16_t *v, int n, uint32_t seed, int count, int y_max) {
     int placed = 0;
     for (uint32_t i = 0; placed < count && i < 100000000u; ++i) {
@@ -114,7 +115,7 @@ void place_emitters(uint16_t *v, int n, uint32_t seed, int count, int y_max) {
         size_t k = at(x, y, z, n, n);
         if (v[k] == AIR || v[k] >= 5) continue;
         if (!exposed(v, n, x, y, z)) continue;
-        v[k] = (uint16_t)(5 + placed);
+        v[k] = (uint16_t)(5 + placed % EMITTER_MATERIALS);
         ++placed;
     }
 }
@@ -259,6 +260,7 @@ int vgi_scene_optics(int workload, uint32_t seed, float *diffuse, float *emissio
     case 4:
     case 5: {
         int emitters = workload == 2 ? 64 : workload == 3 ? 8 : workload == 4 ? 16 : 300;
+        emitters = std::min(emitters, EMITTER_MATERIALS);
         for (int m = 1; m <= 4; ++m)
             for (int c = 0; c < 3; ++c) diffuse[m * 3 + c] = 0.2f + 0.6f * unit(seed ^ 0xa1u, (uint32_t)(m * 3 + c));
         for (int e = 0; e < emitters; ++e) {

@@ -54,8 +54,11 @@ struct RayResult {
     f3 normal;
 };
 
+// Per-thread work counts (kept only when the launch asks for counters):
+// rays traced, traversal steps, and the dependent gathers of the traversal --
+// region headers (16 B), brick-table words (4 B), mixed-brick cell words (4 B).
 struct Tally {
-    uint32_t rays, steps;
+    uint32_t rays, steps, headers, bricks, cells;
 };
 
 __device__ __forceinline__ bool in_bounds(const SceneView &s, int rx, int ry, int rz) {
@@ -155,6 +158,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
         if (!in_bounds(s, rx, ry, rz)) return result;
         if (rx != lrx || ry != lry || rz != lrz) {
             uint4 h = region_header(s, rx, ry, rz);
+            tally.headers += 1u;
             if (h.x == 0u) {
                 result.status = geometry_ahead(s, position, direction, max_distance - distance) ? 2u : 0u;
                 return result;
@@ -174,6 +178,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
                 uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
                 uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
                 uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
+                tally.bricks += 1u;
                 uint32_t st = brick & 3u;
                 if (st == 0u) {
                     uint32_t e = brick >> 2u;
@@ -186,6 +191,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
                 } else {
                     uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
                     uint32_t word = s.cells[(brick >> 2u) + (idx >> 1)];
+                    tally.cells += 1u;
                     uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
                     material = (cell & 1u) != 0u ? cell >> 1u : AIR;
                     if (material == AIR) {

@@ -67,12 +67,17 @@ typedef struct vgi_transport {
     uint32_t history_limit;  /* 0 -> HISTORY_LIMIT 65536 (resident.rs:715-719) */
     float history_recency;   /* >= 1 (resident.rs:721-725) */
     float measured_retention;/* settings.rs:262 */
+    uint32_t plain;          /* 1: paths never end at cached faces (plain multi-bounce tracing);
+                                0: the reference's world-cache termination (voxel_path.wgsl:459-463) */
 } vgi_transport;
 
 /* ---- lifecycle: GpuTracer::new / enable_resident (gpu.rs:309-366) ---- */
 int vgi_create(int device_ordinal, vgi_tracer **out);
 void vgi_destroy(vgi_tracer *tracer);
 int vgi_enable_resident(vgi_tracer *tracer);
+/* The resident clock (ms) that the display EMA reads; negative restores the
+ * wall clock. Mirrors the reference's test clock (gpu/resident.rs:708-710). */
+int vgi_set_clock(vgi_tracer *tracer, int64_t milliseconds);
 
 /* ---- scene: VoxelGiInput::{page_loaded, page_unloaded} (input.rs:54-112) feeding
  * TransportScene::with_page_contents (world.rs:982-1046). Pages are 32^3 u16
@@ -110,7 +115,7 @@ int vgi_read_faces(vgi_tracer *tracer, uint64_t ticket, vgi_face_result *out, ui
  * voxel xyz, face | status << 8, material, cache slot, rays of the sample,
  * known. Requires VGI_RECORD_HITS. */
 int vgi_read_hits(vgi_tracer *tracer, uint64_t ticket, uint32_t *out, uint64_t words);
-/* [rays, steps, 0...] of the ticket (VGI_COUNT_RAYS). */
+/* [rays, steps, header loads, brick-word loads, cell-word loads, 0...] of the ticket (VGI_COUNT_RAYS). */
 int vgi_read_counters(vgi_tracer *tracer, uint64_t ticket, uint64_t *out, uint32_t count);
 /* GPU timestamps (gpu.rs:486-497,1050-1070): ms of [prepare, trace, accumulate, build]. */
 int vgi_timings(vgi_tracer *tracer, float *out, uint32_t count);
@@ -125,8 +130,10 @@ int vgi_run_frame(vgi_tracer *tracer, uint32_t flags);
 int vgi_sync(vgi_tracer *tracer);
 /* Time of the last frame's trace launch on its stream (ms), valid after vgi_sync. */
 float vgi_last_trace_ms(vgi_tracer *tracer);
-/* The bound faces' 96-byte records after the last frame; counters (optional)
- * receive [rays, steps] when the frame ran with VGI_COUNT_RAYS. */
+/* The bound faces' 96-byte records after the last frame (out may be NULL with
+ * count 0 to read only the counters); counters (optional)
+ * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
+ * the frame ran with VGI_COUNT_RAYS. */
 int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uint64_t *counters);
 /* Raw resident state (9 uint4 per slot, voxel_accumulate.wgsl:1-4) of slots [first, first+count). */
 int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
@@ -134,6 +141,11 @@ int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t
 /* ---- retention events (resident.rs:1326-1417 encode_events layout;
  * voxel_retention.wgsl:29-96, apply_events voxel_accumulate.wgsl:295-332) ---- */
 int vgi_apply_events(vgi_tracer *tracer, const uint32_t *events, uint32_t event_count, uint32_t words);
+
+/* ---- measurement plumbing: a plain copy kernel of `bytes` run `repeats`
+ * times (average ms per launch in *ms), the known-size reference that scales
+ * hardware byte counters (not part of the reference seam). ---- */
+int vgi_probe_copy(vgi_tracer *tracer, uint64_t bytes, uint32_t repeats, float *ms);
 
 /* ---- scene inspection for parity checks ---- */
 int vgi_scene_bounds(vgi_tracer *tracer, int32_t out[6]);

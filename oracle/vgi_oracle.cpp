@@ -1621,7 +1621,7 @@ static int trace_faces(Tracer &t, const uint32_t *tasks, uint32_t count, float *
     parallel_for(count, threads, [&](uint32_t b, uint32_t e, int w) {
         for (uint32_t i = b; i < e; ++i)
             trace_one_face(t, &tasks[(size_t)i * 8], &results[(size_t)i * 24], hits ? &hits[(size_t)i * spp * 8] : nullptr,
-                           counters ? &per[w] : nullptr);
+                           (counters || hits) ? &per[w] : nullptr);
     });
     if (counters) {
         Counters s;

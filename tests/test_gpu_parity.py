@@ -56,7 +56,7 @@ def test_box_one_bounce_bit_exact(vgi, orc):
 
 
 def test_noise_multibounce_tree_bit_exact(vgi, orc):
-    setup = make_scene(vgi, orc, 3, 0x5EED0003, edge=64)
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
     tasks = vgi.exposed_faces(setup["grid"])[::7]
     words = setup["tracer"].emitter_words()
     gpu, cpu, gh, ch, gc, cc = trace_both(vgi, setup, tasks, bounces=3, samples=4)
@@ -86,4 +86,35 @@ def test_naadf_build_matches(vgi, orc, workload, edge):
                 a = g.region_canonical(rx, ry, rz)
                 c = setup["scene"].region_canonical(rx, ry, rz)
                 assert np.array_equal(a, c), "region %s: kind %d/%d" % ((rx, ry, rz), a[0], c[0])
+    g.close()
+
+
+def test_resident_frames_bit_exact(vgi, orc):
+    """Several frames of prepare -> trace -> accumulate with world-cache
+    termination (faces with >= 16 samples end paths) on both sides."""
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    tasks = vgi.exposed_faces(setup["grid"])[::3]
+    t = setup["tracer"]
+    t.set_transport(bounces=2, samples=4, environment=tuple(setup["env"]))
+    t.cache_enable(history_limit=0, recency=1.0, measured_retention=1.0)
+    cpu_slots = t.cache_register(tasks)
+    g = gpu_tracer(vgi, setup, resident=True)
+    g.set_transport(bounces=2, samples=4, environment=tuple(setup["env"]))
+    cached_hits = 0
+    for frame in range(6):
+        print("resident frame %d" % frame, flush=True)
+        g.set_clock(16 * frame)
+        ticket = g.submit_compact(tasks, vgi.RECORD_HITS)
+        gpu = g.complete_readback(ticket, len(tasks))
+        gh = g.read_hits(ticket, len(tasks), 4)
+        if frame == 0:
+            assert np.array_equal(g.slots(len(tasks)), cpu_slots)
+        cpu, ch = t.cache_batch(tasks, cpu_slots, hits=True, clock_ms=16 * frame)
+        compare_hits(gh, ch)
+        assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), "frame %d not bit-identical" % frame
+        cached_hits += int((gh[:, 5] != 0xFFFFFFFF).sum())
+    state_gpu = g.cache_state(0, len(tasks))
+    state_cpu = t.cache_state(0, len(tasks))
+    assert np.array_equal(state_gpu, state_cpu)
+    assert cached_hits > 0
     g.close()
