@@ -139,6 +139,11 @@ def main():
     for _ in range(args.warmup):
         g.run_frame(vgi.COUNT_RAYS)
         g.sync()
+    # One untimed frame with every counter: the gather mix per ray that prices
+    # the roofline's algorithmic bytes.
+    g.run_frame(vgi.COUNT_GATHERS)
+    g.sync()
+    gather_counts = g.read_counters_bound().astype(np.float64)
     counters = np.zeros(5, np.float64)
     trace_ms = []
     t_start = time.perf_counter()
@@ -155,10 +160,11 @@ def main():
     mrays = rays / elapsed / 1e6
     ms_per_step = elapsed * 1000.0 / args.steps
     # Algorithmic bytes of one trace launch: the traversal's dependent
-    # gathers (16 B region header, 4 B brick word, 4 B cell word) plus the
-    # face task (32 B) and result record (96 B).
-    gathers = (16.0 * counters[2] + 4.0 * counters[3] + 4.0 * counters[4]) / args.steps
-    per_launch = gathers + 128.0 * len(faces)
+    # gathers (16 B region header, 4 B brick word, 4 B cell word; per-ray mix
+    # from the counting frame, times the timed frames' rays) plus the face
+    # task (32 B) and result record (96 B).
+    gathers_per_ray = (16.0 * gather_counts[2] + 4.0 * gather_counts[3] + 4.0 * gather_counts[4]) / gather_counts[0]
+    per_launch = gathers_per_ray * rays / args.steps + 128.0 * len(faces)
     avg_trace_s = float(np.mean(trace_ms)) / 1000.0
     achieved = per_launch / avg_trace_s / 1e9
     line = {

@@ -23,6 +23,7 @@ AIR = 256
 NO_FACE = 0xFFFFFFFF
 RECORD_HITS = 1
 COUNT_RAYS = 2
+COUNT_GATHERS = 4
 
 FACE_TASK = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("sequence", "<u4"), ("padding", "<u4", (3,))])
 assert FACE_TASK.itemsize == 32

@@ -13,7 +13,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvgi.so")
 BUILD = os.path.join(HERE, "_build")
 
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
 HIP_FLAGS = [
+    "-I" + INCLUDE,
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
@@ -24,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
+HOST_FLAGS = ["-I" + INCLUDE, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 
 HIP_SOURCES = ["vgi_kernels.hip", "vgi_host.cpp"]
 HOST_SOURCES = ["vgi_host_scene.cpp", "vgi_scenes.cpp"]

@@ -11,7 +11,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/vgi.h"
+#include "vgi.h"
 #include "vgi_host_scene.h"
 #include "vgi_kernels.h"
 #include "vgi_trace.h"
@@ -793,7 +793,7 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
     hipEventRecord(t->ev[1], t->stream);
     vgi::launch_trace(s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, k->results.as<float4>(),
                       k->with_hits ? k->hits.as<uint32_t>() : nullptr,
-                      k->with_counters ? k->counters.as<unsigned long long>() : nullptr, t->stream);
+                      (k->with_counters || k->with_hits) ? k->counters.as<unsigned long long>() : nullptr, true, t->stream);
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[2], t->stream);
     if (t->resident)
@@ -904,13 +904,15 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     vgi::Optics o;
     views(*t, s, o);
     uint32_t count = t->bound;
-    if (flags & VGI_COUNT_RAYS) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
+    bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
+    if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[4], t->stream);
     vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), t->b_results.as<float4>(),
-                      nullptr, (flags & VGI_COUNT_RAYS) ? t->b_counters.as<unsigned long long>() : nullptr, t->stream);
+                      nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr, (flags & VGI_COUNT_GATHERS) != 0,
+                      t->stream);
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[5], t->stream);
     if (t->resident)

@@ -32,7 +32,8 @@ struct AccumulateParams {
 };
 
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
-                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, hipStream_t stream);
+                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, bool full_counters,
+                  hipStream_t stream);
 void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
                      RegionInfo *info, hipStream_t stream);
 void launch_scan(RegionInfo *info, uint32_t n, uint32_t cells_base, uint32_t *totals, hipStream_t stream);

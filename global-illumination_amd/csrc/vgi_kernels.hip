@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "vgi_kernels.h"
 #include "vgi_trace.h"
 
@@ -15,7 +17,15 @@ namespace vgi {
 // and writes the 96-byte record after one more analytic direct estimate
 // (write_result, voxel_trace.wgsl:25-37). lanes divides 64, so a face's lanes
 // share a wavefront and the sum uses cross-lane reads instead of LDS.
-__global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optics optics, const uint4 *__restrict__ tasks,
+// Occupancy: the path estimator is one long dependent gather chain, so the
+// kernel is latency bound; it is compiled for VGI_TRACE_WAVES waves per SIMD
+// (the register budget that allows), trading a little spill for latency hiding.
+#ifndef VGI_TRACE_WAVES
+#define VGI_TRACE_WAVES 8
+#endif
+template <class Tally, bool kHits>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void trace_faces_kernel(
+    SceneView scene, Optics optics, const uint4 *__restrict__ tasks,
                                                           uint32_t count, uint32_t samples, uint32_t lanes,
                                                           float4 *__restrict__ results, uint32_t *__restrict__ hits,
                                                           unsigned long long *__restrict__ counters) {
@@ -29,7 +39,7 @@ __global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optic
         t1 = tasks[face_index * 2u + 1u];
         valid = t0.w != NONE;
     }
-    Tally tally{0u, 0u, 0u, 0u, 0u};
+    Tally tally;
     f3 mean = mk(0.0f, 0.0f, 0.0f), second = mk(0.0f, 0.0f, 0.0f);
     uint32_t known = 0u;
     int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
@@ -42,12 +52,12 @@ __global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optic
         for (uint32_t i = lane; i < samples; i += lanes) {
             bool k;
             FirstHit first;
-            uint32_t rays_before = tally.rays;
+            uint32_t rays_before = tally.ray_count();
             f3 value = trace_receiver(scene, optics, surface, normal, stream, t1.x + i, k, first, tally);
-            if (hits) {
+            if (kHits) {
                 uint4 *h = reinterpret_cast<uint4 *>(hits + ((size_t)face_index * samples + i) * 8u);
                 h[0] = make_uint4((uint32_t)first.vx, (uint32_t)first.vy, (uint32_t)first.vz, first.face | (first.status << 8));
-                h[1] = make_uint4(first.material, first.slot, tally.rays - rays_before, k ? 1u : 0u);
+                h[1] = make_uint4(first.material, first.slot, tally.ray_count() - rays_before, k ? 1u : 0u);
             }
             if (!k) continue;
             mean = mean + value;
@@ -80,13 +90,18 @@ __global__ __launch_bounds__(256) void trace_faces_kernel(SceneView scene, Optic
         r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
         r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
     }
-    if (counters) {
+    if (counters == nullptr) return;
+    if constexpr (std::is_same<Tally, FullTally>::value) {
         unsigned long long c[5] = {tally.rays, tally.steps, tally.headers, tally.bricks, tally.cells};
 #pragma unroll
         for (int k = 0; k < 5; ++k)
             for (int off = 32; off > 0; off >>= 1) c[k] += __shfl_xor(c[k], off, 64);
         if ((threadIdx.x & 63u) == 0u)
             for (int k = 0; k < 5; ++k) atomicAdd(&counters[k], c[k]);
+    } else if constexpr (std::is_same<Tally, RayTally>::value) {
+        unsigned long long r = tally.rays;
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+        if ((threadIdx.x & 63u) == 0u) atomicAdd(&counters[0], r);
     }
 }
 
@@ -568,12 +583,26 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
 // ============================================================== launchers
 static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// Kernel variant by what the caller asked for: hit records (with full
+// counters), all counters, rays only, or nothing.
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
-                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, hipStream_t stream) {
+                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, bool full_counters,
+                  hipStream_t stream) {
     uint64_t threads = (uint64_t)count * lanes;
     if (threads == 0) return;
-    hipLaunchKernelGGL(trace_faces_kernel, dim3(blocks(threads, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples,
-                       lanes, results, hits, counters);
+    dim3 grid(blocks(threads, 256)), block(256);
+    if (hits)
+        hipLaunchKernelGGL((trace_faces_kernel<FullTally, true>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
+                           results, hits, counters);
+    else if (counters && full_counters)
+        hipLaunchKernelGGL((trace_faces_kernel<FullTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
+                           results, hits, counters);
+    else if (counters)
+        hipLaunchKernelGGL((trace_faces_kernel<RayTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
+                           results, hits, counters);
+    else
+        hipLaunchKernelGGL((trace_faces_kernel<NoTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
+                           results, hits, counters);
 }
 void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
                      RegionInfo *info, hipStream_t stream) {

@@ -122,21 +122,55 @@ __device__ __forceinline__ uint32_t scramble_reverse(uint32_t input, uint32_t se
     value ^= value * 0x53a22864u;
     return value;
 }
-__device__ __constant__ static const uint32_t kPrimes[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+// The Halton loop's running scale is fl(...fl(1/b)/b...): the same values for
+// every call, so they are folded at compile time (IEEE division in the
+// compiler's constant evaluator, identical to the run-time divide), and the
+// integer divide and modulo take compile-time divisors.
+template <uint32_t B>
+struct HaltonScales {
+    float s[26];
+    constexpr HaltonScales() : s() {
+        float v = 1.0f;
+        for (int k = 0; k < 26; ++k) {
+            v = v / (float)B;
+            s[k] = v;
+        }
+    }
+};
+template <uint32_t B>
+__device__ __forceinline__ float halton(uint32_t sequence) {
+    constexpr HaltonScales<B> scales;
+    uint32_t index = (sequence & 0x00ffffffu) + 1u;
+    float value = 0.0f;
+    int k = 0;
+    while (index != 0u) {
+        value = value + (float)(index % B) * scales.s[k];
+        index = index / B;
+        ++k;
+    }
+    return value;
+}
+__device__ __forceinline__ float radical_inverse(uint32_t dimension, uint32_t sequence) {
+    switch (dimension) {
+    case 0: return halton<2>(sequence);
+    case 1: return halton<3>(sequence);
+    case 2: return halton<5>(sequence);
+    case 3: return halton<7>(sequence);
+    case 4: return halton<11>(sequence);
+    case 5: return halton<13>(sequence);
+    case 6: return halton<17>(sequence);
+    case 7: return halton<19>(sequence);
+    case 8: return halton<23>(sequence);
+    case 9: return halton<29>(sequence);
+    case 10: return halton<31>(sequence);
+    default: return halton<37>(sequence);
+    }
+}
 // voxel_path.wgsl:187-213: Halton with Cranley-Patterson rotation below
 // dimension 12, hash-Owen-scrambled Sobol pairs from 12 on.
 __device__ __forceinline__ float sample_1d(uint32_t stream, uint32_t sequence, uint32_t dimension) {
     if (dimension < 12u) {
-        uint32_t base = kPrimes[dimension];
-        uint32_t index = (sequence & 0x00ffffffu) + 1u;
-        float scale = 1.0f;
-        float value = 0.0f;
-        float fbase = (float)base;
-        while (index != 0u) {
-            scale = scale / fbase;
-            value = value + (float)(index % base) * scale;
-            index = index / base;
-        }
+        float value = radical_inverse(dimension, sequence);
         float rotation = (float)(pcg(stream ^ pcg(dimension)) >> 9u) / 8388608.0f;
         float x = value + rotation;
         return x - floorf(x);

@@ -7,7 +7,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/vgi.h"
+#include "vgi.h"
 
 namespace {
 

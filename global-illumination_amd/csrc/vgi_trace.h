@@ -54,11 +54,32 @@ struct RayResult {
     f3 normal;
 };
 
-// Per-thread work counts (kept only when the launch asks for counters):
-// rays traced, traversal steps, and the dependent gathers of the traversal --
+// Per-thread work counts. The tracing functions are templates on the tally
+// type, so the production launch (NoTally) carries no counting at all, the
+// timed frames count rays only (RayTally), and the parity/roofline launch
+// counts rays, traversal steps and the dependent gathers of the traversal --
 // region headers (16 B), brick-table words (4 B), mixed-brick cell words (4 B).
-struct Tally {
-    uint32_t rays, steps, headers, bricks, cells;
+struct NoTally {
+    __device__ void ray() {}
+    __device__ void step() {}
+    __device__ void header() {}
+    __device__ void brick() {}
+    __device__ void cell() {}
+    __device__ uint32_t ray_count() const { return 0u; }
+};
+struct RayTally : NoTally {
+    uint32_t rays = 0u;
+    __device__ void ray() { rays += 1u; }
+    __device__ uint32_t ray_count() const { return rays; }
+};
+struct FullTally {
+    uint32_t rays = 0u, steps = 0u, headers = 0u, bricks = 0u, cells = 0u;
+    __device__ void ray() { rays += 1u; }
+    __device__ void step() { steps += 1u; }
+    __device__ void header() { headers += 1u; }
+    __device__ void brick() { bricks += 1u; }
+    __device__ void cell() { cells += 1u; }
+    __device__ uint32_t ray_count() const { return rays; }
 };
 
 __device__ __forceinline__ bool in_bounds(const SceneView &s, int rx, int ry, int rz) {
@@ -97,7 +118,7 @@ __device__ __forceinline__ void advance_box(f3 position, f3 direction, int cx, i
 }
 
 // voxel_traversal.wgsl:108-121
-__device__ __noinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 direction, float max_distance) {
+__device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 direction, float max_distance) {
     f3 position = start;
     float remaining = max_distance;
     for (uint32_t step = 0; step < 2048u; ++step) {
@@ -116,6 +137,7 @@ __device__ __noinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 dir
 
 // voxel_traversal.wgsl:126-219 with visible_geometry = false, max_steps = 2048
 // (the path tracer's `trace`, voxel_path.wgsl:239-243).
+template <class Tally>
 __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3 direction, float max_distance, Tally &tally) {
     RayResult result;
     result.vx = result.vy = result.vz = 0;
@@ -123,7 +145,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
     result.position = mk(0.0f, 0.0f, 0.0f);
     result.status = 0u;
     result.normal = mk(0.0f, 0.0f, 0.0f);
-    tally.rays += 1u;
+    tally.ray();
     float near_ = -1e30f, far_ = 1e30f;
     f3 normal = mk(0.0f, 0.0f, 0.0f);
     const float dv[3] = {direction.x, direction.y, direction.z};
@@ -151,14 +173,14 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
     int lrx = 0x7fffffff, lry = 0x7fffffff, lrz = 0x7fffffff;
     uint4 meta = make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t step = 0; step < 2048u; ++step) {
-        tally.steps += 1u;
+        tally.step();
         if (distance > max_distance) return result;
         int vx = (int)floorf(position.x), vy = (int)floorf(position.y), vz = (int)floorf(position.z);
         int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
         if (!in_bounds(s, rx, ry, rz)) return result;
         if (rx != lrx || ry != lry || rz != lrz) {
             uint4 h = region_header(s, rx, ry, rz);
-            tally.headers += 1u;
+            tally.header();
             if (h.x == 0u) {
                 result.status = geometry_ahead(s, position, direction, max_distance - distance) ? 2u : 0u;
                 return result;
@@ -178,7 +200,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
                 uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
                 uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
                 uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
-                tally.bricks += 1u;
+                tally.brick();
                 uint32_t st = brick & 3u;
                 if (st == 0u) {
                     uint32_t e = brick >> 2u;
@@ -191,7 +213,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
                 } else {
                     uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
                     uint32_t word = s.cells[(brick >> 2u) + (idx >> 1)];
-                    tally.cells += 1u;
+                    tally.cell();
                     uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
                     material = (cell & 1u) != 0u ? cell >> 1u : AIR;
                     if (material == AIR) {
@@ -280,7 +302,7 @@ __device__ __forceinline__ float node_left_probability(const Optics &o, const No
     return fclamp(0.05f * power + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
 }
 // voxel_path.wgsl:132-149
-__device__ __noinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 normal, float choice) {
+__device__ __forceinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 normal, float choice) {
     uint32_t index = 0u;
     float uniform = choice;
     float probability = 1.0f;
@@ -298,7 +320,7 @@ __device__ __noinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 no
     }
 }
 // voxel_path.wgsl:158-174 (key_less :150-157)
-__device__ __noinline__ float tree_probability(const Optics &o, f3 position, f3 normal, int vx, int vy, int vz, f3 hit_normal) {
+__device__ __forceinline__ float tree_probability(const Optics &o, f3 position, f3 normal, int vx, int vy, int vz, f3 hit_normal) {
     uint32_t face = 0u;
     for (uint32_t i = 0u; i < 6u; ++i) {
         f3 fn = face_normal(i);
@@ -339,7 +361,8 @@ __device__ __forceinline__ Emitter choose_emitter(const Optics &o, f3 position, 
 __device__ __forceinline__ float connection_reach(const Optics &o) { return o.limited_reach ? o.reach : o.max_distance; }
 
 // voxel_path.wgsl:332-364
-__device__ __noinline__ f3 sample_emitter(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t source_stream,
+template <class Tally>
+__device__ __forceinline__ f3 sample_emitter(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t source_stream,
                                           uint32_t sequence, uint32_t bounce, Tally &tally) {
     f3 zero = mk(0.0f, 0.0f, 0.0f);
     if (o.emitter_faces == 0u) return zero;
@@ -374,7 +397,8 @@ __device__ __noinline__ f3 sample_emitter(const SceneView &s, const Optics &o, f
 }
 
 // voxel_path.wgsl:283-307
-__device__ __noinline__ void direct_at(const SceneView &s, const Optics &o, f3 position, f3 normal, f3 &value, f3 &custom,
+template <class Tally>
+__device__ __forceinline__ void direct_at(const SceneView &s, const Optics &o, f3 position, f3 normal, f3 &value, f3 &custom,
                                        Tally &tally) {
     value = mk(0.0f, 0.0f, 0.0f);
     custom = mk(0.0f, 0.0f, 0.0f);
@@ -428,6 +452,7 @@ __device__ __forceinline__ bool cached_energy(const Optics &o, int vx, int vy, i
 }
 
 // voxel_path.wgsl:271-282
+template <class Tally>
 __device__ __forceinline__ RayResult trace_diffuse(const SceneView &s, const Optics &o, f3 origin, f3 direction, Tally &tally) {
     float reach = connection_reach(o);
     RayResult r = trace_ray(s, origin, direction, reach, tally);
@@ -438,7 +463,8 @@ __device__ __forceinline__ RayResult trace_diffuse(const SceneView &s, const Opt
 }
 
 // voxel_path.wgsl:368-394
-__device__ __noinline__ f3 sample_terminal_emission(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t stream,
+template <class Tally>
+__device__ __forceinline__ f3 sample_terminal_emission(const SceneView &s, const Optics &o, f3 position, f3 normal, uint32_t stream,
                                                     uint32_t sequence, uint32_t bounce, Tally &tally) {
     f3 zero = mk(0.0f, 0.0f, 0.0f);
     if (o.emitter_faces == 0u) return zero;
@@ -472,6 +498,7 @@ struct FirstHit {
 };
 
 // voxel_path.wgsl:418-496 (begin_path + advance_path with terminal = true)
+template <class Tally>
 __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3 start, f3 first_direction, f3 initial_normal,
                                          uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally) {
     f3 energy = mk(0.0f, 0.0f, 0.0f);
@@ -548,6 +575,7 @@ __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3
 }
 
 // voxel_path.wgsl:504-520
+template <class Tally>
 __device__ __forceinline__ f3 trace_receiver(const SceneView &s, const Optics &o, f3 surface, f3 normal, uint32_t stream,
                                              uint32_t sequence, bool &known, FirstHit &first, Tally &tally) {
     f3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));

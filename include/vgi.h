@@ -105,6 +105,7 @@ int vgi_set_transport(vgi_tracer *tracer, const vgi_transport *transport);
  * face-sample first-hit records; VGI_COUNT_RAYS counts rays and steps. ---- */
 #define VGI_RECORD_HITS 1u
 #define VGI_COUNT_RAYS 2u
+#define VGI_COUNT_GATHERS 4u /* frames: also steps and traversal gathers (tickets always count all) */
 int vgi_submit_faces(vgi_tracer *tracer, const vgi_face_task *tasks, uint32_t count, uint32_t flags,
                      uint64_t *ticket);
 /* GpuTracer::snapshot_ready (gpu.rs:1024-1049) */
