@@ -198,6 +198,11 @@ def main():
             "traffic": None,
         },
     }
+    # HBM bytes per launch of this kernel on this workload, from the committed
+    # rocprofv3 PMC passes (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel).
+    traffic_file = os.path.join(ROOT, "profiles", "trace_faces_%s_traffic.json" % args.workload)
+    if os.path.exists(traffic_file):
+        line["roofline"]["traffic"] = round(json.load(open(traffic_file))["traffic_bytes_per_launch"])
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(vgi, grid, coords, pages, optics, faces, bounces, spp, args.cpu_budget)
     g.close()
