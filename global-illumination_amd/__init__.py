@@ -24,6 +24,7 @@ NO_FACE = 0xFFFFFFFF
 RECORD_HITS = 1
 COUNT_RAYS = 2
 COUNT_GATHERS = 4
+MLP_PARAMS = 14528
 
 FACE_TASK = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("sequence", "<u4"), ("padding", "<u4", (3,))])
 assert FACE_TASK.itemsize == 32
@@ -91,6 +92,14 @@ def library():
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
         "vgi_probe_copy": (ctypes.c_int, [P, u64, u32, P]),
+        "vgi_converged": (ctypes.c_int, [P, u32, u32, P]),
+        "vgi_mlp_enable": (ctypes.c_int, [P, u32, u32, u32, f32]),
+        "vgi_mlp_set_learned": (ctypes.c_int, [P, ctypes.c_int]),
+        "vgi_mlp_weights": (ctypes.c_int, [P, P, P]),
+        "vgi_mlp_infer": (ctypes.c_int, [P, P, u32, P]),
+        "vgi_mlp_train": (ctypes.c_int, [P, P, P, u32, ctypes.c_int, P, P]),
+        "vgi_mlp_last_loss": (f32, [P]),
+        "vgi_mlp_pretrain": (ctypes.c_int, [P, u32]),
         "vgi_scene_bounds": (ctypes.c_int, [P, P]),
         "vgi_scene_region_canonical": (ctypes.c_int, [P, i32, i32, i32, P]),
         "vgi_scene_stats": (i64, [P, P]),
@@ -115,7 +124,8 @@ def exported_symbols():
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_read_bound",
-        "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
+        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces",
     ]
@@ -322,6 +332,45 @@ class GpuTracer:
         events = np.ascontiguousarray(events, np.uint32).reshape(-1)
         _check(self._lib.vgi_apply_events(self._h, _ptr(events), event_count, events.size))
 
+    def converged(self, launches, first_sequence=1 << 20):
+        """FP64 sums over `launches` plain launches of the bound faces: (n, 8)."""
+        out = np.zeros((self._bound, 8), np.float64)
+        _check(self._lib.vgi_converged(self._h, launches, first_sequence, _ptr(out)))
+        return out
+
+    # ---- learned radiance cache
+    def mlp_enable(self, seed=1, train_per_frame=65536, steps=4, lr=1e-3):
+        _check(self._lib.vgi_mlp_enable(self._h, seed, train_per_frame, steps, lr))
+
+    def mlp_set_learned(self, on):
+        _check(self._lib.vgi_mlp_set_learned(self._h, int(on)))
+
+    def mlp_weights(self, new=None):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        w = None if new is None else np.ascontiguousarray(new, np.float32)
+        _check(self._lib.vgi_mlp_weights(self._h, _ptr(out), _ptr(w)))
+        return out
+
+    def mlp_infer(self, queries):
+        q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+        out = np.zeros((len(q), 3), np.float32)
+        _check(self._lib.vgi_mlp_infer(self._h, _ptr(q), len(q), _ptr(out)))
+        return out
+
+    def mlp_train(self, queries, targets, update=False):
+        q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+        y = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
+        grad = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(self._lib.vgi_mlp_train(self._h, _ptr(q), _ptr(y), len(q), int(update), _ptr(grad), _ptr(loss)))
+        return grad, float(loss[0])
+
+    def mlp_pretrain(self, frames):
+        _check(self._lib.vgi_mlp_pretrain(self._h, frames))
+
+    def mlp_last_loss(self):
+        return float(self._lib.vgi_mlp_last_loss(self._h))
+
     def probe_copy(self, nbytes, repeats=5):
         ms = np.zeros(1, np.float32)
         _check(self._lib.vgi_probe_copy(self._h, nbytes, repeats, _ptr(ms)))
@@ -347,3 +396,24 @@ class GpuTracer:
         out = np.zeros(n, np.uint32)
         _check(self._lib.vgi_emitter_words(self._h, _ptr(out), n))
         return out
+
+
+# ---------------------------------------------------------------- error harness
+def converged_irradiance(sums, direct):
+    """Per-face converged irradiance (incident mean + analytic direct) from the
+    FP64 sums, and the per-face variance of that mean (the reference's own
+    noise, removed from the error estimate)."""
+    known = np.maximum(sums[:, 6:7], 1.0)
+    mean = sums[:, 0:3] / known
+    second = sums[:, 3:6] / known
+    var_of_mean = np.maximum(second - mean * mean, 0.0) / known
+    return mean + direct.astype(np.float64), var_of_mean
+
+
+def relative_rmse(estimate, reference, reference_var=None):
+    """sqrt(mean over faces and RGB of (estimate - reference)^2) / mean(reference),
+    with the reference's own sampling variance subtracted when given."""
+    err2 = np.mean((estimate.astype(np.float64) - reference) ** 2)
+    if reference_var is not None:
+        err2 = max(err2 - float(np.mean(reference_var)), 0.0)
+    return float(np.sqrt(err2) / np.mean(reference))

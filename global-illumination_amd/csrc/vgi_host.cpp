@@ -14,7 +14,12 @@
 #include "vgi.h"
 #include "vgi_host_scene.h"
 #include "vgi_kernels.h"
+#include "vgi_mlp.h"
 #include "vgi_trace.h"
+
+#include <cmath>
+#include <numeric>
+#include <random>
 
 using vgi_host::Int3;
 
@@ -128,6 +133,15 @@ struct vgi_tracer {
     DeviceBuffer b_tasks, b_slots, b_results, b_counters;
     uint32_t bound = 0;
     float last_trace_ms = 0.0f;
+    uint32_t frame_index = 0;
+    std::vector<uint32_t> bound_material;  // material of each bound face's voxel
+    // ---- learned radiance cache
+    bool mlp_ready = false, learned = false;
+    DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
+    uint32_t mlp_step = 0, train_per_frame = 65536, train_steps = 4, train_cursor = 0;
+    float mlp_lr = 1e-3f, last_loss = 0.0f;
+    DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
+    DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
 };
 
 namespace {
@@ -621,6 +635,65 @@ vgi::AccumulateParams accumulate_params(const vgi_tracer &t, uint32_t now) {
     return p;
 }
 
+vgi::MlpScene mlp_scene(const vgi_tracer &t) {
+    vgi::MlpScene sc;
+    sc.lo_x = (float)(t.bmin.x * 16);
+    sc.lo_y = (float)(t.bmin.y * 16);
+    sc.lo_z = (float)(t.bmin.z * 16);
+    int extent = std::max(std::max(t.bmax.x - t.bmin.x + 1, t.bmax.y - t.bmin.y + 1), t.bmax.z - t.bmin.z + 1) * 16;
+    sc.inv_extent = 1.0f / (float)std::max(extent, 1);
+    sc.materials = t.d_materials.as<float4>();
+    return sc;
+}
+
+// One optimizer step on `count` samples (a multiple of 16) of queries/targets
+// already on the device: fused forward/backward, weight-gradient GEMM, and
+// with `update` an Adam step and a re-pack of the BF16 fragments.
+int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets, uint32_t count, bool update) {
+    size_t n = count;
+    HIP_OK(t.d_acts.reserve(n * 288 * 2));
+    HIP_OK(t.d_grads.reserve(n * 272 * 2));
+    HIP_OK(t.d_train_pred.reserve(n * 12));
+    HIP_OK(hipMemsetAsync(t.d_mlp_loss.ptr, 0, 4, t.stream));
+    vgi::mlp_forward_backward(t.d_mlp_frag.as<uint16_t>(), mlp_scene(t), queries, reinterpret_cast<const float *>(targets), count,
+                              t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), t.d_mlp_loss.as<float>(), t.d_train_pred.as<float>(),
+                              t.stream);
+    vgi::mlp_weight_grad(t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), count, t.d_mlp_g.as<float>(), t.stream);
+    if (update) {
+        t.mlp_step += 1;
+        vgi::mlp_adam(t.d_mlp_w.as<float>(), t.d_mlp_m.as<float>(), t.d_mlp_v.as<float>(), t.d_mlp_g.as<float>(), t.mlp_step,
+                      t.mlp_lr, t.stream);
+        vgi::mlp_pack(t.d_mlp_w.as<float>(), t.d_mlp_frag.as<uint16_t>(), t.stream);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// A learned frame's training: trace a window of the face permutation with
+// plain multi-bounce paths (one sample each) for targets, then Adam steps.
+int learned_training(vgi_tracer &t) {
+    uint32_t total = (t.train_per_frame / 16u) * 16u;
+    if (total == 0 || t.bound == 0) return 0;
+    HIP_OK(t.d_train_tasks.reserve((size_t)total * 32));
+    HIP_OK(t.d_train_q.reserve((size_t)total * 16));
+    HIP_OK(t.d_train_results.reserve((size_t)total * 96));
+    HIP_OK(t.d_targets.reserve((size_t)total * 16));
+    vgi::launch_train_gather(t.b_tasks.as<uint4>(), t.d_perm.as<uint32_t>(), t.d_face_material.as<uint32_t>(), t.bound,
+                             t.train_cursor, total, t.d_train_tasks.as<uint4>(), t.d_train_q.as<vgi::MlpQuery>(), t.stream);
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(t, s, o);
+    o.cache_on = 0u;
+    vgi::launch_trace(s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index, t.d_train_results.as<float4>(),
+                      nullptr, nullptr, false, t.stream);
+    vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), t.stream);
+    uint32_t per = std::max<uint32_t>(16u, (total / std::max<uint32_t>(t.train_steps, 1u) / 16u) * 16u);
+    for (uint32_t off = 0; off + per <= total; off += per)
+        if (mlp_step(t, t.d_train_q.as<vgi::MlpQuery>() + off, t.d_targets.as<float4>() + off, per, true) != 0) return -1;
+    t.train_cursor = (uint32_t)(((uint64_t)t.train_cursor + total) % t.bound);
+    return 0;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -663,7 +736,10 @@ void vgi_destroy(vgi_tracer *t) {
     DeviceBuffer *bufs[] = {&t->d_grid, &t->d_headers, &t->d_cells, &t->d_info, &t->d_list, &t->d_top, &t->d_totals,
                             &t->d_scatter_idx, &t->d_scatter_val, &t->d_materials, &t->d_lights, &t->d_emitters,
                             &t->d_index, &t->d_state, &t->d_checkpoint, &t->d_meta, &t->d_tasks, &t->d_slots,
-                            &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters};
+                            &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters, &t->d_mlp_w, &t->d_mlp_m,
+                            &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->b_learned, &t->b_queries,
+                            &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
+                            &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io};
     for (DeviceBuffer *b : bufs) b->release();
     for (auto &e : t->ev) hipEventDestroy(e);
     hipStreamDestroy(t->stream);
@@ -791,7 +867,7 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
         vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[1], t->stream);
-    vgi::launch_trace(s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, k->results.as<float4>(),
+    vgi::launch_trace(s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, 0u, k->results.as<float4>(),
                       k->with_hits ? k->hits.as<uint32_t>() : nullptr,
                       (k->with_counters || k->with_hits) ? k->counters.as<unsigned long long>() : nullptr, true, t->stream);
     HIP_OK(hipGetLastError());
@@ -891,10 +967,28 @@ int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
         HIP_OK(hipMemcpyAsync(t->b_slots.ptr, slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
         t->last_slots = slots;
     }
+    // Each bound face's material (the cache encodes its albedo) and a fixed
+    // random order for choosing training faces.
+    vgi_host::VoxelSource src{t->grid.data(), t->vmin, t->gx, t->gy, t->gz, t->loaded.data(), t->rmin, t->rsx, t->rsy, t->rsz};
+    t->bound_material.assign(count, 0u);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t m = t->have_regions ? src.material_at(tasks[i].voxel[0], tasks[i].voxel[1], tasks[i].voxel[2]) : 0xffffu;
+        t->bound_material[i] = m < vgi_host::AIR ? m : 0u;
+    }
+    std::vector<uint32_t> perm(count);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937 rng(0x5eedu);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIP_OK(t->d_face_material.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+    HIP_OK(t->d_perm.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+    HIP_OK(hipMemcpyAsync(t->d_face_material.ptr, t->bound_material.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipMemcpyAsync(t->d_perm.ptr, perm.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
     HIP_OK(hipStreamSynchronize(t->stream));
     t->bound = count;
+    t->train_cursor = 0;
     return 0;
 }
+
 
 int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     if (!t) return fail("null tracer");
@@ -910,16 +1004,183 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[4], t->stream);
-    vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), t->b_results.as<float4>(),
-                      nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr, (flags & VGI_COUNT_GATHERS) != 0,
-                      t->stream);
+    if (t->learned) {
+        // Paths stop at their first hit; the network answers every query.
+        size_t samples = (size_t)count * t->transport.samples;
+        HIP_OK(t->b_learned.reserve(samples * 32));
+        HIP_OK(t->b_queries.reserve(samples * 16));
+        HIP_OK(t->b_pred.reserve(samples * 12));
+        o.cache_on = 0u;
+        vgi::launch_trace_learned(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
+                                  t->b_results.as<float4>(), counting ? t->b_counters.as<unsigned long long>() : nullptr,
+                                  t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>(), t->stream);
+        vgi::mlp_infer(t->d_mlp_frag.as<uint16_t>(), mlp_scene(*t), t->b_queries.as<vgi::MlpQuery>(), (uint32_t)samples,
+                       t->b_pred.as<float>(), t->stream);
+        vgi::launch_learned_combine(t->b_learned.as<float4>(), t->b_pred.as<float>(), count, t->transport.samples,
+                                    lanes_for(t->transport), t->b_tasks.as<uint4>(), t->b_results.as<float4>(), t->stream);
+    } else {
+        vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
+                          t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
+                          (flags & VGI_COUNT_GATHERS) != 0, t->stream);
+    }
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[5], t->stream);
+
     if (t->resident)
         vgi::launch_accumulate(t->b_slots.as<uint32_t>(), count, t->b_results.as<float4>(), t->d_state.as<uint32_t>(),
                                t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
     HIP_OK(hipGetLastError());
+    if (t->learned && learned_training(*t) != 0) return -1;
+    t->frame_index += 1;
     return 0;
+}
+
+int vgi_converged(vgi_tracer *t, uint32_t launches, uint32_t first_sequence, double *out) {
+    if (!t || !out) return fail("null argument");
+    if (!t->have_regions) return fail("no scene loaded");
+    if (t->bound == 0) return fail("no bound faces");
+    HIP_OK(hipSetDevice(t->device));
+    if (sync_emitters(*t) != 0) return -1;
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    o.cache_on = 0u;  // the judge of every estimator: plain multi-bounce paths
+    uint32_t count = t->bound;
+    DeviceBuffer sums, results;
+    HIP_OK(sums.reserve((size_t)count * 64));
+    HIP_OK(results.reserve((size_t)count * 96));
+    HIP_OK(hipMemsetAsync(sums.ptr, 0, (size_t)count * 64, t->stream));
+    for (uint32_t l = 0; l < launches; ++l) {
+        vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport),
+                          first_sequence + l * t->transport.samples, results.as<float4>(), nullptr, nullptr, false, t->stream);
+        vgi::launch_fp64_accumulate(results.as<float4>(), count, sums.as<double>(), t->stream);
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, sums.ptr, (size_t)count * 64, hipMemcpyDeviceToHost));
+    sums.release();
+    results.release();
+    return 0;
+}
+
+int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint32_t steps, float lr) {
+    if (!t) return fail("null tracer");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(t->d_mlp_w.reserve(vgi::MLP_PARAMS * 4));
+    HIP_OK(t->d_mlp_m.reserve(vgi::MLP_PARAMS * 4));
+    HIP_OK(t->d_mlp_v.reserve(vgi::MLP_PARAMS * 4));
+    HIP_OK(t->d_mlp_g.reserve(vgi::MLP_PARAMS * 4));
+    HIP_OK(t->d_mlp_frag.reserve(vgi::MLP_FRAG * 256 * 2));
+    HIP_OK(t->d_mlp_loss.reserve(16));
+    // He-uniform initialisation from the seed (fan-in of each layer).
+    std::vector<float> w(vgi::MLP_PARAMS);
+    std::mt19937 rng(seed);
+    const int ins[5] = {vgi::MLP_IN, vgi::MLP_HIDDEN, vgi::MLP_HIDDEN, vgi::MLP_HIDDEN, vgi::MLP_HIDDEN};
+    const int offs[6] = {vgi::MLP_W1, vgi::MLP_W2, vgi::MLP_W3, vgi::MLP_W4, vgi::MLP_W5, vgi::MLP_PARAMS};
+    for (int l = 0; l < 5; ++l) {
+        float a = std::sqrt(6.0f / (float)ins[l]);
+        std::uniform_real_distribution<float> u(-a, a);
+        for (int i = offs[l]; i < offs[l + 1]; ++i) w[i] = u(rng);
+    }
+    HIP_OK(hipMemcpyAsync(t->d_mlp_w.ptr, w.data(), w.size() * 4, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_mlp_m.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_mlp_v.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_mlp_g.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
+    vgi::mlp_pack(t->d_mlp_w.as<float>(), t->d_mlp_frag.as<uint16_t>(), t->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t->stream));
+    t->mlp_ready = true;
+    t->learned = true;
+    t->mlp_step = 0;
+    t->train_per_frame = train_per_frame;
+    t->train_steps = std::max<uint32_t>(steps, 1u);
+    t->mlp_lr = lr;
+    return 0;
+}
+
+int vgi_mlp_set_learned(vgi_tracer *t, int on) {
+    if (!t) return fail("null tracer");
+    if (on && !t->mlp_ready) return fail("the learned cache is not enabled (vgi_mlp_enable)");
+    t->learned = on != 0;
+    return 0;
+}
+
+int vgi_mlp_weights(vgi_tracer *t, float *out, const float *in) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    HIP_OK(hipSetDevice(t->device));
+    if (out) {
+        HIP_OK(hipStreamSynchronize(t->stream));
+        HIP_OK(hipMemcpy(out, t->d_mlp_w.ptr, vgi::MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+    }
+    if (in) {
+        HIP_OK(hipMemcpyAsync(t->d_mlp_w.ptr, in, vgi::MLP_PARAMS * 4, hipMemcpyHostToDevice, t->stream));
+        vgi::mlp_pack(t->d_mlp_w.as<float>(), t->d_mlp_frag.as<uint16_t>(), t->stream);
+        HIP_OK(hipStreamSynchronize(t->stream));
+    }
+    return 0;
+}
+
+int vgi_mlp_infer(vgi_tracer *t, const uint32_t *queries, uint32_t count, float *out) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (count && (!queries || !out)) return fail("null argument");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(t->d_io.reserve((size_t)std::max<uint32_t>(count, 1) * 28));
+    char *base = t->d_io.as<char>();
+    HIP_OK(hipMemcpyAsync(base, queries, (size_t)count * 16, hipMemcpyHostToDevice, t->stream));
+    vgi::mlp_infer(t->d_mlp_frag.as<uint16_t>(), mlp_scene(*t), reinterpret_cast<vgi::MlpQuery *>(base), count,
+                   reinterpret_cast<float *>(base + (size_t)count * 16), t->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, base + (size_t)count * 16, (size_t)count * 12, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_mlp_train(vgi_tracer *t, const uint32_t *queries, const float *targets, uint32_t count, int update, float *grad_out,
+                  float *loss_out) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (count == 0 || (count % 16u) != 0) return fail("training batches are a positive multiple of 16 samples");
+    if (!queries || !targets) return fail("null argument");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(t->d_io.reserve((size_t)count * 32));
+    char *base = t->d_io.as<char>();
+    HIP_OK(hipMemcpyAsync(base, queries, (size_t)count * 16, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipMemcpyAsync(base + (size_t)count * 16, targets, (size_t)count * 16, hipMemcpyHostToDevice, t->stream));
+    if (mlp_step(*t, reinterpret_cast<vgi::MlpQuery *>(base), reinterpret_cast<const float4 *>(base + (size_t)count * 16), count,
+                 update != 0) != 0)
+        return -1;
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (loss_out) HIP_OK(hipMemcpy(loss_out, t->d_mlp_loss.ptr, 4, hipMemcpyDeviceToHost));
+    if (!update) {
+        if (grad_out) HIP_OK(hipMemcpy(grad_out, t->d_mlp_g.ptr, vgi::MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemset(t->d_mlp_g.ptr, 0, vgi::MLP_PARAMS * 4));
+    }
+    return 0;
+}
+
+// Training-only frames: each traces one window of training faces (plain
+// multi-bounce, one sample) and takes the frame's Adam steps, without tracing
+// the bound faces -- the network's warm-up before it serves paths.
+int vgi_mlp_pretrain(vgi_tracer *t, uint32_t frames) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (!t->have_regions) return fail("no scene loaded");
+    if (t->bound == 0) return fail("no bound faces");
+    HIP_OK(hipSetDevice(t->device));
+    if (sync_emitters(*t) != 0) return -1;
+    for (uint32_t f = 0; f < frames; ++f) {
+        if (learned_training(*t) != 0) return -1;
+        t->frame_index += 1;
+    }
+    return 0;
+}
+
+float vgi_mlp_last_loss(vgi_tracer *t) {
+    if (!t || !t->mlp_ready) return 0.0f;
+    float l = 0.0f;
+    hipStreamSynchronize(t->stream);
+    hipMemcpy(&l, t->d_mlp_loss.ptr, 4, hipMemcpyDeviceToHost);
+    return l;
 }
 
 int vgi_sync(vgi_tracer *t) {

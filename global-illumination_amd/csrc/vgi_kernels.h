@@ -7,6 +7,7 @@ namespace vgi {
 
 struct SceneView;
 struct Optics;
+struct MlpQuery;
 
 // Per listed region: the classification of world.rs:243-319 plus the packing
 // offsets the scan assigns (words into the cells buffer).
@@ -32,8 +33,17 @@ struct AccumulateParams {
 };
 
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
-                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, bool full_counters,
-                  hipStream_t stream);
+                  uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits, unsigned long long *counters,
+                  bool full_counters, hipStream_t stream);
+void launch_trace_learned(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
+                          uint32_t lanes, uint32_t sequence_base, float4 *results, unsigned long long *counters, float4 *learned,
+                          MlpQuery *queries, hipStream_t stream);
+void launch_learned_combine(const float4 *learned, const float *pred, uint32_t count, uint32_t samples, uint32_t lanes,
+                            const uint4 *tasks, float4 *results, hipStream_t stream);
+void launch_train_gather(const uint4 *bound, const uint32_t *perm, const uint32_t *face_material, uint32_t n_bound, uint32_t start,
+                         uint32_t count, uint4 *tasks, MlpQuery *queries, hipStream_t stream);
+void launch_train_targets(const float4 *results, uint32_t count, float4 *targets, hipStream_t stream);
+void launch_fp64_accumulate(const float4 *results, uint32_t count, double *sums, hipStream_t stream);
 void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
                      RegionInfo *info, hipStream_t stream);
 void launch_scan(RegionInfo *info, uint32_t n, uint32_t cells_base, uint32_t *totals, hipStream_t stream);

@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "vgi_kernels.h"
+#include "vgi_mlp.h"
 #include "vgi_trace.h"
 
 namespace vgi {
@@ -23,12 +24,15 @@ namespace vgi {
 #ifndef VGI_TRACE_WAVES
 #define VGI_TRACE_WAVES 8
 #endif
-template <class Tally, bool kHits>
+// kLearned: paths stop at their first hit into the learned cache; each
+// sample's partial energy, prediction factor and cache query go to
+// `learned` (2 float4 per sample) and `queries`, and the combine pass forms the
+// face moments once the network has answered.
+template <class Tally, bool kHits, bool kLearned>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void trace_faces_kernel(
-    SceneView scene, Optics optics, const uint4 *__restrict__ tasks,
-                                                          uint32_t count, uint32_t samples, uint32_t lanes,
-                                                          float4 *__restrict__ results, uint32_t *__restrict__ hits,
-                                                          unsigned long long *__restrict__ counters) {
+    SceneView scene, Optics optics, const uint4 *__restrict__ tasks, uint32_t count, uint32_t samples, uint32_t lanes,
+    uint32_t sequence_base, float4 *__restrict__ results, uint32_t *__restrict__ hits, unsigned long long *__restrict__ counters,
+    float4 *__restrict__ learned, MlpQuery *__restrict__ queries) {
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t face_index = tid / lanes;
     uint32_t lane = tid % lanes;
@@ -53,11 +57,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
             bool k;
             FirstHit first;
             uint32_t rays_before = tally.ray_count();
-            f3 value = trace_receiver(scene, optics, surface, normal, stream, t1.x + i, k, first, tally);
+            CacheQuery cq;
+            cq.face_material = NONE;
+            cq.factor = mk(0.0f, 0.0f, 0.0f);
+            f3 value = trace_receiver<Tally, kLearned>(scene, optics, surface, normal, stream, t1.x + sequence_base + i, k, first,
+                                                       tally, &cq);
             if (kHits) {
                 uint4 *h = reinterpret_cast<uint4 *>(hits + ((size_t)face_index * samples + i) * 8u);
                 h[0] = make_uint4((uint32_t)first.vx, (uint32_t)first.vy, (uint32_t)first.vz, first.face | (first.status << 8));
                 h[1] = make_uint4(first.material, first.slot, tally.ray_count() - rays_before, k ? 1u : 0u);
+            }
+            if constexpr (kLearned) {
+                size_t at = (size_t)face_index * samples + i;
+                learned[at * 2u] = make_float4(value.x, value.y, value.z, k ? 1.0f : 0.0f);
+                learned[at * 2u + 1u] = make_float4(cq.factor.x, cq.factor.y, cq.factor.z, 0.0f);
+                queries[at] = MlpQuery{cq.vx, cq.vy, cq.vz, k ? cq.face_material : NONE};
+                continue;
             }
             if (!k) continue;
             mean = mean + value;
@@ -83,9 +98,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
         direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc, tally);
         float inverse = 1.0f / (float)(tknown > 1u ? tknown : 1u);
         float4 *r = results + (size_t)face_index * 6u;
-        r[0] = make_float4(tmean.x * inverse, tmean.y * inverse, tmean.z * inverse, 0.0f);
-        r[1] = make_float4(0.0f, (float)tknown, 0.0f, 0.0f);
-        r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
+        if constexpr (!kLearned) {
+            r[0] = make_float4(tmean.x * inverse, tmean.y * inverse, tmean.z * inverse, 0.0f);
+            r[1] = make_float4(0.0f, (float)tknown, 0.0f, 0.0f);
+            r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
+        }
         r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
         r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
@@ -572,6 +589,83 @@ __global__ void apply_events_kernel(const uint4 *__restrict__ events, uint32_t e
     if (urgent) meta[(size_t)slot * 12u + 11u] |= 0x40000000u;
 }
 
+// Learned mode, per face: each sample's value is its partial energy plus
+// factor * max(prediction, 0); moments are summed per lane subsequence and
+// then in lane order, as trace_faces_kernel does (voxel_trace.wgsl:72-92).
+__global__ void learned_combine_kernel(const float4 *__restrict__ learned, const float *__restrict__ pred, uint32_t count,
+                                       uint32_t samples, uint32_t lanes, const uint4 *__restrict__ tasks,
+                                       float4 *__restrict__ results) {
+    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count || tasks[f * 2u].w == NONE) return;
+    f3 tmean = mk(0.0f, 0.0f, 0.0f), tsecond = mk(0.0f, 0.0f, 0.0f);
+    uint32_t tknown = 0u;
+    for (uint32_t lane = 0u; lane < lanes; ++lane) {
+        f3 mean = mk(0.0f, 0.0f, 0.0f), second = mk(0.0f, 0.0f, 0.0f);
+        uint32_t known = 0u;
+        for (uint32_t i = lane; i < samples; i += lanes) {
+            size_t at = (size_t)f * samples + i;
+            float4 a = learned[at * 2u], b = learned[at * 2u + 1u];
+            if (a.w == 0.0f) continue;
+            const float *p = pred + at * 3u;
+            f3 value = mk(a.x, a.y, a.z) + mk(b.x, b.y, b.z) * mk(fmx(p[0], 0.0f), fmx(p[1], 0.0f), fmx(p[2], 0.0f));
+            mean = mean + value;
+            second = second + value * value;
+            known += 1u;
+        }
+        tmean = tmean + mean;
+        tsecond = tsecond + second;
+        tknown += known;
+    }
+    float inverse = 1.0f / (float)(tknown > 1u ? tknown : 1u);
+    float4 *r = results + (size_t)f * 6u;
+    r[0] = make_float4(tmean.x * inverse, tmean.y * inverse, tmean.z * inverse, 0.0f);
+    r[1] = make_float4(0.0f, (float)tknown, 0.0f, 0.0f);
+    r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
+}
+
+// Training set of a frame: a window of a fixed permutation of the bound
+// faces, as tasks (traced plain, one sample) and as cache queries.
+__global__ void train_gather_kernel(const uint4 *__restrict__ bound, const uint32_t *__restrict__ perm,
+                                    const uint32_t *__restrict__ face_material, uint32_t n_bound, uint32_t start, uint32_t count,
+                                    uint4 *__restrict__ tasks, MlpQuery *__restrict__ queries) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t f = perm[(uint32_t)(((uint64_t)start + i) % n_bound)];
+    uint4 a = bound[f * 2u], b = bound[f * 2u + 1u];
+    tasks[i * 2u] = a;
+    tasks[i * 2u + 1u] = b;
+    queries[i] = MlpQuery{(int)a.x, (int)a.y, (int)a.z, a.w == NONE ? NONE : (a.w | (face_material[f] << 8))};
+}
+
+// Targets: the traced sample's incident radiance, weight 1 when known.
+__global__ void train_targets_kernel(const float4 *__restrict__ results, uint32_t count, float4 *__restrict__ targets) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 m = results[(size_t)i * 6u];
+    float known = results[(size_t)i * 6u + 1u].y;
+    targets[i] = make_float4(m.x, m.y, m.z, known > 0.0f ? 1.0f : 0.0f);
+}
+
+// Long-horizon FP64 sums of per-launch face moments: the converged reference
+// of the error harness. Per face: sum of mean*known (rgb), sum of
+// second*known (rgb), sum of known, and the analytic direct term's luminance
+// slot left for the host.
+__global__ void fp64_accumulate_kernel(const float4 *__restrict__ results, uint32_t count, double *__restrict__ sums) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float4 *r = results + (size_t)i * 6u;
+    float4 m = r[0], k4 = r[1], q = r[2];
+    double k = (double)k4.y;
+    double *s = sums + (size_t)i * 8u;
+    s[0] += (double)m.x * k;
+    s[1] += (double)m.y * k;
+    s[2] += (double)m.z * k;
+    s[3] += (double)q.x * k;
+    s[4] += (double)q.y * k;
+    s[5] += (double)q.z * k;
+    s[6] += k;
+}
+
 // A plain streaming copy (16 B per lane), the known-size reference against
 // which hardware byte counters are scaled.
 __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
@@ -586,23 +680,56 @@ static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n +
 // Kernel variant by what the caller asked for: hit records (with full
 // counters), all counters, rays only, or nothing.
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
-                  uint32_t lanes, float4 *results, uint32_t *hits, unsigned long long *counters, bool full_counters,
-                  hipStream_t stream) {
+                  uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits, unsigned long long *counters,
+                  bool full_counters, hipStream_t stream) {
     uint64_t threads = (uint64_t)count * lanes;
     if (threads == 0) return;
     dim3 grid(blocks(threads, 256)), block(256);
+    float4 *no_l = nullptr;
+    MlpQuery *no_q = nullptr;
     if (hits)
-        hipLaunchKernelGGL((trace_faces_kernel<FullTally, true>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
-                           results, hits, counters);
+        hipLaunchKernelGGL((trace_faces_kernel<FullTally, true, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, hits, counters, no_l, no_q);
     else if (counters && full_counters)
-        hipLaunchKernelGGL((trace_faces_kernel<FullTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
-                           results, hits, counters);
+        hipLaunchKernelGGL((trace_faces_kernel<FullTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, hits, counters, no_l, no_q);
     else if (counters)
-        hipLaunchKernelGGL((trace_faces_kernel<RayTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
-                           results, hits, counters);
+        hipLaunchKernelGGL((trace_faces_kernel<RayTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, hits, counters, no_l, no_q);
     else
-        hipLaunchKernelGGL((trace_faces_kernel<NoTally, false>), grid, block, 0, stream, scene, optics, tasks, count, samples, lanes,
-                           results, hits, counters);
+        hipLaunchKernelGGL((trace_faces_kernel<NoTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, hits, counters, no_l, no_q);
+}
+
+void launch_trace_learned(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
+                          uint32_t lanes, uint32_t sequence_base, float4 *results, unsigned long long *counters, float4 *learned,
+                          MlpQuery *queries, hipStream_t stream) {
+    uint64_t threads = (uint64_t)count * lanes;
+    if (threads == 0) return;
+    dim3 grid(blocks(threads, 256)), block(256);
+    if (counters)
+        hipLaunchKernelGGL((trace_faces_kernel<RayTally, false, true>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, nullptr, counters, learned, queries);
+    else
+        hipLaunchKernelGGL((trace_faces_kernel<NoTally, false, true>), grid, block, 0, stream, scene, optics, tasks, count, samples,
+                           lanes, sequence_base, results, nullptr, counters, learned, queries);
+}
+
+void launch_learned_combine(const float4 *learned, const float *pred, uint32_t count, uint32_t samples, uint32_t lanes,
+                            const uint4 *tasks, float4 *results, hipStream_t stream) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(learned_combine_kernel, dim3(blocks(count, 256)), dim3(256), 0, stream, learned, pred, count, samples, lanes,
+                       tasks, results);
+}
+void launch_train_gather(const uint4 *bound, const uint32_t *perm, const uint32_t *face_material, uint32_t n_bound, uint32_t start,
+                         uint32_t count, uint4 *tasks, MlpQuery *queries, hipStream_t stream) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(train_gather_kernel, dim3(blocks(count, 256)), dim3(256), 0, stream, bound, perm, face_material, n_bound, start,
+                       count, tasks, queries);
+}
+void launch_train_targets(const float4 *results, uint32_t count, float4 *targets, hipStream_t stream) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(train_targets_kernel, dim3(blocks(count, 256)), dim3(256), 0, stream, results, count, targets);
 }
 void launch_classify(const uint16_t *grid, int gx, int gy, int gz, int vx, int vy, int vz, const int4 *list, uint32_t n,
                      RegionInfo *info, hipStream_t stream) {
@@ -646,6 +773,11 @@ void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t hig
     if (high_water == 0) return;
     hipLaunchKernelGGL(apply_events_kernel, dim3(blocks(high_water, 256)), dim3(256), 0, stream, events, event_count, high_water,
                        state, checkpoint, meta, physical, 192u * 192u, 64u * 64u, 48u * 48u);
+}
+
+void launch_fp64_accumulate(const float4 *results, uint32_t count, double *sums, hipStream_t stream) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(fp64_accumulate_kernel, dim3(blocks(count, 256)), dim3(256), 0, stream, results, count, sums);
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream) {

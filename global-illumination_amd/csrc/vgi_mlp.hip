@@ -1,0 +1,358 @@
+// Learned radiance cache kernels for gfx950: BF16 MFMA (16x16x16, FP32
+// accumulate), wave64.
+//
+// Formulation: each wavefront carries 16 samples and computes Y^T = W X^T, so
+// a 16x16 MFMA's A operand is a weight tile and B the activations. With the
+// 16x16x16 lane layout (lane l: column n = l % 16; rows/k = 4 (l / 16) + j for
+// j = 0..3) a layer's accumulator fragment for output tile t is exactly the
+// B fragment of k-step t of the next layer, so activations chain through the
+// five layers in registers. Weights sit in LDS as pre-packed fragments.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "vgi_math.h"
+#include "vgi_mlp.h"
+
+namespace vgi {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest-even float -> bf16 bits (no NaNs arise here).
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ f32x4 mfma(s16x4 a, s16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ s16x4 pack4(float a, float b, float c, float d) {
+    s16x4 r;
+    r.x = (short)to_bf16(a);
+    r.y = (short)to_bf16(b);
+    r.z = (short)to_bf16(c);
+    r.w = (short)to_bf16(d);
+    return r;
+}
+
+// Fragment tile indices (see vgi_mlp.h).
+__device__ __host__ constexpr int fwd_tile(int layer, int t, int s) {
+    return layer == 1 ? t * 2 + s : (layer <= 4 ? 8 + 16 * (layer - 2) + t * 4 + s : 56 + s);
+}
+__device__ __host__ constexpr int bwd_tile(int layer, int t, int s) {
+    return MLP_FWD_TILES + (layer <= 4 ? 16 * (layer - 2) + t * 4 + s : 48 + t);
+}
+
+// Packs FP32 master weights into BF16 A-operand fragments: forward tile (t, s)
+// lane l value j = W[16t + l%16][16s + 4(l/16) + j]; the transposed (backward)
+// tile (t, s) = W[16s + 4(l/16) + j][16t + l%16]. Rows beyond the layer's
+// outputs are zero.
+__global__ void mlp_pack_kernel(const float *__restrict__ w, uint16_t *__restrict__ frags) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= MLP_FRAG * 256) return;
+    int tile = e >> 8, lane = (e >> 2) & 63, j = e & 3;
+    int col16 = lane & 15, k4 = 4 * (lane >> 4) + j;
+    int layer, t, s;
+    bool transposed = tile >= MLP_FWD_TILES;
+    int local = transposed ? tile - MLP_FWD_TILES : tile;
+    if (!transposed) {
+        if (local < 8) { layer = 1; t = local / 2; s = local % 2; }
+        else if (local < 56) { layer = 2 + (local - 8) / 16; t = ((local - 8) % 16) / 4; s = (local - 8) % 4; }
+        else { layer = 5; t = 0; s = local - 56; }
+    } else {
+        if (local < 48) { layer = 2 + local / 16; t = (local % 16) / 4; s = local % 4; }
+        else { layer = 5; t = local - 48; s = 0; }
+    }
+    int in = layer == 1 ? MLP_IN : MLP_HIDDEN;
+    int out = layer == 5 ? MLP_OUT : MLP_HIDDEN;
+    int base = layer == 1 ? MLP_W1 : layer == 2 ? MLP_W2 : layer == 3 ? MLP_W3 : layer == 4 ? MLP_W4 : MLP_W5;
+    int row = transposed ? 16 * s + k4 : 16 * t + col16;
+    int col = transposed ? 16 * t + col16 : 16 * s + k4;
+    float v = (row < out && col < in) ? w[base + row * in + col] : 0.0f;
+    frags[e] = to_bf16(v);
+}
+
+// The 32 input features of a query (frequency encoding of the face centre,
+// the face normal, the albedo, a constant).
+__device__ __forceinline__ float encode(const MlpScene &sc, const MlpQuery &q, int f) {
+    uint32_t face = q.face_material & 0xffu, material = (q.face_material >> 8) & 0xffu;
+    f3 n = face_normal(face);
+    if (f < 24) {
+        int axis = f >> 3, i = (f & 7) >> 1;
+        float c = axis == 0 ? (float)q.vx + 0.5f + n.x * 0.5f : axis == 1 ? (float)q.vy + 0.5f + n.y * 0.5f
+                                                                          : (float)q.vz + 0.5f + n.z * 0.5f;
+        float lo = axis == 0 ? sc.lo_x : axis == 1 ? sc.lo_y : sc.lo_z;
+        float u = (c - lo) * sc.inv_extent;
+        float x = u * (3.14159265358979f * (float)(1 << (2 * i)));
+        float sv, cv;
+        sincos_own(x, sv, cv);
+        return (f & 1) ? cv : sv;
+    }
+    if (f < 27) return get(n, f - 24);
+    if (f < 30) {
+        float4 d = sc.materials[material * 2u];
+        return f == 27 ? d.x : f == 28 ? d.y : d.z;
+    }
+    return f == 30 ? 1.0f : 0.0f;
+}
+
+// Forward of one 16-sample tile held by a wavefront. x0: input fragments;
+// h: per-layer ReLU masks (bit (t*4+j) of masks[layer-1]); returns the output
+// accumulator (lanes 0..15 hold outputs 0..2 of their sample in j = 0..2).
+__device__ __forceinline__ f32x4 mlp_forward_tile(const s16x4 *__restrict__ lds, s16x4 x0[2], uint32_t masks[4], s16x4 acts[4][4]) {
+    const int lane = threadIdx.x & 63;
+    f32x4 acc[4];
+    // layer 1 (K = 32)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc[t] = mfma(lds[fwd_tile(1, t, s) * 64 + lane], x0[s], acc[t]);
+    }
+    s16x4 h[4];
+#pragma unroll
+    for (int layer = 2; layer <= 5; ++layer) {
+        uint32_t mask = 0u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float a = fmaxf(acc[t].x, 0.0f), b = fmaxf(acc[t].y, 0.0f), c = fmaxf(acc[t].z, 0.0f), d = fmaxf(acc[t].w, 0.0f);
+            mask |= (acc[t].x > 0.0f ? 1u : 0u) << (t * 4) | (acc[t].y > 0.0f ? 1u : 0u) << (t * 4 + 1) |
+                    (acc[t].z > 0.0f ? 1u : 0u) << (t * 4 + 2) | (acc[t].w > 0.0f ? 1u : 0u) << (t * 4 + 3);
+            h[t] = pack4(a, b, c, d);
+            acts[layer - 2][t] = h[t];
+        }
+        masks[layer - 2] = mask;
+        if (layer < 5) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) acc[t] = mfma(lds[fwd_tile(layer, t, s) * 64 + lane], h[s], acc[t]);
+            }
+        } else {
+            acc[0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc[0] = mfma(lds[fwd_tile(5, 0, s) * 64 + lane], h[s], acc[0]);
+        }
+    }
+    return acc[0];
+}
+
+__device__ __forceinline__ void load_input(const MlpScene &sc, const MlpQuery &q, bool valid, s16x4 x0[2]) {
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = valid ? encode(sc, q, 16 * s + 4 * g + j) : 0.0f;
+        x0[s] = pack4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ frags, s16x4 *lds, int tiles) {
+    const s16x4 *src = reinterpret_cast<const s16x4 *>(frags);
+    for (int i = threadIdx.x; i < tiles * 64; i += blockDim.x) lds[i] = src[i];
+    __syncthreads();
+}
+
+// Inference over `count` queries: persistent blocks loop over 16-query tiles
+// (one per wavefront); invalid queries give 0.
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t *__restrict__ frags, MlpScene sc,
+                                                        const MlpQuery *__restrict__ queries, uint32_t count,
+                                                        float *__restrict__ pred) {
+    __shared__ s16x4 lds[MLP_FWD_TILES * 64];
+    stage_weights(frags, lds, MLP_FWD_TILES);
+    const int lane = threadIdx.x & 63;
+    const uint32_t tiles = (count + 15u) / 16u;
+    const uint32_t wave = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+    for (uint32_t tile = wave; tile < tiles; tile += waves) {
+        uint32_t n = tile * 16u + (uint32_t)(lane & 15);
+        MlpQuery q = {0, 0, 0, NONE};
+        if (n < count) q = queries[n];
+        bool valid = n < count && q.face_material != NONE;
+        s16x4 x0[2];
+        load_input(sc, q, valid, x0);
+        uint32_t masks[4];
+        s16x4 acts[4][4];
+        f32x4 out = mlp_forward_tile(lds, x0, masks, acts);
+        if (lane < 16 && n < count) {
+            float *p = pred + (size_t)n * 3u;
+            p[0] = valid ? out.x : 0.0f;
+            p[1] = valid ? out.y : 0.0f;
+            p[2] = valid ? out.z : 0.0f;
+        }
+    }
+}
+
+// Training: forward, relative L2 loss (pred - y)^2 / (pred^2 + 0.01) with the
+// denominator held constant, and the backward data pass, one tile per wave.
+// Saves X_i (layer inputs, bf16, sample-major rows) and dY_i (layer output
+// gradients, bf16) for the weight-gradient GEMM.
+// acts layout: layer 1 [n][32] at 0, layers 2..5 [n][64] at 32N + 64N (i-2);
+// grads layout: layers 1..4 [n][64] at 64N (i-1), layer 5 [n][16] at 256N.
+__global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__restrict__ frags, MlpScene sc,
+                                                          const MlpQuery *__restrict__ queries, const float *__restrict__ targets,
+                                                          uint32_t count, uint16_t *__restrict__ acts_out,
+                                                          uint16_t *__restrict__ grads_out, float *__restrict__ loss,
+                                                          float *__restrict__ pred) {
+    __shared__ s16x4 lds[MLP_FRAG * 64];
+    stage_weights(frags, lds, MLP_FRAG);
+    const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    const uint32_t tile = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    if (tile * 16u >= count) return;
+    const uint32_t n = tile * 16u + (uint32_t)col;
+    const size_t N = count;
+    MlpQuery q = queries[n];
+    bool valid = q.face_material != NONE;
+    s16x4 x0[2];
+    load_input(sc, q, valid, x0);
+    uint32_t masks[4];
+    s16x4 acts[4][4];
+    f32x4 out = mlp_forward_tile(lds, x0, masks, acts);
+    // Save layer inputs (8 bytes per lane per 16x16 fragment row group).
+    for (int s = 0; s < 2; ++s) *reinterpret_cast<s16x4 *>(acts_out + (size_t)n * 32 + 16 * s + 4 * g) = x0[s];
+    for (int layer = 2; layer <= 5; ++layer)
+        for (int t = 0; t < 4; ++t)
+            *reinterpret_cast<s16x4 *>(acts_out + 32 * N + 64 * N * (layer - 2) + (size_t)n * 64 + 16 * t + 4 * g) = acts[layer - 2][t];
+    // Loss gradient on lanes 0..15 (rows 0..2 of the output tile).
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, l = 0.0f;
+    if (g == 0) {
+        const float *y = targets + (size_t)n * 4u;
+        float w = y[3] / (float)count;
+        float p[3] = {out.x, out.y, out.z};
+        float dd[3];
+        for (int c = 0; c < 3; ++c) {
+            float den = p[c] * p[c] + 0.01f;
+            float r = p[c] - y[c];
+            l += w * r * r / den;
+            dd[c] = w * 2.0f * r / den;
+        }
+        d0 = dd[0]; d1 = dd[1]; d2 = dd[2];
+        float *pp = pred + (size_t)n * 3u;
+        pp[0] = p[0]; pp[1] = p[1]; pp[2] = p[2];
+    }
+    // Wave sum of the loss, one atomic per wave.
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+    if (lane == 0) atomicAdd(loss, l);
+    s16x4 dy5 = pack4(d0, d1, d2, 0.0f);  // rows 4g+j: nonzero only for g = 0
+    *reinterpret_cast<s16x4 *>(grads_out + 256 * N + (size_t)n * 16 + 4 * g) = dy5;
+    // dH4^T = W5^T dY5^T, masked by layer 4's ReLU.
+    s16x4 dy[4];
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = mfma(lds[bwd_tile(5, t, 0) * 64 + lane], dy5, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    for (int layer = 4; layer >= 1; --layer) {
+        uint32_t mask = masks[layer - 1];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float a = (mask >> (t * 4)) & 1u ? acc[t].x : 0.0f;
+            float b = (mask >> (t * 4 + 1)) & 1u ? acc[t].y : 0.0f;
+            float c = (mask >> (t * 4 + 2)) & 1u ? acc[t].z : 0.0f;
+            float d = (mask >> (t * 4 + 3)) & 1u ? acc[t].w : 0.0f;
+            dy[t] = pack4(a, b, c, d);
+            *reinterpret_cast<s16x4 *>(grads_out + 64 * N * (layer - 1) + (size_t)n * 64 + 16 * t + 4 * g) = dy[t];
+        }
+        if (layer > 1) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) acc[t] = mfma(lds[bwd_tile(layer, t, s) * 64 + lane], dy[s], acc[t]);
+            }
+        }
+    }
+}
+
+// dW_i[o][in] = sum_n dY_i[n][o] X_i[n][in]: one wavefront per (weight tile,
+// chunk of 1024 samples); A = dY^T (o x samples), B = X (samples x in).
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const uint16_t *__restrict__ acts, const uint16_t *__restrict__ grads,
+                                                        uint32_t count, float *__restrict__ grad) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    const uint32_t wave = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    const uint32_t chunks = (count + 1023u) / 1024u;
+    const uint32_t tile = wave / chunks, chunk = wave % chunks;
+    if (tile >= (uint32_t)MLP_FWD_TILES) return;
+    int layer, t, s;
+    if (tile < 8) { layer = 1; t = tile / 2; s = tile % 2; }
+    else if (tile < 56) { layer = 2 + (tile - 8) / 16; t = ((tile - 8) % 16) / 4; s = (tile - 8) % 4; }
+    else { layer = 5; t = 0; s = tile - 56; }
+    const size_t N = count;
+    const int in = layer == 1 ? MLP_IN : MLP_HIDDEN;
+    const int gw = layer == 5 ? 16 : 64;  // row length of dY_i
+    const uint16_t *X = acts + (layer == 1 ? 0 : 32 * N + 64 * N * (layer - 2));
+    const uint16_t *DY = grads + (layer == 5 ? 256 * N : 64 * N * (layer - 1));
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t begin = chunk * 1024u, end = min(count, begin + 1024u);
+    for (uint32_t n0 = begin; n0 < end; n0 += 16u) {
+        s16x4 a, b;
+        short av[4], bv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            size_t n = n0 + 4 * g + j;
+            av[j] = (short)DY[n * gw + 16 * t + col];
+            bv[j] = (short)X[n * in + 16 * s + col];
+        }
+        a.x = av[0]; a.y = av[1]; a.z = av[2]; a.w = av[3];
+        b.x = bv[0]; b.y = bv[1]; b.z = bv[2]; b.w = bv[3];
+        acc = mfma(a, b, acc);
+    }
+    // C layout: lane holds rows o = 16t + 4g + j, column in = 16s + col.
+    const int out = layer == 5 ? MLP_OUT : MLP_HIDDEN;
+    const int base = layer == 1 ? MLP_W1 : layer == 2 ? MLP_W2 : layer == 3 ? MLP_W3 : layer == 4 ? MLP_W4 : MLP_W5;
+    float v[4] = {acc.x, acc.y, acc.z, acc.w};
+    for (int j = 0; j < 4; ++j) {
+        int o = 16 * t + 4 * g + j;
+        if (o < out) atomicAdd(&grad[base + o * in + 16 * s + col], v[j]);
+    }
+}
+
+// Adam (Kingma & Ba 2015), beta1 0.9, beta2 0.99, eps 1e-10; clears the
+// gradient.
+__global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, float *__restrict__ g,
+                                float lr, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= MLP_PARAMS) return;
+    float gi = g[i];
+    float mi = 0.9f * m[i] + 0.1f * gi;
+    float vi = 0.99f * v[i] + 0.01f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-10f);
+    g[i] = 0.0f;
+}
+
+void mlp_pack(const float *master, uint16_t *frags, hipStream_t stream) {
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((MLP_FRAG * 256 + 255) / 256), dim3(256), 0, stream, master, frags);
+}
+void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, uint32_t count, float *pred,
+               hipStream_t stream) {
+    if (count == 0) return;
+    uint32_t tiles = (count + 15u) / 16u;
+    uint32_t blocks = std::min<uint32_t>((tiles + 3u) / 4u, 4096u);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, stream, frags, scene, queries, count, pred);
+}
+void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, const float *targets,
+                          uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, hipStream_t stream) {
+    if (count == 0) return;
+    uint32_t tiles = count / 16u;
+    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((tiles + 3u) / 4u), dim3(256), 0, stream, frags, scene, queries, targets, count,
+                       acts, grads, loss, pred);
+}
+void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream) {
+    if (count == 0) return;
+    uint32_t chunks = (count + 1023u) / 1024u;
+    uint32_t waves = (uint32_t)MLP_FWD_TILES * chunks;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, stream, acts, grads, count, grad);
+}
+void mlp_adam(float *master, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream) {
+    float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, stream, master, m, v, grad, lr, c1, c2);
+}
+
+}  // namespace vgi

@@ -497,10 +497,23 @@ struct FirstHit {
     uint32_t face, status, material, slot;
 };
 
+// Learned mode: where the path stopped into the cache, and the factor that
+// multiplies the cache's predicted incident radiance there.
+struct CacheQuery {
+    int vx, vy, vz;
+    uint32_t face_material;  // NONE: the path ended without a query
+    f3 factor;
+};
+
 // voxel_path.wgsl:418-496 (begin_path + advance_path with terminal = true)
-template <class Tally>
+// kLearned: the path ends at its first explicit hit into the learned cache
+// (the world-cache termination rule of voxel_path.wgsl:459-463 with the
+// cached value replaced by the network's prediction): the returned energy is
+// everything but the prediction term, whose factor goes to `query`.
+template <class Tally, bool kLearned = false>
 __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3 start, f3 first_direction, f3 initial_normal,
-                                         uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally) {
+                                         uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally,
+                                         CacheQuery *query = nullptr) {
     f3 energy = mk(0.0f, 0.0f, 0.0f);
     unknown = false;
     f3 throughput = mk(1.0f, 1.0f, 1.0f);
@@ -541,6 +554,13 @@ __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3
         float light_pdf = selection_pdf * dot3(delta, delta) / fmx(dot3(hit.normal, -direction), 1e-8f);
         float mis = 1.0f;
         if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
+        if constexpr (kLearned) {
+            energy = energy + throughput * (emission * mis + diffuse * dv);
+            query->vx = hit.vx; query->vy = hit.vy; query->vz = hit.vz;
+            query->face_material = face_index_of(hit.normal) | (hit.material << 8);
+            query->factor = throughput * diffuse;
+            break;
+        }
         f3 cached;
         uint32_t cslot;
         bool found = cached_energy(o, hit.vx, hit.vy, hit.vz, face_index_of(hit.normal), cached, cslot);
@@ -575,9 +595,10 @@ __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3
 }
 
 // voxel_path.wgsl:504-520
-template <class Tally>
+template <class Tally, bool kLearned = false>
 __device__ __forceinline__ f3 trace_receiver(const SceneView &s, const Optics &o, f3 surface, f3 normal, uint32_t stream,
-                                             uint32_t sequence, bool &known, FirstHit &first, Tally &tally) {
+                                             uint32_t sequence, bool &known, FirstHit &first, Tally &tally,
+                                             CacheQuery *query = nullptr) {
     f3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));
     f3 value = sample_emitter(s, o, surface, normal, stream, sequence, 0u, tally);
     bool unknown = false;
@@ -587,7 +608,8 @@ __device__ __forceinline__ f3 trace_receiver(const SceneView &s, const Optics &o
     first.material = 0u;
     first.slot = NONE;
     if (dot3(normal, direction) > 0.0f) {
-        f3 c = trace_path(s, o, surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first, tally);
+        f3 c = trace_path<Tally, kLearned>(s, o, surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first,
+                                           tally, query);
         value = value + c;
     }
     known = !unknown;

@@ -139,6 +139,38 @@ int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uin
 /* Raw resident state (9 uint4 per slot, voxel_accumulate.wgsl:1-4) of slots [first, first+count). */
 int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 
+/* ---- error harness: the long-horizon reference. Traces the bound faces
+ * `launches` times (sequences first_sequence + l * samples) with plain
+ * multi-bounce paths (no cache termination) and sums each face's per-launch
+ * moments in FP64 on the GPU: out = 8 doubles per bound face (sum of mean*known
+ * rgb, sum of second*known rgb, sum of known, 0). The analytic direct term is
+ * deterministic and read from any record. ---- */
+int vgi_converged(vgi_tracer *tracer, uint32_t launches, uint32_t first_sequence, double *out);
+
+/* ---- learned radiance cache (no counterpart in the reference): a
+ * 32-64-64-64-64-3 ReLU MLP (BF16 MFMA, FP32 accumulation, Adam) that predicts
+ * a face's incident radiance from a 32-dim frequency encoding of its centre,
+ * its normal and albedo. With it on, bound-face frames end every path at its
+ * first hit into the network (the world-cache termination rule of
+ * voxel_path.wgsl:459-463 with the prediction as the cached value) and train
+ * it on `train_per_frame` plain multi-bounce samples per frame in `steps`
+ * Adam steps. Queries are 4 u32 (voxel xyz, face | material << 8). ---- */
+int vgi_mlp_enable(vgi_tracer *tracer, uint32_t seed, uint32_t train_per_frame, uint32_t steps, float lr);
+int vgi_mlp_set_learned(vgi_tracer *tracer, int on);
+/* Read (out) and/or write (in) the 14,528 FP32 master weights: W1 64x32,
+ * W2..W4 64x64, W5 3x64, row-major (out x in), in that order. */
+int vgi_mlp_weights(vgi_tracer *tracer, float *out, const float *in);
+int vgi_mlp_infer(vgi_tracer *tracer, const uint32_t *queries, uint32_t count, float *out_rgb);
+/* One forward/backward on (query, target rgb + weight) pairs, count a multiple
+ * of 16; update 0 returns the summed gradient (relative L2 loss), 1 takes an
+ * Adam step. */
+int vgi_mlp_train(vgi_tracer *tracer, const uint32_t *queries, const float *targets, uint32_t count, int update,
+                  float *grad_out, float *loss_out);
+/* Training-only frames on the bound faces (targets traced, Adam steps taken,
+ * bound faces not traced): the network's warm-up. */
+int vgi_mlp_pretrain(vgi_tracer *tracer, uint32_t frames);
+float vgi_mlp_last_loss(vgi_tracer *tracer);
+
 /* ---- retention events (resident.rs:1326-1417 encode_events layout;
  * voxel_retention.wgsl:29-96, apply_events voxel_accumulate.wgsl:295-332) ---- */
 int vgi_apply_events(vgi_tracer *tracer, const uint32_t *events, uint32_t event_count, uint32_t words);

@@ -1848,6 +1848,70 @@ static void apply_events(Tracer &t, const uint32_t *events, uint32_t event_count
         if (urgent) t.meta[(size_t)slot * 12 + 11] |= URGENT;
     }
 }
+
+// ---------------------------------------------------------------- learned cache oracle (FP64)
+// The network of the MLP radiance cache (32-64-64-64-64-3, ReLU, no biases),
+// evaluated in double precision from the same BF16-rounded weights, inputs
+// and inter-layer activations the GPU uses: the checker of its MFMA forward
+// and of its gradients. Relative L2 loss (p - y)^2 / (p^2 + 0.01), mean over
+// the batch, weighted per sample.
+static inline float bf16_round(float f) {
+    uint32_t u = fbits(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return bitsf((u >> 16) << 16);
+}
+static const int MLP_OFF[6] = {0, 2048, 6144, 10240, 14336, 14528};
+static const int MLP_INS[5] = {32, 64, 64, 64, 64};
+static const int MLP_OUTS[5] = {64, 64, 64, 64, 3};
+
+static float mlp_encode(const uint32_t *q, int f, const float *diffuse, const float scene[4]) {
+    uint32_t face = q[3] & 0xffu, material = (q[3] >> 8) & 0xffu;
+    F3 n = face_normal(face);
+    if (f < 24) {
+        int axis = f >> 3, i = (f & 7) >> 1;
+        float v = (float)(int32_t)q[axis];
+        float c = v + 0.5f + n[axis] * 0.5f;
+        float u = (c - scene[axis]) * scene[3];
+        float x = u * (3.14159265358979f * (float)(1 << (2 * i)));
+        int k;
+        float r = reduce_pio2(x, &k);
+        float sp = sin_poly(r), cp = cos_poly(r);
+        float sv, cv;
+        switch (k & 3) {
+        case 0: sv = sp; cv = cp; break;
+        case 1: sv = cp; cv = -sp; break;
+        case 2: sv = -sp; cv = -cp; break;
+        default: sv = -cp; cv = sp; break;
+        }
+        return (f & 1) ? cv : sv;
+    }
+    if (f < 27) return n[f - 24];
+    if (f < 30) return diffuse[material * 3 + (f - 27)];
+    return f == 30 ? 1.0f : 0.0f;
+}
+
+struct MlpTrace {
+    double x[5][64];      // layer inputs (bf16-valued)
+    double pre[5][64];    // pre-activations
+    double out[3];
+};
+
+static void mlp_forward(const std::vector<float> &wb, const uint32_t *q, const float *diffuse, const float scene[4], MlpTrace &tr) {
+    bool valid = q[3] != NONE;
+    for (int f = 0; f < 32; ++f) tr.x[0][f] = valid ? (double)bf16_round(mlp_encode(q, f, diffuse, scene)) : 0.0;
+    for (int l = 0; l < 5; ++l) {
+        int in = MLP_INS[l], out = MLP_OUTS[l];
+        for (int o = 0; o < out; ++o) {
+            double a = 0.0;
+            for (int k = 0; k < in; ++k) a += (double)wb[MLP_OFF[l] + o * in + k] * tr.x[l][k];
+            tr.pre[l][o] = a;
+        }
+        if (l < 4)
+            for (int o = 0; o < out; ++o) tr.x[l + 1][o] = (double)bf16_round(tr.pre[l][o] > 0.0 ? (float)tr.pre[l][o] : 0.0f);
+    }
+    for (int c = 0; c < 3; ++c) tr.out[c] = tr.pre[4][c];
+}
+
 }  // namespace
 
 // ================================================================ C API
@@ -2114,6 +2178,54 @@ int orc_cache_apply_events(void *tp, const uint32_t *events, uint32_t event_coun
 }
 
 uint32_t orc_curve_limit(uint32_t d2) { return curve_limit(d2); }
+
+int orc_mlp_forward(const float *w, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out) {
+    std::vector<float> wb(14528);
+    for (int i = 0; i < 14528; ++i) wb[i] = bf16_round(w[i]);
+    for (uint32_t i = 0; i < n; ++i) {
+        MlpTrace tr;
+        mlp_forward(wb, &queries[i * 4], diffuse, scene, tr);
+        for (int c = 0; c < 3; ++c) out[i * 3 + c] = queries[i * 4 + 3] == NONE ? 0.0 : tr.out[c];
+    }
+    return 0;
+}
+
+int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
+                     const float scene[4], double *grad, double *loss) {
+    std::vector<float> wb(14528);
+    for (int i = 0; i < 14528; ++i) wb[i] = bf16_round(w[i]);
+    for (int i = 0; i < 14528; ++i) grad[i] = 0.0;
+    *loss = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        MlpTrace tr;
+        mlp_forward(wb, &queries[i * 4], diffuse, scene, tr);
+        const float *y = &targets[i * 4];
+        double wgt = (double)y[3] / (double)n;
+        double dy[5][64];
+        for (int c = 0; c < 3; ++c) {
+            double p = (double)(float)tr.out[c];
+            double den = p * p + 0.01;
+            double r = p - (double)y[c];
+            *loss += wgt * r * r / den;
+            dy[4][c] = (double)bf16_round((float)(wgt * 2.0 * r / den));
+        }
+        for (int l = 4; l >= 1; --l) {
+            int in = MLP_INS[l], out = MLP_OUTS[l];
+            for (int k = 0; k < in; ++k) {
+                double a = 0.0;
+                for (int o = 0; o < out; ++o) a += (double)wb[MLP_OFF[l] + o * in + k] * dy[l][o];
+                dy[l - 1][k] = (double)bf16_round(tr.pre[l - 1][k] > 0.0 ? (float)a : 0.0f);
+            }
+        }
+        for (int l = 0; l < 5; ++l) {
+            int in = MLP_INS[l], out = MLP_OUTS[l];
+            for (int o = 0; o < out; ++o)
+                for (int k = 0; k < in; ++k) grad[MLP_OFF[l] + o * in + k] += dy[l][o] * tr.x[l][k];
+        }
+    }
+    return 0;
+}
+
 
 // One trace_domain_limited call (voxel_traversal.wgsl:126-219, visible_geometry
 // false, 2048 steps): out_i = voxel xyz, material, status, face; out_f = position, normal.

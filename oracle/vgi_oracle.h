@@ -128,6 +128,15 @@ uint32_t orc_curve_limit(uint32_t d2);
 int orc_trace_ray(void *tracer, const float origin[3], const float direction[3], float max_distance, int32_t *out_i,
                   float *out_f);
 
+/* ---- learned cache (FP64 checker of the BF16 MFMA network): weights are the
+ * 14,528 FP32 master values (W1 64x32, W2..W4 64x64, W5 3x64 row-major);
+ * queries 4 u32 (voxel xyz, face | material << 8); scene = (lo xyz, 1/extent);
+ * targets 4 floats (rgb, weight). ---- */
+int orc_mlp_forward(const float *w, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4],
+                    double *out);
+int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
+                     const float scene[4], double *grad, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,8 @@ def lib():
         "orc_cache_apply_events": (ctypes.c_int, [P, P, u32, u32]),
         "orc_curve_limit": (u32, [u32]),
         "orc_trace_ray": (ctypes.c_int, [P, P, P, f32, P, P]),
+        "orc_mlp_forward": (ctypes.c_int, [P, P, u32, P, P, P]),
+        "orc_mlp_gradient": (ctypes.c_int, [P, P, P, u32, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -222,3 +224,25 @@ class Tracer:
     def apply_events(self, events, event_count):
         events = np.ascontiguousarray(events, np.uint32).reshape(-1)
         lib().orc_cache_apply_events(self.h, ptr(events), event_count, events.size)
+
+
+def mlp_forward(weights, queries, diffuse, scene):
+    w = np.ascontiguousarray(weights, np.float32)
+    q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+    d = np.ascontiguousarray(diffuse, np.float32)
+    sc = np.ascontiguousarray(scene, np.float32)
+    out = np.zeros((len(q), 3), np.float64)
+    lib().orc_mlp_forward(ptr(w), ptr(q), len(q), ptr(d), ptr(sc), ptr(out))
+    return out
+
+
+def mlp_gradient(weights, queries, targets, diffuse, scene):
+    w = np.ascontiguousarray(weights, np.float32)
+    q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+    y = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
+    d = np.ascontiguousarray(diffuse, np.float32)
+    sc = np.ascontiguousarray(scene, np.float32)
+    grad = np.zeros(14528, np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().orc_mlp_gradient(ptr(w), ptr(q), ptr(y), len(q), ptr(d), ptr(sc), ptr(grad), ptr(loss))
+    return grad, float(loss[0])

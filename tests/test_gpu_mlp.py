@@ -1,0 +1,132 @@
+"""The learned radiance cache on the GPU: its BF16 MFMA forward and gradients
+against the FP64 CPU oracle, and what the trained network gains over the
+plain multi-bounce method at equal frames.
+
+Tolerances: BF16 operands are identical on both sides and products are exact
+in FP32; the GPU accumulates in FP32 and rounds inter-layer activations to
+BF16 from FP32 where the oracle rounds from FP64, so an activation can differ
+by one BF16 ulp (2^-8 relative). Outputs and gradients are therefore checked
+to 2% relative (per layer, in norm), not bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_tracer, make_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_of(g):
+    b = g.bounds()
+    extent = max(b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1) * 16
+    return np.array([b[0] * 16, b[1] * 16, b[2] * 16, 1.0 / extent], np.float32)
+
+
+def random_queries(vgi, setup, n, rng):
+    faces = vgi.exposed_faces(setup["grid"])
+    pick = faces[rng.choice(len(faces), n, replace=False)]
+    q = np.zeros((n, 4), np.uint32)
+    q[:, 0:3] = pick["voxel"].astype(np.uint32)
+    mats = setup["grid"][pick["voxel"][:, 2], pick["voxel"][:, 1], pick["voxel"][:, 0]].astype(np.uint32)
+    q[:, 3] = pick["face"] | (mats << 8)
+    return q
+
+
+LAYERS = [(0, 2048), (2048, 6144), (6144, 10240), (10240, 14336), (14336, 14528)]
+
+
+def test_mlp_forward_matches_fp64_oracle(vgi, orc):
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    g = gpu_tracer(vgi, setup)
+    g.mlp_enable(seed=7)
+    rng = np.random.default_rng(3)
+    q = random_queries(vgi, setup, 4096, rng)
+    q[::97, 3] = 0xFFFFFFFF  # invalid queries answer 0
+    w = g.mlp_weights()
+    gpu = g.mlp_infer(q)
+    cpu = orc.mlp_forward(w, q, setup["diffuse"], scene_of(g))
+    scale = np.abs(cpu).mean()
+    assert scale > 1e-3
+    err = np.abs(gpu - cpu).max() / scale
+    print("forward max error / mean |out| = %.3g" % err, flush=True)
+    assert err < 2e-2
+    assert np.all(gpu[::97] == 0.0)
+    g.close()
+
+
+def test_mlp_gradient_matches_fp64_oracle(vgi, orc):
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    g = gpu_tracer(vgi, setup)
+    g.mlp_enable(seed=11)
+    rng = np.random.default_rng(4)
+    q = random_queries(vgi, setup, 512, rng)
+    targets = np.zeros((512, 4), np.float32)
+    targets[:, :3] = rng.random((512, 3)).astype(np.float32) * 2.0
+    targets[:, 3] = (rng.random(512) > 0.1).astype(np.float32)
+    w = g.mlp_weights()
+    grad, loss = g.mlp_train(q, targets, update=False)
+    ref, ref_loss = orc.mlp_gradient(w, q, targets, setup["diffuse"], scene_of(g))
+    print("loss gpu %.6f oracle %.6f" % (loss, ref_loss), flush=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss) + 1e-6
+    for a, b in LAYERS:
+        e = np.linalg.norm(grad[a:b] - ref[a:b]) / max(np.linalg.norm(ref[a:b]), 1e-30)
+        cos = np.dot(grad[a:b], ref[a:b]) / (np.linalg.norm(grad[a:b]) * np.linalg.norm(ref[a:b]) + 1e-30)
+        print("layer [%d,%d): relative error %.3g cosine %.6f" % (a, b, e, cos), flush=True)
+        assert e < 2e-2 and cos > 0.9995
+    # A second identical call returns the same gradient (the buffer is cleared).
+    grad2, _ = g.mlp_train(q, targets, update=False)
+    assert np.allclose(grad, grad2, rtol=1e-5, atol=1e-9)
+    g.close()
+
+
+def test_adam_steps_reduce_the_loss(vgi, orc):
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    g = gpu_tracer(vgi, setup)
+    g.mlp_enable(seed=5, lr=1e-2)
+    rng = np.random.default_rng(9)
+    q = random_queries(vgi, setup, 1024, rng)
+    targets = np.zeros((1024, 4), np.float32)
+    targets[:, :3] = (0.2 + 0.5 * (q[:, 1:2] % 7) / 7.0).astype(np.float32)  # a learnable field of the height
+    targets[:, 3] = 1.0
+    _, first = g.mlp_train(q, targets, update=True)
+    for _ in range(200):
+        _, last = g.mlp_train(q, targets, update=True)
+    print("loss %.4f -> %.4f" % (first, last), flush=True)
+    assert last < 0.2 * first
+    g.close()
+
+
+def test_learned_cache_gains_over_plain(vgi, orc):
+    """At equal frame counts on a 128^3 noise scene, the cache (warmed up on
+    200 training-only frames of 64k path-traced samples) gives face irradiance
+    closer to the FP64 long-horizon reference than the plain multi-bounce
+    tracer (relative RMSE with the reference's own noise removed)."""
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=128)
+    faces = vgi.exposed_faces(setup["grid"])
+    sub = faces[::8]
+    ref_t = gpu_tracer(vgi, setup, resident=True)
+    ref_t.set_transport(bounces=4, samples=4, plain=True)
+    ref_t.bind_faces(sub)
+    sums = ref_t.converged(512)
+    ref_t.run_frame()
+    rec, _ = ref_t.read_bound()
+    reference, var = vgi.converged_irradiance(sums, rec[:, 16:19])
+    ref_t.close()
+    curves = {}
+    for mode in ("plain", "learned"):
+        g = gpu_tracer(vgi, setup, resident=True)
+        g.set_transport(bounces=4, samples=4, plain=True)
+        if mode == "learned":
+            g.mlp_enable(seed=1, train_per_frame=16384, steps=4, lr=1e-3)
+        g.bind_faces(faces)
+        curve = []
+        for f in range(24):
+            g.run_frame()
+            g.sync()
+            rec, _ = g.read_bound()
+            est = rec[::8, 0:3] + rec[::8, 16:19]
+            curve.append(vgi.relative_rmse(est, reference, var))
+        curves[mode] = curve
+        print(mode, ["%.3f" % c for c in curve], flush=True)
+        g.close()
+    assert curves["learned"][-1] < curves["plain"][-1]
