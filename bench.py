@@ -85,6 +85,109 @@ def cpu_baseline(vgi, grid, coords, pages, optics, faces, bounces, spp, budget_s
     }
 
 
+def run_to_target(vgi, g, judged, reference, var, max_frames, target, every=4):
+    """Frames on the bound faces (the judged ones bound first) until the
+    judged faces' relative RMSE against the reference reaches `target`.
+    Returns the (frame, cumulative GPU ms, rmse) points evaluated."""
+    points = []
+    gpu_ms = 0.0
+    for f in range(1, max_frames + 1):
+        g.run_frame()
+        g.sync()
+        gpu_ms += g.last_frame_ms()
+        if f <= 8 or f % every == 0 or f == max_frames:
+            rec, _ = g.read_bound(judged)
+            r = vgi.relative_rmse(rec[:, 0:3] + rec[:, 16:19], reference, var)
+            points.append((f, gpu_ms, r))
+            if r <= target:
+                break
+    return points
+
+
+def at_budget(points, budget_ms):
+    """RMSE of the last evaluated frame within a GPU-time budget."""
+    best = None
+    for f, ms, r in points:
+        if ms <= budget_ms:
+            best = r
+    return None if best is None else round(best, 4)
+
+
+def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
+    """GPU time to reach the target face-irradiance RMSE, plain 4-bounce
+    paths against the learned cache, judged on every 64th face against an
+    FP64 long-horizon reference."""
+    diffuse, emission, lights, env = optics
+    judged_idx = np.arange(0, len(faces), 64)
+    rest = np.setdiff1d(np.arange(len(faces)), judged_idx)
+    ordered = np.concatenate([faces[judged_idx], faces[rest]])
+    judged = len(judged_idx)
+
+    def tracer():
+        g = vgi.GpuTracer(0)
+        g.enable_resident()
+        g.upload_pages(coords, pages)
+        g.set_materials(diffuse, emission)
+        g.set_lights(lights)
+        g.set_transport(bounces=bounces, samples=spp, environment=tuple(env), plain=True)
+        return g
+
+    t0 = time.time()
+    g = tracer()
+    g.bind_faces(faces[judged_idx])
+    sums = g.converged(args.reference_launches)
+    g.run_frame()
+    rec, _ = g.read_bound()
+    reference, var = vgi.converged_irradiance(sums, rec[:, 16:19])
+    g.close()
+    noise = float(np.sqrt(var.mean()) / reference.mean())
+    print("reference: %d faces x %d samples, noise %.4f, %.1f s" % (judged, args.reference_launches * spp, noise, time.time() - t0),
+          file=sys.stderr, flush=True)
+    out = {
+        "target_rmse": args.target_rmse,
+        "judged_faces": judged,
+        "reference": "FP64 sums of %d plain launches x %d rays per judged face (relative noise %.4f, its variance subtracted)" % (
+            args.reference_launches, spp, noise),
+    }
+    legs = {}
+    g = tracer()
+    g.bind_faces(ordered)
+    pts = run_to_target(vgi, g, judged, reference, var, args.plain_frames, args.target_rmse)
+    g.close()
+    legs["plain"] = (pts, 0.0)
+    print("plain: %d frames, %.0f GPU ms, rmse %.4f" % pts[-1], file=sys.stderr, flush=True)
+    g = tracer()
+    g.bind_faces(ordered)
+    g.mlp_enable(seed=1, train_per_frame=65536, steps=8, lr=1e-3)
+    t1 = time.perf_counter()
+    g.mlp_pretrain(args.pretrain_frames)
+    g.sync()
+    pretrain_ms = (time.perf_counter() - t1) * 1000.0
+    pts = run_to_target(vgi, g, judged, reference, var, args.learned_frames, args.target_rmse)
+    g.close()
+    legs["learned"] = (pts, pretrain_ms)
+    print("learned: %d frames, %.0f GPU ms, rmse %.4f (pretrain %.0f ms)" % (pts[-1] + (pretrain_ms,)), file=sys.stderr, flush=True)
+    budgets = [1000.0, 3000.0, 10000.0]
+    for name, (pts, pre_ms) in legs.items():
+        reached = pts[-1][2] <= args.target_rmse
+        frames = pts[-1][0]
+        entry = {
+            "gpu_ms_per_frame": round(pts[-1][1] / frames, 3),
+            "frames_to_target": frames if reached else None,
+            "gpu_ms_to_target": round(pts[-1][1] + pre_ms, 1) if reached else None,
+            "frames_run": frames,
+            "rmse_last": round(pts[-1][2], 4),
+            "rmse_curve": [[f, round(ms + pre_ms, 1), round(r, 4)] for f, ms, r in pts[:: max(1, len(pts) // 12)]],
+            "rmse_at_gpu_ms": {str(int(b)): at_budget([(f, ms + pre_ms, r) for f, ms, r in pts], b) for b in budgets},
+        }
+        if name == "learned":
+            entry["pretrain_frames"] = args.pretrain_frames
+            entry["pretrain_ms"] = round(pre_ms, 1)
+            entry["training"] = "64k one-sample 3-bounce targets per frame, 8 Adam steps (lr 1e-3), BF16 MFMA"
+        out[name] = entry
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +197,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle tracing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="MiB copied by a plain copy kernel (PMC calibration)")
+    ap.add_argument("--no-convergence", action="store_true", help="skip the time-to-RMSE comparison")
+    ap.add_argument("--target-rmse", type=float, default=0.02)
+    ap.add_argument("--reference-launches", type=int, default=2048)
+    ap.add_argument("--plain-frames", type=int, default=900)
+    ap.add_argument("--learned-frames", type=int, default=400)
+    ap.add_argument("--pretrain-frames", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -203,9 +312,11 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "trace_faces_%s_traffic.json" % args.workload)
     if os.path.exists(traffic_file):
         line["roofline"]["traffic"] = round(json.load(open(traffic_file))["traffic_bytes_per_launch"])
+    g.close()
+    if not args.no_convergence and args.workload == "cave512":
+        line["convergence"] = convergence(vgi, coords, pages, optics, faces, bounces, spp, args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(vgi, grid, coords, pages, optics, faces, bounces, spp, args.cpu_budget)
-    g.close()
     print(json.dumps(line))
 
 

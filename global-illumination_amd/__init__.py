@@ -88,6 +88,7 @@ def library():
         "vgi_run_frame": (ctypes.c_int, [P, u32]),
         "vgi_sync": (ctypes.c_int, [P]),
         "vgi_last_trace_ms": (f32, [P]),
+        "vgi_last_frame_ms": (f32, [P]),
         "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
@@ -123,7 +124,7 @@ def exported_symbols():
         "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_set_clock", "vgi_upload_pages", "vgi_edit_voxels",
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
-        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_read_bound",
+        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_read_bound",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
@@ -312,10 +313,15 @@ class GpuTracer:
     def last_trace_ms(self):
         return float(self._lib.vgi_last_trace_ms(self._h))
 
-    def read_bound(self):
-        out = np.zeros((self._bound, 24), np.float32)
+    def last_frame_ms(self):
+        return float(self._lib.vgi_last_frame_ms(self._h))
+
+    def read_bound(self, count=None):
+        """Records of the first `count` bound faces (all by default)."""
+        count = self._bound if count is None else int(count)
+        out = np.zeros((count, 24), np.float32)
         counters = np.zeros(5, np.uint64)
-        _check(self._lib.vgi_read_bound(self._h, _ptr(out), self._bound, _ptr(counters)))
+        _check(self._lib.vgi_read_bound(self._h, _ptr(out), count, _ptr(counters)))
         return out, counters
 
     def read_counters_bound(self):

@@ -88,7 +88,7 @@ struct Ticket {
 struct vgi_tracer {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     float timings[4] = {0, 0, 0, 0};
     // ---- scene (host authority for page contents, as the reference's RegionMap)
     std::unordered_map<uint64_t, Int3> pages;
@@ -134,6 +134,8 @@ struct vgi_tracer {
     uint32_t bound = 0;
     float last_trace_ms = 0.0f;
     uint32_t frame_index = 0;
+    bool frame_recorded = false;
+    float last_frame_ms = 0.0f;
     std::vector<uint32_t> bound_material;  // material of each bound face's voxel
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
@@ -542,6 +544,7 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.cache_state = t.d_state.as<uint32_t>();
     o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
     o.cache_on = (t.resident && !t.transport.plain) ? 1u : 0u;
+    o.receiver_nee = 1u;
 }
 
 // ---------------------------------------------------------------- resident slots
@@ -684,6 +687,12 @@ int learned_training(vgi_tracer &t) {
     vgi::Optics o;
     views(t, s, o);
     o.cache_on = 0u;
+    // The prediction replaces what a path gathers after its first hit beyond
+    // that hit's direct sun and emitter sample, with one bounce of the budget
+    // spent: targets are a face's indirect incident radiance, traced one
+    // bounce short.
+    o.bounces = o.bounces > 1u ? o.bounces - 1u : 1u;
+    o.receiver_nee = 0u;  // the prediction is the indirect part only
     vgi::launch_trace(s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index, t.d_train_results.as<float4>(),
                       nullptr, nullptr, false, t.stream);
     vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), t.stream);
@@ -999,13 +1008,15 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     views(*t, s, o);
     uint32_t count = t->bound;
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
+    hipEventRecord(t->ev[8], t->stream);
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[4], t->stream);
-    if (t->learned) {
+    if (t->learned && t->transport.bounces >= 2u) {
         // Paths stop at their first hit; the network answers every query.
+        // (A one-bounce path has nothing after its first hit to predict.)
         size_t samples = (size_t)count * t->transport.samples;
         HIP_OK(t->b_learned.reserve(samples * 32));
         HIP_OK(t->b_queries.reserve(samples * 16));
@@ -1025,12 +1036,14 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     }
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[5], t->stream);
+    t->frame_recorded = true;
 
     if (t->resident)
         vgi::launch_accumulate(t->b_slots.as<uint32_t>(), count, t->b_results.as<float4>(), t->d_state.as<uint32_t>(),
                                t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
     HIP_OK(hipGetLastError());
     if (t->learned && learned_training(*t) != 0) return -1;
+    hipEventRecord(t->ev[9], t->stream);
     t->frame_index += 1;
     return 0;
 }
@@ -1186,11 +1199,17 @@ float vgi_mlp_last_loss(vgi_tracer *t) {
 int vgi_sync(vgi_tracer *t) {
     if (!t) return fail("null tracer");
     HIP_OK(hipStreamSynchronize(t->stream));
-    hipEventElapsedTime(&t->last_trace_ms, t->ev[4], t->ev[5]);
+    // Only a frame records the trace events; timing them otherwise would
+    // leave an error behind for the next call.
+    if (t->frame_recorded) {
+        HIP_OK(hipEventElapsedTime(&t->last_trace_ms, t->ev[4], t->ev[5]));
+        HIP_OK(hipEventElapsedTime(&t->last_frame_ms, t->ev[8], t->ev[9]));
+    }
     return 0;
 }
 
 float vgi_last_trace_ms(vgi_tracer *t) { return t ? t->last_trace_ms : 0.0f; }
+float vgi_last_frame_ms(vgi_tracer *t) { return t ? t->last_frame_ms : 0.0f; }
 
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
     if (!t || (count && !out)) return fail("null argument");

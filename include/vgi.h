@@ -131,6 +131,10 @@ int vgi_run_frame(vgi_tracer *tracer, uint32_t flags);
 int vgi_sync(vgi_tracer *tracer);
 /* Time of the last frame's trace launch on its stream (ms), valid after vgi_sync. */
 float vgi_last_trace_ms(vgi_tracer *tracer);
+/* GPU time of the whole last frame (prepare, trace or learned trace +
+ * inference, accumulate, and the learned cache's training), valid after
+ * vgi_sync. */
+float vgi_last_frame_ms(vgi_tracer *tracer);
 /* The bound faces' 96-byte records after the last frame (out may be NULL with
  * count 0 to read only the counters); counters (optional)
  * receive [rays, steps, header loads, brick-word loads, cell-word loads] when

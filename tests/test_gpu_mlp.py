@@ -97,10 +97,16 @@ def test_adam_steps_reduce_the_loss(vgi, orc):
 
 
 def test_learned_cache_gains_over_plain(vgi, orc):
-    """At equal frame counts on a 128^3 noise scene, the cache (warmed up on
-    200 training-only frames of 64k path-traced samples) gives face irradiance
-    closer to the FP64 long-horizon reference than the plain multi-bounce
-    tracer (relative RMSE with the reference's own noise removed)."""
+    """What the trained network gains over the plain multi-bounce tracer on a
+    128^3 noise scene (4 bounces, 4 samples per face per frame): after 200
+    training-only frames of 64k path-traced samples, the first frames of face
+    irradiance are closer to the FP64 long-horizon reference (relative RMSE
+    with the reference's own noise removed) at equal frame counts, while each
+    learned frame traces only to the first hit.
+
+    The network's fitting error leaves a floor under the learned estimate
+    (about 5% here) that the unbiased plain estimate passes after some thirty
+    frames; the gain asserted is the early one, with margin."""
     setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=128)
     faces = vgi.exposed_faces(setup["grid"])
     sub = faces[::8]
@@ -112,21 +118,28 @@ def test_learned_cache_gains_over_plain(vgi, orc):
     rec, _ = ref_t.read_bound()
     reference, var = vgi.converged_irradiance(sums, rec[:, 16:19])
     ref_t.close()
-    curves = {}
+    curves, rays = {}, {}
     for mode in ("plain", "learned"):
         g = gpu_tracer(vgi, setup, resident=True)
         g.set_transport(bounces=4, samples=4, plain=True)
-        if mode == "learned":
-            g.mlp_enable(seed=1, train_per_frame=16384, steps=4, lr=1e-3)
         g.bind_faces(faces)
+        if mode == "learned":
+            g.mlp_enable(seed=1, train_per_frame=65536, steps=8, lr=1e-3)
+            g.mlp_pretrain(200)
+            print("pretrained: loss %.4f" % g.mlp_last_loss(), flush=True)
         curve = []
-        for f in range(24):
-            g.run_frame()
+        for f in range(8):
+            g.run_frame(flags=vgi.COUNT_RAYS if f == 0 else 0)
             g.sync()
-            rec, _ = g.read_bound()
+            rec, counters = g.read_bound()
+            if f == 0:
+                rays[mode] = int(counters[0])
             est = rec[::8, 0:3] + rec[::8, 16:19]
             curve.append(vgi.relative_rmse(est, reference, var))
         curves[mode] = curve
-        print(mode, ["%.3f" % c for c in curve], flush=True)
+        print(mode, "rays/frame", rays[mode], ["%.3f" % c for c in curve], flush=True)
         g.close()
-    assert curves["learned"][-1] < curves["plain"][-1]
+    for f in (0, 3, 7):
+        assert curves["learned"][f] < 0.92 * curves["plain"][f]
+    # The learned frame stops at the first hit: fewer rays than plain paths.
+    assert rays["learned"] < 0.8 * rays["plain"]
