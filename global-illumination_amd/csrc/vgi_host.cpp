@@ -544,7 +544,6 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.cache_state = t.d_state.as<uint32_t>();
     o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
     o.cache_on = (t.resident && !t.transport.plain) ? 1u : 0u;
-    o.receiver_nee = 1u;
 }
 
 // ---------------------------------------------------------------- resident slots
@@ -688,11 +687,9 @@ int learned_training(vgi_tracer &t) {
     views(t, s, o);
     o.cache_on = 0u;
     // The prediction replaces what a path gathers after its first hit beyond
-    // that hit's direct sun and emitter sample, with one bounce of the budget
-    // spent: targets are a face's indirect incident radiance, traced one
-    // bounce short.
+    // that hit's direct sun, with one bounce of the budget spent: targets are
+    // traced one bounce short.
     o.bounces = o.bounces > 1u ? o.bounces - 1u : 1u;
-    o.receiver_nee = 0u;  // the prediction is the indirect part only
     vgi::launch_trace(s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index, t.d_train_results.as<float4>(),
                       nullptr, nullptr, false, t.stream);
     vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), t.stream);

@@ -44,9 +44,6 @@ struct Optics {
     const uint32_t *__restrict__ cache_state;  // 36 words per slot
     uint32_t cache_mask;                    // capacity - 1, 0 with no cache
     uint32_t cache_on;
-    // 0: a receiver's own emitter sample is left out, so the result is the
-    // indirect part alone (the learned cache's training targets).
-    uint32_t receiver_nee;
 };
 
 struct RayResult {
@@ -512,9 +509,8 @@ struct CacheQuery {
 // kLearned: the path ends at its first explicit hit into the learned cache
 // (the world-cache termination rule of voxel_path.wgsl:459-463 with the
 // cached value replaced by the network's prediction of the hit face's
-// indirect incident radiance; its direct sun and emitter sample are taken
-// as on any other hit): the returned energy is everything but the
-// prediction term, whose factor goes to `query`.
+// incident radiance beyond its direct sun): the returned energy is
+// everything but the prediction term, whose factor goes to `query`.
 template <class Tally, bool kLearned = false>
 __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3 start, f3 first_direction, f3 initial_normal,
                                          uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally,
@@ -560,10 +556,7 @@ __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3
         float mis = 1.0f;
         if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
         if constexpr (kLearned) {
-            // Direct sun and emitter NEE stay explicit at the hit; the
-            // prediction stands for the indirect rest.
-            f3 nee = sample_emitter(s, o, hit.position, hit.normal, stream, sequence, bounce + 1u, tally);
-            energy = energy + throughput * (emission * mis + diffuse * (dv + nee));
+            energy = energy + throughput * (emission * mis + diffuse * dv);
             query->vx = hit.vx; query->vy = hit.vy; query->vz = hit.vz;
             query->face_material = face_index_of(hit.normal) | (hit.material << 8);
             query->factor = throughput * diffuse;
@@ -608,7 +601,7 @@ __device__ __forceinline__ f3 trace_receiver(const SceneView &s, const Optics &o
                                              uint32_t sequence, bool &known, FirstHit &first, Tally &tally,
                                              CacheQuery *query = nullptr) {
     f3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));
-    f3 value = o.receiver_nee ? sample_emitter(s, o, surface, normal, stream, sequence, 0u, tally) : mk(0.0f, 0.0f, 0.0f);
+    f3 value = sample_emitter(s, o, surface, normal, stream, sequence, 0u, tally);
     bool unknown = false;
     first.vx = first.vy = first.vz = 0;
     first.face = 6u;
