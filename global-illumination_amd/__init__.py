@@ -89,6 +89,11 @@ def library():
         "vgi_sync": (ctypes.c_int, [P]),
         "vgi_last_trace_ms": (f32, [P]),
         "vgi_last_frame_ms": (f32, [P]),
+        "vgi_bind_exposed": (i32, [P]),
+        "vgi_scene_edits": (i64, [u32, u32, u32, u32, P, P, i64]),
+        "vgi_bound_count": (u32, [P]),
+        "vgi_read_bound_tasks": (i32, [P, P, u32]),
+        "vgi_read_events": (i64, [P, P, u32]),
         "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
@@ -125,10 +130,11 @@ def exported_symbols():
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_read_bound",
+        "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
-        "vgi_scene_optics", "vgi_scene_faces",
+        "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits",
     ]
 
 
@@ -174,6 +180,16 @@ def exposed_faces(grid, interior_only=False):
     out = np.zeros(n, FACE_TASK)
     lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), _ptr(out), n)
     return out
+
+
+def scene_edits(seed, frame, count, edge):
+    """One frame of the seeded edit stream: (xyz int32 [n,3], values uint16 [n])."""
+    lib = library()
+    n = _check(lib.vgi_scene_edits(seed, frame, count, edge, None, None, 0))
+    xyz = np.zeros((n, 3), np.int32)
+    values = np.zeros(n, np.uint16)
+    lib.vgi_scene_edits(seed, frame, count, edge, _ptr(xyz), _ptr(values), n)
+    return xyz, values
 
 
 def pages_of(grid, origin=(0, 0, 0)):
@@ -315,6 +331,25 @@ class GpuTracer:
 
     def last_frame_ms(self):
         return float(self._lib.vgi_last_frame_ms(self._h))
+
+    def bind_exposed(self):
+        """Bind every exposed face and follow the exposed surface through edits."""
+        _check(self._lib.vgi_bind_exposed(self._h))
+        self._bound = int(self._lib.vgi_bound_count(self._h))
+
+    def bound_tasks(self):
+        self._bound = int(self._lib.vgi_bound_count(self._h))
+        out = np.zeros(self._bound, FACE_TASK)
+        _check(self._lib.vgi_read_bound_tasks(self._h, _ptr(out), self._bound))
+        return out
+
+    def events(self):
+        """Encoded retention events of the last applied edits (uint32 words)."""
+        n = int(self._lib.vgi_read_events(self._h, None, 0))
+        out = np.zeros(n, np.uint32)
+        if n:
+            _check(self._lib.vgi_read_events(self._h, _ptr(out), n))
+        return out
 
     def read_bound(self, count=None):
         """Records of the first `count` bound faces (all by default)."""

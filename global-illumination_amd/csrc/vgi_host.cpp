@@ -144,6 +144,18 @@ struct vgi_tracer {
     float mlp_lr = 1e-3f, last_loss = 0.0f;
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
+    // ---- edit retention: events of edits, applied with the next batch or frame
+    std::vector<vgi_host::RetentionEvent> pending_events;
+    DeviceBuffer d_events, d_measures, d_measure_max, d_material_light;
+    uint32_t last_event_words = 0;
+    bool material_light_dirty = true;
+    // ---- bound faces that follow the exposed surface through edits
+    bool follow_exposure = false;
+    std::vector<vgi_face_task> bound_tasks;
+    std::vector<uint32_t> bound_slots;
+    std::vector<uint32_t> bound_of_slot;  // slot -> bound index or NONE
+    std::vector<uint16_t> slot_material;  // material in each slot's identity word
+    DeviceBuffer d_bound_index, d_bound_upd, d_bound_slot_upd, d_meta_writes;
 };
 
 namespace {
@@ -408,29 +420,72 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
 }
 
 // Voxel edits (pages.rs:498 page overlay): scatter into the GPU grid, re-pack
-// the edited regions and update the top level if any region's emptiness flipped.
+// the edited regions and update the top level if any region's emptiness
+// flipped. With the resident cache, the edit's retention event is queued
+// for the next batch (pages.rs:446-490), and bound faces that follow the
+// exposed surface are added and removed.
+int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed);
+
 int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     if (!t.have_regions) return 0;
-    std::vector<uint32_t> idx;
-    std::vector<uint16_t> val;
+    std::vector<uint32_t> accepted;
     std::vector<int> dirty;
-    idx.reserve(count);
-    val.reserve(count);
+    accepted.reserve(count);
     for (uint32_t e = 0; e < count; ++e) {
         int x = xyz[e * 3], y = xyz[e * 3 + 1], z = xyz[e * 3 + 2];
         int rx = floor_div(x, 16) - t.rmin.x, ry = floor_div(y, 16) - t.rmin.y, rz = floor_div(z, 16) - t.rmin.z;
         if (rx < 0 || ry < 0 || rz < 0 || rx >= t.rsx || ry >= t.rsy || rz >= t.rsz) continue;
         size_t ri = lin3(rx, ry, rz, t.rsx, t.rsy);
         if (!t.loaded[ri]) continue;  // edits outside loaded pages wait for the page (dropped here)
+        accepted.push_back(e);
+        dirty.push_back((int)ri);
+    }
+    if (accepted.empty()) return 0;
+    std::sort(dirty.begin(), dirty.end());
+    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
+    auto region_origin = [&](int ri) {
+        int x = ri % t.rsx, y = (ri / t.rsx) % t.rsy, z = ri / (t.rsx * t.rsy);
+        return Int3{(t.rmin.x + x) * 16, (t.rmin.y + y) * 16, (t.rmin.z + z) * 16};
+    };
+    auto copy_region = [&](int ri, std::vector<uint16_t> &out) {
+        Int3 o = region_origin(ri);
+        out.resize(4096);
+        for (int z = 0; z < 16; ++z)
+            for (int y = 0; y < 16; ++y)
+                std::memcpy(&out[(size_t)16 * (y + 16 * z)], &t.grid[lin3(o.x - t.vmin.x, o.y - t.vmin.y + y, o.z - t.vmin.z + z, t.gx, t.gy)], 32);
+    };
+    // The scene before the edit, for its retention event.
+    bool events = t.resident;
+    std::vector<vgi_host::RegionDelta> deltas;
+    std::vector<uint8_t> solid_before;
+    if (events) {
+        solid_before = t.solid;
+        deltas.resize(dirty.size());
+        for (size_t k = 0; k < dirty.size(); ++k) {
+            int ri = dirty[k];
+            Int3 o = region_origin(ri);
+            deltas[k].coord = Int3{o.x / 16, o.y / 16, o.z / 16};
+            copy_region(ri, deltas[k].before);
+            deltas[k].solid_before = t.solid[ri] != 0;
+        }
+    }
+    // Edits apply in order (a later edit of a voxel wins); the GPU grid gets
+    // each edited voxel's final value once.
+    std::vector<uint32_t> idx;
+    std::vector<vgi_host::Int3> edited;
+    idx.reserve(accepted.size());
+    edited.reserve(accepted.size());
+    for (uint32_t e : accepted) {
+        int x = xyz[e * 3], y = xyz[e * 3 + 1], z = xyz[e * 3 + 2];
         size_t gi = lin3(x - t.vmin.x, y - t.vmin.y, z - t.vmin.z, t.gx, t.gy);
         t.grid[gi] = values[e];
         idx.push_back((uint32_t)gi);
-        val.push_back(values[e]);
-        dirty.push_back((int)ri);
+        edited.push_back(Int3{x, y, z});
     }
-    if (idx.empty()) return 0;
-    std::sort(dirty.begin(), dirty.end());
-    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+    std::vector<uint16_t> val(idx.size());
+    for (size_t k = 0; k < idx.size(); ++k) val[k] = t.grid[idx[k]];
     HIP_OK(t.d_scatter_idx.reserve(idx.size() * 4));
     HIP_OK(t.d_scatter_val.reserve(val.size() * 2));
     HIP_OK(hipMemcpyAsync(t.d_scatter_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, t.stream));
@@ -486,6 +541,38 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     if (upload_top(t) != 0) return -1;
     hipEventSynchronize(t.ev[7]);
     hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    if (!events) return 0;
+    // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
+    std::vector<vgi_host::Int3> changed;
+    for (size_t k = 0; k < dirty.size(); ++k) {
+        copy_region(dirty[k], deltas[k].after);
+        deltas[k].solid_after = t.solid[dirty[k]] != 0;
+        const vgi_host::RegionDelta &d = deltas[k];
+        for (int i = 0; i < 4096; ++i)
+            if (d.before[i] != d.after[i])
+                changed.push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
+    }
+    vgi_host::PreviousScene prev;
+    prev.any_solid = false;
+    prev.solid_min = Int3{INT32_MAX, INT32_MAX, INT32_MAX};
+    prev.solid_max = Int3{INT32_MIN, INT32_MIN, INT32_MIN};
+    prev.page_min = Int3{t.rmin.x / 2, t.rmin.y / 2, t.rmin.z / 2};
+    prev.psx = t.rsx / 2; prev.psy = t.rsy / 2; prev.psz = t.rsz / 2;
+    std::vector<uint8_t> page_solid((size_t)prev.psx * prev.psy * prev.psz, 0);
+    for (int z = 0; z < t.rsz; ++z)
+        for (int y = 0; y < t.rsy; ++y)
+            for (int x = 0; x < t.rsx; ++x) {
+                if (!solid_before[lin3(x, y, z, t.rsx, t.rsy)]) continue;
+                Int3 r{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z};
+                prev.any_solid = true;
+                prev.solid_min = Int3{std::min(prev.solid_min.x, r.x), std::min(prev.solid_min.y, r.y), std::min(prev.solid_min.z, r.z)};
+                prev.solid_max = Int3{std::max(prev.solid_max.x, r.x), std::max(prev.solid_max.y, r.y), std::max(prev.solid_max.z, r.z)};
+                page_solid[lin3(x / 2, y / 2, z / 2, prev.psx, prev.psy)] = 1;
+            }
+    prev.page_solid = page_solid.data();
+    vgi_host::RetentionEvent ev;
+    if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev)) t.pending_events.push_back(std::move(ev));
+    if (t.follow_exposure && !changed.empty()) return update_exposure(t, changed);
     return 0;
 }
 
@@ -551,13 +638,26 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
 int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, std::vector<uint32_t> &slots) {
     slots.resize(count);
     uint32_t first_new = t.high_water;
-    std::vector<uint32_t> identities;
+    std::vector<uint32_t> identities, meta_writes;
     vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
     for (uint32_t i = 0; i < count; ++i) {
         const vgi_face_task &f = tasks[i];
         if (f.face == VGI_NO_FACE) { slots[i] = vgi_host::NONE; continue; }
         long at = t.face_slots.find(f.voxel[0], f.voxel[1], f.voxel[2], f.face);
-        if (at >= 0) { slots[i] = t.face_slots.entries[at][3] & 0x1fffffffu; continue; }
+        if (at >= 0) {
+            uint32_t slot = t.face_slots.entries[at][3] & 0x1fffffffu;
+            slots[i] = slot;
+            // Material replacement (resident.rs:443-450): a relit face keeps
+            // its slot; its identity word takes the voxel's new material.
+            uint32_t m = t.have_regions ? src.material_at(f.voxel[0], f.voxel[1], f.voxel[2]) : 0xffffu;
+            uint16_t material = (uint16_t)(m < vgi_host::AIR ? m : 0u);
+            if (slot < t.slot_material.size() && t.slot_material[slot] != material) {
+                t.slot_material[slot] = material;
+                meta_writes.push_back(slot * 12u + 3u);
+                meta_writes.push_back(f.face | 8u | ((uint32_t)material << 16));
+            }
+            continue;
+        }
         uint32_t slot;
         if (!t.free_slots.empty()) { slot = t.free_slots.back(); t.free_slots.pop_back(); }
         else slot = t.high_water++;
@@ -568,6 +668,8 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         identities.push_back(slot);
         identities.push_back((uint32_t)f.voxel[0]); identities.push_back((uint32_t)f.voxel[1]); identities.push_back((uint32_t)f.voxel[2]);
         identities.push_back(f.face | 8u | (material << 16));
+        if (t.slot_material.size() <= slot) t.slot_material.resize((size_t)slot + 1, 0xffffu);
+        t.slot_material[slot] = (uint16_t)material;
         slots[i] = slot;
     }
     // Grow the resident arrays, keeping existing slots.
@@ -610,13 +712,35 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         HIP_OK(hipStreamSynchronize(t.stream));
     }
     (void)first_new;
+    if (!meta_writes.empty()) {
+        HIP_OK(t.d_meta_writes.reserve(meta_writes.size() * 4));
+        HIP_OK(hipMemcpyAsync(t.d_meta_writes.ptr, meta_writes.data(), meta_writes.size() * 4, hipMemcpyHostToDevice, t.stream));
+        vgi::launch_scatter_words(t.d_meta_writes.as<uint2>(), (uint32_t)(meta_writes.size() / 2), t.d_meta.as<uint32_t>(), t.stream);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(t.stream));
+    }
     if (t.face_slots.dirty) {
         size_t n = t.face_slots.entries.size();
         HIP_OK(t.d_index.reserve(n * 16));
         HIP_OK(hipMemcpyAsync(t.d_index.ptr, t.face_slots.entries.data(), n * 16, hipMemcpyHostToDevice, t.stream));
         HIP_OK(hipStreamSynchronize(t.stream));
         t.face_slots.dirty = false;
+    } else if (!t.face_slots.touched.empty()) {
+        // Only the written entries travel (4 words each).
+        std::vector<uint32_t> &tw = t.face_slots.touched;
+        std::sort(tw.begin(), tw.end());
+        tw.erase(std::unique(tw.begin(), tw.end()), tw.end());
+        std::vector<uint32_t> writes;
+        writes.reserve(tw.size() * 8);
+        for (uint32_t i : tw)
+            for (int k = 0; k < 4; ++k) { writes.push_back(i * 4u + (uint32_t)k); writes.push_back(t.face_slots.entries[i][k]); }
+        HIP_OK(t.d_meta_writes.reserve(writes.size() * 4));
+        HIP_OK(hipMemcpyAsync(t.d_meta_writes.ptr, writes.data(), writes.size() * 4, hipMemcpyHostToDevice, t.stream));
+        vgi::launch_scatter_words(t.d_meta_writes.as<uint2>(), (uint32_t)(writes.size() / 2), t.d_index.as<uint32_t>(), t.stream);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(t.stream));
     }
+    t.face_slots.touched.clear();
     return 0;
 }
 
@@ -697,6 +821,174 @@ int learned_training(vgi_tracer &t) {
     for (uint32_t off = 0; off + per <= total; off += per)
         if (mlp_step(t, t.d_train_q.as<vgi::MlpQuery>() + off, t.d_targets.as<float4>() + off, per, true) != 0) return -1;
     t.train_cursor = (uint32_t)(((uint64_t)t.train_cursor + total) % t.bound);
+    return 0;
+}
+
+// The queued retention events of edits, on the GPU before the batch's
+// prepare (resident.rs:1005-1061): measure the edits' flux from the resident
+// statistics, then retention over every slot.
+int flush_events(vgi_tracer &t) {
+    if (t.pending_events.empty() || !t.resident) return 0;
+    vgi_host::EncodedEvents enc;
+    vgi_host::encode_events(t.pending_events, enc);
+    t.pending_events.clear();
+    if (t.material_light_dirty) {
+        // resident.rs:815-838: luminance of each material's albedo and emission.
+        std::vector<float> light(512);
+        for (int m = 0; m < 256; ++m) {
+            const float *d = &t.diffuse[m * 3], *e = &t.emission[m * 3];
+            light[m * 2] = 0.2126f * d[0] + 0.7152f * d[1] + 0.0722f * d[2];
+            light[m * 2 + 1] = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+        }
+        HIP_OK(t.d_material_light.reserve(light.size() * 4));
+        HIP_OK(hipMemcpyAsync(t.d_material_light.ptr, light.data(), light.size() * 4, hipMemcpyHostToDevice, t.stream));
+        t.material_light_dirty = false;
+    }
+    HIP_OK(t.d_events.reserve(enc.words.size() * 4));
+    HIP_OK(hipMemcpyAsync(t.d_events.ptr, enc.words.data(), enc.words.size() * 4, hipMemcpyHostToDevice, t.stream));
+    if (enc.entries) {
+        HIP_OK(t.d_measures.reserve(enc.measures.size() * 4));
+        HIP_OK(t.d_measure_max.reserve((size_t)enc.entries * 4));
+        HIP_OK(hipMemcpyAsync(t.d_measures.ptr, enc.measures.data(), enc.measures.size() * 4, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipMemsetAsync(t.d_measure_max.ptr, 0, (size_t)enc.entries * 4, t.stream));
+        vgi::launch_measure_edits(t.d_measures.as<uint4>(), enc.entries, enc.positions, t.d_index.as<uint4>(),
+                                  (uint32_t)(t.face_slots.entries.size() - 1), t.d_state.as<uint32_t>(), t.d_meta.as<uint32_t>(),
+                                  t.d_material_light.as<float2>(), t.d_measure_max.as<uint32_t>(), t.d_events.as<uint4>(), t.stream);
+        HIP_OK(hipGetLastError());
+    }
+    vgi::launch_apply_events(t.d_events.as<uint4>(), enc.events, t.high_water, t.d_state.as<uint32_t>(), t.d_checkpoint.as<uint32_t>(),
+                             t.d_meta.as<uint32_t>(), std::max(t.transport.measured_retention, 0.0f), t.stream);
+    HIP_OK(hipGetLastError());
+    t.last_event_words = (uint32_t)enc.words.size();
+    return 0;
+}
+
+// Each bound face's material (the learned cache encodes its albedo) and a
+// fixed random order for choosing training faces.
+int bind_materials(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
+    vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
+    t.bound_material.assign(count, 0u);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t m = t.have_regions ? src.material_at(tasks[i].voxel[0], tasks[i].voxel[1], tasks[i].voxel[2]) : 0xffffu;
+        t.bound_material[i] = m < vgi_host::AIR ? m : 0u;
+    }
+    std::vector<uint32_t> perm(count);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937 rng(0x5eedu);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIP_OK(t.d_face_material.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+    HIP_OK(t.d_perm.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+    HIP_OK(hipMemcpyAsync(t.d_face_material.ptr, t.bound_material.data(), (size_t)count * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipMemcpyAsync(t.d_perm.ptr, perm.data(), (size_t)count * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipStreamSynchronize(t.stream));
+    t.train_cursor = 0;
+    return 0;
+}
+
+// An exposed face: a solid voxel whose neighbour across the face is loaded air.
+const int FACE_STEP[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+bool face_exposed(const vgi_host::VoxelSource &src, int x, int y, int z, uint32_t f) {
+    if (src.material_at(x, y, z) >= vgi_host::AIR) return false;
+    return src.material_at(x + FACE_STEP[f][0], y + FACE_STEP[f][1], z + FACE_STEP[f][2]) == vgi_host::AIR;
+}
+
+// Bound faces follow the exposed surface after an edit of the `changed`
+// voxels: a face that lost its air neighbour leaves (the last bound face
+// takes its place), a newly exposed face joins at the end (publication's
+// rescan of the edited pages, pages.rs:451-458).
+int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed) {
+    vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
+    struct Cand { int32_t x, y, z; uint32_t f; };
+    std::vector<Cand> cand;
+    cand.reserve(changed.size() * 12);
+    for (const Int3 &c : changed)
+        for (uint32_t f = 0; f < 6; ++f) {
+            cand.push_back(Cand{c.x, c.y, c.z, f});
+            cand.push_back(Cand{c.x + FACE_STEP[f][0], c.y + FACE_STEP[f][1], c.z + FACE_STEP[f][2], f ^ 1u});
+        }
+    auto key_less = [](const Cand &a, const Cand &b) {
+        if (a.z != b.z) return a.z < b.z;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.x != b.x) return a.x < b.x;
+        return a.f < b.f;
+    };
+    std::sort(cand.begin(), cand.end(), key_less);
+    cand.erase(std::unique(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.x == b.x && a.y == b.y && a.z == b.z && a.f == b.f; }),
+               cand.end());
+    std::vector<uint32_t> removals;
+    std::vector<vgi_face_task> additions, refresh;
+    for (const Cand &c : cand) {
+        bool exposed = face_exposed(src, c.x, c.y, c.z, c.f);
+        long at = t.face_slots.find(c.x, c.y, c.z, c.f);
+        uint32_t slot = at >= 0 ? (t.face_slots.entries[at][3] & 0x1fffffffu) : vgi_host::NONE;
+        uint32_t b = (slot != vgi_host::NONE && slot < t.bound_of_slot.size()) ? t.bound_of_slot[slot] : vgi_host::NONE;
+        if (exposed && b == vgi_host::NONE) {
+            vgi_face_task task{};
+            task.voxel[0] = c.x; task.voxel[1] = c.y; task.voxel[2] = c.z;
+            task.face = c.f;
+            additions.push_back(task);
+        } else if (!exposed && b != vgi_host::NONE) {
+            removals.push_back(b);
+        } else if (exposed && b != vgi_host::NONE) {
+            refresh.push_back(t.bound_tasks[b]);  // stays bound; its material may have changed
+        }
+    }
+    std::vector<uint32_t> refreshed;
+    if (register_faces(t, refresh.data(), (uint32_t)refresh.size(), refreshed) != 0) return -1;
+    // Removals, highest index first, each filled by the current last face.
+    std::sort(removals.rbegin(), removals.rend());
+    std::vector<uint32_t> dirty_index;
+    for (uint32_t b : removals) {
+        uint32_t last = (uint32_t)t.bound_tasks.size() - 1;
+        t.bound_of_slot[t.bound_slots[b]] = vgi_host::NONE;
+        if (b != last) {
+            t.bound_tasks[b] = t.bound_tasks[last];
+            t.bound_slots[b] = t.bound_slots[last];
+            t.bound_of_slot[t.bound_slots[b]] = b;
+            dirty_index.push_back(b);
+        }
+        t.bound_tasks.pop_back();
+        t.bound_slots.pop_back();
+    }
+    std::vector<uint32_t> new_slots;
+    if (register_faces(t, additions.data(), (uint32_t)additions.size(), new_slots) != 0) return -1;
+    for (size_t k = 0; k < additions.size(); ++k) {
+        uint32_t b = (uint32_t)t.bound_tasks.size();
+        t.bound_tasks.push_back(additions[k]);
+        t.bound_slots.push_back(new_slots[k]);
+        if (t.bound_of_slot.size() <= new_slots[k]) t.bound_of_slot.resize((size_t)new_slots[k] + 1, vgi_host::NONE);
+        t.bound_of_slot[new_slots[k]] = b;
+        dirty_index.push_back(b);
+    }
+    uint32_t count = (uint32_t)t.bound_tasks.size();
+    std::sort(dirty_index.begin(), dirty_index.end());
+    dirty_index.erase(std::unique(dirty_index.begin(), dirty_index.end()), dirty_index.end());
+    dirty_index.erase(std::remove_if(dirty_index.begin(), dirty_index.end(), [&](uint32_t i) { return i >= count; }), dirty_index.end());
+    size_t need_tasks = (size_t)std::max<uint32_t>(count, 1) * 32;
+    if (need_tasks > t.b_tasks.bytes) {
+        // Grown past the allocation: upload the whole set again.
+        HIP_OK(t.b_tasks.reserve(need_tasks));
+        HIP_OK(t.b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+        HIP_OK(hipMemcpyAsync(t.b_tasks.ptr, t.bound_tasks.data(), (size_t)count * 32, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipMemcpyAsync(t.b_slots.ptr, t.bound_slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t.stream));
+    } else if (!dirty_index.empty()) {
+        std::vector<vgi_face_task> up(dirty_index.size());
+        std::vector<uint32_t> up_slots(dirty_index.size());
+        for (size_t k = 0; k < dirty_index.size(); ++k) { up[k] = t.bound_tasks[dirty_index[k]]; up_slots[k] = t.bound_slots[dirty_index[k]]; }
+        HIP_OK(t.d_bound_index.reserve(dirty_index.size() * 4));
+        HIP_OK(t.d_bound_upd.reserve(up.size() * 32));
+        HIP_OK(t.d_bound_slot_upd.reserve(up_slots.size() * 4));
+        HIP_OK(hipMemcpyAsync(t.d_bound_index.ptr, dirty_index.data(), dirty_index.size() * 4, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipMemcpyAsync(t.d_bound_upd.ptr, up.data(), up.size() * 32, hipMemcpyHostToDevice, t.stream));
+        HIP_OK(hipMemcpyAsync(t.d_bound_slot_upd.ptr, up_slots.data(), up_slots.size() * 4, hipMemcpyHostToDevice, t.stream));
+        vgi::launch_scatter_bound(t.d_bound_index.as<uint32_t>(), t.d_bound_upd.as<uint4>(), t.d_bound_slot_upd.as<uint32_t>(),
+                                  (uint32_t)dirty_index.size(), t.b_tasks.as<uint4>(), t.b_slots.as<uint32_t>(), t.stream);
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(t.b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
+    HIP_OK(hipStreamSynchronize(t.stream));
+    t.bound = count;
+    if (t.mlp_ready && bind_materials(t, t.bound_tasks.data(), count) != 0) return -1;
     return 0;
 }
 
@@ -808,6 +1100,7 @@ int vgi_set_materials(vgi_tracer *t, const float *diffuse_rgb, const float *emis
             t->emission[m * 3 + c] = e[c];
         }
     }
+    t->material_light_dirty = true;
     HIP_OK(t->d_materials.reserve(packed.size() * 4));
     HIP_OK(hipMemcpyAsync(t->d_materials.ptr, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, t->stream));
     HIP_OK(hipStreamSynchronize(t->stream));
@@ -868,6 +1161,7 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
     }
     views(*t, s, o);
     uint32_t lanes = lanes_for(t->transport);
+    if (flush_events(*t) != 0) return -1;
     hipEventRecord(t->ev[0], t->stream);
     if (t->resident)
         vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
@@ -966,6 +1260,10 @@ int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
     HIP_OK(hipMemcpyAsync(t->b_tasks.ptr, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
     HIP_OK(t->b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
     HIP_OK(t->b_counters.reserve(64));
+    t->follow_exposure = false;
+    t->bound_tasks.clear();
+    t->bound_slots.clear();
+    t->bound_of_slot.clear();
     if (t->resident) {
         std::vector<uint32_t> slots;
         if (register_faces(*t, tasks, count, slots) != 0) return -1;
@@ -973,26 +1271,59 @@ int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
         HIP_OK(hipMemcpyAsync(t->b_slots.ptr, slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
         t->last_slots = slots;
     }
-    // Each bound face's material (the cache encodes its albedo) and a fixed
-    // random order for choosing training faces.
-    vgi_host::VoxelSource src{t->grid.data(), t->vmin, t->gx, t->gy, t->gz, t->loaded.data(), t->rmin, t->rsx, t->rsy, t->rsz};
-    t->bound_material.assign(count, 0u);
-    for (uint32_t i = 0; i < count; ++i) {
-        uint32_t m = t->have_regions ? src.material_at(tasks[i].voxel[0], tasks[i].voxel[1], tasks[i].voxel[2]) : 0xffffu;
-        t->bound_material[i] = m < vgi_host::AIR ? m : 0u;
-    }
-    std::vector<uint32_t> perm(count);
-    std::iota(perm.begin(), perm.end(), 0u);
-    std::mt19937 rng(0x5eedu);
-    std::shuffle(perm.begin(), perm.end(), rng);
-    HIP_OK(t->d_face_material.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
-    HIP_OK(t->d_perm.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
-    HIP_OK(hipMemcpyAsync(t->d_face_material.ptr, t->bound_material.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
-    HIP_OK(hipMemcpyAsync(t->d_perm.ptr, perm.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
-    HIP_OK(hipStreamSynchronize(t->stream));
+    if (bind_materials(*t, tasks, count) != 0) return -1;
     t->bound = count;
-    t->train_cursor = 0;
     return 0;
+}
+
+int vgi_bind_exposed(vgi_tracer *t) {
+    if (!t) return fail("null tracer");
+    if (!t->resident) return fail("bound faces follow edits through the resident cache: call vgi_enable_resident first");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    vgi_host::VoxelSource src{t->grid.data(), t->vmin, t->gx, t->gy, t->gz, t->loaded.data(), t->rmin, t->rsx, t->rsy, t->rsz};
+    std::vector<vgi_face_task> faces;
+    for (int z = 0; z < t->gz; ++z)
+        for (int y = 0; y < t->gy; ++y)
+            for (int x = 0; x < t->gx; ++x) {
+                int vx = t->vmin.x + x, vy = t->vmin.y + y, vz = t->vmin.z + z;
+                if (t->grid[lin3(x, y, z, t->gx, t->gy)] >= vgi_host::AIR) continue;
+                for (uint32_t f = 0; f < 6; ++f) {
+                    if (!face_exposed(src, vx, vy, vz, f)) continue;
+                    vgi_face_task task{};
+                    task.voxel[0] = vx; task.voxel[1] = vy; task.voxel[2] = vz;
+                    task.face = f;
+                    faces.push_back(task);
+                }
+            }
+    if (faces.size() >= (1u << 31)) return fail("too many exposed faces");
+    if (vgi_bind_faces(t, faces.data(), (uint32_t)faces.size()) != 0) return -1;
+    t->bound_slots = t->last_slots;
+    t->bound_tasks.swap(faces);
+    t->bound_of_slot.assign(t->high_water, vgi_host::NONE);
+    for (size_t i = 0; i < t->bound_slots.size(); ++i) t->bound_of_slot[t->bound_slots[i]] = (uint32_t)i;
+    t->follow_exposure = true;
+    return 0;
+}
+
+int vgi_read_bound_tasks(vgi_tracer *t, vgi_face_task *out, uint32_t count) {
+    if (!t || (count && !out)) return fail("null argument");
+    if (count > t->bound) return fail("more tasks than bound faces");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (count) HIP_OK(hipMemcpy(out, t->b_tasks.ptr, (size_t)count * 32, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+uint32_t vgi_bound_count(vgi_tracer *t) { return t ? t->bound : 0u; }
+
+int64_t vgi_read_events(vgi_tracer *t, uint32_t *out, uint32_t capacity) {
+    if (!t) return fail("null tracer");
+    if (out && capacity) {
+        HIP_OK(hipStreamSynchronize(t->stream));
+        uint32_t n = std::min(capacity, t->last_event_words);
+        if (n) HIP_OK(hipMemcpy(out, t->d_events.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
+    return (int64_t)t->last_event_words;
 }
 
 
@@ -1006,6 +1337,7 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     uint32_t count = t->bound;
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
     hipEventRecord(t->ev[8], t->stream);
+    if (flush_events(*t) != 0) return -1;
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),

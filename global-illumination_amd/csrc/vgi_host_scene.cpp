@@ -370,18 +370,19 @@ long FaceSlots::find(int32_t x, int32_t y, int32_t z, uint32_t face) const {
     }
 }
 
-static void place(std::vector<std::array<uint32_t, 4>> &entries, const std::array<uint32_t, 4> &e) {
+static size_t place(std::vector<std::array<uint32_t, 4>> &entries, const std::array<uint32_t, 4> &e) {
     size_t mask = entries.size() - 1;
     size_t i = (hash_cell((int32_t)e[0], (int32_t)e[1], (int32_t)e[2]) ^ pcg((e[3] >> 29) + 1u)) & mask;
     while (entries[i][3] != NONE) i = (i + 1) & mask;
     entries[i] = e;
+    return i;
 }
 
 void FaceSlots::insert(int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t slot) {
     long f = find(x, y, z, face);
-    dirty = true;
     if (f >= 0) {
         entries[f][3] = (face << 29) | slot;
+        touched.push_back((uint32_t)f);
         return;
     }
     if ((live + 1) * 2 > entries.size()) {
@@ -390,9 +391,200 @@ void FaceSlots::insert(int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t 
         entries.assign(old.size() * 2, {0u, 0u, 0u, NONE});
         for (auto &e : old)
             if (e[3] != NONE) place(entries, e);
+        dirty = true;
     }
-    place(entries, {(uint32_t)x, (uint32_t)y, (uint32_t)z, (face << 29) | slot});
+    touched.push_back((uint32_t)place(entries, {(uint32_t)x, (uint32_t)y, (uint32_t)z, (face << 29) | slot}));
     ++live;
+}
+
+// ---------------------------------------------------------------- edit retention
+namespace {
+
+// Bounds::distance_squared (geometry.rs:239-247): per-axis separation,
+// saturated at 4096.
+int64_t box_distance2(Int3 amin, Int3 amax, Int3 bmin, Int3 bmax) {
+    int64_t d[3] = {std::max<int64_t>(std::max<int64_t>((int64_t)amin.x - bmax.x, (int64_t)bmin.x - amax.x), 0),
+                    std::max<int64_t>(std::max<int64_t>((int64_t)amin.y - bmax.y, (int64_t)bmin.y - amax.y), 0),
+                    std::max<int64_t>(std::max<int64_t>((int64_t)amin.z - bmax.z, (int64_t)bmin.z - amax.z), 0)};
+    int64_t s = 0;
+    for (int64_t v : d) {
+        v = std::min<int64_t>(v, 4096);
+        s += v * v;
+    }
+    return s;
+}
+
+int64_t floor_div64(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    if ((a % b) != 0 && ((a < 0) != (b < 0))) --q;
+    return q;
+}
+
+// geometry.rs:92-100
+float emission_luminance(const float *emission_rgb, uint16_t material) {
+    if (material >= AIR) return 0.0f;
+    const float *e = emission_rgb + (size_t)material * 3;
+    return 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+}
+
+bool box_less(const ChangeBox &a, const ChangeBox &b) {
+    int32_t ka[6] = {a.min.x, a.min.y, a.min.z, a.max.x, a.max.y, a.max.z};
+    int32_t kb[6] = {b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z};
+    return std::lexicographical_compare(ka, ka + 6, kb, kb + 6);
+}
+
+bool box_equal(const ChangeBox &a, const ChangeBox &b) { return a.min == b.min && a.max == b.max; }
+
+}  // namespace
+
+bool PreviousScene::page_occupied(Int3 page) const {
+    int x = page.x - page_min.x, y = page.y - page_min.y, z = page.z - page_min.z;
+    if (x < 0 || y < 0 || z < 0 || x >= psx || y >= psy || z >= psz) return false;
+    return page_solid[(size_t)x + (size_t)psx * ((size_t)y + (size_t)psy * (size_t)z)] != 0;
+}
+
+bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
+                const std::vector<Int3> &edits, RetentionEvent &out) {
+    out.boxes.clear();
+    std::vector<ChangeBox> boxes;
+    for (const RegionDelta &d : deltas) {
+        Int3 origin{d.coord.x * 16, d.coord.y * 16, d.coord.z * 16};
+        Int3 tmin{INT32_MAX, INT32_MAX, INT32_MAX}, tmax{INT32_MIN, INT32_MIN, INT32_MIN};
+        uint32_t changed = 0;
+        float flux = 0.0f;
+        bool uniform_before = std::all_of(d.before.begin(), d.before.end(), [&](uint16_t v) { return v == d.before[0]; });
+        bool uniform_after = std::all_of(d.after.begin(), d.after.end(), [&](uint16_t v) { return v == d.after[0]; });
+        if (uniform_before && uniform_after) {
+            // One material on each side: the region changes as one box of
+            // 4096 voxels (geometry.rs:170-172).
+            if (d.before[0] != d.after[0]) {
+                tmin = origin;
+                tmax = Int3{origin.x + 15, origin.y + 15, origin.z + 15};
+                changed = 4096;
+                flux = std::fabs(emission_luminance(emission_rgb, d.before[0]) - emission_luminance(emission_rgb, d.after[0])) * 6.0f * 4096.0f;
+            }
+        } else
+        // Voxel by voxel in z, y, x order.
+        for (int z = 0; z < 16; ++z)
+            for (int y = 0; y < 16; ++y)
+                for (int x = 0; x < 16; ++x) {
+                    size_t i = (size_t)x + 16 * ((size_t)y + 16 * (size_t)z);
+                    uint16_t a = d.before[i], b = d.after[i];
+                    if (a == b) continue;
+                    Int3 p{origin.x + x, origin.y + y, origin.z + z};
+                    tmin = Int3{std::min(tmin.x, p.x), std::min(tmin.y, p.y), std::min(tmin.z, p.z)};
+                    tmax = Int3{std::max(tmax.x, p.x), std::max(tmax.y, p.y), std::max(tmax.z, p.z)};
+                    changed += 1;
+                    flux += std::fabs(emission_luminance(emission_rgb, a) - emission_luminance(emission_rgb, b)) * 6.0f * 1.0f;
+                }
+        bool in_occupied = previous.any_solid && d.coord.x >= previous.solid_min.x && d.coord.y >= previous.solid_min.y &&
+                           d.coord.z >= previous.solid_min.z && d.coord.x <= previous.solid_max.x && d.coord.y <= previous.solid_max.y &&
+                           d.coord.z <= previous.solid_max.z;
+        bool transport = d.solid_before || d.solid_after || in_occupied;
+        if (!transport || changed == 0) continue;
+        boxes.push_back(ChangeBox{tmin, tmax, flux, changed, 3u, false});
+    }
+    // Only changes within the influence radius of a page that held a solid
+    // region can touch an existing receiver (from_local_bounds and
+    // from_resident_bounds select the same set).
+    const int64_t r2 = (int64_t)INFLUENCE_RADIUS * INFLUENCE_RADIUS;
+    for (const ChangeBox &b : boxes) {
+        int64_t lo[3] = {floor_div64((int64_t)b.min.x - INFLUENCE_RADIUS, 32), floor_div64((int64_t)b.min.y - INFLUENCE_RADIUS, 32),
+                         floor_div64((int64_t)b.min.z - INFLUENCE_RADIUS, 32)};
+        int64_t hi[3] = {floor_div64((int64_t)b.max.x + INFLUENCE_RADIUS, 32), floor_div64((int64_t)b.max.y + INFLUENCE_RADIUS, 32),
+                         floor_div64((int64_t)b.max.z + INFLUENCE_RADIUS, 32)};
+        // Clip the page range to the pages the previous scene holds.
+        lo[0] = std::max<int64_t>(lo[0], previous.page_min.x); hi[0] = std::min<int64_t>(hi[0], previous.page_min.x + previous.psx - 1);
+        lo[1] = std::max<int64_t>(lo[1], previous.page_min.y); hi[1] = std::min<int64_t>(hi[1], previous.page_min.y + previous.psy - 1);
+        lo[2] = std::max<int64_t>(lo[2], previous.page_min.z); hi[2] = std::min<int64_t>(hi[2], previous.page_min.z + previous.psz - 1);
+        bool near = false;
+        for (int64_t z = lo[2]; z <= hi[2] && !near; ++z)
+            for (int64_t y = lo[1]; y <= hi[1] && !near; ++y)
+                for (int64_t x = lo[0]; x <= hi[0] && !near; ++x) {
+                    Int3 page{(int32_t)x, (int32_t)y, (int32_t)z};
+                    Int3 pmin{page.x * 32, page.y * 32, page.z * 32}, pmax{pmin.x + 31, pmin.y + 31, pmin.z + 31};
+                    if (box_distance2(b.min, b.max, pmin, pmax) > r2) continue;
+                    near = previous.page_occupied(page);
+                }
+        if (near) out.boxes.push_back(b);
+    }
+    if (out.boxes.empty()) return false;
+    // Deterministic order and no duplicates (retention.rs:114-125). Two
+    // regions never share a tight box, so the weights stay one per box.
+    std::sort(out.boxes.begin(), out.boxes.end(), box_less);
+    out.boxes.erase(std::unique(out.boxes.begin(), out.boxes.end(), box_equal), out.boxes.end());
+    // finalize_change_flux: the GPU measures an edit while its one-voxel
+    // scans fit the budget; later edits get unknown flux. The reference walks
+    // a hash map here; this walks the sorted boxes, so the choice is
+    // reproducible.
+    uint64_t budget = FLUX_SCAN_BUDGET;
+    for (ChangeBox &b : out.boxes) {
+        int64_t sx = std::max<int64_t>((int64_t)b.max.x - b.min.x + 3, 1), sy = std::max<int64_t>((int64_t)b.max.y - b.min.y + 3, 1),
+                sz = std::max<int64_t>((int64_t)b.max.z - b.min.z + 3, 1);
+        uint64_t positions = (uint64_t)(sx * sy * sz);
+        if (positions <= budget) {
+            budget -= positions;
+            b.kind = 6u;
+        } else {
+            b.kind = 0u;
+            b.flux = 0.0f;
+        }
+    }
+    // Urgent: a box holding an edited voxel (retention.rs:151-158).
+    for (ChangeBox &b : out.boxes)
+        for (const Int3 &v : edits)
+            if (v.x >= b.min.x && v.y >= b.min.y && v.z >= b.min.z && v.x <= b.max.x && v.y <= b.max.y && v.z <= b.max.z) {
+                b.urgent = true;
+                break;
+            }
+    out.union_min = Int3{INT32_MAX, INT32_MAX, INT32_MAX};
+    out.union_max = Int3{INT32_MIN, INT32_MIN, INT32_MIN};
+    for (const ChangeBox &b : out.boxes) {
+        out.union_min = Int3{std::min(out.union_min.x, b.min.x), std::min(out.union_min.y, b.min.y), std::min(out.union_min.z, b.min.z)};
+        out.union_max = Int3{std::max(out.union_max.x, b.max.x), std::max(out.union_max.y, b.max.y), std::max(out.union_max.z, b.max.z)};
+    }
+    return true;
+}
+
+void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out) {
+    out.words.clear();
+    out.measures.clear();
+    out.events = (uint32_t)events.size();
+    out.entries = 0;
+    out.positions = 0;
+    auto push = [](std::vector<uint32_t> &v, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+        v.push_back(a); v.push_back(b); v.push_back(c); v.push_back(d);
+    };
+    uint32_t first = 0;
+    for (const RetentionEvent &e : events) {
+        push(out.words, (uint32_t)e.union_min.x, (uint32_t)e.union_min.y, (uint32_t)e.union_min.z, 0u);
+        push(out.words, (uint32_t)e.union_max.x, (uint32_t)e.union_max.y, (uint32_t)e.union_max.z, 0u);
+        uint32_t count = (uint32_t)e.boxes.size();
+        push(out.words, 0u, 0u, first, count);
+        first += count;
+    }
+    uint32_t record = out.events * 3u;
+    for (const RetentionEvent &e : events)
+        for (const ChangeBox &b : e.boxes) {
+            uint32_t high = b.urgent ? 8u : 0u;
+            if (b.kind == 6u) {
+                Int3 m{b.min.x - 1, b.min.y - 1, b.min.z - 1};
+                uint32_t sx = (uint32_t)(b.max.x - b.min.x + 3), sy = (uint32_t)(b.max.y - b.min.y + 3), sz = (uint32_t)(b.max.z - b.min.z + 3);
+                uint32_t count = sx * sy * sz;
+                uint32_t flux_bits;
+                std::memcpy(&flux_bits, &b.flux, 4);
+                push(out.measures, record, out.positions, count, b.voxels);
+                push(out.measures, (uint32_t)m.x, (uint32_t)m.y, (uint32_t)m.z, flux_bits);
+                push(out.measures, sx, sy, sz, 0u);
+                out.entries += 1;
+                out.positions += count;
+            }
+            uint32_t flux_bits;
+            std::memcpy(&flux_bits, &b.flux, 4);
+            push(out.words, (uint32_t)b.min.x, (uint32_t)b.min.y, (uint32_t)b.min.z, flux_bits);
+            push(out.words, (uint32_t)b.max.x, (uint32_t)b.max.y, (uint32_t)b.max.z, b.kind | high);
+            record += 2u;
+        }
 }
 
 }  // namespace vgi_host

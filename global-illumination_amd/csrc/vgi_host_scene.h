@@ -78,10 +78,65 @@ void build_emitters(const VoxelSource &scene, const std::vector<Int3> &regions, 
 struct FaceSlots {
     std::vector<std::array<uint32_t, 4>> entries;
     size_t live = 0;
-    bool dirty = true;
+    bool dirty = true;                // the whole table needs an upload (new or grown)
+    std::vector<uint32_t> touched;    // entries written since the last upload
     FaceSlots();
     long find(int32_t x, int32_t y, int32_t z, uint32_t face) const;
     void insert(int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t slot);
 };
+
+// ---------------------------------------------------------------- edit retention
+// One changed box of an edit with its measured-retention weight
+// (geometry.rs:110-229 ChangeWeight; retention.rs:65-74 RetentionEvent).
+struct ChangeBox {
+    Int3 min, max;
+    float flux;       // the changed voxels' own emission change (measured edits
+                      // add the brightest outgoing luminance on the GPU)
+    uint32_t voxels;  // changed voxels
+    uint32_t kind;    // 0 unknown flux, 3 edit, 6 edit measured on the GPU
+    bool urgent;      // holds an edited voxel
+};
+
+struct RetentionEvent {
+    Int3 union_min, union_max;
+    std::vector<ChangeBox> boxes;  // sorted by (min, max)
+};
+
+// A region's 16^3 materials (x fastest) before and after an edit.
+struct RegionDelta {
+    Int3 coord;
+    std::vector<uint16_t> before, after;
+    bool solid_before, solid_after;
+};
+
+// The scene before the edit, as the event needs it: the bounding box of its
+// solid regions and which 32^3 pages hold a solid region.
+struct PreviousScene {
+    bool any_solid;
+    Int3 solid_min, solid_max;  // region coordinates
+    Int3 page_min;              // page coordinate of page_solid[0]
+    int psx, psy, psz;
+    const uint8_t *page_solid;  // x fastest
+    bool page_occupied(Int3 page) const;
+};
+
+// The retention event of one edit: SceneChanges::with_materials
+// (geometry.rs:110-229) over the changed regions (sorted z, y, x),
+// GeometryChange::from_bounds (geometry.rs:299-407) at the influence radius,
+// finalize_change_flux (publication.rs:423-454) and SpatialHistory::event
+// (retention.rs:110-167). Returns false when nothing within reach changed.
+bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
+                const std::vector<Int3> &edits, RetentionEvent &out);
+
+// encode_events (gpu/resident.rs:1319-1417): event headers, then change
+// records; measures are three uint4 per edit whose flux the GPU measures.
+struct EncodedEvents {
+    std::vector<uint32_t> words, measures;
+    uint32_t events = 0, entries = 0, positions = 0;
+};
+void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out);
+
+constexpr int INFLUENCE_RADIUS = 192;     // retention.rs:32-39 at HISTORY_LIMIT 65536
+constexpr uint32_t FLUX_SCAN_BUDGET = 16384;  // publication.rs:413
 
 }  // namespace vgi_host

@@ -60,6 +60,16 @@ void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint3
 void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t high_water, uint32_t *state, uint32_t *checkpoint,
                          uint32_t *meta, float physical, hipStream_t stream);
 
+// Edit flux from the resident statistics (measure_edits + finish_edits);
+// measure_max must be cleared first (one word per entry).
+void launch_measure_edits(const uint4 *measures, uint32_t entries, uint32_t positions, const uint4 *index, uint32_t mask,
+                          const uint32_t *state, const uint32_t *meta, const float2 *material_light, uint32_t *measure_max,
+                          uint4 *events, hipStream_t stream);
+
+void launch_scatter_bound(const uint32_t *index, const uint4 *tasks, const uint32_t *slots, uint32_t n, uint4 *bound_tasks,
+                          uint32_t *bound_slots, hipStream_t stream);
+void launch_scatter_words(const uint2 *writes, uint32_t n, uint32_t *words, hipStream_t stream);
+
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 
 }  // namespace vgi

@@ -666,6 +666,99 @@ __global__ void fp64_accumulate_kernel(const float4 *__restrict__ results, uint3
     s[6] += k;
 }
 
+// The face index (voxel_accumulate.wgsl:357-373 face_slot): a face's slot
+// or NONE.
+__device__ __forceinline__ uint32_t face_slot(const uint4 *__restrict__ index, uint32_t mask, int vx, int vy, int vz, uint32_t face) {
+    uint32_t probe = (hash_voxel(vx, vy, vz) ^ pcg(face + 1u)) & mask;
+    for (uint32_t attempt = 0u; attempt < 64u; ++attempt) {
+        uint4 e = index[probe];
+        if (e.w == NONE) return NONE;
+        if ((int)e.x == vx && (int)e.y == vy && (int)e.z == vz && (e.w >> 29u) == face) return e.w & 0x1fffffffu;
+        probe = (probe + 1u) & mask;
+    }
+    return NONE;
+}
+
+// measure_edits (voxel_accumulate.wgsl:375-407): one scan position per
+// thread; the brightest outgoing luminance (albedo luminance times cached
+// incident plus emission luminance) of any resident face of that voxel,
+// max-reduced per edit. Entries hold ascending, contiguous position ranges,
+// so a binary search finds the same entry as the reference's linear scan.
+__global__ void measure_edits_kernel(const uint4 *__restrict__ measures, uint32_t entries, uint32_t positions,
+                                     const uint4 *__restrict__ index, uint32_t mask, const uint32_t *__restrict__ state,
+                                     const uint32_t *__restrict__ meta, const float2 *__restrict__ material_light,
+                                     uint32_t *__restrict__ measure_max) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= positions) return;
+    uint32_t lo = 0u, hi = entries;  // the last entry whose first position <= t
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (measures[mid * 3u].y <= t) lo = mid;
+        else hi = mid;
+    }
+    uint32_t e = lo;
+    uint4 a = measures[e * 3u], b = measures[e * 3u + 1u], c = measures[e * 3u + 2u];
+    uint32_t local = t - a.y;
+    int px = (int)b.x + (int)(local % c.x), py = (int)b.y + (int)((local / c.x) % c.y), pz = (int)b.z + (int)(local / (c.x * c.y));
+    const f3 w = mk(0.2126f, 0.7152f, 0.0722f);
+    float brightest = 0.0f;
+    bool observed = false;
+    for (uint32_t face = 0u; face < 6u; ++face) {
+        uint32_t slot = face_slot(index, mask, px, py, pz, face);
+        if (slot == NONE) continue;
+        uint32_t identity = meta[(size_t)slot * 12u + 3u];
+        if ((identity & 8u) == 0u) continue;
+        const uint32_t *st = state + (size_t)slot * 36u;
+        if ((st[5] & 0x7fffffffu) == 0u) continue;
+        observed = true;
+        float incident = dot3(w, load3(st)) + dot3(w, load3(st + 16));
+        float2 light = material_light[identity >> 16];
+        brightest = fmx(brightest, light.x * incident + light.y);
+    }
+    if (observed) atomicMax(measure_max + e, __float_as_uint(brightest) | 0x80000000u);
+}
+
+// finish_edits (voxel_accumulate.wgsl:409-430): each measured edit's flux,
+// base plus the brightest outgoing luminance times two faces per changed
+// voxel; an edit no resident face observed falls back to unknown flux.
+__global__ void finish_edits_kernel(const uint4 *__restrict__ measures, uint32_t entries, const uint32_t *__restrict__ measure_max,
+                                    uint4 *__restrict__ events) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= entries) return;
+    uint4 a = measures[e * 3u];
+    float base_flux = __uint_as_float(measures[e * 3u + 1u].w);
+    uint32_t observation = measure_max[e];
+    if ((observation & 0x80000000u) == 0u) {
+        events[a.x + 1u].w &= 0xfffffff8u;
+        return;
+    }
+    float brightest = __uint_as_float(observation & 0x7fffffffu);
+    events[a.x].w = __float_as_uint(base_flux + brightest * 2.0f * (float)a.w);
+    events[a.x + 1u].w = (events[a.x + 1u].w & 0xfffffff8u) | 3u;
+}
+
+// Bound-face updates after edits: entries (task, slot) written at their
+// indices; indices are distinct.
+__global__ void scatter_bound_kernel(const uint32_t *__restrict__ index, const uint4 *__restrict__ tasks,
+                                     const uint32_t *__restrict__ slots, uint32_t n, uint4 *__restrict__ bound_tasks,
+                                     uint32_t *__restrict__ bound_slots) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t at = index[i];
+    bound_tasks[at * 2u] = tasks[i * 2u];
+    bound_tasks[at * 2u + 1u] = tasks[i * 2u + 1u];
+    if (bound_slots) bound_slots[at] = slots[i];
+}
+
+// Single-word writes into the slot metadata (identity words after a face's
+// material changed): pairs (word index, value), indices distinct.
+__global__ void scatter_words_kernel(const uint2 *__restrict__ writes, uint32_t n, uint32_t *__restrict__ words) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint2 w = writes[i];
+    words[w.x] = w.y;
+}
+
 // A plain streaming copy (16 B per lane), the known-size reference against
 // which hardware byte counters are scaled.
 __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
@@ -773,6 +866,26 @@ void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t hig
     if (high_water == 0) return;
     hipLaunchKernelGGL(apply_events_kernel, dim3(blocks(high_water, 256)), dim3(256), 0, stream, events, event_count, high_water,
                        state, checkpoint, meta, physical, 192u * 192u, 64u * 64u, 48u * 48u);
+}
+
+void launch_measure_edits(const uint4 *measures, uint32_t entries, uint32_t positions, const uint4 *index, uint32_t mask,
+                          const uint32_t *state, const uint32_t *meta, const float2 *material_light, uint32_t *measure_max,
+                          uint4 *events, hipStream_t stream) {
+    if (entries == 0) return;
+    if (positions)
+        hipLaunchKernelGGL(measure_edits_kernel, dim3(blocks(positions, 256)), dim3(256), 0, stream, measures, entries, positions, index,
+                           mask, state, meta, material_light, measure_max);
+    hipLaunchKernelGGL(finish_edits_kernel, dim3(blocks(entries, 256)), dim3(256), 0, stream, measures, entries, measure_max, events);
+}
+
+void launch_scatter_bound(const uint32_t *index, const uint4 *tasks, const uint32_t *slots, uint32_t n, uint4 *bound_tasks,
+                          uint32_t *bound_slots, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(scatter_bound_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, index, tasks, slots, n, bound_tasks, bound_slots);
+}
+void launch_scatter_words(const uint2 *writes, uint32_t n, uint32_t *words, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(scatter_words_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, writes, n, words);
 }
 
 void launch_fp64_accumulate(const float4 *results, uint32_t count, double *sums, hipStream_t stream) {

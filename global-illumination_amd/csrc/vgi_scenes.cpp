@@ -306,4 +306,40 @@ int64_t vgi_scene_faces(const uint16_t *v, const int32_t dims[3], int interior_o
     return count;
 }
 
+int64_t vgi_scene_edits(uint32_t seed, uint32_t frame, uint32_t count, uint32_t edge, int32_t *xyz, uint16_t *values,
+                        int64_t capacity) {
+    int64_t n = 0;
+    auto emit = [&](int x, int y, int z, uint16_t v) {
+        if (xyz && n < capacity) {
+            xyz[n * 3] = x; xyz[n * 3 + 1] = y; xyz[n * 3 + 2] = z;
+            values[n] = v;
+        }
+        ++n;
+    };
+    uint32_t stream = pcg(seed ^ pcg(frame + 0x7f4a7c15u));
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t kind = draw(stream, i * 6u) % 3u;
+        int x = (int)(draw(stream, i * 6u + 1u) % edge), y = (int)(draw(stream, i * 6u + 2u) % edge),
+            z = (int)(draw(stream, i * 6u + 3u) % edge);
+        if (kind == 0u) {
+            emit(x, y, z, (uint16_t)(1u + draw(stream, i * 6u + 4u) % 4u));
+        } else if (kind == 1u) {
+            emit(x, y, z, AIR);
+        } else {
+            float r = 1.0f + 4.0f * unit(stream, i * 6u + 5u);
+            float cx = (float)x + 0.5f, cy = (float)y + 0.5f, cz = (float)z + 0.5f;
+            int lo[3] = {(int)std::floor(cx - r), (int)std::floor(cy - r), (int)std::floor(cz - r)};
+            int hi[3] = {(int)std::ceil(cx + r), (int)std::ceil(cy + r), (int)std::ceil(cz + r)};
+            for (int c = 0; c < 3; ++c) { lo[c] = std::max(lo[c], 0); hi[c] = std::min(hi[c], (int)edge - 1); }
+            for (int vz = lo[2]; vz <= hi[2]; ++vz)
+                for (int vy = lo[1]; vy <= hi[1]; ++vy)
+                    for (int vx = lo[0]; vx <= hi[0]; ++vx) {
+                        float dx = vx + 0.5f - cx, dy = vy + 0.5f - cy, dz = vz + 0.5f - cz;
+                        if (dx * dx + dy * dy + dz * dz <= r * r) emit(vx, vy, vz, AIR);
+                    }
+        }
+    }
+    return n;
+}
+
 }  // extern "C"

@@ -85,9 +85,17 @@ int vgi_set_clock(vgi_tracer *tracer, int64_t milliseconds);
  * the GPU; the top level is grown in exact FIFO order on the host. ---- */
 int vgi_upload_pages(vgi_tracer *tracer, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels,
                      const int32_t *removed_xyz, uint32_t removed);
-/* VoxelGiInput::edit_voxel (input.rs:90-93): set voxels; edited regions are
- * re-packed on the GPU and the top level is updated incrementally
- * (TopLevelAadf::updated, world.rs:477-583). */
+/* VoxelGiInput::edit_voxel (input.rs:90-93): set voxels (in order; a later
+ * edit of a voxel wins); edited regions are re-packed on the GPU and the top
+ * level is updated incrementally (TopLevelAadf::updated, world.rs:477-583).
+ * With the resident cache, the edit's retention event is built
+ * (SceneChanges::with_materials geometry.rs:110-229, GeometryChange::from_bounds
+ * geometry.rs:299-407, finalize_change_flux publication.rs:423-454,
+ * SpatialHistory::event retention.rs:110-167) and queued; the next batch or
+ * frame measures its flux on the GPU (measure_edits / finish_edits,
+ * voxel_accumulate.wgsl:375-430) and applies it to every slot
+ * (apply_events, voxel_accumulate.wgsl:295-332) before its prepare. With
+ * vgi_bind_exposed, bound faces follow the edit (pages.rs:451-458 rescan). */
 int vgi_edit_voxels(vgi_tracer *tracer, const int32_t *xyz, const uint16_t *values, uint32_t count);
 
 /* ---- optics and sources: VoxelGiMaterials (world.rs:90-174) as packed by
@@ -140,6 +148,21 @@ float vgi_last_frame_ms(vgi_tracer *tracer);
  * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
  * the frame ran with VGI_COUNT_RAYS. */
 int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uint64_t *counters);
+/* Bind every exposed face of the loaded scene (a solid voxel with loaded air
+ * across the face; z, y, x, face order) and keep the bound set equal to the
+ * exposed set through later edits: covered faces leave (the last bound face
+ * takes the place), newly exposed faces join at the end. Needs the resident
+ * cache. The reference discovers these faces by scanning changed pages for
+ * surface probes (world.rs scan_surface_probe_pages, pages.rs:451-458). */
+int vgi_bind_exposed(vgi_tracer *tracer);
+uint32_t vgi_bound_count(vgi_tracer *tracer);
+/* The bound faces' 32-byte tasks in bound order (sequence words as the last
+ * prepare left them). */
+int vgi_read_bound_tasks(vgi_tracer *tracer, vgi_face_task *out, uint32_t count);
+/* The encoded retention events of the last applied edits, after the GPU's
+ * finish_edits filled in measured flux (resident.rs:1326-1417 layout); returns
+ * the word count, copies at most `capacity` words when out is not NULL. */
+int64_t vgi_read_events(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
 /* Raw resident state (9 uint4 per slot, voxel_accumulate.wgsl:1-4) of slots [first, first+count). */
 int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 
@@ -202,9 +225,16 @@ int vgi_scene_generate(int workload, uint32_t seed, uint32_t edge, uint16_t *vox
 /* Materials and lights of a workload (256x3 diffuse, 256x3 emission, lights). */
 int vgi_scene_optics(int workload, uint32_t seed, float *diffuse_rgb, float *emission_rgb, vgi_light *lights,
                      uint32_t *light_count, float environment[3]);
-/* Exposed faces of a dense grid (solid voxel, air or out-of-grid neighbour),
+/* Exposed faces of a dense grid (solid voxel, air neighbour inside the grid),
  * optionally limited to the box interior; returns the count. */
 int64_t vgi_scene_faces(const uint16_t *voxels, const int32_t dims[3], int interior_only, vgi_face_task *out,
+                        int64_t capacity);
+/* One frame of the edit stream over an edge^3 box: `count` seeded edits drawn
+ * from PCG(seed, frame, i), each a voxel set (material 1-4), a voxel clear,
+ * or a sphere removal of radius U[1,5] (every voxel whose centre lies within
+ * the radius), expanded to voxel edits in order; returns the voxel count
+ * (copies at most `capacity`: xyz 3 x i32 and value u16 per voxel). */
+int64_t vgi_scene_edits(uint32_t seed, uint32_t frame, uint32_t count, uint32_t edge, int32_t *xyz, uint16_t *values,
                         int64_t capacity);
 
 #ifdef __cplusplus

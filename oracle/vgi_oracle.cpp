@@ -26,6 +26,7 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -72,6 +73,7 @@ static inline F3 normalize3(F3 a) {
 }
 static inline F3 to_f3(I3 v) { return f3((float)v.x, (float)v.y, (float)v.z); }
 static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float fval(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
 static inline uint64_t key_of(I3 v) {
@@ -1029,6 +1031,7 @@ struct Tracer {
     float measured_retention = 1.0f;
     float response = 0.075f;  // settings.rs:266 display_response 75 ms
     std::unordered_map<uint64_t, uint32_t> slot_of;
+    std::vector<uint32_t> last_event;  // words of the last edit's retention event
 };
 
 // ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
@@ -2152,9 +2155,16 @@ int orc_cache_register(void *tp, const uint32_t *tasks, uint32_t count, uint32_t
         I3 v{(int32_t)tasks[i * 8], (int32_t)tasks[i * 8 + 1], (int32_t)tasks[i * 8 + 2]};
         uint32_t face = tasks[i * 8 + 3];
         long f = t.face_index.find(v, face);
-        if (f >= 0) { slots[i] = t.face_index.entries[f][3] & 0x1fffffffu; continue; }
         uint32_t m = t.scene->material_at(v);
-        slots[i] = allocate_slot(t, v, face, m == 0xffffu || m == AIR ? 0u : m);
+        uint32_t material = m == 0xffffu || m == AIR ? 0u : m;
+        if (f >= 0) {
+            slots[i] = t.face_index.entries[f][3] & 0x1fffffffu;
+            // resident.rs:443-450 material replacement of a relit face.
+            uint32_t &identity = t.meta[(size_t)slots[i] * 12 + 3];
+            if ((identity >> 16) != material) identity = face | 8u | (material << 16);
+            continue;
+        }
+        slots[i] = allocate_slot(t, v, face, material);
     }
     return 0;
 }
@@ -2171,10 +2181,228 @@ int orc_cache_state(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
     return 0;
 }
 
+int orc_cache_meta(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)(first + count) * 12 > t.meta.size()) return -1;
+    memcpy(out, &t.meta[(size_t)first * 12], (size_t)count * 12 * 4);
+    return 0;
+}
+
 int orc_cache_apply_events(void *tp, const uint32_t *events, uint32_t event_count, uint32_t words) {
     (void)words;
     apply_events(*(Tracer *)tp, events, event_count, 192u * 192u, 64u * 64u, 48u * 48u);
     return 0;
+}
+
+// ---------------------------------------------------------------- edit retention events
+// One change box with its weight (geometry.rs:55-66 ChangeWeight).
+struct OrcChange {
+    I3 mn, mx;
+    int kind;  // 0 unknown, 3 edit
+    bool measure;
+    float flux;
+    uint32_t voxels;
+};
+
+static int64_t bounds_distance2(I3 amin, I3 amax, I3 bmin, I3 bmax) {  // geometry.rs:239-247
+    int64_t d[3] = {std::max<int64_t>({(int64_t)amin.x - bmax.x, (int64_t)bmin.x - amax.x, 0}),
+                    std::max<int64_t>({(int64_t)amin.y - bmax.y, (int64_t)bmin.y - amax.y, 0}),
+                    std::max<int64_t>({(int64_t)amin.z - bmax.z, (int64_t)bmin.z - amax.z, 0})};
+    int64_t s = 0;
+    for (int64_t v : d) { v = std::min<int64_t>(v, 4096); s += v * v; }
+    return s;
+}
+
+static float emission_lum(const Tracer &t, uint16_t m) {  // geometry.rs:92-100
+    if (m == AIR) return 0.0f;
+    const float *e = &t.emission_rgb[(size_t)m * 3];
+    return 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+}
+
+// Edit the scene and build, encode and measure the edit's retention event:
+// SceneChanges::with_materials (geometry.rs:110-229) over the rebuilt regions
+// of the edited pages, GeometryChange::from_resident_bounds (geometry.rs:368-
+// 403) at INFLUENCE_RADIUS 192, finalize_change_flux (publication.rs:423-454;
+// the measure budget is spent in sorted-box order, where the reference walks
+// a hash map), SpatialHistory::event (retention.rs:110-167), encode_events
+// (gpu/resident.rs:1326-1417), then measure_edits and finish_edits
+// (voxel_accumulate.wgsl:375-430) over this tracer's resident statistics.
+// Keeps the event words (one event) for orc_last_event and returns their
+// count, 0 when nothing within reach changed.
+int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uint32_t count) {
+    Tracer &t = *(Tracer *)tp;
+    t.last_event.clear();
+    Scene &s = *t.scene;
+    // The previous scene: its regions (kept alive), solid bounds, solid pages.
+    std::unordered_map<uint64_t, std::shared_ptr<Region>> previous = s.regions;
+    bool any_solid = false;
+    I3 smin{0, 0, 0}, smax{0, 0, 0};
+    std::map<std::tuple<int, int, int>, I3> solid_pages;
+    for (auto &kv : previous) {
+        const Region &r = *kv.second;
+        if (!r.any_solid) continue;
+        if (!any_solid) { smin = smax = r.coord; any_solid = true; }
+        smin = I3{std::min(smin.x, r.coord.x), std::min(smin.y, r.coord.y), std::min(smin.z, r.coord.z)};
+        smax = I3{std::max(smax.x, r.coord.x), std::max(smax.y, r.coord.y), std::max(smax.z, r.coord.z)};
+        I3 page{floordiv(r.coord.x, 2), floordiv(r.coord.y, 2), floordiv(r.coord.z, 2)};
+        solid_pages[std::make_tuple(page.z, page.y, page.x)] = page;
+    }
+    std::map<std::tuple<int, int, int>, I3> edited_pages;
+    std::vector<I3> edits;
+    for (uint32_t i = 0; i < count; ++i) {
+        I3 v{xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]};
+        I3 p{floordiv(v.x, 32), floordiv(v.y, 32), floordiv(v.z, 32)};
+        if (!s.pages.count(key_of(p))) continue;
+        edited_pages[std::make_tuple(p.z, p.y, p.x)] = p;
+        edits.push_back(v);
+    }
+    orc_scene_edit(&s, xyz, values, count);
+    // Changed regions: the eight of every rebuilt page, z, y, x order.
+    std::map<std::tuple<int, int, int>, I3> changed;
+    for (auto &kv : edited_pages)
+        for (int o = 0; o < 8; ++o) {
+            I3 c{kv.second.x * 2 + (o & 1), kv.second.y * 2 + ((o >> 1) & 1), kv.second.z * 2 + (o >> 2)};
+            changed[std::make_tuple(c.z, c.y, c.x)] = c;
+        }
+    std::vector<OrcChange> bounds;
+    for (auto &kv : changed) {
+        I3 c = kv.second;
+        auto pb = previous.find(key_of(c));
+        const Region *before = pb == previous.end() ? nullptr : pb->second.get();
+        const Region *after = s.region(c);
+        I3 origin{c.x * 16, c.y * 16, c.z * 16};
+        bool tight_empty = true;
+        I3 tmin{0, 0, 0}, tmax{0, 0, 0};
+        uint32_t changed_voxels = 0;
+        float flux = 0.0f;
+        auto record = [&](uint16_t a, uint16_t b, I3 emin, I3 emax) {
+            if (a == b) return;
+            if (tight_empty) { tmin = emin; tmax = emax; tight_empty = false; }
+            tmin = I3{std::min(tmin.x, emin.x), std::min(tmin.y, emin.y), std::min(tmin.z, emin.z)};
+            tmax = I3{std::max(tmax.x, emax.x), std::max(tmax.y, emax.y), std::max(tmax.z, emax.z)};
+            uint32_t volume = (uint32_t)std::max<int64_t>((int64_t)(emax.x - emin.x + 1) * (emax.y - emin.y + 1) * (emax.z - emin.z + 1), 1);
+            changed_voxels += volume;
+            flux += std::fabs(emission_lum(t, a) - emission_lum(t, b)) * 6.0f * (float)volume;
+        };
+        auto uniform_of = [](const Region *r, uint16_t &m) {
+            if (!r) { m = AIR; return true; }
+            if (r->storage == ST_EMPTY) { m = AIR; return true; }
+            if (r->storage == ST_UNIFORM) { m = r->uniform; return true; }
+            return false;
+        };
+        uint16_t ua, ub;
+        if (uniform_of(before, ua) && uniform_of(after, ub)) {
+            record(ua, ub, origin, I3{origin.x + 15, origin.y + 15, origin.z + 15});
+        } else {
+            for (int z = 0; z < 16; ++z)
+                for (int y = 0; y < 16; ++y)
+                    for (int x = 0; x < 16; ++x) {
+                        I3 p{origin.x + x, origin.y + y, origin.z + z};
+                        record(before ? before->material(x, y, z) : AIR, after ? after->material(x, y, z) : AIR, p, p);
+                    }
+        }
+        bool in_occupied = any_solid && c.x >= smin.x && c.y >= smin.y && c.z >= smin.z && c.x <= smax.x && c.y <= smax.y && c.z <= smax.z;
+        bool transport = (before && before->any_solid) || (after && after->any_solid) || in_occupied;
+        bool residency = before == nullptr || after == nullptr;
+        if (!transport || residency || tight_empty) continue;  // edits never change residency
+        bounds.push_back(OrcChange{tmin, tmax, 3, false, flux, changed_voxels});
+    }
+    // from_resident_bounds: each solid page gathers the changes within reach.
+    const int64_t r2 = 192 * 192;
+    std::vector<OrcChange> nearby;
+    for (auto &kv : solid_pages) {
+        I3 pg = kv.second;
+        I3 pmin{pg.x * 32, pg.y * 32, pg.z * 32}, pmax{pmin.x + 31, pmin.y + 31, pmin.z + 31};
+        for (const OrcChange &c : bounds)
+            if (bounds_distance2(c.mn, c.mx, pmin, pmax) <= r2) nearby.push_back(c);
+    }
+    auto less = [](const OrcChange &a, const OrcChange &b) {
+        return std::make_tuple(a.mn.x, a.mn.y, a.mn.z, a.mx.x, a.mx.y, a.mx.z) < std::make_tuple(b.mn.x, b.mn.y, b.mn.z, b.mx.x, b.mx.y, b.mx.z);
+    };
+    auto same = [](const OrcChange &a, const OrcChange &b) {
+        return a.mn.x == b.mn.x && a.mn.y == b.mn.y && a.mn.z == b.mn.z && a.mx.x == b.mx.x && a.mx.y == b.mx.y && a.mx.z == b.mx.z;
+    };
+    std::sort(nearby.begin(), nearby.end(), less);
+    nearby.erase(std::unique(nearby.begin(), nearby.end(), same), nearby.end());
+    if (nearby.empty()) return 0;
+    uint64_t budget = 16384;
+    for (OrcChange &c : nearby) {
+        int64_t sx = std::max<int64_t>((int64_t)c.mx.x - c.mn.x + 3, 1), sy = std::max<int64_t>((int64_t)c.mx.y - c.mn.y + 3, 1),
+                sz = std::max<int64_t>((int64_t)c.mx.z - c.mn.z + 3, 1);
+        uint64_t positions = (uint64_t)(sx * sy * sz);
+        if (positions <= budget) { budget -= positions; c.measure = true; }
+        else { c.kind = 0; c.flux = 0.0f; }
+    }
+    // encode_events, one event.
+    std::vector<uint32_t> w;
+    auto push = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) { w.push_back(a); w.push_back(b); w.push_back(c); w.push_back(d); };
+    I3 umin = nearby[0].mn, umax = nearby[0].mx;
+    for (const OrcChange &c : nearby) {
+        umin = I3{std::min(umin.x, c.mn.x), std::min(umin.y, c.mn.y), std::min(umin.z, c.mn.z)};
+        umax = I3{std::max(umax.x, c.mx.x), std::max(umax.y, c.mx.y), std::max(umax.z, c.mx.z)};
+    }
+    push((uint32_t)umin.x, (uint32_t)umin.y, (uint32_t)umin.z, 0u);
+    push((uint32_t)umax.x, (uint32_t)umax.y, (uint32_t)umax.z, 0u);
+    push(0u, 0u, 0u, (uint32_t)nearby.size());
+    struct Measure { uint32_t record; I3 mn; uint32_t sx, sy, sz, voxels; float flux; };
+    std::vector<Measure> measures;
+    uint32_t record = 3;
+    for (const OrcChange &c : nearby) {
+        bool urgent = false;
+        for (const I3 &v : edits)
+            if (bounds_distance2(c.mn, c.mx, v, v) == 0) { urgent = true; break; }
+        uint32_t kind = c.kind == 0 ? 0u : (c.measure ? 6u : 3u);
+        if (c.measure)
+            measures.push_back(Measure{record, I3{c.mn.x - 1, c.mn.y - 1, c.mn.z - 1}, (uint32_t)(c.mx.x - c.mn.x + 3),
+                                       (uint32_t)(c.mx.y - c.mn.y + 3), (uint32_t)(c.mx.z - c.mn.z + 3), c.voxels, c.flux});
+        push((uint32_t)c.mn.x, (uint32_t)c.mn.y, (uint32_t)c.mn.z, fbits(c.flux));
+        push((uint32_t)c.mx.x, (uint32_t)c.mx.y, (uint32_t)c.mx.z, kind | (urgent ? 8u : 0u));
+        record += 2;
+    }
+    // measure_edits + finish_edits from this tracer's statistics.
+    float light[256][2];
+    for (int m = 0; m < 256; ++m) {
+        const float *d = &t.diffuse_rgb[m * 3], *e = &t.emission_rgb[m * 3];
+        light[m][0] = 0.2126f * d[0] + 0.7152f * d[1] + 0.0722f * d[2];
+        light[m][1] = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+    }
+    for (const Measure &m : measures) {
+        bool observed = false;
+        float brightest = 0.0f;
+        for (uint32_t z = 0; z < m.sz; ++z)
+            for (uint32_t y = 0; y < m.sy; ++y)
+                for (uint32_t x = 0; x < m.sx; ++x) {
+                    I3 p{m.mn.x + (int)x, m.mn.y + (int)y, m.mn.z + (int)z};
+                    for (uint32_t face = 0; face < 6; ++face) {
+                        long f = t.face_index.find(p, face);
+                        if (f < 0) continue;
+                        uint32_t slot = t.face_index.entries[f][3] & 0x1fffffffu;
+                        uint32_t identity = t.meta[(size_t)slot * 12 + 3];
+                        if ((identity & 8u) == 0u) continue;
+                        const uint32_t *st = &t.state[(size_t)slot * 36];
+                        if ((st[5] & 0x7fffffffu) == 0u) continue;
+                        observed = true;
+                        float mean_l = 0.2126f * fval(st[0]) + 0.7152f * fval(st[1]) + 0.0722f * fval(st[2]);
+                        float direct_l = 0.2126f * fval(st[16]) + 0.7152f * fval(st[17]) + 0.0722f * fval(st[18]);
+                        float incident = mean_l + direct_l;
+                        const float *lm = light[identity >> 16];
+                        brightest = std::max(brightest, lm[0] * incident + lm[1]);
+                    }
+                }
+        uint32_t *flux_word = &w[(size_t)m.record * 4 + 3];
+        uint32_t *kind_word = &w[(size_t)(m.record + 1) * 4 + 3];
+        if (!observed) { *kind_word &= 0xfffffff8u; continue; }
+        *flux_word = fbits(m.flux + brightest * 2.0f * (float)m.voxels);
+        *kind_word = (*kind_word & 0xfffffff8u) | 3u;
+    }
+    t.last_event = w;
+    return (int64_t)w.size();
+}
+
+int64_t orc_last_event(void *tp, uint32_t *out, int64_t capacity) {
+    const Tracer &t = *(const Tracer *)tp;
+    if (out) memcpy(out, t.last_event.data(), (size_t)std::min<int64_t>((int64_t)t.last_event.size(), capacity) * 4);
+    return (int64_t)t.last_event.size();
 }
 
 uint32_t orc_curve_limit(uint32_t d2) { return curve_limit(d2); }

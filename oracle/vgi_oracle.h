@@ -119,9 +119,16 @@ int orc_cache_batch(void *tracer, uint32_t *tasks, const uint32_t *slots, uint32
 /* Raw state words (9 uvec4 per slot) of `count` slots from `first`. */
 int orc_cache_state(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
 /* Retention events: see voxel_retention.wgsl:29-96 for the layout. */
+int orc_cache_meta(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
 int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_count,
                            uint32_t words);
 /* voxel_retention.wgsl:9-12 */
+/* Edit the scene and build its retention event (one event's encoded words,
+ * measured against this tracer's resident statistics); returns the word
+ * count, 0 when nothing within reach changed. orc_last_event copies the
+ * words (at most `capacity`). */
+int64_t orc_edit_event(void *tracer, const int32_t *xyz, const uint16_t *values, uint32_t count);
+int64_t orc_last_event(void *tracer, uint32_t *out, int64_t capacity);
 uint32_t orc_curve_limit(uint32_t d2);
 /* One traversal (voxel_traversal.wgsl:126-219): out_i = voxel xyz, material,
  * status, face; out_f = position xyz, normal xyz. */

@@ -74,6 +74,9 @@ def lib():
         "orc_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_apply_events": (ctypes.c_int, [P, P, u32, u32]),
         "orc_curve_limit": (u32, [u32]),
+        "orc_edit_event": (i64, [P, P, P, u32]),
+        "orc_last_event": (i64, [P, P, i64]),
+        "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_trace_ray": (ctypes.c_int, [P, P, P, f32, P, P]),
         "orc_mlp_forward": (ctypes.c_int, [P, P, u32, P, P, P]),
         "orc_mlp_gradient": (ctypes.c_int, [P, P, P, u32, P, P, P, P]),
@@ -219,6 +222,22 @@ class Tracer:
     def cache_state(self, first, count):
         out = np.zeros((count, 36), np.uint32)
         assert lib().orc_cache_state(self.h, first, count, ptr(out)) == 0
+        return out
+
+    def edit_event(self, xyz, values):
+        """Edit the oracle scene; the edit's encoded retention event (measured
+        against this tracer's resident statistics), empty when none."""
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        values = np.ascontiguousarray(values, np.uint16).reshape(-1)
+        n = lib().orc_edit_event(self.h, ptr(xyz), ptr(values), len(values))
+        out = np.zeros(n, np.uint32)
+        if n:
+            lib().orc_last_event(self.h, ptr(out), n)
+        return out
+
+    def cache_meta(self, first, count):
+        out = np.zeros((count, 12), np.uint32)
+        assert lib().orc_cache_meta(self.h, first, count, ptr(out)) == 0
         return out
 
     def apply_events(self, events, event_count):
