@@ -188,6 +188,61 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     return out
 
 
+def edit_stream(vgi, args):
+    """The edit stream: a 512^3 noise terrain (~30% solid, 16 emitters, sun),
+    each frame 1000 seeded edits (voxel set / clear / sphere removal r 1-5),
+    the incremental NAADF update and retention event, the bound faces
+    following the exposed surface, then one resident frame of 2 rays per face
+    (2 bounces, world cache on, temporal merge)."""
+    gen, seed, edge = 4, 0x5EED0004, 512
+    t0 = time.time()
+    grid = vgi.generate(gen, seed, edge)
+    diffuse, emission, lights, env = vgi.optics(gen, seed)
+    coords, pages = vgi.pages_of(grid)
+    g = vgi.GpuTracer(0)
+    g.enable_resident()
+    g.upload_pages(coords, pages)
+    g.set_materials(diffuse, emission)
+    g.set_lights(lights)
+    g.set_transport(bounces=2, samples=2, environment=tuple(env), measured_retention=1.0)
+    g.bind_exposed()
+    print("edit stream: %d faces bound, set up in %.1f s" % (g.bound_count(), time.time() - t0), file=sys.stderr, flush=True)
+    edits = [vgi.scene_edits(seed, f + 1, 1000, edge) for f in range(args.edit_frames)]
+    for _ in range(2):
+        g.run_frame()
+    g.sync()
+    edit_ms, build_ms, frame_ms, trace_ms, rays, voxels = [], [], [], [], 0, 0
+    t_start = time.perf_counter()
+    for f in range(args.edit_frames):
+        xyz, values = edits[f]
+        t1 = time.perf_counter()
+        g.edit_voxels(xyz, values)
+        edit_ms.append((time.perf_counter() - t1) * 1000.0)
+        build_ms.append(float(g.timings()[3]))
+        g.run_frame(vgi.COUNT_RAYS)
+        g.sync()
+        frame_ms.append(g.last_frame_ms())
+        trace_ms.append(g.last_trace_ms())
+        rays += int(g.read_counters_bound()[0])
+        voxels += len(values)
+    elapsed = time.perf_counter() - t_start
+    faces = g.bound_count()
+    g.close()
+    return {
+        "workload": "512^3 noise terrain ~30%% solid, 16 emitters, sun; %d frames x 1000 edits (set/clear/sphere r1-5); "
+                    "2 rays/face, 2 bounces, world cache, measured retention" % args.edit_frames,
+        "frames": args.edit_frames,
+        "voxel_edits_per_frame": round(voxels / args.edit_frames, 1),
+        "faces_bound_at_end": int(faces),
+        "ms_per_frame_wall": round(elapsed * 1000.0 / args.edit_frames, 3),
+        "edit_ms_host_median": round(float(np.median(edit_ms)), 3),
+        "naadf_pack_gpu_ms_median": round(float(np.median(build_ms)), 3),
+        "frame_gpu_ms_median": round(float(np.median(frame_ms)), 3),
+        "trace_gpu_ms_median": round(float(np.median(trace_ms)), 3),
+        "mrays_per_s_trace": round(rays / (float(np.sum(trace_ms)) / 1000.0) / 1e6, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +258,8 @@ def main():
     ap.add_argument("--plain-frames", type=int, default=900)
     ap.add_argument("--learned-frames", type=int, default=400)
     ap.add_argument("--pretrain-frames", type=int, default=400)
+    ap.add_argument("--no-edit-stream", action="store_true", help="skip the edit-stream leg")
+    ap.add_argument("--edit-frames", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -313,6 +370,8 @@ def main():
     if os.path.exists(traffic_file):
         line["roofline"]["traffic"] = round(json.load(open(traffic_file))["traffic_bytes_per_launch"])
     g.close()
+    if not args.no_edit_stream and args.workload == "cave512":
+        line["edit_stream"] = edit_stream(vgi, args)
     if not args.no_convergence and args.workload == "cave512":
         line["convergence"] = convergence(vgi, coords, pages, optics, faces, bounces, spp, args)
     if not args.no_cpu_baseline:

@@ -337,6 +337,10 @@ class GpuTracer:
         _check(self._lib.vgi_bind_exposed(self._h))
         self._bound = int(self._lib.vgi_bound_count(self._h))
 
+    def bound_count(self):
+        self._bound = int(self._lib.vgi_bound_count(self._h))
+        return self._bound
+
     def bound_tasks(self):
         self._bound = int(self._lib.vgi_bound_count(self._h))
         out = np.zeros(self._bound, FACE_TASK)
