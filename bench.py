@@ -243,6 +243,77 @@ def edit_stream(vgi, args):
     }
 
 
+def batched_scenes(vgi, args):
+    """32 seeded noise scenes of log-uniform edges in [32, 512] (multiples of
+    8), 2 rays per face, 2 bounces, traced in one persistent launch per frame
+    inside a HIP graph with the autotuned workgroup / lanes / tile; compared
+    with one launch per scene."""
+    t0 = time.time()
+    edges = vgi.batch_edges(32, 0x5EED0003)
+    tracers, faces = [], []
+    for k, edge in enumerate(edges):
+        seed = (0x5EED0003 + 0x10000 * k) & 0xFFFFFFFF
+        grid = vgi.padded(vgi.generate(3, seed, edge))
+        diffuse, emission, lights, env = vgi.optics(3, seed)
+        coords, pages = vgi.pages_of(grid)
+        g = vgi.GpuTracer(0)
+        g.upload_pages(coords, pages)
+        g.set_materials(diffuse, emission)
+        g.set_lights(lights)
+        g.set_transport(bounces=2, samples=2, environment=tuple(env))
+        f = vgi.exposed_faces(grid)
+        g.bind_faces(f)
+        tracers.append(g)
+        faces.append(len(f))
+        del grid, coords, pages
+    setup_s = time.time() - t0
+    print("batched scenes: %d scenes, %d faces, set up in %.1f s" % (len(tracers), sum(faces), setup_s), file=sys.stderr, flush=True)
+    # One launch per scene, back to back (the unbatched way).
+    single_ms = []
+    for g in tracers:
+        g.run_frame()
+        g.sync()
+        g.run_frame()
+        g.sync()
+        single_ms.append(g.last_frame_ms())
+    batch = vgi.GpuBatch(0)
+    for g in tracers:
+        batch.add(g)
+    t1 = time.time()
+    tuned = batch.autotune(frames=1)
+    tune_s = time.time() - t1
+    batch.set_sequence(0)
+    batch.run_frame()
+    batch.sync()
+    ms, rays = [], 0
+    for _ in range(args.batch_frames):
+        batch.run_frame()
+        batch.sync()
+        ms.append(batch.last_ms())
+        rays += batch.latency()[1]
+    latency, _ = batch.latency()
+    batch.close()
+    for g in tracers:
+        g.close()
+    total = float(np.sum(ms))
+    order = np.argsort(faces)
+    return {
+        "workload": "32 noise scenes, edges 8*round(exp(U[ln32,ln512])/8) (%d..%d), 2 rays/face, 2 bounces; "
+                    "one persistent launch per frame in a HIP graph" % (min(edges), max(edges)),
+        "scenes": len(tracers),
+        "faces": int(sum(faces)),
+        "faces_min_max": [int(min(faces)), int(max(faces))],
+        "frames": args.batch_frames,
+        "autotuned": tuned,
+        "autotune_s": round(tune_s, 1),
+        "gpu_ms_per_frame": round(total / len(ms), 3),
+        "faces_per_s": round(sum(faces) * len(ms) / (total / 1000.0), 1),
+        "mrays_per_s": round(rays / (total / 1000.0) / 1e6, 1),
+        "one_launch_per_scene_gpu_ms": round(float(np.sum(single_ms)), 3),
+        "latency_ms_by_faces": [[int(faces[i]), round(float(latency[i]), 3)] for i in order],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +331,8 @@ def main():
     ap.add_argument("--pretrain-frames", type=int, default=400)
     ap.add_argument("--no-edit-stream", action="store_true", help="skip the edit-stream leg")
     ap.add_argument("--edit-frames", type=int, default=120)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-frames", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -372,6 +445,8 @@ def main():
     g.close()
     if not args.no_edit_stream and args.workload == "cave512":
         line["edit_stream"] = edit_stream(vgi, args)
+    if not args.no_batch and args.workload == "cave512":
+        line["batched_scenes"] = batched_scenes(vgi, args)
     if not args.no_convergence and args.workload == "cave512":
         line["convergence"] = convergence(vgi, coords, pages, optics, faces, bounces, spp, args)
     if not args.no_cpu_baseline:

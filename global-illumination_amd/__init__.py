@@ -12,6 +12,7 @@ Everything runs in the HIP library; there is no CPU fallback. Loading fails
 loudly when libvgi.so is missing.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -90,6 +91,16 @@ def library():
         "vgi_last_trace_ms": (f32, [P]),
         "vgi_last_frame_ms": (f32, [P]),
         "vgi_bind_exposed": (i32, [P]),
+        "vgi_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "vgi_batch_destroy": (None, [P]),
+        "vgi_batch_add": (i32, [P, P]),
+        "vgi_batch_configure": (i32, [P, u32, u32, u32, u32]),
+        "vgi_batch_set_sequence": (i32, [P, u32]),
+        "vgi_batch_run_frame": (i32, [P]),
+        "vgi_batch_sync": (i32, [P]),
+        "vgi_batch_last_ms": (f32, [P]),
+        "vgi_batch_latency": (i32, [P, P, u32, P]),
+        "vgi_batch_autotune": (i32, [P, u32, P]),
         "vgi_scene_edits": (i64, [u32, u32, u32, u32, P, P, i64]),
         "vgi_bound_count": (u32, [P]),
         "vgi_read_bound_tasks": (i32, [P, P, u32]),
@@ -134,7 +145,9 @@ def exported_symbols():
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
-        "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits",
+        "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
+        "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
+        "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune",
     ]
 
 
@@ -201,6 +214,33 @@ def pages_of(grid, origin=(0, 0, 0)):
     pz, py, px = np.meshgrid(np.arange(z // 32), np.arange(y // 32), np.arange(x // 32), indexing="ij")
     coords = np.stack([px.ravel(), py.ravel(), pz.ravel()], axis=1).astype(np.int32) + np.asarray(origin, np.int32) // 32
     return coords, np.ascontiguousarray(g.reshape(-1, 32768))
+
+
+def _pcg(v):
+    v = (v * 747796405 + 2891336453) & 0xFFFFFFFF
+    w = (((v >> ((v >> 28) + 4)) ^ v) * 277803737) & 0xFFFFFFFF
+    return (w >> 22) ^ w
+
+
+def batch_edges(count=32, seed=0x5EED0003):
+    """Edges of the batched-scenes workload: 8 * round(exp(U[ln 32, ln 512]) / 8)."""
+    out = []
+    for i in range(count):
+        u = (_pcg(seed ^ _pcg(i + 0x9E3779B9 & 0xFFFFFFFF)) >> 8) / float(1 << 24)
+        edge = math.exp(math.log(32.0) + u * (math.log(512.0) - math.log(32.0)))
+        out.append(int(8 * round(edge / 8.0)))
+    return out
+
+
+def padded(grid, multiple=32):
+    """The grid padded with air to edges that are multiples of `multiple`."""
+    z, y, x = grid.shape
+    shape = tuple(-(-d // multiple) * multiple for d in (z, y, x))
+    if shape == grid.shape:
+        return grid
+    out = np.full(shape, AIR, np.uint16)
+    out[:z, :y, :x] = grid
+    return out
 
 
 # ---------------------------------------------------------------- tracer
@@ -462,3 +502,58 @@ def relative_rmse(estimate, reference, reference_var=None):
     if reference_var is not None:
         err2 = max(err2 - float(np.mean(reference_var)), 0.0)
     return float(np.sqrt(err2) / np.mean(reference))
+
+
+class GpuBatch:
+    """Many bound tracers traced in one persistent launch per frame, captured
+    into a HIP graph (vgi_batch_*)."""
+
+    def __init__(self, device=0):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        _check(self._lib.vgi_batch_create(device, ctypes.byref(h)))
+        self._h = h
+        self._scenes = []
+
+    def close(self):
+        if self._h:
+            self._lib.vgi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, tracer):
+        _check(self._lib.vgi_batch_add(self._h, tracer._h))
+        self._scenes.append(tracer)
+
+    def configure(self, block=256, lanes=1, tile=1024, grid=0):
+        _check(self._lib.vgi_batch_configure(self._h, block, lanes, tile, grid))
+
+    def set_sequence(self, base):
+        _check(self._lib.vgi_batch_set_sequence(self._h, base))
+
+    def run_frame(self):
+        _check(self._lib.vgi_batch_run_frame(self._h))
+
+    def sync(self):
+        _check(self._lib.vgi_batch_sync(self._h))
+
+    def last_ms(self):
+        return float(self._lib.vgi_batch_last_ms(self._h))
+
+    def latency(self):
+        """(per-scene ms from frame start to the scene's last tile, rays)."""
+        out = np.zeros(len(self._scenes), np.float32)
+        rays = ctypes.c_uint64()
+        _check(self._lib.vgi_batch_latency(self._h, _ptr(out), len(out), ctypes.byref(rays)))
+        return out, int(rays.value)
+
+    def autotune(self, frames=3):
+        out = np.zeros(5, np.uint32)
+        _check(self._lib.vgi_batch_autotune(self._h, frames, _ptr(out)))
+        return {"block": int(out[0]), "lanes": int(out[1]), "tile": int(out[2]),
+                "ms": float(out[3:4].view(np.float32)[0]), "candidates": int(out[4])}

@@ -1642,4 +1642,242 @@ int64_t vgi_emitter_words(vgi_tracer *t, uint32_t *out, int64_t capacity) {
     return n;
 }
 
+// ================================================================ batched frames
+// Many tracers (scenes, or chunks of one scene bound to several tracers) in
+// one persistent launch per frame, captured once into a HIP graph and
+// replayed; workgroup size, lanes per face and queue tile come from
+// vgi_batch_configure or the autotuner.
+}  // extern "C"
+
+struct vgi_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {};
+    std::vector<vgi_tracer *> scenes;
+    DeviceBuffer d_scenes, d_tiles, d_control, d_done, d_finish;
+    uint32_t block = 256, lanes = 1, tile = 1024, grid = 0;
+    uint32_t samples = 0, n_tiles = 0;
+    uint32_t next_sequence = 0;
+    bool sequence_dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    float last_ms = 0.0f;
+    int clock_khz = 0, cus = 0;
+    std::vector<uint32_t> order;  // scenes by ascending face count
+};
+
+namespace {
+
+void drop_graph(vgi_batch &b) {
+    if (b.exec) hipGraphExecDestroy(b.exec);
+    if (b.graph) hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+}
+
+// Lane counts that keep every face's sum in the reference's order: one
+// lane, at least one lane per sample, or the reference's own count.
+bool lanes_allowed(uint32_t lanes, uint32_t samples, uint32_t reference) {
+    if (lanes != 1u && lanes != 2u && lanes != 4u && lanes != 8u) return false;
+    return lanes == 1u || lanes >= samples || lanes == reference;
+}
+
+// Descriptors, tile queue and the captured frame graph.
+int batch_prepare(vgi_batch &b) {
+    if (!b.dirty && b.exec) return 0;
+    drop_graph(b);
+    uint32_t n = (uint32_t)b.scenes.size();
+    if (n == 0) return fail("the batch holds no scenes");
+    std::vector<vgi::BatchScene> desc(n);
+    b.order.resize(n);
+    std::iota(b.order.begin(), b.order.end(), 0u);
+    std::stable_sort(b.order.begin(), b.order.end(), [&](uint32_t a, uint32_t c) { return b.scenes[a]->bound < b.scenes[c]->bound; });
+    std::vector<vgi::BatchTile> tiles;
+    for (uint32_t k : b.order) {
+        vgi_tracer &t = *b.scenes[k];
+        if (!t.have_regions) return fail("a batch scene has no scene loaded");
+        if (sync_emitters(t) != 0) return -1;
+        views(t, desc[k].scene, desc[k].optics);
+        desc[k].tasks = t.b_tasks.as<uint4>();
+        desc[k].results = t.b_results.as<float4>();
+        desc[k].count = t.bound;
+        uint32_t first_tile = (uint32_t)tiles.size();
+        for (uint32_t f = 0; f < t.bound; f += b.tile) tiles.push_back(vgi::BatchTile{k, f, std::min(b.tile, t.bound - f), 0u});
+        desc[k].tiles = (uint32_t)tiles.size() - first_tile;
+    }
+    b.n_tiles = (uint32_t)tiles.size();
+    HIP_OK(b.d_scenes.reserve(desc.size() * sizeof(vgi::BatchScene)));
+    HIP_OK(b.d_tiles.reserve(std::max<size_t>(tiles.size(), 1) * sizeof(vgi::BatchTile)));
+    HIP_OK(b.d_done.reserve((size_t)n * 4));
+    HIP_OK(b.d_finish.reserve((size_t)n * 8));
+    HIP_OK(b.d_control.reserve(sizeof(vgi::BatchControl)));
+    HIP_OK(hipMemcpyAsync(b.d_scenes.ptr, desc.data(), desc.size() * sizeof(vgi::BatchScene), hipMemcpyHostToDevice, b.stream));
+    if (!tiles.empty())
+        HIP_OK(hipMemcpyAsync(b.d_tiles.ptr, tiles.data(), tiles.size() * sizeof(vgi::BatchTile), hipMemcpyHostToDevice, b.stream));
+    HIP_OK(hipStreamSynchronize(b.stream));
+    uint32_t grid = b.grid;
+    if (grid == 0) grid = (uint32_t)std::max(1, b.cus) * (4u * VGI_TRACE_WAVES) / (b.block / 64u);
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+    vgi::launch_batch_frame(b.d_scenes.as<vgi::BatchScene>(), n, b.d_tiles.as<vgi::BatchTile>(), b.n_tiles, b.lanes, b.block, grid,
+                            b.d_control.as<vgi::BatchControl>(), b.d_done.as<uint32_t>(), b.d_finish.as<unsigned long long>(), b.stream);
+    hipError_t launch = hipGetLastError();
+    hipError_t end = hipStreamEndCapture(b.stream, &graph);
+    if (launch != hipSuccess) { if (graph) hipGraphDestroy(graph); return fail(std::string("batch capture: ") + hipGetErrorString(launch)); }
+    HIP_OK(end);
+    b.graph = graph;
+    HIP_OK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    b.dirty = false;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vgi_batch_create(int device_ordinal, vgi_batch **out) {
+    if (!out) return fail("null output");
+    int count = 0;
+    HIP_OK(hipGetDeviceCount(&count));
+    if (device_ordinal < 0 || device_ordinal >= count) return fail("no such HIP device");
+    HIP_OK(hipSetDevice(device_ordinal));
+    std::unique_ptr<vgi_batch> b(new vgi_batch());
+    b->device = device_ordinal;
+    HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto &e : b->ev) HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipDeviceGetAttribute(&b->clock_khz, hipDeviceAttributeWallClockRate, device_ordinal));
+    HIP_OK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device_ordinal));
+    *out = b.release();
+    return 0;
+}
+
+void vgi_batch_destroy(vgi_batch *b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    drop_graph(*b);
+    b->d_scenes.release(); b->d_tiles.release(); b->d_control.release(); b->d_done.release(); b->d_finish.release();
+    for (auto &e : b->ev) if (e) hipEventDestroy(e);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int vgi_batch_add(vgi_batch *b, vgi_tracer *t) {
+    if (!b || !t) return fail("null argument");
+    if (t->device != b->device) return fail("a batch scene lives on another device");
+    if (t->bound == 0) return fail("bind the scene's faces before adding it to a batch");
+    uint32_t samples = t->transport.samples;
+    if (!b->scenes.empty() && samples != b->samples) return fail("every scene of a batch traces the same samples per face");
+    if (!lanes_allowed(b->lanes, samples, lanes_for(t->transport))) b->lanes = 1u;
+    b->samples = samples;
+    b->scenes.push_back(t);
+    b->dirty = true;
+    return 0;
+}
+
+int vgi_batch_configure(vgi_batch *b, uint32_t block, uint32_t lanes, uint32_t tile, uint32_t grid) {
+    if (!b) return fail("null batch");
+    if (block != 64u && block != 128u && block != 256u) return fail("workgroup size must be 64, 128 or 256");
+    if (tile == 0 || tile > (1u << 24)) return fail("queue tile must hold 1 to 2^24 faces");
+    for (vgi_tracer *t : b->scenes)
+        if (!lanes_allowed(lanes, t->transport.samples, lanes_for(t->transport)))
+            return fail("lanes per face must be 1, at least the samples per face, or the reference's lane count");
+    if (block % lanes) return fail("lanes per face must divide the workgroup");
+    b->block = block; b->lanes = lanes; b->tile = tile; b->grid = grid;
+    b->dirty = true;
+    return 0;
+}
+
+// The next frame's sequence base (then advancing by the samples per face
+// each frame, on the device).
+int vgi_batch_set_sequence(vgi_batch *b, uint32_t base) {
+    if (!b) return fail("null batch");
+    b->next_sequence = base;
+    b->sequence_dirty = true;
+    return 0;
+}
+
+int vgi_batch_run_frame(vgi_batch *b) {
+    if (!b) return fail("null batch");
+    HIP_OK(hipSetDevice(b->device));
+    if (batch_prepare(*b) != 0) return -1;
+    if (b->sequence_dirty) {
+        vgi::BatchControl c{};
+        c.next_seq = b->next_sequence;
+        c.samples = b->samples;
+        HIP_OK(hipMemcpyAsync(b->d_control.ptr, &c, sizeof(c), hipMemcpyHostToDevice, b->stream));
+        b->sequence_dirty = false;
+    }
+    HIP_OK(hipEventRecord(b->ev[0], b->stream));
+    HIP_OK(hipGraphLaunch(b->exec, b->stream));
+    HIP_OK(hipEventRecord(b->ev[1], b->stream));
+    return 0;
+}
+
+int vgi_batch_sync(vgi_batch *b) {
+    if (!b) return fail("null batch");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipEventElapsedTime(&b->last_ms, b->ev[0], b->ev[1]));
+    return 0;
+}
+
+float vgi_batch_last_ms(vgi_batch *b) { return b ? b->last_ms : 0.0f; }
+
+// Per scene (in the order added): ms from the frame's start to its last
+// tile's end; rays traced in the frame.
+int vgi_batch_latency(vgi_batch *b, float *out_ms, uint32_t count, uint64_t *rays) {
+    if (!b) return fail("null batch");
+    if (count > b->scenes.size()) return fail("more scenes than the batch holds");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    vgi::BatchControl c{};
+    HIP_OK(hipMemcpy(&c, b->d_control.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    if (rays) *rays = c.rays;
+    if (count && out_ms) {
+        std::vector<unsigned long long> finish(count);
+        HIP_OK(hipMemcpy(finish.data(), b->d_finish.ptr, (size_t)count * 8, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < count; ++k)
+            out_ms[k] = finish[k] >= c.start ? (float)((double)(finish[k] - c.start) / (double)std::max(b->clock_khz, 1)) : -1.0f;
+    }
+    return 0;
+}
+
+// Try every workgroup size, allowed lane count and queue tile on `frames`
+// timed frames each (after one warm-up) and keep the fastest by median GPU
+// time. out: block, lanes, tile, ms per frame (as float bits), candidates.
+int vgi_batch_autotune(vgi_batch *b, uint32_t frames, uint32_t out[5]) {
+    if (!b || !out) return fail("null argument");
+    if (b->scenes.empty()) return fail("the batch holds no scenes");
+    frames = std::max<uint32_t>(frames, 1);
+    const uint32_t blocks[] = {64u, 128u, 256u};
+    const uint32_t lane_options[] = {1u, 2u, 4u, 8u};
+    const uint32_t tile_options[] = {128u, 512u, 2048u, 8192u};
+    float best = 1e30f;
+    uint32_t best_cfg[3] = {b->block, b->lanes, b->tile}, tried = 0;
+    uint32_t saved_sequence = b->next_sequence;
+    for (uint32_t block : blocks)
+        for (uint32_t lanes : lane_options)
+            for (uint32_t tile : tile_options) {
+                bool ok = block % lanes == 0;
+                for (vgi_tracer *t : b->scenes) ok = ok && lanes_allowed(lanes, t->transport.samples, lanes_for(t->transport));
+                if (!ok) continue;
+                if (vgi_batch_configure(b, block, lanes, tile, 0) != 0) return -1;
+                std::vector<float> ms;
+                for (uint32_t f = 0; f <= frames; ++f) {
+                    if (vgi_batch_run_frame(b) != 0 || vgi_batch_sync(b) != 0) return -1;
+                    if (f > 0) ms.push_back(b->last_ms);
+                }
+                std::sort(ms.begin(), ms.end());
+                float med = ms[ms.size() / 2];
+                ++tried;
+                if (med < best) { best = med; best_cfg[0] = block; best_cfg[1] = lanes; best_cfg[2] = tile; }
+            }
+    if (vgi_batch_configure(b, best_cfg[0], best_cfg[1], best_cfg[2], 0) != 0) return -1;
+    vgi_batch_set_sequence(b, saved_sequence);
+    out[0] = best_cfg[0]; out[1] = best_cfg[1]; out[2] = best_cfg[2];
+    std::memcpy(&out[3], &best, 4);
+    out[4] = tried;
+    return 0;
+}
+
 }  // extern "C"

@@ -3,11 +3,29 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Waves per SIMD the trace kernels are compiled for (their register budget).
+#ifndef VGI_TRACE_WAVES
+#define VGI_TRACE_WAVES 8
+#endif
+
 namespace vgi {
 
 struct SceneView;
 struct Optics;
 struct MlpQuery;
+struct BatchScene;
+
+// A queue tile of a batched frame: `faces` bound faces of one scene.
+struct BatchTile {
+    uint32_t scene, first, faces, pad;
+};
+// Device-side frame state of a batched launch (written only by kernels, so a
+// captured graph replays it): the tile queue head, this frame's sequence base
+// and the next one, the start timestamp.
+struct BatchControl {
+    uint32_t head, frame_seq, next_seq, samples;
+    unsigned long long start, rays;
+};
 
 // Per listed region: the classification of world.rs:243-319 plus the packing
 // offsets the scan assigns (words into the cells buffer).
@@ -69,6 +87,13 @@ void launch_measure_edits(const uint4 *measures, uint32_t entries, uint32_t posi
 void launch_scatter_bound(const uint32_t *index, const uint4 *tasks, const uint32_t *slots, uint32_t n, uint4 *bound_tasks,
                           uint32_t *bound_slots, hipStream_t stream);
 void launch_scatter_words(const uint2 *writes, uint32_t n, uint32_t *words, hipStream_t stream);
+
+// One batched frame: reset the queue (and advance the sequence), then a
+// persistent launch of `grid` workgroups of `block` threads that take tiles
+// until the queue is empty; per scene, the wall clock when its last tile ended.
+void launch_batch_frame(const BatchScene *scenes, uint32_t n_scenes, const BatchTile *tiles, uint32_t n_tiles, uint32_t lanes,
+                        uint32_t block, uint32_t grid, BatchControl *control, uint32_t *done, unsigned long long *finish,
+                        hipStream_t stream);
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 

@@ -21,29 +21,23 @@ namespace vgi {
 // Occupancy: the path estimator is one long dependent gather chain, so the
 // kernel is latency bound; it is compiled for VGI_TRACE_WAVES waves per SIMD
 // (the register budget that allows), trading a little spill for latency hiding.
-#ifndef VGI_TRACE_WAVES
-#define VGI_TRACE_WAVES 8
-#endif
+// One face's samples on `lanes` consecutive threads of a wavefront (every
+// thread of the wavefront calls this; out-of-range faces pass valid = false).
 // kLearned: paths stop at their first hit into the learned cache; each
 // sample's partial energy, prediction factor and cache query go to
 // `learned` (2 float4 per sample) and `queries`, and the combine pass forms the
 // face moments once the network has answered.
 template <class Tally, bool kHits, bool kLearned>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void trace_faces_kernel(
-    SceneView scene, Optics optics, const uint4 *__restrict__ tasks, uint32_t count, uint32_t samples, uint32_t lanes,
-    uint32_t sequence_base, float4 *__restrict__ results, uint32_t *__restrict__ hits, unsigned long long *__restrict__ counters,
-    float4 *__restrict__ learned, MlpQuery *__restrict__ queries) {
-    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t face_index = tid / lanes;
-    uint32_t lane = tid % lanes;
-    bool valid = face_index < count;
+__device__ __forceinline__ void trace_face(SceneView scene, Optics optics, const uint4 *__restrict__ tasks,
+                                           uint32_t face_index, bool valid, uint32_t lane, uint32_t samples, uint32_t lanes,
+                                           uint32_t sequence_base, float4 *__restrict__ results, uint32_t *__restrict__ hits,
+                                           float4 *__restrict__ learned, MlpQuery *__restrict__ queries, Tally &tally) {
     uint4 t0 = make_uint4(0u, 0u, 0u, NONE), t1 = make_uint4(0u, 0u, 0u, 0u);
     if (valid) {
         t0 = tasks[face_index * 2u];
         t1 = tasks[face_index * 2u + 1u];
         valid = t0.w != NONE;
     }
-    Tally tally;
     f3 mean = mk(0.0f, 0.0f, 0.0f), second = mk(0.0f, 0.0f, 0.0f);
     uint32_t known = 0u;
     int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
@@ -107,6 +101,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
         r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
         r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
     }
+}
+
+template <class Tally, bool kHits, bool kLearned>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void trace_faces_kernel(
+    SceneView scene, Optics optics, const uint4 *__restrict__ tasks, uint32_t count, uint32_t samples, uint32_t lanes,
+    uint32_t sequence_base, float4 *__restrict__ results, uint32_t *__restrict__ hits, unsigned long long *__restrict__ counters,
+    float4 *__restrict__ learned, MlpQuery *__restrict__ queries) {
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t face_index = tid / lanes;
+    uint32_t lane = tid % lanes;
+    Tally tally;
+    trace_face<Tally, kHits, kLearned>(scene, optics, tasks, face_index, face_index < count, lane, samples, lanes, sequence_base,
+                                       results, hits, learned, queries, tally);
     if (counters == nullptr) return;
     if constexpr (std::is_same<Tally, FullTally>::value) {
         unsigned long long c[5] = {tally.rays, tally.steps, tally.headers, tally.bricks, tally.cells};
@@ -120,6 +127,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
         for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
         if ((threadIdx.x & 63u) == 0u) atomicAdd(&counters[0], r);
     }
+}
+
+// ============================================================== batched frames
+// Many scenes (or uneven chunks) in one persistent launch: workgroups take
+// queue tiles (scene, face range) from a device counter until the queue is
+// empty, and trace each tile's faces exactly as trace_faces_kernel does, so
+// every scene's records equal its own launch's bit for bit. The last tile of
+// a scene stamps the scene's finish time (per-scene latency).
+__global__ void batch_reset_kernel(BatchControl *__restrict__ control, uint32_t *__restrict__ done,
+                                   unsigned long long *__restrict__ finish, uint32_t scenes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) {
+        control->head = 0u;
+        control->frame_seq = control->next_seq;
+        control->next_seq = control->next_seq + control->samples;
+        control->rays = 0ull;
+        control->start = wall_clock64();
+    }
+    for (uint32_t k = i; k < scenes; k += gridDim.x * blockDim.x) {
+        done[k] = 0u;
+        finish[k] = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void batch_trace_kernel(
+    const BatchScene *__restrict__ scenes, const BatchTile *__restrict__ tiles, uint32_t n_tiles, uint32_t lanes,
+    BatchControl *__restrict__ control, uint32_t *__restrict__ done, unsigned long long *__restrict__ finish) {
+    __shared__ uint32_t s_tile;
+    const uint32_t samples = control->samples;
+    const uint32_t sequence = control->frame_seq;
+    const uint32_t per_pass = blockDim.x / lanes;
+    const uint32_t local = threadIdx.x / lanes, lane = threadIdx.x % lanes;
+    RayTally tally;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0u) s_tile = atomicAdd(&control->head, 1u);
+        __syncthreads();
+        uint32_t ti = __builtin_amdgcn_readfirstlane(s_tile);
+        if (ti >= n_tiles) break;
+        BatchTile tile = tiles[ti];
+        const BatchScene &sc = scenes[tile.scene];
+        for (uint32_t f = 0u; f < tile.faces; f += per_pass) {
+            uint32_t at = f + local;
+            trace_face<RayTally, false, false>(sc.scene, sc.optics, sc.tasks, tile.first + at, at < tile.faces, lane, samples, lanes,
+                                               sequence, sc.results, nullptr, nullptr, nullptr, tally);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0u) {
+            __threadfence();
+            uint32_t d = atomicAdd(&done[tile.scene], 1u);
+            if (d + 1u == sc.tiles) finish[tile.scene] = wall_clock64();
+        }
+    }
+    unsigned long long r = tally.rays;
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    if ((threadIdx.x & 63u) == 0u) atomicAdd(&control->rays, r);
 }
 
 // ============================================================== NAADF build
@@ -886,6 +949,13 @@ void launch_scatter_bound(const uint32_t *index, const uint4 *tasks, const uint3
 void launch_scatter_words(const uint2 *writes, uint32_t n, uint32_t *words, hipStream_t stream) {
     if (n == 0) return;
     hipLaunchKernelGGL(scatter_words_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, writes, n, words);
+}
+
+void launch_batch_frame(const BatchScene *scenes, uint32_t n_scenes, const BatchTile *tiles, uint32_t n_tiles, uint32_t lanes,
+                        uint32_t block, uint32_t grid, BatchControl *control, uint32_t *done, unsigned long long *finish,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, stream, control, done, finish, n_scenes);
+    hipLaunchKernelGGL(batch_trace_kernel, dim3(grid), dim3(block), 0, stream, scenes, tiles, n_tiles, lanes, control, done, finish);
 }
 
 void launch_fp64_accumulate(const float4 *results, uint32_t count, double *sums, hipStream_t stream) {

@@ -46,6 +46,17 @@ struct Optics {
     uint32_t cache_on;
 };
 
+// One scene of a batched launch: its scene and optics views and its bound
+// faces; `tiles` queue tiles of this scene end the scene when all are done.
+struct BatchScene {
+    SceneView scene;
+    Optics optics;
+    const uint4 *__restrict__ tasks;
+    float4 *__restrict__ results;
+    uint32_t count;
+    uint32_t tiles;
+};
+
 struct RayResult {
     int vx, vy, vz;
     uint32_t material;

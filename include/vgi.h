@@ -27,6 +27,7 @@ extern "C" {
 #define VGI_NO_FACE 0xffffffffu /* voxel_trace.wgsl:64 unused pick capacity */
 
 typedef struct vgi_tracer vgi_tracer;
+typedef struct vgi_batch vgi_batch;
 
 /* gpu.rs:136-141 GpuLight: position_range.w < 0 marks a directional source
  * whose xyz is the unit direction toward the light (sources.rs:130). */
@@ -201,6 +202,37 @@ float vgi_mlp_last_loss(vgi_tracer *tracer);
 /* ---- retention events (resident.rs:1326-1417 encode_events layout;
  * voxel_retention.wgsl:29-96, apply_events voxel_accumulate.wgsl:295-332) ---- */
 int vgi_apply_events(vgi_tracer *tracer, const uint32_t *events, uint32_t event_count, uint32_t words);
+
+/* ---- batched frames (no counterpart in the reference, which runs one world;
+ * docs/architecture.md:294): many tracers -- separate scenes, or chunks of
+ * uneven face counts bound to several tracers -- traced in ONE persistent
+ * launch per frame that takes (scene, face range) tiles from a device queue,
+ * captured once into a HIP graph and replayed. Each scene's bound faces are
+ * traced exactly as vgi_run_frame would trace them (non-resident: records go
+ * to the tracer's bound records, read with vgi_read_bound), with sequence
+ * base set by vgi_batch_set_sequence and advancing by the samples per face
+ * every frame. All scenes trace the same samples per face. ---- */
+int vgi_batch_create(int device_ordinal, vgi_batch **out);
+void vgi_batch_destroy(vgi_batch *batch);
+/* Add a tracer whose faces are bound (vgi_bind_faces / vgi_bind_exposed). */
+int vgi_batch_add(vgi_batch *batch, vgi_tracer *tracer);
+/* Workgroup size (64/128/256), lanes per face (1, 2, 4 or 8: one lane, at
+ * least one lane per sample, or the reference's count, so sums keep the
+ * reference's order), queue tile in faces, workgroups (0: enough to fill
+ * every CU at the kernel's occupancy). */
+int vgi_batch_configure(vgi_batch *batch, uint32_t block, uint32_t lanes, uint32_t tile, uint32_t grid);
+int vgi_batch_set_sequence(vgi_batch *batch, uint32_t base);
+int vgi_batch_run_frame(vgi_batch *batch);
+int vgi_batch_sync(vgi_batch *batch);
+/* GPU time of the last frame (ms), valid after vgi_batch_sync. */
+float vgi_batch_last_ms(vgi_batch *batch);
+/* Per scene in the order added: ms from the frame's start until its last
+ * tile ended; rays traced in the frame (optional). */
+int vgi_batch_latency(vgi_batch *batch, float *out_ms, uint32_t count, uint64_t *rays);
+/* Time every allowed (workgroup, lanes, tile) on `frames` frames after a
+ * warm-up and keep the fastest: out = block, lanes, tile, median ms (float
+ * bits), candidates tried. */
+int vgi_batch_autotune(vgi_batch *batch, uint32_t frames, uint32_t out[5]);
 
 /* ---- measurement plumbing: a plain copy kernel of `bytes` run `repeats`
  * times (average ms per launch in *ms), the known-size reference that scales
