@@ -331,6 +331,7 @@ def main():
     ap.add_argument("--pretrain-frames", type=int, default=400)
     ap.add_argument("--no-edit-stream", action="store_true", help="skip the edit-stream leg")
     ap.add_argument("--edit-frames", type=int, default=120)
+    ap.add_argument("--learned-only", action="store_true", help="time learned frames only (profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-frames", type=int, default=20)
     args = ap.parse_args()
@@ -372,6 +373,29 @@ def main():
     stats = g.stats()
     print("uploaded: %d regions, %d cell words, GPU NAADF build+pack %.2f ms, %d emitter faces, upload %.1f s"
           % (stats[0], stats[2], build_ms, stats[5], upload_s), file=sys.stderr, flush=True)
+
+    if args.learned_only:
+        # Learned frames alone (for profiling): the cache warmed up on
+        # `pretrain_frames` training-only frames, then `steps` timed frames.
+        g.mlp_enable(seed=1, train_per_frame=65536, steps=8, lr=1e-3)
+        g.mlp_pretrain(args.pretrain_frames)
+        for _ in range(args.warmup):
+            g.run_frame()
+            g.sync()
+        frame_ms, trace_ms, rays = [], [], 0
+        for _ in range(args.steps):
+            g.run_frame(vgi.COUNT_RAYS)
+            g.sync()
+            frame_ms.append(g.last_frame_ms())
+            trace_ms.append(g.last_trace_ms())
+            rays += int(g.read_counters_bound()[0])
+        g.close()
+        print(json.dumps({"mode": "learned", "workload": desc, "steps": args.steps, "pretrain_frames": args.pretrain_frames,
+                          "frame_gpu_ms": round(float(np.median(frame_ms)), 3),
+                          "trace_infer_combine_gpu_ms": round(float(np.median(trace_ms)), 3),
+                          "rays_per_frame": rays // args.steps,
+                          "queries_per_frame": int(len(faces) * spp)}))
+        return
 
     # The library owns its HIP stream; vgi_sync drains it on both sides of the
     # timed region (one process, one GPU: no barrier to take).
