@@ -211,14 +211,16 @@ def edit_stream(vgi, args):
     for _ in range(2):
         g.run_frame()
     g.sync()
-    edit_ms, build_ms, frame_ms, trace_ms, rays, voxels = [], [], [], [], 0, 0
+    edit_ms, build_ms, frame_ms, trace_ms, rays, voxels, phases = [], [], [], [], 0, 0, []
     t_start = time.perf_counter()
     for f in range(args.edit_frames):
         xyz, values = edits[f]
         t1 = time.perf_counter()
         g.edit_voxels(xyz, values)
         edit_ms.append((time.perf_counter() - t1) * 1000.0)
-        build_ms.append(float(g.timings()[3]))
+        tm = g.timings()
+        build_ms.append(float(tm[3]))
+        phases.append(tm[4:8])
         g.run_frame(vgi.COUNT_RAYS)
         g.sync()
         frame_ms.append(g.last_frame_ms())
@@ -237,6 +239,8 @@ def edit_stream(vgi, args):
         "ms_per_frame_wall": round(elapsed * 1000.0 / args.edit_frames, 3),
         "edit_ms_host_median": round(float(np.median(edit_ms)), 3),
         "naadf_pack_gpu_ms_median": round(float(np.median(build_ms)), 3),
+        "edit_host_ms_median_by_phase": dict(zip(["regions_and_repack", "top_level", "retention_event", "exposure"],
+                                                 [round(float(v), 3) for v in np.median(np.array(phases), axis=0)])),
         "frame_gpu_ms_median": round(float(np.median(frame_ms)), 3),
         "trace_gpu_ms_median": round(float(np.median(trace_ms)), 3),
         "mrays_per_s_trace": round(rays / (float(np.sum(trace_ms)) / 1000.0) / 1e6, 1),

@@ -345,8 +345,10 @@ class GpuTracer:
         return out
 
     def timings(self):
-        out = np.zeros(4, np.float32)
-        _check(self._lib.vgi_timings(self._h, _ptr(out), 4))
+        """[prepare, trace, accumulate, build] GPU ms, then the last edit's host ms
+        [regions + re-pack, top level, retention event, exposure update]."""
+        out = np.zeros(8, np.float32)
+        _check(self._lib.vgi_timings(self._h, _ptr(out), 8))
         return out
 
     def slots(self, count):

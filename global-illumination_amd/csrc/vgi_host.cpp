@@ -89,7 +89,7 @@ struct vgi_tracer {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
-    float timings[4] = {0, 0, 0, 0};
+    float timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // GPU ms: prepare, trace, accumulate, build; host ms of the last edit: regions + GPU re-pack, top level, event, exposure
     // ---- scene (host authority for page contents, as the reference's RegionMap)
     std::unordered_map<uint64_t, Int3> pages;
     Int3 vmin{0, 0, 0};           // voxel origin of the dense box
@@ -144,6 +144,8 @@ struct vgi_tracer {
     float mlp_lr = 1e-3f, last_loss = 0.0f;
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
+    hipStream_t side = nullptr;           // training paths beside a learned frame's trace
+    hipEvent_t side_ev[2] = {};           // [tasks prepared, targets ready]
     // ---- edit retention: events of edits, applied with the next batch or frame
     std::vector<vgi_host::RetentionEvent> pending_events;
     DeviceBuffer d_events, d_measures, d_measure_max, d_material_light;
@@ -424,10 +426,14 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
 // flipped. With the resident cache, the edit's retention event is queued
 // for the next batch (pages.rs:446-490), and bound faces that follow the
 // exposed surface are added and removed.
-int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed);
+int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, const std::vector<vgi_host::RegionDelta> &deltas);
 
 int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     if (!t.have_regions) return 0;
+    using clock = std::chrono::steady_clock;
+    auto ms_since = [](clock::time_point a) { return std::chrono::duration<float, std::milli>(clock::now() - a).count(); };
+    auto t_start = clock::now();
+    t.timings[4] = t.timings[5] = t.timings[6] = t.timings[7] = 0.0f;
     std::vector<uint32_t> accepted;
     std::vector<int> dirty;
     accepted.reserve(count);
@@ -532,6 +538,8 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     if (gpu_pack(t, region_list(t, repack), full) != 0) return -1;
     if (full) t.cells_live = t.cells_used;
     hipEventRecord(t.ev[7], t.stream);
+    t.timings[4] = ms_since(t_start);
+    auto t_top = clock::now();
     if (!flips.empty()) {
         std::sort(flips.begin(), flips.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
         std::vector<Int3> keys;
@@ -541,7 +549,9 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     if (upload_top(t) != 0) return -1;
     hipEventSynchronize(t.ev[7]);
     hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    t.timings[5] = ms_since(t_top);
     if (!events) return 0;
+    auto t_event = clock::now();
     // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
     std::vector<vgi_host::Int3> changed;
     for (size_t k = 0; k < dirty.size(); ++k) {
@@ -572,7 +582,12 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     prev.page_solid = page_solid.data();
     vgi_host::RetentionEvent ev;
     if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev)) t.pending_events.push_back(std::move(ev));
-    if (t.follow_exposure && !changed.empty()) return update_exposure(t, changed);
+    t.timings[6] = ms_since(t_event);
+    if (t.follow_exposure && !changed.empty()) {
+        auto t_exposure = clock::now();
+        if (update_exposure(t, changed, deltas) != 0) return -1;
+        t.timings[7] = ms_since(t_exposure);
+    }
     return 0;
 }
 
@@ -797,7 +812,11 @@ int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets,
 
 // A learned frame's training: trace a window of the face permutation with
 // plain multi-bounce paths (one sample each) for targets, then Adam steps.
-int learned_training(vgi_tracer &t) {
+// The training set of a frame, on `stream`: a window of the face permutation
+// traced with plain multi-bounce paths (one sample each) into targets. It
+// needs only the scene and the bound tasks, not the network, so a learned
+// frame traces it beside its own paths.
+int learned_targets(vgi_tracer &t, hipStream_t stream) {
     uint32_t total = (t.train_per_frame / 16u) * 16u;
     if (total == 0 || t.bound == 0) return 0;
     HIP_OK(t.d_train_tasks.reserve((size_t)total * 32));
@@ -805,7 +824,7 @@ int learned_training(vgi_tracer &t) {
     HIP_OK(t.d_train_results.reserve((size_t)total * 96));
     HIP_OK(t.d_targets.reserve((size_t)total * 16));
     vgi::launch_train_gather(t.b_tasks.as<uint4>(), t.d_perm.as<uint32_t>(), t.d_face_material.as<uint32_t>(), t.bound,
-                             t.train_cursor, total, t.d_train_tasks.as<uint4>(), t.d_train_q.as<vgi::MlpQuery>(), t.stream);
+                             t.train_cursor, total, t.d_train_tasks.as<uint4>(), t.d_train_q.as<vgi::MlpQuery>(), stream);
     vgi::SceneView s;
     vgi::Optics o;
     views(t, s, o);
@@ -815,13 +834,26 @@ int learned_training(vgi_tracer &t) {
     // traced one bounce short.
     o.bounces = o.bounces > 1u ? o.bounces - 1u : 1u;
     vgi::launch_trace(s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index, t.d_train_results.as<float4>(),
-                      nullptr, nullptr, false, t.stream);
-    vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), t.stream);
+                      nullptr, nullptr, false, stream);
+    vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), stream);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// The frame's Adam steps on its training set (on the tracer's stream).
+int learned_steps(vgi_tracer &t) {
+    uint32_t total = (t.train_per_frame / 16u) * 16u;
+    if (total == 0 || t.bound == 0) return 0;
     uint32_t per = std::max<uint32_t>(16u, (total / std::max<uint32_t>(t.train_steps, 1u) / 16u) * 16u);
     for (uint32_t off = 0; off + per <= total; off += per)
         if (mlp_step(t, t.d_train_q.as<vgi::MlpQuery>() + off, t.d_targets.as<float4>() + off, per, true) != 0) return -1;
     t.train_cursor = (uint32_t)(((uint64_t)t.train_cursor + total) % t.bound);
     return 0;
+}
+
+int learned_training(vgi_tracer &t) {
+    if (learned_targets(t, t.stream) != 0) return -1;
+    return learned_steps(t);
 }
 
 // The queued retention events of edits, on the GPU before the batch's
@@ -896,43 +928,57 @@ bool face_exposed(const vgi_host::VoxelSource &src, int x, int y, int z, uint32_
 // voxels: a face that lost its air neighbour leaves (the last bound face
 // takes its place), a newly exposed face joins at the end (publication's
 // rescan of the edited pages, pages.rs:451-458).
-int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed) {
-    vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
-    struct Cand { int32_t x, y, z; uint32_t f; };
-    std::vector<Cand> cand;
-    cand.reserve(changed.size() * 12);
-    for (const Int3 &c : changed)
-        for (uint32_t f = 0; f < 6; ++f) {
-            cand.push_back(Cand{c.x, c.y, c.z, f});
-            cand.push_back(Cand{c.x + FACE_STEP[f][0], c.y + FACE_STEP[f][1], c.z + FACE_STEP[f][2], f ^ 1u});
-        }
-    auto key_less = [](const Cand &a, const Cand &b) {
-        if (a.z != b.z) return a.z < b.z;
-        if (a.y != b.y) return a.y < b.y;
-        if (a.x != b.x) return a.x < b.x;
-        return a.f < b.f;
+int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, const std::vector<vgi_host::RegionDelta> &deltas) {
+    // The bound set equals the exposed set before the edit, so only faces
+    // whose exposure flips (or whose voxel changed material) need work, and
+    // they are found from the voxels' values before and after: a changed
+    // voxel's own six faces, and its neighbours' faces towards it unless the
+    // neighbour changed too (then they are its own faces) -- each face once.
+    std::vector<int32_t> delta_of((size_t)t.rsx * t.rsy * t.rsz, -1);
+    for (size_t k = 0; k < deltas.size(); ++k) {
+        const Int3 &c = deltas[k].coord;
+        delta_of[lin3(c.x - t.rmin.x, c.y - t.rmin.y, c.z - t.rmin.z, t.rsx, t.rsy)] = (int32_t)k;
+    }
+    // Material before and after (0xffff outside the loaded scene), and whether it changed.
+    auto lookup = [&](int x, int y, int z, uint32_t &before, uint32_t &after) {
+        int rx = floor_div(x, 16) - t.rmin.x, ry = floor_div(y, 16) - t.rmin.y, rz = floor_div(z, 16) - t.rmin.z;
+        if (rx < 0 || ry < 0 || rz < 0 || rx >= t.rsx || ry >= t.rsy || rz >= t.rsz) { before = after = 0xffffu; return false; }
+        size_t ri = lin3(rx, ry, rz, t.rsx, t.rsy);
+        if (!t.loaded[ri]) { before = after = 0xffffu; return false; }
+        after = t.grid[lin3(x - t.vmin.x, y - t.vmin.y, z - t.vmin.z, t.gx, t.gy)];
+        int32_t k = delta_of[ri];
+        if (k < 0) { before = after; return false; }
+        before = deltas[k].before[(size_t)floor_mod(x, 16) + 16 * ((size_t)floor_mod(y, 16) + 16 * (size_t)floor_mod(z, 16))];
+        return before != after;
     };
-    std::sort(cand.begin(), cand.end(), key_less);
-    cand.erase(std::unique(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.x == b.x && a.y == b.y && a.z == b.z && a.f == b.f; }),
-               cand.end());
     std::vector<uint32_t> removals;
     std::vector<vgi_face_task> additions, refresh;
-    for (const Cand &c : cand) {
-        bool exposed = face_exposed(src, c.x, c.y, c.z, c.f);
-        long at = t.face_slots.find(c.x, c.y, c.z, c.f);
+    auto visit = [&](int x, int y, int z, uint32_t f, bool own_changed) {
+        uint32_t vb, va, nb, na;
+        lookup(x, y, z, vb, va);
+        lookup(x + FACE_STEP[f][0], y + FACE_STEP[f][1], z + FACE_STEP[f][2], nb, na);
+        bool was = vb < vgi_host::AIR && nb == vgi_host::AIR, now = va < vgi_host::AIR && na == vgi_host::AIR;
+        if (!was && !now) return;
+        vgi_face_task task{};
+        task.voxel[0] = x; task.voxel[1] = y; task.voxel[2] = z;
+        task.face = f;
+        if (now && !was) { additions.push_back(task); return; }
+        if (now && was) {
+            if (own_changed) refresh.push_back(task);  // stays bound; its material changed
+            return;
+        }
+        long at = t.face_slots.find(x, y, z, f);
         uint32_t slot = at >= 0 ? (t.face_slots.entries[at][3] & 0x1fffffffu) : vgi_host::NONE;
         uint32_t b = (slot != vgi_host::NONE && slot < t.bound_of_slot.size()) ? t.bound_of_slot[slot] : vgi_host::NONE;
-        if (exposed && b == vgi_host::NONE) {
-            vgi_face_task task{};
-            task.voxel[0] = c.x; task.voxel[1] = c.y; task.voxel[2] = c.z;
-            task.face = c.f;
-            additions.push_back(task);
-        } else if (!exposed && b != vgi_host::NONE) {
-            removals.push_back(b);
-        } else if (exposed && b != vgi_host::NONE) {
-            refresh.push_back(t.bound_tasks[b]);  // stays bound; its material may have changed
+        if (b != vgi_host::NONE) removals.push_back(b);
+    };
+    for (const Int3 &c : changed)
+        for (uint32_t f = 0; f < 6; ++f) {
+            visit(c.x, c.y, c.z, f, true);
+            int nx = c.x + FACE_STEP[f][0], ny = c.y + FACE_STEP[f][1], nz = c.z + FACE_STEP[f][2];
+            uint32_t b, a;
+            if (!lookup(nx, ny, nz, b, a)) visit(nx, ny, nz, f ^ 1u, false);
         }
-    }
     std::vector<uint32_t> refreshed;
     if (register_faces(t, refresh.data(), (uint32_t)refresh.size(), refreshed) != 0) return -1;
     // Removals, highest index first, each filled by the current last face.
@@ -1037,9 +1083,14 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters, &t->d_mlp_w, &t->d_mlp_m,
                             &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->b_learned, &t->b_queries,
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
-                            &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io};
+                            &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io,
+                            &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
+                            &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes};
+    if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
     for (auto &e : t->ev) hipEventDestroy(e);
+    for (auto &e : t->side_ev) if (e) hipEventDestroy(e);
+    if (t->side) hipStreamDestroy(t->side);
     hipStreamDestroy(t->stream);
     delete t;
 }
@@ -1067,6 +1118,9 @@ int vgi_upload_pages(vgi_tracer *t, const int32_t *page_xyz, uint32_t pages, con
     std::vector<Int3> rem;
     for (uint32_t i = 0; i < removed; ++i) rem.push_back(Int3{removed_xyz[i * 3], removed_xyz[i * 3 + 1], removed_xyz[i * 3 + 2]});
     int rc = apply_pages(*t, incoming, rem);
+    // Pages change residency, not just voxels: bound faces that follow the
+    // exposed surface are found again over the new scene.
+    if (rc == 0 && t->follow_exposure) rc = vgi_bind_exposed(t);
     if (rc == 0) t->cells_live = t->cells_used;
     return rc;
 }
@@ -1242,7 +1296,7 @@ int vgi_read_counters(vgi_tracer *t, uint64_t ticket, uint64_t *out, uint32_t co
 
 int vgi_timings(vgi_tracer *t, float *out, uint32_t count) {
     if (!t || !out) return fail("null argument");
-    for (uint32_t i = 0; i < count && i < 4; ++i) out[i] = t->timings[i];
+    for (uint32_t i = 0; i < count && i < 8; ++i) out[i] = t->timings[i];
     return 0;
 }
 
@@ -1343,7 +1397,20 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[4], t->stream);
-    if (t->learned && t->transport.bounces >= 2u) {
+    bool learned_frame = t->learned && t->transport.bounces >= 2u;
+    if (t->learned) {
+        // The frame's training paths run on a second stream beside the
+        // trace (they need the prepared tasks, not the network).
+        if (!t->side) {
+            HIP_OK(hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking));
+            for (auto &e : t->side_ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        HIP_OK(hipEventRecord(t->side_ev[0], t->stream));
+        HIP_OK(hipStreamWaitEvent(t->side, t->side_ev[0], 0));
+        if (learned_targets(*t, t->side) != 0) return -1;
+        HIP_OK(hipEventRecord(t->side_ev[1], t->side));
+    }
+    if (learned_frame) {
         // Paths stop at their first hit; the network answers every query.
         // (A one-bounce path has nothing after its first hit to predict.)
         size_t samples = (size_t)count * t->transport.samples;
@@ -1371,7 +1438,10 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         vgi::launch_accumulate(t->b_slots.as<uint32_t>(), count, t->b_results.as<float4>(), t->d_state.as<uint32_t>(),
                                t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
     HIP_OK(hipGetLastError());
-    if (t->learned && learned_training(*t) != 0) return -1;
+    if (t->learned) {
+        HIP_OK(hipStreamWaitEvent(t->stream, t->side_ev[1], 0));
+        if (learned_steps(*t) != 0) return -1;
+    }
     hipEventRecord(t->ev[9], t->stream);
     t->frame_index += 1;
     return 0;

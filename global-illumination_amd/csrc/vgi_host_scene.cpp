@@ -497,7 +497,10 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
         lo[0] = std::max<int64_t>(lo[0], previous.page_min.x); hi[0] = std::min<int64_t>(hi[0], previous.page_min.x + previous.psx - 1);
         lo[1] = std::max<int64_t>(lo[1], previous.page_min.y); hi[1] = std::min<int64_t>(hi[1], previous.page_min.y + previous.psy - 1);
         lo[2] = std::max<int64_t>(lo[2], previous.page_min.z); hi[2] = std::min<int64_t>(hi[2], previous.page_min.z + previous.psz - 1);
-        bool near = false;
+        // The page holding the box first (a box lies inside one region): in
+        // a populated world it usually decides at once.
+        bool near = previous.page_occupied(Int3{(int32_t)floor_div64(b.min.x, 32), (int32_t)floor_div64(b.min.y, 32),
+                                                (int32_t)floor_div64(b.min.z, 32)});
         for (int64_t z = lo[2]; z <= hi[2] && !near; ++z)
             for (int64_t y = lo[1]; y <= hi[1] && !near; ++y)
                 for (int64_t x = lo[0]; x <= hi[0] && !near; ++x) {
@@ -530,13 +533,27 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
             b.flux = 0.0f;
         }
     }
-    // Urgent: a box holding an edited voxel (retention.rs:151-158).
-    for (ChangeBox &b : out.boxes)
-        for (const Int3 &v : edits)
+    // Urgent: a box holding an edited voxel (retention.rs:151-158). A box
+    // lies inside one region, so only that region's edits are tested.
+    auto region_key = [](const Int3 &v) {
+        return ((uint64_t)(uint32_t)(floor_div64(v.z, 16) + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(floor_div64(v.y, 16) + (1 << 20)) << 21) |
+               (uint64_t)(uint32_t)(floor_div64(v.x, 16) + (1 << 20));
+    };
+    std::vector<std::pair<uint64_t, Int3>> by_region;
+    by_region.reserve(edits.size());
+    for (const Int3 &v : edits) by_region.push_back({region_key(v), v});
+    std::sort(by_region.begin(), by_region.end(), [](const auto &a, const auto &c) { return a.first < c.first; });
+    for (ChangeBox &b : out.boxes) {
+        uint64_t key = region_key(b.min);
+        auto it = std::lower_bound(by_region.begin(), by_region.end(), key, [](const auto &a, uint64_t k) { return a.first < k; });
+        for (; it != by_region.end() && it->first == key; ++it) {
+            const Int3 &v = it->second;
             if (v.x >= b.min.x && v.y >= b.min.y && v.z >= b.min.z && v.x <= b.max.x && v.y <= b.max.y && v.z <= b.max.z) {
                 b.urgent = true;
                 break;
             }
+        }
+    }
     out.union_min = Int3{INT32_MAX, INT32_MAX, INT32_MAX};
     out.union_max = Int3{INT32_MIN, INT32_MIN, INT32_MIN};
     for (const ChangeBox &b : out.boxes) {

@@ -127,7 +127,9 @@ int vgi_read_faces(vgi_tracer *tracer, uint64_t ticket, vgi_face_result *out, ui
 int vgi_read_hits(vgi_tracer *tracer, uint64_t ticket, uint32_t *out, uint64_t words);
 /* [rays, steps, header loads, brick-word loads, cell-word loads, 0...] of the ticket (VGI_COUNT_RAYS). */
 int vgi_read_counters(vgi_tracer *tracer, uint64_t ticket, uint64_t *out, uint32_t count);
-/* GPU timestamps (gpu.rs:486-497,1050-1070): ms of [prepare, trace, accumulate, build]. */
+/* GPU timestamps (gpu.rs:486-497,1050-1070): ms of [prepare, trace, accumulate,
+ * build]; count up to 8 adds the last edit's host ms: [regions + GPU re-pack,
+ * top level, retention event, exposure update]. */
 int vgi_timings(vgi_tracer *tracer, float *out, uint32_t count);
 /* Resident slots of the last batch's faces (face_index.rs:71-97 lookups). */
 int vgi_read_slots(vgi_tracer *tracer, uint32_t *out, uint32_t count);
