@@ -90,6 +90,7 @@ def library():
         "vgi_sync": (ctypes.c_int, [P]),
         "vgi_last_trace_ms": (f32, [P]),
         "vgi_last_frame_ms": (f32, [P]),
+        "vgi_set_wavefront": (i32, [P, i32]),
         "vgi_bind_exposed": (i32, [P]),
         "vgi_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
         "vgi_batch_destroy": (None, [P]),
@@ -140,7 +141,7 @@ def exported_symbols():
         "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_set_clock", "vgi_upload_pages", "vgi_edit_voxels",
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
-        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_read_bound",
+        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
@@ -370,6 +371,9 @@ class GpuTracer:
 
     def last_trace_ms(self):
         return float(self._lib.vgi_last_trace_ms(self._h))
+
+    def set_wavefront(self, on):
+        _check(self._lib.vgi_set_wavefront(self._h, int(on)))
 
     def last_frame_ms(self):
         return float(self._lib.vgi_last_frame_ms(self._h))

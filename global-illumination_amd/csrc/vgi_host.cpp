@@ -15,6 +15,7 @@
 #include "vgi_host_scene.h"
 #include "vgi_kernels.h"
 #include "vgi_mlp.h"
+#include "vgi_wavefront.h"
 #include "vgi_trace.h"
 
 #include <cmath>
@@ -144,6 +145,12 @@ struct vgi_tracer {
     float mlp_lr = 1e-3f, last_loss = 0.0f;
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
+    // ---- wavefront path tracing of bound frames
+    bool wavefront = false;
+    int cus = 0;
+    DeviceBuffer d_order;      // bound faces in Morton order of their voxels
+    bool has_order = false;
+    DeviceBuffer wf_fields[vgi::WF_PATH_FIELDS], wf_queue, wf_hit0, wf_hit1, wf_vis, wf_control;
     hipStream_t side = nullptr;           // training paths beside a learned frame's trace
     hipEvent_t side_ev[2] = {};           // [tasks prepared, targets ready]
     // ---- edit retention: events of edits, applied with the next batch or frame
@@ -621,7 +628,12 @@ int sync_emitters(vgi_tracer &t) {
 
 uint32_t lanes_for(const vgi_transport &tr) {
     if (tr.lanes) return tr.lanes;
-    return tr.samples < 16u ? 4u : 8u;
+    // voxel_trace.wgsl's lanes: 4 below 16 samples, else 8. When no lane
+    // would hold more than one sample, one lane per face adds the samples in
+    // the same order (the lane sums are the samples themselves) and wastes
+    // no idle lanes.
+    uint32_t reference = tr.samples < 16u ? 4u : 8u;
+    return tr.samples <= reference ? 1u : reference;
 }
 
 void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
@@ -856,6 +868,74 @@ int learned_training(vgi_tracer &t) {
     return learned_steps(t);
 }
 
+// Threads take the bound faces in Morton order of their voxels: each
+// wavefront's faces -- and the scene their rays start in -- stay together
+// in the caches. Records still go to each face's own index.
+uint64_t spread_bits(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+int compute_order(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
+    t.has_order = false;
+    if (count == 0) return 0;
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+    for (uint32_t i = 0; i < count; ++i)
+        for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], tasks[i].voxel[a]);
+    std::vector<std::pair<uint64_t, uint32_t>> keys(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) c[a] = (uint32_t)std::min<int64_t>((int64_t)tasks[i].voxel[a] - lo[a], 0x1fffff);
+        keys[i] = {spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2), i};
+    }
+    std::sort(keys.begin(), keys.end());
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i) order[i] = keys[i].second;
+    HIP_OK(t.d_order.reserve((size_t)count * 4));
+    HIP_OK(hipMemcpyAsync(t.d_order.ptr, order.data(), (size_t)count * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipStreamSynchronize(t.stream));
+    t.has_order = true;
+    return 0;
+}
+
+const uint32_t *order_of(const vgi_tracer &t) { return t.has_order ? t.d_order.as<uint32_t>() : nullptr; }
+
+// A frame of bound faces by wavefront path tracing (same records as the
+// megakernel): path state, ray queue and hit buffers sized to the frame.
+int wavefront_trace(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o, const uint4 *tasks, uint32_t count,
+                    uint32_t sequence_base, float4 *results, unsigned long long *rays) {
+    uint32_t samples = t.transport.samples;
+    size_t paths = (size_t)count * samples;
+    if (paths == 0) return 0;
+    if (paths * 8 >= (1ull << 32)) return fail("too many paths for one wavefront frame");
+    for (auto &f : t.wf_fields) HIP_OK(f.reserve(paths * 16));
+    HIP_OK(t.wf_queue.reserve(paths * (2 + o.light_count) * sizeof(vgi::WfRay)));
+    HIP_OK(t.wf_hit0.reserve(paths * sizeof(vgi::WfHit)));
+    HIP_OK(t.wf_hit1.reserve(paths * sizeof(vgi::WfHit)));
+    HIP_OK(t.wf_vis.reserve(paths * 8));
+    HIP_OK(t.wf_control.reserve(16));
+    vgi::WfBuffers b;
+    float4 **f[vgi::WF_PATH_FIELDS] = {&b.paths.throughput, &b.paths.energy, &b.paths.origin, &b.paths.direction, &b.paths.previous,
+                                       &b.paths.nee0, &b.paths.pending_t, &b.paths.pending_e, &b.paths.pending_d, &b.paths.hit,
+                                       &b.paths.candidate};
+    for (uint32_t k = 0; k < vgi::WF_PATH_FIELDS; ++k) *f[k] = t.wf_fields[k].as<float4>();
+    b.queue = t.wf_queue.as<vgi::WfRay>();
+    b.hit0 = t.wf_hit0.as<vgi::WfHit>();
+    b.hit1 = t.wf_hit1.as<vgi::WfHit>();
+    b.vis = t.wf_vis.as<uint8_t>();
+    b.control = t.wf_control.as<uint32_t>();
+    b.order = order_of(t);
+    uint32_t grid = (uint32_t)std::max(t.cus, 1) * (4u * VGI_TRACE_WAVES) / 4u;
+    vgi::launch_wavefront(s, o, tasks, count, samples, lanes_for(t.transport), sequence_base, b, grid, results, rays, t.stream);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
 // The queued retention events of edits, on the GPU before the batch's
 // prepare (resident.rs:1005-1061): measure the edits' flux from the resident
 // statistics, then retention over every slot.
@@ -1034,6 +1114,7 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
     HIP_OK(t.b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
     HIP_OK(hipStreamSynchronize(t.stream));
     t.bound = count;
+    t.has_order = false;  // the bound set changed: faces are taken in bound order
     if (t.mlp_ready && bind_materials(t, t.bound_tasks.data(), count) != 0) return -1;
     return 0;
 }
@@ -1053,6 +1134,7 @@ int vgi_create(int device_ordinal, vgi_tracer **out) {
     HIP_OK(hipSetDevice(device_ordinal));
     std::unique_ptr<vgi_tracer> t(new vgi_tracer());
     t->device = device_ordinal;
+    HIP_OK(hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, device_ordinal));
     HIP_OK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
     for (auto &e : t->ev) HIP_OK(hipEventCreate(&e));
     for (int m = 0; m < 256; ++m)
@@ -1085,7 +1167,9 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
                             &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io,
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
-                            &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes};
+                            &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
+                            &t->wf_vis, &t->wf_control, &t->d_order};
+    for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
     for (auto &e : t->ev) hipEventDestroy(e);
@@ -1326,6 +1410,7 @@ int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
         t->last_slots = slots;
     }
     if (bind_materials(*t, tasks, count) != 0) return -1;
+    if (compute_order(*t, tasks, count) != 0) return -1;
     t->bound = count;
     return 0;
 }
@@ -1420,15 +1505,19 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         o.cache_on = 0u;
         vgi::launch_trace_learned(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
                                   t->b_results.as<float4>(), counting ? t->b_counters.as<unsigned long long>() : nullptr,
-                                  t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>(), t->stream);
+                                  t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>(), t->stream, order_of(*t));
         vgi::mlp_infer(t->d_mlp_frag.as<uint16_t>(), mlp_scene(*t), t->b_queries.as<vgi::MlpQuery>(), (uint32_t)samples,
                        t->b_pred.as<float>(), t->stream);
         vgi::launch_learned_combine(t->b_learned.as<float4>(), t->b_pred.as<float>(), count, t->transport.samples,
                                     lanes_for(t->transport), t->b_tasks.as<uint4>(), t->b_results.as<float4>(), t->stream);
+    } else if (t->wavefront && (flags & VGI_COUNT_GATHERS) == 0 && o.light_count <= vgi::WF_MAX_LIGHTS && o.bounces <= 15u) {
+        if (wavefront_trace(*t, s, o, t->b_tasks.as<uint4>(), count, 0u, t->b_results.as<float4>(),
+                            counting ? t->b_counters.as<unsigned long long>() : nullptr) != 0)
+            return -1;
     } else {
         vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
                           t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
-                          (flags & VGI_COUNT_GATHERS) != 0, t->stream);
+                          (flags & VGI_COUNT_GATHERS) != 0, t->stream, order_of(*t));
     }
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[5], t->stream);
@@ -1464,7 +1553,8 @@ int vgi_converged(vgi_tracer *t, uint32_t launches, uint32_t first_sequence, dou
     HIP_OK(hipMemsetAsync(sums.ptr, 0, (size_t)count * 64, t->stream));
     for (uint32_t l = 0; l < launches; ++l) {
         vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport),
-                          first_sequence + l * t->transport.samples, results.as<float4>(), nullptr, nullptr, false, t->stream);
+                          first_sequence + l * t->transport.samples, results.as<float4>(), nullptr, nullptr, false, t->stream,
+                          order_of(*t));
         vgi::launch_fp64_accumulate(results.as<float4>(), count, sums.as<double>(), t->stream);
     }
     HIP_OK(hipGetLastError());
@@ -1609,6 +1699,12 @@ int vgi_sync(vgi_tracer *t) {
 
 float vgi_last_trace_ms(vgi_tracer *t) { return t ? t->last_trace_ms : 0.0f; }
 float vgi_last_frame_ms(vgi_tracer *t) { return t ? t->last_frame_ms : 0.0f; }
+
+int vgi_set_wavefront(vgi_tracer *t, int on) {
+    if (!t) return fail("null tracer");
+    t->wavefront = on != 0;
+    return 0;
+}
 
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
     if (!t || (count && !out)) return fail("null argument");

@@ -50,12 +50,14 @@ struct AccumulateParams {
     float response;  // display response seconds
 };
 
+// `order` (optional): the bound faces in the order threads take them (a
+// spatially coherent order); records still go to each face's own index.
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
                   uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits, unsigned long long *counters,
-                  bool full_counters, hipStream_t stream);
+                  bool full_counters, hipStream_t stream, const uint32_t *order = nullptr);
 void launch_trace_learned(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
                           uint32_t lanes, uint32_t sequence_base, float4 *results, unsigned long long *counters, float4 *learned,
-                          MlpQuery *queries, hipStream_t stream);
+                          MlpQuery *queries, hipStream_t stream, const uint32_t *order = nullptr);
 void launch_learned_combine(const float4 *learned, const float *pred, uint32_t count, uint32_t samples, uint32_t lanes,
                             const uint4 *tasks, float4 *results, hipStream_t stream);
 void launch_train_gather(const uint4 *bound, const uint32_t *perm, const uint32_t *face_material, uint32_t n_bound, uint32_t start,

@@ -107,13 +107,15 @@ template <class Tally, bool kHits, bool kLearned>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_WAVES))) void trace_faces_kernel(
     SceneView scene, Optics optics, const uint4 *__restrict__ tasks, uint32_t count, uint32_t samples, uint32_t lanes,
     uint32_t sequence_base, float4 *__restrict__ results, uint32_t *__restrict__ hits, unsigned long long *__restrict__ counters,
-    float4 *__restrict__ learned, MlpQuery *__restrict__ queries) {
+    float4 *__restrict__ learned, MlpQuery *__restrict__ queries, const uint32_t *__restrict__ order) {
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t face_index = tid / lanes;
+    uint32_t slot = tid / lanes;
     uint32_t lane = tid % lanes;
+    bool valid = slot < count;
+    uint32_t face_index = (order && valid) ? order[slot] : slot;
     Tally tally;
-    trace_face<Tally, kHits, kLearned>(scene, optics, tasks, face_index, face_index < count, lane, samples, lanes, sequence_base,
-                                       results, hits, learned, queries, tally);
+    trace_face<Tally, kHits, kLearned>(scene, optics, tasks, face_index, valid, lane, samples, lanes, sequence_base, results, hits,
+                                       learned, queries, tally);
     if (counters == nullptr) return;
     if constexpr (std::is_same<Tally, FullTally>::value) {
         unsigned long long c[5] = {tally.rays, tally.steps, tally.headers, tally.bricks, tally.cells};
@@ -837,7 +839,7 @@ static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n +
 // counters), all counters, rays only, or nothing.
 void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
                   uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits, unsigned long long *counters,
-                  bool full_counters, hipStream_t stream) {
+                  bool full_counters, hipStream_t stream, const uint32_t *order) {
     uint64_t threads = (uint64_t)count * lanes;
     if (threads == 0) return;
     dim3 grid(blocks(threads, 256)), block(256);
@@ -845,30 +847,30 @@ void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tas
     MlpQuery *no_q = nullptr;
     if (hits)
         hipLaunchKernelGGL((trace_faces_kernel<FullTally, true, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, hits, counters, no_l, no_q);
+                           lanes, sequence_base, results, hits, counters, no_l, no_q, order);
     else if (counters && full_counters)
         hipLaunchKernelGGL((trace_faces_kernel<FullTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, hits, counters, no_l, no_q);
+                           lanes, sequence_base, results, hits, counters, no_l, no_q, order);
     else if (counters)
         hipLaunchKernelGGL((trace_faces_kernel<RayTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, hits, counters, no_l, no_q);
+                           lanes, sequence_base, results, hits, counters, no_l, no_q, order);
     else
         hipLaunchKernelGGL((trace_faces_kernel<NoTally, false, false>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, hits, counters, no_l, no_q);
+                           lanes, sequence_base, results, hits, counters, no_l, no_q, order);
 }
 
 void launch_trace_learned(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples,
                           uint32_t lanes, uint32_t sequence_base, float4 *results, unsigned long long *counters, float4 *learned,
-                          MlpQuery *queries, hipStream_t stream) {
+                          MlpQuery *queries, hipStream_t stream, const uint32_t *order) {
     uint64_t threads = (uint64_t)count * lanes;
     if (threads == 0) return;
     dim3 grid(blocks(threads, 256)), block(256);
     if (counters)
         hipLaunchKernelGGL((trace_faces_kernel<RayTally, false, true>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, nullptr, counters, learned, queries);
+                           lanes, sequence_base, results, nullptr, counters, learned, queries, order);
     else
         hipLaunchKernelGGL((trace_faces_kernel<NoTally, false, true>), grid, block, 0, stream, scene, optics, tasks, count, samples,
-                           lanes, sequence_base, results, nullptr, counters, learned, queries);
+                           lanes, sequence_base, results, nullptr, counters, learned, queries, order);
 }
 
 void launch_learned_combine(const float4 *learned, const float *pred, uint32_t count, uint32_t samples, uint32_t lanes,

@@ -147,10 +147,24 @@ __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 
 }
 
 // voxel_traversal.wgsl:126-219 with visible_geometry = false, max_steps = 2048
-// (the path tracer's `trace`, voxel_path.wgsl:239-243).
+// (the path tracer's `trace`, voxel_path.wgsl:239-243), split into a start
+// and single steps so that a lane can take up a new ray whenever its current
+// one ends (the wavefront tracer); trace_ray runs one ray to its end.
+struct RayWalk {
+    f3 origin, direction;
+    float max_distance;
+    f3 position;
+    float distance;
+    f3 normal;
+    int lrx, lry, lrz;
+    uint4 meta;
+    uint32_t step;
+};
+
+// Clip to the scene box. Returns true when the ray already has its result.
 template <class Tally>
-__device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3 direction, float max_distance, Tally &tally) {
-    RayResult result;
+__device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 direction, float max_distance, RayWalk &w,
+                                          RayResult &result, Tally &tally) {
     result.vx = result.vy = result.vz = 0;
     result.material = 0u;
     result.position = mk(0.0f, 0.0f, 0.0f);
@@ -166,7 +180,7 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
         if (fabsf(dv[axis]) <= 1e-8f) {
-            if (ov[axis] < lo[axis] || ov[axis] > hi[axis]) return result;
+            if (ov[axis] < lo[axis] || ov[axis] > hi[axis]) return true;
             continue;
         }
         float first = (lo[axis] - ov[axis]) / dv[axis];
@@ -178,77 +192,102 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
         }
         far_ = fmn(far_, fmx(first, second));
     }
-    if (far_ < fmx(near_, 0.0f)) return result;
-    float distance = fmx(near_, 0.0f) + 0.001f;
-    f3 position = origin + direction * distance;
-    int lrx = 0x7fffffff, lry = 0x7fffffff, lrz = 0x7fffffff;
-    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
-    for (uint32_t step = 0; step < 2048u; ++step) {
-        tally.step();
-        if (distance > max_distance) return result;
-        int vx = (int)floorf(position.x), vy = (int)floorf(position.y), vz = (int)floorf(position.z);
-        int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
-        if (!in_bounds(s, rx, ry, rz)) return result;
-        if (rx != lrx || ry != lry || rz != lrz) {
-            uint4 h = region_header(s, rx, ry, rz);
-            tally.header();
-            if (h.x == 0u) {
-                result.status = geometry_ahead(s, position, direction, max_distance - distance) ? 2u : 0u;
-                return result;
-            }
-            meta = h;
-            lrx = rx; lry = ry; lrz = rz;
+    if (far_ < fmx(near_, 0.0f)) return true;
+    w.origin = origin;
+    w.direction = direction;
+    w.max_distance = max_distance;
+    w.distance = fmx(near_, 0.0f) + 0.001f;
+    w.position = origin + direction * w.distance;
+    w.normal = normal;
+    w.lrx = w.lry = w.lrz = 0x7fffffff;
+    w.meta = make_uint4(0u, 0u, 0u, 0u);
+    w.step = 0u;
+    return false;
+}
+
+// One step of the walk. Returns true when the ray has its result.
+template <class Tally>
+__device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayResult &result, Tally &tally) {
+    if (w.step >= 2048u) {
+        result.status = 2u;
+        result.material = NONE;
+        return true;
+    }
+    w.step += 1u;
+    tally.step();
+    const f3 position = w.position, direction = w.direction;
+    if (w.distance > w.max_distance) return true;
+    int vx = (int)floorf(position.x), vy = (int)floorf(position.y), vz = (int)floorf(position.z);
+    int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
+    if (!in_bounds(s, rx, ry, rz)) return true;
+    if (rx != w.lrx || ry != w.lry || rz != w.lrz) {
+        uint4 h = region_header(s, rx, ry, rz);
+        tally.header();
+        if (h.x == 0u) {
+            result.status = geometry_ahead(s, position, direction, w.max_distance - w.distance) ? 2u : 0u;
+            return true;
         }
-        f3 adv_n = mk(0.0f, 0.0f, 0.0f);
-        float adv_w = 0.0f;
-        if (meta.x == 1u) {
-            advance_box(position, direction, rx, ry, rz, (int)(meta.z & 255u), (int)((meta.z >> 8) & 255u),
-                        (int)((meta.z >> 16) & 255u), (int)(meta.w & 255u), (int)((meta.w >> 8) & 255u),
-                        (int)((meta.w >> 16) & 255u), 16.0f, adv_n, adv_w);
-        } else {
-            uint32_t material = meta.y;
-            if (meta.x == 3u) {
-                uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
-                uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
-                uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
-                tally.brick();
-                uint32_t st = brick & 3u;
-                if (st == 0u) {
-                    uint32_t e = brick >> 2u;
-                    advance_box(position, direction, rx * 4 + (int)bx, ry * 4 + (int)by, rz * 4 + (int)bz, (int)(e & 3u),
-                                (int)((e >> 4) & 3u), (int)((e >> 8) & 3u), (int)((e >> 2) & 3u), (int)((e >> 6) & 3u),
-                                (int)((e >> 10) & 3u), 4.0f, adv_n, adv_w);
-                    material = AIR;
-                } else if (st == 1u) {
-                    material = brick >> 2u;
-                } else {
-                    uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
-                    uint32_t word = s.cells[(brick >> 2u) + (idx >> 1)];
-                    tally.cell();
-                    uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
-                    material = (cell & 1u) != 0u ? cell >> 1u : AIR;
-                    if (material == AIR) {
-                        uint32_t e = cell >> 1u;
-                        advance_box(position, direction, vx, vy, vz, (int)(e & 3u), (int)((e >> 4) & 3u), (int)((e >> 8) & 3u),
-                                    (int)((e >> 2) & 3u), (int)((e >> 6) & 3u), (int)((e >> 10) & 3u), 1.0f, adv_n, adv_w);
-                    }
+        w.meta = h;
+        w.lrx = rx; w.lry = ry; w.lrz = rz;
+    }
+    const uint4 meta = w.meta;
+    f3 adv_n = mk(0.0f, 0.0f, 0.0f);
+    float adv_w = 0.0f;
+    if (meta.x == 1u) {
+        advance_box(position, direction, rx, ry, rz, (int)(meta.z & 255u), (int)((meta.z >> 8) & 255u),
+                    (int)((meta.z >> 16) & 255u), (int)(meta.w & 255u), (int)((meta.w >> 8) & 255u),
+                    (int)((meta.w >> 16) & 255u), 16.0f, adv_n, adv_w);
+    } else {
+        uint32_t material = meta.y;
+        if (meta.x == 3u) {
+            uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
+            uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
+            uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
+            tally.brick();
+            uint32_t st = brick & 3u;
+            if (st == 0u) {
+                uint32_t e = brick >> 2u;
+                advance_box(position, direction, rx * 4 + (int)bx, ry * 4 + (int)by, rz * 4 + (int)bz, (int)(e & 3u),
+                            (int)((e >> 4) & 3u), (int)((e >> 8) & 3u), (int)((e >> 2) & 3u), (int)((e >> 6) & 3u),
+                            (int)((e >> 10) & 3u), 4.0f, adv_n, adv_w);
+                material = AIR;
+            } else if (st == 1u) {
+                material = brick >> 2u;
+            } else {
+                uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
+                uint32_t word = s.cells[(brick >> 2u) + (idx >> 1)];
+                tally.cell();
+                uint32_t cell = (word >> ((idx & 1u) * 16u)) & 65535u;
+                material = (cell & 1u) != 0u ? cell >> 1u : AIR;
+                if (material == AIR) {
+                    uint32_t e = cell >> 1u;
+                    advance_box(position, direction, vx, vy, vz, (int)(e & 3u), (int)((e >> 4) & 3u), (int)((e >> 8) & 3u),
+                                (int)((e >> 2) & 3u), (int)((e >> 6) & 3u), (int)((e >> 10) & 3u), 1.0f, adv_n, adv_w);
                 }
             }
-            if (material != AIR) {
-                result.vx = vx; result.vy = vy; result.vz = vz;
-                result.material = material;
-                result.position = position;
-                result.normal = normal;
-                result.status = 1u;
-                return result;
-            }
         }
-        distance = distance + adv_w;
-        position = position + direction * adv_w;
-        normal = adv_n;
+        if (material != AIR) {
+            result.vx = vx; result.vy = vy; result.vz = vz;
+            result.material = material;
+            result.position = position;
+            result.normal = w.normal;
+            result.status = 1u;
+            return true;
+        }
     }
-    result.status = 2u;
-    result.material = NONE;
+    w.distance = w.distance + adv_w;
+    w.position = position + direction * adv_w;
+    w.normal = adv_n;
+    return false;
+}
+
+template <class Tally>
+__device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3 direction, float max_distance, Tally &tally) {
+    RayResult result;
+    RayWalk w;
+    if (ray_start(s, origin, direction, max_distance, w, result, tally)) return result;
+    while (!ray_advance(s, w, result, tally)) {
+    }
     return result;
 }
 

@@ -146,6 +146,11 @@ float vgi_last_trace_ms(vgi_tracer *tracer);
  * inference, accumulate, and the learned cache's training), valid after
  * vgi_sync. */
 float vgi_last_frame_ms(vgi_tracer *tracer);
+/* Bound-face frames by wavefront path tracing (on) or by the one-kernel
+ * estimator (off): the same records bit for bit. Frames that count gathers
+ * (VGI_COUNT_GATHERS), learned frames and more than 5 lights use the
+ * one-kernel estimator either way. */
+int vgi_set_wavefront(vgi_tracer *tracer, int on);
 /* The bound faces' 96-byte records after the last frame (out may be NULL with
  * count 0 to read only the counters); counters (optional)
  * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
