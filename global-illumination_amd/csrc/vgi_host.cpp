@@ -1867,6 +1867,7 @@ int batch_prepare(vgi_batch &b) {
         views(t, desc[k].scene, desc[k].optics);
         desc[k].tasks = t.b_tasks.as<uint4>();
         desc[k].results = t.b_results.as<float4>();
+        desc[k].order = order_of(t);
         desc[k].count = t.bound;
         uint32_t first_tile = (uint32_t)tiles.size();
         for (uint32_t f = 0; f < t.bound; f += b.tile) tiles.push_back(vgi::BatchTile{k, f, std::min(b.tile, t.bound - f), 0u});

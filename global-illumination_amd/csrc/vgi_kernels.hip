@@ -172,8 +172,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
         const BatchScene &sc = scenes[tile.scene];
         for (uint32_t f = 0u; f < tile.faces; f += per_pass) {
             uint32_t at = f + local;
-            trace_face<RayTally, false, false>(sc.scene, sc.optics, sc.tasks, tile.first + at, at < tile.faces, lane, samples, lanes,
-                                               sequence, sc.results, nullptr, nullptr, nullptr, tally);
+            bool valid = at < tile.faces;
+            uint32_t face = tile.first + at;
+            if (sc.order && valid) face = sc.order[face];
+            trace_face<RayTally, false, false>(sc.scene, sc.optics, sc.tasks, face, valid, lane, samples, lanes, sequence, sc.results,
+                                               nullptr, nullptr, nullptr, tally);
         }
         __syncthreads();
         if (threadIdx.x == 0u) {

@@ -53,6 +53,7 @@ struct BatchScene {
     Optics optics;
     const uint4 *__restrict__ tasks;
     float4 *__restrict__ results;
+    const uint32_t *__restrict__ order;  // optional: faces in processing order
     uint32_t count;
     uint32_t tiles;
 };
