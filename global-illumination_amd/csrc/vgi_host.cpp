@@ -828,6 +828,11 @@ int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets,
 // traced with plain multi-bounce paths (one sample each) into targets. It
 // needs only the scene and the bound tasks, not the network, so a learned
 // frame traces it beside its own paths.
+int trace_bound_faces(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o, const uint4 *tasks, uint32_t count,
+                      uint32_t samples, uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits,
+                      unsigned long long *counters, bool full_counters, hipStream_t stream, const uint32_t *order,
+                      float4 *learned = nullptr, vgi::MlpQuery *queries = nullptr);
+
 int learned_targets(vgi_tracer &t, hipStream_t stream) {
     uint32_t total = (t.train_per_frame / 16u) * 16u;
     if (total == 0 || t.bound == 0) return 0;
@@ -845,8 +850,9 @@ int learned_targets(vgi_tracer &t, hipStream_t stream) {
     // that hit's direct sun, with one bounce of the budget spent: targets are
     // traced one bounce short.
     o.bounces = o.bounces > 1u ? o.bounces - 1u : 1u;
-    vgi::launch_trace(s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index, t.d_train_results.as<float4>(),
-                      nullptr, nullptr, false, stream);
+    if (trace_bound_faces(t, s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index,
+                          t.d_train_results.as<float4>(), nullptr, nullptr, false, stream, nullptr) != 0)
+        return -1;
     vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), stream);
     HIP_OK(hipGetLastError());
     return 0;
@@ -904,6 +910,21 @@ int compute_order(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
 }
 
 const uint32_t *order_of(const vgi_tracer &t) { return t.has_order ? t.d_order.as<uint32_t>() : nullptr; }
+
+// Bound faces of a frame (learned frames: with `learned` and `queries`).
+int trace_bound_faces(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o, const uint4 *tasks, uint32_t count,
+                      uint32_t samples, uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits,
+                      unsigned long long *counters, bool full_counters, hipStream_t stream, const uint32_t *order,
+                      float4 *learned, vgi::MlpQuery *queries) {
+    (void)t;
+    if (learned) {
+        vgi::launch_trace_learned(s, o, tasks, count, samples, lanes, sequence_base, results, counters, learned, queries, stream, order);
+    } else {
+        vgi::launch_trace(s, o, tasks, count, samples, lanes, sequence_base, results, hits, counters, full_counters, stream, order);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
 
 // A frame of bound faces by wavefront path tracing (same records as the
 // megakernel): path state, ray queue and hit buffers sized to the frame.
@@ -1305,9 +1326,11 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
         vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), t->stream);
     hipEventRecord(t->ev[1], t->stream);
-    vgi::launch_trace(s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, 0u, k->results.as<float4>(),
-                      k->with_hits ? k->hits.as<uint32_t>() : nullptr,
-                      (k->with_counters || k->with_hits) ? k->counters.as<unsigned long long>() : nullptr, true, t->stream);
+    if (trace_bound_faces(*t, s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, 0u, k->results.as<float4>(),
+                          k->with_hits ? k->hits.as<uint32_t>() : nullptr,
+                          (k->with_counters || k->with_hits) ? k->counters.as<unsigned long long>() : nullptr, true, t->stream,
+                          nullptr) != 0)
+        return -1;
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[2], t->stream);
     if (t->resident)
@@ -1503,9 +1526,10 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         HIP_OK(t->b_queries.reserve(samples * 16));
         HIP_OK(t->b_pred.reserve(samples * 12));
         o.cache_on = 0u;
-        vgi::launch_trace_learned(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
-                                  t->b_results.as<float4>(), counting ? t->b_counters.as<unsigned long long>() : nullptr,
-                                  t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>(), t->stream, order_of(*t));
+        if (trace_bound_faces(*t, s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
+                              t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr, false,
+                              t->stream, order_of(*t), t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>()) != 0)
+            return -1;
         vgi::mlp_infer(t->d_mlp_frag.as<uint16_t>(), mlp_scene(*t), t->b_queries.as<vgi::MlpQuery>(), (uint32_t)samples,
                        t->b_pred.as<float>(), t->stream);
         vgi::launch_learned_combine(t->b_learned.as<float4>(), t->b_pred.as<float>(), count, t->transport.samples,
@@ -1515,9 +1539,10 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
                             counting ? t->b_counters.as<unsigned long long>() : nullptr) != 0)
             return -1;
     } else {
-        vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
-                          t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
-                          (flags & VGI_COUNT_GATHERS) != 0, t->stream, order_of(*t));
+        if (trace_bound_faces(*t, s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
+                              t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
+                              (flags & VGI_COUNT_GATHERS) != 0, t->stream, order_of(*t)) != 0)
+            return -1;
     }
     HIP_OK(hipGetLastError());
     hipEventRecord(t->ev[5], t->stream);
@@ -1552,9 +1577,10 @@ int vgi_converged(vgi_tracer *t, uint32_t launches, uint32_t first_sequence, dou
     HIP_OK(results.reserve((size_t)count * 96));
     HIP_OK(hipMemsetAsync(sums.ptr, 0, (size_t)count * 64, t->stream));
     for (uint32_t l = 0; l < launches; ++l) {
-        vgi::launch_trace(s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport),
-                          first_sequence + l * t->transport.samples, results.as<float4>(), nullptr, nullptr, false, t->stream,
-                          order_of(*t));
+        if (trace_bound_faces(*t, s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport),
+                              first_sequence + l * t->transport.samples, results.as<float4>(), nullptr, nullptr, false, t->stream,
+                              order_of(*t)) != 0)
+            return -1;
         vgi::launch_fp64_accumulate(results.as<float4>(), count, sums.as<double>(), t->stream);
     }
     HIP_OK(hipGetLastError());
@@ -1705,6 +1731,7 @@ int vgi_set_wavefront(vgi_tracer *t, int on) {
     t->wavefront = on != 0;
     return 0;
 }
+
 
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
     if (!t || (count && !out)) return fail("null argument");

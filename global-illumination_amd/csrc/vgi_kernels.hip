@@ -52,6 +52,7 @@ __device__ __forceinline__ void trace_face(SceneView scene, Optics optics, const
             FirstHit first;
             uint32_t rays_before = tally.ray_count();
             CacheQuery cq;
+            cq.vx = cq.vy = cq.vz = 0;
             cq.face_material = NONE;
             cq.factor = mk(0.0f, 0.0f, 0.0f);
             f3 value = trace_receiver<Tally, kLearned>(scene, optics, surface, normal, stream, t1.x + sequence_base + i, k, first,

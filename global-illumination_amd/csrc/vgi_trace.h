@@ -206,14 +206,19 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
     return false;
 }
 
-// One step of the walk. Returns true when the ray has its result.
+// One step of the walk. Returns true when the ray has ended, with its
+// `status` (0 escaped or beyond its distance, 1 hit, 2 unknown) and a hit's
+// material (NONE when the step limit ended the ray, else 0 for a miss). A hit
+// leaves the walk where it hit (w.position, w.normal).
 template <class Tally>
-__device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayResult &result, Tally &tally) {
+__device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_t &status, uint32_t &hit_material, Tally &tally) {
     if (w.step >= 2048u) {
-        result.status = 2u;
-        result.material = NONE;
+        status = 2u;
+        hit_material = NONE;
         return true;
     }
+    status = 0u;
+    hit_material = 0u;
     w.step += 1u;
     tally.step();
     const f3 position = w.position, direction = w.direction;
@@ -225,7 +230,7 @@ __device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayR
         uint4 h = region_header(s, rx, ry, rz);
         tally.header();
         if (h.x == 0u) {
-            result.status = geometry_ahead(s, position, direction, w.max_distance - w.distance) ? 2u : 0u;
+            status = geometry_ahead(s, position, direction, w.max_distance - w.distance) ? 2u : 0u;
             return true;
         }
         w.meta = h;
@@ -268,11 +273,8 @@ __device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayR
             }
         }
         if (material != AIR) {
-            result.vx = vx; result.vy = vy; result.vz = vz;
-            result.material = material;
-            result.position = position;
-            result.normal = w.normal;
-            result.status = 1u;
+            status = 1u;
+            hit_material = material;
             return true;
         }
     }
@@ -280,6 +282,30 @@ __device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayR
     w.position = position + direction * adv_w;
     w.normal = adv_n;
     return false;
+}
+
+// The ended ray's record as ray_start initialised it and ray_step ended it.
+__device__ __forceinline__ void ray_record(const RayWalk &w, uint32_t status, uint32_t hit_material, RayResult &result) {
+    result.status = status;
+    if (status == 1u) {
+        result.vx = (int)floorf(w.position.x);
+        result.vy = (int)floorf(w.position.y);
+        result.vz = (int)floorf(w.position.z);
+        result.material = hit_material;
+        result.position = w.position;
+        result.normal = w.normal;
+    } else if (hit_material == NONE) {
+        result.material = NONE;
+    }
+}
+
+// One step of the walk. Returns true when the ray has its result.
+template <class Tally>
+__device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayResult &result, Tally &tally) {
+    uint32_t status, hit_material;
+    if (!ray_step(s, w, status, hit_material, tally)) return false;
+    ray_record(w, status, hit_material, result);
+    return true;
 }
 
 template <class Tally>
@@ -543,6 +569,84 @@ __device__ __forceinline__ f3 sample_terminal_emission(const SceneView &s, const
     return mk(me.x, me.y, me.z) * factor;
 }
 
+// ---------------------------------------------------------------- estimator pieces for split tracers
+// A hit normal as 6 bits (per axis: nonzero, sign bit) and back; the round
+// trip keeps every value advance_box and ray_start produce, signed zeros
+// included.
+__device__ __forceinline__ uint32_t normal_code(f3 n) {
+    auto c = [](float v) { return (v != 0.0f ? 1u : 0u) | ((__float_as_uint(v) >> 31) << 1); };
+    return c(n.x) | (c(n.y) << 2) | (c(n.z) << 4);
+}
+__device__ __forceinline__ f3 normal_of(uint32_t code) {
+    auto v = [](uint32_t c) {
+        float x = (c & 1u) ? 1.0f : 0.0f;
+        return (c & 2u) ? -x : x;
+    };
+    return mk(v(code & 3u), v((code >> 2) & 3u), v((code >> 4) & 3u));
+}
+// The emitter sample of sample_emitter (voxel_path.wgsl:332-364) up to its
+// shadow ray: false when the sample contributes nothing; else the candidate
+// value (counted if the shadow ray is unoccluded) and the shadow ray.
+__device__ __forceinline__ bool emitter_candidate(const Optics &o, f3 position, f3 normal, uint32_t source_stream, uint32_t sequence,
+                                                  uint32_t bounce, f3 &candidate, f3 &origin, f3 &direction, float &distance) {
+    if (o.emitter_faces == 0u) return false;
+    uint32_t stream = source_stream ^ pcg(bounce + 0xa341316cu);
+    Emitter e = choose_emitter(o, position, normal, sample_1d(stream, sequence, 12u));
+    f3 en = face_normal(e.face);
+    uint32_t axis = e.face / 2u;
+    f3 tangent = axis == 0u ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    f3 bitangent = axis == 2u ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f);
+    float u13 = sample_1d(stream, sequence, 13u);
+    float u14 = sample_1d(stream, sequence, 14u);
+    f3 light_position = mk((float)e.vx, (float)e.vy, (float)e.vz) + mk(0.5f, 0.5f, 0.5f) + en * 0.5f + tangent * (u13 - 0.5f) +
+                        bitangent * (u14 - 0.5f);
+    f3 delta = light_position - position;
+    float d2 = dot3(delta, delta);
+    float reach = connection_reach(o);
+    if (d2 < 0.0001f || d2 > reach * reach) return false;
+    float dist = fsqrt(d2);
+    f3 dir = delta / dist;
+    float receiver_cosine = fmx(dot3(normal, dir), 0.0f);
+    float emitter_cosine = fmx(dot3(en, -dir), 0.0f);
+    if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || e.probability <= 0.0f) return false;
+    float light_pdf = e.probability * d2 / emitter_cosine;
+    float directional_pdf = receiver_cosine / 3.14159265359f;
+    f3 shadow_origin = position + normal * 0.002f;
+    f3 shadow_delta = light_position + en * 0.002f - shadow_origin;
+    float shadow_distance = len3(shadow_delta);
+    float4 em = o.materials[(uint32_t)e.material * 2u + 1u];
+    float factor = directional_pdf / (light_pdf + directional_pdf);
+    candidate = mk(em.x, em.y, em.z) * factor;
+    origin = shadow_origin;
+    direction = shadow_delta / shadow_distance;
+    distance = shadow_distance;
+    return true;
+}
+
+// direct_at's lights (voxel_path.wgsl:283-307): light i's shadow ray and its
+// weight; false when the light cannot reach the point.
+__device__ __forceinline__ bool light_ray(const Optics &o, uint32_t i, f3 position, f3 normal, f3 &direction, float &distance,
+                                          float &weight) {
+    float4 pr = o.lights[i * 3u];
+    f3 lp = mk(pr.x, pr.y, pr.z);
+    direction = normalize3(lp);
+    distance = o.max_distance;
+    float attenuation = 1.0f;
+    if (pr.w >= 0.0f) {
+        f3 delta = lp - position;
+        float d2 = dot3(delta, delta);
+        if (d2 <= 0.0000001f || d2 > pr.w * pr.w) return false;
+        distance = fsqrt(d2);
+        direction = delta / distance;
+        distance = distance - 0.004f;
+        attenuation = 1.0f / fmx(d2, 1.0f);
+    }
+    float cosine = fmx(dot3(normal, direction), 0.0f);
+    if (!(cosine > 0.0f)) return false;
+    weight = cosine * attenuation;
+    return true;
+}
+
 struct FirstHit {
     int vx, vy, vz;
     uint32_t face, status, material, slot;
@@ -556,94 +660,124 @@ struct CacheQuery {
     f3 factor;
 };
 
-// voxel_path.wgsl:418-496 (begin_path + advance_path with terminal = true)
+// A path of trace_path between bounces (voxel_path.wgsl:418-496): the energy
+// gathered so far, the throughput, the next ray, and the previous vertex the
+// next hit's emitter MIS weight is taken against. `bounce` counts the hits
+// shaded so far (trace_path's step).
+struct PathState {
+    f3 energy, throughput, origin, direction, previous_normal, previous_position;
+    float previous_pdf;
+    uint32_t bounce;
+};
+
+// begin_path: false when the path gathers nothing (no bounce budget or a
+// direction below the surface).
+__device__ __forceinline__ bool path_begin(const Optics &o, f3 start, f3 first_direction, f3 initial_normal, PathState &p) {
+    p.energy = mk(0.0f, 0.0f, 0.0f);
+    p.throughput = mk(1.0f, 1.0f, 1.0f);
+    p.origin = start;
+    p.direction = first_direction;
+    p.previous_normal = initial_normal;
+    p.previous_pdf = cosine_pdf(initial_normal, first_direction);
+    p.previous_position = start - initial_normal * 0.001f;
+    p.bounce = 0u;
+    return p.previous_pdf > 0.0f && o.bounces != 0u;
+}
+
+// One iteration of advance_path's loop (terminal = true): traces the path ray,
+// shades its hit (direct light, emission with its MIS weight, the emitter or
+// terminal sample, or the world cache's value) and draws the next direction.
+// Returns true when the path goes on.
 // kLearned: the path ends at its first explicit hit into the learned cache
 // (the world-cache termination rule of voxel_path.wgsl:459-463 with the
 // cached value replaced by the network's prediction of the hit face's
-// incident radiance beyond its direct sun): the returned energy is
-// everything but the prediction term, whose factor goes to `query`.
+// incident radiance beyond its direct sun): the energy gathered is everything
+// but the prediction term, whose factor goes to `query`.
+template <class Tally, bool kLearned = false>
+__device__ __forceinline__ bool path_bounce(const SceneView &s, const Optics &o, PathState &p, uint32_t stream, uint32_t sequence,
+                                            bool &unknown, FirstHit &first, Tally &tally, CacheQuery *query = nullptr) {
+    const uint32_t bounce = p.bounce;
+    RayResult hit = trace_diffuse(s, o, p.origin, p.direction, tally);
+    if (bounce == 0u) {
+        first.vx = hit.vx; first.vy = hit.vy; first.vz = hit.vz;
+        first.status = hit.status;
+        first.material = hit.material;
+        first.face = hit.status == 1u ? face_index_of(hit.normal) : 6u;
+    }
+    if (hit.status == 0u) {
+        float horizon = fclamp(p.direction.y * 0.35f + 0.65f, 0.15f, 1.0f);
+        p.energy = p.energy + p.throughput * mk(o.env_r, o.env_g, o.env_b) * horizon;
+        return false;
+    }
+    if (hit.status == 2u) {
+        unknown = true;
+        return false;
+    }
+    if (hit.status == 3u) return false;
+    float4 md = o.materials[hit.material * 2u];
+    float4 me = o.materials[hit.material * 2u + 1u];
+    f3 diffuse = mk(md.x, md.y, md.z);
+    f3 emission = mk(me.x, me.y, me.z);
+    f3 dv, dc;
+    direct_at(s, o, hit.position, hit.normal, dv, dc, tally);
+    float emission_weight = md.w;
+    f3 delta = hit.position - p.origin;
+    float selection_pdf = emission_weight / fmx(o.total_weight, 1e-20f);
+    if (emission_weight > 0.0f && o.emitter_faces != 0u && o.has_tree)
+        selection_pdf = tree_probability(o, p.previous_position, p.previous_normal, hit.vx, hit.vy, hit.vz, hit.normal);
+    float light_pdf = selection_pdf * dot3(delta, delta) / fmx(dot3(hit.normal, -p.direction), 1e-8f);
+    float mis = 1.0f;
+    if (light_pdf > 0.0f) mis = p.previous_pdf / (p.previous_pdf + light_pdf);
+    if constexpr (kLearned) {
+        p.energy = p.energy + p.throughput * (emission * mis + diffuse * dv);
+        query->vx = hit.vx; query->vy = hit.vy; query->vz = hit.vz;
+        query->face_material = face_index_of(hit.normal) | (hit.material << 8);
+        query->factor = p.throughput * diffuse;
+        return false;
+    }
+    f3 cached;
+    uint32_t cslot;
+    bool found = cached_energy(o, hit.vx, hit.vy, hit.vz, face_index_of(hit.normal), cached, cslot);
+    if (bounce == 0u) first.slot = cslot;
+    if (found) {
+        p.energy = p.energy + p.throughput * (emission * mis + diffuse * (dv + cached));
+        return false;
+    }
+    bool last = bounce + 1u == o.bounces;
+    f3 nee = last ? sample_terminal_emission(s, o, hit.position, hit.normal, stream, sequence, bounce, tally)
+                  : sample_emitter(s, o, hit.position, hit.normal, stream, sequence, bounce + 1u, tally);
+    p.energy = p.energy + p.throughput * (emission * mis + diffuse * (dv + nee));
+    if (last) return false;
+    p.throughput = p.throughput * diffuse;
+    float maximum = fmx(p.throughput.x, fmx(p.throughput.y, p.throughput.z));
+    if (maximum == 0.0f) return false;
+    if (bounce >= 1u) {
+        float survival = fclamp(maximum, 0.1f, 0.95f);
+        if (continuation_sample(stream, sequence, bounce, 2u) > survival) return false;
+        p.throughput = p.throughput / survival;
+    }
+    p.origin = hit.position + hit.normal * 0.002f;
+    p.previous_position = hit.position;
+    p.previous_normal = hit.normal;
+    p.direction = hemisphere(hit.normal, continuation_sample(stream, sequence, bounce, 0u),
+                             continuation_sample(stream, sequence, bounce, 1u));
+    p.previous_pdf = cosine_pdf(hit.normal, p.direction);
+    if (p.previous_pdf <= 0.0f) return false;
+    p.bounce = bounce + 1u;
+    return true;
+}
+
+// voxel_path.wgsl:418-496 (begin_path + advance_path with terminal = true)
 template <class Tally, bool kLearned = false>
 __device__ __forceinline__ f3 trace_path(const SceneView &s, const Optics &o, f3 start, f3 first_direction, f3 initial_normal,
                                          uint32_t stream, uint32_t sequence, bool &unknown, FirstHit &first, Tally &tally,
                                          CacheQuery *query = nullptr) {
-    f3 energy = mk(0.0f, 0.0f, 0.0f);
     unknown = false;
-    f3 throughput = mk(1.0f, 1.0f, 1.0f);
-    f3 origin = start;
-    f3 direction = first_direction;
-    f3 previous_normal = initial_normal;
-    float previous_pdf = cosine_pdf(initial_normal, first_direction);
-    f3 previous_position = start - initial_normal * 0.001f;
-    if (previous_pdf <= 0.0f) return energy;
-    uint32_t steps = o.bounces;
-    uint32_t bounce = 0u;
-    for (uint32_t step = 0u; step < steps; ++step) {
-        RayResult hit = trace_diffuse(s, o, origin, direction, tally);
-        if (bounce == 0u) {
-            first.vx = hit.vx; first.vy = hit.vy; first.vz = hit.vz;
-            first.status = hit.status;
-            first.material = hit.material;
-            first.face = hit.status == 1u ? face_index_of(hit.normal) : 6u;
-        }
-        if (hit.status == 0u) {
-            float horizon = fclamp(direction.y * 0.35f + 0.65f, 0.15f, 1.0f);
-            energy = energy + throughput * mk(o.env_r, o.env_g, o.env_b) * horizon;
-            break;
-        }
-        if (hit.status == 2u) { unknown = true; break; }
-        if (hit.status == 3u) break;
-        float4 md = o.materials[hit.material * 2u];
-        float4 me = o.materials[hit.material * 2u + 1u];
-        f3 diffuse = mk(md.x, md.y, md.z);
-        f3 emission = mk(me.x, me.y, me.z);
-        f3 dv, dc;
-        direct_at(s, o, hit.position, hit.normal, dv, dc, tally);
-        float emission_weight = md.w;
-        f3 delta = hit.position - origin;
-        float selection_pdf = emission_weight / fmx(o.total_weight, 1e-20f);
-        if (emission_weight > 0.0f && o.emitter_faces != 0u && o.has_tree)
-            selection_pdf = tree_probability(o, previous_position, previous_normal, hit.vx, hit.vy, hit.vz, hit.normal);
-        float light_pdf = selection_pdf * dot3(delta, delta) / fmx(dot3(hit.normal, -direction), 1e-8f);
-        float mis = 1.0f;
-        if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
-        if constexpr (kLearned) {
-            energy = energy + throughput * (emission * mis + diffuse * dv);
-            query->vx = hit.vx; query->vy = hit.vy; query->vz = hit.vz;
-            query->face_material = face_index_of(hit.normal) | (hit.material << 8);
-            query->factor = throughput * diffuse;
-            break;
-        }
-        f3 cached;
-        uint32_t cslot;
-        bool found = cached_energy(o, hit.vx, hit.vy, hit.vz, face_index_of(hit.normal), cached, cslot);
-        if (bounce == 0u) first.slot = cslot;
-        if (found) {
-            energy = energy + throughput * (emission * mis + diffuse * (dv + cached));
-            break;
-        }
-        bool last = step + 1u == steps;
-        f3 nee = last ? sample_terminal_emission(s, o, hit.position, hit.normal, stream, sequence, bounce, tally)
-                      : sample_emitter(s, o, hit.position, hit.normal, stream, sequence, bounce + 1u, tally);
-        energy = energy + throughput * (emission * mis + diffuse * (dv + nee));
-        if (last) break;
-        throughput = throughput * diffuse;
-        float maximum = fmx(throughput.x, fmx(throughput.y, throughput.z));
-        if (maximum == 0.0f) break;
-        if (bounce >= 1u) {
-            float survival = fclamp(maximum, 0.1f, 0.95f);
-            if (continuation_sample(stream, sequence, bounce, 2u) > survival) break;
-            throughput = throughput / survival;
-        }
-        origin = hit.position + hit.normal * 0.002f;
-        previous_position = hit.position;
-        previous_normal = hit.normal;
-        direction = hemisphere(hit.normal, continuation_sample(stream, sequence, bounce, 0u),
-                               continuation_sample(stream, sequence, bounce, 1u));
-        previous_pdf = cosine_pdf(hit.normal, direction);
-        if (previous_pdf <= 0.0f) break;
-        bounce += 1u;
+    PathState p;
+    if (!path_begin(o, start, first_direction, initial_normal, p)) return p.energy;
+    while (path_bounce<Tally, kLearned>(s, o, p, stream, sequence, unknown, first, tally, query)) {
     }
-    return energy;
+    return p.energy;
 }
 
 // voxel_path.wgsl:504-520

@@ -47,17 +47,6 @@ constexpr uint32_t K_TERM_NEE = 2u;   // (candidate x visibility) x 2 (slot 1)
 constexpr uint32_t K_TERM_HIT = 3u;   // terminal connection through the slot-7 hit
 constexpr uint32_t K_CACHED = 4u;     // the world cache's value, no emitter term
 
-__device__ __forceinline__ uint32_t normal_code(f3 n) {
-    auto c = [](float v) { return (v != 0.0f ? 1u : 0u) | ((__float_as_uint(v) >> 31) << 1); };
-    return c(n.x) | (c(n.y) << 2) | (c(n.z) << 4);
-}
-__device__ __forceinline__ f3 normal_of(uint32_t code) {
-    auto v = [](uint32_t c) {
-        float x = (c & 1u) ? 1.0f : 0.0f;
-        return (c & 2u) ? -x : x;
-    };
-    return mk(v(code & 3u), v((code >> 2) & 3u), v((code >> 4) & 3u));
-}
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 __device__ __forceinline__ float4 pack(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ float4 pack_bits(f3 v, uint32_t w) { return make_float4(v.x, v.y, v.z, __uint_as_float(w)); }
@@ -91,69 +80,6 @@ struct Emit {
 __device__ __forceinline__ void emit_all(const Emit &e, WfRay *__restrict__ queue, uint32_t *__restrict__ qcount) {
     uint32_t at = queue_reserve(qcount, e.n);
     for (uint32_t k = 0u; k < e.n; ++k) queue[at + k] = e.r[k];
-}
-
-// The emitter sample of sample_emitter (voxel_path.wgsl:332-364) up to its
-// shadow ray: false when the sample contributes nothing; else the candidate
-// value (counted if the shadow ray is unoccluded) and the shadow ray.
-__device__ __forceinline__ bool emitter_candidate(const Optics &o, f3 position, f3 normal, uint32_t source_stream, uint32_t sequence,
-                                                  uint32_t bounce, f3 &candidate, f3 &origin, f3 &direction, float &distance) {
-    if (o.emitter_faces == 0u) return false;
-    uint32_t stream = source_stream ^ pcg(bounce + 0xa341316cu);
-    Emitter e = choose_emitter(o, position, normal, sample_1d(stream, sequence, 12u));
-    f3 en = face_normal(e.face);
-    uint32_t axis = e.face / 2u;
-    f3 tangent = axis == 0u ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-    f3 bitangent = axis == 2u ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f);
-    float u13 = sample_1d(stream, sequence, 13u);
-    float u14 = sample_1d(stream, sequence, 14u);
-    f3 light_position = mk((float)e.vx, (float)e.vy, (float)e.vz) + mk(0.5f, 0.5f, 0.5f) + en * 0.5f + tangent * (u13 - 0.5f) +
-                        bitangent * (u14 - 0.5f);
-    f3 delta = light_position - position;
-    float d2 = dot3(delta, delta);
-    float reach = connection_reach(o);
-    if (d2 < 0.0001f || d2 > reach * reach) return false;
-    float dist = fsqrt(d2);
-    f3 dir = delta / dist;
-    float receiver_cosine = fmx(dot3(normal, dir), 0.0f);
-    float emitter_cosine = fmx(dot3(en, -dir), 0.0f);
-    if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || e.probability <= 0.0f) return false;
-    float light_pdf = e.probability * d2 / emitter_cosine;
-    float directional_pdf = receiver_cosine / 3.14159265359f;
-    f3 shadow_origin = position + normal * 0.002f;
-    f3 shadow_delta = light_position + en * 0.002f - shadow_origin;
-    float shadow_distance = len3(shadow_delta);
-    float4 em = o.materials[(uint32_t)e.material * 2u + 1u];
-    float factor = directional_pdf / (light_pdf + directional_pdf);
-    candidate = mk(em.x, em.y, em.z) * factor;
-    origin = shadow_origin;
-    direction = shadow_delta / shadow_distance;
-    distance = shadow_distance;
-    return true;
-}
-
-// direct_at's lights (voxel_path.wgsl:283-307): light i's shadow ray and its
-// weight; false when the light cannot reach the point.
-__device__ __forceinline__ bool light_ray(const Optics &o, uint32_t i, f3 position, f3 normal, f3 &direction, float &distance,
-                                          float &weight) {
-    float4 pr = o.lights[i * 3u];
-    f3 lp = mk(pr.x, pr.y, pr.z);
-    direction = normalize3(lp);
-    distance = o.max_distance;
-    float attenuation = 1.0f;
-    if (pr.w >= 0.0f) {
-        f3 delta = lp - position;
-        float d2 = dot3(delta, delta);
-        if (d2 <= 0.0000001f || d2 > pr.w * pr.w) return false;
-        distance = fsqrt(d2);
-        direction = delta / distance;
-        distance = distance - 0.004f;
-        attenuation = 1.0f / fmx(d2, 1.0f);
-    }
-    float cosine = fmx(dot3(normal, direction), 0.0f);
-    if (!(cosine > 0.0f)) return false;
-    weight = cosine * attenuation;
-    return true;
 }
 
 __device__ __forceinline__ void face_of_path(const uint4 *__restrict__ tasks, uint32_t face_index, uint32_t sample, uint32_t sequence_base,
