@@ -247,6 +247,55 @@ def edit_stream(vgi, args):
     }
 
 
+def work_selection(vgi, g, faces, args):
+    """GPU work selection over the cave's resident faces (one slot per bound
+    face): pick batches of `--select-picks` faces ranked by a focus at the
+    scene's centre. The first eight batches re-rank every quarter twice (the
+    timed frames' host claims expire after four); the next `--select-batches`
+    are timed. Algorithmic bytes of the selection passes per batch: the
+    classified quarter's 48-byte metadata read and 4-byte key written, every
+    slot's key read by the histogram and by emit, each pick's metadata, state
+    word, task, slot and cleared key, each mirror record's 112 bytes written
+    and its 96 + 48 bytes read."""
+    slots = int(len(faces))
+    vox = faces["voxel"]
+    centre = tuple(int(v) for v in (vox.min(axis=0) + vox.max(axis=0)) // 2)
+    g.set_schedule(focus=centre, transport_revision=1, stable_error=0.12, stable_samples=32, audit_ms=1000,
+                   mirror_budget=65536)
+    picks = args.select_picks
+    timings, picked, mirrors = [], [], []
+    for i in range(8 + args.select_batches):
+        g.select_frame(picks)
+        ms = g.select_timings()
+        if i >= 8:
+            sel = g.selection()
+            timings.append(ms)
+            picked.append(int(min(sel[1036], picks)))
+            mirrors.append(int(min(sel[1037], 65536)))
+    timings = np.array(timings)
+    select_ms = float(np.median(timings[:, 0]))
+    quarter = (slots + 3) // 4
+    n_pick = float(np.mean(picked))
+    n_mirror = float(np.mean(mirrors))
+    algorithmic = quarter * (48 + 4) + slots * 4 * 2 + n_pick * (48 + 4 + 32 + 4 + 4) + n_mirror * (112 + 96 + 48)
+    achieved = algorithmic / (select_ms / 1000.0) / 1e9
+    print("work selection: %d slots, %d picks, passes %.3f ms, trace %.3f ms, accumulate %.3f ms" % (
+        slots, picks, select_ms, float(np.median(timings[:, 1])), float(np.median(timings[:, 2]))), file=sys.stderr, flush=True)
+    return {
+        "workload": "512^3 worm cave, every bound face resident; focus at the scene centre",
+        "slots": slots,
+        "picks_budget": picks,
+        "picks_per_batch": round(n_pick, 1),
+        "mirror_records_per_batch": round(n_mirror, 1),
+        "batches_timed": args.select_batches,
+        "selection_passes_ms": round(select_ms, 4),
+        "pick_trace_ms": round(float(np.median(timings[:, 1])), 3),
+        "pick_accumulate_ms": round(float(np.median(timings[:, 2])), 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": int(algorithmic)},
+    }
+
+
 def batched_scenes(vgi, args):
     """32 seeded noise scenes of log-uniform edges in [32, 512] (multiples of
     8), 2 rays per face, 2 bounces, traced in one persistent launch per frame
@@ -338,6 +387,9 @@ def main():
     ap.add_argument("--learned-only", action="store_true", help="time learned frames only (profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-frames", type=int, default=20)
+    ap.add_argument("--no-select", action="store_true", help="skip the GPU work-selection leg")
+    ap.add_argument("--select-picks", type=int, default=262144)
+    ap.add_argument("--select-batches", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -470,6 +522,8 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "trace_faces_%s_traffic.json" % args.workload)
     if os.path.exists(traffic_file):
         line["roofline"]["traffic"] = round(json.load(open(traffic_file))["traffic_bytes_per_launch"])
+    if not args.no_select and args.workload == "cave512":
+        line["work_selection"] = work_selection(vgi, g, faces, args)
     g.close()
     if not args.no_edit_stream and args.workload == "cave512":
         line["edit_stream"] = edit_stream(vgi, args)

@@ -55,6 +55,19 @@ class Transport(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [
+        ("focus", ctypes.c_int32 * 3),
+        ("transport_revision", ctypes.c_uint32),
+        ("stable_error", ctypes.c_float),
+        ("stable_samples", ctypes.c_uint32),
+        ("audit_ms", ctypes.c_uint32),
+        ("mirror_budget", ctypes.c_uint32),
+    ]
+
+
+SEL_THRESHOLD, SEL_REMAINDER, SEL_TIES, SEL_TOTAL, SEL_MIRRORS, SEL_WORDS = 1024, 1028, 1032, 1036, 1037, 1038
+
 _lib = None
 
 
@@ -108,6 +121,14 @@ def library():
         "vgi_read_events": (i64, [P, P, u32]),
         "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
+        "vgi_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
+        "vgi_set_schedule": (ctypes.c_int, [P, ctypes.POINTER(Schedule)]),
+        "vgi_select_frame": (ctypes.c_int, [P, u32, u32]),
+        "vgi_select_timings": (ctypes.c_int, [P, P]),
+        "vgi_read_picks": (ctypes.c_int, [P, P, u32, ctypes.POINTER(u32)]),
+        "vgi_read_selection": (ctypes.c_int, [P, P, u32]),
+        "vgi_read_keys": (ctypes.c_int, [P, u32, u32, P]),
+        "vgi_read_mirror": (ctypes.c_int, [P, P, u32, ctypes.POINTER(u32)]),
         "vgi_apply_events": (ctypes.c_int, [P, P, u32, u32]),
         "vgi_probe_copy": (ctypes.c_int, [P, u64, u32, P]),
         "vgi_converged": (ctypes.c_int, [P, u32, u32, P]),
@@ -148,7 +169,8 @@ def exported_symbols():
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
-        "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune",
+        "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule",
+        "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
 
@@ -417,6 +439,53 @@ class GpuTracer:
     def cache_state(self, first, count):
         out = np.zeros((count, 36), np.uint32)
         _check(self._lib.vgi_cache_state(self._h, first, count, _ptr(out)))
+        return out
+
+    def cache_meta(self, first, count):
+        out = np.zeros((count, 12), np.uint32)
+        _check(self._lib.vgi_cache_meta(self._h, first, count, _ptr(out)))
+        return out
+
+    # ---- GPU work selection
+    def set_schedule(self, focus=(0, 0, 0), transport_revision=0, stable_error=0.0, stable_samples=0, audit_ms=0,
+                     mirror_budget=0):
+        sc = Schedule()
+        for i in range(3):
+            sc.focus[i] = int(focus[i])
+        sc.transport_revision, sc.stable_error, sc.stable_samples = transport_revision, stable_error, stable_samples
+        sc.audit_ms, sc.mirror_budget = audit_ms, mirror_budget
+        _check(self._lib.vgi_set_schedule(self._h, ctypes.byref(sc)))
+
+    def select_frame(self, picks, flags=0):
+        _check(self._lib.vgi_select_frame(self._h, picks, flags))
+
+    def select_timings(self):
+        out = np.zeros(3, np.float32)
+        _check(self._lib.vgi_select_timings(self._h, _ptr(out)))
+        return out
+
+    def picks(self):
+        count = ctypes.c_uint32()
+        _check(self._lib.vgi_read_picks(self._h, None, 0, ctypes.byref(count)))
+        out = np.zeros(count.value, np.uint32)
+        _check(self._lib.vgi_read_picks(self._h, _ptr(out), len(out), ctypes.byref(count)))
+        return out
+
+    def selection(self):
+        out = np.zeros(SEL_WORDS, np.uint32)
+        _check(self._lib.vgi_read_selection(self._h, _ptr(out), SEL_WORDS))
+        return out
+
+    def keys(self, first, count):
+        out = np.zeros(count, np.uint32)
+        _check(self._lib.vgi_read_keys(self._h, first, count, _ptr(out)))
+        return out
+
+    def mirror(self):
+        count = ctypes.c_uint32()
+        _check(self._lib.vgi_read_mirror(self._h, None, 0, ctypes.byref(count)))
+        out = np.zeros((count.value, 28), np.uint32)
+        _check(self._lib.vgi_read_mirror(self._h, _ptr(out), count.value, ctypes.byref(count)))
         return out
 
     def apply_events(self, events, event_count):

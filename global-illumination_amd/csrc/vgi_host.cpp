@@ -123,6 +123,15 @@ struct vgi_tracer {
     uint32_t high_water = 0;
     DeviceBuffer d_index, d_state, d_checkpoint, d_meta;
     size_t slot_capacity = 0;
+    // GPU work selection: the batch counter (28 bits, never 0 once a batch
+    // ran), the scheduling settings record_stability and selection read, the
+    // per-slot selection keys (persisting across batches) and the pick batch.
+    uint32_t batch_id = 0;
+    vgi_schedule schedule{};
+    DeviceBuffer d_keys, d_selection, d_pick_tasks, d_pick_slots, d_pick_results, d_mirror;
+    uint32_t picks = 0, mirror_budget = 0;
+    hipEvent_t sel_ev[4] = {};
+    bool select_recorded = false;
     std::vector<uint32_t> last_slots;
     std::chrono::steady_clock::time_point origin = std::chrono::steady_clock::now();
     int64_t fixed_clock = -1;  // resident.rs:709-710 test clock; < 0: wall clock
@@ -715,9 +724,14 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
             HIP_OK(hipMemcpyAsync(nc.ptr, t.d_checkpoint.ptr, t.slot_capacity * 48, hipMemcpyDeviceToDevice, t.stream));
             HIP_OK(hipMemcpyAsync(nm.ptr, t.d_meta.ptr, t.slot_capacity * 48, hipMemcpyDeviceToDevice, t.stream));
         }
+        // Selection keys persist; slots never classified have none.
+        DeviceBuffer nk;
+        HIP_OK(nk.reserve(cap * 4));
+        HIP_OK(hipMemsetAsync(nk.ptr, 0xff, cap * 4, t.stream));
+        if (t.slot_capacity) HIP_OK(hipMemcpyAsync(nk.ptr, t.d_keys.ptr, t.slot_capacity * 4, hipMemcpyDeviceToDevice, t.stream));
         HIP_OK(hipStreamSynchronize(t.stream));
-        t.d_state.release(); t.d_checkpoint.release(); t.d_meta.release();
-        t.d_state = ns; t.d_checkpoint = nc; t.d_meta = nm;
+        t.d_state.release(); t.d_checkpoint.release(); t.d_meta.release(); t.d_keys.release();
+        t.d_state = ns; t.d_checkpoint = nc; t.d_meta = nm; t.d_keys = nk;
         t.slot_capacity = cap;
     }
     // Seed new slots (resident.rs:1426-1470 from an empty FaceWork: all zero)
@@ -785,7 +799,18 @@ vgi::AccumulateParams accumulate_params(const vgi_tracer &t, uint32_t now) {
     p.recency = t.transport.history_recency >= 1.0f ? t.transport.history_recency : 1.0f;
     p.physical = std::max(t.transport.measured_retention, 0.0f);
     p.response = 0.075f;  // settings.rs:266 display_response 75 ms
+    p.revision = t.schedule.transport_revision;
+    p.stable_error = t.schedule.stable_error;
+    p.stable_samples = t.schedule.stable_samples;
+    p.picked = 0u;
     return p;
+}
+
+// dispatch.rs:126-131: batches that trace faces advance the batch counter
+// (28 bits, never 0); GPU picks leave a host job's face alone for four.
+uint32_t next_batch(vgi_tracer &t) {
+    t.batch_id = (t.batch_id % 0x0fffffffu) + 1u;
+    return t.batch_id;
 }
 
 vgi::MlpScene mlp_scene(const vgi_tracer &t) {
@@ -1189,12 +1214,14 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io,
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
-                            &t->wf_vis, &t->wf_control, &t->d_order};
+                            &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
+                            &t->d_pick_results, &t->d_mirror};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
     for (auto &e : t->ev) hipEventDestroy(e);
     for (auto &e : t->side_ev) if (e) hipEventDestroy(e);
+    for (auto &e : t->sel_ev) if (e) hipEventDestroy(e);
     if (t->side) hipStreamDestroy(t->side);
     hipStreamDestroy(t->stream);
     delete t;
@@ -1324,7 +1351,7 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
     hipEventRecord(t->ev[0], t->stream);
     if (t->resident)
         vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
-                            std::max(t->transport.measured_retention, 0.0f), t->stream);
+                            std::max(t->transport.measured_retention, 0.0f), next_batch(*t), t->d_keys.as<uint32_t>(), t->stream);
     hipEventRecord(t->ev[1], t->stream);
     if (trace_bound_faces(*t, s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, 0u, k->results.as<float4>(),
                           k->with_hits ? k->hits.as<uint32_t>() : nullptr,
@@ -1503,7 +1530,7 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
-                            std::max(t->transport.measured_retention, 0.0f), t->stream);
+                            std::max(t->transport.measured_retention, 0.0f), next_batch(*t), t->d_keys.as<uint32_t>(), t->stream);
     hipEventRecord(t->ev[4], t->stream);
     bool learned_frame = t->learned && t->transport.bounces >= 2u;
     if (t->learned) {
@@ -1755,6 +1782,140 @@ int vgi_apply_events(vgi_tracer *t, const uint32_t *events, uint32_t event_count
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(t->stream));
     ev.release();
+    return 0;
+}
+
+int vgi_set_schedule(vgi_tracer *t, const vgi_schedule *schedule) {
+    if (!t || !schedule) return fail("null argument");
+    t->schedule = *schedule;
+    return 0;
+}
+
+// A resident batch of GPU picks (resident.rs:1093-1164 selection pass, then
+// mark_changed, the trace and accumulate of the picked faces): after any
+// queued retention, the selection passes pick up to `picks` faces among all
+// resident slots, ranked by the schedule's focus and stability settings; the
+// picks are traced and merged like host-requested faces, and their host
+// mirror goes stale until classification emits their statistics.
+int vgi_select_frame(vgi_tracer *t, uint32_t picks, uint32_t flags) {
+    if (!t) return fail("null tracer");
+    if (!t->resident) return fail("GPU selection picks among resident faces: call vgi_enable_resident first");
+    if (!t->have_regions) return fail("no scene loaded");
+    if (picks == 0) return fail("a pick batch needs a budget");
+    HIP_OK(hipSetDevice(t->device));
+    if (sync_emitters(*t) != 0) return -1;
+    if (!t->sel_ev[0])
+        for (auto &e : t->sel_ev) HIP_OK(hipEventCreate(&e));
+    uint32_t batch = next_batch(*t);
+    if (flush_events(*t) != 0) return -1;
+    uint32_t mirror_budget = std::max<uint32_t>(t->schedule.mirror_budget, 1u);
+    HIP_OK(t->d_selection.reserve(vgi::SEL_WORDS * 4));
+    HIP_OK(t->d_pick_tasks.reserve((size_t)picks * 32));
+    HIP_OK(t->d_pick_slots.reserve((size_t)picks * 4));
+    HIP_OK(t->d_pick_results.reserve((size_t)picks * 96));
+    HIP_OK(t->d_mirror.reserve((size_t)mirror_budget * 112));
+    HIP_OK(t->b_counters.reserve(64));
+    vgi::SelectParams p;
+    p.fx = t->schedule.focus[0]; p.fy = t->schedule.focus[1]; p.fz = t->schedule.focus[2];
+    p.budget = picks;
+    p.batch = batch;
+    p.now_ms = clock_ms(*t);
+    p.revision = t->schedule.transport_revision;
+    p.stable_samples = t->schedule.stable_samples;
+    p.audit_ms = t->schedule.audit_ms;
+    // resident.rs:685-690: each batch re-ranks one quarter of the slots.
+    uint32_t chunk = (t->high_water + 3u) / 4u;
+    uint32_t first = (batch % 4u) * chunk;
+    p.first = std::min(first, t->high_water);
+    p.end = std::min(first + chunk, t->high_water);
+    p.slots = t->high_water;
+    p.mirror_budget = mirror_budget;
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
+    if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
+    hipEventRecord(t->sel_ev[0], t->stream);
+    vgi::launch_select(p, t->d_meta.as<uint32_t>(), t->d_state.as<uint32_t>(), t->d_keys.as<uint32_t>(), t->d_selection.as<uint32_t>(),
+                       t->d_mirror.as<uint4>(), t->d_pick_tasks.as<uint4>(), t->d_pick_slots.as<uint32_t>(),
+                       (uint32_t)std::max(t->cus, 1) * 4u, t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->sel_ev[1], t->stream);
+    // mark_changed for the picks (no host claim: batch 0).
+    vgi::launch_prepare(t->d_pick_tasks.as<uint4>(), t->d_pick_slots.as<uint32_t>(), picks, t->d_state.as<uint32_t>(),
+                        t->d_meta.as<uint32_t>(), std::max(t->transport.measured_retention, 0.0f), 0u, nullptr, t->stream);
+    if (trace_bound_faces(*t, s, o, t->d_pick_tasks.as<uint4>(), picks, t->transport.samples, lanes_for(t->transport), 0u,
+                          t->d_pick_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
+                          (flags & VGI_COUNT_GATHERS) != 0, t->stream, nullptr) != 0)
+        return -1;
+    hipEventRecord(t->sel_ev[2], t->stream);
+    vgi::AccumulateParams ap = accumulate_params(*t, p.now_ms);
+    ap.picked = 1u;
+    vgi::launch_accumulate(t->d_pick_slots.as<uint32_t>(), picks, t->d_pick_results.as<float4>(), t->d_state.as<uint32_t>(),
+                           t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), ap, t->stream);
+    HIP_OK(hipGetLastError());
+    hipEventRecord(t->sel_ev[3], t->stream);
+    t->picks = picks;
+    t->mirror_budget = mirror_budget;
+    t->select_recorded = true;
+    return 0;
+}
+
+int vgi_select_timings(vgi_tracer *t, float *out) {
+    if (!t || !out) return fail("null argument");
+    if (!t->select_recorded) return fail("no pick batch yet");
+    HIP_OK(hipEventSynchronize(t->sel_ev[3]));
+    HIP_OK(hipEventElapsedTime(&out[0], t->sel_ev[0], t->sel_ev[1]));
+    HIP_OK(hipEventElapsedTime(&out[1], t->sel_ev[1], t->sel_ev[2]));
+    HIP_OK(hipEventElapsedTime(&out[2], t->sel_ev[2], t->sel_ev[3]));
+    return 0;
+}
+
+int vgi_read_picks(vgi_tracer *t, uint32_t *slots, uint32_t capacity, uint32_t *count) {
+    if (!t || !count || (capacity && !slots)) return fail("null argument");
+    if (!t->select_recorded) return fail("no pick batch yet");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    uint32_t n = std::min(capacity, t->picks);
+    if (n) HIP_OK(hipMemcpy(slots, t->d_pick_slots.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    *count = t->picks;
+    return 0;
+}
+
+int vgi_read_selection(vgi_tracer *t, uint32_t *words, uint32_t count) {
+    if (!t || !words) return fail("null argument");
+    if (!t->select_recorded) return fail("no pick batch yet");
+    if (count > vgi::SEL_WORDS) return fail("more words than the selection scratch holds");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(words, t->d_selection.ptr, (size_t)count * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_read_keys(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t *keys) {
+    if (!t || (count && !keys)) return fail("null argument");
+    if ((size_t)first + count > t->high_water) return fail("slot range beyond the high-water mark");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (count) HIP_OK(hipMemcpy(keys, t->d_keys.as<uint32_t>() + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_read_mirror(vgi_tracer *t, uint32_t *words, uint32_t capacity, uint32_t *count) {
+    if (!t || !count || (capacity && !words)) return fail("null argument");
+    if (!t->select_recorded) return fail("no pick batch yet");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    uint32_t emitted = 0;
+    HIP_OK(hipMemcpy(&emitted, t->d_selection.as<uint32_t>() + vgi::SEL_MIRRORS, 4, hipMemcpyDeviceToHost));
+    emitted = std::min(emitted, t->mirror_budget);
+    uint32_t n = std::min(capacity, emitted);
+    if (n) HIP_OK(hipMemcpy(words, t->d_mirror.ptr, (size_t)n * 112, hipMemcpyDeviceToHost));
+    *count = emitted;
+    return 0;
+}
+
+int vgi_cache_meta(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t *out) {
+    if (!t || !out) return fail("null argument");
+    if ((size_t)first + count > t->high_water) return fail("slot range beyond the high-water mark");
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, t->d_meta.as<uint32_t>() + (size_t)first * 12, (size_t)count * 48, hipMemcpyDeviceToHost));
     return 0;
 }
 

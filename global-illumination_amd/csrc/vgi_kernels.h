@@ -48,7 +48,43 @@ struct AccumulateParams {
     float recency;
     float physical;  // measured retention c
     float response;  // display response seconds
+    // record_stability: the transport revision, the relative error and the
+    // samples a face needs to count as stable; picked: GPU-picked faces (their
+    // host mirror goes stale)
+    uint32_t revision;
+    float stable_error;
+    uint32_t stable_samples;
+    uint32_t picked;
 };
+
+// GPU work selection (vgi_select.hip). Scratch words: four 256-bucket key
+// histograms (coverage, refinement, sweep, urgent), then per class the
+// threshold bucket, the remainder taken at it and the ties seen at it, then
+// the pick total and the mirror-record count (resident.rs SELECTION_WORDS).
+constexpr uint32_t SEL_THRESHOLD = 1024u, SEL_REMAINDER = 1028u, SEL_TIES = 1032u, SEL_TOTAL = 1036u, SEL_MIRRORS = 1037u;
+constexpr uint32_t SEL_WORDS = 1038u;
+struct SelectParams {
+    int fx, fy, fz;           // focus voxel
+    uint32_t budget;          // GPU picks this batch
+    uint32_t batch;           // batch id (28 bits, never 0)
+    uint32_t now_ms;          // the merge clock
+    uint32_t revision;        // transport revision
+    uint32_t stable_samples;  // samples a stable face needs
+    uint32_t audit_ms;        // age at which a stable face is audited
+    uint32_t first, end;      // this batch's classified slots (a rotating quarter)
+    uint32_t slots;           // high-water mark
+    uint32_t mirror_budget;   // statistics-mirror records per batch
+};
+// Clear the scratch and the pick list, then classify, histogram, threshold and
+// emit: pick_tasks (2 uint4 per pick, face task layout) and pick_slots
+// (NONE: no pick) hold `budget` entries, mirror `mirror_budget` records of 7 uint4.
+void launch_select(const SelectParams &p, uint32_t *meta, const uint32_t *state, uint32_t *keys, uint32_t *selection, uint4 *mirror,
+                   uint4 *pick_tasks, uint32_t *pick_slots, uint32_t histogram_blocks, hipStream_t stream);
+
+// Slot ranking flags (voxel_accumulate.wgsl:460-461, resident.rs META_BYTES):
+// an edit's change binds the face's history; its host mirror is stale.
+constexpr uint32_t RANK_URGENT = 0x40000000u;
+constexpr uint32_t MIRROR_STALE = 0x20000000u;
 
 // `order` (optional): the bound faces in the order threads take them (a
 // spatially coherent order); records still go to each face's own index.
@@ -73,8 +109,10 @@ void launch_headers(const int4 *list, uint32_t n, const RegionInfo *info, const 
                     int rz, int sx, int sy, hipStream_t stream);
 void launch_top(const uint32_t *top, uint32_t cells, uint4 *headers, hipStream_t stream);
 void launch_scatter_voxels(uint16_t *grid, const uint32_t *index, const uint16_t *values, uint32_t n, hipStream_t stream);
+// batch != 0: host-requested faces of batch `batch` (claimed from GPU
+// selection; their selection keys, when `keys` is given, are cleared).
 void launch_prepare(uint4 *tasks, const uint32_t *slots, uint32_t n, uint32_t *state, uint32_t *meta, float physical,
-                    hipStream_t stream);
+                    uint32_t batch, uint32_t *keys, hipStream_t stream);
 void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint32_t *state, uint32_t *checkpoint, uint32_t *meta,
                        const AccumulateParams &p, hipStream_t stream);
 void launch_apply_events(const uint4 *events, uint32_t event_count, uint32_t high_water, uint32_t *state, uint32_t *checkpoint,

@@ -474,15 +474,27 @@ __device__ __forceinline__ bool binds(const uint32_t *st, float change, float ph
     return !lit || keep < (float)(st[5] & 0x7fffffffu);
 }
 
+// prepare (voxel_accumulate.wgsl:83-133) for the faces a batch traces:
+// begin_trace (the task takes the slot's sequence) and, for host-requested
+// faces (batch != 0), mirror_count and the CPU job's claim on the face (GPU
+// selection leaves it for four batches); then mark_changed
+// (voxel_accumulate.wgsl:432-446).
 __global__ void prepare_kernel(uint4 *__restrict__ tasks, const uint32_t *__restrict__ slots, uint32_t count,
-                               uint32_t *__restrict__ state, uint32_t *__restrict__ meta, float physical) {
+                               uint32_t *__restrict__ state, uint32_t *__restrict__ meta, float physical, uint32_t batch,
+                               uint32_t *__restrict__ keys) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     uint32_t s = slots[i];
     if (s == NONE) return;
     uint32_t *st = state + (size_t)s * 36u;
     tasks[i * 2u + 1u].x = st[6];
-    uint32_t *pw = meta + (size_t)s * 12u + 4u;
+    uint32_t *m = meta + (size_t)s * 12u;
+    if (batch != 0u) {
+        m[11] = (m[11] & 0xe0000000u) | (st[5] & 0x1fffffffu);
+        m[5] = (m[5] & 0xfu) | (batch << 4);
+        if (keys) keys[s] = NONE;
+    }
+    uint32_t *pw = m + 4u;
     float pending = __uint_as_float(*pw);
     if (pending > 0.0f) {
         if (binds(st, pending, physical)) st[5] |= 0x80000000u;
@@ -490,10 +502,64 @@ __global__ void prepare_kernel(uint4 *__restrict__ tasks, const uint32_t *__rest
     }
 }
 
+// Worst per-channel relative standard deviation of one sample, RGB with a
+// 0.002 floor (voxel_accumulate.wgsl:132-137 sample_noise).
+__device__ __forceinline__ float sample_noise(f3 mean, f3 second) {
+    return max3(mk(fsqrt(fmx(second.x - mean.x * mean.x, 0.0f)) / (0.002f + fabsf(mean.x)),
+                   fsqrt(fmx(second.y - mean.y * mean.y, 0.0f)) / (0.002f + fabsf(mean.y)),
+                   fsqrt(fmx(second.z - mean.z * mean.z, 0.0f)) / (0.002f + fabsf(mean.z))));
+}
+// voxel_accumulate.wgsl:145-151 refinement_error: the same over sqrt(count), at most 8.
+__device__ __forceinline__ float refinement_error(f3 mean, f3 second, uint32_t count) {
+    if (count < 2u) return 1.0f;
+    float c = (float)count;
+    return fmn(max3(mk(fsqrt(fmx(second.x - mean.x * mean.x, 0.0f) / c) / (0.002f + fabsf(mean.x)),
+                       fsqrt(fmx(second.y - mean.y * mean.y, 0.0f) / c) / (0.002f + fabsf(mean.y)),
+                       fsqrt(fmx(second.z - mean.z * mean.z, 0.0f) / c) / (0.002f + fabsf(mean.z)))),
+               8.0f);
+}
+// voxel_accumulate.wgsl:138-144: dark faces need more evidence before resting.
+__device__ __forceinline__ uint32_t required_samples(f3 mean, uint32_t stable_samples) {
+    return max3(mean) < 0.02f ? max(stable_samples, 256u) : stable_samples;
+}
+
+// record_stability (voxel_accumulate.wgsl:99-130, scheduling.rs
+// record_face_batch): a face's stable-batch count advances while it stays in
+// the same transport revision, its display has settled on its mean, it has
+// its required samples and both its relative error and this batch's drift
+// are small; then its ranking record (noise, merge time, brightest channel,
+// measured | mirror-stale | count) for GPU selection.
+__device__ __forceinline__ void record_stability(uint32_t *__restrict__ m, const uint32_t *__restrict__ st, float4 before,
+                                                 const AccumulateParams &p) {
+    uint32_t schedule = m[5];
+    bool same_revision = (schedule & 8u) != 0u && m[7] == p.revision;
+    uint32_t count = st[5] & 0x7fffffffu;
+    f3 after = load3(st);
+    f3 second = load3(st + 8);
+    float tolerance = p.stable_error;
+    f3 b = mk(before.x, before.y, before.z);
+    float drift = max3(mk(fabsf(b.x - after.x) / (0.002f + fmx(fabsf(b.x), fabsf(after.x))),
+                          fabsf(b.y - after.y) / (0.002f + fmx(fabsf(b.y), fabsf(after.y))),
+                          fabsf(b.z - after.z) / (0.002f + fmx(fabsf(b.z), fabsf(after.z)))));
+    f3 display = load3(st + 24);
+    bool settled = max3(mk(fabsf(display.x - after.x), fabsf(display.y - after.y), fabsf(display.z - after.z))) <=
+                   0.001f + max3(after) * tolerance * 0.5f;
+    uint32_t stable = (same_revision && settled && count >= required_samples(after, p.stable_samples) &&
+                       refinement_error(after, second, count) <= tolerance && drift <= tolerance * 0.5f)
+                          ? min((schedule & 7u) + 1u, 4u)
+                          : 0u;
+    m[5] = (schedule & 0xfffffff0u) | 8u | stable;
+    m[7] = p.revision;
+    uint32_t stale = p.picked ? MIRROR_STALE : (m[11] & MIRROR_STALE);
+    m[8] = __float_as_uint(sample_noise(after, second));
+    m[9] = p.clock_ms;
+    m[10] = __float_as_uint(max3(after));
+    m[11] = 0x80000000u | count | stale;
+}
+
 // accumulate (voxel_accumulate.wgsl:135-227): merge the batch's raw moments
 // into the slot with the history cap, recency and measured-retention keep,
-// then the display EMA. record_stability (selection ranking) is not part of
-// this path and leaves the settled flag clear.
+// then the display EMA; record_stability; the settled flag on the record.
 __global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t count, float4 *__restrict__ results,
                                   uint32_t *__restrict__ state, uint32_t *__restrict__ checkpoint, uint32_t *__restrict__ meta,
                                   AccumulateParams p) {
@@ -526,6 +592,7 @@ __global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t c
         previous_count = previous_count < kept ? previous_count : kept;
         cp[9] = cp[9] < previous_count ? cp[9] : previous_count;
     }
+    const float4 before = make_float4(__uint_as_float(st[0]), __uint_as_float(st[1]), __uint_as_float(st[2]), __uint_as_float(st[3]));
     uint32_t total = previous_count + fresh_count;
     uint32_t cnt = total < p.history_limit ? total : p.history_limit;
     float alpha = fresh_count > 0u ? fmn(p.recency * (float)fresh_count / (float)(cnt > 1u ? cnt : 1u), 1.0f) : 0.0f;
@@ -572,6 +639,9 @@ __global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t c
     const uint4 *s4 = reinterpret_cast<const uint4 *>(st);
 #pragma unroll
     for (int k = 0; k < 6; ++k) o4[k] = s4[k];
+    uint32_t *m = meta + (size_t)s * 12u;
+    record_stability(m, st, before, p);
+    if ((m[5] & 15u) == 12u) out[5] |= 0x80000000u;  // settled_flag
 }
 
 // voxel_retention.wgsl:14-18
@@ -921,9 +991,10 @@ void launch_scatter_voxels(uint16_t *grid, const uint32_t *index, const uint16_t
     hipLaunchKernelGGL(scatter_voxels_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, grid, index, values, n);
 }
 void launch_prepare(uint4 *tasks, const uint32_t *slots, uint32_t n, uint32_t *state, uint32_t *meta, float physical,
-                    hipStream_t stream) {
+                    uint32_t batch, uint32_t *keys, hipStream_t stream) {
     if (n == 0) return;
-    hipLaunchKernelGGL(prepare_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, tasks, slots, n, state, meta, physical);
+    hipLaunchKernelGGL(prepare_kernel, dim3(blocks(n, 256)), dim3(256), 0, stream, tasks, slots, n, state, meta, physical, batch,
+                       keys);
 }
 void launch_accumulate(const uint32_t *slots, uint32_t n, float4 *results, uint32_t *state, uint32_t *checkpoint, uint32_t *meta,
                        const AccumulateParams &p, hipStream_t stream) {

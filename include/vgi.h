@@ -171,6 +171,46 @@ int vgi_read_bound_tasks(vgi_tracer *tracer, vgi_face_task *out, uint32_t count)
  * finish_edits filled in measured flux (resident.rs:1326-1417 layout); returns
  * the word count, copies at most `capacity` words when out is not NULL. */
 int64_t vgi_read_events(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
+/* ---- GPU work selection (voxel_accumulate.wgsl:448-695; resident.rs:1093-1164) ---- */
+/* Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
+ * stable_samples, audit_ms, mirror_budget}, work.rs:38-50; the plugin fills
+ * them from settings.rs:337-357). Every resident batch's accumulate ranks its
+ * faces with them (record_stability, voxel_accumulate.wgsl:99-130) and GPU
+ * pick batches classify by them. All zero until set (TraceWork::default()). */
+typedef struct vgi_schedule {
+    int32_t focus[3];
+    uint32_t transport_revision; /* low 32 bits */
+    float stable_error;
+    uint32_t stable_samples;
+    uint32_t audit_ms;
+    uint32_t mirror_budget;      /* statistics-mirror records per pick batch (at least 1) */
+} vgi_schedule;
+int vgi_set_schedule(vgi_tracer *tracer, const vgi_schedule *schedule);
+/* A resident batch whose faces the GPU picks itself (TraceWork.picks with no
+ * host faces): after queued retention, the selection passes rank every
+ * resident slot (classification re-ranks one quarter per batch) and pick up
+ * to `picks` faces (urgent first, then coverage 1/4, refinement 1/2, sweep
+ * 1/4); the picks are traced and merged like host faces. Non-blocking;
+ * flags as vgi_run_frame (counters read with vgi_read_bound's counters). */
+int vgi_select_frame(vgi_tracer *tracer, uint32_t picks, uint32_t flags);
+/* GPU ms of the last pick batch: [selection passes, trace, accumulate]. */
+int vgi_select_timings(vgi_tracer *tracer, float *out);
+/* The last pick batch's pick list in pick order: resident slots, 0xffffffff
+ * for an unused place; *count = the budget. */
+int vgi_read_picks(vgi_tracer *tracer, uint32_t *slots, uint32_t capacity, uint32_t *count);
+/* The selection scratch of the last pick batch (up to 1038 words): four
+ * 256-bucket histograms (coverage, refinement, sweep, urgent), per class the
+ * threshold bucket (256: none), the remainder taken at it and the ties seen
+ * at it, then the pick total and the mirror-record count. */
+int vgi_read_selection(vgi_tracer *tracer, uint32_t *words, uint32_t count);
+/* Selection keys (class << 8 | bucket, 0xffffffff: none) of slots [first, first+count). */
+int vgi_read_keys(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *keys);
+/* Statistics-mirror records of the last pick batch, 28 words each: (slot,
+ * incarnation, revision tag, 0) and the face's six 16-byte result words, the
+ * settled flag in bit 31 of word 5 (voxel_accumulate.wgsl:537-550). */
+int vgi_read_mirror(vgi_tracer *tracer, uint32_t *words, uint32_t capacity, uint32_t *count);
+/* Slot metadata (3 uint4 per slot: identity, schedule, ranking; voxel_accumulate.wgsl:36-41) of slots [first, first+count). */
+int vgi_cache_meta(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 /* Raw resident state (9 uint4 per slot, voxel_accumulate.wgsl:1-4) of slots [first, first+count). */
 int vgi_cache_state(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 

@@ -1032,6 +1032,13 @@ struct Tracer {
     float response = 0.075f;  // settings.rs:266 display_response 75 ms
     std::unordered_map<uint64_t, uint32_t> slot_of;
     std::vector<uint32_t> last_event;  // words of the last edit's retention event
+    // GPU work selection (voxel_accumulate.wgsl:448-695): batch counter,
+    // scheduling settings, per-slot keys (NONE: not a candidate).
+    uint32_t batch_id = 0;
+    I3 focus{0, 0, 0};
+    uint32_t revision = 0, stable_samples = 0, audit_ms = 0, mirror_budget = 0;
+    float stable_error = 0.0f;
+    std::vector<uint32_t> keys;
 };
 
 // ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
@@ -1657,6 +1664,7 @@ static uint32_t allocate_slot(Tracer &t, I3 voxel, uint32_t face, uint32_t mater
         t.state.resize((size_t)(index + 1) * 36, 0);
         t.checkpoint.resize((size_t)(index + 1) * 12, 0);
         t.meta.resize((size_t)(index + 1) * 12, 0);
+        t.keys.resize((size_t)index + 1, NONE);
     }
     uint32_t *m = &t.meta[(size_t)index * 12];
     uint32_t words[12] = {(uint32_t)voxel.x, (uint32_t)voxel.y, (uint32_t)voxel.z, face | META_LIVE | (material << 16), 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1680,9 +1688,57 @@ static bool binds(const Tracer &t, uint32_t slot, float change) {
     return !lit || keep < (float)(st[5] & 0x7fffffffu);
 }
 
-// voxel_accumulate.wgsl:135-227 (record_stability and the settled flag belong
-// to GPU work selection and are not restated: the flag stays clear).
-static void accumulate_one(Tracer &t, uint32_t slot, float *result, uint32_t samples, uint32_t clock_ms) {
+constexpr uint32_t MIRROR_STALE = 0x20000000u;  // voxel_accumulate.wgsl:461
+
+// voxel_accumulate.wgsl:132-151 (sample_noise, required_samples, refinement_error)
+static float sample_noise(F3 mean, F3 second) {
+    return max3f(f3(std::sqrt(fmax_(second.x - mean.x * mean.x, 0.0f)) / (0.002f + std::fabs(mean.x)),
+                    std::sqrt(fmax_(second.y - mean.y * mean.y, 0.0f)) / (0.002f + std::fabs(mean.y)),
+                    std::sqrt(fmax_(second.z - mean.z * mean.z, 0.0f)) / (0.002f + std::fabs(mean.z))));
+}
+static uint32_t required_samples(F3 mean, uint32_t stable_samples) {
+    return max3f(mean) < 0.02f ? std::max(stable_samples, 256u) : stable_samples;
+}
+static float refinement_error(F3 mean, F3 second, uint32_t count) {
+    if (count < 2u) return 1.0f;
+    float c = (float)count;
+    return fmin_(max3f(f3(std::sqrt(fmax_(second.x - mean.x * mean.x, 0.0f) / c) / (0.002f + std::fabs(mean.x)),
+                          std::sqrt(fmax_(second.y - mean.y * mean.y, 0.0f) / c) / (0.002f + std::fabs(mean.y)),
+                          std::sqrt(fmax_(second.z - mean.z * mean.z, 0.0f) / c) / (0.002f + std::fabs(mean.z)))),
+                 8.0f);
+}
+
+// voxel_accumulate.wgsl:99-130 (record_stability)
+static void record_stability(Tracer &t, uint32_t slot, const float before[4], bool picked, uint32_t clock_ms) {
+    uint32_t *m = &t.meta[(size_t)slot * 12];
+    const uint32_t *st = &t.state[(size_t)slot * 36];
+    uint32_t schedule = m[5];
+    bool same_revision = (schedule & 8u) != 0u && m[7] == t.revision;
+    uint32_t count = st[5] & 0x7fffffffu;
+    F3 after = load3(&st[0]);
+    F3 second = load3(&st[8]);
+    float tolerance = t.stable_error;
+    float drift = max3f(f3(std::fabs(before[0] - after.x) / (0.002f + fmax_(std::fabs(before[0]), std::fabs(after.x))),
+                           std::fabs(before[1] - after.y) / (0.002f + fmax_(std::fabs(before[1]), std::fabs(after.y))),
+                           std::fabs(before[2] - after.z) / (0.002f + fmax_(std::fabs(before[2]), std::fabs(after.z)))));
+    F3 display = load3(&st[24]);
+    bool settled = max3f(f3(std::fabs(display.x - after.x), std::fabs(display.y - after.y), std::fabs(display.z - after.z))) <=
+                   0.001f + max3f(after) * tolerance * 0.5f;
+    uint32_t stable = (same_revision && settled && count >= required_samples(after, t.stable_samples) &&
+                       refinement_error(after, second, count) <= tolerance && drift <= tolerance * 0.5f)
+                          ? std::min((schedule & 7u) + 1u, 4u)
+                          : 0u;
+    m[5] = (schedule & 0xfffffff0u) | 8u | stable;
+    m[7] = t.revision;
+    uint32_t stale = picked ? MIRROR_STALE : (m[11] & MIRROR_STALE);
+    m[8] = fbits(sample_noise(after, second));
+    m[9] = clock_ms;
+    m[10] = fbits(max3f(after));
+    m[11] = 0x80000000u | count | stale;
+}
+
+// voxel_accumulate.wgsl:135-227, record_stability and the settled flag.
+static void accumulate_one(Tracer &t, uint32_t slot, float *result, uint32_t samples, uint32_t clock_ms, bool picked = false) {
     uint32_t *st = &t.state[(size_t)slot * 36];
     uint32_t *cp = &t.checkpoint[(size_t)slot * 12];
     uint32_t *out = (uint32_t *)result;
@@ -1709,6 +1765,7 @@ static void accumulate_one(Tracer &t, uint32_t slot, float *result, uint32_t sam
         previous_count = std::min(previous_count, (uint32_t)fmin_(keep, 65536.0f));
         cp[9] = std::min(cp[9], previous_count);
     }
+    const float before[4] = {bitsf(st[0]), bitsf(st[1]), bitsf(st[2]), bitsf(st[3])};
     uint32_t count = std::min(previous_count + fresh_count, t.history_limit);
     float alpha = fresh_count > 0u ? fmin_(t.recency * (float)fresh_count / (float)std::max(count, 1u), 1.0f) : 0.0f;
     for (int k = 0; k < 4; ++k) {
@@ -1748,13 +1805,30 @@ static void accumulate_one(Tracer &t, uint32_t slot, float *result, uint32_t sam
     st[7] = st[24];
     st[13] = st[25]; st[14] = st[26]; st[15] = st[27];
     memcpy(out, st, 24 * 4);
+    record_stability(t, slot, before, picked, clock_ms);
+    if ((t.meta[(size_t)slot * 12 + 5] & 15u) == 12u) out[5] |= 0x80000000u;  // settled_flag
+}
+
+// dispatch.rs:126-131: the batch counter (28 bits, never 0).
+static uint32_t next_batch(Tracer &t) {
+    t.batch_id = (t.batch_id % 0x0fffffffu) + 1u;
+    return t.batch_id;
 }
 
 static int cache_batch(Tracer &t, uint32_t *tasks, const uint32_t *slots, uint32_t count, float *results, uint32_t *hits,
                        uint64_t *counters, int threads, uint32_t clock_ms) {
     tracer_sync(t);
-    // prepare (voxel_accumulate.wgsl:83-133): each traced face takes its slot's sequence.
-    for (uint32_t i = 0; i < count; ++i) tasks[(size_t)i * 8 + 4] = t.state[(size_t)slots[i] * 36 + 6];
+    // prepare (voxel_accumulate.wgsl:83-133): each traced face takes its slot's
+    // sequence; mirror_count; the host job claims the face from GPU selection.
+    uint32_t batch = next_batch(t);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t s = slots[i];
+        tasks[(size_t)i * 8 + 4] = t.state[(size_t)s * 36 + 6];
+        uint32_t *m = &t.meta[(size_t)s * 12];
+        m[11] = (m[11] & 0xe0000000u) | (t.state[(size_t)s * 36 + 5] & 0x1fffffffu);
+        m[5] = (m[5] & 0xfu) | (batch << 4);
+        t.keys[s] = NONE;
+    }
     // mark_changed (voxel_accumulate.wgsl:432-446)
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t s = slots[i];
@@ -1767,6 +1841,183 @@ static int cache_batch(Tracer &t, uint32_t *tasks, const uint32_t *slots, uint32
     trace_faces(t, tasks, count, results, hits, counters, threads);
     for (uint32_t i = 0; i < count; ++i) accumulate_one(t, slots[i], &results[(size_t)i * 24], t.transport.samples, clock_ms);
     return 0;
+}
+
+// ---------------------------------------------------------------- GPU work selection (voxel_accumulate.wgsl:448-695)
+constexpr uint32_t SEL_THRESHOLD = 1024u, SEL_REMAINDER = 1028u, SEL_TIES = 1032u, SEL_TOTAL = 1036u, SEL_MIRRORS = 1037u;
+constexpr uint32_t SEL_WORDS = 1038u;
+
+// log2 as the selection kernels evaluate it: exponent plus the atanh series
+// of the mantissa in [sqrt(1/2), sqrt(2)) to z^11, the same operations in the
+// same order (the buckets need it to a few 1e-7).
+static float log2_sel(float x) {
+    if (!(x > 0.0f)) return -INFINITY;
+    uint32_t u = fbits(x);
+    int e = (int)((u >> 23) & 255u);
+    if (e == 0) {
+        x = x * 8388608.0f;
+        u = fbits(x);
+        e = (int)((u >> 23) & 255u) - 23;
+    }
+    e -= 127;
+    float m = bitsf((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float z = (m - 1.0f) / (m + 1.0f);
+    float z2 = z * z;
+    float series = 1.0f / 11.0f;
+    series = series * z2 + 1.0f / 9.0f;
+    series = series * z2 + 1.0f / 7.0f;
+    series = series * z2 + 1.0f / 5.0f;
+    series = series * z2 + 1.0f / 3.0f;
+    series = series * z2 + 1.0f;
+    float ln_m = 2.0f * z * series;
+    return (float)e + ln_m * 1.44269504f;
+}
+
+static inline int page_of(int v) { return (v - (v < 0 ? 31 : 0)) / 32; }
+
+// voxel_accumulate.wgsl:474-516 (candidate)
+static uint32_t candidate_key(const Tracer &t, uint32_t slot, uint32_t batch, uint32_t now_ms) {
+    const uint32_t *m = &t.meta[(size_t)slot * 12];
+    if ((m[3] & META_LIVE) == 0u) return NONE;
+    uint32_t schedule = m[5];
+    uint32_t last_cpu = schedule >> 4;
+    if (last_cpu != 0u && ((batch - last_cpu) & 0x0fffffffu) < 4u) return NONE;
+    uint32_t rank_w = m[11];
+    uint32_t count = rank_w & 0x1fffffffu;
+    bool fresh = (schedule & 8u) != 0u && m[7] == t.revision;
+    uint32_t since = now_ms - m[9];
+    uint32_t age_ms = rank_w < 0x80000000u ? 0x7fffffffu : (since > 0x7fffffffu ? 0u : since);
+    uint32_t required = bitsf(m[10]) < 0.02f ? std::max(t.stable_samples, 256u) : t.stable_samples;
+    bool due = !fresh || count < required || (schedule & 7u) < 4u || age_ms >= t.audit_ms || (rank_w & URGENT) != 0u;
+    if (!due) return NONE;
+    int px = (int32_t)m[0], py = (int32_t)m[1], pz = (int32_t)m[2];
+    if ((rank_w & URGENT) != 0u || count < 8u) {
+        float dx = (float)(px - t.focus.x), dy = (float)(py - t.focus.y), dz = (float)(pz - t.focus.z);
+        uint32_t bucket = std::min((uint32_t)(log2_sel(dx * dx + dy * dy + dz * dz + 1.0f) * 8.0f), 255u);
+        return (((rank_w & URGENT) != 0u ? 3u : 0u) << 8) | bucket;
+    }
+    int gx = std::abs(page_of(px) - page_of(t.focus.x)), gy = std::abs(page_of(py) - page_of(t.focus.y)),
+        gz = std::abs(page_of(pz) - page_of(t.focus.z));
+    int page_distance = std::max(gx, std::max(gy, gz));
+    if (page_distance <= 2) {
+        float age = fmin_((float)age_ms * 0.001f, 10.0f);
+        float error = count < 2u ? 1.0f : fmin_(bitsf(m[8]) / std::sqrt((float)count), 8.0f);
+        float priority = error + age * 0.1f + (page_distance <= 1 ? 0.25f : 0.0f);
+        uint32_t level = std::min((uint32_t)(fmax_(log2_sel(priority) + 8.0f, 0.0f) * 16.0f), 255u);
+        return (1u << 8) | (255u - level);
+    }
+    uint32_t level = std::min((uint32_t)(log2_sel((float)age_ms + 1.0f) * 8.0f), 255u);
+    return (2u << 8) | (255u - level);
+}
+
+// voxel_accumulate.wgsl:577-633 (select_threshold) on the histograms in
+// selection[0..1023]: inclusive scans, the class shares of the budget with
+// lending, and each class's threshold bucket and remainder.
+static void select_thresholds(uint32_t *selection, uint32_t picks) {
+    uint32_t scan[4][256];
+    for (uint32_t c = 0; c < 4; ++c) {
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < 256; ++b) { acc += selection[c * 256 + b]; scan[c][b] = acc; }
+    }
+    uint32_t take[4];
+    take[3] = std::min(scan[3][255], picks - picks / 4u);
+    uint32_t rest = picks - take[3];
+    take[0] = std::min(rest / 4u, scan[0][255]);
+    take[1] = std::min(rest / 2u, scan[1][255]);
+    take[2] = std::min(rest - rest / 4u - rest / 2u, scan[2][255]);
+    uint32_t spare = picks - take[0] - take[1] - take[2] - take[3];
+    const uint32_t lend[4] = {1u, 0u, 2u, 3u};
+    for (uint32_t o = 0; o < 4; ++o) {
+        uint32_t c = lend[o];
+        uint32_t extra = std::min(spare, scan[c][255] - take[c]);
+        take[c] += extra;
+        spare -= extra;
+    }
+    for (uint32_t c = 0; c < 4; ++c) {
+        selection[SEL_THRESHOLD + c] = 256u;
+        selection[SEL_REMAINDER + c] = 0u;
+        for (uint32_t b = 0; b < 256; ++b) {
+            uint32_t below = b > 0 ? scan[c][b - 1] : 0u;
+            if (take[c] > 0u && below < take[c] && scan[c][b] >= take[c]) {
+                selection[SEL_THRESHOLD + c] = b;
+                selection[SEL_REMAINDER + c] = take[c] - below;
+            }
+        }
+    }
+}
+
+// One pick batch up to its pick list: classify this batch's quarter
+// (resident.rs:685-690) with its mirror records (in slot order), the key
+// histograms, and the class thresholds (select_threshold). `selection` gets
+// the scratch as the GPU leaves it; the pick list itself is the GPU's
+// (orc_cache_picks), since which ties at a threshold are taken depends on the
+// order the GPU reserves them.
+static void select_batch(Tracer &t, uint32_t picks, uint32_t now_ms, uint32_t *selection, std::vector<uint32_t> &mirrored) {
+    uint32_t batch = next_batch(t);
+    memset(selection, 0, SEL_WORDS * 4);
+    uint32_t chunk = (t.high_water + 3u) / 4u;
+    uint32_t first = std::min((batch % 4u) * chunk, t.high_water), end = std::min((batch % 4u) * chunk + chunk, t.high_water);
+    mirrored.clear();
+    for (uint32_t slot = first; slot < end; ++slot) {
+        t.keys[slot] = candidate_key(t, slot, batch, now_ms);
+        uint32_t *m = &t.meta[(size_t)slot * 12];
+        if ((m[11] & MIRROR_STALE) != 0u && (m[3] & META_LIVE) != 0u) {
+            uint32_t record = selection[SEL_MIRRORS]++;
+            if (record < std::max(t.mirror_budget, 1u)) {
+                m[11] &= ~MIRROR_STALE;
+                mirrored.push_back(slot);
+            }
+        }
+    }
+    for (uint32_t slot = 0; slot < t.high_water; ++slot) {
+        uint32_t key = t.keys[slot];
+        if (key != NONE) selection[std::min(key >> 8, 3u) * 256u + (key & 255u)] += 1u;
+    }
+    select_thresholds(selection, picks);
+    // select_emit's counters: ties seen at each threshold, keys accepted.
+    for (uint32_t slot = 0; slot < t.high_water; ++slot) {
+        uint32_t key = t.keys[slot];
+        if (key == NONE) continue;
+        uint32_t c = std::min(key >> 8, 3u), bucket = key & 255u, thr = selection[SEL_THRESHOLD + c];
+        if (bucket > thr) continue;
+        if (bucket == thr) {
+            uint32_t tie = selection[SEL_TIES + c]++;
+            if (tie >= selection[SEL_REMAINDER + c]) continue;
+        }
+        selection[SEL_TOTAL] += 1u;
+    }
+}
+
+// The GPU's pick list traced and merged (select_emit's key clear and
+// begin_trace, mark_changed, trace, accumulate with the mirror going stale).
+static int cache_picks(Tracer &t, const uint32_t *pick_slots, uint32_t n, float *results, int threads, uint32_t clock_ms) {
+    tracer_sync(t);
+    std::vector<uint32_t> slots, tasks;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t s = pick_slots[i];
+        if (s == NONE) continue;
+        t.keys[s] = NONE;
+        const uint32_t *m = &t.meta[(size_t)s * 12];
+        slots.push_back(s);
+        uint32_t task[8] = {m[0], m[1], m[2], m[3] & 7u, t.state[(size_t)s * 36 + 6], 0u, 0u, 0u};
+        tasks.insert(tasks.end(), task, task + 8);
+    }
+    for (uint32_t s : slots) {
+        uint32_t &pw = t.meta[(size_t)s * 12 + 4];
+        float pending = bitsf(pw);
+        if (pending <= 0.0f) continue;
+        if (binds(t, s, pending)) t.state[(size_t)s * 36 + 5] |= 0x80000000u;
+        else pw = 0u;
+    }
+    std::vector<float> out(slots.size() * 24);
+    trace_faces(t, tasks.data(), (uint32_t)slots.size(), out.data(), nullptr, nullptr, threads);
+    for (size_t i = 0; i < slots.size(); ++i) accumulate_one(t, slots[i], &out[i * 24], t.transport.samples, clock_ms, true);
+    if (results) memcpy(results, out.data(), out.size() * 4);
+    return (int)slots.size();
 }
 
 // voxel_retention.wgsl:14-18
@@ -2186,6 +2437,61 @@ int orc_cache_meta(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
     if ((size_t)(first + count) * 12 > t.meta.size()) return -1;
     memcpy(out, &t.meta[(size_t)first * 12], (size_t)count * 12 * 4);
     return 0;
+}
+
+int orc_cache_schedule(void *tp, const int32_t focus[3], uint32_t revision, float stable_error, uint32_t stable_samples,
+                       uint32_t audit_ms, uint32_t mirror_budget) {
+    Tracer &t = *(Tracer *)tp;
+    t.focus = I3{focus[0], focus[1], focus[2]};
+    t.revision = revision;
+    t.stable_error = stable_error;
+    t.stable_samples = stable_samples;
+    t.audit_ms = audit_ms;
+    t.mirror_budget = mirror_budget;
+    return 0;
+}
+
+int orc_select(void *tp, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity) {
+    Tracer &t = *(Tracer *)tp;
+    std::vector<uint32_t> slots;
+    select_batch(t, picks, clock_ms, selection, slots);
+    for (size_t i = 0; i < slots.size() && i < capacity; ++i) mirrored[i] = slots[i];
+    return (int)slots.size();
+}
+
+float orc_select_log2(float x) { return log2_sel(x); }
+
+uint32_t orc_select_key(void *tp, uint32_t slot, uint32_t batch, uint32_t now_ms) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)slot * 12 + 12 > t.meta.size()) return NONE;
+    return candidate_key(t, slot, batch, now_ms);
+}
+
+int orc_select_threshold(const uint32_t *histograms, uint32_t picks, uint32_t *thresholds, uint32_t *remainders) {
+    uint32_t selection[SEL_WORDS] = {};
+    memcpy(selection, histograms, 1024 * 4);
+    select_thresholds(selection, picks);
+    memcpy(thresholds, &selection[SEL_THRESHOLD], 16);
+    memcpy(remainders, &selection[SEL_REMAINDER], 16);
+    return 0;
+}
+
+int orc_cache_set_meta(void *tp, uint32_t first, uint32_t count, const uint32_t *words) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)(first + count) * 12 > t.meta.size()) return -1;
+    memcpy(&t.meta[(size_t)first * 12], words, (size_t)count * 48);
+    return 0;
+}
+
+int orc_cache_keys(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)first + count > t.keys.size()) return -1;
+    memcpy(out, &t.keys[first], (size_t)count * 4);
+    return 0;
+}
+
+int orc_cache_picks(void *tp, const uint32_t *pick_slots, uint32_t count, float *results, int threads, uint32_t clock_ms) {
+    return cache_picks(*(Tracer *)tp, pick_slots, count, results, threads, clock_ms);
 }
 
 int orc_cache_apply_events(void *tp, const uint32_t *events, uint32_t event_count, uint32_t words) {

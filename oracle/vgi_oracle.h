@@ -122,6 +122,28 @@ int orc_cache_state(void *tracer, uint32_t first, uint32_t count, uint32_t *out)
 int orc_cache_meta(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
 int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_count,
                            uint32_t words);
+/* ---- GPU work selection (voxel_accumulate.wgsl:99-130, 448-695) ---- */
+/* Scheduling settings record_stability and classification read. */
+int orc_cache_schedule(void *tracer, const int32_t focus[3], uint32_t revision, float stable_error, uint32_t stable_samples,
+                       uint32_t audit_ms, uint32_t mirror_budget);
+/* One pick batch's selection passes up to the pick list: `selection` gets the
+ * 1038 scratch words; returns the number of mirror records, their slots (in
+ * slot order) copied to `mirrored` (at most `capacity`). */
+int orc_select(void *tracer, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity);
+/* The selection's log2 (the bucket formulas' logarithm). */
+float orc_select_log2(float x);
+/* A slot's selection key (class << 8 | bucket, or 0xffffffff) for batch
+ * `batch` at `now_ms` under the tracer's schedule (voxel_accumulate.wgsl:474-516). */
+uint32_t orc_select_key(void *tracer, uint32_t slot, uint32_t batch, uint32_t now_ms);
+/* select_threshold on given histograms (4 x 256): per class the threshold bucket (256: none) and remainder. */
+int orc_select_threshold(const uint32_t *histograms, uint32_t picks, uint32_t *thresholds, uint32_t *remainders);
+/* Overwrite slot metadata (12 words per slot) of allocated slots. */
+int orc_cache_set_meta(void *tracer, uint32_t first, uint32_t count, const uint32_t *words);
+/* Selection keys of slots [first, first+count). */
+int orc_cache_keys(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
+/* The pick list (resident slots, 0xffffffff skipped) traced and merged as GPU
+ * picks; results (optional): 24 floats per traced pick. Returns picks traced. */
+int orc_cache_picks(void *tracer, const uint32_t *pick_slots, uint32_t count, float *results, int threads, uint32_t clock_ms);
 /* voxel_retention.wgsl:9-12 */
 /* Edit the scene and build its retention event (one event's encoded words,
  * measured against this tracer's resident statistics); returns the word

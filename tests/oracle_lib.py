@@ -77,6 +77,14 @@ def lib():
         "orc_edit_event": (i64, [P, P, P, u32]),
         "orc_last_event": (i64, [P, P, i64]),
         "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
+        "orc_cache_schedule": (ctypes.c_int, [P, P, u32, f32, u32, u32, u32]),
+        "orc_select": (ctypes.c_int, [P, u32, u32, P, P, u32]),
+        "orc_cache_keys": (ctypes.c_int, [P, u32, u32, P]),
+        "orc_cache_picks": (ctypes.c_int, [P, P, u32, P, ctypes.c_int, u32]),
+        "orc_select_log2": (f32, [f32]),
+        "orc_select_key": (u32, [P, u32, u32, u32]),
+        "orc_select_threshold": (ctypes.c_int, [P, u32, P, P]),
+        "orc_cache_set_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_trace_ray": (ctypes.c_int, [P, P, P, f32, P, P]),
         "orc_mlp_forward": (ctypes.c_int, [P, P, u32, P, P, P]),
         "orc_mlp_gradient": (ctypes.c_int, [P, P, P, u32, P, P, P, P]),
@@ -243,6 +251,47 @@ class Tracer:
     def apply_events(self, events, event_count):
         events = np.ascontiguousarray(events, np.uint32).reshape(-1)
         lib().orc_cache_apply_events(self.h, ptr(events), event_count, events.size)
+
+    def set_schedule(self, focus=(0, 0, 0), transport_revision=0, stable_error=0.0, stable_samples=0, audit_ms=0,
+                     mirror_budget=0):
+        f = np.asarray(focus, np.int32)
+        lib().orc_cache_schedule(self.h, ptr(f), transport_revision, stable_error, stable_samples, audit_ms, mirror_budget)
+
+    def select(self, picks, clock_ms=0):
+        """One pick batch's selection passes: (1038 scratch words, mirrored slots)."""
+        sel = np.zeros(1038, np.uint32)
+        cap = 1 << 22
+        mirrored = np.zeros(cap, np.uint32)
+        n = lib().orc_select(self.h, picks, clock_ms, ptr(sel), ptr(mirrored), cap)
+        return sel, mirrored[:min(n, cap)].copy()
+
+    def select_key(self, slot, batch, now_ms):
+        return int(lib().orc_select_key(self.h, slot, batch, now_ms))
+
+    def set_meta(self, first, words):
+        words = np.ascontiguousarray(words, np.uint32).reshape(-1, 12)
+        assert lib().orc_cache_set_meta(self.h, first, len(words), ptr(words)) == 0
+
+    def cache_keys(self, first, count):
+        out = np.zeros(count, np.uint32)
+        assert lib().orc_cache_keys(self.h, first, count, ptr(out)) == 0
+        return out
+
+    def cache_picks(self, pick_slots, threads=8, clock_ms=0):
+        pick_slots = np.ascontiguousarray(pick_slots, np.uint32)
+        return lib().orc_cache_picks(self.h, ptr(pick_slots), len(pick_slots), None, threads, clock_ms)
+
+
+def select_log2(x):
+    return float(lib().orc_select_log2(float(x)))
+
+
+def select_threshold(histograms, picks):
+    h = np.ascontiguousarray(histograms, np.uint32).reshape(1024)
+    thr = np.zeros(4, np.uint32)
+    rem = np.zeros(4, np.uint32)
+    lib().orc_select_threshold(ptr(h), picks, ptr(thr), ptr(rem))
+    return thr, rem
 
 
 def mlp_forward(weights, queries, diffuse, scene):
