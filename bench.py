@@ -254,9 +254,10 @@ def work_selection(vgi, g, faces, args):
     timed frames' host claims expire after four); the next `--select-batches`
     are timed. Algorithmic bytes of the selection passes per batch: the
     classified quarter's 48-byte metadata read and 4-byte key written, every
-    slot's key read by the histogram and by emit, each pick's metadata, state
-    word, task, slot and cleared key, each mirror record's 112 bytes written
-    and its 96 + 48 bytes read."""
+    slot's key read by the histogram, the count and the emit pass, each
+    pick's cleared place (36 bytes) and its metadata, state word, task, slot
+    and cleared key, each mirror record's 112 bytes written and its 96 + 48
+    bytes read."""
     slots = int(len(faces))
     vox = faces["voxel"]
     centre = tuple(int(v) for v in (vox.min(axis=0) + vox.max(axis=0)) // 2)
@@ -277,7 +278,8 @@ def work_selection(vgi, g, faces, args):
     quarter = (slots + 3) // 4
     n_pick = float(np.mean(picked))
     n_mirror = float(np.mean(mirrors))
-    algorithmic = quarter * (48 + 4) + slots * 4 * 2 + n_pick * (48 + 4 + 32 + 4 + 4) + n_mirror * (112 + 96 + 48)
+    algorithmic = (quarter * (48 + 4) + slots * 4 * 3 + picks * 36 + n_pick * (48 + 4 + 32 + 4 + 4)
+                   + n_mirror * (112 + 96 + 48))
     achieved = algorithmic / (select_ms / 1000.0) / 1e9
     print("work selection: %d slots, %d picks, passes %.3f ms, trace %.3f ms, accumulate %.3f ms" % (
         slots, picks, select_ms, float(np.median(timings[:, 1])), float(np.median(timings[:, 2]))), file=sys.stderr, flush=True)

@@ -128,7 +128,7 @@ struct vgi_tracer {
     // per-slot selection keys (persisting across batches) and the pick batch.
     uint32_t batch_id = 0;
     vgi_schedule schedule{};
-    DeviceBuffer d_keys, d_selection, d_pick_tasks, d_pick_slots, d_pick_results, d_mirror;
+    DeviceBuffer d_keys, d_selection, d_pick_tasks, d_pick_slots, d_pick_results, d_mirror, d_select_ranges;
     uint32_t picks = 0, mirror_budget = 0;
     hipEvent_t sel_ev[4] = {};
     bool select_recorded = false;
@@ -1215,7 +1215,7 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
-                            &t->d_pick_results, &t->d_mirror};
+                            &t->d_pick_results, &t->d_mirror, &t->d_select_ranges};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
@@ -1814,6 +1814,7 @@ int vgi_select_frame(vgi_tracer *t, uint32_t picks, uint32_t flags) {
     HIP_OK(t->d_pick_slots.reserve((size_t)picks * 4));
     HIP_OK(t->d_pick_results.reserve((size_t)picks * 96));
     HIP_OK(t->d_mirror.reserve((size_t)mirror_budget * 112));
+    HIP_OK(t->d_select_ranges.reserve((size_t)vgi::SELECT_RANGES * 16 * 4));
     HIP_OK(t->b_counters.reserve(64));
     vgi::SelectParams p;
     p.fx = t->schedule.focus[0]; p.fy = t->schedule.focus[1]; p.fz = t->schedule.focus[2];
@@ -1838,7 +1839,7 @@ int vgi_select_frame(vgi_tracer *t, uint32_t picks, uint32_t flags) {
     hipEventRecord(t->sel_ev[0], t->stream);
     vgi::launch_select(p, t->d_meta.as<uint32_t>(), t->d_state.as<uint32_t>(), t->d_keys.as<uint32_t>(), t->d_selection.as<uint32_t>(),
                        t->d_mirror.as<uint4>(), t->d_pick_tasks.as<uint4>(), t->d_pick_slots.as<uint32_t>(),
-                       (uint32_t)std::max(t->cus, 1) * 4u, t->stream);
+                       t->d_select_ranges.as<uint32_t>(), (uint32_t)std::max(t->cus, 1) * 4u, t->stream);
     HIP_OK(hipGetLastError());
     hipEventRecord(t->sel_ev[1], t->stream);
     // mark_changed for the picks (no host claim: batch 0).

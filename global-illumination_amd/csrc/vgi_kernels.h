@@ -78,8 +78,10 @@ struct SelectParams {
 // Clear the scratch and the pick list, then classify, histogram, threshold and
 // emit: pick_tasks (2 uint4 per pick, face task layout) and pick_slots
 // (NONE: no pick) hold `budget` entries, mirror `mirror_budget` records of 7 uint4.
+// `ranges`: SELECT_RANGES x 16 words of emit scratch (per-range counts and offsets).
+constexpr uint32_t SELECT_RANGES = 2048u;
 void launch_select(const SelectParams &p, uint32_t *meta, const uint32_t *state, uint32_t *keys, uint32_t *selection, uint4 *mirror,
-                   uint4 *pick_tasks, uint32_t *pick_slots, uint32_t histogram_blocks, hipStream_t stream);
+                   uint4 *pick_tasks, uint32_t *pick_slots, uint32_t *ranges, uint32_t histogram_blocks, hipStream_t stream);
 
 // Slot ranking flags (voxel_accumulate.wgsl:460-461, resident.rs META_BYTES):
 // an edit's change binds the face's history; its host mirror is stale.

@@ -1956,7 +1956,8 @@ static void select_thresholds(uint32_t *selection, uint32_t picks) {
 // the scratch as the GPU leaves it; the pick list itself is the GPU's
 // (orc_cache_picks), since which ties at a threshold are taken depends on the
 // order the GPU reserves them.
-static void select_batch(Tracer &t, uint32_t picks, uint32_t now_ms, uint32_t *selection, std::vector<uint32_t> &mirrored) {
+static void select_batch(Tracer &t, uint32_t picks, uint32_t now_ms, uint32_t *selection, std::vector<uint32_t> &mirrored,
+                         std::vector<uint32_t> *pick_list = nullptr) {
     uint32_t batch = next_batch(t);
     memset(selection, 0, SEL_WORDS * 4);
     uint32_t chunk = (t.high_water + 3u) / 4u;
@@ -1978,7 +1979,10 @@ static void select_batch(Tracer &t, uint32_t picks, uint32_t now_ms, uint32_t *s
         if (key != NONE) selection[std::min(key >> 8, 3u) * 256u + (key & 255u)] += 1u;
     }
     select_thresholds(selection, picks);
-    // select_emit's counters: ties seen at each threshold, keys accepted.
+    // select_emit: ties seen at each threshold, keys accepted (the ties in
+    // slot order up to the remainder), the pick list in slot order up to the
+    // budget (a retired slot leaves a hole).
+    if (pick_list) pick_list->assign(picks, NONE);
     for (uint32_t slot = 0; slot < t.high_water; ++slot) {
         uint32_t key = t.keys[slot];
         if (key == NONE) continue;
@@ -1988,7 +1992,8 @@ static void select_batch(Tracer &t, uint32_t picks, uint32_t now_ms, uint32_t *s
             uint32_t tie = selection[SEL_TIES + c]++;
             if (tie >= selection[SEL_REMAINDER + c]) continue;
         }
-        selection[SEL_TOTAL] += 1u;
+        uint32_t pick = selection[SEL_TOTAL]++;
+        if (pick_list && pick < picks && (t.meta[(size_t)slot * 12 + 3] & META_LIVE) != 0u) (*pick_list)[pick] = slot;
     }
 }
 
@@ -2451,11 +2456,13 @@ int orc_cache_schedule(void *tp, const int32_t focus[3], uint32_t revision, floa
     return 0;
 }
 
-int orc_select(void *tp, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity) {
+int orc_select(void *tp, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity,
+               uint32_t *pick_list) {
     Tracer &t = *(Tracer *)tp;
-    std::vector<uint32_t> slots;
-    select_batch(t, picks, clock_ms, selection, slots);
+    std::vector<uint32_t> slots, list;
+    select_batch(t, picks, clock_ms, selection, slots, &list);
     for (size_t i = 0; i < slots.size() && i < capacity; ++i) mirrored[i] = slots[i];
+    if (pick_list) memcpy(pick_list, list.data(), (size_t)picks * 4);
     return (int)slots.size();
 }
 

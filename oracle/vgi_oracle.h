@@ -126,10 +126,13 @@ int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_
 /* Scheduling settings record_stability and classification read. */
 int orc_cache_schedule(void *tracer, const int32_t focus[3], uint32_t revision, float stable_error, uint32_t stable_samples,
                        uint32_t audit_ms, uint32_t mirror_budget);
-/* One pick batch's selection passes up to the pick list: `selection` gets the
- * 1038 scratch words; returns the number of mirror records, their slots (in
- * slot order) copied to `mirrored` (at most `capacity`). */
-int orc_select(void *tracer, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity);
+/* One pick batch's selection passes: `selection` gets the 1038 scratch words,
+ * `pick_list` (optional, `picks` words) the pick list (accepted keys in slot
+ * order, ties up to each class's remainder; 0xffffffff where unused); returns
+ * the number of mirror records, their slots (in slot order) copied to
+ * `mirrored` (at most `capacity`). The picks are traced by orc_cache_picks. */
+int orc_select(void *tracer, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity,
+               uint32_t *pick_list);
 /* The selection's log2 (the bucket formulas' logarithm). */
 float orc_select_log2(float x);
 /* A slot's selection key (class << 8 | bucket, or 0xffffffff) for batch

@@ -78,7 +78,7 @@ def lib():
         "orc_last_event": (i64, [P, P, i64]),
         "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_schedule": (ctypes.c_int, [P, P, u32, f32, u32, u32, u32]),
-        "orc_select": (ctypes.c_int, [P, u32, u32, P, P, u32]),
+        "orc_select": (ctypes.c_int, [P, u32, u32, P, P, u32, P]),
         "orc_cache_keys": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_picks": (ctypes.c_int, [P, P, u32, P, ctypes.c_int, u32]),
         "orc_select_log2": (f32, [f32]),
@@ -258,12 +258,14 @@ class Tracer:
         lib().orc_cache_schedule(self.h, ptr(f), transport_revision, stable_error, stable_samples, audit_ms, mirror_budget)
 
     def select(self, picks, clock_ms=0):
-        """One pick batch's selection passes: (1038 scratch words, mirrored slots)."""
+        """One pick batch's selection passes: (1038 scratch words, mirrored
+        slots, pick list)."""
         sel = np.zeros(1038, np.uint32)
         cap = 1 << 22
         mirrored = np.zeros(cap, np.uint32)
-        n = lib().orc_select(self.h, picks, clock_ms, ptr(sel), ptr(mirrored), cap)
-        return sel, mirrored[:min(n, cap)].copy()
+        pick_list = np.zeros(picks, np.uint32)
+        n = lib().orc_select(self.h, picks, clock_ms, ptr(sel), ptr(mirrored), cap, ptr(pick_list))
+        return sel, mirrored[:min(n, cap)].copy(), pick_list
 
     def select_key(self, slot, batch, now_ms):
         return int(lib().orc_select_key(self.h, slot, batch, now_ms))

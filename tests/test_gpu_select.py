@@ -3,10 +3,10 @@ own native test asks of it (resident.rs:2066-2190).
 
 Bar: the selection scratch (four 256-bucket histograms, thresholds,
 remainders, tie and pick totals, mirror count) and every slot's key
-bit-identical; the pick list exactly the keys below each class threshold
-plus `remainder` of the ties at it (which ties is the GPU's reservation
-order); the mirror records the oracle's set; after the picks are traced and
-merged, every slot's state and metadata bit-identical.
+bit-identical; the pick list identical (the keys below each class threshold
+plus the first `remainder` ties at it, in slot order); the mirror records
+the oracle's set; after the picks are traced and merged, every slot's state
+and metadata bit-identical.
 """
 import numpy as np
 import pytest
@@ -81,13 +81,14 @@ def test_selection_matches_oracle(vgi, orc, budget):
         state_before = t.cache_state(0, hw)
         g.set_clock(clock)
         g.select_frame(budget)
-        sel_cpu, mirrored = t.select(budget, clock_ms=clock)
+        sel_cpu, mirrored, picks_cpu = t.select(budget, clock_ms=clock)
         keys_before = t.cache_keys(0, hw)
         sel_gpu = g.selection()
         assert np.array_equal(sel_gpu, sel_cpu), "round %d: selection scratch differs at %s" % (
             rnd, np.nonzero(sel_gpu != sel_cpu)[0][:8].tolist())
         picks = g.picks()
         check_picks(picks, keys_before, sel_gpu, budget)
+        assert np.array_equal(picks, picks_cpu), "round %d: pick lists differ" % rnd
         mirror = g.mirror()
         assert sorted(mirror[:, 0].tolist()) == sorted(mirrored.tolist())
         for rec in mirror:
