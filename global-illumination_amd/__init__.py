@@ -123,6 +123,7 @@ def library():
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_set_schedule": (ctypes.c_int, [P, ctypes.POINTER(Schedule)]),
+        "vgi_set_footprints": (ctypes.c_int, [P, ctypes.c_uint32]),
         "vgi_select_frame": (ctypes.c_int, [P, u32, u32]),
         "vgi_select_timings": (ctypes.c_int, [P, P]),
         "vgi_read_picks": (ctypes.c_int, [P, P, u32, ctypes.POINTER(u32)]),
@@ -169,7 +170,7 @@ def exported_symbols():
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
-        "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule",
+        "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -455,6 +456,9 @@ class GpuTracer:
         sc.transport_revision, sc.stable_error, sc.stable_samples = transport_revision, stable_error, stable_samples
         sc.audit_ms, sc.mirror_budget = audit_ms, mirror_budget
         _check(self._lib.vgi_set_schedule(self._h, ctypes.byref(sc)))
+
+    def set_footprints(self, on):
+        _check(self._lib.vgi_set_footprints(self._h, 1 if on else 0))
 
     def select_frame(self, picks, flags=0):
         _check(self._lib.vgi_select_frame(self._h, picks, flags))

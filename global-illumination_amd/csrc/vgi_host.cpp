@@ -128,7 +128,9 @@ struct vgi_tracer {
     // per-slot selection keys (persisting across batches) and the pick batch.
     uint32_t batch_id = 0;
     vgi_schedule schedule{};
+    bool footprints = true;  // change footprints traced with each edit's events
     DeviceBuffer d_keys, d_selection, d_pick_tasks, d_pick_slots, d_pick_results, d_mirror, d_select_ranges;
+    DeviceBuffer d_footprint_tasks, d_footprint_sums;
     uint32_t picks = 0, mirror_budget = 0;
     hipEvent_t sel_ev[4] = {};
     bool select_recorded = false;
@@ -655,6 +657,7 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.lights = t.d_lights.as<float4>();
     o.emitters = t.d_emitters.as<uint4>();
     o.light_count = t.light_count;
+    o.directional = std::min(t.directional, t.light_count);
     o.emitter_faces = t.emitters.faces;
     o.total_weight = t.emitters.total_weight;
     o.has_tree = t.emitters.nodes > 0 ? 1u : 0u;
@@ -987,8 +990,14 @@ int wavefront_trace(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o
 // statistics, then retention over every slot.
 int flush_events(vgi_tracer &t) {
     if (t.pending_events.empty() || !t.resident) return 0;
+    // Change footprints reserve their Light events before the encoding
+    // (gpu/resident.rs:606-626).
+    std::vector<vgi_host::Footprint> footprints;
+    if (t.footprints) footprints = vgi_host::plan_footprints(t.pending_events);
     vgi_host::EncodedEvents enc;
     vgi_host::encode_events(t.pending_events, enc);
+    std::vector<int32_t> footprint_words;
+    uint32_t footprint_threads = vgi_host::encode_footprints(footprints, t.pending_events, footprint_words);
     t.pending_events.clear();
     if (t.material_light_dirty) {
         // resident.rs:815-838: luminance of each material's albedo and emission.
@@ -1004,6 +1013,26 @@ int flush_events(vgi_tracer &t) {
     }
     HIP_OK(t.d_events.reserve(enc.words.size() * 4));
     HIP_OK(hipMemcpyAsync(t.d_events.ptr, enc.words.data(), enc.words.size() * 4, hipMemcpyHostToDevice, t.stream));
+    if (!footprints.empty()) {
+        // Traced in the new scene, ahead of the measurement and the retention
+        // that apply the events (gpu.rs:864-874, dispatch_footprints).
+        std::vector<uint32_t> sums(footprints.size() * 8);
+        for (size_t k = 0; k < footprints.size(); ++k) {
+            uint32_t init[8] = {vgi::NONE, vgi::NONE, vgi::NONE, 0u, 0u, 0u, 0u, 0u};
+            std::memcpy(&sums[k * 8], init, sizeof(init));
+        }
+        HIP_OK(t.d_footprint_tasks.reserve(footprint_words.size() * 4));
+        HIP_OK(t.d_footprint_sums.reserve(sums.size() * 4));
+        HIP_OK(hipMemcpyAsync(t.d_footprint_tasks.ptr, footprint_words.data(), footprint_words.size() * 4, hipMemcpyHostToDevice,
+                              t.stream));
+        HIP_OK(hipMemcpyAsync(t.d_footprint_sums.ptr, sums.data(), sums.size() * 4, hipMemcpyHostToDevice, t.stream));
+        vgi::SceneView s;
+        vgi::Optics o;
+        views(t, s, o);
+        vgi::launch_footprints(s, o, t.d_footprint_tasks.as<int4>(), (uint32_t)footprints.size(), footprint_threads,
+                               t.d_footprint_sums.as<uint32_t>(), t.d_events.as<uint4>(), t.stream);
+        HIP_OK(hipGetLastError());
+    }
     if (enc.entries) {
         HIP_OK(t.d_measures.reserve(enc.measures.size() * 4));
         HIP_OK(t.d_measure_max.reserve((size_t)enc.entries * 4));
@@ -1215,7 +1244,8 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
-                            &t->d_pick_results, &t->d_mirror, &t->d_select_ranges};
+                            &t->d_pick_results, &t->d_mirror, &t->d_select_ranges,
+                            &t->d_footprint_tasks, &t->d_footprint_sums};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
@@ -1782,6 +1812,12 @@ int vgi_apply_events(vgi_tracer *t, const uint32_t *events, uint32_t event_count
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(t->stream));
     ev.release();
+    return 0;
+}
+
+int vgi_set_footprints(vgi_tracer *t, uint32_t on) {
+    if (!t) return fail("null tracer");
+    t->footprints = on != 0u;
     return 0;
 }
 

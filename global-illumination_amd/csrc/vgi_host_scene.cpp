@@ -604,4 +604,53 @@ void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out
         }
 }
 
+
+std::vector<Footprint> plan_footprints(std::vector<RetentionEvent> &events) {
+    std::vector<Footprint> boxes;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool edits = pass == 0;
+        for (size_t e = 0; e < events.size(); ++e)
+            for (const ChangeBox &b : events[e].boxes) {
+                bool wanted = (b.kind == 3u || b.kind == 6u) ? edits : (b.kind == 0u ? !edits : false);
+                if (wanted && boxes.size() < MAX_FOOTPRINTS) boxes.push_back(Footprint{b.min, b.max, 0});
+            }
+    }
+    const Int3 lo{1 << 29, 1 << 29, 1 << 29}, hi{-(1 << 29), -(1 << 29), -(1 << 29)};
+    for (Footprint &f : boxes) {
+        RetentionEvent light;
+        light.union_min = lo;
+        light.union_max = hi;
+        light.boxes.push_back(ChangeBox{lo, hi, 0.0f, 0u, 4u, false});
+        events.push_back(light);
+        f.event = events.size() - 1;
+    }
+    return boxes;
+}
+
+uint32_t encode_footprints(const std::vector<Footprint> &footprints, const std::vector<RetentionEvent> &events,
+                           std::vector<int32_t> &words) {
+    words.clear();
+    std::vector<uint32_t> first_bound(events.size());
+    uint32_t first = 0;
+    for (size_t e = 0; e < events.size(); ++e) {
+        first_bound[e] = first;
+        first += (uint32_t)events[e].boxes.size();
+    }
+    const uint32_t records = (uint32_t)events.size() * 3u;
+    auto push = [&](int32_t a, int32_t b, int32_t c, int32_t d) {
+        words.push_back(a); words.push_back(b); words.push_back(c); words.push_back(d);
+    };
+    push((int32_t)footprints.size(), 0, 0, 0);
+    uint32_t threads = 0;
+    for (const Footprint &f : footprints) {
+        float dx = (float)(f.max.x - f.min.x + 1), dy = (float)(f.max.y - f.min.y + 1), dz = (float)(f.max.z - f.min.z + 1);
+        float length = std::sqrt(dx * dx + dy * dy + dz * dz);
+        uint32_t grid = (uint32_t)std::min(std::max(std::ceil(length), 1.0f), 64.0f);
+        push(f.min.x, f.min.y, f.min.z, (int32_t)(f.event * 3u));
+        push(f.max.x, f.max.y, f.max.z, (int32_t)(records + first_bound[f.event] * 2u));
+        push((int32_t)threads, (int32_t)grid, 0, 0);
+        threads += grid * grid;
+    }
+    return threads;
+}
 }  // namespace vgi_host

@@ -136,6 +136,21 @@ struct EncodedEvents {
 };
 void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out);
 
+// Change footprints (gpu/resident.rs:1215-1310): plan_footprints reserves one
+// Light event (kind 4, empty bound) per edited box of `events` (edits first,
+// then boxes of unknown flux; at most 64 a batch) and returns each box with
+// its event's index; encode_footprints lays out the GPU tasks (a count, then
+// per task (min, header record) (max, bound record) (first thread, grid, 0,
+// 0)) for the events as encode_events lays them out, and returns the threads.
+constexpr uint32_t MAX_FOOTPRINTS = 64;
+struct Footprint {
+    Int3 min, max;
+    size_t event;
+};
+std::vector<Footprint> plan_footprints(std::vector<RetentionEvent> &events);
+uint32_t encode_footprints(const std::vector<Footprint> &footprints, const std::vector<RetentionEvent> &events,
+                           std::vector<int32_t> &words);
+
 constexpr int INFLUENCE_RADIUS = 192;     // retention.rs:32-39 at HISTORY_LIMIT 65536
 constexpr uint32_t FLUX_SCAN_BUDGET = 16384;  // publication.rs:413
 

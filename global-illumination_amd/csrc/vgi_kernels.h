@@ -137,6 +137,12 @@ void launch_batch_frame(const BatchScene *scenes, uint32_t n_scenes, const Batch
                         uint32_t block, uint32_t grid, BatchControl *control, uint32_t *done, unsigned long long *finish,
                         hipStream_t stream);
 
+// Change footprints (voxel_trace.wgsl:94-202): trace the tasks' tubes into
+// `sums` (8 words per task, initialised to min 0xffffffff x3, max 0 x3, flux
+// 0, spare 0) and write each landing into its reserved Light event.
+void launch_footprints(const SceneView &scene, const Optics &optics, const int4 *tasks, uint32_t task_count, uint32_t threads,
+                       uint32_t *sums, uint4 *events, hipStream_t stream);
+
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 
 }  // namespace vgi

@@ -906,6 +906,107 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+// ============================================================== change footprints
+// voxel_trace.wgsl:94-202: sunlight a changed box admits or blocks, traced in
+// the new scene. One task per changed box, one thread per tube of a square
+// grid across the box's diagonal, per light: a tube whose sun side is clear
+// beyond the box is traced back through it; where it lands, the landing
+// voxel widens the task's footprint and the light reflected there adds
+// flux. Tasks (int4): a count, then per task (min, header record), (max,
+// bound record), (first thread, grid, 0, 0). Sums (8 words per task): the
+// landing voxel's biased min xyz and max xyz, the flux x 64, spare.
+constexpr int FOOTPRINT_BIAS = 1073741824;
+constexpr float FOOTPRINT_FLUX_SCALE = 64.0f;
+
+// The last task whose first thread is at most t (first threads ascend).
+__device__ __forceinline__ uint32_t footprint_task(const int4 *__restrict__ tasks, uint32_t t) {
+    uint32_t lo = 0u, hi = (uint32_t)tasks[0].x;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) / 2u;
+        if ((uint32_t)tasks[1u + mid * 3u + 2u].x <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void footprint_trace_kernel(SceneView scene, Optics optics, const int4 *__restrict__ tasks,
+                                                              uint32_t threads, uint32_t *__restrict__ sums) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= threads || tasks[0].x == 0) return;
+    uint32_t e = footprint_task(tasks, id);
+    int4 a = tasks[1u + e * 3u], b = tasks[1u + e * 3u + 1u], c = tasks[1u + e * 3u + 2u];
+    uint32_t grid = (uint32_t)c.y, local = id - (uint32_t)c.x;
+    if (local >= grid * grid) return;
+    f3 lo = mk((float)a.x, (float)a.y, (float)a.z);
+    f3 hi = mk((float)b.x, (float)b.y, (float)b.z) + mk(1.0f, 1.0f, 1.0f);
+    f3 center = (lo + hi) * 0.5f;
+    // A square grid across the box's diagonal covers its section from any
+    // direction; tubes that miss the box are skipped below.
+    float diagonal = len3(hi - lo);
+    float spacing = diagonal / (float)grid;
+    float ox = ((float)(local % grid) + 0.5f) * spacing - 0.5f * diagonal;
+    float oy = ((float)(local / grid) + 0.5f) * spacing - 0.5f * diagonal;
+    const f3 w = mk(0.2126f, 0.7152f, 0.0722f);
+    const float reach = optics.max_distance;
+    const uint32_t base = e * 8u;
+    NoTally tally;
+    float flux = 0.0f;
+    for (uint32_t i = 0u; i < optics.directional; ++i) {
+        float4 pr = optics.lights[i * 3u], en = optics.lights[i * 3u + 1u], ex = optics.lights[i * 3u + 2u];
+        f3 color = mk(fmx(en.x - ex.y, 0.0f), fmx(en.y - ex.z, 0.0f), fmx(en.z - ex.w, 0.0f));
+        if (color.x == 0.0f && color.y == 0.0f && color.z == 0.0f) continue;
+        f3 direction = normalize3(mk(pr.x, pr.y, pr.z));
+        f3 side = fabsf(direction.y) > 0.9f ? mk(1.0f, 0.0f, 0.0f) : mk(0.0f, 1.0f, 0.0f);
+        f3 u = normalize3(cross3(direction, side));
+        f3 v = cross3(direction, u);
+        f3 point = center + u * ox + v * oy;
+        // Exact axis directions would make 0 * inf; a tiny component is exact enough.
+        f3 inverse = mk(1.0f / (fabsf(direction.x) < 1e-8f ? 1e-8f : direction.x), 1.0f / (fabsf(direction.y) < 1e-8f ? 1e-8f : direction.y),
+                        1.0f / (fabsf(direction.z) < 1e-8f ? 1e-8f : direction.z));
+        f3 nr = (lo - point) * inverse, fr = (hi - point) * inverse;
+        float enter = fmx(fmx(fmn(nr.x, fr.x), fmn(nr.y, fr.y)), fmn(nr.z, fr.z));
+        float exit = fmn(fmn(fmx(nr.x, fr.x), fmx(nr.y, fr.y)), fmx(nr.z, fr.z));
+        if (enter > exit) continue;
+        // Shadowed beyond the box either way: the change cannot matter.
+        if (trace_ray<NoTally, true>(scene, point + direction * (exit + 0.01f), direction, reach, tally).status != 0u) continue;
+        f3 start = point + direction * (enter - 0.01f);
+        RayResult hit = trace_ray<NoTally, true>(scene, start, -direction, reach, tally);
+        // A tube starting inside solid (the box rests on it) lands nowhere.
+        if (hit.status != 1u || dot3(hit.normal, direction) <= 0.0f || len3(hit.position - start) < 0.02f) continue;
+        // Reflected power through the tube: irradiance across its section
+        // (perpendicular to the light, so no cosine) times the albedo.
+        float4 md = optics.materials[hit.material * 2u];
+        flux = flux + dot3(color, w) * dot3(mk(md.x, md.y, md.z), w) * spacing * spacing;
+        uint32_t px = (uint32_t)(hit.vx + FOOTPRINT_BIAS), py = (uint32_t)(hit.vy + FOOTPRINT_BIAS), pz = (uint32_t)(hit.vz + FOOTPRINT_BIAS);
+        atomicMin(&sums[base], px);
+        atomicMin(&sums[base + 1u], py);
+        atomicMin(&sums[base + 2u], pz);
+        atomicMax(&sums[base + 3u], px);
+        atomicMax(&sums[base + 4u], py);
+        atomicMax(&sums[base + 5u], pz);
+    }
+    if (flux > 0.0f) atomicAdd(&sums[base + 6u], (uint32_t)(flux * FOOTPRINT_FLUX_SCALE + 0.5f));
+}
+
+// Each task's landing patch becomes its reserved Light event: the header's
+// union and its one bound. A task that landed nowhere keeps its empty bound,
+// out of every receiver's reach.
+__global__ void footprint_finish_kernel(const int4 *__restrict__ tasks, const uint32_t *__restrict__ sums, uint4 *__restrict__ events) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint32_t)tasks[0].x) return;
+    uint32_t base = e * 8u;
+    uint32_t x = sums[base];
+    if (x == NONE) return;
+    int mnx = (int)x - FOOTPRINT_BIAS, mny = (int)sums[base + 1u] - FOOTPRINT_BIAS, mnz = (int)sums[base + 2u] - FOOTPRINT_BIAS;
+    int mxx = (int)sums[base + 3u] - FOOTPRINT_BIAS, mxy = (int)sums[base + 4u] - FOOTPRINT_BIAS, mxz = (int)sums[base + 5u] - FOOTPRINT_BIAS;
+    float flux = (float)sums[base + 6u] / FOOTPRINT_FLUX_SCALE;
+    uint32_t header = (uint32_t)tasks[1u + e * 3u].w, bound = (uint32_t)tasks[1u + e * 3u + 1u].w;
+    events[header] = make_uint4((uint32_t)mnx, (uint32_t)mny, (uint32_t)mnz, events[header].w);
+    events[header + 1u] = make_uint4((uint32_t)mxx, (uint32_t)mxy, (uint32_t)mxz, events[header + 1u].w);
+    events[bound] = make_uint4((uint32_t)mnx, (uint32_t)mny, (uint32_t)mnz, __float_as_uint(flux));
+    events[bound + 1u] = make_uint4((uint32_t)mxx, (uint32_t)mxy, (uint32_t)mxz, 4u);
+}
+
 // ============================================================== launchers
 static inline uint32_t blocks(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
@@ -1038,6 +1139,13 @@ void launch_batch_frame(const BatchScene *scenes, uint32_t n_scenes, const Batch
 void launch_fp64_accumulate(const float4 *results, uint32_t count, double *sums, hipStream_t stream) {
     if (count == 0) return;
     hipLaunchKernelGGL(fp64_accumulate_kernel, dim3(blocks(count, 256)), dim3(256), 0, stream, results, count, sums);
+}
+
+void launch_footprints(const SceneView &scene, const Optics &optics, const int4 *tasks, uint32_t task_count, uint32_t threads,
+                       uint32_t *sums, uint4 *events, hipStream_t stream) {
+    if (task_count == 0u) return;
+    if (threads) hipLaunchKernelGGL(footprint_trace_kernel, dim3(blocks(threads, 256)), dim3(256), 0, stream, scene, optics, tasks, threads, sums);
+    hipLaunchKernelGGL(footprint_finish_kernel, dim3(blocks(task_count, 64)), dim3(64), 0, stream, tasks, sums, events);
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream) {

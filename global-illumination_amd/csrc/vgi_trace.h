@@ -31,6 +31,7 @@ struct Optics {
     const float4 *__restrict__ lights;      // 3 per light (gpu.rs:136-141)
     const uint4 *__restrict__ emitters;     // 2 uint4 per emissive face, then 4 per tree node
     uint32_t light_count;
+    uint32_t directional;                   // params.maximum.w: the first `directional` lights are suns
     uint32_t emitter_faces;                 // params.extra.w
     float total_weight;                     // params.extra.z
     uint32_t has_tree;
@@ -210,7 +211,10 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
 // `status` (0 escaped or beyond its distance, 1 hit, 2 unknown) and a hit's
 // material (NONE when the step limit ended the ray, else 0 for a miss). A hit
 // leaves the walk where it hit (w.position, w.normal).
-template <class Tally>
+// kVisible (voxel_traversal.wgsl:160-166, visible_geometry = true): unloaded
+// regions are walked through instead of ending the ray (change footprints
+// look for the real occluders beyond gaps in the loaded set).
+template <class Tally, bool kVisible = false>
 __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_t &status, uint32_t &hit_material, Tally &tally) {
     if (w.step >= 2048u) {
         status = 2u;
@@ -230,6 +234,15 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
         uint4 h = region_header(s, rx, ry, rz);
         tally.header();
         if (h.x == 0u) {
+            if constexpr (kVisible) {
+                f3 n;
+                float skip;
+                advance_box(position, direction, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, skip);
+                w.distance = w.distance + skip;
+                w.position = position + direction * skip;
+                w.normal = n;
+                return false;
+            }
             status = geometry_ahead(s, position, direction, w.max_distance - w.distance) ? 2u : 0u;
             return true;
         }
@@ -300,20 +313,20 @@ __device__ __forceinline__ void ray_record(const RayWalk &w, uint32_t status, ui
 }
 
 // One step of the walk. Returns true when the ray has its result.
-template <class Tally>
+template <class Tally, bool kVisible = false>
 __device__ __forceinline__ bool ray_advance(const SceneView &s, RayWalk &w, RayResult &result, Tally &tally) {
     uint32_t status, hit_material;
-    if (!ray_step(s, w, status, hit_material, tally)) return false;
+    if (!ray_step<Tally, kVisible>(s, w, status, hit_material, tally)) return false;
     ray_record(w, status, hit_material, result);
     return true;
 }
 
-template <class Tally>
+template <class Tally, bool kVisible = false>
 __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3 direction, float max_distance, Tally &tally) {
     RayResult result;
     RayWalk w;
     if (ray_start(s, origin, direction, max_distance, w, result, tally)) return result;
-    while (!ray_advance(s, w, result, tally)) {
+    while (!ray_advance<Tally, kVisible>(s, w, result, tally)) {
     }
     return result;
 }

@@ -171,6 +171,12 @@ int vgi_read_bound_tasks(vgi_tracer *tracer, vgi_face_task *out, uint32_t count)
  * finish_edits filled in measured flux (resident.rs:1326-1417 layout); returns
  * the word count, copies at most `capacity` words when out is not NULL. */
 int64_t vgi_read_events(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
+/* Change footprints (resident.rs:1215-1310, voxel_trace.wgsl:94-202): each
+ * edit's retention events carry one Light event per edited box (at most 64),
+ * whose bound and flux the GPU traces along the lights before the events
+ * apply. On by default; off plans the CPU's events alone (gpu.rs:419-425,
+ * without_footprints). */
+int vgi_set_footprints(vgi_tracer *tracer, uint32_t on);
 /* ---- GPU work selection (voxel_accumulate.wgsl:448-695; resident.rs:1093-1164) ---- */
 /* Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
  * stable_samples, audit_ms, mirror_budget}, work.rs:38-50; the plugin fills

@@ -1037,6 +1037,7 @@ struct Tracer {
     uint32_t batch_id = 0;
     I3 focus{0, 0, 0};
     uint32_t revision = 0, stable_samples = 0, audit_ms = 0, mirror_budget = 0;
+    bool footprints = true;  // gpu/resident.rs:100 trace_footprints
     float stable_error = 0.0f;
     std::vector<uint32_t> keys;
 };
@@ -2456,6 +2457,11 @@ int orc_cache_schedule(void *tp, const int32_t focus[3], uint32_t revision, floa
     return 0;
 }
 
+int orc_set_footprints(void *tp, uint32_t on) {
+    ((Tracer *)tp)->footprints = on != 0u;
+    return 0;
+}
+
 int orc_select(void *tp, uint32_t picks, uint32_t clock_ms, uint32_t *selection, uint32_t *mirrored, uint32_t capacity,
                uint32_t *pick_list) {
     Tracer &t = *(Tracer *)tp;
@@ -2542,6 +2548,51 @@ static float emission_lum(const Tracer &t, uint16_t m) {  // geometry.rs:92-100
 // (voxel_accumulate.wgsl:375-430) over this tracer's resident statistics.
 // Keeps the event words (one event) for orc_last_event and returns their
 // count, 0 when nothing within reach changed.
+// voxel_trace.wgsl:120-168 (trace_footprints): one tube of a changed box's
+// footprint grid, every light; widens the landing bounds (biased) and adds
+// the tube's reflected flux x 64 to sums.
+static void footprint_tube(const Tracer &t, I3 bmin, I3 bmax, uint32_t grid, uint32_t local, uint32_t *sums) {
+    F3 lo = f3((float)bmin.x, (float)bmin.y, (float)bmin.z);
+    F3 hi = f3((float)bmax.x, (float)bmax.y, (float)bmax.z) + f3(1.0f, 1.0f, 1.0f);
+    F3 center = (lo + hi) * 0.5f;
+    float diagonal = length3(hi - lo);
+    float spacing = diagonal / (float)grid;
+    float ox = ((float)(local % grid) + 0.5f) * spacing - 0.5f * diagonal;
+    float oy = ((float)(local / grid) + 0.5f) * spacing - 0.5f * diagonal;
+    const F3 w = f3(0.2126f, 0.7152f, 0.0722f);
+    const float reach = t.transport.max_distance;
+    const int32_t bias = 1073741824;
+    float flux = 0.0f;
+    // The directional lights (listed first; params.maximum.w of them).
+    for (uint32_t i = 0; i < std::min<uint32_t>(t.directional, (uint32_t)t.lights.size()); ++i) {
+        const Light &l = t.lights[i];
+        F3 color = f3(fmax_(l.energy[0] - l.extra[1], 0.0f), fmax_(l.energy[1] - l.extra[2], 0.0f), fmax_(l.energy[2] - l.extra[3], 0.0f));
+        if (color.x == 0.0f && color.y == 0.0f && color.z == 0.0f) continue;
+        F3 direction = normalize3(f3(l.pr[0], l.pr[1], l.pr[2]));
+        F3 side = std::fabs(direction.y) > 0.9f ? f3(1.0f, 0.0f, 0.0f) : f3(0.0f, 1.0f, 0.0f);
+        F3 u = normalize3(cross3(direction, side));
+        F3 v = cross3(direction, u);
+        F3 point = center + u * ox + v * oy;
+        F3 inverse = f3(1.0f / (std::fabs(direction.x) < 1e-8f ? 1e-8f : direction.x), 1.0f / (std::fabs(direction.y) < 1e-8f ? 1e-8f : direction.y),
+                        1.0f / (std::fabs(direction.z) < 1e-8f ? 1e-8f : direction.z));
+        F3 nr = (lo - point) * inverse, fr = (hi - point) * inverse;
+        float enter = fmax_(fmax_(fmin_(nr.x, fr.x), fmin_(nr.y, fr.y)), fmin_(nr.z, fr.z));
+        float exit = fmin_(fmin_(fmax_(nr.x, fr.x), fmax_(nr.y, fr.y)), fmax_(nr.z, fr.z));
+        if (enter > exit) continue;
+        if (trace_domain_limited(t.packed, point + direction * (exit + 0.01f), direction, reach, true, 2048u, nullptr).status != 0u) continue;
+        F3 start = point + direction * (enter - 0.01f);
+        RayResult hit = trace_domain_limited(t.packed, start, -direction, reach, true, 2048u, nullptr);
+        if (hit.status != 1u || dot3(hit.normal, direction) <= 0.0f || length3(hit.position - start) < 0.02f) continue;
+        flux = flux + dot3(color, w) * dot3(t.materials[hit.material].diffuse, w) * spacing * spacing;
+        uint32_t p[3] = {(uint32_t)(hit.voxel.x + bias), (uint32_t)(hit.voxel.y + bias), (uint32_t)(hit.voxel.z + bias)};
+        for (int a = 0; a < 3; ++a) {
+            sums[a] = std::min(sums[a], p[a]);
+            sums[3 + a] = std::max(sums[3 + a], p[a]);
+        }
+    }
+    if (flux > 0.0f) sums[6] += (uint32_t)(flux * 64.0f + 0.5f);
+}
+
 int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     Tracer &t = *(Tracer *)tp;
     t.last_event.clear();
@@ -2646,7 +2697,18 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
         if (positions <= budget) { budget -= positions; c.measure = true; }
         else { c.kind = 0; c.flux = 0.0f; }
     }
-    // encode_events, one event.
+    // Change footprints (gpu/resident.rs:1215-1270): one Light event per
+    // edited box (edits first, then boxes of unknown flux; at most 64), its
+    // bound empty until traced below.
+    std::vector<size_t> footprint_of;
+    for (int pass = 0; pass < (t.footprints ? 2 : 0); ++pass)
+        for (size_t i = 0; i < nearby.size(); ++i) {
+            bool edit = nearby[i].kind != 0;
+            if ((pass == 0) == edit && footprint_of.size() < 64) footprint_of.push_back(i);
+        }
+    const uint32_t n_events = 1u + (uint32_t)footprint_of.size();
+    const int32_t far_lo = 1 << 29, far_hi = -(1 << 29);
+    // encode_events: the edit's event, then the footprints' (headers, then bounds).
     std::vector<uint32_t> w;
     auto push = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) { w.push_back(a); w.push_back(b); w.push_back(c); w.push_back(d); };
     I3 umin = nearby[0].mn, umax = nearby[0].mx;
@@ -2657,9 +2719,14 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
     push((uint32_t)umin.x, (uint32_t)umin.y, (uint32_t)umin.z, 0u);
     push((uint32_t)umax.x, (uint32_t)umax.y, (uint32_t)umax.z, 0u);
     push(0u, 0u, 0u, (uint32_t)nearby.size());
+    for (size_t k = 0; k < footprint_of.size(); ++k) {
+        push((uint32_t)far_lo, (uint32_t)far_lo, (uint32_t)far_lo, 0u);
+        push((uint32_t)far_hi, (uint32_t)far_hi, (uint32_t)far_hi, 0u);
+        push(0u, 0u, (uint32_t)(nearby.size() + k), 1u);
+    }
     struct Measure { uint32_t record; I3 mn; uint32_t sx, sy, sz, voxels; float flux; };
     std::vector<Measure> measures;
-    uint32_t record = 3;
+    uint32_t record = n_events * 3u;
     for (const OrcChange &c : nearby) {
         bool urgent = false;
         for (const I3 &v : edits)
@@ -2671,6 +2738,32 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
         push((uint32_t)c.mn.x, (uint32_t)c.mn.y, (uint32_t)c.mn.z, fbits(c.flux));
         push((uint32_t)c.mx.x, (uint32_t)c.mx.y, (uint32_t)c.mx.z, kind | (urgent ? 8u : 0u));
         record += 2;
+    }
+    for (size_t k = 0; k < footprint_of.size(); ++k) {
+        push((uint32_t)far_lo, (uint32_t)far_lo, (uint32_t)far_lo, fbits(0.0f));
+        push((uint32_t)far_hi, (uint32_t)far_hi, (uint32_t)far_hi, 4u);
+    }
+    // trace_footprints + finish_footprints (voxel_trace.wgsl:94-202) in the
+    // new scene.
+    tracer_sync(t);
+    for (size_t k = 0; k < footprint_of.size(); ++k) {
+        const OrcChange &c = nearby[footprint_of[k]];
+        float dx = (float)(c.mx.x - c.mn.x + 1), dy = (float)(c.mx.y - c.mn.y + 1), dz = (float)(c.mx.z - c.mn.z + 1);
+        uint32_t grid = (uint32_t)std::min(std::max(std::ceil(std::sqrt(dx * dx + dy * dy + dz * dz)), 1.0f), 64.0f);
+        uint32_t sums[7] = {NONE, NONE, NONE, 0u, 0u, 0u, 0u};
+        for (uint32_t local = 0; local < grid * grid; ++local)
+            footprint_tube(t, c.mn, c.mx, grid, local, sums);
+        if (sums[0] == NONE) continue;
+        const int32_t bias = 1073741824;
+        I3 mn{(int32_t)sums[0] - bias, (int32_t)sums[1] - bias, (int32_t)sums[2] - bias};
+        I3 mx{(int32_t)sums[3] - bias, (int32_t)sums[4] - bias, (int32_t)sums[5] - bias};
+        float flux = (float)sums[6] / 64.0f;
+        size_t header = (1 + k) * 3, bound = (size_t)n_events * 3 + (nearby.size() + k) * 2;
+        uint32_t *h0 = &w[header * 4], *h1 = &w[(header + 1) * 4], *b0 = &w[bound * 4], *b1 = &w[(bound + 1) * 4];
+        h0[0] = (uint32_t)mn.x; h0[1] = (uint32_t)mn.y; h0[2] = (uint32_t)mn.z;
+        h1[0] = (uint32_t)mx.x; h1[1] = (uint32_t)mx.y; h1[2] = (uint32_t)mx.z;
+        b0[0] = (uint32_t)mn.x; b0[1] = (uint32_t)mn.y; b0[2] = (uint32_t)mn.z; b0[3] = fbits(flux);
+        b1[0] = (uint32_t)mx.x; b1[1] = (uint32_t)mx.y; b1[2] = (uint32_t)mx.z; b1[3] = 4u;
     }
     // measure_edits + finish_edits from this tracer's statistics.
     float light[256][2];

@@ -122,6 +122,9 @@ int orc_cache_state(void *tracer, uint32_t first, uint32_t count, uint32_t *out)
 int orc_cache_meta(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
 int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_count,
                            uint32_t words);
+/* Change footprints on (default) or off for edit_event (gpu.rs:419-425,
+ * without_footprints: tests of the CPU-planned events alone). */
+int orc_set_footprints(void *tracer, uint32_t on);
 /* ---- GPU work selection (voxel_accumulate.wgsl:99-130, 448-695) ---- */
 /* Scheduling settings record_stability and classification read. */
 int orc_cache_schedule(void *tracer, const int32_t focus[3], uint32_t revision, float stable_error, uint32_t stable_samples,

@@ -78,6 +78,7 @@ def lib():
         "orc_last_event": (i64, [P, P, i64]),
         "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_schedule": (ctypes.c_int, [P, P, u32, f32, u32, u32, u32]),
+        "orc_set_footprints": (ctypes.c_int, [P, u32]),
         "orc_select": (ctypes.c_int, [P, u32, u32, P, P, u32, P]),
         "orc_cache_keys": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_picks": (ctypes.c_int, [P, P, u32, P, ctypes.c_int, u32]),
@@ -257,6 +258,9 @@ class Tracer:
         f = np.asarray(focus, np.int32)
         lib().orc_cache_schedule(self.h, ptr(f), transport_revision, stable_error, stable_samples, audit_ms, mirror_budget)
 
+    def set_footprints(self, on):
+        lib().orc_set_footprints(self.h, 1 if on else 0)
+
     def select(self, picks, clock_ms=0):
         """One pick batch's selection passes: (1038 scratch words, mirrored
         slots, pick list)."""
@@ -282,6 +286,16 @@ class Tracer:
     def cache_picks(self, pick_slots, threads=8, clock_ms=0):
         pick_slots = np.ascontiguousarray(pick_slots, np.uint32)
         return lib().orc_cache_picks(self.h, ptr(pick_slots), len(pick_slots), None, threads, clock_ms)
+
+
+def event_count(words):
+    """Events in an encoding of one edit's event followed by its change
+    footprints' (one box each): records = 3 E + 2 (boxes + E - 1)."""
+    if len(words) == 0:
+        return 0
+    w = np.asarray(words, np.uint32).reshape(-1, 4)
+    boxes = int(w[2, 3])
+    return (len(w) - 2 * boxes + 2) // 5
 
 
 def select_log2(x):

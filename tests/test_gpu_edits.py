@@ -40,7 +40,7 @@ def test_edit_stream_bit_exact(vgi, orc, edits_per_frame):
             g.edit_voxels(xyz, values)
             words = t.edit_event(xyz, values)
             if len(words):
-                t.apply_events(words, 1)
+                t.apply_events(words, orc.event_count(words))
             grid[xyz[:, 2], xyz[:, 1], xyz[:, 0]] = values  # later edits of a voxel win, as in the library
         tasks = g.bound_tasks()
         assert face_keys(tasks) == face_keys(vgi.exposed_faces(grid)), "bound faces differ from the exposed surface"
@@ -54,7 +54,7 @@ def test_edit_stream_bit_exact(vgi, orc, edits_per_frame):
         if frame > 0:
             ev = g.events()
             assert np.array_equal(ev, words), "event words differ (%d vs %d words)" % (len(ev), len(words))
-            kinds = words.reshape(-1, 4)[3::2, 3] & 7 if len(words) else np.zeros(0)
+            kinds = words.reshape(-1, 4)[3 * orc.event_count(words)::2, 3] & 7 if len(words) else np.zeros(0)
             measured += int((kinds == 3).sum())
         cpu, ch = t.cache_batch(tasks, cpu_slots, hits=True, clock_ms=16 * frame)
         bad = np.nonzero(np.any(gh != ch, axis=1))[0]

@@ -5,7 +5,8 @@ against the reference's own tests of that pipeline.
 Event words (gpu/resident.rs:1319-1417): three uint4 headers per event
 (union min, union max, (0, 0, first box, boxes)), then two uint4 per box
 ((min, flux bits), (max, kind | urgent << 3)); kinds 0 unknown flux, 3 edit,
-6 edit to be measured.
+4 light (a change footprint), 6 edit to be measured. The edit's event comes
+first, then one footprint event per edited box (gpu/resident.rs:1215-1270).
 """
 import math
 
@@ -55,12 +56,14 @@ def floor_scene(edge=64, emitters=()):
 
 
 def boxes(words):
-    """(min, max, flux, kind, urgent) of each box of a one-event encoding."""
+    """(min, max, flux, kind, urgent) of each box of the edit's event (the
+    first of the encoding; the rest are its change footprints)."""
     w = words.reshape(-1, 4)
     n = int(w[2, 3])
+    first = 3 * orc.event_count(words)
     out = []
     for b in range(n):
-        lo, hi = w[3 + 2 * b], w[4 + 2 * b]
+        lo, hi = w[first + 2 * b], w[first + 1 + 2 * b]
         out.append((tuple(lo[:3].view(np.int32)), tuple(hi[:3].view(np.int32)), lo[3:4].view(np.float32)[0], int(hi[3] & 7),
                     bool(hi[3] & 8)))
     return w, out
@@ -74,7 +77,7 @@ def test_single_voxel_edit_is_one_urgent_box():
     _, _, tracer, _, _ = floor_scene()
     words = tracer.edit_event([[20, 4, 20]], [2])
     w, b = boxes(words)
-    assert len(words) == 4 * (3 + 2)
+    assert len(words) == 4 * (2 * 3 + 2 * 2)  # the edit's event and its footprint's
     assert tuple(w[0, :3].view(np.int32)) == (20, 4, 20) and tuple(w[1, :3].view(np.int32)) == (20, 4, 20)
     assert b == [((20, 4, 20), (20, 4, 20), f32(0.0), 0, True)]
 
@@ -141,8 +144,9 @@ def test_reach_filter_keeps_changes_near_solid_pages():
     assert len(tracer.edit_event([[224, 31, 31]], [1])) == 0
     assert len(tracer.edit_event([[250, 250, 250]], [1])) == 0
     # (223,31,31): exactly 192 from the page, still in reach; (100,10,10): 69.
-    assert len(tracer.edit_event([[223, 31, 31]], [1])) == 4 * 5
-    assert len(tracer.edit_event([[100, 10, 10]], [1])) == 4 * 5
+    # Kept: the edit's event (three headers, one box) and its footprint's.
+    assert len(tracer.edit_event([[223, 31, 31]], [1])) == 4 * 10
+    assert len(tracer.edit_event([[100, 10, 10]], [1])) == 4 * 10
 
 
 def test_measured_flux_and_urgency_follow_the_reference_statement():
@@ -153,6 +157,7 @@ def test_measured_flux_and_urgency_follow_the_reference_statement():
     face within one voxel; retention then marks urgent exactly the faces
     within 64 voxels whose history the change binds."""
     grid, scene, tracer, diffuse, emission = floor_scene()
+    tracer.set_footprints(False)  # as the reference's test: the CPU-planned event alone
     tracer.set_transport(bounces=2, samples=16, environment=(0.4, 0.45, 0.5))  # a sky: faces are lit
     faces = []
     for z in range(0, 32):
