@@ -131,6 +131,11 @@ struct vgi_tracer {
     bool footprints = true;  // change footprints traced with each edit's events
     DeviceBuffer d_keys, d_selection, d_pick_tasks, d_pick_slots, d_pick_results, d_mirror, d_select_ranges;
     DeviceBuffer d_footprint_tasks, d_footprint_sums;
+    // Sun corner words: one per resident slot (0 until traced), the last
+    // call's flip region, and the flips binned until they publish.
+    DeviceBuffer d_corners, d_sun_flux, d_sun_ranges, d_sun_region;
+    std::vector<uint32_t> sun_region;
+    vgi_host::SunChanges sun_changes;
     uint32_t picks = 0, mirror_budget = 0;
     hipEvent_t sel_ev[4] = {};
     bool select_recorded = false;
@@ -446,6 +451,30 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
 // exposed surface are added and removed.
 int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, const std::vector<vgi_host::RegionDelta> &deltas);
 
+// The solid regions of a scene as retention events need them: their bounding
+// box and which 32^3 pages hold one (`solid`: one byte per region of the box).
+vgi_host::PreviousScene scene_pages(const vgi_tracer &t, const std::vector<uint8_t> &solid, std::vector<uint8_t> &page_solid) {
+    vgi_host::PreviousScene prev;
+    prev.any_solid = false;
+    prev.solid_min = Int3{INT32_MAX, INT32_MAX, INT32_MAX};
+    prev.solid_max = Int3{INT32_MIN, INT32_MIN, INT32_MIN};
+    prev.page_min = Int3{t.rmin.x / 2, t.rmin.y / 2, t.rmin.z / 2};
+    prev.psx = t.rsx / 2; prev.psy = t.rsy / 2; prev.psz = t.rsz / 2;
+    page_solid.assign((size_t)prev.psx * prev.psy * prev.psz, 0);
+    for (int z = 0; z < t.rsz; ++z)
+        for (int y = 0; y < t.rsy; ++y)
+            for (int x = 0; x < t.rsx; ++x) {
+                if (!solid[lin3(x, y, z, t.rsx, t.rsy)]) continue;
+                Int3 r{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z};
+                prev.any_solid = true;
+                prev.solid_min = Int3{std::min(prev.solid_min.x, r.x), std::min(prev.solid_min.y, r.y), std::min(prev.solid_min.z, r.z)};
+                prev.solid_max = Int3{std::max(prev.solid_max.x, r.x), std::max(prev.solid_max.y, r.y), std::max(prev.solid_max.z, r.z)};
+                page_solid[lin3(x / 2, y / 2, z / 2, prev.psx, prev.psy)] = 1;
+            }
+    prev.page_solid = page_solid.data();
+    return prev;
+}
+
 int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     if (!t.have_regions) return 0;
     using clock = std::chrono::steady_clock;
@@ -580,24 +609,8 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
             if (d.before[i] != d.after[i])
                 changed.push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
     }
-    vgi_host::PreviousScene prev;
-    prev.any_solid = false;
-    prev.solid_min = Int3{INT32_MAX, INT32_MAX, INT32_MAX};
-    prev.solid_max = Int3{INT32_MIN, INT32_MIN, INT32_MIN};
-    prev.page_min = Int3{t.rmin.x / 2, t.rmin.y / 2, t.rmin.z / 2};
-    prev.psx = t.rsx / 2; prev.psy = t.rsy / 2; prev.psz = t.rsz / 2;
-    std::vector<uint8_t> page_solid((size_t)prev.psx * prev.psy * prev.psz, 0);
-    for (int z = 0; z < t.rsz; ++z)
-        for (int y = 0; y < t.rsy; ++y)
-            for (int x = 0; x < t.rsx; ++x) {
-                if (!solid_before[lin3(x, y, z, t.rsx, t.rsy)]) continue;
-                Int3 r{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z};
-                prev.any_solid = true;
-                prev.solid_min = Int3{std::min(prev.solid_min.x, r.x), std::min(prev.solid_min.y, r.y), std::min(prev.solid_min.z, r.z)};
-                prev.solid_max = Int3{std::max(prev.solid_max.x, r.x), std::max(prev.solid_max.y, r.y), std::max(prev.solid_max.z, r.z)};
-                page_solid[lin3(x / 2, y / 2, z / 2, prev.psx, prev.psy)] = 1;
-            }
-    prev.page_solid = page_solid.data();
+    std::vector<uint8_t> page_solid;
+    vgi_host::PreviousScene prev = scene_pages(t, solid_before, page_solid);
     vgi_host::RetentionEvent ev;
     if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev)) t.pending_events.push_back(std::move(ev));
     t.timings[6] = ms_since(t_event);
@@ -732,9 +745,14 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         HIP_OK(nk.reserve(cap * 4));
         HIP_OK(hipMemsetAsync(nk.ptr, 0xff, cap * 4, t.stream));
         if (t.slot_capacity) HIP_OK(hipMemcpyAsync(nk.ptr, t.d_keys.ptr, t.slot_capacity * 4, hipMemcpyDeviceToDevice, t.stream));
+        // Sun corner words persist; new slots have none (0).
+        DeviceBuffer nw;
+        HIP_OK(nw.reserve(cap * 4));
+        HIP_OK(hipMemsetAsync(nw.ptr, 0, cap * 4, t.stream));
+        if (t.slot_capacity) HIP_OK(hipMemcpyAsync(nw.ptr, t.d_corners.ptr, t.slot_capacity * 4, hipMemcpyDeviceToDevice, t.stream));
         HIP_OK(hipStreamSynchronize(t.stream));
-        t.d_state.release(); t.d_checkpoint.release(); t.d_meta.release(); t.d_keys.release();
-        t.d_state = ns; t.d_checkpoint = nc; t.d_meta = nm; t.d_keys = nk;
+        t.d_state.release(); t.d_checkpoint.release(); t.d_meta.release(); t.d_keys.release(); t.d_corners.release();
+        t.d_state = ns; t.d_checkpoint = nc; t.d_meta = nm; t.d_keys = nk; t.d_corners = nw;
         t.slot_capacity = cap;
     }
     // Seed new slots (resident.rs:1426-1470 from an empty FaceWork: all zero)
@@ -752,6 +770,7 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         if (n * 5 != identities.size()) return fail("resident slots were not allocated as one range");
         HIP_OK(hipMemsetAsync(t.d_state.as<uint32_t>() + (size_t)lo * 36, 0, n * 144, t.stream));
         HIP_OK(hipMemsetAsync(t.d_checkpoint.as<uint32_t>() + (size_t)lo * 12, 0, n * 48, t.stream));
+        HIP_OK(hipMemsetAsync(t.d_corners.as<uint32_t>() + lo, 0, n * 4, t.stream));
         HIP_OK(hipMemcpyAsync(t.d_meta.as<uint32_t>() + (size_t)lo * 12, meta.data(), n * 48, hipMemcpyHostToDevice, t.stream));
         HIP_OK(hipStreamSynchronize(t.stream));
     }
@@ -1245,7 +1264,8 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
                             &t->d_pick_results, &t->d_mirror, &t->d_select_ranges,
-                            &t->d_footprint_tasks, &t->d_footprint_sums};
+                            &t->d_footprint_tasks, &t->d_footprint_sums, &t->d_corners, &t->d_sun_flux,
+                            &t->d_sun_ranges, &t->d_sun_region};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
@@ -1812,6 +1832,58 @@ int vgi_apply_events(vgi_tracer *t, const uint32_t *events, uint32_t event_count
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(t->stream));
     ev.release();
+    return 0;
+}
+
+// Sun corner words of the bound faces (voxel_view_runtime.wgsl:168-268 for
+// the world's faces): refresh, flips in bound order, then the host's
+// SunChanges (sun_changes.rs) and, when due, a Light retention event applied
+// with the next batch.
+int vgi_sun_corners(vgi_tracer *t, uint32_t frame, uint32_t refresh_frames) {
+    if (!t) return fail("null tracer");
+    if (!t->resident) return fail("sun corner words are kept per resident slot: call vgi_enable_resident first");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    uint32_t count = t->bound;
+    if (count && t->b_slots.bytes < (size_t)count * 4) return fail("bound faces have no resident slots: bind them with the cache on");
+    HIP_OK(t->d_sun_flux.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
+    HIP_OK(t->d_sun_ranges.reserve((size_t)2 * vgi::SUN_RANGES * 4));
+    HIP_OK(t->d_sun_region.reserve((size_t)vgi::SUN_FLIP_WORDS * 4));
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    vgi::launch_sun_corners(s, o, t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_meta.as<uint32_t>(),
+                            t->d_corners.as<uint32_t>(), t->d_sun_flux.as<float>(), frame, refresh_frames,
+                            t->d_sun_ranges.as<uint32_t>(), t->d_sun_region.as<uint32_t>(), t->stream);
+    HIP_OK(hipGetLastError());
+    t->sun_region.resize(vgi::SUN_FLIP_WORDS);
+    HIP_OK(hipMemcpyAsync(t->sun_region.data(), t->d_sun_region.ptr, (size_t)vgi::SUN_FLIP_WORDS * 4, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    std::vector<vgi_host::SunFlip> flips;
+    vgi_host::decode_sun_flips(t->sun_region.data(), vgi::SUN_FLIP_RECORDS, flips);
+    t->sun_changes.add(flips);
+    std::vector<vgi_host::SunFlip> due;
+    if (t->sun_changes.take(clock_ms(*t), due)) {
+        std::vector<uint8_t> page_solid;
+        vgi_host::PreviousScene scene = scene_pages(*t, t->solid, page_solid);
+        vgi_host::RetentionEvent ev;
+        if (vgi_host::light_event(due, scene, ev)) t->pending_events.push_back(std::move(ev));
+    }
+    return 0;
+}
+
+int64_t vgi_read_sun_flips(vgi_tracer *t, uint32_t *out, uint32_t capacity) {
+    if (!t) return fail("null tracer");
+    if (out) std::memcpy(out, t->sun_region.data(), (size_t)std::min<size_t>(capacity, t->sun_region.size()) * 4);
+    return (int64_t)t->sun_region.size();
+}
+
+int vgi_read_sun_corners(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t *out) {
+    if (!t || (count && !out)) return fail("null argument");
+    if ((size_t)first + count > t->high_water) return fail("slot range beyond the resident slots");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (count) HIP_OK(hipMemcpy(out, t->d_corners.as<uint32_t>() + first, (size_t)count * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -443,6 +443,35 @@ bool PreviousScene::page_occupied(Int3 page) const {
     return page_solid[(size_t)x + (size_t)psx * ((size_t)y + (size_t)psy * (size_t)z)] != 0;
 }
 
+namespace {
+// Whether a box lies within the influence radius of a page holding a solid
+// region (from_local_bounds and from_resident_bounds select the same set).
+bool within_reach(Int3 bmin, Int3 bmax, const PreviousScene &previous) {
+    const int64_t r2 = (int64_t)INFLUENCE_RADIUS * INFLUENCE_RADIUS;
+    int64_t lo[3] = {floor_div64((int64_t)bmin.x - INFLUENCE_RADIUS, 32), floor_div64((int64_t)bmin.y - INFLUENCE_RADIUS, 32),
+                     floor_div64((int64_t)bmin.z - INFLUENCE_RADIUS, 32)};
+    int64_t hi[3] = {floor_div64((int64_t)bmax.x + INFLUENCE_RADIUS, 32), floor_div64((int64_t)bmax.y + INFLUENCE_RADIUS, 32),
+                     floor_div64((int64_t)bmax.z + INFLUENCE_RADIUS, 32)};
+    // Clip the page range to the pages the scene holds.
+    lo[0] = std::max<int64_t>(lo[0], previous.page_min.x); hi[0] = std::min<int64_t>(hi[0], previous.page_min.x + previous.psx - 1);
+    lo[1] = std::max<int64_t>(lo[1], previous.page_min.y); hi[1] = std::min<int64_t>(hi[1], previous.page_min.y + previous.psy - 1);
+    lo[2] = std::max<int64_t>(lo[2], previous.page_min.z); hi[2] = std::min<int64_t>(hi[2], previous.page_min.z + previous.psz - 1);
+    // The page holding the box's minimum first: in a populated world it
+    // usually decides at once.
+    if (previous.page_occupied(Int3{(int32_t)floor_div64(bmin.x, 32), (int32_t)floor_div64(bmin.y, 32), (int32_t)floor_div64(bmin.z, 32)}))
+        return true;
+    for (int64_t z = lo[2]; z <= hi[2]; ++z)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+            for (int64_t x = lo[0]; x <= hi[0]; ++x) {
+                Int3 page{(int32_t)x, (int32_t)y, (int32_t)z};
+                Int3 pmin{page.x * 32, page.y * 32, page.z * 32}, pmax{pmin.x + 31, pmin.y + 31, pmin.z + 31};
+                if (box_distance2(bmin, bmax, pmin, pmax) > r2) continue;
+                if (previous.page_occupied(page)) return true;
+            }
+    return false;
+}
+}  // namespace
+
 bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
                 const std::vector<Int3> &edits, RetentionEvent &out) {
     out.boxes.clear();
@@ -487,30 +516,8 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
     // Only changes within the influence radius of a page that held a solid
     // region can touch an existing receiver (from_local_bounds and
     // from_resident_bounds select the same set).
-    const int64_t r2 = (int64_t)INFLUENCE_RADIUS * INFLUENCE_RADIUS;
-    for (const ChangeBox &b : boxes) {
-        int64_t lo[3] = {floor_div64((int64_t)b.min.x - INFLUENCE_RADIUS, 32), floor_div64((int64_t)b.min.y - INFLUENCE_RADIUS, 32),
-                         floor_div64((int64_t)b.min.z - INFLUENCE_RADIUS, 32)};
-        int64_t hi[3] = {floor_div64((int64_t)b.max.x + INFLUENCE_RADIUS, 32), floor_div64((int64_t)b.max.y + INFLUENCE_RADIUS, 32),
-                         floor_div64((int64_t)b.max.z + INFLUENCE_RADIUS, 32)};
-        // Clip the page range to the pages the previous scene holds.
-        lo[0] = std::max<int64_t>(lo[0], previous.page_min.x); hi[0] = std::min<int64_t>(hi[0], previous.page_min.x + previous.psx - 1);
-        lo[1] = std::max<int64_t>(lo[1], previous.page_min.y); hi[1] = std::min<int64_t>(hi[1], previous.page_min.y + previous.psy - 1);
-        lo[2] = std::max<int64_t>(lo[2], previous.page_min.z); hi[2] = std::min<int64_t>(hi[2], previous.page_min.z + previous.psz - 1);
-        // The page holding the box first (a box lies inside one region): in
-        // a populated world it usually decides at once.
-        bool near = previous.page_occupied(Int3{(int32_t)floor_div64(b.min.x, 32), (int32_t)floor_div64(b.min.y, 32),
-                                                (int32_t)floor_div64(b.min.z, 32)});
-        for (int64_t z = lo[2]; z <= hi[2] && !near; ++z)
-            for (int64_t y = lo[1]; y <= hi[1] && !near; ++y)
-                for (int64_t x = lo[0]; x <= hi[0] && !near; ++x) {
-                    Int3 page{(int32_t)x, (int32_t)y, (int32_t)z};
-                    Int3 pmin{page.x * 32, page.y * 32, page.z * 32}, pmax{pmin.x + 31, pmin.y + 31, pmin.z + 31};
-                    if (box_distance2(b.min, b.max, pmin, pmax) > r2) continue;
-                    near = previous.page_occupied(page);
-                }
-        if (near) out.boxes.push_back(b);
-    }
+    for (const ChangeBox &b : boxes)
+        if (within_reach(b.min, b.max, previous)) out.boxes.push_back(b);
     if (out.boxes.empty()) return false;
     // Deterministic order and no duplicates (retention.rs:114-125). Two
     // regions never share a tight box, so the weights stay one per box.
@@ -604,6 +611,65 @@ void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out
         }
 }
 
+
+void decode_sun_flips(const uint32_t *region, uint32_t records, std::vector<SunFlip> &out) {
+    uint32_t reported = region[0];
+    for (uint32_t k = 0; k < std::min(reported, records); ++k) {
+        const uint32_t *r = region + 4u + 4u * k;
+        Int3 v{(int32_t)r[0], (int32_t)r[1], (int32_t)r[2]};
+        float flux;
+        std::memcpy(&flux, &r[3], 4);
+        out.push_back(SunFlip{v, v, flux});
+    }
+    if (reported > records) {
+        const uint32_t *o = region + 4u + 4u * records;
+        const int64_t bias = (int64_t)1 << 30;
+        Int3 mn{(int32_t)(bias - (int64_t)o[0]), (int32_t)(bias - (int64_t)o[1]), (int32_t)(bias - (int64_t)o[2])};
+        Int3 mx{(int32_t)((int64_t)o[3] - bias), (int32_t)((int64_t)o[4] - bias), (int32_t)((int64_t)o[5] - bias)};
+        out.push_back(SunFlip{mn, mx, (float)o[6] / 64.0f});
+    }
+}
+
+void SunChanges::add(const std::vector<SunFlip> &flips) {
+    for (const SunFlip &f : flips) {
+        auto key = std::make_tuple((int)floor_div64(f.min.z, 16), (int)floor_div64(f.min.y, 16), (int)floor_div64(f.min.x, 16));
+        auto it = buckets.find(key);
+        if (it == buckets.end()) {
+            buckets.emplace(key, f);
+            continue;
+        }
+        SunFlip &b = it->second;
+        b.min = Int3{std::min(b.min.x, f.min.x), std::min(b.min.y, f.min.y), std::min(b.min.z, f.min.z)};
+        b.max = Int3{std::max(b.max.x, f.max.x), std::max(b.max.y, f.max.y), std::max(b.max.z, f.max.z)};
+        b.flux += f.flux;
+    }
+}
+
+bool SunChanges::take(uint32_t now_ms, std::vector<SunFlip> &out) {
+    out.clear();
+    if (buckets.empty() || (published && now_ms - published_ms < 250u)) return false;
+    published = true;
+    published_ms = now_ms;
+    for (auto &kv : buckets) out.push_back(kv.second);
+    buckets.clear();
+    return true;
+}
+
+bool light_event(const std::vector<SunFlip> &flips, const PreviousScene &scene, RetentionEvent &out) {
+    out.boxes.clear();
+    for (const SunFlip &f : flips)
+        if (within_reach(f.min, f.max, scene)) out.boxes.push_back(ChangeBox{f.min, f.max, f.flux, 0u, 4u, false});
+    if (out.boxes.empty()) return false;
+    std::sort(out.boxes.begin(), out.boxes.end(), box_less);
+    out.boxes.erase(std::unique(out.boxes.begin(), out.boxes.end(), box_equal), out.boxes.end());
+    out.union_min = out.boxes[0].min;
+    out.union_max = out.boxes[0].max;
+    for (const ChangeBox &b : out.boxes) {
+        out.union_min = Int3{std::min(out.union_min.x, b.min.x), std::min(out.union_min.y, b.min.y), std::min(out.union_min.z, b.min.z)};
+        out.union_max = Int3{std::max(out.union_max.x, b.max.x), std::max(out.union_max.y, b.max.y), std::max(out.union_max.z, b.max.z)};
+    }
+    return true;
+}
 
 std::vector<Footprint> plan_footprints(std::vector<RetentionEvent> &events) {
     std::vector<Footprint> boxes;

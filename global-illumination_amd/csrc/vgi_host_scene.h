@@ -9,6 +9,8 @@
 #include <array>
 #include <cstddef>
 #include <deque>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace vgi_host {
@@ -150,6 +152,31 @@ struct Footprint {
 std::vector<Footprint> plan_footprints(std::vector<RetentionEvent> &events);
 uint32_t encode_footprints(const std::vector<Footprint> &footprints, const std::vector<RetentionEvent> &events,
                            std::vector<int32_t> &words);
+
+// Sun flips (voxel_view_runtime.wgsl:222-268 report_sun_flips): boxes whose
+// corner sunlight changed, with the light reflected there.
+struct SunFlip {
+    Int3 min, max;
+    float flux;
+};
+// decode_sun_flips (gpu/view/timing.rs:22-49): a flip region (a count, up to
+// `records` (voxel, flux bits) records from word 4, then the overflow box as
+// biased maxima and its flux x 64 at word 4 + 4 records) as boxes.
+void decode_sun_flips(const uint32_t *region, uint32_t records, std::vector<SunFlip> &out);
+// SunChanges (sun_changes.rs:18-69): flips binned per 16^3 region of their
+// minimum corner (bounds widened, flux summed in arrival order), published
+// together at most every 250 ms of the tracer clock.
+struct SunChanges {
+    std::map<std::tuple<int, int, int>, SunFlip> buckets;  // (z, y, x) region
+    bool published = false;
+    uint32_t published_ms = 0;
+    void add(const std::vector<SunFlip> &flips);
+    bool take(uint32_t now_ms, std::vector<SunFlip> &out);
+};
+// The published flips as a Light retention event (GeometryChange::from_bounds
+// over `scene`'s solid pages, then SpatialHistory::event): boxes within reach,
+// kind 4, sorted; false when none is in reach.
+bool light_event(const std::vector<SunFlip> &flips, const PreviousScene &scene, RetentionEvent &out);
 
 constexpr int INFLUENCE_RADIUS = 192;     // retention.rs:32-39 at HISTORY_LIMIT 65536
 constexpr uint32_t FLUX_SCAN_BUDGET = 16384;  // publication.rs:413

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "vgi_blockscan.h"
 #include "vgi_kernels.h"
 #include "vgi_math.h"
 
@@ -100,36 +101,6 @@ __device__ __forceinline__ uint32_t candidate_key(const uint32_t (&m)[12], const
     }
     uint32_t level = min((uint32_t)(log2_own((float)age_ms + 1.0f) * 8.0f), 255u);
     return (2u << 8) | (255u - level);
-}
-
-// Workgroup-wide exclusive ranks of up to 8 flags (ballots per wavefront,
-// wavefront totals through LDS): rank[k] for this thread, total[k] for the
-// workgroup. blockDim.x == 256.
-template <int K>
-__device__ __forceinline__ void block_ranks(const bool (&flag)[K], uint32_t (&rank)[K], uint32_t (&total)[K],
-                                            uint32_t (*wave_totals)[4]) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t local[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        unsigned long long b = __ballot(flag[k]);
-        local[k] = (uint32_t)__popcll(b & below);
-        if (lane == 0u) wave_totals[k][wave] = (uint32_t)__popcll(b);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        uint32_t before = 0u, sum = 0u;
-        for (uint32_t w = 0; w < 4u; ++w) {
-            uint32_t n = wave_totals[k][w];
-            before += w < wave ? n : 0u;
-            sum += n;
-        }
-        rank[k] = before + local[k];
-        total[k] = sum;
-    }
-    __syncthreads();
 }
 
 __global__ void select_clear_kernel(uint32_t *__restrict__ selection, uint4 *__restrict__ pick_tasks,
@@ -300,34 +271,6 @@ __global__ __launch_bounds__(256) void select_count_kernel(const uint32_t *__res
     if (threadIdx.x < 5u)
         counts[blockIdx.x * 8u + threadIdx.x] = wave_sums[threadIdx.x][0] + wave_sums[threadIdx.x][1] + wave_sums[threadIdx.x][2] +
                                                 wave_sums[threadIdx.x][3];
-}
-
-// Exclusive prefix sums of part[k][0..255] for k < K in one workgroup of 256
-// (Hillis-Steele in LDS); returns the totals.
-template <int K>
-__device__ __forceinline__ void block_exclusive_scan(uint32_t (*part)[256], uint32_t (&total)[K]) {
-    const uint32_t t = threadIdx.x;
-    uint32_t mine[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) mine[k] = part[k][t];
-    for (uint32_t step = 1u; step < 256u; step <<= 1) {
-        uint32_t add[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) add[k] = t >= step ? part[k][t - step] : 0u;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < K; ++k) part[k][t] += add[k];
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        total[k] = part[k][255];
-        mine[k] = part[k][t] - mine[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) part[k][t] = mine[k];
-    __syncthreads();
 }
 
 // One workgroup of 256: per range, the ties before it per class, the ties it

@@ -177,6 +177,28 @@ int64_t vgi_read_events(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
  * apply. On by default; off plans the CPU's events alone (gpu.rs:419-425,
  * without_footprints). */
 int vgi_set_footprints(vgi_tracer *tracer, uint32_t on);
+/* Sun corner words of the bound faces (sky_corner_visibility,
+ * voxel_path.wgsl:245-266; the view tier's per-face words and
+ * report_sun_flips, voxel_view_runtime.wgsl:168-268, here for the world's
+ * resident faces): per face four corner bytes (bit 7 known, bit i sees
+ * coloured directional light i, at most four lights), kept per resident slot.
+ * A face whose word is known is re-traced every `refresh_frames` calls
+ * (staggered by bound index: (frame + index) % refresh_frames == 0; 0 or 1:
+ * every call); a face facing no directional light gets 0x80808080. Faces whose
+ * known corners flipped report the light gained or lost (flux); the flips are
+ * binned per 16^3 region and published as one Light retention event at most
+ * every 250 ms of the tracer clock (sun_changes.rs:18-69), applied with the
+ * next resident batch. Synchronous. Requires vgi_enable_resident and bound
+ * faces. */
+int vgi_sun_corners(vgi_tracer *tracer, uint32_t frame, uint32_t refresh_frames);
+/* The last vgi_sun_corners call's flip region (gpu/view/runtime.rs:556-561
+ * layout: count, 2,048 records (voxel xyz, flux bits) from word 4 in
+ * bound-face order, then the overflow box (2^30 - min, max + 2^30 as maxima)
+ * and its flux x 64): returns its word count (8,204), copies at most
+ * `capacity` words when out is not NULL. */
+int64_t vgi_read_sun_flips(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
+/* Corner words of resident slots first .. first + count - 1 (0: not traced). */
+int vgi_read_sun_corners(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 /* ---- GPU work selection (voxel_accumulate.wgsl:448-695; resident.rs:1093-1164) ---- */
 /* Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
  * stable_samples, audit_ms, mirror_budget}, work.rs:38-50; the plugin fills

@@ -1040,6 +1040,13 @@ struct Tracer {
     bool footprints = true;  // gpu/resident.rs:100 trace_footprints
     float stable_error = 0.0f;
     std::vector<uint32_t> keys;
+    // Sun corner words per slot (0 until traced) and SunChanges
+    // (sun_changes.rs): buckets per 16^3 region of a flip's minimum corner,
+    // (z, y, x) order, each (min xyz, max xyz, flux).
+    std::vector<uint32_t> corners;
+    std::map<std::tuple<int, int, int>, std::pair<std::array<int, 6>, float>> sun_buckets;
+    bool sun_published = false;
+    uint32_t sun_published_ms = 0;
 };
 
 // ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
@@ -2860,6 +2867,241 @@ int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targe
     return 0;
 }
 
+
+// ---------------------------------------------------------------- sun corner words
+// voxel_path.wgsl:245-266 (sky_corner_visibility): the first four coloured
+// directional lights a lattice point sees (bit 31 known; 0 when a ray ran out
+// of steps).
+static uint32_t sky_corner_visibility(const Tracer &t, F3 position) {
+    uint32_t word = 0x80000000u, bit = 0u;
+    for (uint32_t i = 0; i < std::min<uint32_t>(t.directional, (uint32_t)t.lights.size()) && bit < 4u; ++i) {
+        const Light &l = t.lights[i];
+        if (fmax_(l.energy[0] - l.extra[1], 0.0f) == 0.0f && fmax_(l.energy[1] - l.extra[2], 0.0f) == 0.0f &&
+            fmax_(l.energy[2] - l.extra[3], 0.0f) == 0.0f)
+            continue;
+        F3 direction = normalize3(f3(l.pr[0], l.pr[1], l.pr[2]));
+        F3 origin = position + direction * 0.002f;
+        uint32_t visibility = trace_domain_limited(t.packed, origin, direction, t.transport.max_distance, true, 2048u, nullptr).status;
+        if (visibility == 2u) { word = 0u; break; }
+        if (visibility == 0u) word |= 1u << bit;
+        bit += 1u;
+    }
+    return word;
+}
+
+// voxel_view_runtime.wgsl:231-247 (report_sun_flips' flux): a quarter face per
+// flipped corner and light, times the cosine, then the albedo.
+static float sun_flip_flux(const Tracer &t, F3 normal, uint32_t material, uint32_t before, uint32_t after) {
+    const F3 w = f3(0.2126f, 0.7152f, 0.0722f);
+    float flux = 0.0f;
+    uint32_t bit = 0u;
+    for (uint32_t i = 0; i < std::min<uint32_t>(t.directional, (uint32_t)t.lights.size()) && bit < 4u; ++i) {
+        const Light &l = t.lights[i];
+        F3 color = f3(fmax_(l.energy[0] - l.extra[1], 0.0f), fmax_(l.energy[1] - l.extra[2], 0.0f), fmax_(l.energy[2] - l.extra[3], 0.0f));
+        if (color.x == 0.0f && color.y == 0.0f && color.z == 0.0f) continue;
+        float cosine = fmax_(dot3(normal, normalize3(f3(l.pr[0], l.pr[1], l.pr[2]))), 0.0f);
+        for (uint32_t corner = 0; corner < 4u; ++corner) {
+            uint32_t a = (before >> (corner * 8u)) & 255u, b = (after >> (corner * 8u)) & 255u;
+            if ((a & b & 128u) != 0u && ((a ^ b) & (1u << bit)) != 0u) flux = flux + 0.25f * dot3(color, w) * cosine;
+        }
+        bit += 1u;
+    }
+    if (flux <= 0.0f) return 0.0f;
+    return flux * dot3(t.materials[material & 255u].diffuse, w);
+}
+
+static const uint32_t SUN_RECORDS = 2048u, SUN_WORDS = 4u + 4u * 2048u + 8u;
+
+// gpu/view/timing.rs:22-49 (decode_sun_flips): records, then the overflow box.
+static void sun_decode(const uint32_t *region, uint32_t records, std::vector<std::pair<std::array<int, 6>, float>> &out) {
+    uint32_t reported = region[0];
+    for (uint32_t k = 0; k < std::min(reported, records); ++k) {
+        const uint32_t *r = region + 4u + 4u * k;
+        out.push_back({{(int)r[0], (int)r[1], (int)r[2], (int)r[0], (int)r[1], (int)r[2]}, bitsf(r[3])});
+    }
+    if (reported > records) {
+        const uint32_t *o = region + 4u + 4u * records;
+        const int64_t bias = (int64_t)1 << 30;
+        out.push_back({{(int)(bias - (int64_t)o[0]), (int)(bias - (int64_t)o[1]), (int)(bias - (int64_t)o[2]), (int)((int64_t)o[3] - bias),
+                        (int)((int64_t)o[4] - bias), (int)((int64_t)o[5] - bias)},
+                       (float)o[6] / 64.0f});
+    }
+}
+
+// sun_changes.rs:27-40 (SunChanges::add)
+static void sun_add(Tracer &t, const std::vector<std::pair<std::array<int, 6>, float>> &flips) {
+    for (const auto &f : flips) {
+        auto key = std::make_tuple(floordiv(f.first[2], 16), floordiv(f.first[1], 16), floordiv(f.first[0], 16));
+        auto it = t.sun_buckets.find(key);
+        if (it == t.sun_buckets.end()) { t.sun_buckets.emplace(key, f); continue; }
+        auto &b = it->second;
+        for (int a = 0; a < 3; ++a) {
+            b.first[a] = std::min(b.first[a], f.first[a]);
+            b.first[3 + a] = std::max(b.first[3 + a], f.first[3 + a]);
+        }
+        b.second = b.second + f.second;
+    }
+}
+
+// sun_changes.rs:42-69 (SunChanges::take): everything binned, at most every 250 ms.
+static bool sun_take(Tracer &t, uint32_t now_ms, std::vector<std::pair<std::array<int, 6>, float>> &out) {
+    out.clear();
+    if (t.sun_buckets.empty() || (t.sun_published && now_ms - t.sun_published_ms < 250u)) return false;
+    t.sun_published = true;
+    t.sun_published_ms = now_ms;
+    for (auto &kv : t.sun_buckets) out.push_back(kv.second);
+    t.sun_buckets.clear();
+    return true;
+}
+
+// The published flips as one Light retention event (GeometryChange::from_bounds
+// over the current scene's solid pages, geometry.rs:299-407; SpatialHistory::
+// event's sorted boxes): three headers, then (min, flux) (max, kind 4) per box.
+static void sun_light_event(const Tracer &t, const std::vector<std::pair<std::array<int, 6>, float>> &due, std::vector<uint32_t> &w) {
+    w.clear();
+    const Scene &s = *t.scene;
+    std::vector<I3> solid_pages;
+    {
+        std::map<std::tuple<int, int, int>, I3> pages;
+        for (auto &kv : s.regions) {
+            const Region &r = *kv.second;
+            if (!r.any_solid) continue;
+            I3 pg{floordiv(r.coord.x, 2), floordiv(r.coord.y, 2), floordiv(r.coord.z, 2)};
+            pages[std::make_tuple(pg.z, pg.y, pg.x)] = pg;
+        }
+        for (auto &kv : pages) solid_pages.push_back(kv.second);
+    }
+    const int64_t r2 = 192 * 192;
+    std::vector<std::pair<std::array<int, 6>, float>> kept;
+    for (const auto &f : due) {
+        I3 mn{f.first[0], f.first[1], f.first[2]}, mx{f.first[3], f.first[4], f.first[5]};
+        for (const I3 &pg : solid_pages) {
+            I3 pmin{pg.x * 32, pg.y * 32, pg.z * 32}, pmax{pmin.x + 31, pmin.y + 31, pmin.z + 31};
+            if (bounds_distance2(mn, mx, pmin, pmax) <= r2) { kept.push_back(f); break; }
+        }
+    }
+    if (kept.empty()) return;
+    std::sort(kept.begin(), kept.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    kept.erase(std::unique(kept.begin(), kept.end(), [](const auto &a, const auto &b) { return a.first == b.first; }), kept.end());
+    std::array<int, 6> u = kept[0].first;
+    for (const auto &f : kept)
+        for (int a = 0; a < 3; ++a) { u[a] = std::min(u[a], f.first[a]); u[3 + a] = std::max(u[3 + a], f.first[3 + a]); }
+    auto push = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) { w.push_back(a); w.push_back(b); w.push_back(c); w.push_back(d); };
+    push((uint32_t)u[0], (uint32_t)u[1], (uint32_t)u[2], 0u);
+    push((uint32_t)u[3], (uint32_t)u[4], (uint32_t)u[5], 0u);
+    push(0u, 0u, 0u, (uint32_t)kept.size());
+    for (const auto &f : kept) {
+        push((uint32_t)f.first[0], (uint32_t)f.first[1], (uint32_t)f.first[2], fbits(f.second));
+        push((uint32_t)f.first[3], (uint32_t)f.first[4], (uint32_t)f.first[5], 4u);
+    }
+}
+
+// voxel_view_runtime.wgsl:168-200 over bound faces (tasks of 8 words and
+// their resident slots, in bound order): each face's corner word, refreshed
+// every `refresh` calls (staggered by bound index) once known; flips (both
+// words known, a corner bit changed) in bound order into `region` (count,
+// 2,048 records, overflow); then SunChanges at `clock_ms`. Returns the Light
+// event's word count (0 when nothing was published or none is in reach),
+// copying at most `capacity` words to `events`.
+int64_t orc_sun_corners(void *tp, const uint32_t *tasks, const uint32_t *slots, uint32_t count, uint32_t frame, uint32_t refresh,
+                        uint32_t clock_ms, uint32_t *region, uint32_t *events, uint32_t capacity) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    if (t.corners.size() < t.high_water) t.corners.resize(t.high_water, 0u);
+    std::vector<uint32_t> r(SUN_WORDS, 0u);
+    uint32_t reported = 0;
+    const int32_t bias = 1073741824;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *task = tasks + (size_t)i * 8;
+        uint32_t face = task[3], slot = slots[i];
+        if (face >= 6u || slot == NONE) continue;
+        F3 normal = face_normal(face);
+        bool lit_face = false;
+        for (uint32_t l = 0; l < std::min<uint32_t>(t.directional, (uint32_t)t.lights.size()); ++l)
+            lit_face = lit_face || dot3(normal, f3(t.lights[l].pr[0], t.lights[l].pr[1], t.lights[l].pr[2])) > 0.0f;
+        uint32_t old = t.corners[slot], word = 0x80808080u;
+        bool carried = old != 0u && refresh > 1u && (frame + i) % refresh != 0u;
+        if (carried) word = old;
+        float flux = 0.0f;
+        if (lit_face && !carried) {
+            F3 u = f3(0.0f, 1.0f, 0.0f), v = f3(0.0f, 0.0f, 1.0f);
+            if (face / 2u == 1u) u = f3(1.0f, 0.0f, 0.0f);
+            else if (face / 2u == 2u) { u = f3(1.0f, 0.0f, 0.0f); v = f3(0.0f, 1.0f, 0.0f); }
+            F3 base = f3((float)(int32_t)task[0], (float)(int32_t)task[1], (float)(int32_t)task[2]) +
+                      f3(fmax_(normal.x, 0.0f), fmax_(normal.y, 0.0f), fmax_(normal.z, 0.0f));
+            uint32_t packed = 0u;
+            for (uint32_t corner = 0; corner < 4u; ++corner) {
+                F3 position = base + u * (float)(corner & 1u) + v * (float)(corner >> 1u);
+                uint32_t cw = sky_corner_visibility(t, position);
+                packed |= ((cw & 15u) | ((cw >> 24u) & 128u)) << (corner * 8u);
+            }
+            word = packed;
+            if (old != 0u) flux = sun_flip_flux(t, normal, t.meta[(size_t)slot * 12 + 3] >> 16, old, packed);
+        }
+        t.corners[slot] = word;
+        if (!(flux > 0.0f)) continue;
+        int x = (int32_t)task[0], y = (int32_t)task[1], z = (int32_t)task[2];
+        uint32_t record = reported++;
+        if (record < SUN_RECORDS) {
+            uint32_t *rec = &r[4u + 4u * record];
+            rec[0] = (uint32_t)x; rec[1] = (uint32_t)y; rec[2] = (uint32_t)z; rec[3] = fbits(flux);
+            continue;
+        }
+        uint32_t *o = &r[4u + 4u * SUN_RECORDS];
+        o[0] = std::max(o[0], (uint32_t)(bias - x)); o[1] = std::max(o[1], (uint32_t)(bias - y)); o[2] = std::max(o[2], (uint32_t)(bias - z));
+        o[3] = std::max(o[3], (uint32_t)(x + bias)); o[4] = std::max(o[4], (uint32_t)(y + bias)); o[5] = std::max(o[5], (uint32_t)(z + bias));
+        o[6] += (uint32_t)(flux * 64.0f + 0.5f);
+    }
+    r[0] = reported;
+    if (region) memcpy(region, r.data(), r.size() * 4);
+    std::vector<std::pair<std::array<int, 6>, float>> flips, due;
+    sun_decode(r.data(), SUN_RECORDS, flips);
+    sun_add(t, flips);
+    std::vector<uint32_t> w;
+    if (sun_take(t, clock_ms, due)) sun_light_event(t, due, w);
+    if (events) memcpy(events, w.data(), std::min<size_t>(capacity, w.size()) * 4);
+    return (int64_t)w.size();
+}
+
+// Test hooks of the host logic alone: decode a flip region into boxes (7
+// words each: min xyz, max xyz, flux bits); bin boxes; take the bins due at
+// now_ms (-1 when none is due).
+int orc_sun_decode(const uint32_t *region, uint32_t records, uint32_t *out, uint32_t capacity) {
+    std::vector<std::pair<std::array<int, 6>, float>> flips;
+    sun_decode(region, records, flips);
+    for (size_t k = 0; k < flips.size() && k < capacity; ++k) {
+        for (int a = 0; a < 6; ++a) out[k * 7 + a] = (uint32_t)flips[k].first[a];
+        out[k * 7 + 6] = fbits(flips[k].second);
+    }
+    return (int)flips.size();
+}
+
+int orc_sun_corner_words(void *tp, uint32_t first, uint32_t count, uint32_t *out) {
+    Tracer &t = *(Tracer *)tp;
+    for (uint32_t k = 0; k < count; ++k) out[k] = first + k < t.corners.size() ? t.corners[first + k] : 0u;
+    return 0;
+}
+
+int orc_sun_add(void *tp, const uint32_t *boxes, uint32_t n) {
+    std::vector<std::pair<std::array<int, 6>, float>> flips;
+    for (uint32_t k = 0; k < n; ++k) {
+        std::array<int, 6> b;
+        for (int a = 0; a < 6; ++a) b[a] = (int32_t)boxes[k * 7 + a];
+        flips.push_back({b, bitsf(boxes[k * 7 + 6])});
+    }
+    sun_add(*(Tracer *)tp, flips);
+    return 0;
+}
+
+int orc_sun_take(void *tp, uint32_t now_ms, uint32_t *out, uint32_t capacity) {
+    std::vector<std::pair<std::array<int, 6>, float>> due;
+    if (!sun_take(*(Tracer *)tp, now_ms, due)) return -1;
+    for (size_t k = 0; k < due.size() && k < capacity; ++k) {
+        for (int a = 0; a < 6; ++a) out[k * 7 + a] = (uint32_t)due[k].first[a];
+        out[k * 7 + 6] = fbits(due[k].second);
+    }
+    return (int)due.size();
+}
 
 // One trace_domain_limited call (voxel_traversal.wgsl:126-219, visible_geometry
 // false, 2048 steps): out_i = voxel xyz, material, status, face; out_f = position, normal.

@@ -122,6 +122,14 @@ int orc_cache_state(void *tracer, uint32_t first, uint32_t count, uint32_t *out)
 int orc_cache_meta(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
 int orc_cache_apply_events(void *tracer, const uint32_t *events, uint32_t event_count,
                            uint32_t words);
+/* ---- Sun corner words (voxel_path.wgsl:245-266; voxel_view_runtime.wgsl:168-268;
+ * sun_changes.rs; gpu/view/timing.rs:22-49) ---- */
+int64_t orc_sun_corners(void *tracer, const uint32_t *tasks, const uint32_t *slots, uint32_t count, uint32_t frame, uint32_t refresh,
+                        uint32_t clock_ms, uint32_t *region, uint32_t *events, uint32_t capacity);
+int orc_sun_decode(const uint32_t *region, uint32_t records, uint32_t *out, uint32_t capacity);
+int orc_sun_corner_words(void *tracer, uint32_t first, uint32_t count, uint32_t *out);
+int orc_sun_add(void *tracer, const uint32_t *boxes, uint32_t n);
+int orc_sun_take(void *tracer, uint32_t now_ms, uint32_t *out, uint32_t capacity);
 /* Change footprints on (default) or off for edit_event (gpu.rs:419-425,
  * without_footprints: tests of the CPU-planned events alone). */
 int orc_set_footprints(void *tracer, uint32_t on);

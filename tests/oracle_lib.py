@@ -79,6 +79,11 @@ def lib():
         "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_schedule": (ctypes.c_int, [P, P, u32, f32, u32, u32, u32]),
         "orc_set_footprints": (ctypes.c_int, [P, u32]),
+        "orc_sun_corners": (ctypes.c_int64, [P, P, P, u32, u32, u32, u32, P, P, u32]),
+        "orc_sun_decode": (ctypes.c_int, [P, u32, P, u32]),
+        "orc_sun_corner_words": (ctypes.c_int, [P, u32, u32, P]),
+        "orc_sun_add": (ctypes.c_int, [P, P, u32]),
+        "orc_sun_take": (ctypes.c_int, [P, u32, P, u32]),
         "orc_select": (ctypes.c_int, [P, u32, u32, P, P, u32, P]),
         "orc_cache_keys": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_picks": (ctypes.c_int, [P, P, u32, P, ctypes.c_int, u32]),
@@ -261,6 +266,33 @@ class Tracer:
     def set_footprints(self, on):
         lib().orc_set_footprints(self.h, 1 if on else 0)
 
+    def sun_corners(self, tasks, slots, frame=0, refresh=1, clock_ms=0):
+        """Corner words of the bound faces (tasks in bound order, their
+        slots): (flip region words, Light event words or empty)."""
+        tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8)
+        slots = np.ascontiguousarray(slots, np.uint32)
+        region = np.zeros(SUN_WORDS, np.uint32)
+        cap = 1 << 20
+        events = np.zeros(cap, np.uint32)
+        n = lib().orc_sun_corners(self.h, ptr(tasks), ptr(slots), len(slots), frame, refresh, clock_ms, ptr(region), ptr(events), cap)
+        return region, events[:n].copy()
+
+    def sun_corner_words(self, first, count):
+        out = np.zeros(count, np.uint32)
+        lib().orc_sun_corner_words(self.h, first, count, ptr(out))
+        return out
+
+    def sun_add(self, boxes):
+        """Bin flip boxes ((min xyz, max xyz, flux) each)."""
+        words = sun_box_words(boxes)
+        lib().orc_sun_add(self.h, ptr(words), len(boxes))
+
+    def sun_take(self, now_ms):
+        """The bins due at now_ms as (min, max, flux) boxes, or None."""
+        out = np.zeros(7 * 4096, np.uint32)
+        n = lib().orc_sun_take(self.h, now_ms, ptr(out), 4096)
+        return None if n < 0 else sun_boxes_of(out[:7 * n])
+
     def select(self, picks, clock_ms=0):
         """One pick batch's selection passes: (1038 scratch words, mirrored
         slots, pick list)."""
@@ -286,6 +318,33 @@ class Tracer:
     def cache_picks(self, pick_slots, threads=8, clock_ms=0):
         pick_slots = np.ascontiguousarray(pick_slots, np.uint32)
         return lib().orc_cache_picks(self.h, ptr(pick_slots), len(pick_slots), None, threads, clock_ms)
+
+
+SUN_RECORDS = 2048
+SUN_WORDS = 4 + 4 * SUN_RECORDS + 8
+
+
+def sun_box_words(boxes):
+    w = np.zeros((len(boxes), 7), np.uint32)
+    for k, (mn, mx, flux) in enumerate(boxes):
+        w[k, 0:3] = np.asarray(mn, np.int32).view(np.uint32)
+        w[k, 3:6] = np.asarray(mx, np.int32).view(np.uint32)
+        w[k, 6] = np.float32(flux).view(np.uint32)
+    return np.ascontiguousarray(w.ravel())
+
+
+def sun_boxes_of(words):
+    w = np.asarray(words, np.uint32).reshape(-1, 7)
+    return [(tuple(r[0:3].view(np.int32).tolist()), tuple(r[3:6].view(np.int32).tolist()), float(r[6:7].view(np.float32)[0]))
+            for r in w]
+
+
+def sun_decode(region, records=SUN_RECORDS):
+    """A flip region as (min, max, flux) boxes (gpu/view/timing.rs:22-49)."""
+    region = np.ascontiguousarray(region, np.uint32)
+    out = np.zeros(7 * (records + 1), np.uint32)
+    n = lib().orc_sun_decode(ptr(region), records, ptr(out), records + 1)
+    return sun_boxes_of(out[:7 * n])
 
 
 def event_count(words):
