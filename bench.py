@@ -18,6 +18,7 @@ the same scene's faces on the host cores.
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
@@ -298,6 +299,49 @@ def work_selection(vgi, g, faces, args):
     }
 
 
+def sun_corner_words(vgi, g, faces, lights, args):
+    """Sun corner words of the cave's bound resident faces: the first call
+    traces every face turned towards the sun; then the sun moves by one
+    degree of azimuth per call, `--sun-calls` calls re-tracing every face and
+    as many re-tracing a quarter (the reference's refresh every fourth frame).
+    Each lit face traces four corner rays (visible-geometry walks to the
+    scene's end or a hit)."""
+    base = np.array(lights, np.float32).copy()
+    d = base[0, 0:3].astype(np.float64)
+    elevation = math.asin(d[1] / np.linalg.norm(d))
+    azimuth = math.atan2(d[2], d[0])
+    normals = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)], np.float64)
+    out = {}
+    g.sun_corners(0, 1)
+    for refresh in (1, 4):
+        gpu_ms, host_ms, flips, rays = [], [], [], []
+        for k in range(args.sun_calls):
+            a = azimuth + math.radians(1.0 + k + (args.sun_calls if refresh == 4 else 0))
+            light = base.copy()
+            light[0, 0:3] = (math.cos(elevation) * math.cos(a), math.sin(elevation), math.cos(elevation) * math.sin(a))
+            g.set_lights(light)
+            g.sun_corners(k + 1, refresh)
+            ms = g.sun_timings()
+            gpu_ms.append(float(ms[0]))
+            host_ms.append(float(ms[1]))
+            flips.append(int(g.sun_flips()[0]))
+            lit = (normals[faces["face"]] @ light[0, 0:3].astype(np.float64)) > 0
+            due = ((k + 1 + np.arange(len(faces))) % refresh == 0) if refresh > 1 else np.ones(len(faces), bool)
+            rays.append(4 * int((lit & due).sum()))
+        med = float(np.median(gpu_ms))
+        out["refresh_%d" % refresh] = {
+            "gpu_ms_median": round(med, 3),
+            "host_ms_median": round(float(np.median(host_ms)), 3),
+            "corner_rays_per_call": int(np.median(rays)),
+            "mrays_per_s": round(float(np.median(rays)) / (med / 1000.0) / 1e6, 1),
+            "flips_per_call_median": int(np.median(flips)),
+        }
+    g.set_lights(base)
+    print("sun corners: %s" % out, file=sys.stderr, flush=True)
+    return dict({"workload": "512^3 worm cave, every bound face resident; the sun's azimuth moves 1 degree per call",
+                 "faces": int(len(faces)), "calls_timed": args.sun_calls}, **out)
+
+
 def batched_scenes(vgi, args):
     """32 seeded noise scenes of log-uniform edges in [32, 512] (multiples of
     8), 2 rays per face, 2 bounces, traced in one persistent launch per frame
@@ -392,6 +436,8 @@ def main():
     ap.add_argument("--no-select", action="store_true", help="skip the GPU work-selection leg")
     ap.add_argument("--select-picks", type=int, default=262144)
     ap.add_argument("--select-batches", type=int, default=8)
+    ap.add_argument("--no-sun", action="store_true", help="skip the sun corner-word leg")
+    ap.add_argument("--sun-calls", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -526,6 +572,8 @@ def main():
         line["roofline"]["traffic"] = round(json.load(open(traffic_file))["traffic_bytes_per_launch"])
     if not args.no_select and args.workload == "cave512":
         line["work_selection"] = work_selection(vgi, g, faces, args)
+    if not args.no_sun and args.workload == "cave512":
+        line["sun_corner_words"] = sun_corner_words(vgi, g, faces, lights, args)
     g.close()
     if not args.no_edit_stream and args.workload == "cave512":
         line["edit_stream"] = edit_stream(vgi, args)

@@ -127,6 +127,7 @@ def library():
         "vgi_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32]),
         "vgi_read_sun_flips": (ctypes.c_int64, [P, P, ctypes.c_uint32]),
         "vgi_read_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
+        "vgi_sun_timings": (ctypes.c_int, [P, P, ctypes.c_uint32]),
         "vgi_select_frame": (ctypes.c_int, [P, u32, u32]),
         "vgi_select_timings": (ctypes.c_int, [P, P]),
         "vgi_read_picks": (ctypes.c_int, [P, P, u32, ctypes.POINTER(u32)]),
@@ -174,7 +175,7 @@ def exported_symbols():
         "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
-        "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners",
+        "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -472,6 +473,11 @@ class GpuTracer:
         n = _check(self._lib.vgi_read_sun_flips(self._h, None, 0))
         out = np.zeros(n, np.uint32)
         self._lib.vgi_read_sun_flips(self._h, _ptr(out), n)
+        return out
+
+    def sun_timings(self):
+        out = np.zeros(2, np.float32)
+        _check(self._lib.vgi_sun_timings(self._h, _ptr(out), 2))
         return out
 
     def sun_corner_words(self, first, count):

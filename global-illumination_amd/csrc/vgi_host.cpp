@@ -133,9 +133,11 @@ struct vgi_tracer {
     DeviceBuffer d_footprint_tasks, d_footprint_sums;
     // Sun corner words: one per resident slot (0 until traced), the last
     // call's flip region, and the flips binned until they publish.
-    DeviceBuffer d_corners, d_sun_flux, d_sun_ranges, d_sun_region;
+    DeviceBuffer d_corners, d_sun_flux, d_sun_queue, d_sun_ranges, d_sun_region;
     std::vector<uint32_t> sun_region;
     vgi_host::SunChanges sun_changes;
+    hipEvent_t sun_ev[2] = {};
+    float sun_ms[2] = {0.0f, 0.0f};  // GPU passes, host binning and event
     uint32_t picks = 0, mirror_budget = 0;
     hipEvent_t sel_ev[4] = {};
     bool select_recorded = false;
@@ -1264,7 +1266,7 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
                             &t->d_pick_results, &t->d_mirror, &t->d_select_ranges,
-                            &t->d_footprint_tasks, &t->d_footprint_sums, &t->d_corners, &t->d_sun_flux,
+                            &t->d_footprint_tasks, &t->d_footprint_sums, &t->d_corners, &t->d_sun_flux, &t->d_sun_queue,
                             &t->d_sun_ranges, &t->d_sun_region};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
@@ -1272,6 +1274,7 @@ void vgi_destroy(vgi_tracer *t) {
     for (auto &e : t->ev) hipEventDestroy(e);
     for (auto &e : t->side_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->sel_ev) if (e) hipEventDestroy(e);
+    for (auto &e : t->sun_ev) if (e) hipEventDestroy(e);
     if (t->side) hipStreamDestroy(t->side);
     hipStreamDestroy(t->stream);
     delete t;
@@ -1847,18 +1850,28 @@ int vgi_sun_corners(vgi_tracer *t, uint32_t frame, uint32_t refresh_frames) {
     uint32_t count = t->bound;
     if (count && t->b_slots.bytes < (size_t)count * 4) return fail("bound faces have no resident slots: bind them with the cache on");
     HIP_OK(t->d_sun_flux.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
-    HIP_OK(t->d_sun_ranges.reserve((size_t)2 * vgi::SUN_RANGES * 4));
+    HIP_OK(t->d_sun_queue.reserve(((size_t)count + 1) * 4));
+    HIP_OK(t->d_sun_ranges.reserve((size_t)10 * vgi::SUN_RANGES * 4));
     HIP_OK(t->d_sun_region.reserve((size_t)vgi::SUN_FLIP_WORDS * 4));
+    if (!t->sun_ev[0]) {
+        HIP_OK(hipEventCreate(&t->sun_ev[0]));
+        HIP_OK(hipEventCreate(&t->sun_ev[1]));
+    }
     vgi::SceneView s;
     vgi::Optics o;
     views(*t, s, o);
+    HIP_OK(hipEventRecord(t->sun_ev[0], t->stream));
     vgi::launch_sun_corners(s, o, t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_meta.as<uint32_t>(),
-                            t->d_corners.as<uint32_t>(), t->d_sun_flux.as<float>(), frame, refresh_frames,
+                            t->d_corners.as<uint32_t>(), t->d_sun_flux.as<float>(), t->d_sun_queue.as<uint32_t>(),
+                            (uint32_t)std::max(t->cus, 1) * 32u, frame, refresh_frames,
                             t->d_sun_ranges.as<uint32_t>(), t->d_sun_region.as<uint32_t>(), t->stream);
     HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(t->sun_ev[1], t->stream));
     t->sun_region.resize(vgi::SUN_FLIP_WORDS);
     HIP_OK(hipMemcpyAsync(t->sun_region.data(), t->d_sun_region.ptr, (size_t)vgi::SUN_FLIP_WORDS * 4, hipMemcpyDeviceToHost, t->stream));
     HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipEventElapsedTime(&t->sun_ms[0], t->sun_ev[0], t->sun_ev[1]));
+    auto t_host = std::chrono::steady_clock::now();
     std::vector<vgi_host::SunFlip> flips;
     vgi_host::decode_sun_flips(t->sun_region.data(), vgi::SUN_FLIP_RECORDS, flips);
     t->sun_changes.add(flips);
@@ -1869,6 +1882,13 @@ int vgi_sun_corners(vgi_tracer *t, uint32_t frame, uint32_t refresh_frames) {
         vgi_host::RetentionEvent ev;
         if (vgi_host::light_event(due, scene, ev)) t->pending_events.push_back(std::move(ev));
     }
+    t->sun_ms[1] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_host).count();
+    return 0;
+}
+
+int vgi_sun_timings(vgi_tracer *t, float *out, uint32_t n) {
+    if (!t || (n && !out)) return fail("null argument");
+    for (uint32_t i = 0; i < n && i < 2; ++i) out[i] = t->sun_ms[i];
     return 0;
 }
 

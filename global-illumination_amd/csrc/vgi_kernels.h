@@ -146,7 +146,9 @@ void launch_footprints(const SceneView &scene, const Optics &optics, const int4 
 // Sun corner words (voxel_path.wgsl:245-266; voxel_view_runtime.wgsl:168-268)
 // of `count` bound faces (tasks: 2 uint4 each; slots: their resident slots):
 // `corners` holds one word per slot (0 = none yet); `flips` (one float per
-// bound face) and `ranges` (2 x SUN_RANGES words) are scratch. The flip region
+// bound face), `queue` (1 + count words: the faces that trace) and `ranges`
+// (10 x SUN_RANGES words: per-range counts, offsets, overflow parts) are scratch; the tracing pass runs at most
+// `trace_blocks` workgroups of 256 over the queue. The flip region
 // gets the count (word 0), up to SUN_FLIP_RECORDS records (voxel, flux bits)
 // from word 4 in bound-face order, then the overflow box (biased maxima of
 // 2^30 - min and max + 2^30) and its flux x 64 at SUN_FLIP_OVERFLOW.
@@ -154,10 +156,10 @@ constexpr uint32_t SUN_FLIP_RECORDS = 2048u;
 constexpr uint32_t SUN_FLIP_OVERFLOW = 4u + 4u * SUN_FLIP_RECORDS;
 constexpr uint32_t SUN_FLIP_WORDS = SUN_FLIP_OVERFLOW + 8u;
 constexpr int SUN_FLIP_BIAS = 1073741824;
-constexpr uint32_t SUN_RANGES = 1024u;
+constexpr uint32_t SUN_RANGES = 4096u;
 void launch_sun_corners(const SceneView &scene, const Optics &optics, const uint4 *tasks, const uint32_t *slots, uint32_t count,
-                        const uint32_t *meta, uint32_t *corners, float *flips, uint32_t frame, uint32_t refresh, uint32_t *ranges,
-                        uint32_t *region, hipStream_t stream);
+                        const uint32_t *meta, uint32_t *corners, float *flips, uint32_t *queue, uint32_t trace_blocks, uint32_t frame,
+                        uint32_t refresh, uint32_t *ranges, uint32_t *region, hipStream_t stream);
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 
