@@ -28,6 +28,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
+# Learned-cache training per frame: one-sample training paths, Adam steps.
+LEARNED_TRAIN = (131072, 16)
+
 WORKLOADS = {
     # name: (generator id, seed, edge, bounces, spp, description)
     "cave512": (5, 0x5EED0005, 512, 4, 4, "512^3 worm cave, 200 worms r3-8, 300 emitters, 3 sun shafts; plain 4-bounce, 4 rays/face"),
@@ -159,7 +162,7 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     print("plain: %d frames, %.0f GPU ms, rmse %.4f" % pts[-1], file=sys.stderr, flush=True)
     g = tracer()
     g.bind_faces(ordered)
-    g.mlp_enable(seed=1, train_per_frame=65536, steps=8, lr=1e-3)
+    g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
     t1 = time.perf_counter()
     g.mlp_pretrain(args.pretrain_frames)
     g.sync()
@@ -184,7 +187,8 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
         if name == "learned":
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
-            entry["training"] = "64k one-sample 3-bounce targets per frame, 8 Adam steps (lr 1e-3), BF16 MFMA"
+            entry["training"] = ("%dk one-sample 3-bounce targets per frame, %d Adam steps (lr 1e-3, hash table 1e-2), BF16 MFMA; "
+                                 "encoding: 8-level hash grid + sin/cos(pi u), normal, albedo" % (LEARNED_TRAIN[0] // 1024, LEARNED_TRAIN[1]))
         out[name] = entry
     return out
 
@@ -481,7 +485,7 @@ def main():
     if args.learned_only:
         # Learned frames alone (for profiling): the cache warmed up on
         # `pretrain_frames` training-only frames, then `steps` timed frames.
-        g.mlp_enable(seed=1, train_per_frame=65536, steps=8, lr=1e-3)
+        g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
         g.mlp_pretrain(args.pretrain_frames)
         for _ in range(args.warmup):
             g.run_frame()

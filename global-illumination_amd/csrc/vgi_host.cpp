@@ -159,8 +159,11 @@ struct vgi_tracer {
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
     DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
+    DeviceBuffer d_table, d_table_m, d_table_v, d_table_g;  // hash-grid encoding (FP32, Adam)
+    std::vector<float> table_grad_last;                      // of the last vgi_mlp_train without update
     uint32_t mlp_step = 0, train_per_frame = 65536, train_steps = 4, train_cursor = 0;
     float mlp_lr = 1e-3f, last_loss = 0.0f;
+    float table_lr = 1e-2f;  // the hash table's Adam rate
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
     // ---- wavefront path tracing of bound frames
@@ -845,6 +848,8 @@ vgi::MlpScene mlp_scene(const vgi_tracer &t) {
     int extent = std::max(std::max(t.bmax.x - t.bmin.x + 1, t.bmax.y - t.bmin.y + 1), t.bmax.z - t.bmin.z + 1) * 16;
     sc.inv_extent = 1.0f / (float)std::max(extent, 1);
     sc.materials = t.d_materials.as<float4>();
+    sc.table = t.d_table.as<float2>();
+    for (int l = 0; l < vgi::MLP_HASH_LEVELS; ++l) sc.res[l] = vgi::mlp_hash_res(l, std::max(extent, 1));
     return sc;
 }
 
@@ -859,12 +864,14 @@ int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets,
     HIP_OK(hipMemsetAsync(t.d_mlp_loss.ptr, 0, 4, t.stream));
     vgi::mlp_forward_backward(t.d_mlp_frag.as<uint16_t>(), mlp_scene(t), queries, reinterpret_cast<const float *>(targets), count,
                               t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), t.d_mlp_loss.as<float>(), t.d_train_pred.as<float>(),
-                              t.stream);
+                              t.d_table_g.as<float>(), t.stream);
     vgi::mlp_weight_grad(t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), count, t.d_mlp_g.as<float>(), t.stream);
     if (update) {
         t.mlp_step += 1;
         vgi::mlp_adam(t.d_mlp_w.as<float>(), t.d_mlp_m.as<float>(), t.d_mlp_v.as<float>(), t.d_mlp_g.as<float>(), t.mlp_step,
                       t.mlp_lr, t.stream);
+        vgi::mlp_table_adam(t.d_table.as<float>(), t.d_table_m.as<float>(), t.d_table_v.as<float>(), t.d_table_g.as<float>(),
+                            t.mlp_step, t.table_lr, t.stream);
         vgi::mlp_pack(t.d_mlp_w.as<float>(), t.d_mlp_frag.as<uint16_t>(), t.stream);
     }
     HIP_OK(hipGetLastError());
@@ -1259,7 +1266,8 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_scatter_idx, &t->d_scatter_val, &t->d_materials, &t->d_lights, &t->d_emitters,
                             &t->d_index, &t->d_state, &t->d_checkpoint, &t->d_meta, &t->d_tasks, &t->d_slots,
                             &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters, &t->d_mlp_w, &t->d_mlp_m,
-                            &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->b_learned, &t->b_queries,
+                            &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->d_table,
+                            &t->d_table_m, &t->d_table_v, &t->d_table_g, &t->b_learned, &t->b_queries,
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
                             &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io,
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
@@ -1680,6 +1688,10 @@ int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint3
     HIP_OK(t->d_mlp_g.reserve(vgi::MLP_PARAMS * 4));
     HIP_OK(t->d_mlp_frag.reserve(vgi::MLP_FRAG * 256 * 2));
     HIP_OK(t->d_mlp_loss.reserve(16));
+    HIP_OK(t->d_table.reserve((size_t)vgi::MLP_TABLE * 4));
+    HIP_OK(t->d_table_m.reserve((size_t)vgi::MLP_TABLE * 4));
+    HIP_OK(t->d_table_v.reserve((size_t)vgi::MLP_TABLE * 4));
+    HIP_OK(t->d_table_g.reserve((size_t)vgi::MLP_TABLE * 4));
     // He-uniform initialisation from the seed (fan-in of each layer).
     std::vector<float> w(vgi::MLP_PARAMS);
     std::mt19937 rng(seed);
@@ -1691,6 +1703,14 @@ int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint3
         for (int i = offs[l]; i < offs[l + 1]; ++i) w[i] = u(rng);
     }
     HIP_OK(hipMemcpyAsync(t->d_mlp_w.ptr, w.data(), w.size() * 4, hipMemcpyHostToDevice, t->stream));
+    // Hash table entries uniform in +-1e-4 (the encoding's initialisation).
+    std::vector<float> table(vgi::MLP_TABLE);
+    std::uniform_real_distribution<float> small(-1e-4f, 1e-4f);
+    for (float &x : table) x = small(rng);
+    HIP_OK(hipMemcpyAsync(t->d_table.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_table_m.ptr, 0, (size_t)vgi::MLP_TABLE * 4, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_table_v.ptr, 0, (size_t)vgi::MLP_TABLE * 4, t->stream));
+    HIP_OK(hipMemsetAsync(t->d_table_g.ptr, 0, (size_t)vgi::MLP_TABLE * 4, t->stream));
     HIP_OK(hipMemsetAsync(t->d_mlp_m.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
     HIP_OK(hipMemsetAsync(t->d_mlp_v.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
     HIP_OK(hipMemsetAsync(t->d_mlp_g.ptr, 0, vgi::MLP_PARAMS * 4, t->stream));
@@ -1703,6 +1723,16 @@ int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint3
     t->train_per_frame = train_per_frame;
     t->train_steps = std::max<uint32_t>(steps, 1u);
     t->mlp_lr = lr;
+    // The table learns ten times faster than the network by default (its
+    // entries see few samples each).
+    t->table_lr = 10.0f * lr;
+    return 0;
+}
+
+int vgi_mlp_table_lr(vgi_tracer *t, float lr) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (!(lr >= 0.0f)) return fail("the table's learning rate is not a non-negative number");
+    t->table_lr = lr;
     return 0;
 }
 
@@ -1763,6 +1793,27 @@ int vgi_mlp_train(vgi_tracer *t, const uint32_t *queries, const float *targets, 
     if (!update) {
         if (grad_out) HIP_OK(hipMemcpy(grad_out, t->d_mlp_g.ptr, vgi::MLP_PARAMS * 4, hipMemcpyDeviceToHost));
         HIP_OK(hipMemset(t->d_mlp_g.ptr, 0, vgi::MLP_PARAMS * 4));
+        t->table_grad_last.resize(vgi::MLP_TABLE);
+        HIP_OK(hipMemcpy(t->table_grad_last.data(), t->d_table_g.ptr, (size_t)vgi::MLP_TABLE * 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemset(t->d_table_g.ptr, 0, (size_t)vgi::MLP_TABLE * 4));
+    }
+    return 0;
+}
+
+int vgi_mlp_table(vgi_tracer *t, float *out, const float *in, float *grad_out) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    HIP_OK(hipSetDevice(t->device));
+    if (out) {
+        HIP_OK(hipStreamSynchronize(t->stream));
+        HIP_OK(hipMemcpy(out, t->d_table.ptr, (size_t)vgi::MLP_TABLE * 4, hipMemcpyDeviceToHost));
+    }
+    if (in) {
+        HIP_OK(hipMemcpyAsync(t->d_table.ptr, in, (size_t)vgi::MLP_TABLE * 4, hipMemcpyHostToDevice, t->stream));
+        HIP_OK(hipStreamSynchronize(t->stream));
+    }
+    if (grad_out) {
+        if (t->table_grad_last.size() != vgi::MLP_TABLE) return fail("no gradient yet: call vgi_mlp_train without update");
+        std::memcpy(grad_out, t->table_grad_last.data(), (size_t)vgi::MLP_TABLE * 4);
     }
     return 0;
 }

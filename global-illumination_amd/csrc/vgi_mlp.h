@@ -3,6 +3,16 @@
 // voxel_path.wgsl:63-87) from an encoding of its position, normal and albedo.
 // BF16 weights and activations on MFMA with FP32 accumulation; FP32 master
 // weights with Adam.
+//
+// Position encoding: a multiresolution hash grid (Mueller et al. 2022,
+// "Instant neural graphics primitives"): 8 levels of 2 trainable features,
+// level l a grid of res_l cells across the scene's largest extent (res_l =
+// floor(16 b^l), b = (extent / 16)^(1/7)), the features trilinearly
+// interpolated from the cell's 8 corners; a level whose (res_l + 2)^3 corners
+// fit the 2^18-entry table indexes them densely, finer levels by the spatial
+// hash x ^ 2654435761 y ^ 805459861 z (mod 2^18). Features 0..15: the grid;
+// 16..21: sin and cos of pi u per axis; 22..24: the normal; 25..27: the
+// albedo; 28: 1 (the bias); 29..31: 0.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,9 +32,10 @@ constexpr int MLP_W5 = MLP_W4 + MLP_HIDDEN * MLP_HIDDEN;
 constexpr int MLP_PARAMS = MLP_W5 + MLP_OUT * MLP_HIDDEN;
 // BF16 MFMA fragments (64 lanes x 4 values per 16x16 tile): forward W tiles
 // (W1 4x2, W2..W4 4x4, W5 1x4 with rows padded to 16) then transposed tiles
-// for the backward data pass (W2..W4 4x4, W5^T 4x1).
+// for the backward data pass (W2..W4 4x4, W5^T 4x1, then W1^T's first row of
+// tiles, 1x4: the gradient of the hash features).
 constexpr int MLP_FWD_TILES = 8 + 16 * 3 + 4;
-constexpr int MLP_BWD_TILES = 16 * 3 + 4;
+constexpr int MLP_BWD_TILES = 16 * 3 + 4 + 4;
 constexpr int MLP_FRAG = MLP_FWD_TILES + MLP_BWD_TILES;
 
 // One query of the cache: the face whose incident radiance is wanted.
@@ -33,11 +44,25 @@ struct MlpQuery {
     uint32_t face_material;  // face | material << 8; 0xffffffff: no query
 };
 
+constexpr int MLP_HASH_LEVELS = 8;
+constexpr int MLP_HASH_FEATURES = 2;
+constexpr uint32_t MLP_HASH_SIZE = 1u << 18;  // entries per level
+constexpr uint32_t MLP_TABLE = (uint32_t)MLP_HASH_LEVELS * MLP_HASH_SIZE * MLP_HASH_FEATURES;
+
 struct MlpScene {
     float lo_x, lo_y, lo_z;  // scene minimum corner (voxels)
     float inv_extent;        // 1 / largest scene extent
     const float4 *materials; // (diffuse rgb, emission power), (emission rgb, 0)
+    const float2 *table;     // level-major, MLP_HASH_SIZE entries of 2 features per level
+    uint32_t res[MLP_HASH_LEVELS];
 };
+
+// Cells across the largest extent at each level (host and oracle alike).
+inline uint32_t mlp_hash_res(int level, int extent) {
+    double top = extent > 16 ? (double)extent : 16.0;
+    double r = 16.0 * __builtin_exp((double)level * __builtin_log(top / 16.0) / (double)(MLP_HASH_LEVELS - 1));
+    return (uint32_t)__builtin_floor(r + 1e-3);
+}
 
 // Launchers (vgi_mlp.hip).
 void mlp_pack(const float *master, uint16_t *frags, hipStream_t stream);
@@ -46,11 +71,15 @@ void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *que
 // Fused forward + loss + backward-data over `count` samples (count a multiple
 // of 16): writes per-layer inputs X_i and output gradients dY_i (bf16) for the
 // weight-gradient pass, and the summed loss.
+// `table_grad` gets the hash table's gradient added (FP32 atomics).
 void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, const float *targets,
-                          uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, hipStream_t stream);
+                          uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, float *table_grad,
+                          hipStream_t stream);
 // dW_i = sum over samples of dY_i^T X_i on MFMA (split over sample chunks,
 // FP32 atomics into `grad`).
 void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream);
 void mlp_adam(float *master, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream);
+// Adam over the hash table (eps 1e-15); clears its gradient.
+void mlp_table_adam(float *table, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream);
 
 }  // namespace vgi

@@ -45,7 +45,7 @@ __device__ __host__ constexpr int fwd_tile(int layer, int t, int s) {
     return layer == 1 ? t * 2 + s : (layer <= 4 ? 8 + 16 * (layer - 2) + t * 4 + s : 56 + s);
 }
 __device__ __host__ constexpr int bwd_tile(int layer, int t, int s) {
-    return MLP_FWD_TILES + (layer <= 4 ? 16 * (layer - 2) + t * 4 + s : 48 + t);
+    return MLP_FWD_TILES + (layer == 1 ? 52 + s : layer <= 4 ? 16 * (layer - 2) + t * 4 + s : 48 + t);
 }
 
 // Packs FP32 master weights into BF16 A-operand fragments: forward tile (t, s)
@@ -66,7 +66,8 @@ __global__ void mlp_pack_kernel(const float *__restrict__ w, uint16_t *__restric
         else { layer = 5; t = 0; s = local - 56; }
     } else {
         if (local < 48) { layer = 2 + local / 16; t = (local % 16) / 4; s = local % 4; }
-        else { layer = 5; t = local - 48; s = 0; }
+        else if (local < 52) { layer = 5; t = local - 48; s = 0; }
+        else { layer = 1; t = 0; s = local - 52; }
     }
     int in = layer == 1 ? MLP_IN : MLP_HIDDEN;
     int out = layer == 5 ? MLP_OUT : MLP_HIDDEN;
@@ -77,28 +78,69 @@ __global__ void mlp_pack_kernel(const float *__restrict__ w, uint16_t *__restric
     frags[e] = to_bf16(v);
 }
 
-// The 32 input features of a query (frequency encoding of the face centre,
-// the face normal, the albedo, a constant).
-__device__ __forceinline__ float encode(const MlpScene &sc, const MlpQuery &q, int f) {
+// The face centre in scene units (0..1 across the largest extent).
+__device__ __forceinline__ f3 query_position(const MlpScene &sc, const MlpQuery &q, f3 n) {
+    return mk(((float)q.vx + 0.5f + n.x * 0.5f - sc.lo_x) * sc.inv_extent, ((float)q.vy + 0.5f + n.y * 0.5f - sc.lo_y) * sc.inv_extent,
+              ((float)q.vz + 0.5f + n.z * 0.5f - sc.lo_z) * sc.inv_extent);
+}
+
+// One level of the hash grid at position u: the 8 corner entries and
+// trilinear weights (corner c = x + 2y + 4z bits), in the order the features
+// sum them.
+struct HashCorners {
+    uint32_t index[8];
+    float weight[8];
+};
+__device__ __forceinline__ HashCorners hash_corners(uint32_t res, f3 u) {
+    HashCorners h;
+    float px = u.x * (float)res, py = u.y * (float)res, pz = u.z * (float)res;
+    float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    float wx = px - fx, wy = py - fy, wz = pz - fz;
+    uint32_t x0 = (uint32_t)(int)fx, y0 = (uint32_t)(int)fy, z0 = (uint32_t)(int)fz;
+    const uint32_t side = res + 2u;
+    const bool dense = side * side * side <= MLP_HASH_SIZE;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        uint32_t x = x0 + (uint32_t)(c & 1), y = y0 + (uint32_t)((c >> 1) & 1), z = z0 + (uint32_t)(c >> 2);
+        uint32_t i = dense ? x + side * (y + side * z) : x ^ (y * 2654435761u) ^ (z * 805459861u);
+        h.index[c] = i & (MLP_HASH_SIZE - 1u);
+        h.weight[c] = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
+    }
+    return h;
+}
+
+// A level's two features: the trilinear interpolation of its corners.
+__device__ __forceinline__ float2 hash_level(const MlpScene &sc, int level, f3 u) {
+    HashCorners h = hash_corners(sc.res[level], u);
+    const float2 *t = sc.table + (size_t)level * MLP_HASH_SIZE;
+    float a = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float2 e = t[h.index[c]];
+        a = a + h.weight[c] * e.x;
+        b = b + h.weight[c] * e.y;
+    }
+    return make_float2(a, b);
+}
+
+// Features 16..31 of a query: sin and cos of pi u per axis, the normal, the
+// albedo, the constant.
+__device__ __forceinline__ float encode_tail(const MlpScene &sc, const MlpQuery &q, int f) {
     uint32_t face = q.face_material & 0xffu, material = (q.face_material >> 8) & 0xffu;
     f3 n = face_normal(face);
-    if (f < 24) {
-        int axis = f >> 3, i = (f & 7) >> 1;
-        float c = axis == 0 ? (float)q.vx + 0.5f + n.x * 0.5f : axis == 1 ? (float)q.vy + 0.5f + n.y * 0.5f
-                                                                          : (float)q.vz + 0.5f + n.z * 0.5f;
-        float lo = axis == 0 ? sc.lo_x : axis == 1 ? sc.lo_y : sc.lo_z;
-        float u = (c - lo) * sc.inv_extent;
-        float x = u * (3.14159265358979f * (float)(1 << (2 * i)));
+    if (f < 22) {
+        f3 u = query_position(sc, q, n);
+        float x = get(u, (f - 16) >> 1) * 3.14159265358979f;
         float sv, cv;
         sincos_own(x, sv, cv);
         return (f & 1) ? cv : sv;
     }
-    if (f < 27) return get(n, f - 24);
-    if (f < 30) {
+    if (f < 25) return get(n, f - 22);
+    if (f < 28) {
         float4 d = sc.materials[material * 2u];
-        return f == 27 ? d.x : f == 28 ? d.y : d.z;
+        return f == 25 ? d.x : f == 26 ? d.y : d.z;
     }
-    return f == 30 ? 1.0f : 0.0f;
+    return f == 28 ? 1.0f : 0.0f;
 }
 
 // Forward of one 16-sample tile held by a wavefront. x0: input fragments;
@@ -143,14 +185,39 @@ __device__ __forceinline__ f32x4 mlp_forward_tile(const s16x4 *__restrict__ lds,
     return acc[0];
 }
 
+// The lane group g of a wavefront's 16-sample tile holds input rows 4g..4g+3
+// (hash levels 2g and 2g + 1) and 16 + 4g..16 + 4g + 3.
 __device__ __forceinline__ void load_input(const MlpScene &sc, const MlpQuery &q, bool valid, s16x4 x0[2]) {
     const int g = (threadIdx.x & 63) >> 4;
+    float2 a = make_float2(0.0f, 0.0f), b = make_float2(0.0f, 0.0f);
+    if (valid) {
+        f3 u = query_position(sc, q, face_normal(q.face_material & 0xffu));
+        a = hash_level(sc, 2 * g, u);
+        b = hash_level(sc, 2 * g + 1, u);
+    }
+    x0[0] = pack4(a.x, a.y, b.x, b.y);
+    float v[4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        float v[4];
+    for (int j = 0; j < 4; ++j) v[j] = valid ? encode_tail(sc, q, 16 + 4 * g + j) : 0.0f;
+    x0[1] = pack4(v[0], v[1], v[2], v[3]);
+}
+
+// Adds dL/d(feature) of a lane's two hash levels (rows 4g..4g+3 of dX0) into
+// the table gradient, spread over each level's corners by their weights.
+__device__ __forceinline__ void scatter_hash_grad(const MlpScene &sc, const MlpQuery &q, f32x4 dx, float *__restrict__ table_grad) {
+    const int g = (threadIdx.x & 63) >> 4;
+    f3 u = query_position(sc, q, face_normal(q.face_material & 0xffu));
+    float d[4] = {dx.x, dx.y, dx.z, dx.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = valid ? encode(sc, q, 16 * s + 4 * g + j) : 0.0f;
-        x0[s] = pack4(v[0], v[1], v[2], v[3]);
+    for (int k = 0; k < 2; ++k) {
+        int level = 2 * g + k;
+        HashCorners h = hash_corners(sc.res[level], u);
+        float *t = table_grad + (size_t)level * MLP_HASH_SIZE * 2u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            atomicAdd(&t[h.index[c] * 2u], h.weight[c] * d[2 * k]);
+            atomicAdd(&t[h.index[c] * 2u + 1u], h.weight[c] * d[2 * k + 1]);
+        }
     }
 }
 
@@ -200,7 +267,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
                                                           const MlpQuery *__restrict__ queries, const float *__restrict__ targets,
                                                           uint32_t count, uint16_t *__restrict__ acts_out,
                                                           uint16_t *__restrict__ grads_out, float *__restrict__ loss,
-                                                          float *__restrict__ pred) {
+                                                          float *__restrict__ pred, float *__restrict__ table_grad) {
     __shared__ s16x4 lds[MLP_FRAG * 64];
     stage_weights(frags, lds, MLP_FRAG);
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
@@ -267,6 +334,12 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
             }
         }
     }
+    // dX0 rows 0..15 = (W1^T)[0..15] dY1: the hash features' gradient, lane
+    // group g holding the rows of the levels it encoded.
+    f32x4 dx = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) dx = mfma(lds[bwd_tile(1, 0, s) * 64 + lane], dy[s], dx);
+    if (valid) scatter_hash_grad(sc, q, dx, table_grad);
 }
 
 // dW_i[o][in] = sum_n dY_i[n][o] X_i[n][in]: one wavefront per (weight tile,
@@ -327,6 +400,26 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ m, fl
     g[i] = 0.0f;
 }
 
+// Adam over the hash table (eps 1e-15, as the hash-grid encoding is trained
+// in its paper); clears the gradient.
+__global__ void mlp_table_adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, float *__restrict__ g,
+                                      float lr, float c1, float c2) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= MLP_TABLE) return;
+    float gi = g[i];
+    float mi = 0.9f * m[i] + 0.1f * gi;
+    float vi = 0.99f * v[i] + 0.01f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-15f);
+    g[i] = 0.0f;
+}
+
+void mlp_table_adam(float *table, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream) {
+    float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
+    hipLaunchKernelGGL(mlp_table_adam_kernel, dim3((MLP_TABLE + 255u) / 256u), dim3(256), 0, stream, table, m, v, grad, lr, c1, c2);
+}
+
 void mlp_pack(const float *master, uint16_t *frags, hipStream_t stream) {
     hipLaunchKernelGGL(mlp_pack_kernel, dim3((MLP_FRAG * 256 + 255) / 256), dim3(256), 0, stream, master, frags);
 }
@@ -338,11 +431,12 @@ void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *que
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, stream, frags, scene, queries, count, pred);
 }
 void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, const float *targets,
-                          uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, hipStream_t stream) {
+                          uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, float *table_grad,
+                          hipStream_t stream) {
     if (count == 0) return;
     uint32_t tiles = count / 16u;
     hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((tiles + 3u) / 4u), dim3(256), 0, stream, frags, scene, queries, targets, count,
-                       acts, grads, loss, pred);
+                       acts, grads, loss, pred, table_grad);
 }
 void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream) {
     if (count == 0) return;

@@ -255,8 +255,9 @@ int vgi_converged(vgi_tracer *tracer, uint32_t launches, uint32_t first_sequence
 
 /* ---- learned radiance cache (no counterpart in the reference): a
  * 32-64-64-64-64-3 ReLU MLP (BF16 MFMA, FP32 accumulation, Adam) that predicts
- * a face's incident radiance from a 32-dim frequency encoding of its centre,
- * its normal and albedo. With it on, bound-face frames end every path at its
+ * a face's incident radiance from a 32-dim encoding of its centre (a trainable
+ * multiresolution hash grid, 8 levels x 2 features, and sin/cos of pi u per
+ * axis), its normal and albedo. With it on, bound-face frames end every path at its
  * first hit into the network (the world-cache termination rule of
  * voxel_path.wgsl:459-463 with the prediction as the cached value) and train
  * it on `train_per_frame` plain multi-bounce samples per frame in `steps`
@@ -272,6 +273,12 @@ int vgi_mlp_infer(vgi_tracer *tracer, const uint32_t *queries, uint32_t count, f
  * Adam step. */
 int vgi_mlp_train(vgi_tracer *tracer, const uint32_t *queries, const float *targets, uint32_t count, int update,
                   float *grad_out, float *loss_out);
+/* The hash grid: read (out) and/or write (in) its 4,194,304 FP32 entries
+ * (level-major, 2^18 entries of 2 features per level), and copy the table
+ * gradient of the last vgi_mlp_train without update (grad_out). */
+int vgi_mlp_table(vgi_tracer *tracer, float *out, const float *in, float *grad_out);
+/* The hash table's Adam learning rate (ten times the network's by default). */
+int vgi_mlp_table_lr(vgi_tracer *tracer, float lr);
 /* Training-only frames on the bound faces (targets traced, Adam steps taken,
  * bound faces not traced): the network's warm-up. */
 int vgi_mlp_pretrain(vgi_tracer *tracer, uint32_t frames);

@@ -2127,19 +2127,84 @@ static inline float bf16_round(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return bitsf((u >> 16) << 16);
 }
+// Exact mode (a test hook): no BF16 rounding anywhere and the hash features
+// summed in double, so the gradient can be checked against central finite
+// differences of the double-precision forward.
+static bool mlp_exact = false;
+static double mlp_round(double x) { return mlp_exact ? x : (double)bf16_round((float)x); }
 static const int MLP_OFF[6] = {0, 2048, 6144, 10240, 14336, 14528};
 static const int MLP_INS[5] = {32, 64, 64, 64, 64};
 static const int MLP_OUTS[5] = {64, 64, 64, 64, 3};
 
-static float mlp_encode(const uint32_t *q, int f, const float *diffuse, const float scene[4]) {
+// The encoding's hash grid (Mueller et al. 2022): 8 levels of 2 features,
+// level l spanning the largest extent with floor(16 b^l) cells, b =
+// (extent / 16)^(1/7); a level whose (res + 2)^3 corners fit its 2^18 entries
+// indexes them densely, finer ones by x ^ 2654435761 y ^ 805459861 z mod 2^18.
+static const int HASH_LEVELS = 8;
+static const uint32_t HASH_SIZE = 1u << 18;
+static const size_t HASH_TABLE = (size_t)HASH_LEVELS * HASH_SIZE * 2;
+
+struct MlpGrid {
+    const float *table;  // level-major, HASH_SIZE entries of 2 floats per level
+    uint32_t res[HASH_LEVELS];
+};
+
+static MlpGrid mlp_grid(const float *table, const float scene[4]) {
+    MlpGrid g;
+    g.table = table;
+    int extent = (int)std::lround(1.0 / (double)scene[3]);
+    double top = extent > 16 ? (double)extent : 16.0;
+    for (int l = 0; l < HASH_LEVELS; ++l)
+        g.res[l] = (uint32_t)std::floor(16.0 * std::exp((double)l * std::log(top / 16.0) / (double)(HASH_LEVELS - 1)) + 1e-3);
+    return g;
+}
+
+// The face centre in scene units.
+static F3 mlp_position(const uint32_t *q, const float scene[4]) {
+    F3 n = face_normal(q[3] & 0xffu);
+    float u[3];
+    for (int a = 0; a < 3; ++a) u[a] = ((float)(int32_t)q[a] + 0.5f + n[a] * 0.5f - scene[a]) * scene[3];
+    return f3(u[0], u[1], u[2]);
+}
+
+// A level's 8 corner entries and trilinear weights (corner c = x + 2y + 4z).
+static void mlp_corners(uint32_t res, F3 u, uint32_t index[8], float weight[8]) {
+    float px = u.x * (float)res, py = u.y * (float)res, pz = u.z * (float)res;
+    float fx = std::floor(px), fy = std::floor(py), fz = std::floor(pz);
+    float wx = px - fx, wy = py - fy, wz = pz - fz;
+    uint32_t x0 = (uint32_t)(int)fx, y0 = (uint32_t)(int)fy, z0 = (uint32_t)(int)fz;
+    const uint32_t side = res + 2u;
+    const bool dense = (uint64_t)side * side * side <= HASH_SIZE;
+    for (int c = 0; c < 8; ++c) {
+        uint32_t x = x0 + (uint32_t)(c & 1), y = y0 + (uint32_t)((c >> 1) & 1), z = z0 + (uint32_t)(c >> 2);
+        uint32_t i = dense ? x + side * (y + side * z) : x ^ (y * 2654435761u) ^ (z * 805459861u);
+        index[c] = i & (HASH_SIZE - 1u);
+        weight[c] = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
+    }
+}
+
+static double mlp_hash_feature(const MlpGrid &g, int level, int feature, F3 u) {
+    uint32_t index[8];
+    float weight[8];
+    mlp_corners(g.res[level], u, index, weight);
+    const float *t = g.table + (size_t)level * HASH_SIZE * 2;
+    if (mlp_exact) {
+        double d = 0.0;
+        for (int c = 0; c < 8; ++c) d += (double)weight[c] * (double)t[index[c] * 2u + (uint32_t)feature];
+        return d;
+    }
+    float a = 0.0f;
+    for (int c = 0; c < 8; ++c) a = a + weight[c] * t[index[c] * 2u + (uint32_t)feature];
+    return (double)a;
+}
+
+static double mlp_encode(const uint32_t *q, int f, const float *diffuse, const float scene[4], const MlpGrid &g) {
     uint32_t face = q[3] & 0xffu, material = (q[3] >> 8) & 0xffu;
     F3 n = face_normal(face);
-    if (f < 24) {
-        int axis = f >> 3, i = (f & 7) >> 1;
-        float v = (float)(int32_t)q[axis];
-        float c = v + 0.5f + n[axis] * 0.5f;
-        float u = (c - scene[axis]) * scene[3];
-        float x = u * (3.14159265358979f * (float)(1 << (2 * i)));
+    F3 u = mlp_position(q, scene);
+    if (f < 16) return mlp_hash_feature(g, f >> 1, f & 1, u);
+    if (f < 22) {
+        float x = u[(f - 16) >> 1] * 3.14159265358979f;
         int k;
         float r = reduce_pio2(x, &k);
         float sp = sin_poly(r), cp = cos_poly(r);
@@ -2152,9 +2217,9 @@ static float mlp_encode(const uint32_t *q, int f, const float *diffuse, const fl
         }
         return (f & 1) ? cv : sv;
     }
-    if (f < 27) return n[f - 24];
-    if (f < 30) return diffuse[material * 3 + (f - 27)];
-    return f == 30 ? 1.0f : 0.0f;
+    if (f < 25) return n[f - 22];
+    if (f < 28) return diffuse[material * 3 + (f - 25)];
+    return f == 28 ? 1.0f : 0.0f;
 }
 
 struct MlpTrace {
@@ -2163,9 +2228,10 @@ struct MlpTrace {
     double out[3];
 };
 
-static void mlp_forward(const std::vector<float> &wb, const uint32_t *q, const float *diffuse, const float scene[4], MlpTrace &tr) {
+static void mlp_forward(const std::vector<float> &wb, const uint32_t *q, const float *diffuse, const float scene[4], const MlpGrid &g,
+                        MlpTrace &tr) {
     bool valid = q[3] != NONE;
-    for (int f = 0; f < 32; ++f) tr.x[0][f] = valid ? (double)bf16_round(mlp_encode(q, f, diffuse, scene)) : 0.0;
+    for (int f = 0; f < 32; ++f) tr.x[0][f] = valid ? mlp_round(mlp_encode(q, f, diffuse, scene, g)) : 0.0;
     for (int l = 0; l < 5; ++l) {
         int in = MLP_INS[l], out = MLP_OUTS[l];
         for (int o = 0; o < out; ++o) {
@@ -2174,7 +2240,7 @@ static void mlp_forward(const std::vector<float> &wb, const uint32_t *q, const f
             tr.pre[l][o] = a;
         }
         if (l < 4)
-            for (int o = 0; o < out; ++o) tr.x[l + 1][o] = (double)bf16_round(tr.pre[l][o] > 0.0 ? (float)tr.pre[l][o] : 0.0f);
+            for (int o = 0; o < out; ++o) tr.x[l + 1][o] = tr.pre[l][o] > 0.0 ? mlp_round(tr.pre[l][o]) : 0.0;
     }
     for (int c = 0; c < 3; ++c) tr.out[c] = tr.pre[4][c];
 }
@@ -2820,42 +2886,46 @@ int64_t orc_last_event(void *tp, uint32_t *out, int64_t capacity) {
 
 uint32_t orc_curve_limit(uint32_t d2) { return curve_limit(d2); }
 
-int orc_mlp_forward(const float *w, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out) {
+int orc_mlp_forward(const float *w, const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4],
+                    double *out) {
     std::vector<float> wb(14528);
-    for (int i = 0; i < 14528; ++i) wb[i] = bf16_round(w[i]);
+    for (int i = 0; i < 14528; ++i) wb[i] = mlp_exact ? w[i] : bf16_round(w[i]);
+    MlpGrid g = mlp_grid(table, scene);
     for (uint32_t i = 0; i < n; ++i) {
         MlpTrace tr;
-        mlp_forward(wb, &queries[i * 4], diffuse, scene, tr);
+        mlp_forward(wb, &queries[i * 4], diffuse, scene, g, tr);
         for (int c = 0; c < 3; ++c) out[i * 3 + c] = queries[i * 4 + 3] == NONE ? 0.0 : tr.out[c];
     }
     return 0;
 }
 
-int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
-                     const float scene[4], double *grad, double *loss) {
+int orc_mlp_gradient(const float *w, const float *table, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
+                     const float scene[4], double *grad, double *table_grad, double *loss) {
     std::vector<float> wb(14528);
-    for (int i = 0; i < 14528; ++i) wb[i] = bf16_round(w[i]);
+    for (int i = 0; i < 14528; ++i) wb[i] = mlp_exact ? w[i] : bf16_round(w[i]);
     for (int i = 0; i < 14528; ++i) grad[i] = 0.0;
+    for (size_t i = 0; i < HASH_TABLE; ++i) table_grad[i] = 0.0;
+    MlpGrid g = mlp_grid(table, scene);
     *loss = 0.0;
     for (uint32_t i = 0; i < n; ++i) {
         MlpTrace tr;
-        mlp_forward(wb, &queries[i * 4], diffuse, scene, tr);
+        mlp_forward(wb, &queries[i * 4], diffuse, scene, g, tr);
         const float *y = &targets[i * 4];
         double wgt = (double)y[3] / (double)n;
         double dy[5][64];
         for (int c = 0; c < 3; ++c) {
-            double p = (double)(float)tr.out[c];
+            double p = mlp_exact ? tr.out[c] : (double)(float)tr.out[c];
             double den = p * p + 0.01;
             double r = p - (double)y[c];
             *loss += wgt * r * r / den;
-            dy[4][c] = (double)bf16_round((float)(wgt * 2.0 * r / den));
+            dy[4][c] = mlp_round(wgt * 2.0 * r / den);
         }
         for (int l = 4; l >= 1; --l) {
             int in = MLP_INS[l], out = MLP_OUTS[l];
             for (int k = 0; k < in; ++k) {
                 double a = 0.0;
                 for (int o = 0; o < out; ++o) a += (double)wb[MLP_OFF[l] + o * in + k] * dy[l][o];
-                dy[l - 1][k] = (double)bf16_round(tr.pre[l - 1][k] > 0.0 ? (float)a : 0.0f);
+                dy[l - 1][k] = tr.pre[l - 1][k] > 0.0 ? mlp_round(a) : 0.0;
             }
         }
         for (int l = 0; l < 5; ++l) {
@@ -2863,7 +2933,48 @@ int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targe
             for (int o = 0; o < out; ++o)
                 for (int k = 0; k < in; ++k) grad[MLP_OFF[l] + o * in + k] += dy[l][o] * tr.x[l][k];
         }
+        // The hash features' gradient W1^T dY1, spread over each level's
+        // corners by their weights.
+        if (queries[i * 4 + 3] == NONE) continue;
+        F3 u = mlp_position(&queries[i * 4], scene);
+        for (int level = 0; level < HASH_LEVELS; ++level) {
+            uint32_t index[8];
+            float weight[8];
+            mlp_corners(g.res[level], u, index, weight);
+            for (int f = 0; f < 2; ++f) {
+                int k = 2 * level + f;
+                double dx = 0.0;
+                for (int o = 0; o < 64; ++o) dx += (double)wb[MLP_OFF[0] + o * 32 + k] * dy[0][o];
+                for (int c = 0; c < 8; ++c) table_grad[((size_t)level * HASH_SIZE + index[c]) * 2 + (size_t)f] += (double)weight[c] * dx;
+            }
+        }
     }
+    return 0;
+}
+
+// The hash-grid features (16 per query, before BF16 rounding) and each
+// level's corner entries and weights: test hooks for the encoding's pins.
+int orc_mlp_hash(const float *table, const uint32_t *queries, uint32_t n, const float scene[4], float *features, uint32_t *index,
+                 float *weight) {
+    MlpGrid g = mlp_grid(table, scene);
+    for (uint32_t i = 0; i < n; ++i) {
+        F3 u = mlp_position(&queries[i * 4], scene);
+        for (int level = 0; level < HASH_LEVELS; ++level) {
+            for (int f = 0; f < 2; ++f) features[i * 16 + level * 2 + f] = (float)mlp_hash_feature(g, level, f, u);
+            mlp_corners(g.res[level], u, &index[(i * HASH_LEVELS + level) * 8], &weight[(i * HASH_LEVELS + level) * 8]);
+        }
+    }
+    return 0;
+}
+
+int orc_mlp_exact(int on) {
+    mlp_exact = on != 0;
+    return 0;
+}
+
+int orc_mlp_levels(const float scene[4], uint32_t *res) {
+    MlpGrid g = mlp_grid(nullptr, scene);
+    for (int l = 0; l < HASH_LEVELS; ++l) res[l] = g.res[l];
     return 0;
 }
 

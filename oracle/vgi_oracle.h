@@ -175,10 +175,18 @@ int orc_trace_ray(void *tracer, const float origin[3], const float direction[3],
  * 14,528 FP32 master values (W1 64x32, W2..W4 64x64, W5 3x64 row-major);
  * queries 4 u32 (voxel xyz, face | material << 8); scene = (lo xyz, 1/extent);
  * targets 4 floats (rgb, weight). ---- */
-int orc_mlp_forward(const float *w, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4],
+int orc_mlp_forward(const float *w, const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4],
                     double *out);
-int orc_mlp_gradient(const float *w, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
-                     const float scene[4], double *grad, double *loss);
+int orc_mlp_gradient(const float *w, const float *table, const uint32_t *queries, const float *targets, uint32_t n, const float *diffuse,
+                     const float scene[4], double *grad, double *table_grad, double *loss);
+/* The hash-grid encoding alone: 16 features per query, 8 corner entries and
+ * weights per level; the cells per level for a scene. */
+int orc_mlp_hash(const float *table, const uint32_t *queries, uint32_t n, const float scene[4], float *features, uint32_t *index,
+                 float *weight);
+int orc_mlp_levels(const float scene[4], uint32_t *res);
+/* Exact mode on/off: no BF16 rounding and double hash features (the gradient
+ * is then the exact derivative of the double-precision forward). */
+int orc_mlp_exact(int on);
 
 #ifdef __cplusplus
 }

@@ -92,8 +92,11 @@ def lib():
         "orc_select_threshold": (ctypes.c_int, [P, u32, P, P]),
         "orc_cache_set_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_trace_ray": (ctypes.c_int, [P, P, P, f32, P, P]),
-        "orc_mlp_forward": (ctypes.c_int, [P, P, u32, P, P, P]),
-        "orc_mlp_gradient": (ctypes.c_int, [P, P, P, u32, P, P, P, P]),
+        "orc_mlp_forward": (ctypes.c_int, [P, P, P, u32, P, P, P]),
+        "orc_mlp_gradient": (ctypes.c_int, [P, P, P, P, u32, P, P, P, P, P]),
+        "orc_mlp_hash": (ctypes.c_int, [P, P, u32, P, P, P, P]),
+        "orc_mlp_levels": (ctypes.c_int, [P, P]),
+        "orc_mlp_exact": (ctypes.c_int, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -369,23 +372,54 @@ def select_threshold(histograms, picks):
     return thr, rem
 
 
-def mlp_forward(weights, queries, diffuse, scene):
+MLP_TABLE = 8 * (1 << 18) * 2
+
+
+def mlp_forward(weights, table, queries, diffuse, scene):
     w = np.ascontiguousarray(weights, np.float32)
+    t = np.ascontiguousarray(table, np.float32)
     q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
     d = np.ascontiguousarray(diffuse, np.float32)
     sc = np.ascontiguousarray(scene, np.float32)
     out = np.zeros((len(q), 3), np.float64)
-    lib().orc_mlp_forward(ptr(w), ptr(q), len(q), ptr(d), ptr(sc), ptr(out))
+    lib().orc_mlp_forward(ptr(w), ptr(t), ptr(q), len(q), ptr(d), ptr(sc), ptr(out))
     return out
 
 
-def mlp_gradient(weights, queries, targets, diffuse, scene):
+def mlp_gradient(weights, table, queries, targets, diffuse, scene):
+    """(weight gradient, hash-table gradient, loss)."""
     w = np.ascontiguousarray(weights, np.float32)
+    t = np.ascontiguousarray(table, np.float32)
     q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
     y = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
     d = np.ascontiguousarray(diffuse, np.float32)
     sc = np.ascontiguousarray(scene, np.float32)
     grad = np.zeros(14528, np.float64)
+    table_grad = np.zeros(MLP_TABLE, np.float64)
     loss = np.zeros(1, np.float64)
-    lib().orc_mlp_gradient(ptr(w), ptr(q), ptr(y), len(q), ptr(d), ptr(sc), ptr(grad), ptr(loss))
-    return grad, float(loss[0])
+    lib().orc_mlp_gradient(ptr(w), ptr(t), ptr(q), ptr(y), len(q), ptr(d), ptr(sc), ptr(grad), ptr(table_grad), ptr(loss))
+    return grad, table_grad, float(loss[0])
+
+
+def mlp_hash(table, queries, scene):
+    """Hash-grid features (n x 16) and per level the corner entries and
+    weights (n x 8 x 8 each)."""
+    t = np.ascontiguousarray(table, np.float32)
+    q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+    sc = np.ascontiguousarray(scene, np.float32)
+    feats = np.zeros((len(q), 16), np.float32)
+    index = np.zeros((len(q), 8, 8), np.uint32)
+    weight = np.zeros((len(q), 8, 8), np.float32)
+    lib().orc_mlp_hash(ptr(t), ptr(q), len(q), ptr(sc), ptr(feats), ptr(index), ptr(weight))
+    return feats, index, weight
+
+
+def mlp_exact(on):
+    lib().orc_mlp_exact(1 if on else 0)
+
+
+def mlp_levels(scene):
+    sc = np.ascontiguousarray(scene, np.float32)
+    res = np.zeros(8, np.uint32)
+    lib().orc_mlp_levels(ptr(sc), ptr(res))
+    return res

@@ -43,8 +43,12 @@ def test_mlp_forward_matches_fp64_oracle(vgi, orc):
     q = random_queries(vgi, setup, 4096, rng)
     q[::97, 3] = 0xFFFFFFFF  # invalid queries answer 0
     w = g.mlp_weights()
+    # A hash table far from its initialisation, so the grid features count.
+    table = rng.uniform(-0.5, 0.5, vgi.MLP_TABLE).astype(np.float32)
+    g.mlp_table(table)
+    assert np.array_equal(g.mlp_table(), table)
     gpu = g.mlp_infer(q)
-    cpu = orc.mlp_forward(w, q, setup["diffuse"], scene_of(g))
+    cpu = orc.mlp_forward(w, table, q, setup["diffuse"], scene_of(g))
     scale = np.abs(cpu).mean()
     assert scale > 1e-3
     err = np.abs(gpu - cpu).max() / scale
@@ -64,8 +68,11 @@ def test_mlp_gradient_matches_fp64_oracle(vgi, orc):
     targets[:, :3] = rng.random((512, 3)).astype(np.float32) * 2.0
     targets[:, 3] = (rng.random(512) > 0.1).astype(np.float32)
     w = g.mlp_weights()
+    table = rng.uniform(-0.5, 0.5, vgi.MLP_TABLE).astype(np.float32)
+    g.mlp_table(table)
     grad, loss = g.mlp_train(q, targets, update=False)
-    ref, ref_loss = orc.mlp_gradient(w, q, targets, setup["diffuse"], scene_of(g))
+    table_grad = g.mlp_table_grad()
+    ref, ref_table, ref_loss = orc.mlp_gradient(w, table, q, targets, setup["diffuse"], scene_of(g))
     print("loss gpu %.6f oracle %.6f" % (loss, ref_loss), flush=True)
     assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss) + 1e-6
     for a, b in LAYERS:
@@ -73,9 +80,19 @@ def test_mlp_gradient_matches_fp64_oracle(vgi, orc):
         cos = np.dot(grad[a:b], ref[a:b]) / (np.linalg.norm(grad[a:b]) * np.linalg.norm(ref[a:b]) + 1e-30)
         print("layer [%d,%d): relative error %.3g cosine %.6f" % (a, b, e, cos), flush=True)
         assert e < 2e-2 and cos > 0.9995
-    # A second identical call returns the same gradient (the buffer is cleared).
+    # The hash table's gradient, per level.
+    size = 1 << 19
+    for level in range(8):
+        a, b = table_grad[level * size:(level + 1) * size], ref_table[level * size:(level + 1) * size]
+        e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+        print("hash level %d: relative error %.3g cosine %.6f, %d entries" % (level, e, cos, int((b != 0).sum())), flush=True)
+        assert e < 2e-2 and cos > 0.9995
+        assert np.all(a[b == 0] == 0), "only the entries the queries touch get gradient"
+    # A second identical call returns the same gradient (the buffers are cleared).
     grad2, _ = g.mlp_train(q, targets, update=False)
     assert np.allclose(grad, grad2, rtol=1e-5, atol=1e-9)
+    assert np.allclose(g.mlp_table_grad(), table_grad, rtol=1e-4, atol=1e-9)
     g.close()
 
 
