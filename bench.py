@@ -28,8 +28,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
-# Learned-cache training per frame: one-sample training paths, Adam steps.
-LEARNED_TRAIN = (131072, 16)
+# Learned-cache configuration: training targets per frame, Adam steps per
+# frame, paths averaged into each target; rays per face of a learned frame.
+LEARNED_TRAIN = (131072, 8, 16)
+LEARNED_SPP = 1
 
 WORKLOADS = {
     # name: (generator id, seed, edge, bounces, spp, description)
@@ -127,13 +129,13 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     ordered = np.concatenate([faces[judged_idx], faces[rest]])
     judged = len(judged_idx)
 
-    def tracer():
+    def tracer(samples=spp):
         g = vgi.GpuTracer(0)
         g.enable_resident()
         g.upload_pages(coords, pages)
         g.set_materials(diffuse, emission)
         g.set_lights(lights)
-        g.set_transport(bounces=bounces, samples=spp, environment=tuple(env), plain=True)
+        g.set_transport(bounces=bounces, samples=samples, environment=tuple(env), plain=True)
         return g
 
     t0 = time.time()
@@ -160,9 +162,10 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     g.close()
     legs["plain"] = (pts, 0.0)
     print("plain: %d frames, %.0f GPU ms, rmse %.4f" % pts[-1], file=sys.stderr, flush=True)
-    g = tracer()
+    g = tracer(LEARNED_SPP)
     g.bind_faces(ordered)
     g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
+    g.mlp_train_samples(LEARNED_TRAIN[2])
     t1 = time.perf_counter()
     g.mlp_pretrain(args.pretrain_frames)
     g.sync()
@@ -171,7 +174,7 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     g.close()
     legs["learned"] = (pts, pretrain_ms)
     print("learned: %d frames, %.0f GPU ms, rmse %.4f (pretrain %.0f ms)" % (pts[-1] + (pretrain_ms,)), file=sys.stderr, flush=True)
-    budgets = [1000.0, 3000.0, 10000.0]
+    budgets = [1000.0, 3000.0, 10000.0, 20000.0]
     for name, (pts, pre_ms) in legs.items():
         reached = pts[-1][2] <= args.target_rmse
         frames = pts[-1][0]
@@ -187,8 +190,10 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
         if name == "learned":
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
-            entry["training"] = ("%dk one-sample 3-bounce targets per frame, %d Adam steps (lr 1e-3, hash table 1e-2), BF16 MFMA; "
-                                 "encoding: 8-level hash grid + sin/cos(pi u), normal, albedo" % (LEARNED_TRAIN[0] // 1024, LEARNED_TRAIN[1]))
+            entry["rays_per_face_per_frame"] = LEARNED_SPP
+            entry["training"] = ("%dk 3-bounce targets per frame of %d paths each, %d Adam steps (lr 1e-3, hash table 1e-2), BF16 MFMA; "
+                                 "encoding: 8-level hash grid + sin/cos(pi u), normal, albedo" % (
+                                     LEARNED_TRAIN[0] // 1024, LEARNED_TRAIN[2], LEARNED_TRAIN[1]))
         out[name] = entry
     return out
 
@@ -430,7 +435,7 @@ def main():
     ap.add_argument("--target-rmse", type=float, default=0.02)
     ap.add_argument("--reference-launches", type=int, default=2048)
     ap.add_argument("--plain-frames", type=int, default=900)
-    ap.add_argument("--learned-frames", type=int, default=400)
+    ap.add_argument("--learned-frames", type=int, default=2400)
     ap.add_argument("--pretrain-frames", type=int, default=400)
     ap.add_argument("--no-edit-stream", action="store_true", help="skip the edit-stream leg")
     ap.add_argument("--edit-frames", type=int, default=120)
@@ -485,7 +490,9 @@ def main():
     if args.learned_only:
         # Learned frames alone (for profiling): the cache warmed up on
         # `pretrain_frames` training-only frames, then `steps` timed frames.
+        g.set_transport(bounces=bounces, samples=LEARNED_SPP, environment=tuple(optics[3]), plain=True)
         g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
+        g.mlp_train_samples(LEARNED_TRAIN[2])
         g.mlp_pretrain(args.pretrain_frames)
         for _ in range(args.warmup):
             g.run_frame()
@@ -502,7 +509,7 @@ def main():
                           "frame_gpu_ms": round(float(np.median(frame_ms)), 3),
                           "trace_infer_combine_gpu_ms": round(float(np.median(trace_ms)), 3),
                           "rays_per_frame": rays // args.steps,
-                          "queries_per_frame": int(len(faces) * spp)}))
+                          "queries_per_frame": int(len(faces) * LEARNED_SPP)}))
         return
 
     # The library owns its HIP stream; vgi_sync drains it on both sides of the

@@ -145,6 +145,7 @@ def library():
         "vgi_mlp_train": (ctypes.c_int, [P, P, P, u32, ctypes.c_int, P, P]),
         "vgi_mlp_table": (ctypes.c_int, [P, P, P, P]),
         "vgi_mlp_table_lr": (ctypes.c_int, [P, ctypes.c_float]),
+        "vgi_mlp_train_samples": (ctypes.c_int, [P, ctypes.c_uint32]),
         "vgi_mlp_last_loss": (f32, [P]),
         "vgi_mlp_pretrain": (ctypes.c_int, [P, u32]),
         "vgi_scene_bounds": (ctypes.c_int, [P, P]),
@@ -173,7 +174,7 @@ def exported_symbols():
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
-        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
@@ -565,6 +566,9 @@ class GpuTracer:
 
     def mlp_table_lr(self, lr):
         _check(self._lib.vgi_mlp_table_lr(self._h, lr))
+
+    def mlp_train_samples(self, samples):
+        _check(self._lib.vgi_mlp_train_samples(self._h, samples))
 
     def mlp_table_grad(self):
         out = np.zeros(MLP_TABLE, np.float32)

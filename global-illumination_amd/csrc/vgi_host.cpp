@@ -164,6 +164,7 @@ struct vgi_tracer {
     uint32_t mlp_step = 0, train_per_frame = 65536, train_steps = 4, train_cursor = 0;
     float mlp_lr = 1e-3f, last_loss = 0.0f;
     float table_lr = 1e-2f;  // the hash table's Adam rate
+    uint32_t train_samples = 1;  // paths averaged into each training target
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
     // ---- wavefront path tracing of bound frames
@@ -906,7 +907,12 @@ int learned_targets(vgi_tracer &t, hipStream_t stream) {
     // that hit's direct sun, with one bounce of the budget spent: targets are
     // traced one bounce short.
     o.bounces = o.bounces > 1u ? o.bounces - 1u : 1u;
-    if (trace_bound_faces(t, s, o, t.d_train_tasks.as<uint4>(), total, 1u, 1u, 0x40000000u + t.frame_index,
+    // A target's paths on separate lanes (up to 16, a power of two dividing
+    // its count): a training window is too small to fill the chip one face
+    // per lane.
+    uint32_t lanes = 1u;
+    while (lanes < 16u && t.train_samples % (lanes * 2u) == 0u) lanes *= 2u;
+    if (trace_bound_faces(t, s, o, t.d_train_tasks.as<uint4>(), total, t.train_samples, lanes, 0x40000000u + t.frame_index,
                           t.d_train_results.as<float4>(), nullptr, nullptr, false, stream, nullptr) != 0)
         return -1;
     vgi::launch_train_targets(t.d_train_results.as<float4>(), total, t.d_targets.as<float4>(), stream);
@@ -1726,6 +1732,13 @@ int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint3
     // The table learns ten times faster than the network by default (its
     // entries see few samples each).
     t->table_lr = 10.0f * lr;
+    return 0;
+}
+
+int vgi_mlp_train_samples(vgi_tracer *t, uint32_t samples) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (samples < 1 || samples > 64) return fail("training samples per target must be 1..64");
+    t->train_samples = samples;
     return 0;
 }
 

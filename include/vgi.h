@@ -279,6 +279,8 @@ int vgi_mlp_train(vgi_tracer *tracer, const uint32_t *queries, const float *targ
 int vgi_mlp_table(vgi_tracer *tracer, float *out, const float *in, float *grad_out);
 /* The hash table's Adam learning rate (ten times the network's by default). */
 int vgi_mlp_table_lr(vgi_tracer *tracer, float lr);
+/* Paths averaged into each training target (1..64; 1 by default). */
+int vgi_mlp_train_samples(vgi_tracer *tracer, uint32_t samples);
 /* Training-only frames on the bound faces (targets traced, Adam steps taken,
  * bound faces not traced): the network's warm-up. */
 int vgi_mlp_pretrain(vgi_tracer *tracer, uint32_t frames);
