@@ -13,6 +13,7 @@
 
 #include "vgi.h"
 #include "vgi_host_scene.h"
+#include "vgi_pool.h"
 #include "vgi_kernels.h"
 #include "vgi_mlp.h"
 #include "vgi_wavefront.h"
@@ -459,6 +460,12 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
 
 // The solid regions of a scene as retention events need them: their bounding
 // box and which 32^3 pages hold one (`solid`: one byte per region of the box).
+// The edit path's host threads (up to 8 with the caller), shared by tracers.
+vgi_host::WorkPool &edit_pool() {
+    static vgi_host::WorkPool pool(std::min(std::max(std::thread::hardware_concurrency(), 1u), 8u) - 1u);
+    return pool;
+}
+
 vgi_host::PreviousScene scene_pages(const vgi_tracer &t, const std::vector<uint8_t> &solid, std::vector<uint8_t> &page_solid) {
     vgi_host::PreviousScene prev;
     prev.any_solid = false;
@@ -517,16 +524,19 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     bool events = t.resident;
     std::vector<vgi_host::RegionDelta> deltas;
     std::vector<uint8_t> solid_before;
+    vgi_host::WorkPool &pool = edit_pool();
     if (events) {
         solid_before = t.solid;
         deltas.resize(dirty.size());
-        for (size_t k = 0; k < dirty.size(); ++k) {
-            int ri = dirty[k];
-            Int3 o = region_origin(ri);
-            deltas[k].coord = Int3{o.x / 16, o.y / 16, o.z / 16};
-            copy_region(ri, deltas[k].before);
-            deltas[k].solid_before = t.solid[ri] != 0;
-        }
+        pool.run(dirty.size(), [&](size_t begin, size_t end) {
+            for (size_t k = begin; k < end; ++k) {
+                int ri = dirty[k];
+                Int3 o = region_origin(ri);
+                deltas[k].coord = Int3{o.x / 16, o.y / 16, o.z / 16};
+                copy_region(ri, deltas[k].before);
+                deltas[k].solid_before = t.solid[ri] != 0;
+            }
+        });
     }
     // Edits apply in order (a later edit of a voxel wins); the GPU grid gets
     // each edited voxel's final value once.
@@ -562,21 +572,40 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         const uint64_t *m = &t.masks[(size_t)ri * 4];
         return (m[0] & emitting[0]) | (m[1] & emitting[1]) | (m[2] & emitting[2]) | (m[3] & emitting[3]);
     };
-    std::vector<std::pair<std::tuple<int, int, int>, Int3>> flips;
-    for (int ri : dirty) {
+    // An edit next to an emitting region (before or after the edit) changes
+    // the emitter distribution. Regions are rescanned in parallel (each
+    // writes only its own masks and solidity); the emitter test reads the
+    // neighbourhood's masks before and after, in passes of their own.
+    const int nb[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    auto near_emitting = [&](int ri) {
         int x = ri % t.rsx, y = (ri / t.rsx) % t.rsy, z = ri / (t.rsx * t.rsy);
-        bool near_emitter = false;
-        const int nb[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
         for (auto &o : nb) {
             int nx = x + o[0], ny = y + o[1], nz = z + o[2];
             if (nx < 0 || ny < 0 || nz < 0 || nx >= t.rsx || ny >= t.rsy || nz >= t.rsz) continue;
-            if (emits((int)lin3(nx, ny, nz, t.rsx, t.rsy))) near_emitter = true;
+            if (emits((int)lin3(nx, ny, nz, t.rsx, t.rsy))) return true;
         }
-        uint8_t was = t.solid[ri];
-        scan_region(t, ri);
-        if (emits(ri)) near_emitter = true;
-        if (near_emitter) t.emitters_dirty = true;
-        if (was != t.solid[ri]) flips.push_back({std::make_tuple(t.rmin.z + z, t.rmin.y + y, t.rmin.x + x), Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z}});
+        return false;
+    };
+    std::vector<uint8_t> was(dirty.size()), near(dirty.size(), 0);
+    pool.run(dirty.size(), [&](size_t begin, size_t end) {
+        for (size_t k = begin; k < end; ++k) {
+            was[k] = t.solid[dirty[k]];
+            near[k] = near_emitting(dirty[k]) ? 1 : 0;
+        }
+    });
+    pool.run(dirty.size(), [&](size_t begin, size_t end) {
+        for (size_t k = begin; k < end; ++k) scan_region(t, dirty[k]);
+    });
+    pool.run(dirty.size(), [&](size_t begin, size_t end) {
+        for (size_t k = begin; k < end; ++k)
+            if (near_emitting(dirty[k])) near[k] = 1;
+    });
+    std::vector<std::pair<std::tuple<int, int, int>, Int3>> flips;
+    for (size_t k = 0; k < dirty.size(); ++k) {
+        int ri = dirty[k];
+        int x = ri % t.rsx, y = (ri / t.rsx) % t.rsy, z = ri / (t.rsx * t.rsy);
+        if (near[k]) t.emitters_dirty = true;
+        if (was[k] != t.solid[ri]) flips.push_back({std::make_tuple(t.rmin.z + z, t.rmin.y + y, t.rmin.x + x), Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z}});
     }
     // Keep the cells buffer bounded: re-pack everything once appended tables
     // and bricks outgrow four times the last full pack.
@@ -606,19 +635,23 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     if (!events) return 0;
     auto t_event = clock::now();
     // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
+    std::vector<std::vector<vgi_host::Int3>> changed_of(dirty.size());
+    pool.run(dirty.size(), [&](size_t begin, size_t end) {
+        for (size_t k = begin; k < end; ++k) {
+            copy_region(dirty[k], deltas[k].after);
+            deltas[k].solid_after = t.solid[dirty[k]] != 0;
+            const vgi_host::RegionDelta &d = deltas[k];
+            for (int i = 0; i < 4096; ++i)
+                if (d.before[i] != d.after[i])
+                    changed_of[k].push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
+        }
+    });
     std::vector<vgi_host::Int3> changed;
-    for (size_t k = 0; k < dirty.size(); ++k) {
-        copy_region(dirty[k], deltas[k].after);
-        deltas[k].solid_after = t.solid[dirty[k]] != 0;
-        const vgi_host::RegionDelta &d = deltas[k];
-        for (int i = 0; i < 4096; ++i)
-            if (d.before[i] != d.after[i])
-                changed.push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
-    }
+    for (const auto &c : changed_of) changed.insert(changed.end(), c.begin(), c.end());
     std::vector<uint8_t> page_solid;
     vgi_host::PreviousScene prev = scene_pages(t, solid_before, page_solid);
     vgi_host::RetentionEvent ev;
-    if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev)) t.pending_events.push_back(std::move(ev));
+    if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev, &pool)) t.pending_events.push_back(std::move(ev));
     t.timings[6] = ms_since(t_event);
     if (t.follow_exposure && !changed.empty()) {
         auto t_exposure = clock::now();
@@ -1140,9 +1173,16 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
         before = deltas[k].before[(size_t)floor_mod(x, 16) + 16 * ((size_t)floor_mod(y, 16) + 16 * (size_t)floor_mod(z, 16))];
         return before != after;
     };
-    std::vector<uint32_t> removals;
-    std::vector<vgi_face_task> additions, refresh;
-    auto visit = [&](int x, int y, int z, uint32_t f, bool own_changed) {
+    // The visits only read the scene and the face index: contiguous ranges of
+    // the changed voxels run on the pool, each with its own lists, joined in
+    // range order.
+    struct Found {
+        std::vector<uint32_t> removals;
+        std::vector<vgi_face_task> additions, refresh;
+    };
+    auto visit = [&](Found &out, int x, int y, int z, uint32_t f, bool own_changed) {
+        std::vector<uint32_t> &removals = out.removals;
+        std::vector<vgi_face_task> &additions = out.additions, &refresh = out.refresh;
         uint32_t vb, va, nb, na;
         lookup(x, y, z, vb, va);
         lookup(x + FACE_STEP[f][0], y + FACE_STEP[f][1], z + FACE_STEP[f][2], nb, na);
@@ -1161,13 +1201,27 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
         uint32_t b = (slot != vgi_host::NONE && slot < t.bound_of_slot.size()) ? t.bound_of_slot[slot] : vgi_host::NONE;
         if (b != vgi_host::NONE) removals.push_back(b);
     };
-    for (const Int3 &c : changed)
-        for (uint32_t f = 0; f < 6; ++f) {
-            visit(c.x, c.y, c.z, f, true);
-            int nx = c.x + FACE_STEP[f][0], ny = c.y + FACE_STEP[f][1], nz = c.z + FACE_STEP[f][2];
-            uint32_t b, a;
-            if (!lookup(nx, ny, nz, b, a)) visit(nx, ny, nz, f ^ 1u, false);
-        }
+    const size_t per = 2048;
+    std::vector<Found> found((changed.size() + per - 1) / per);
+    edit_pool().run(found.size(), [&](size_t begin, size_t end) {
+        for (size_t chunk = begin; chunk < end; ++chunk)
+            for (size_t i = chunk * per; i < std::min(changed.size(), chunk * per + per); ++i) {
+                const Int3 &c = changed[i];
+                for (uint32_t f = 0; f < 6; ++f) {
+                    visit(found[chunk], c.x, c.y, c.z, f, true);
+                    int nx = c.x + FACE_STEP[f][0], ny = c.y + FACE_STEP[f][1], nz = c.z + FACE_STEP[f][2];
+                    uint32_t b, a;
+                    if (!lookup(nx, ny, nz, b, a)) visit(found[chunk], nx, ny, nz, f ^ 1u, false);
+                }
+            }
+    }, 1);
+    std::vector<uint32_t> removals;
+    std::vector<vgi_face_task> additions, refresh;
+    for (const Found &fnd : found) {
+        removals.insert(removals.end(), fnd.removals.begin(), fnd.removals.end());
+        additions.insert(additions.end(), fnd.additions.begin(), fnd.additions.end());
+        refresh.insert(refresh.end(), fnd.refresh.begin(), fnd.refresh.end());
+    }
     std::vector<uint32_t> refreshed;
     if (register_faces(t, refresh.data(), (uint32_t)refresh.size(), refreshed) != 0) return -1;
     // Removals, highest index first, each filled by the current last face.

@@ -1,5 +1,6 @@
 // Host-side scene bookkeeping (see vgi_host_scene.h).
 #include "vgi_host_scene.h"
+#include "vgi_pool.h"
 
 #include <algorithm>
 #include <cmath>
@@ -473,10 +474,14 @@ bool within_reach(Int3 bmin, Int3 bmax, const PreviousScene &previous) {
 }  // namespace
 
 bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
-                const std::vector<Int3> &edits, RetentionEvent &out) {
+                const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool) {
     out.boxes.clear();
-    std::vector<ChangeBox> boxes;
-    for (const RegionDelta &d : deltas) {
+    // Each region's box, on the pool's threads when given: per-region slots,
+    // then kept in region order.
+    std::vector<ChangeBox> region_box(deltas.size());
+    std::vector<uint8_t> has_box(deltas.size(), 0);
+    auto scan = [&](size_t k) {
+        const RegionDelta &d = deltas[k];
         Int3 origin{d.coord.x * 16, d.coord.y * 16, d.coord.z * 16};
         Int3 tmin{INT32_MAX, INT32_MAX, INT32_MAX}, tmax{INT32_MIN, INT32_MIN, INT32_MIN};
         uint32_t changed = 0;
@@ -510,14 +515,22 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
                            d.coord.z >= previous.solid_min.z && d.coord.x <= previous.solid_max.x && d.coord.y <= previous.solid_max.y &&
                            d.coord.z <= previous.solid_max.z;
         bool transport = d.solid_before || d.solid_after || in_occupied;
-        if (!transport || changed == 0) continue;
-        boxes.push_back(ChangeBox{tmin, tmax, flux, changed, 3u, false});
-    }
-    // Only changes within the influence radius of a page that held a solid
-    // region can touch an existing receiver (from_local_bounds and
-    // from_resident_bounds select the same set).
-    for (const ChangeBox &b : boxes)
-        if (within_reach(b.min, b.max, previous)) out.boxes.push_back(b);
+        if (!transport || changed == 0) return;
+        // Only changes within the influence radius of a page that held a
+        // solid region can touch an existing receiver (from_local_bounds and
+        // from_resident_bounds select the same set).
+        if (!within_reach(tmin, tmax, previous)) return;
+        region_box[k] = ChangeBox{tmin, tmax, flux, changed, 3u, false};
+        has_box[k] = 1;
+    };
+    if (pool)
+        pool->run(deltas.size(), [&](size_t begin, size_t end) {
+            for (size_t k = begin; k < end; ++k) scan(k);
+        });
+    else
+        for (size_t k = 0; k < deltas.size(); ++k) scan(k);
+    for (size_t k = 0; k < deltas.size(); ++k)
+        if (has_box[k]) out.boxes.push_back(region_box[k]);
     if (out.boxes.empty()) return false;
     // Deterministic order and no duplicates (retention.rs:114-125). Two
     // regions never share a tight box, so the weights stay one per box.

@@ -127,8 +127,11 @@ struct PreviousScene {
 // GeometryChange::from_bounds (geometry.rs:299-407) at the influence radius,
 // finalize_change_flux (publication.rs:423-454) and SpatialHistory::event
 // (retention.rs:110-167). Returns false when nothing within reach changed.
+class WorkPool;
+// `pool` (optional) runs the per-region scans on host threads; the result does
+// not depend on it.
 bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
-                const std::vector<Int3> &edits, RetentionEvent &out);
+                const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool = nullptr);
 
 // encode_events (gpu/resident.rs:1319-1417): event headers, then change
 // records; measures are three uint4 per edit whose flux the GPU measures.
