@@ -37,3 +37,32 @@ def test_errors_surface_as_messages():
     lib = vgi.library()
     assert lib.vgi_set_transport(None, None) < 0
     assert b"null" in lib.vgi_last_error()
+
+
+def test_every_entry_point_rejects_a_null_handle():
+    """Every entry point taking a tracer or batch handle returns a negative
+    code and leaves a message when the handle is NULL (no GPU call is made);
+    the float and count queries answer 0."""
+    vgi = load_package()
+    lib = vgi.library()
+    checked = 0
+    for name in vgi.exported_symbols():
+        if name.startswith("vgi_scene_") or name in ("vgi_last_error", "vgi_destroy", "vgi_batch_destroy"):
+            continue
+        fn = getattr(lib, name)
+        args = fn.argtypes or []
+        if not args or args[0] is not ctypes.c_void_p:
+            continue
+        call = [None if (a is ctypes.c_void_p or hasattr(a, "contents") or a.__name__.startswith("LP_")) else 0 for a in args]
+        rc = fn(*call)
+        if fn.restype in (ctypes.c_float,):
+            assert rc == 0.0, name
+        elif fn.restype in (ctypes.c_uint32,):
+            assert rc == 0, name
+        else:
+            assert rc < 0, name
+            assert len(lib.vgi_last_error()) > 0, name
+        checked += 1
+    assert checked > 50
+    lib.vgi_destroy(None)
+    lib.vgi_batch_destroy(None)
