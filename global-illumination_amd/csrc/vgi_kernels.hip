@@ -567,14 +567,34 @@ __global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t c
     if (i >= count) return;
     uint32_t s = slots[i];
     if (s == NONE) return;
-    uint32_t *st = state + (size_t)s * 36u;
-    uint32_t *cp = checkpoint + (size_t)s * 12u;
-    uint32_t *out = reinterpret_cast<uint32_t *>(results + (size_t)i * 6u);
+    // The slot's state and metadata rows and the batch record, held in
+    // registers: whole rows move as 16-byte loads and stores (the rows are
+    // 144, 48 and 96 bytes) instead of word by word.
+    uint4 *srow = reinterpret_cast<uint4 *>(state + (size_t)s * 36u);
+    uint4 *crow = reinterpret_cast<uint4 *>(checkpoint + (size_t)s * 12u);
+    uint4 *mrow = reinterpret_cast<uint4 *>(meta + (size_t)s * 12u);
+    uint4 *orow = reinterpret_cast<uint4 *>(results + (size_t)i * 6u);
+    uint32_t st[36], cp[12], mw[12], out[24];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        uint4 v = srow[k];
+        st[4 * k] = v.x; st[4 * k + 1] = v.y; st[4 * k + 2] = v.z; st[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        uint4 v = orow[k];
+        out[4 * k] = v.x; out[4 * k + 1] = v.y; out[4 * k + 2] = v.z; out[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint4 v = mrow[k];
+        mw[4 * k] = v.x; mw[4 * k + 1] = v.y; mw[4 * k + 2] = v.z; mw[4 * k + 3] = v.w;
+    }
     cp[0] = st[0]; cp[1] = st[1]; cp[2] = st[2]; cp[3] = st[3];
     cp[4] = st[8]; cp[5] = st[9]; cp[6] = st[10]; cp[7] = st[11];
     cp[8] = st[4]; cp[9] = st[5] & 0x7fffffffu; cp[10] = st[12]; cp[11] = 0u;
     uint32_t fresh_count = (uint32_t)__uint_as_float(out[5]);
-    uint32_t *pw = meta + (size_t)s * 12u + 4u;
+    uint32_t *pw = mw + 4;
     float pending = __uint_as_float(*pw);
     if (fresh_count > 0u) *pw = 0u;
     uint32_t previous_count = st[5] & 0x7fffffffu;
@@ -635,13 +655,18 @@ __global__ void accumulate_kernel(const uint32_t *__restrict__ slots, uint32_t c
     st[30] = 0xffffffffu;
     st[7] = st[24];
     st[13] = st[25]; st[14] = st[26]; st[15] = st[27];
-    uint4 *o4 = reinterpret_cast<uint4 *>(out);
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(st);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) o4[k] = s4[k];
-    uint32_t *m = meta + (size_t)s * 12u;
-    record_stability(m, st, before, p);
-    if ((m[5] & 15u) == 12u) out[5] |= 0x80000000u;  // settled_flag
+    for (int k = 0; k < 24; ++k) out[k] = st[k];
+    record_stability(mw, st, before, p);
+    if ((mw[5] & 15u) == 12u) out[5] |= 0x80000000u;  // settled_flag
+#pragma unroll
+    for (int k = 0; k < 9; ++k) srow[k] = make_uint4(st[4 * k], st[4 * k + 1], st[4 * k + 2], st[4 * k + 3]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) crow[k] = make_uint4(cp[4 * k], cp[4 * k + 1], cp[4 * k + 2], cp[4 * k + 3]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mrow[k] = make_uint4(mw[4 * k], mw[4 * k + 1], mw[4 * k + 2], mw[4 * k + 3]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) orow[k] = make_uint4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
 }
 
 // voxel_retention.wgsl:14-18
