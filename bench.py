@@ -514,7 +514,8 @@ def main():
             trace_ms.append(g.last_trace_ms())
             rays += int(g.read_counters_bound()[0])
         g.close()
-        print(json.dumps({"mode": "learned", "workload": desc, "steps": args.steps, "pretrain_frames": args.pretrain_frames,
+        print(json.dumps({"mode": "learned", "workload": desc, "rays_per_face_per_frame": LEARNED_SPP, "steps": args.steps,
+                          "pretrain_frames": args.pretrain_frames,
                           "frame_gpu_ms": round(float(np.median(frame_ms)), 3),
                           "trace_infer_combine_gpu_ms": round(float(np.median(trace_ms)), 3),
                           "rays_per_frame": rays // args.steps,
