@@ -2,7 +2,8 @@
 // loops. run(n, fn) calls fn(begin, end) over chunks of [0, n) on the pool and
 // the calling thread and returns when all are done; callers write per-index
 // results and merge them in index order, so the outcome does not depend on
-// the thread count or the schedule.
+// the thread count or the schedule. Calls from several host threads take
+// turns (the pool serves one job at a time); fn must not call run.
 #pragma once
 #include <algorithm>
 #include <atomic>
@@ -38,6 +39,7 @@ public:
             fn(0, n);
             return;
         }
+        std::lock_guard<std::mutex> turn(turn_);
         {
             std::lock_guard<std::mutex> lock(mutex_);
             job_ = &fn;
@@ -78,6 +80,7 @@ private:
     }
 
     std::vector<std::thread> workers_;
+    std::mutex turn_;  // one job at a time
     std::mutex mutex_;
     std::condition_variable wake_, done_;
     const std::function<void(size_t, size_t)> *job_ = nullptr;
