@@ -32,6 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # frame, paths averaged into each target; rays per face of a learned frame.
 LEARNED_TRAIN = (131072, 8, 16)
 LEARNED_SPP = 1
+# Adam rate of the network (the hash table's is ten times it).
+LEARNED_LR = 3e-3
 
 WORKLOADS = {
     # name: (generator id, seed, edge, bounces, spp, description)
@@ -168,7 +170,7 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     for name, rays in (("learned", LEARNED_SPP),):
         g = tracer(rays)
         g.bind_faces(ordered)
-        g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
+        g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=LEARNED_LR)
         g.mlp_train_samples(LEARNED_TRAIN[2])
         t1 = time.perf_counter()
         g.mlp_pretrain(args.pretrain_frames)
@@ -196,9 +198,9 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
         if name.startswith("learned"):
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
-            entry["training"] = ("%dk 3-bounce targets per frame of %d paths each, %d Adam steps (lr 1e-3, hash table 1e-2), BF16 MFMA; "
+            entry["training"] = ("%dk 3-bounce targets per frame of %d paths each, %d Adam steps (lr %g, hash table %g), BF16 MFMA; "
                                  "encoding: 8-level hash grid + sin/cos(pi u), normal, albedo" % (
-                                     LEARNED_TRAIN[0] // 1024, LEARNED_TRAIN[2], LEARNED_TRAIN[1]))
+                                     LEARNED_TRAIN[0] // 1024, LEARNED_TRAIN[2], LEARNED_TRAIN[1], LEARNED_LR, 10 * LEARNED_LR))
         out[name] = entry
     # The target's terms: the learned form at <= target RMSE, its GPU time per
     # frame against plain 4-bounce tracing's.
@@ -500,7 +502,7 @@ def main():
         # Learned frames alone (for profiling): the cache warmed up on
         # `pretrain_frames` training-only frames, then `steps` timed frames.
         g.set_transport(bounces=bounces, samples=LEARNED_SPP, environment=tuple(optics[3]), plain=True)
-        g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=1e-3)
+        g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=LEARNED_LR)
         g.mlp_train_samples(LEARNED_TRAIN[2])
         g.mlp_pretrain(args.pretrain_frames)
         for _ in range(args.warmup):
