@@ -1718,7 +1718,7 @@ int vgi_converged(vgi_tracer *t, uint32_t launches, uint32_t first_sequence, dou
     vgi::SceneView s;
     vgi::Optics o;
     views(*t, s, o);
-    o.cache_on = 0u;  // the judge of every estimator: plain multi-bounce paths
+    o.cache_on = 0u;  // the reference every estimator is measured against: plain multi-bounce paths
     uint32_t count = t->bound;
     DeviceBuffer sums, results;
     HIP_OK(sums.reserve((size_t)count * 64));

@@ -112,7 +112,7 @@ def test_picks_cover_nearest_first_and_leave_host_jobs(vgi, orc):
     # resident.rs:2066-2190 on this build's API: every face resident and
     # unmeasured; one host batch claims one face; the first four pick batches
     # (budget 1) rank the four quarters while picking at most one face and
-    # never the claimed one; at round 4 coverage goes nearest the focus first
+    # never the claimed one; at batch 4 coverage goes nearest the focus first
     # (log-distance buckets); after enough rounds every face is covered and
     # every picked face reached the statistics mirror.
     setup, tasks, t, g = setup_pair(vgi, orc, edge=64, stride=16)
