@@ -121,9 +121,10 @@ def test_learned_cache_gains_over_plain(vgi, orc):
     with the reference's own noise removed) at equal frame counts, while each
     learned frame traces only to the first hit.
 
-    The network's fitting error leaves a floor under the learned estimate
-    (about 5% here) that the unbiased plain estimate passes after some thirty
-    frames; the gain asserted is the early one, with margin."""
+    With one-path targets and a short warm-up the learned estimate still
+    carries the network's fit error, which the unbiased plain estimate
+    eventually passes; the gain asserted is the early one, with margin (the
+    convergence to 2% is measured by the bench on the 512^3 cave)."""
     setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=128)
     faces = vgi.exposed_faces(setup["grid"])
     sub = faces[::8]
