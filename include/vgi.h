@@ -197,10 +197,13 @@ int vgi_sun_corners(vgi_tracer *tracer, uint32_t frame, uint32_t refresh_frames)
  * and its flux x 64): returns its word count (8,204), copies at most
  * `capacity` words when out is not NULL. */
 int64_t vgi_read_sun_flips(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
-/* The last vgi_sun_corners call: [0] GPU ms of its passes (corner words and
- * the flip compaction), [1] host ms of decoding, binning and the event. */
+/* The last vgi_sun_corners call (as vgi_timings for batches, gpu.rs:486-497):
+ * [0] GPU ms of its passes (corner words and the flip compaction), [1] host
+ * ms of decoding, binning and the event. */
 int vgi_sun_timings(vgi_tracer *tracer, float *out, uint32_t n);
-/* Corner words of resident slots first .. first + count - 1 (0: not traced). */
+/* Corner words of resident slots first .. first + count - 1 (0: not traced;
+ * the view tier keeps the same word per receiver, estimate.identity.z,
+ * voxel_view_runtime.wgsl:175-198). */
 int vgi_read_sun_corners(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
 /* ---- GPU work selection (voxel_accumulate.wgsl:448-695; resident.rs:1093-1164) ---- */
 /* Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
