@@ -156,6 +156,12 @@ struct vgi_tracer {
     uint32_t frame_index = 0;
     bool frame_recorded = false;
     float last_frame_ms = 0.0f;
+    // A bound face's direct terms (results words 16-23) depend only on the
+    // geometry, the lights and the reach: every call that may change one of
+    // them, or the bound set, advances direct_epoch, and a frame whose trace
+    // wrote the terms records the epoch it wrote them at. While the two
+    // agree, frames keep the terms instead of tracing them again.
+    uint64_t direct_epoch = 1, direct_written = 0;
     std::vector<uint32_t> bound_material;  // material of each bound face's voxel
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
@@ -722,6 +728,7 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.cache_state = t.d_state.as<uint32_t>();
     o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
     o.cache_on = (t.resident && !t.transport.plain) ? 1u : 0u;
+    o.direct_reuse = 0u;
 }
 
 // ---------------------------------------------------------------- resident slots
@@ -1356,6 +1363,7 @@ int vgi_set_clock(vgi_tracer *t, int64_t milliseconds) {
 
 int vgi_enable_resident(vgi_tracer *t) {
     if (!t) return fail("null tracer");
+    t->direct_epoch += 1;
     t->resident = true;
     return 0;
 }
@@ -1364,6 +1372,7 @@ int vgi_upload_pages(vgi_tracer *t, const int32_t *page_xyz, uint32_t pages, con
                      uint32_t removed) {
     if (!t) return fail("null tracer");
     if (pages && (!page_xyz || !voxels)) return fail("null page data");
+    t->direct_epoch += 1;
     HIP_OK(hipSetDevice(t->device));
     std::vector<std::pair<Int3, const uint16_t *>> incoming;
     for (uint32_t i = 0; i < pages; ++i)
@@ -1381,12 +1390,14 @@ int vgi_upload_pages(vgi_tracer *t, const int32_t *page_xyz, uint32_t pages, con
 int vgi_edit_voxels(vgi_tracer *t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     if (!t) return fail("null tracer");
     if (count && (!xyz || !values)) return fail("null edit data");
+    t->direct_epoch += 1;
     HIP_OK(hipSetDevice(t->device));
     return apply_edits(*t, xyz, values, count);
 }
 
 int vgi_set_materials(vgi_tracer *t, const float *diffuse_rgb, const float *emission_rgb) {
     if (!t || !diffuse_rgb || !emission_rgb) return fail("null materials");
+    t->direct_epoch += 1;
     HIP_OK(hipSetDevice(t->device));
     std::vector<float> packed(256 * 8);
     for (int m = 0; m < 256; ++m) {
@@ -1417,6 +1428,7 @@ int vgi_set_materials(vgi_tracer *t, const float *diffuse_rgb, const float *emis
 int vgi_set_lights(vgi_tracer *t, const vgi_light *lights, uint32_t count, uint32_t directional) {
     if (!t) return fail("null tracer");
     if (count && !lights) return fail("null lights");
+    t->direct_epoch += 1;
     HIP_OK(hipSetDevice(t->device));
     t->lights.assign((size_t)std::max<uint32_t>(count, 1) * 12, 0.0f);
     for (uint32_t i = 0; i < count; ++i) std::memcpy(&t->lights[i * 12], &lights[i], 48);
@@ -1435,6 +1447,7 @@ int vgi_set_transport(vgi_tracer *t, const vgi_transport *tr) {
     if (tr->lanes != 0 && (tr->lanes > 64 || (64 % tr->lanes) != 0)) return fail("lanes must divide 64");
     t->transport = *tr;
     if (!(t->transport.max_distance > 0.0f)) t->transport.max_distance = 768.0f;
+    t->direct_epoch += 1;
     return 0;
 }
 
@@ -1564,6 +1577,7 @@ int vgi_read_slots(vgi_tracer *t, uint32_t *out, uint32_t count) {
 
 int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
     if (!t || (count && !tasks)) return fail("null argument");
+    t->direct_epoch += 1;
     HIP_OK(hipSetDevice(t->device));
     HIP_OK(t->b_tasks.reserve((size_t)std::max<uint32_t>(count, 1) * 32));
     HIP_OK(hipMemcpyAsync(t->b_tasks.ptr, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
@@ -1648,6 +1662,7 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
     hipEventRecord(t->ev[8], t->stream);
     if (flush_events(*t) != 0) return -1;
+    o.direct_reuse = t->direct_written == t->direct_epoch ? 1u : 0u;
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
@@ -1693,6 +1708,7 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
             return -1;
     }
     HIP_OK(hipGetLastError());
+    t->direct_written = t->direct_epoch;
     hipEventRecord(t->ev[5], t->stream);
     t->frame_recorded = true;
 

@@ -89,8 +89,6 @@ __device__ __forceinline__ void trace_face(SceneView scene, Optics optics, const
         tknown += kn;
     }
     if (valid && lane == 0u) {
-        f3 dv, dc;
-        direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc, tally);
         float inverse = 1.0f / (float)(tknown > 1u ? tknown : 1u);
         float4 *r = results + (size_t)face_index * 6u;
         if constexpr (!kLearned) {
@@ -99,8 +97,12 @@ __device__ __forceinline__ void trace_face(SceneView scene, Optics optics, const
             r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
         }
         r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
-        r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
+        if (!optics.direct_reuse) {
+            f3 dv, dc;
+            direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc, tally);
+            r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+            r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
+        }
     }
 }
 

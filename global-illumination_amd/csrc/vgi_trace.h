@@ -45,6 +45,10 @@ struct Optics {
     const uint32_t *__restrict__ cache_state;  // 36 words per slot
     uint32_t cache_mask;                    // capacity - 1, 0 with no cache
     uint32_t cache_on;
+    // 1: every valid face's results row already holds its direct terms (words
+    // 16-23) for this scene, these lights and this reach, so the trace leaves
+    // them in place instead of tracing the face's light rays again.
+    uint32_t direct_reuse;
 };
 
 // One scene of a batched launch: its scene and optics views and its bound

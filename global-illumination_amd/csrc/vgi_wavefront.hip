@@ -444,18 +444,21 @@ __global__ __launch_bounds__(256) void wf_final_kernel(SceneView scene, Optics o
                 tsecond = tsecond + second;
                 tknown += known;
             }
-            int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
-            f3 normal = face_normal(t0.w);
-            f3 dv, dc;
-            direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc, tally);
             float inverse = 1.0f / (float)(tknown > 1u ? tknown : 1u);
             float4 *r = results + (size_t)f * 6u;
             r[0] = make_float4(tmean.x * inverse, tmean.y * inverse, tmean.z * inverse, 0.0f);
             r[1] = make_float4(0.0f, (float)tknown, 0.0f, 0.0f);
             r[2] = make_float4(tsecond.x * inverse, tsecond.y * inverse, tsecond.z * inverse, 0.0f);
             r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
-            r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
+            if (!optics.direct_reuse) {
+                int vx = (int)t0.x, vy = (int)t0.y, vz = (int)t0.z;
+                f3 normal = face_normal(t0.w);
+                f3 dv, dc;
+                direct_at(scene, optics, mk((float)vx, (float)vy, (float)vz) + mk(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc,
+                          tally);
+                r[4] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+                r[5] = make_float4(0.0f, dc.x, dc.y, dc.z);
+            }
         }
     }
     if (!rays_counter) return;
