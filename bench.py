@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 
 # Learned-cache configuration: training targets per frame, Adam steps per
 # frame, paths averaged into each target; rays per face of a learned frame.
-LEARNED_TRAIN = (131072, 8, 16)
+LEARNED_TRAIN = (81920, 8, 16)
 LEARNED_SPP = 1
 # Adam rate of the network (the hash table's is ten times it).
 LEARNED_LR = 3e-3
