@@ -233,8 +233,12 @@ def edit_stream(vgi, args):
         g.run_frame()
     g.sync()
     edit_ms, build_ms, frame_ms, trace_ms, rays, voxels, phases = [], [], [], [], 0, 0, []
+    # First half: each edit after the previous frame has finished (the host
+    # edit's own time and phases); second half: each edit issued while the
+    # previous frame runs, so its host work overlaps the GPU.
+    half = args.edit_frames // 2
     t_start = time.perf_counter()
-    for f in range(args.edit_frames):
+    for f in range(half):
         xyz, values = edits[f]
         t1 = time.perf_counter()
         g.edit_voxels(xyz, values)
@@ -248,7 +252,19 @@ def edit_stream(vgi, args):
         trace_ms.append(g.last_trace_ms())
         rays += int(g.read_counters_bound()[0])
         voxels += len(values)
-    elapsed = time.perf_counter() - t_start
+    sequential = time.perf_counter() - t_start
+    t_start = time.perf_counter()
+    g.edit_voxels(*edits[half])
+    for f in range(half, args.edit_frames):
+        voxels += len(edits[f][1])
+        g.run_frame(vgi.COUNT_RAYS)
+        if f + 1 < args.edit_frames:
+            g.edit_voxels(*edits[f + 1])
+        g.sync()
+        frame_ms.append(g.last_frame_ms())
+        trace_ms.append(g.last_trace_ms())
+        rays += int(g.read_counters_bound()[0])
+    overlapped = time.perf_counter() - t_start
     faces = g.bound_count()
     g.close()
     return {
@@ -257,7 +273,8 @@ def edit_stream(vgi, args):
         "frames": args.edit_frames,
         "voxel_edits_per_frame": round(voxels / args.edit_frames, 1),
         "faces_bound_at_end": int(faces),
-        "ms_per_frame_wall": round(elapsed * 1000.0 / args.edit_frames, 3),
+        "ms_per_frame_wall": round(overlapped * 1000.0 / (args.edit_frames - half), 3),
+        "ms_per_frame_wall_edit_after_frame": round(sequential * 1000.0 / half, 3),
         "edit_ms_host_median": round(float(np.median(edit_ms)), 3),
         "naadf_pack_gpu_ms_median": round(float(np.median(build_ms)), 3),
         "edit_host_ms_median_by_phase": dict(zip(["regions_and_repack", "top_level", "retention_event", "exposure"],

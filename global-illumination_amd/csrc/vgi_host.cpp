@@ -545,7 +545,7 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         });
     }
     // Edits apply in order (a later edit of a voxel wins); the GPU grid gets
-    // each edited voxel's final value once.
+    // each edited voxel's final value once, before the re-pack.
     std::vector<uint32_t> idx;
     std::vector<vgi_host::Int3> edited;
     idx.reserve(accepted.size());
@@ -561,13 +561,6 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
     std::vector<uint16_t> val(idx.size());
     for (size_t k = 0; k < idx.size(); ++k) val[k] = t.grid[idx[k]];
-    HIP_OK(t.d_scatter_idx.reserve(idx.size() * 4));
-    HIP_OK(t.d_scatter_val.reserve(val.size() * 2));
-    HIP_OK(hipMemcpyAsync(t.d_scatter_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, t.stream));
-    HIP_OK(hipMemcpyAsync(t.d_scatter_val.ptr, val.data(), val.size() * 2, hipMemcpyHostToDevice, t.stream));
-    vgi::launch_scatter_voxels(t.d_grid.as<uint16_t>(), t.d_scatter_idx.as<uint32_t>(), t.d_scatter_val.as<uint16_t>(),
-                               (uint32_t)idx.size(), t.stream);
-    HIP_OK(hipGetLastError());
     // Emitting materials near an edit change the emitter distribution.
     uint64_t emitting[4] = {0, 0, 0, 0};
     for (int m = 0; m < 256; ++m) {
@@ -613,6 +606,39 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         if (near[k]) t.emitters_dirty = true;
         if (was[k] != t.solid[ri]) flips.push_back({std::make_tuple(t.rmin.z + z, t.rmin.y + y, t.rmin.x + x), Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z}});
     }
+    // The retention event and the exposure update need only the host's scene
+    // and the stream-ordered uploads of their own buffers, so they run before
+    // the first wait on the stream: a caller that edits while a frame is in
+    // flight overlaps them with that frame.
+    float regions_ms = ms_since(t_start);
+    if (events) {
+        auto t_event = clock::now();
+        // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
+        std::vector<std::vector<vgi_host::Int3>> changed_of(dirty.size());
+        pool.run(dirty.size(), [&](size_t begin, size_t end) {
+            for (size_t k = begin; k < end; ++k) {
+                copy_region(dirty[k], deltas[k].after);
+                deltas[k].solid_after = t.solid[dirty[k]] != 0;
+                const vgi_host::RegionDelta &d = deltas[k];
+                for (int i = 0; i < 4096; ++i)
+                    if (d.before[i] != d.after[i])
+                        changed_of[k].push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
+            }
+        });
+        std::vector<vgi_host::Int3> changed;
+        for (const auto &c : changed_of) changed.insert(changed.end(), c.begin(), c.end());
+        std::vector<uint8_t> page_solid;
+        vgi_host::PreviousScene prev = scene_pages(t, solid_before, page_solid);
+        vgi_host::RetentionEvent ev;
+        if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev, &pool)) t.pending_events.push_back(std::move(ev));
+        t.timings[6] = ms_since(t_event);
+        if (t.follow_exposure && !changed.empty()) {
+            auto t_exposure = clock::now();
+            if (update_exposure(t, changed, deltas) != 0) return -1;
+            t.timings[7] = ms_since(t_exposure);
+        }
+    }
+    auto t_pack = clock::now();
     // Keep the cells buffer bounded: re-pack everything once appended tables
     // and bricks outgrow four times the last full pack.
     bool full = t.cells_used > 4u * std::max<uint32_t>(t.cells_live, 1u << 16);
@@ -622,11 +648,20 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         for (size_t i = 0; i < t.loaded.size(); ++i)
             if (t.loaded[i]) repack.push_back((int)i);
     }
+    // The GPU grid's copies of the edited voxels, read by the re-pack (an
+    // upload from pageable memory waits for the stream's earlier work).
+    HIP_OK(t.d_scatter_idx.reserve(idx.size() * 4));
+    HIP_OK(t.d_scatter_val.reserve(val.size() * 2));
+    HIP_OK(hipMemcpyAsync(t.d_scatter_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, t.stream));
+    HIP_OK(hipMemcpyAsync(t.d_scatter_val.ptr, val.data(), val.size() * 2, hipMemcpyHostToDevice, t.stream));
+    vgi::launch_scatter_voxels(t.d_grid.as<uint16_t>(), t.d_scatter_idx.as<uint32_t>(), t.d_scatter_val.as<uint16_t>(),
+                               (uint32_t)idx.size(), t.stream);
+    HIP_OK(hipGetLastError());
     hipEventRecord(t.ev[6], t.stream);
     if (gpu_pack(t, region_list(t, repack), full) != 0) return -1;
     if (full) t.cells_live = t.cells_used;
     hipEventRecord(t.ev[7], t.stream);
-    t.timings[4] = ms_since(t_start);
+    t.timings[4] = regions_ms + ms_since(t_pack);
     auto t_top = clock::now();
     if (!flips.empty()) {
         std::sort(flips.begin(), flips.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
@@ -638,33 +673,8 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     hipEventSynchronize(t.ev[7]);
     hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
     t.timings[5] = ms_since(t_top);
-    if (!events) return 0;
-    auto t_event = clock::now();
-    // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
-    std::vector<std::vector<vgi_host::Int3>> changed_of(dirty.size());
-    pool.run(dirty.size(), [&](size_t begin, size_t end) {
-        for (size_t k = begin; k < end; ++k) {
-            copy_region(dirty[k], deltas[k].after);
-            deltas[k].solid_after = t.solid[dirty[k]] != 0;
-            const vgi_host::RegionDelta &d = deltas[k];
-            for (int i = 0; i < 4096; ++i)
-                if (d.before[i] != d.after[i])
-                    changed_of[k].push_back(Int3{d.coord.x * 16 + (i & 15), d.coord.y * 16 + ((i >> 4) & 15), d.coord.z * 16 + (i >> 8)});
-        }
-    });
-    std::vector<vgi_host::Int3> changed;
-    for (const auto &c : changed_of) changed.insert(changed.end(), c.begin(), c.end());
-    std::vector<uint8_t> page_solid;
-    vgi_host::PreviousScene prev = scene_pages(t, solid_before, page_solid);
-    vgi_host::RetentionEvent ev;
-    if (vgi_host::edit_event(deltas, prev, t.emission, edited, ev, &pool)) t.pending_events.push_back(std::move(ev));
-    t.timings[6] = ms_since(t_event);
-    if (t.follow_exposure && !changed.empty()) {
-        auto t_exposure = clock::now();
-        if (update_exposure(t, changed, deltas) != 0) return -1;
-        t.timings[7] = ms_since(t_exposure);
-    }
     return 0;
+
 }
 
 // ---------------------------------------------------------------- optics / emitters

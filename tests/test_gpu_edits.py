@@ -69,3 +69,61 @@ def test_edit_stream_bit_exact(vgi, orc, edits_per_frame):
             for rx in range(b[0], b[3] + 1):
                 assert np.array_equal(g.region_canonical(rx, ry, rz), setup["scene"].region_canonical(rx, ry, rz)), (rx, ry, rz)
     g.close()
+
+
+def test_edits_issued_while_a_frame_runs_match_edits_after_it(vgi, orc):
+    """vgi_edit_voxels issued right after vgi_run_frame, while that frame is
+    still on the GPU (its host work overlapping the frame), gives the same
+    event words every edit and the same bound faces, records, resident state
+    and packed regions at the end as edits issued after the frame finished."""
+    seed, edge = 0x5EED0004, 128
+    setup = make_scene(vgi, orc, 4, seed, edge=edge)
+    edits = [vgi.scene_edits(seed, f, 400, edge) for f in range(1, 6)]
+    out = {}
+    for mode in ("after", "overlapped"):
+        print("edits %s the frame" % mode, flush=True)
+        g = gpu_tracer(vgi, setup, resident=True)
+        g.set_transport(bounces=2, samples=2, environment=tuple(setup["env"]), measured_retention=1.0)
+        g.bind_exposed()
+        g.set_clock(0)
+        g.run_frame()
+        g.sync()
+        words = []
+        if mode == "after":
+            for f, (xyz, values) in enumerate(edits):
+                g.edit_voxels(xyz, values)
+                words.append(g.events().copy())
+                g.set_clock(16 * (f + 1))
+                g.run_frame()
+                g.sync()
+        else:
+            g.edit_voxels(*edits[0])
+            words.append(g.events().copy())
+            for f in range(len(edits)):
+                g.set_clock(16 * (f + 1))
+                g.run_frame()
+                if f + 1 < len(edits):
+                    g.edit_voxels(*edits[f + 1])
+                    words.append(g.events().copy())
+                g.sync()
+        g.set_clock(16 * (len(edits) + 1))
+        g.run_frame()
+        g.sync()
+        tasks = g.bound_tasks()
+        rec, _ = g.read_bound()
+        n = len(tasks)
+        b = setup["scene"].bounds()
+        regions = [g.region_canonical(rx, ry, rz) for rz in range(b[2], b[5] + 1) for ry in range(b[1], b[4] + 1)
+                   for rx in range(b[0], b[3] + 1)]
+        out[mode] = (words, tasks.copy(), rec.copy(), g.cache_state(0, n), g.cache_meta(0, n), regions)
+        g.close()
+    a, b = out["after"], out["overlapped"]
+    assert len(a[0]) == len(b[0]) == len(edits)
+    for f in range(len(edits)):
+        assert np.array_equal(a[0][f], b[0][f]), "edit %d: event words differ" % f
+    assert np.array_equal(a[1], b[1]), "bound faces differ"
+    assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32)), "records differ"
+    assert np.array_equal(a[3], b[3]), "resident state differs"
+    assert np.array_equal(a[4], b[4]), "slot metadata differs"
+    for ra, rb in zip(a[5], b[5]):
+        assert np.array_equal(ra, rb), "packed region differs"
