@@ -162,6 +162,15 @@ struct vgi_tracer {
     // wrote the terms records the epoch it wrote them at. While the two
     // agree, frames keep the terms instead of tracing them again.
     uint64_t direct_epoch = 1, direct_written = 0;
+    // Pinned staging for the edit path's small uploads (an upload from
+    // pinned memory returns at once; one from pageable memory waits for the
+    // stream's earlier work). A bump region, started over only once the
+    // stream has passed every copy issued from it (stage_done follows the
+    // latest copy).
+    uint8_t *stage = nullptr;
+    size_t stage_bytes = 0, stage_used = 0;
+    hipEvent_t stage_done = nullptr;
+    bool stage_pending = false;
     std::vector<uint32_t> bound_material;  // material of each bound face's voxel
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
@@ -494,12 +503,16 @@ vgi_host::PreviousScene scene_pages(const vgi_tracer &t, const std::vector<uint8
     return prev;
 }
 
+int staged_upload(vgi_tracer &t, void *dst, const void *src, size_t bytes);
+void stage_recycle(vgi_tracer &t);
+
 int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     if (!t.have_regions) return 0;
     using clock = std::chrono::steady_clock;
     auto ms_since = [](clock::time_point a) { return std::chrono::duration<float, std::milli>(clock::now() - a).count(); };
     auto t_start = clock::now();
     t.timings[4] = t.timings[5] = t.timings[6] = t.timings[7] = 0.0f;
+    stage_recycle(t);
     std::vector<uint32_t> accepted;
     std::vector<int> dirty;
     accepted.reserve(count);
@@ -648,12 +661,12 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         for (size_t i = 0; i < t.loaded.size(); ++i)
             if (t.loaded[i]) repack.push_back((int)i);
     }
-    // The GPU grid's copies of the edited voxels, read by the re-pack (an
-    // upload from pageable memory waits for the stream's earlier work).
+    // The GPU grid's copies of the edited voxels, read by the re-pack.
     HIP_OK(t.d_scatter_idx.reserve(idx.size() * 4));
     HIP_OK(t.d_scatter_val.reserve(val.size() * 2));
-    HIP_OK(hipMemcpyAsync(t.d_scatter_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, t.stream));
-    HIP_OK(hipMemcpyAsync(t.d_scatter_val.ptr, val.data(), val.size() * 2, hipMemcpyHostToDevice, t.stream));
+    if (staged_upload(t, t.d_scatter_idx.ptr, idx.data(), idx.size() * 4) != 0 ||
+        staged_upload(t, t.d_scatter_val.ptr, val.data(), val.size() * 2) != 0)
+        return -1;
     vgi::launch_scatter_voxels(t.d_grid.as<uint16_t>(), t.d_scatter_idx.as<uint32_t>(), t.d_scatter_val.as<uint16_t>(),
                                (uint32_t)idx.size(), t.stream);
     HIP_OK(hipGetLastError());
@@ -739,6 +752,42 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.cache_mask = t.resident ? (uint32_t)(t.face_slots.entries.size() - 1) : 0u;
     o.cache_on = (t.resident && !t.transport.plain) ? 1u : 0u;
     o.direct_reuse = 0u;
+}
+
+// ---------------------------------------------------------------- staged uploads
+// `bytes` from host `src` to device `dst` on the tracer's stream through the
+// pinned staging region: the host may reuse `src` at once and does not wait.
+int staged_upload(vgi_tracer &t, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return 0;
+    size_t need = (bytes + 255) & ~(size_t)255;
+    if (t.stage_used + need > t.stage_bytes) {
+        if (t.stage_pending) HIP_OK(hipEventSynchronize(t.stage_done));
+        t.stage_pending = false;
+        t.stage_used = 0;
+        if (need > t.stage_bytes) {
+            if (t.stage) HIP_OK(hipHostFree(t.stage));
+            t.stage = nullptr;
+            t.stage_bytes = 0;
+            size_t cap = std::max<size_t>(need, (size_t)32 << 20);
+            HIP_OK(hipHostMalloc((void **)&t.stage, cap, hipHostMallocDefault));
+            t.stage_bytes = cap;
+        }
+    }
+    uint8_t *p = t.stage + t.stage_used;
+    std::memcpy(p, src, bytes);
+    t.stage_used += need;
+    HIP_OK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, t.stream));
+    if (!t.stage_done) HIP_OK(hipEventCreateWithFlags(&t.stage_done, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(t.stage_done, t.stream));
+    t.stage_pending = true;
+    return 0;
+}
+
+// Start the staging region over if the stream has passed its copies.
+void stage_recycle(vgi_tracer &t) {
+    if (t.stage_pending && hipEventQuery(t.stage_done) != hipSuccess) return;
+    t.stage_pending = false;
+    t.stage_used = 0;
 }
 
 // ---------------------------------------------------------------- resident slots
@@ -827,16 +876,14 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         HIP_OK(hipMemsetAsync(t.d_state.as<uint32_t>() + (size_t)lo * 36, 0, n * 144, t.stream));
         HIP_OK(hipMemsetAsync(t.d_checkpoint.as<uint32_t>() + (size_t)lo * 12, 0, n * 48, t.stream));
         HIP_OK(hipMemsetAsync(t.d_corners.as<uint32_t>() + lo, 0, n * 4, t.stream));
-        HIP_OK(hipMemcpyAsync(t.d_meta.as<uint32_t>() + (size_t)lo * 12, meta.data(), n * 48, hipMemcpyHostToDevice, t.stream));
-        HIP_OK(hipStreamSynchronize(t.stream));
+        if (staged_upload(t, t.d_meta.as<uint32_t>() + (size_t)lo * 12, meta.data(), n * 48) != 0) return -1;
     }
     (void)first_new;
     if (!meta_writes.empty()) {
         HIP_OK(t.d_meta_writes.reserve(meta_writes.size() * 4));
-        HIP_OK(hipMemcpyAsync(t.d_meta_writes.ptr, meta_writes.data(), meta_writes.size() * 4, hipMemcpyHostToDevice, t.stream));
+        if (staged_upload(t, t.d_meta_writes.ptr, meta_writes.data(), meta_writes.size() * 4) != 0) return -1;
         vgi::launch_scatter_words(t.d_meta_writes.as<uint2>(), (uint32_t)(meta_writes.size() / 2), t.d_meta.as<uint32_t>(), t.stream);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipStreamSynchronize(t.stream));
     }
     if (t.face_slots.dirty) {
         size_t n = t.face_slots.entries.size();
@@ -854,10 +901,9 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         for (uint32_t i : tw)
             for (int k = 0; k < 4; ++k) { writes.push_back(i * 4u + (uint32_t)k); writes.push_back(t.face_slots.entries[i][k]); }
         HIP_OK(t.d_meta_writes.reserve(writes.size() * 4));
-        HIP_OK(hipMemcpyAsync(t.d_meta_writes.ptr, writes.data(), writes.size() * 4, hipMemcpyHostToDevice, t.stream));
+        if (staged_upload(t, t.d_meta_writes.ptr, writes.data(), writes.size() * 4) != 0) return -1;
         vgi::launch_scatter_words(t.d_meta_writes.as<uint2>(), (uint32_t)(writes.size() / 2), t.d_index.as<uint32_t>(), t.stream);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipStreamSynchronize(t.stream));
     }
     t.face_slots.touched.clear();
     return 0;
@@ -1271,7 +1317,8 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
     dirty_index.erase(std::unique(dirty_index.begin(), dirty_index.end()), dirty_index.end());
     dirty_index.erase(std::remove_if(dirty_index.begin(), dirty_index.end(), [&](uint32_t i) { return i >= count; }), dirty_index.end());
     size_t need_tasks = (size_t)std::max<uint32_t>(count, 1) * 32;
-    if (need_tasks > t.b_tasks.bytes) {
+    bool whole = need_tasks > t.b_tasks.bytes;
+    if (whole) {
         // Grown past the allocation: upload the whole set again.
         HIP_OK(t.b_tasks.reserve(need_tasks));
         HIP_OK(t.b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
@@ -1284,15 +1331,17 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
         HIP_OK(t.d_bound_index.reserve(dirty_index.size() * 4));
         HIP_OK(t.d_bound_upd.reserve(up.size() * 32));
         HIP_OK(t.d_bound_slot_upd.reserve(up_slots.size() * 4));
-        HIP_OK(hipMemcpyAsync(t.d_bound_index.ptr, dirty_index.data(), dirty_index.size() * 4, hipMemcpyHostToDevice, t.stream));
-        HIP_OK(hipMemcpyAsync(t.d_bound_upd.ptr, up.data(), up.size() * 32, hipMemcpyHostToDevice, t.stream));
-        HIP_OK(hipMemcpyAsync(t.d_bound_slot_upd.ptr, up_slots.data(), up_slots.size() * 4, hipMemcpyHostToDevice, t.stream));
+        if (staged_upload(t, t.d_bound_index.ptr, dirty_index.data(), dirty_index.size() * 4) != 0 ||
+            staged_upload(t, t.d_bound_upd.ptr, up.data(), up.size() * 32) != 0 ||
+            staged_upload(t, t.d_bound_slot_upd.ptr, up_slots.data(), up_slots.size() * 4) != 0)
+            return -1;
         vgi::launch_scatter_bound(t.d_bound_index.as<uint32_t>(), t.d_bound_upd.as<uint4>(), t.d_bound_slot_upd.as<uint32_t>(),
                                   (uint32_t)dirty_index.size(), t.b_tasks.as<uint4>(), t.b_slots.as<uint32_t>(), t.stream);
         HIP_OK(hipGetLastError());
     }
     HIP_OK(t.b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
-    HIP_OK(hipStreamSynchronize(t.stream));
+    // The whole set travels from pageable memory: wait for it.
+    if (whole) HIP_OK(hipStreamSynchronize(t.stream));
     t.bound = count;
     t.has_order = false;  // the bound set changed: faces are taken in bound order
     if (t.mlp_ready && bind_materials(t, t.bound_tasks.data(), count) != 0) return -1;
@@ -1360,6 +1409,8 @@ void vgi_destroy(vgi_tracer *t) {
     for (auto &e : t->side_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->sel_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->sun_ev) if (e) hipEventDestroy(e);
+    if (t->stage_done) hipEventDestroy(t->stage_done);
+    if (t->stage) hipHostFree(t->stage);
     if (t->side) hipStreamDestroy(t->side);
     hipStreamDestroy(t->stream);
     delete t;
