@@ -121,6 +121,7 @@ def library():
         "vgi_read_bound_tasks": (i32, [P, P, u32]),
         "vgi_read_events": (i64, [P, P, u32]),
         "vgi_read_bound": (ctypes.c_int, [P, P, u32, P]),
+        "vgi_read_bound_hits": (ctypes.c_int, [P, P, u64]),
         "vgi_cache_state": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "vgi_set_schedule": (ctypes.c_int, [P, ctypes.POINTER(Schedule)]),
@@ -171,7 +172,7 @@ def exported_symbols():
         "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_set_clock", "vgi_upload_pages", "vgi_edit_voxels",
         "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
-        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound",
+        "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound", "vgi_read_bound_hits",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
@@ -440,6 +441,12 @@ class GpuTracer:
         counters = np.zeros(5, np.uint64)
         _check(self._lib.vgi_read_bound(self._h, _ptr(out), count, _ptr(counters)))
         return out, counters
+
+    def read_bound_hits(self):
+        """First-hit records (count * samples, 8) of the last frame run with RECORD_HITS."""
+        out = np.zeros((self._bound * self.transport.samples, 8), np.uint32)
+        _check(self._lib.vgi_read_bound_hits(self._h, _ptr(out), out.size))
+        return out
 
     def read_counters_bound(self):
         counters = np.zeros(5, np.uint64)

@@ -4,6 +4,7 @@ Float flags matter for parity: -ffp-contract=off keeps every a*b+c as two
 rounded operations, and correctly rounded fp32 division and square root keep
 `/` and sqrt IEEE, as the CPU restatement computes them.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -41,22 +42,24 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
-    """Compile every source and link libvgi.so. Always rebuilds."""
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel) and link libvgi.so. Always rebuilds."""
     os.makedirs(BUILD, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objects = []
+    commands, objects = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        cmd = [hipcc] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        commands.append([hipcc] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objects.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        commands.append(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objects.append(obj)
+    jobs = jobs or max(1, min(len(commands), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for out in pool.map(_run, commands):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objects + ["-lpthread"])
     os.replace(tmp, LIB)

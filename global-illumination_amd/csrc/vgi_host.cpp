@@ -150,8 +150,9 @@ struct vgi_tracer {
     std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
     uint64_t next_ticket = 1;
     // ---- bound frames
-    DeviceBuffer b_tasks, b_slots, b_results, b_counters;
+    DeviceBuffer b_tasks, b_slots, b_results, b_counters, b_hits;
     uint32_t bound = 0;
+    uint64_t bound_hit_words = 0;  // first-hit words of the last frame run with VGI_RECORD_HITS (0: none)
     float last_trace_ms = 0.0f;
     uint32_t frame_index = 0;
     bool frame_recorded = false;
@@ -437,9 +438,9 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
                 HIP_OK(hipMemsetAsync(t.d_headers.as<uint4>() + i, 0, 16, t.stream));
             }
     }
-    hipEventRecord(t.ev[6], t.stream);
+    HIP_OK(hipEventRecord(t.ev[6], t.stream));
     if (gpu_pack(t, region_list(t, repack), relayout) != 0) return -1;
-    hipEventRecord(t.ev[7], t.stream);
+    HIP_OK(hipEventRecord(t.ev[7], t.stream));
     // Regions whose barrier state flipped, in z, y, x order (world.rs:595-601).
     std::vector<std::pair<std::tuple<int, int, int>, Int3>> flips;
     for (auto &kv : before) {
@@ -670,10 +671,10 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     vgi::launch_scatter_voxels(t.d_grid.as<uint16_t>(), t.d_scatter_idx.as<uint32_t>(), t.d_scatter_val.as<uint16_t>(),
                                (uint32_t)idx.size(), t.stream);
     HIP_OK(hipGetLastError());
-    hipEventRecord(t.ev[6], t.stream);
+    HIP_OK(hipEventRecord(t.ev[6], t.stream));
     if (gpu_pack(t, region_list(t, repack), full) != 0) return -1;
     if (full) t.cells_live = t.cells_used;
-    hipEventRecord(t.ev[7], t.stream);
+    HIP_OK(hipEventRecord(t.ev[7], t.stream));
     t.timings[4] = regions_ms + ms_since(t_pack);
     auto t_top = clock::now();
     if (!flips.empty()) {
@@ -1543,23 +1544,23 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
     views(*t, s, o);
     uint32_t lanes = lanes_for(t->transport);
     if (flush_events(*t) != 0) return -1;
-    hipEventRecord(t->ev[0], t->stream);
+    HIP_OK(hipEventRecord(t->ev[0], t->stream));
     if (t->resident)
         vgi::launch_prepare(t->d_tasks.as<uint4>(), t->d_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), next_batch(*t), t->d_keys.as<uint32_t>(), t->stream);
-    hipEventRecord(t->ev[1], t->stream);
+    HIP_OK(hipEventRecord(t->ev[1], t->stream));
     if (trace_bound_faces(*t, s, o, t->d_tasks.as<uint4>(), count, t->transport.samples, lanes, 0u, k->results.as<float4>(),
                           k->with_hits ? k->hits.as<uint32_t>() : nullptr,
                           (k->with_counters || k->with_hits) ? k->counters.as<unsigned long long>() : nullptr, true, t->stream,
                           nullptr) != 0)
         return -1;
     HIP_OK(hipGetLastError());
-    hipEventRecord(t->ev[2], t->stream);
+    HIP_OK(hipEventRecord(t->ev[2], t->stream));
     if (t->resident)
         vgi::launch_accumulate(t->d_slots.as<uint32_t>(), count, k->results.as<float4>(), t->d_state.as<uint32_t>(),
                                t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), accumulate_params(*t, clock_ms(*t)), t->stream);
     HIP_OK(hipGetLastError());
-    hipEventRecord(t->ev[3], t->stream);
+    HIP_OK(hipEventRecord(t->ev[3], t->stream));
     HIP_OK(hipEventRecord(k->done, t->stream));
     uint64_t id = t->next_ticket++;
     t->tickets[id] = std::move(k);
@@ -1715,20 +1716,32 @@ int64_t vgi_read_events(vgi_tracer *t, uint32_t *out, uint32_t capacity) {
 int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     if (!t) return fail("null tracer");
     if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    bool record_hits = (flags & VGI_RECORD_HITS) != 0;
+    if (record_hits && t->learned)
+        return fail("learned frames keep no first-hit records (their paths end in the network)");
     if (sync_emitters(*t) != 0) return -1;
     vgi::SceneView s;
     vgi::Optics o;
     views(*t, s, o);
     uint32_t count = t->bound;
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
-    hipEventRecord(t->ev[8], t->stream);
+    t->bound_hit_words = 0;
+    uint32_t *hits = nullptr;
+    if (record_hits) {
+        uint64_t words = (uint64_t)count * t->transport.samples * 8u;
+        HIP_OK(t->b_hits.reserve(std::max<uint64_t>(words, 1) * 4));
+        hits = t->b_hits.as<uint32_t>();
+        t->bound_hit_words = words;
+    }
+    HIP_OK(hipEventRecord(t->ev[8], t->stream));
     if (flush_events(*t) != 0) return -1;
     o.direct_reuse = t->direct_written == t->direct_epoch ? 1u : 0u;
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
     if (t->resident)
         vgi::launch_prepare(t->b_tasks.as<uint4>(), t->b_slots.as<uint32_t>(), count, t->d_state.as<uint32_t>(), t->d_meta.as<uint32_t>(),
                             std::max(t->transport.measured_retention, 0.0f), next_batch(*t), t->d_keys.as<uint32_t>(), t->stream);
-    hipEventRecord(t->ev[4], t->stream);
+    HIP_OK(hipEventRecord(t->ev[4], t->stream));
     bool learned_frame = t->learned && t->transport.bounces >= 2u;
     if (t->learned) {
         // The frame's training paths run on a second stream beside the
@@ -1758,19 +1771,20 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
                        t->b_pred.as<float>(), t->stream);
         vgi::launch_learned_combine(t->b_learned.as<float4>(), t->b_pred.as<float>(), count, t->transport.samples,
                                     lanes_for(t->transport), t->b_tasks.as<uint4>(), t->b_results.as<float4>(), t->stream);
-    } else if (t->wavefront && (flags & VGI_COUNT_GATHERS) == 0 && o.light_count <= vgi::WF_MAX_LIGHTS && o.bounces <= 15u) {
+    } else if (t->wavefront && !record_hits && (flags & VGI_COUNT_GATHERS) == 0 && o.light_count <= vgi::WF_MAX_LIGHTS &&
+               o.bounces <= 15u) {
         if (wavefront_trace(*t, s, o, t->b_tasks.as<uint4>(), count, 0u, t->b_results.as<float4>(),
                             counting ? t->b_counters.as<unsigned long long>() : nullptr) != 0)
             return -1;
     } else {
         if (trace_bound_faces(*t, s, o, t->b_tasks.as<uint4>(), count, t->transport.samples, lanes_for(t->transport), 0u,
-                              t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
+                              t->b_results.as<float4>(), hits, counting ? t->b_counters.as<unsigned long long>() : nullptr,
                               (flags & VGI_COUNT_GATHERS) != 0, t->stream, order_of(*t)) != 0)
             return -1;
     }
     HIP_OK(hipGetLastError());
     t->direct_written = t->direct_epoch;
-    hipEventRecord(t->ev[5], t->stream);
+    HIP_OK(hipEventRecord(t->ev[5], t->stream));
     t->frame_recorded = true;
 
     if (t->resident)
@@ -1781,8 +1795,18 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
         HIP_OK(hipStreamWaitEvent(t->stream, t->side_ev[1], 0));
         if (learned_steps(*t) != 0) return -1;
     }
-    hipEventRecord(t->ev[9], t->stream);
+    HIP_OK(hipEventRecord(t->ev[9], t->stream));
     t->frame_index += 1;
+    return 0;
+}
+
+int vgi_read_bound_hits(vgi_tracer *t, uint32_t *out, uint64_t words) {
+    if (!t || (words && !out)) return fail("null argument");
+    if (t->bound_hit_words == 0) return fail("the last frame kept no first-hit records (run it with VGI_RECORD_HITS)");
+    if (words > t->bound_hit_words) return fail("more hit words than the last frame recorded");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (words) HIP_OK(hipMemcpy(out, t->b_hits.ptr, words * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2159,12 +2183,12 @@ int vgi_select_frame(vgi_tracer *t, uint32_t picks, uint32_t flags) {
     views(*t, s, o);
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
     if (counting) HIP_OK(hipMemsetAsync(t->b_counters.ptr, 0, 64, t->stream));
-    hipEventRecord(t->sel_ev[0], t->stream);
+    HIP_OK(hipEventRecord(t->sel_ev[0], t->stream));
     vgi::launch_select(p, t->d_meta.as<uint32_t>(), t->d_state.as<uint32_t>(), t->d_keys.as<uint32_t>(), t->d_selection.as<uint32_t>(),
                        t->d_mirror.as<uint4>(), t->d_pick_tasks.as<uint4>(), t->d_pick_slots.as<uint32_t>(),
                        t->d_select_ranges.as<uint32_t>(), (uint32_t)std::max(t->cus, 1) * 4u, t->stream);
     HIP_OK(hipGetLastError());
-    hipEventRecord(t->sel_ev[1], t->stream);
+    HIP_OK(hipEventRecord(t->sel_ev[1], t->stream));
     // mark_changed for the picks (no host claim: batch 0).
     vgi::launch_prepare(t->d_pick_tasks.as<uint4>(), t->d_pick_slots.as<uint32_t>(), picks, t->d_state.as<uint32_t>(),
                         t->d_meta.as<uint32_t>(), std::max(t->transport.measured_retention, 0.0f), 0u, nullptr, t->stream);
@@ -2172,13 +2196,13 @@ int vgi_select_frame(vgi_tracer *t, uint32_t picks, uint32_t flags) {
                           t->d_pick_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr,
                           (flags & VGI_COUNT_GATHERS) != 0, t->stream, nullptr) != 0)
         return -1;
-    hipEventRecord(t->sel_ev[2], t->stream);
+    HIP_OK(hipEventRecord(t->sel_ev[2], t->stream));
     vgi::AccumulateParams ap = accumulate_params(*t, p.now_ms);
     ap.picked = 1u;
     vgi::launch_accumulate(t->d_pick_slots.as<uint32_t>(), picks, t->d_pick_results.as<float4>(), t->d_state.as<uint32_t>(),
                            t->d_checkpoint.as<uint32_t>(), t->d_meta.as<uint32_t>(), ap, t->stream);
     HIP_OK(hipGetLastError());
-    hipEventRecord(t->sel_ev[3], t->stream);
+    HIP_OK(hipEventRecord(t->sel_ev[3], t->stream));
     t->picks = picks;
     t->mirror_budget = mirror_budget;
     t->select_recorded = true;
@@ -2258,10 +2282,10 @@ int vgi_probe_copy(vgi_tracer *t, uint64_t bytes, uint32_t repeats, float *ms) {
     HIP_OK(a.reserve(bytes));
     HIP_OK(b.reserve(bytes));
     HIP_OK(hipMemsetAsync(a.ptr, 1, bytes, t->stream));
-    hipEventRecord(t->ev[6], t->stream);
+    HIP_OK(hipEventRecord(t->ev[6], t->stream));
     for (uint32_t r = 0; r < repeats; ++r) vgi::launch_copy(a.ptr, b.ptr, bytes, t->stream);
     HIP_OK(hipGetLastError());
-    hipEventRecord(t->ev[7], t->stream);
+    HIP_OK(hipEventRecord(t->ev[7], t->stream));
     HIP_OK(hipStreamSynchronize(t->stream));
     float total = 0.0f;
     hipEventElapsedTime(&total, t->ev[6], t->ev[7]);

@@ -156,6 +156,11 @@ int vgi_set_wavefront(vgi_tracer *tracer, int on);
  * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
  * the frame ran with VGI_COUNT_RAYS. */
 int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uint64_t *counters);
+/* First-hit records of the last bound-face frame run with VGI_RECORD_HITS (as
+ * vgi_read_hits: 8 u32 per face-sample, bound face i's samples at
+ * (i * samples + s) * 8; cf. voxel_view_trace.wgsl:21-38, the reference's
+ * debug trace of the same frame). Learned frames keep none. */
+int vgi_read_bound_hits(vgi_tracer *tracer, uint32_t *out, uint64_t words);
 /* Bind every exposed face of the loaded scene (a solid voxel with loaded air
  * across the face; z, y, x, face order) and keep the bound set equal to the
  * exposed set through later edits: covered faces leave (the last bound face

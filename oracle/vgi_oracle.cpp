@@ -28,6 +28,7 @@
 #include <thread>
 #include <tuple>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -1020,6 +1021,7 @@ struct Tracer {
     bool materials_changed = true;
     // resident cache
     bool cache_on = false;
+    bool plain = false;  // slots and accumulation kept, but paths never end at cached faces
     FaceIndex face_index;
     std::vector<uint32_t> state;       // 36 words per slot
     std::vector<uint32_t> checkpoint;  // 12 words per slot
@@ -1060,6 +1062,7 @@ struct RayResult {
 
 struct Counters {
     uint64_t rays = 0, steps = 0, region_loads = 0, brick_loads = 0, cell_loads = 0, paths = 0;
+    uint64_t direct_rays = 0;  // rays of write_result's analytic direct term (a resident frame may keep it)
 };
 
 struct Ctx {
@@ -1286,7 +1289,7 @@ struct Estimator {
     // voxel_path.wgsl:66-87
     bool cached_energy(I3 voxel, uint32_t face, F3 &value, uint32_t *slot_out) const {
         if (slot_out) *slot_out = NONE;
-        if (!t.cache_on || face > 5u) return false;
+        if (!t.cache_on || t.plain || face > 5u) return false;
         uint32_t mask = (uint32_t)t.face_index.mask();
         uint32_t slot = (hash_voxel(voxel) ^ pcg(face + 1u)) & mask;
         for (uint32_t attempt = 0; attempt < 64u; ++attempt) {
@@ -1598,7 +1601,9 @@ static void trace_one_face(const Tracer &t, const uint32_t *task, float *out, ui
     }
     // write_result (voxel_trace.wgsl:25-37): one more analytic direct estimate.
     F3 dv, dc;
+    uint64_t rays_before_direct = cn ? cn->rays : 0;
     est.direct_at(to_f3(voxel) + f3(0.5f, 0.5f, 0.5f) + normal * 0.5f, normal, dv, dc);
+    if (cn) cn->direct_rays += cn->rays - rays_before_direct;
     float inverse = 1.0f / (float)std::max(tknown, 1u);
     F3 m = tmean * inverse, s = tsecond * inverse;
     float rec[24] = {m.x, m.y, m.z, 0.0f, 0.0f, (float)tknown, 0.0f, 0.0f, s.x, s.y, s.z, 0.0f,
@@ -1646,9 +1651,11 @@ static int trace_faces(Tracer &t, const uint32_t *tasks, uint32_t count, float *
         for (auto &c : per) {
             s.rays += c.rays; s.steps += c.steps; s.region_loads += c.region_loads;
             s.brick_loads += c.brick_loads; s.cell_loads += c.cell_loads; s.paths += c.paths;
+            s.direct_rays += c.direct_rays;
         }
         counters[0] = s.rays; counters[1] = s.steps; counters[2] = s.region_loads;
         counters[3] = s.brick_loads; counters[4] = s.cell_loads; counters[5] = s.paths;
+        counters[6] = s.direct_rays;
     }
     return 0;
 }
@@ -2051,7 +2058,11 @@ static inline uint32_t curve_limit(uint32_t d2) {
 static void apply_events(Tracer &t, const uint32_t *events, uint32_t event_count, uint32_t reach, uint32_t urgent_measured,
                          uint32_t urgent_curve) {
     bool measured_mode = t.measured_retention > 0.0f;
-    for (uint32_t slot = 0; slot < t.high_water; ++slot) {
+    // Slots are independent (each reads and writes only its own rows), so
+    // ranges of slots run on host threads.
+    int threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    parallel_for(t.high_water, threads, [&](uint32_t first, uint32_t end, int) {
+    for (uint32_t slot = first; slot < end; ++slot) {
         uint32_t *id = &t.meta[(size_t)slot * 12];
         if ((id[3] & META_LIVE) == 0u) continue;
         I3 p{(int32_t)id[0], (int32_t)id[1], (int32_t)id[2]};
@@ -2114,6 +2125,7 @@ static void apply_events(Tracer &t, const uint32_t *events, uint32_t event_count
         if (woken || urgent) t.meta[(size_t)slot * 12 + 5] &= 0xfffffff0u;
         if (urgent) t.meta[(size_t)slot * 12 + 11] |= URGENT;
     }
+    });
 }
 
 // ---------------------------------------------------------------- learned cache oracle (FP64)
@@ -2470,9 +2482,10 @@ int orc_trace_faces(void *tp, const uint32_t *tasks, uint32_t count, float *resu
 }
 
 
-int orc_cache_enable(void *tp, uint32_t history_limit, float recency, float measured_retention) {
+int orc_cache_enable(void *tp, uint32_t history_limit, float recency, float measured_retention, uint32_t plain) {
     Tracer &t = *(Tracer *)tp;
     t.cache_on = true;
+    t.plain = plain != 0u;
     t.history_limit = history_limit == 0 ? 65536u : std::min(history_limit, 65536u);  // resident.rs:715-719
     t.recency = recency >= 1.0f ? recency : 1.0f;
     t.measured_retention = fmax_(measured_retention, 0.0f);
@@ -2800,10 +2813,20 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
     struct Measure { uint32_t record; I3 mn; uint32_t sx, sy, sz, voxels; float flux; };
     std::vector<Measure> measures;
     uint32_t record = n_events * 3u;
+    std::unordered_set<uint64_t> edited_voxels;
+    for (const I3 &v : edits) edited_voxels.insert(key_of(v));
     for (const OrcChange &c : nearby) {
+        // Urgent: an edited voxel lies inside the box (its distance is 0).
         bool urgent = false;
-        for (const I3 &v : edits)
-            if (bounds_distance2(c.mn, c.mx, v, v) == 0) { urgent = true; break; }
+        uint64_t volume = (uint64_t)(c.mx.x - c.mn.x + 1) * (uint64_t)(c.mx.y - c.mn.y + 1) * (uint64_t)(c.mx.z - c.mn.z + 1);
+        if (volume <= edits.size()) {
+            for (int z = c.mn.z; z <= c.mx.z && !urgent; ++z)
+                for (int y = c.mn.y; y <= c.mx.y && !urgent; ++y)
+                    for (int x = c.mn.x; x <= c.mx.x && !urgent; ++x) urgent = edited_voxels.count(key_of(I3{x, y, z})) != 0;
+        } else {
+            for (const I3 &v : edits)
+                if (bounds_distance2(c.mn, c.mx, v, v) == 0) { urgent = true; break; }
+        }
         uint32_t kind = c.kind == 0 ? 0u : (c.measure ? 6u : 3u);
         if (c.measure)
             measures.push_back(Measure{record, I3{c.mn.x - 1, c.mn.y - 1, c.mn.z - 1}, (uint32_t)(c.mx.x - c.mn.x + 3),

@@ -108,7 +108,8 @@ int orc_trace_faces(void *tracer, const uint32_t *tasks, uint32_t count, float *
                     uint32_t *hits, uint64_t *counters, int threads);
 
 /* ---- resident world cache (voxel_accumulate.wgsl, face_index.rs, resident.rs) ---- */
-int orc_cache_enable(void *tracer, uint32_t history_limit, float recency, float measured_retention);
+/* plain != 0: slots and accumulation as usual, but paths never end at cached faces. */
+int orc_cache_enable(void *tracer, uint32_t history_limit, float recency, float measured_retention, uint32_t plain);
 /* Registers faces (allocating slots as resident.rs:770-812) and writes their slots. */
 int orc_cache_register(void *tracer, const uint32_t *tasks, uint32_t count, uint32_t *slots);
 /* One batch: sequence from each slot's state, trace, accumulate. results as above

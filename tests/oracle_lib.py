@@ -68,7 +68,7 @@ def lib():
         "orc_tracer_sync": (ctypes.c_int, [P]),
         "orc_tracer_emitters": (i64, [P, P, i64]),
         "orc_trace_faces": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
-        "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32]),
+        "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32, u32]),
         "orc_cache_register": (ctypes.c_int, [P, P, u32, P]),
         "orc_cache_batch": (ctypes.c_int, [P, P, P, u32, P, P, P, ctypes.c_int, u32]),
         "orc_cache_state": (ctypes.c_int, [P, u32, u32, P]),
@@ -218,8 +218,8 @@ class Tracer:
         return oi, of
 
     # resident cache
-    def cache_enable(self, history_limit=0, recency=1.0, measured_retention=1.0):
-        lib().orc_cache_enable(self.h, history_limit, recency, measured_retention)
+    def cache_enable(self, history_limit=0, recency=1.0, measured_retention=1.0, plain=False):
+        lib().orc_cache_enable(self.h, history_limit, recency, measured_retention, int(plain))
 
     def cache_register(self, tasks):
         tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
@@ -227,14 +227,19 @@ class Tracer:
         lib().orc_cache_register(self.h, ptr(tasks), len(tasks), ptr(slots))
         return slots
 
-    def cache_batch(self, tasks, slots, hits=False, threads=8, clock_ms=0):
+    def cache_batch(self, tasks, slots, hits=False, threads=8, clock_ms=0, counters=False):
+        """prepare -> trace -> accumulate of `tasks` on their slots: (records,
+        hits or None) and, with counters=True, the counters [rays, steps,
+        header loads, brick loads, cell loads, paths, write_result direct
+        rays, 0] as a third item."""
         tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
         slots = np.ascontiguousarray(slots, np.uint32)
         n = len(tasks)
         results = np.zeros((n, 24), np.float32)
         h = np.zeros((n * self.samples, 8), np.uint32) if hits else None
-        lib().orc_cache_batch(self.h, ptr(tasks), ptr(slots), n, ptr(results), ptr(h), None, threads, clock_ms)
-        return results, h
+        c = np.zeros(8, np.uint64) if counters else None
+        lib().orc_cache_batch(self.h, ptr(tasks), ptr(slots), n, ptr(results), ptr(h), ptr(c), threads, clock_ms)
+        return (results, h, c) if counters else (results, h)
 
     def cache_state(self, first, count):
         out = np.zeros((count, 36), np.uint32)
