@@ -58,6 +58,28 @@ struct DeviceBuffer {
         if (e == hipSuccess) bytes = cap;
         return e;
     }
+    // Grow to `need` bytes keeping the first `keep` bytes: the copy follows
+    // the stream's earlier work, and the old allocation is freed only once
+    // the stream has passed it.
+    hipError_t grow_keep(size_t need, size_t keep, hipStream_t stream) {
+        if (need <= bytes && ptr) return hipSuccess;
+        size_t cap = 256;
+        while (cap < need) cap <<= 1;
+        void *fresh = nullptr;
+        hipError_t e = hipMalloc(&fresh, cap);
+        if (e != hipSuccess) return e;
+        keep = std::min(keep, bytes);
+        if (ptr && keep) e = hipMemcpyAsync(fresh, ptr, keep, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess && ptr) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(fresh);
+            return e;
+        }
+        if (ptr) e = hipFree(ptr);
+        ptr = fresh;
+        bytes = cap;
+        return e;
+    }
     hipError_t release() {
         hipError_t e = hipSuccess;
         if (ptr) e = hipFree(ptr);
@@ -153,6 +175,15 @@ struct vgi_tracer {
     DeviceBuffer b_tasks, b_slots, b_results, b_counters, b_hits;
     uint32_t bound = 0;
     uint64_t bound_hit_words = 0;  // first-hit words of the last frame run with VGI_RECORD_HITS (0: none)
+    // Changes of the bound set since the last frame (edits with bound faces
+    // that follow the exposed surface): the last frame's records stay in the
+    // rows of its own bound order, and reads map them to the faces bound
+    // now. Each entry (index, source): the face at `source` took the place
+    // of `index` and the set lost its last place; (NONE, 0): a face joined
+    // at the end (no record of the last frame).
+    std::vector<std::pair<uint32_t, uint32_t>> bound_moves;
+    uint32_t frame_bound = 0;        // bound faces when the last frame ran
+    bool bound_broken = false;       // an exposure update failed part way: rebind before the next frame
     float last_trace_ms = 0.0f;
     uint32_t frame_index = 0;
     bool frame_recorded = false;
@@ -172,6 +203,7 @@ struct vgi_tracer {
     size_t stage_bytes = 0, stage_used = 0;
     hipEvent_t stage_done = nullptr;
     bool stage_pending = false;
+    uint32_t stage_wraps = 0, stage_grows = 0;  // times the region started over / was allocated larger
     std::vector<uint32_t> bound_material;  // material of each bound face's voxel
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
@@ -625,6 +657,7 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
     // the first wait on the stream: a caller that edits while a frame is in
     // flight overlaps them with that frame.
     float regions_ms = ms_since(t_start);
+    std::string exposure_error;
     if (events) {
         auto t_event = clock::now();
         // The retention event (SceneChanges::with_materials .. SpatialHistory::event).
@@ -648,7 +681,13 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         t.timings[6] = ms_since(t_event);
         if (t.follow_exposure && !changed.empty()) {
             auto t_exposure = clock::now();
-            if (update_exposure(t, changed, deltas) != 0) return -1;
+            if (update_exposure(t, changed, deltas) != 0) {
+                // The scene edit itself still reaches the GPU below; the
+                // bound set is left part way and frames refuse it until the
+                // caller binds again.
+                exposure_error = g_error;
+                t.bound_broken = true;
+            }
             t.timings[7] = ms_since(t_exposure);
         }
     }
@@ -684,11 +723,13 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
         t.top.update(t.bmin, t.bmax, keys, barrier_map(t));
     }
     if (upload_top(t) != 0) return -1;
-    hipEventSynchronize(t.ev[7]);
-    hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    HIP_OK(hipEventSynchronize(t.ev[7]));
+    HIP_OK(hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]));
     t.timings[5] = ms_since(t_top);
+    if (!exposure_error.empty())
+        return fail("the edit was applied, but the bound faces could not follow it (" + exposure_error +
+                    "): bind faces again before the next frame");
     return 0;
-
 }
 
 // ---------------------------------------------------------------- optics / emitters
@@ -758,20 +799,34 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
 // ---------------------------------------------------------------- staged uploads
 // `bytes` from host `src` to device `dst` on the tracer's stream through the
 // pinned staging region: the host may reuse `src` at once and does not wait.
+const size_t STAGE_LIMIT = (size_t)64 << 20;  // largest pinned staging region per tracer
+
 int staged_upload(vgi_tracer &t, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return 0;
     size_t need = (bytes + 255) & ~(size_t)255;
     if (t.stage_used + need > t.stage_bytes) {
+        bool had = t.stage_bytes > 0;
         if (t.stage_pending) HIP_OK(hipEventSynchronize(t.stage_done));
         t.stage_pending = false;
         t.stage_used = 0;
         if (need > t.stage_bytes) {
+            // Grow by doubling from 256 KiB to what one edit needs; an upload
+            // beyond the limit goes from the caller's memory and waits.
+            if (need > STAGE_LIMIT) {
+                HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, t.stream));
+                HIP_OK(hipStreamSynchronize(t.stream));
+                return 0;
+            }
             if (t.stage) HIP_OK(hipHostFree(t.stage));
             t.stage = nullptr;
             t.stage_bytes = 0;
-            size_t cap = std::max<size_t>(need, (size_t)32 << 20);
+            size_t cap = (size_t)256 << 10;
+            while (cap < need) cap <<= 1;
             HIP_OK(hipHostMalloc((void **)&t.stage, cap, hipHostMallocDefault));
             t.stage_bytes = cap;
+            t.stage_grows += 1;
+        } else if (had) {
+            t.stage_wraps += 1;
         }
     }
     uint8_t *p = t.stage + t.stage_used;
@@ -1070,6 +1125,38 @@ int compute_order(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
 
 const uint32_t *order_of(const vgi_tracer &t) { return t.has_order ? t.d_order.as<uint32_t>() : nullptr; }
 
+// The first `count` rows (of `row_bytes`) of a per-bound-face buffer of the
+// last frame, for the faces bound now: while no edit has moved the bound
+// set since that frame the rows are read as they are; otherwise the moves
+// are replayed over the frame's row numbers, and a face that joined since
+// reads zeros.
+int read_bound_rows(const vgi_tracer &t, const void *rows, size_t row_bytes, uint32_t count, void *out) {
+    if (t.bound_moves.empty()) {
+        HIP_OK(hipMemcpy(out, rows, (size_t)count * row_bytes, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<uint32_t> row_of(t.frame_bound);
+    for (uint32_t i = 0; i < t.frame_bound; ++i) row_of[i] = i;
+    for (const auto &m : t.bound_moves) {
+        if (m.first == vgi_host::NONE) {
+            row_of.push_back(vgi_host::NONE);
+            continue;
+        }
+        if (m.second >= row_of.size() || m.first >= row_of.size()) return fail("bound-set moves out of range");
+        row_of[m.first] = row_of[m.second];
+        row_of.pop_back();
+    }
+    if (row_of.size() < count) return fail("bound-set moves disagree with the bound count");
+    std::vector<uint8_t> frame((size_t)t.frame_bound * row_bytes);
+    if (!frame.empty()) HIP_OK(hipMemcpy(frame.data(), rows, frame.size(), hipMemcpyDeviceToHost));
+    uint8_t *dst = static_cast<uint8_t *>(out);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (row_of[i] == vgi_host::NONE) std::memset(dst + (size_t)i * row_bytes, 0, row_bytes);
+        else std::memcpy(dst + (size_t)i * row_bytes, frame.data() + (size_t)row_of[i] * row_bytes, row_bytes);
+    }
+    return 0;
+}
+
 // Bound faces of a frame (learned frames: with `learned` and `queries`).
 int trace_bound_faces(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o, const uint4 *tasks, uint32_t count,
                       uint32_t samples, uint32_t lanes, uint32_t sequence_base, float4 *results, uint32_t *hits,
@@ -1293,6 +1380,7 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
     std::vector<uint32_t> dirty_index;
     for (uint32_t b : removals) {
         uint32_t last = (uint32_t)t.bound_tasks.size() - 1;
+        t.bound_moves.push_back({b, last});
         t.bound_of_slot[t.bound_slots[b]] = vgi_host::NONE;
         if (b != last) {
             t.bound_tasks[b] = t.bound_tasks[last];
@@ -1307,6 +1395,7 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
     if (register_faces(t, additions.data(), (uint32_t)additions.size(), new_slots) != 0) return -1;
     for (size_t k = 0; k < additions.size(); ++k) {
         uint32_t b = (uint32_t)t.bound_tasks.size();
+        t.bound_moves.push_back({vgi_host::NONE, 0u});
         t.bound_tasks.push_back(additions[k]);
         t.bound_slots.push_back(new_slots[k]);
         if (t.bound_of_slot.size() <= new_slots[k]) t.bound_of_slot.resize((size_t)new_slots[k] + 1, vgi_host::NONE);
@@ -1318,9 +1407,12 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
     dirty_index.erase(std::unique(dirty_index.begin(), dirty_index.end()), dirty_index.end());
     dirty_index.erase(std::remove_if(dirty_index.begin(), dirty_index.end(), [&](uint32_t i) { return i >= count; }), dirty_index.end());
     size_t need_tasks = (size_t)std::max<uint32_t>(count, 1) * 32;
-    bool whole = need_tasks > t.b_tasks.bytes;
+    size_t need_slots = (size_t)std::max<uint32_t>(count, 1) * 4;
+    bool whole = need_tasks > t.b_tasks.bytes || need_slots > t.b_slots.bytes || t.b_slots.ptr == nullptr;
     if (whole) {
-        // Grown past the allocation: upload the whole set again.
+        // Grown past the allocation: upload the whole set again (after the
+        // stream has passed the frame that may still read the old set).
+        HIP_OK(hipStreamSynchronize(t.stream));
         HIP_OK(t.b_tasks.reserve(need_tasks));
         HIP_OK(t.b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
         HIP_OK(hipMemcpyAsync(t.b_tasks.ptr, t.bound_tasks.data(), (size_t)count * 32, hipMemcpyHostToDevice, t.stream));
@@ -1340,7 +1432,7 @@ int update_exposure(vgi_tracer &t, const std::vector<vgi_host::Int3> &changed, c
                                   (uint32_t)dirty_index.size(), t.b_tasks.as<uint4>(), t.b_slots.as<uint32_t>(), t.stream);
         HIP_OK(hipGetLastError());
     }
-    HIP_OK(t.b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
+    HIP_OK(t.b_results.grow_keep((size_t)std::max<uint32_t>(count, 1) * 96, (size_t)t.frame_bound * 96, t.stream));
     // The whole set travels from pageable memory: wait for it.
     if (whole) HIP_OK(hipStreamSynchronize(t.stream));
     t.bound = count;
@@ -1645,14 +1737,18 @@ int vgi_bind_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count) {
     HIP_OK(hipMemcpyAsync(t->b_tasks.ptr, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
     HIP_OK(t->b_results.reserve((size_t)std::max<uint32_t>(count, 1) * 96));
     HIP_OK(t->b_counters.reserve(64));
+    HIP_OK(t->b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
     t->follow_exposure = false;
+    t->bound_broken = false;
+    t->bound_moves.clear();
+    t->frame_bound = 0;
+    t->bound_hit_words = 0;
     t->bound_tasks.clear();
     t->bound_slots.clear();
     t->bound_of_slot.clear();
     if (t->resident) {
         std::vector<uint32_t> slots;
         if (register_faces(*t, tasks, count, slots) != 0) return -1;
-        HIP_OK(t->b_slots.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
         HIP_OK(hipMemcpyAsync(t->b_slots.ptr, slots.data(), (size_t)count * 4, hipMemcpyHostToDevice, t->stream));
         t->last_slots = slots;
     }
@@ -1720,6 +1816,7 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     bool record_hits = (flags & VGI_RECORD_HITS) != 0;
     if (record_hits && t->learned)
         return fail("learned frames keep no first-hit records (their paths end in the network)");
+    if (t->bound_broken) return fail("the bound faces did not follow the last edit: bind faces again");
     if (sync_emitters(*t) != 0) return -1;
     vgi::SceneView s;
     vgi::Optics o;
@@ -1727,6 +1824,8 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     uint32_t count = t->bound;
     bool counting = (flags & (VGI_COUNT_RAYS | VGI_COUNT_GATHERS)) != 0;
     t->bound_hit_words = 0;
+    t->bound_moves.clear();
+    t->frame_bound = count;
     uint32_t *hits = nullptr;
     if (record_hits) {
         uint64_t words = (uint64_t)count * t->transport.samples * 8u;
@@ -1803,10 +1902,12 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
 int vgi_read_bound_hits(vgi_tracer *t, uint32_t *out, uint64_t words) {
     if (!t || (words && !out)) return fail("null argument");
     if (t->bound_hit_words == 0) return fail("the last frame kept no first-hit records (run it with VGI_RECORD_HITS)");
-    if (words > t->bound_hit_words) return fail("more hit words than the last frame recorded");
+    uint64_t row_words = (uint64_t)t->transport.samples * 8u;
+    if (row_words == 0 || words > (uint64_t)t->bound * row_words) return fail("more hit words than the bound faces' samples");
+    if (words % row_words) return fail("hit words must cover whole faces (samples x 8 words each)");
     HIP_OK(hipSetDevice(t->device));
     HIP_OK(hipStreamSynchronize(t->stream));
-    if (words) HIP_OK(hipMemcpy(out, t->b_hits.ptr, words * 4, hipMemcpyDeviceToHost));
+    if (words && read_bound_rows(*t, t->b_hits.ptr, row_words * 4, (uint32_t)(words / row_words), out) != 0) return -1;
     return 0;
 }
 
@@ -2035,8 +2136,9 @@ int vgi_set_wavefront(vgi_tracer *t, int on) {
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
     if (!t || (count && !out)) return fail("null argument");
     if (count > t->bound) return fail("more records than bound faces");
+    HIP_OK(hipSetDevice(t->device));
     HIP_OK(hipStreamSynchronize(t->stream));
-    if (count) HIP_OK(hipMemcpy(out, t->b_results.ptr, (size_t)count * 96, hipMemcpyDeviceToHost));
+    if (count && read_bound_rows(*t, t->b_results.ptr, 96, count, out) != 0) return -1;
     if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 40, hipMemcpyDeviceToHost));
     return 0;
 }
@@ -2332,7 +2434,8 @@ int64_t vgi_scene_stats(vgi_tracer *t, int64_t out[8]) {
     int64_t loaded = 0, solid = 0;
     for (size_t i = 0; i < t->loaded.size(); ++i) { loaded += t->loaded[i]; solid += t->loaded[i] && t->solid[i]; }
     out[0] = loaded; out[1] = solid; out[2] = t->cells_used; out[3] = (int64_t)t->top.fields.size();
-    out[4] = t->high_water; out[5] = (int64_t)t->emitters.faces; out[6] = (int64_t)t->emitters.nodes; out[7] = 0;
+    out[4] = t->high_water; out[5] = (int64_t)t->emitters.faces; out[6] = (int64_t)t->emitters.nodes;
+    out[7] = ((int64_t)t->stage_grows << 32) | t->stage_wraps;
     return loaded;
 }
 
@@ -2511,6 +2614,11 @@ int vgi_batch_run_frame(vgi_batch *b) {
         c.samples = b->samples;
         HIP_OK(hipMemcpyAsync(b->d_control.ptr, &c, sizeof(c), hipMemcpyHostToDevice, b->stream));
         b->sequence_dirty = false;
+    }
+    for (vgi_tracer *t : b->scenes) {
+        t->bound_moves.clear();
+        t->frame_bound = t->bound;
+        t->bound_hit_words = 0;
     }
     HIP_OK(hipEventRecord(b->ev[0], b->stream));
     HIP_OK(hipGraphLaunch(b->exec, b->stream));

@@ -98,6 +98,9 @@ int vgi_upload_pages(vgi_tracer *tracer, const int32_t *page_xyz, uint32_t pages
  * (apply_events, voxel_accumulate.wgsl:295-332) before its prepare. With
  * vgi_bind_exposed, bound faces follow the edit (pages.rs:451-458 rescan). */
 int vgi_edit_voxels(vgi_tracer *tracer, const int32_t *xyz, const uint16_t *values, uint32_t count);
+/* (An edit whose bound-face update fails, e.g. at the resident slot limit,
+ * is still applied to the scene; it returns the error, and frames refuse
+ * the bound set until faces are bound again.) */
 
 /* ---- optics and sources: VoxelGiMaterials (world.rs:90-174) as packed by
  * gpu.rs:714-727; TransportSources (sources.rs:105-160) as packed by
@@ -154,7 +157,11 @@ int vgi_set_wavefront(vgi_tracer *tracer, int on);
 /* The bound faces' 96-byte records after the last frame (out may be NULL with
  * count 0 to read only the counters); counters (optional)
  * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
- * the frame ran with VGI_COUNT_RAYS. */
+ * the frame ran with VGI_COUNT_RAYS. Records follow the faces bound now: after
+ * an edit that moved bound faces (vgi_bind_exposed), including one issued
+ * while the frame still ran, each face reads its record of the last frame,
+ * and a face bound since reads zeros -- the same records in either call
+ * order. */
 int vgi_read_bound(vgi_tracer *tracer, vgi_face_result *out, uint32_t count, uint64_t *counters);
 /* First-hit records of the last bound-face frame run with VGI_RECORD_HITS (as
  * vgi_read_hits: 8 u32 per face-sample, bound face i's samples at
@@ -339,6 +346,9 @@ int vgi_scene_bounds(vgi_tracer *tracer, int32_t out[6]);
 /* The canonical per-region form (see oracle): kind, material, top-level
  * bytes, 64 brick words (pointer replaced by tag 2), 64x32 brick cell words. */
 int vgi_scene_region_canonical(vgi_tracer *tracer, int32_t rx, int32_t ry, int32_t rz, uint32_t *out);
+/* [loaded regions, solid regions, cell words, top-level cells, resident
+ * slots, emissive faces, light-tree nodes, pinned staging (allocations << 32 |
+ * times the region started over)] */
 int64_t vgi_scene_stats(vgi_tracer *tracer, int64_t out[8]);
 int64_t vgi_emitter_words(vgi_tracer *tracer, uint32_t *out, int64_t capacity);
 
