@@ -27,6 +27,7 @@ COUNT_RAYS = 2
 COUNT_GATHERS = 4
 MLP_TABLE = 8 * (1 << 18) * 2  # hash-grid entries (level-major, 2 features each)
 MLP_PARAMS = 14528
+NOALLOC_MAX_FACES = 262144
 
 FACE_TASK = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("sequence", "<u4"), ("padding", "<u4", (3,))])
 assert FACE_TASK.itemsize == 32
@@ -95,6 +96,7 @@ def library():
         "vgi_submit_faces": (ctypes.c_int, [P, P, u32, u32, ctypes.POINTER(u64)]),
         "vgi_poll": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_int)]),
         "vgi_read_faces": (ctypes.c_int, [P, u64, P, u32]),
+        "vgi_trace_faces_noalloc": (ctypes.c_int, [P, P, u32, P]),
         "vgi_read_hits": (ctypes.c_int, [P, u64, P, u64]),
         "vgi_read_counters": (ctypes.c_int, [P, u64, P, u32]),
         "vgi_timings": (ctypes.c_int, [P, P, u32]),
@@ -170,7 +172,7 @@ def exported_symbols():
     """Names the C ABI declares (include/vgi.h)."""
     return [
         "vgi_create", "vgi_destroy", "vgi_enable_resident", "vgi_set_clock", "vgi_upload_pages", "vgi_edit_voxels",
-        "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_poll",
+        "vgi_set_materials", "vgi_set_lights", "vgi_set_transport", "vgi_submit_faces", "vgi_trace_faces_noalloc", "vgi_poll",
         "vgi_read_faces", "vgi_read_hits", "vgi_read_counters", "vgi_timings", "vgi_read_slots",
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound", "vgi_read_bound_hits",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
@@ -357,6 +359,14 @@ class GpuTracer:
         _check(self._lib.vgi_submit_faces(self._h, _ptr(tasks), len(tasks), flags, ctypes.byref(ticket)))
         self._last_count = len(tasks)
         return ticket.value
+
+    def trace_faces_noalloc(self, tasks):
+        """The records of `tasks` traced without any device allocation
+        (at most NOALLOC_MAX_FACES faces; non-resident tracers only)."""
+        tasks = np.ascontiguousarray(tasks, FACE_TASK)
+        out = np.zeros((len(tasks), 24), np.float32)
+        _check(self._lib.vgi_trace_faces_noalloc(self._h, _ptr(tasks), len(tasks), _ptr(out)))
+        return out
 
     def snapshot_ready(self, ticket):
         ready = ctypes.c_int()

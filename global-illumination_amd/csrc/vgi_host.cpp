@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -28,6 +29,17 @@ using vgi_host::Int3;
 namespace {
 
 thread_local std::string g_error;
+
+// The device scratch of vgi_trace_faces_noalloc: face tasks, then records,
+// for at most VGI_NOALLOC_MAX_FACES faces (128 bytes each: 32 MiB). Each GPU
+// the library runs on holds its own copy until the process ends; host code
+// uses only the address resolved per device (scratch_of), never the name.
+constexpr size_t NOALLOC_TASKS = 0, NOALLOC_RECORDS = (size_t)VGI_NOALLOC_MAX_FACES * 32;
+constexpr size_t NOALLOC_BYTES = (size_t)VGI_NOALLOC_MAX_FACES * 128;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch[MAX_DEVICES] = {};
 
 int fail(const std::string &message) {
     g_error = message;
@@ -733,7 +745,7 @@ int apply_edits(vgi_tracer &t, const int32_t *xyz, const uint16_t *values, uint3
 }
 
 // ---------------------------------------------------------------- optics / emitters
-int sync_emitters(vgi_tracer &t) {
+int sync_emitters(vgi_tracer &t, bool no_alloc = false) {
     if (!t.emitters_dirty) return 0;
     vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
     uint64_t emitting[4] = {0, 0, 0, 0};
@@ -753,6 +765,9 @@ int sync_emitters(vgi_tracer &t) {
                         regions.push_back(Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z});
                 }
     vgi_host::build_emitters(src, regions, t.emission, t.emitters);
+    if (no_alloc && (t.emitters.words.size() * 4 > t.d_emitters.bytes || t.d_emitters.ptr == nullptr))
+        return fail("the emitter table outgrew its device buffer: one allocating call (vgi_submit_faces or vgi_run_frame) "
+                    "must follow this scene or material change");
     HIP_OK(t.d_emitters.reserve(t.emitters.words.size() * 4));
     HIP_OK(hipMemcpyAsync(t.d_emitters.ptr, t.emitters.words.data(), t.emitters.words.size() * 4, hipMemcpyHostToDevice, t.stream));
     HIP_OK(hipStreamSynchronize(t.stream));
@@ -1454,6 +1469,18 @@ int vgi_create(int device_ordinal, vgi_tracer **out) {
     HIP_OK(hipGetDeviceCount(&count));
     if (device_ordinal < 0 || device_ordinal >= count) return fail("no such HIP device");
     HIP_OK(hipSetDevice(device_ordinal));
+    if (device_ordinal >= MAX_DEVICES) return fail("device ordinal beyond the library's device table");
+    {
+        // This device's copy of the no-allocation scratch, resolved once
+        // with the device current.
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (!g_scratch[device_ordinal]) {
+            void *ptr = nullptr;
+            HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            if (!ptr) return fail("no device address for the scratch region");
+            g_scratch[device_ordinal] = ptr;
+        }
+    }
     std::unique_ptr<vgi_tracer> t(new vgi_tracer());
     t->device = device_ordinal;
     HIP_OK(hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, device_ordinal));
@@ -1657,6 +1684,40 @@ int vgi_submit_faces(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, 
     uint64_t id = t->next_ticket++;
     t->tickets[id] = std::move(k);
     *ticket = id;
+    return 0;
+}
+
+int vgi_trace_faces_noalloc(vgi_tracer *t, const vgi_face_task *tasks, uint32_t count, vgi_face_result *out) {
+    if (!t || (count && (!tasks || !out))) return fail("null argument");
+    if (count > VGI_NOALLOC_MAX_FACES)
+        return fail("vgi_trace_faces_noalloc takes at most " + std::to_string(VGI_NOALLOC_MAX_FACES) + " faces per call (got " +
+                    std::to_string(count) + ")");
+    if (t->resident) return fail("vgi_trace_faces_noalloc traces without the resident cache (use vgi_submit_faces)");
+    if (!t->have_regions) return fail("no scene loaded");
+    if (count == 0) return 0;
+    HIP_OK(hipSetDevice(t->device));
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        base = g_scratch[t->device];
+    }
+    if (!base) return fail("the scratch region of this device was not resolved");
+    if (sync_emitters(*t, true) != 0) return -1;
+    uint8_t *scratch_ptr = static_cast<uint8_t *>(base);
+    uint4 *d_tasks = reinterpret_cast<uint4 *>(scratch_ptr + NOALLOC_TASKS);
+    float4 *d_records = reinterpret_cast<float4 *>(scratch_ptr + NOALLOC_RECORDS);
+    HIP_OK(hipMemcpyAsync(d_tasks, tasks, (size_t)count * 32, hipMemcpyHostToDevice, t->stream));
+    // The scratch keeps the last call's records: clear them (faces marked
+    // VGI_NO_FACE are not written by the trace).
+    vgi::launch_fill_words(reinterpret_cast<uint32_t *>(d_records), (uint64_t)count * 24, 0u, t->stream);
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    vgi::launch_trace(s, o, d_tasks, count, t->transport.samples, lanes_for(t->transport), 0u, d_records, nullptr, nullptr, false,
+                      t->stream, nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, d_records, (size_t)count * 96, hipMemcpyDeviceToHost, t->stream));
+    HIP_OK(hipStreamSynchronize(t->stream));
     return 0;
 }
 

@@ -162,5 +162,6 @@ void launch_sun_corners(const SceneView &scene, const Optics &optics, const uint
                         uint32_t refresh, uint32_t *ranges, uint32_t *region, hipStream_t stream);
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
+void launch_fill_words(uint32_t *words, uint64_t n, uint32_t value, hipStream_t stream);
 
 }  // namespace vgi

@@ -1175,6 +1175,17 @@ void launch_footprints(const SceneView &scene, const Optics &optics, const int4 
     hipLaunchKernelGGL(footprint_finish_kernel, dim3(blocks(task_count, 64)), dim3(64), 0, stream, tasks, sums, events);
 }
 
+// Fills `words` u32 with `value` (a kernel, so it may sit in captured graphs).
+__global__ void fill_words_kernel(uint32_t *__restrict__ words, uint64_t n, uint32_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) words[i] = value;
+}
+
+void launch_fill_words(uint32_t *words, uint64_t n, uint32_t value, hipStream_t stream) {
+    if (n == 0) return;
+    uint64_t want = (n + 255) / 256;
+    hipLaunchKernelGGL(fill_words_kernel, dim3((uint32_t)(want < 4096 ? want : 4096)), dim3(256), 0, stream, words, n, value);
+}
+
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream) {
     size_t n = bytes / 16;
     if (n == 0) return;

@@ -120,6 +120,20 @@ int vgi_set_transport(vgi_tracer *tracer, const vgi_transport *transport);
 #define VGI_COUNT_GATHERS 4u /* frames: also steps and traversal gathers (tickets always count all) */
 int vgi_submit_faces(vgi_tracer *tracer, const vgi_face_task *tasks, uint32_t count, uint32_t flags,
                      uint64_t *ticket);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture elsewhere in the process, tight
+ * serving loops, harnesses that forbid hipMalloc): the trace of
+ * vgi_submit_faces + vgi_read_faces on a tracer without the resident cache,
+ * synchronous, with the tasks and records staged in a fixed device scratch
+ * region (VGI_NOALLOC_MAX_FACES faces, 128 bytes each: 32 MiB per GPU, held
+ * until the process ends) instead of buffers the call allocates. The same
+ * records bit for bit. It allocates nothing: more faces than the maximum, a
+ * resident tracer, or an emitter table that would have to grow are errors
+ * (set-up calls allocate; run one vgi_submit_faces after a scene or material
+ * change). Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (several threads, several tracers) race. */
+#define VGI_NOALLOC_MAX_FACES 262144u
+int vgi_trace_faces_noalloc(vgi_tracer *tracer, const vgi_face_task *tasks, uint32_t count, vgi_face_result *out);
 /* GpuTracer::snapshot_ready (gpu.rs:1024-1049) */
 int vgi_poll(vgi_tracer *tracer, uint64_t ticket, int *ready);
 /* GpuTracer::complete_readback (gpu.rs:1050-1170): 96-byte records. */
