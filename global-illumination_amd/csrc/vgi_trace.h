@@ -108,20 +108,49 @@ __device__ __forceinline__ uint4 region_header(const SceneView &s, int rx, int r
     return s.headers[i];
 }
 
+// n / d rounded to binary32 as IEEE division rounds it, from r = RN64(1/d)
+// (binary64, correctly rounded, taken once per ray): RN32(RN64(n * r)).
+// Why it is exact: RN64(n * r) = (n/d)(1 + e) with |e| <= 2^-52 (1 + 2^-53).
+// A binary32 quotient n/d is never a midpoint between two binary32 numbers
+// (a 25-bit odd significand times d's significand has more than 24
+// significant bits unless d is a power of two, and then n/d is exact), and
+// when it is not exact it lies at least 2^-49 (relative) from every
+// midpoint: with n = N 2^a, d = D 2^b, midpoint M 2^c (N, D < 2^24,
+// M < 2^25), n - d m is a nonzero multiple of 2^(b+c) and d m < 2^(49+b+c).
+// So both roundings of the binary64 product land on the binary32 value of
+// n/d (the final rounding is to nearest even, as IEEE division's). The
+// quotients here stay far from binary32 overflow and subnormals (|n| <
+// 2^20, 1e-8 < |d| <= 1). It costs two conversions and one binary64
+// multiply instead of the ~10-instruction correctly rounded binary32 divide.
+__device__ __forceinline__ double recip64(float d) { return 1.0 / (double)d; }
+__device__ __forceinline__ float div_by(float n, double r) { return (float)((double)n * r); }
+
+// The reciprocals a ray's box steps divide by: 1/d for an axis the ray moves
+// along (|d| > 1e-8), 1 otherwise (voxel_traversal.wgsl:94-99 divides by 1).
+struct RayInverse {
+    double x, y, z;
+};
+__device__ __forceinline__ RayInverse ray_inverse(f3 direction) {
+    return RayInverse{fabsf(direction.x) > 1e-8f ? recip64(direction.x) : 1.0,
+                      fabsf(direction.y) > 1e-8f ? recip64(direction.y) : 1.0,
+                      fabsf(direction.z) > 1e-8f ? recip64(direction.z) : 1.0};
+}
+
 // voxel_traversal.wgsl:90-107
-__device__ __forceinline__ void advance_box(f3 position, f3 direction, int cx, int cy, int cz, int nx, int ny, int nz,
-                                            int px, int py, int pz, float size, f3 &normal_out, float &dist_out) {
+__device__ __forceinline__ void advance_box(f3 position, f3 direction, const RayInverse &inv, int cx, int cy, int cz, int nx, int ny,
+                                            int nz, int px, int py, int pz, float size, f3 &normal_out, float &dist_out) {
     float mnx = (float)(cx - nx) * size, mny = (float)(cy - ny) * size, mnz = (float)(cz - nz) * size;
     float mxx = (float)(cx + px + 1) * size, mxy = (float)(cy + py + 1) * size, mxz = (float)(cz + pz + 1) * size;
     float tr[3];
     const float dv[3] = {direction.x, direction.y, direction.z};
     const float pv[3] = {position.x, position.y, position.z};
+    const double iv[3] = {inv.x, inv.y, inv.z};
     const float lo[3] = {mnx, mny, mnz}, hi[3] = {mxx, mxy, mxz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         bool moving = fabsf(dv[a]) > 1e-8f;
         float plane = dv[a] > 0.0f ? hi[a] : lo[a];
-        float raw = (plane - pv[a]) / (moving ? dv[a] : 1.0f);
+        float raw = div_by(plane - pv[a], iv[a]);  // (plane - p) / (moving ? d : 1)
         tr[a] = (moving && raw >= -0.001f && raw < 1e30f) ? raw : 1e30f;
     }
     bool choose_y = tr[1] < fmx(tr[0], 0.0f);
@@ -138,6 +167,7 @@ __device__ __forceinline__ void advance_box(f3 position, f3 direction, int cx, i
 __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 direction, float max_distance) {
     f3 position = start;
     float remaining = max_distance;
+    const RayInverse inv = ray_inverse(direction);
     for (uint32_t step = 0; step < 2048u; ++step) {
         int rx = (int)floorf(position.x / 16.0f), ry = (int)floorf(position.y / 16.0f), rz = (int)floorf(position.z / 16.0f);
         if (remaining < 0.0f || !in_bounds(s, rx, ry, rz)) return false;
@@ -145,7 +175,7 @@ __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 
         if (h.x != 0u && h.x != 1u) return true;
         f3 n;
         float travel;
-        advance_box(position, direction, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, travel);
+        advance_box(position, direction, inv, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, travel);
         position = position + direction * travel;
         remaining = remaining - travel;
     }
@@ -158,6 +188,7 @@ __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 
 // one ends (the wavefront tracer); trace_ray runs one ray to its end.
 struct RayWalk {
     f3 origin, direction;
+    RayInverse inv;
     float max_distance;
     f3 position;
     float distance;
@@ -179,6 +210,8 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
     tally.ray();
     float near_ = -1e30f, far_ = 1e30f;
     f3 normal = mk(0.0f, 0.0f, 0.0f);
+    const RayInverse inv = ray_inverse(direction);
+    const double iv[3] = {inv.x, inv.y, inv.z};
     const float dv[3] = {direction.x, direction.y, direction.z};
     const float ov[3] = {origin.x, origin.y, origin.z};
     const float lo[3] = {(float)s.minx * 16.0f, (float)s.miny * 16.0f, (float)s.minz * 16.0f};
@@ -189,8 +222,8 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
             if (ov[axis] < lo[axis] || ov[axis] > hi[axis]) return true;
             continue;
         }
-        float first = (lo[axis] - ov[axis]) / dv[axis];
-        float second = (hi[axis] - ov[axis]) / dv[axis];
+        float first = div_by(lo[axis] - ov[axis], iv[axis]);   // (lo - o) / d
+        float second = div_by(hi[axis] - ov[axis], iv[axis]);  // (hi - o) / d
         if (fmn(first, second) > near_) {
             near_ = fmn(first, second);
             float sg = first < second ? -1.0f : 1.0f;
@@ -201,6 +234,7 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
     if (far_ < fmx(near_, 0.0f)) return true;
     w.origin = origin;
     w.direction = direction;
+    w.inv = inv;
     w.max_distance = max_distance;
     w.distance = fmx(near_, 0.0f) + 0.001f;
     w.position = origin + direction * w.distance;
@@ -241,7 +275,7 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
             if constexpr (kVisible) {
                 f3 n;
                 float skip;
-                advance_box(position, direction, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, skip);
+                advance_box(position, direction, w.inv, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, skip);
                 w.distance = w.distance + skip;
                 w.position = position + direction * skip;
                 w.normal = n;
@@ -254,13 +288,15 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
         w.lrx = rx; w.lry = ry; w.lrz = rz;
     }
     const uint4 meta = w.meta;
-    f3 adv_n = mk(0.0f, 0.0f, 0.0f);
-    float adv_w = 0.0f;
-    if (meta.x == 1u) {
-        advance_box(position, direction, rx, ry, rz, (int)(meta.z & 255u), (int)((meta.z >> 8) & 255u),
-                    (int)((meta.z >> 16) & 255u), (int)(meta.w & 255u), (int)((meta.w >> 8) & 255u),
-                    (int)((meta.w >> 16) & 255u), 16.0f, adv_n, adv_w);
-    } else {
+    // The empty box to skip -- the region's top-level box (kind 1), an air
+    // brick's box, or an air voxel's box -- found per lane, then one
+    // advance_box for every lane (lanes at different levels share it).
+    int cx = rx, cy = ry, cz = rz;
+    uint32_t neg = meta.z, pos = meta.w;  // region level: 8 bits per axis
+    uint32_t shift = 8u, field = 255u;
+    float size = 16.0f;
+    bool skip = meta.x == 1u;
+    if (!skip) {
         uint32_t material = meta.y;
         if (meta.x == 3u) {
             uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
@@ -270,9 +306,11 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
             uint32_t st = brick & 3u;
             if (st == 0u) {
                 uint32_t e = brick >> 2u;
-                advance_box(position, direction, rx * 4 + (int)bx, ry * 4 + (int)by, rz * 4 + (int)bz, (int)(e & 3u),
-                            (int)((e >> 4) & 3u), (int)((e >> 8) & 3u), (int)((e >> 2) & 3u), (int)((e >> 6) & 3u),
-                            (int)((e >> 10) & 3u), 4.0f, adv_n, adv_w);
+                cx = rx * 4 + (int)bx; cy = ry * 4 + (int)by; cz = rz * 4 + (int)bz;
+                neg = e; pos = e >> 2;  // 2 bits per axis, 4 apart
+                shift = 4u; field = 3u;
+                size = 4.0f;
+                skip = true;
                 material = AIR;
             } else if (st == 1u) {
                 material = brick >> 2u;
@@ -284,8 +322,11 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
                 material = (cell & 1u) != 0u ? cell >> 1u : AIR;
                 if (material == AIR) {
                     uint32_t e = cell >> 1u;
-                    advance_box(position, direction, vx, vy, vz, (int)(e & 3u), (int)((e >> 4) & 3u), (int)((e >> 8) & 3u),
-                                (int)((e >> 2) & 3u), (int)((e >> 6) & 3u), (int)((e >> 10) & 3u), 1.0f, adv_n, adv_w);
+                    cx = vx; cy = vy; cz = vz;
+                    neg = e; pos = e >> 2;
+                    shift = 4u; field = 3u;
+                    size = 1.0f;
+                    skip = true;
                 }
             }
         }
@@ -295,6 +336,12 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
             return true;
         }
     }
+    f3 adv_n = mk(0.0f, 0.0f, 0.0f);
+    float adv_w = 0.0f;
+    if (skip)
+        advance_box(position, direction, w.inv, cx, cy, cz, (int)(neg & field), (int)((neg >> shift) & field),
+                    (int)((neg >> (2u * shift)) & field), (int)(pos & field), (int)((pos >> shift) & field),
+                    (int)((pos >> (2u * shift)) & field), size, adv_n, adv_w);
     w.distance = w.distance + adv_w;
     w.position = position + direction * adv_w;
     w.normal = adv_n;
