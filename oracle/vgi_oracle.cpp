@@ -6,10 +6,14 @@
 // reference), compiled with -O2 -ffp-contract=off: every float operation is
 // one IEEE binary32 operation, in the order the WGSL writes it.
 //
-// Parity pins (see tests/): build_aadf against the reference's frozen
-// full-sweep construction (naadf/construction_tests.rs) and closed forms
-// (naadf.rs:201-208); the top level against the fixed-point invariant
-// (world.rs:596-684); packing against the byte KATs of gpu_scene.rs tests;
+// Parity pins (see tests/): build_aadf against an independent restatement
+// of the reference's frozen full-sweep construction
+// (naadf/construction_tests.rs) and closed forms (naadf.rs:201-208, runs along
+// one axis); regions against the frozen region constructor
+// (world/construction_tests.rs:8-107); the top level against the fixed-point
+// invariant (world.rs:596-684); packing against the KATs of gpu_scene.rs
+// tests (sizes, interning, decode); the face index against a map
+// (face_index.rs:231-261); pcg against an independently written inverse;
 // traversal against a brute-force voxel walk; the estimator against the
 // furnace closed forms E*sum(r^k) and E/(1-r) (gpu/view_tests.rs:387-459,
 // gpu/cache_tests.rs:43-101); emitter tables against emission.rs:401-432;
@@ -699,6 +703,8 @@ struct Packed {
     std::vector<uint32_t> headers;   // 4 per region
     std::vector<uint32_t> cells;
     uint32_t mask = 0;
+    uint32_t unique_bricks = 0;     // interned mixed bricks (gpu_scene.rs brick_lookup)
+    uint32_t brick_references = 0;  // table entries pointing at a mixed brick
     I3 minimum{0, 0, 0}, maximum{0, 0, 0};
 };
 
@@ -778,6 +784,7 @@ static void pack_scene(const Scene &scene, Packed &out) {
                         interned[data] = pointer;
                     }
                     table[index] = (pointer << 2) | 2u;
+                    out.brick_references += 1;
                 }
             }
             memcpy(&out.cells[offset], table, sizeof(table));
@@ -805,6 +812,7 @@ static void pack_scene(const Scene &scene, Packed &out) {
         capacity *= 2;
     }
     out.mask = (uint32_t)(capacity - 1);
+    out.unique_bricks = (uint32_t)interned.size();
     I3 mn{0, 0, 0}, mx{0, 0, 0};
     scene.bounds(mn, mx);
     // gpu.rs:808-812: the top level's bounds, which are the loaded regions' bounds.
@@ -1005,6 +1013,37 @@ struct FaceIndex {  // gpu/face_index.rs:9-188
         if ((len + 1) * 2 > entries.size()) grow();
         raw_insert({(uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (face << 29) | slot});
         ++len;
+    }
+    size_t home(const std::array<uint32_t, 4> &e) const {
+        return stream_id(I3{(int32_t)e[0], (int32_t)e[1], (int32_t)e[2]}, e[3] >> 29) & mask();
+    }
+    // face_index.rs remove: backward-shift deletion (no tombstones): after
+    // the hole, an entry moves back into it unless its home lies cyclically
+    // within (hole, position].
+    void remove(I3 v, uint32_t face) {
+        long f = find(v, face);
+        if (f < 0) return;
+        size_t m = mask(), hole = (size_t)f, next = hole;
+        --len;
+        for (;;) {
+            entries[hole] = {0, 0, 0, NONE};
+            for (;;) {
+                next = (next + 1) & m;
+                if (!live(entries[next])) return;
+                size_t ideal = home(entries[next]);
+                bool stays = hole <= next ? (hole < ideal && ideal <= next) : (hole < ideal || ideal <= next);
+                if (!stays) break;
+            }
+            entries[hole] = entries[next];
+            hole = next;
+        }
+    }
+    // The longest probe of a live entry from its home (face_index.rs max_probe).
+    size_t max_probe() const {
+        size_t m = mask(), worst = 0;
+        for (size_t i = 0; i < entries.size(); ++i)
+            if (live(entries[i])) worst = std::max(worst, (i + entries.size() - home(entries[i])) & m);
+        return worst;
     }
 };
 
@@ -2908,6 +2947,102 @@ int64_t orc_last_event(void *tp, uint32_t *out, int64_t capacity) {
 }
 
 uint32_t orc_curve_limit(uint32_t d2) { return curve_limit(d2); }
+
+// The packed scene of a tracer (gpu_scene.rs PackedScene): [cell words,
+// regions (headers), directory capacity, interned mixed bricks, brick-table
+// references to them, 0, 0, 0].
+int orc_packed_info(void *tp, int64_t out[8]) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    const Packed &p = t.packed;
+    int64_t v[8] = {(int64_t)p.cells.size(), (int64_t)(p.headers.size() / 4), (int64_t)(p.directory.size() / 4),
+                    (int64_t)p.unique_bricks, (int64_t)p.brick_references, 0, 0, 0};
+    memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+// A voxel's material decoded from the packed directory, headers and cells
+// alone (the decode oracle of gpu_scene.rs:604-632): AIR for an empty
+// region, 0xffff outside the packed regions.
+uint32_t orc_packed_material(void *tp, int32_t x, int32_t y, int32_t z) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    const Packed &p = t.packed;
+    I3 key{floordiv(x, 16), floordiv(y, 16), floordiv(z, 16)};
+    size_t m = p.mask, slot = hash_voxel(key) & m;
+    long header = -1;
+    for (int probe = 0; probe < 32; ++probe) {
+        const int32_t *e = &p.directory[slot * 4];
+        if (e[3] == -1) break;
+        if (e[0] == key.x && e[1] == key.y && e[2] == key.z) { header = e[3]; break; }
+        slot = (slot + 1) & m;
+    }
+    if (header < 0) return 0xffffu;
+    const uint32_t *h = &p.headers[(size_t)header * 4];
+    if (h[0] == 1u) return AIR;
+    if (h[0] == 2u) return h[1];
+    int lx = floormod(x, 16), ly = floormod(y, 16), lz = floormod(z, 16);
+    uint32_t entry = p.cells[h[1] + (size_t)lin(lx / 4, ly / 4, lz / 4, 4, 4)];
+    switch (entry & 3u) {
+    case 0u: return AIR;
+    case 1u: return entry >> 2;
+    case 2u: {
+        size_t index = lin(lx % 4, ly % 4, lz % 4, 4, 4);
+        uint32_t word = p.cells[(entry >> 2) + index / 2];
+        uint32_t cell = (word >> ((index & 1) * 16)) & 65535u;
+        return (cell & 1u) == 0u ? AIR : cell >> 1;
+    }
+    default: return 0xfffeu;
+    }
+}
+
+// The packed tables' words: which 0 = directory (4 per slot), 1 = headers
+// (4 per region), 2 = cells. Returns the word count, copies at most capacity.
+int64_t orc_packed_words(void *tp, int which, uint32_t *out, int64_t capacity) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    const Packed &p = t.packed;
+    const uint32_t *src = which == 0 ? (const uint32_t *)p.directory.data() : which == 1 ? p.headers.data() : p.cells.data();
+    int64_t n = which == 0 ? (int64_t)p.directory.size() : which == 1 ? (int64_t)p.headers.size() : (int64_t)p.cells.size();
+    if (out && capacity > 0) memcpy(out, src, (size_t)std::min(n, capacity) * 4);
+    return n;
+}
+
+// A stand-alone face index (gpu/face_index.rs) for its map test.
+void *orc_face_index_new(uint32_t capacity) {
+    FaceIndex *f = new FaceIndex();
+    f->entries.assign(capacity, {0, 0, 0, NONE});
+    return f;
+}
+void orc_face_index_free(void *f) { delete (FaceIndex *)f; }
+void orc_face_index_insert(void *f, int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t slot) {
+    ((FaceIndex *)f)->insert(I3{x, y, z}, face, slot);
+}
+void orc_face_index_remove(void *f, int32_t x, int32_t y, int32_t z, uint32_t face) { ((FaceIndex *)f)->remove(I3{x, y, z}, face); }
+int64_t orc_face_index_get(void *f, int32_t x, int32_t y, int32_t z, uint32_t face) {
+    const FaceIndex &fi = *(FaceIndex *)f;
+    long i = fi.find(I3{x, y, z}, face);
+    return i < 0 ? -1 : (int64_t)(fi.entries[i][3] & 0x1fffffffu);
+}
+int64_t orc_face_index_len(void *f) { return (int64_t)((FaceIndex *)f)->len; }
+int64_t orc_face_index_max_probe(void *f) { return (int64_t)((FaceIndex *)f)->max_probe(); }
+int64_t orc_face_index_capacity(void *f) { return (int64_t)((FaceIndex *)f)->entries.size(); }
+
+// Write raw resident state (36 words per slot), and merge one record into a
+// slot as accumulate does (voxel_accumulate.wgsl:135-227): `record` (24
+// words) is the fresh batch on input and the merged record on output.
+int orc_cache_set_state(void *tp, uint32_t first, uint32_t count, const uint32_t *words) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)(first + count) * 36 > t.state.size()) return -1;
+    memcpy(&t.state[(size_t)first * 36], words, (size_t)count * 36 * 4);
+    return 0;
+}
+int orc_cache_accumulate(void *tp, uint32_t slot, float *record, uint32_t samples, uint32_t clock_ms) {
+    Tracer &t = *(Tracer *)tp;
+    if ((size_t)(slot + 1) * 36 > t.state.size()) return -1;
+    accumulate_one(t, slot, record, samples, clock_ms);
+    return 0;
+}
 
 int orc_mlp_forward(const float *w, const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4],
                     double *out) {

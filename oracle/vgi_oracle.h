@@ -189,6 +189,25 @@ int orc_mlp_levels(const float scene[4], uint32_t *res);
  * is then the exact derivative of the double-precision forward). */
 int orc_mlp_exact(int on);
 
+/* Packed scene of a tracer: [cell words, regions, directory capacity,
+ * interned mixed bricks, brick references, 0, 0, 0]; a voxel's material
+ * decoded from the packed tables (0xffff outside them). */
+int orc_packed_info(void *tracer, int64_t out[8]);
+uint32_t orc_packed_material(void *tracer, int32_t x, int32_t y, int32_t z);
+int64_t orc_packed_words(void *tracer, int which, uint32_t *out, int64_t capacity);
+/* A stand-alone face index (insert, backward-shift remove, get -> slot or -1). */
+void *orc_face_index_new(uint32_t capacity);
+void orc_face_index_free(void *index);
+void orc_face_index_insert(void *index, int32_t x, int32_t y, int32_t z, uint32_t face, uint32_t slot);
+void orc_face_index_remove(void *index, int32_t x, int32_t y, int32_t z, uint32_t face);
+int64_t orc_face_index_get(void *index, int32_t x, int32_t y, int32_t z, uint32_t face);
+int64_t orc_face_index_len(void *index);
+int64_t orc_face_index_max_probe(void *index);
+int64_t orc_face_index_capacity(void *index);
+/* Raw resident state writes; one accumulate merge of a 24-word record into a slot. */
+int orc_cache_set_state(void *tracer, uint32_t first, uint32_t count, const uint32_t *words);
+int orc_cache_accumulate(void *tracer, uint32_t slot, float *record, uint32_t samples, uint32_t clock_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,19 @@ def lib():
         "orc_mlp_hash": (ctypes.c_int, [P, P, u32, P, P, P, P]),
         "orc_mlp_levels": (ctypes.c_int, [P, P]),
         "orc_mlp_exact": (ctypes.c_int, [ctypes.c_int]),
+        "orc_packed_info": (ctypes.c_int, [P, P]),
+        "orc_packed_material": (u32, [P, i32, i32, i32]),
+        "orc_packed_words": (i64, [P, ctypes.c_int, P, i64]),
+        "orc_face_index_new": (P, [u32]),
+        "orc_face_index_free": (None, [P]),
+        "orc_face_index_insert": (None, [P, i32, i32, i32, u32, u32]),
+        "orc_face_index_remove": (None, [P, i32, i32, i32, u32]),
+        "orc_face_index_get": (i64, [P, i32, i32, i32, u32]),
+        "orc_face_index_len": (i64, [P]),
+        "orc_face_index_max_probe": (i64, [P]),
+        "orc_face_index_capacity": (i64, [P]),
+        "orc_cache_set_state": (ctypes.c_int, [P, u32, u32, P]),
+        "orc_cache_accumulate": (ctypes.c_int, [P, u32, P, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -240,6 +253,32 @@ class Tracer:
         c = np.zeros(8, np.uint64) if counters else None
         lib().orc_cache_batch(self.h, ptr(tasks), ptr(slots), n, ptr(results), ptr(h), ptr(c), threads, clock_ms)
         return (results, h, c) if counters else (results, h)
+
+    def packed_info(self):
+        """[cell words, regions, directory capacity, interned bricks, brick references]."""
+        out = np.zeros(8, np.int64)
+        lib().orc_packed_info(self.h, ptr(out))
+        return out[:5]
+
+    def packed_words(self, which):
+        """0: directory, 1: headers, 2: cells (u32 words)."""
+        n = lib().orc_packed_words(self.h, which, None, 0)
+        out = np.zeros(n, np.uint32)
+        lib().orc_packed_words(self.h, which, ptr(out), n)
+        return out
+
+    def packed_material(self, x, y, z):
+        return int(lib().orc_packed_material(self.h, x, y, z))
+
+    def set_state(self, first, words):
+        words = np.ascontiguousarray(words, np.uint32).reshape(-1, 36)
+        assert lib().orc_cache_set_state(self.h, first, len(words), ptr(words)) == 0
+
+    def accumulate(self, slot, record, samples, clock_ms=0):
+        """Merge a 24-float record into `slot`; returns the merged record."""
+        rec = np.ascontiguousarray(record, np.float32).reshape(24).copy()
+        assert lib().orc_cache_accumulate(self.h, slot, ptr(rec), samples, clock_ms) == 0
+        return rec
 
     def cache_state(self, first, count):
         out = np.zeros((count, 36), np.uint32)
@@ -428,3 +467,34 @@ def mlp_levels(scene):
     res = np.zeros(8, np.uint32)
     lib().orc_mlp_levels(ptr(sc), ptr(res))
     return res
+
+
+class FaceIndex:
+    """The oracle's face index (gpu/face_index.rs) on its own."""
+
+    def __init__(self, capacity=16):
+        self.h = lib().orc_face_index_new(capacity)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_face_index_free(self.h)
+            self.h = None
+
+    def insert(self, voxel, face, slot):
+        lib().orc_face_index_insert(self.h, int(voxel[0]), int(voxel[1]), int(voxel[2]), face, slot)
+
+    def remove(self, voxel, face):
+        lib().orc_face_index_remove(self.h, int(voxel[0]), int(voxel[1]), int(voxel[2]), face)
+
+    def get(self, voxel, face):
+        v = lib().orc_face_index_get(self.h, int(voxel[0]), int(voxel[1]), int(voxel[2]), face)
+        return None if v < 0 else int(v)
+
+    def __len__(self):
+        return int(lib().orc_face_index_len(self.h))
+
+    def max_probe(self):
+        return int(lib().orc_face_index_max_probe(self.h))
+
+    def capacity(self):
+        return int(lib().orc_face_index_capacity(self.h))
