@@ -368,7 +368,86 @@ std::vector<int32_t> region_list(const vgi_tracer &t, const std::vector<int> &in
 // Pages changed: re-lay the dense box if the page bounds moved, re-pack the
 // affected regions on the GPU and update the top level for the regions whose
 // barrier state flipped (TransportScene::with_page_contents, world.rs:982-1046).
+vgi_host::PreviousScene scene_pages(const vgi_tracer &t, const std::vector<uint8_t> &solid, std::vector<uint8_t> &page_solid);
+vgi_host::WorkPool &edit_pool();
+
+// A region's 16^3 materials (x fastest) from the host grid; false (and air)
+// when the region is not loaded.
+bool region_materials(const vgi_tracer &t, Int3 r, std::vector<uint16_t> &out) {
+    out.assign(4096, vgi_host::AIR);
+    if (!t.have_regions) return false;
+    int x = r.x - t.rmin.x, y = r.y - t.rmin.y, z = r.z - t.rmin.z;
+    if (x < 0 || y < 0 || z < 0 || x >= t.rsx || y >= t.rsy || z >= t.rsz) return false;
+    if (!t.loaded[lin3(x, y, z, t.rsx, t.rsy)]) return false;
+    for (int lz = 0; lz < 16; ++lz)
+        for (int ly = 0; ly < 16; ++ly)
+            std::memcpy(&out[(size_t)16 * (ly + 16 * lz)], &t.grid[lin3(r.x * 16 - t.vmin.x, r.y * 16 - t.vmin.y + ly, r.z * 16 - t.vmin.z + lz, t.gx, t.gy)],
+                        32);
+    return true;
+}
+
+bool region_solid(const vgi_tracer &t, Int3 r) {
+    if (!t.have_regions) return false;
+    int x = r.x - t.rmin.x, y = r.y - t.rmin.y, z = r.z - t.rmin.z;
+    if (x < 0 || y < 0 || z < 0 || x >= t.rsx || y >= t.rsy || z >= t.rsz) return false;
+    size_t i = lin3(x, y, z, t.rsx, t.rsy);
+    return t.loaded[i] && t.solid[i];
+}
+
+// The retention event of a page upload with the resident cache: every region
+// of a loaded, replaced or removed page whose content or residency changed
+// (world.rs:987-1022 changed regions) as SceneChanges::with_materials builds
+// it (geometry.rs:110-229), with finalize_change_flux's load flux
+// (publication.rs:423-454); queued for the next batch or frame.
+float sky_luminance(const vgi_tracer &t) {
+    auto lum = [](const float *c) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; };
+    float sky = lum(t.transport.environment);
+    float sun = 0.0f;
+    for (uint32_t i = 0; i < t.light_count; ++i) {
+        const float *l = &t.lights[(size_t)i * 12];
+        if (l[3] < 0.0f) sun = std::max(sun, 0.5f * lum(&l[4]));
+    }
+    return std::max(sky, sun);
+}
+
 int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t *>> &incoming, const std::vector<Int3> &removed) {
+    // The scene before the upload, for the retention event: the touched
+    // regions' contents and residency, and which pages held solid regions.
+    bool events = t.resident && t.have_regions;
+    std::vector<uint8_t> page_solid_before;
+    vgi_host::PreviousScene previous{};
+    std::map<std::tuple<int, int, int>, vgi_host::RegionDelta> deltas;  // z, y, x order
+    if (events) {
+        previous = scene_pages(t, t.solid, page_solid_before);
+        auto capture = [&](Int3 page) {
+            for (int o = 0; o < 8; ++o) {
+                Int3 r{page.x * 2 + (o & 1), page.y * 2 + ((o >> 1) & 1), page.z * 2 + (o >> 2)};
+                auto key = std::make_tuple(r.z, r.y, r.x);
+                if (deltas.count(key)) continue;
+                vgi_host::RegionDelta &d = deltas[key];
+                d.coord = r;
+                d.loaded_before = region_materials(t, r, d.before);
+                d.solid_before = region_solid(t, r);
+            }
+        };
+        for (auto &p : incoming) capture(p.first);
+        for (auto &p : removed) capture(p);
+    }
+    auto finish_event = [&]() {
+        if (!events) return;
+        std::vector<vgi_host::RegionDelta> changed;
+        for (auto &kv : deltas) {
+            vgi_host::RegionDelta &d = kv.second;
+            d.loaded_after = region_materials(t, d.coord, d.after);
+            d.solid_after = region_solid(t, d.coord);
+            if (!d.loaded_before && !d.loaded_after) continue;
+            if (d.loaded_before && d.loaded_after && d.before == d.after) continue;  // the same region (world.rs:1008-1014)
+            changed.push_back(std::move(d));
+        }
+        vgi_host::RetentionEvent ev;
+        if (vgi_host::scene_event(changed, previous, t.emission, sky_luminance(t), {}, ev, &edit_pool()))
+            t.pending_events.push_back(std::move(ev));
+    };
     // Previous state per region coordinate (for the flip list).
     std::unordered_map<uint64_t, uint8_t> before;  // bit0 loaded, bit1 solid
     auto note = [&](int rx, int ry, int rz) {
@@ -400,6 +479,7 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
         t.grid.clear();
         t.top = vgi_host::TopLevelField();
         t.emitters_dirty = true;
+        finish_event();
         return 0;
     }
     Int3 pmn{INT32_MAX, INT32_MAX, INT32_MAX}, pmx{INT32_MIN, INT32_MIN, INT32_MIN};
@@ -505,9 +585,10 @@ int apply_pages(vgi_tracer &t, const std::vector<std::pair<Int3, const uint16_t 
         t.top.update(t.bmin, t.bmax, keys, barrier_map(t));
     }
     if (upload_top(t) != 0) return -1;
-    hipEventSynchronize(t.ev[7]);
-    hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]);
+    HIP_OK(hipEventSynchronize(t.ev[7]));
+    HIP_OK(hipEventElapsedTime(&t.timings[3], t.ev[6], t.ev[7]));
     t.emitters_dirty = true;
+    finish_event();
     return 0;
 }
 
@@ -1222,7 +1303,10 @@ int wavefront_trace(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o
 // prepare (resident.rs:1005-1061): measure the edits' flux from the resident
 // statistics, then retention over every slot.
 int flush_events(vgi_tracer &t) {
-    if (t.pending_events.empty() || !t.resident) return 0;
+    if (t.pending_events.empty() || !t.resident) {
+        t.last_event_words = 0;  // this batch applied no retention event
+        return 0;
+    }
     // Change footprints reserve their Light events before the encoding
     // (gpu/resident.rs:606-626).
     std::vector<vgi_host::Footprint> footprints;

@@ -475,6 +475,11 @@ bool within_reach(Int3 bmin, Int3 bmax, const PreviousScene &previous) {
 
 bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
                 const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool) {
+    return scene_event(deltas, previous, emission_rgb, 0.0f, edits, out, pool);
+}
+
+bool scene_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb, float sky,
+                 const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool) {
     out.boxes.clear();
     // Each region's box, on the pool's threads when given: per-region slots,
     // then kept in region order.
@@ -515,12 +520,35 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
                            d.coord.z >= previous.solid_min.z && d.coord.x <= previous.solid_max.x && d.coord.y <= previous.solid_max.y &&
                            d.coord.z <= previous.solid_max.z;
         bool transport = d.solid_before || d.solid_after || in_occupied;
-        if (!transport || changed == 0) return;
+        // Residency changes knowledge of air as well as solids: a loaded or
+        // unloaded region changes as its whole box (geometry.rs:181-189).
+        bool residency = !d.loaded_before || !d.loaded_after;
+        if (!transport || (!residency && changed == 0)) return;
+        Int3 bmin = residency ? origin : tmin, bmax = residency ? Int3{origin.x + 15, origin.y + 15, origin.z + 15} : tmax;
         // Only changes within the influence radius of a page that held a
         // solid region can touch an existing receiver (from_local_bounds and
         // from_resident_bounds select the same set).
-        if (!within_reach(tmin, tmax, previous)) return;
-        region_box[k] = ChangeBox{tmin, tmax, flux, changed, 3u, false};
+        if (!within_reach(bmin, bmax, previous)) return;
+        ChangeBox box{bmin, bmax, flux, changed, 3u, false};
+        if (!d.loaded_before) {
+            // Load: the top solid row of the region (geometry.rs:192-204);
+            // the flux per unit of height above a receiver is the sky's
+            // over the region's width (publication.rs:438).
+            box.kind = 2u;
+            box.top = INT32_MIN;
+            for (int y = 15; y >= 0 && box.top == INT32_MIN; --y)
+                for (int z = 0; z < 16 && box.top == INT32_MIN; ++z)
+                    for (int x = 0; x < 16; ++x)
+                        if (d.after[(size_t)x + 16 * ((size_t)y + 16 * (size_t)z)] != AIR) {
+                            box.top = origin.y + y;
+                            break;
+                        }
+            box.flux = sky * 16.0f;
+        } else if (!d.loaded_after) {
+            box.kind = 1u;  // Unload: its samples become unknown, not contradicted
+            box.flux = 0.0f;
+        }
+        region_box[k] = box;
         has_box[k] = 1;
     };
     if (pool)
@@ -542,6 +570,7 @@ bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &pre
     // reproducible.
     uint64_t budget = FLUX_SCAN_BUDGET;
     for (ChangeBox &b : out.boxes) {
+        if (b.kind != 3u) continue;  // loads and unloads keep their weight
         int64_t sx = std::max<int64_t>((int64_t)b.max.x - b.min.x + 3, 1), sy = std::max<int64_t>((int64_t)b.max.y - b.min.y + 3, 1),
                 sz = std::max<int64_t>((int64_t)b.max.z - b.min.z + 3, 1);
         uint64_t positions = (uint64_t)(sx * sy * sz);
@@ -603,7 +632,12 @@ void encode_events(const std::vector<RetentionEvent> &events, EncodedEvents &out
     uint32_t record = out.events * 3u;
     for (const RetentionEvent &e : events)
         for (const ChangeBox &b : e.boxes) {
+            // Edits and unknown changes carry the player-edit flag in bit 3;
+            // loads carry their top there, saturated to 28 bits
+            // (resident.rs:1319-1324, 1379-1382).
             uint32_t high = b.urgent ? 8u : 0u;
+            if (b.kind == 1u) high = 0u;
+            if (b.kind == 2u) high = (uint32_t)std::min(std::max(b.top, -(1 << 27)), (1 << 27) - 1) << 3;
             if (b.kind == 6u) {
                 Int3 m{b.min.x - 1, b.min.y - 1, b.min.z - 1};
                 uint32_t sx = (uint32_t)(b.max.x - b.min.x + 3), sy = (uint32_t)(b.max.y - b.min.y + 3), sz = (uint32_t)(b.max.z - b.min.z + 3);

@@ -95,8 +95,9 @@ struct ChangeBox {
     float flux;       // the changed voxels' own emission change (measured edits
                       // add the brightest outgoing luminance on the GPU)
     uint32_t voxels;  // changed voxels
-    uint32_t kind;    // 0 unknown flux, 3 edit, 6 edit measured on the GPU
+    uint32_t kind;    // 0 unknown flux, 1 unload, 2 load, 3 edit, 6 edit measured on the GPU
     bool urgent;      // holds an edited voxel
+    int32_t top = 0;  // load: the region's top solid voxel row (INT32_MIN: air only)
 };
 
 struct RetentionEvent {
@@ -104,11 +105,13 @@ struct RetentionEvent {
     std::vector<ChangeBox> boxes;  // sorted by (min, max)
 };
 
-// A region's 16^3 materials (x fastest) before and after an edit.
+// A region's 16^3 materials (x fastest) before and after an edit or a page
+// upload; a region not loaded on one side holds air there.
 struct RegionDelta {
     Int3 coord;
     std::vector<uint16_t> before, after;
     bool solid_before, solid_after;
+    bool loaded_before = true, loaded_after = true;
 };
 
 // The scene before the edit, as the event needs it: the bounding box of its
@@ -132,6 +135,14 @@ class WorkPool;
 // not depend on it.
 bool edit_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb,
                 const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool = nullptr);
+// The same for any scene change, page loads and unloads included
+// (SceneChanges::with_materials geometry.rs:190-215): a region that was not
+// loaded before is a Load of its whole box, its top solid row kept, with flux
+// `sky` x 16 (finalize_change_flux publication.rs:438: sky = the brighter of
+// the environment's luminance and half of each directional light's); one not
+// loaded after is an Unload of its whole box, flux 0; the rest are edits.
+bool scene_event(const std::vector<RegionDelta> &deltas, const PreviousScene &previous, const float *emission_rgb, float sky,
+                 const std::vector<Int3> &edits, RetentionEvent &out, WorkPool *pool = nullptr);
 
 // encode_events (gpu/resident.rs:1319-1417): event headers, then change
 // records; measures are three uint4 per edit whose flux the GPU measures.

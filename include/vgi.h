@@ -83,7 +83,13 @@ int vgi_set_clock(vgi_tracer *tracer, int64_t milliseconds);
 /* ---- scene: VoxelGiInput::{page_loaded, page_unloaded} (input.rs:54-112) feeding
  * TransportScene::with_page_contents (world.rs:982-1046). Pages are 32^3 u16
  * voxels, x fastest, VGI_AIR for air. The NAADF brick/voxel fields are built on
- * the GPU; the top level is grown in exact FIFO order on the host. ---- */
+ * the GPU; the top level is grown in exact FIFO order on the host. With the
+ * resident cache, an upload that changes a loaded scene queues its retention
+ * event for the next batch or frame: regions that load are Load changes of
+ * their whole box with their top solid row and the sky's flux, regions that
+ * unload are Unload changes, regions of a page sent again with new content
+ * are edits measured on the GPU (SceneChanges::with_materials
+ * geometry.rs:110-229, finalize_change_flux publication.rs:423-454). ---- */
 int vgi_upload_pages(vgi_tracer *tracer, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels,
                      const int32_t *removed_xyz, uint32_t removed);
 /* VoxelGiInput::edit_voxel (input.rs:90-93): set voxels (in order; a later
@@ -193,9 +199,11 @@ uint32_t vgi_bound_count(vgi_tracer *tracer);
 /* The bound faces' 32-byte tasks in bound order (sequence words as the last
  * prepare left them). */
 int vgi_read_bound_tasks(vgi_tracer *tracer, vgi_face_task *out, uint32_t count);
-/* The encoded retention events of the last applied edits, after the GPU's
- * finish_edits filled in measured flux (resident.rs:1326-1417 layout); returns
- * the word count, copies at most `capacity` words when out is not NULL. */
+/* The encoded retention events the last resident batch or frame applied
+ * (edits, page loads and unloads, change footprints, sun flips; none: 0
+ * words), after the GPU's finish_edits filled in measured flux
+ * (resident.rs:1326-1417 layout); returns the word count, copies at most
+ * `capacity` words when out is not NULL. */
 int64_t vgi_read_events(vgi_tracer *tracer, uint32_t *out, uint32_t capacity);
 /* Change footprints (resident.rs:1215-1310, voxel_trace.wgsl:94-202): each
  * edit's retention events carry one Light event per edited box (at most 64),

@@ -2642,10 +2642,11 @@ int orc_cache_apply_events(void *tp, const uint32_t *events, uint32_t event_coun
 // One change box with its weight (geometry.rs:55-66 ChangeWeight).
 struct OrcChange {
     I3 mn, mx;
-    int kind;  // 0 unknown, 3 edit
+    int kind;  // 0 unknown, 1 unload, 2 load, 3 edit
     bool measure;
     float flux;
     uint32_t voxels;
+    int32_t top;  // load: top solid row (INT32_MIN: none)
 };
 
 static int64_t bounds_distance2(I3 amin, I3 amax, I3 bmin, I3 bmax) {  // geometry.rs:239-247
@@ -2718,24 +2719,48 @@ static void footprint_tube(const Tracer &t, I3 bmin, I3 bmax, uint32_t grid, uin
     if (flux > 0.0f) sums[6] += (uint32_t)(flux * 64.0f + 0.5f);
 }
 
+// The scene before a change: its regions (kept alive), the bounds of its
+// solid regions, and the pages holding a solid region.
+struct PreviousRegions {
+    std::unordered_map<uint64_t, std::shared_ptr<Region>> regions;
+    bool any_solid = false;
+    I3 smin{0, 0, 0}, smax{0, 0, 0};
+    std::map<std::tuple<int, int, int>, I3> solid_pages;
+};
+static PreviousRegions previous_regions(const Scene &s) {
+    PreviousRegions p;
+    p.regions = s.regions;
+    for (auto &kv : p.regions) {
+        const Region &r = *kv.second;
+        if (!r.any_solid) continue;
+        if (!p.any_solid) { p.smin = p.smax = r.coord; p.any_solid = true; }
+        p.smin = I3{std::min(p.smin.x, r.coord.x), std::min(p.smin.y, r.coord.y), std::min(p.smin.z, r.coord.z)};
+        p.smax = I3{std::max(p.smax.x, r.coord.x), std::max(p.smax.y, r.coord.y), std::max(p.smax.z, r.coord.z)};
+        I3 page{floordiv(r.coord.x, 2), floordiv(r.coord.y, 2), floordiv(r.coord.z, 2)};
+        p.solid_pages[std::make_tuple(page.z, page.y, page.x)] = page;
+    }
+    return p;
+}
+
+// finalize_change_flux (publication.rs:428-434): the brighter of the
+// environment's luminance and half of each directional light's.
+static float sky_luminance(const Tracer &t) {
+    auto lum = [](const float *c) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; };
+    float sky = lum(t.transport.environment);
+    float sun = 0.0f;
+    for (const Light &l : t.lights)
+        if (l.pr[3] < 0.0f) sun = fmax_(sun, 0.5f * lum(l.energy));
+    return fmax_(sky, sun);
+}
+
+static int64_t change_event(Tracer &t, const PreviousRegions &prev, const std::map<std::tuple<int, int, int>, I3> &changed,
+                            const std::vector<I3> &edits);
+
 int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uint32_t count) {
     Tracer &t = *(Tracer *)tp;
     t.last_event.clear();
     Scene &s = *t.scene;
-    // The previous scene: its regions (kept alive), solid bounds, solid pages.
-    std::unordered_map<uint64_t, std::shared_ptr<Region>> previous = s.regions;
-    bool any_solid = false;
-    I3 smin{0, 0, 0}, smax{0, 0, 0};
-    std::map<std::tuple<int, int, int>, I3> solid_pages;
-    for (auto &kv : previous) {
-        const Region &r = *kv.second;
-        if (!r.any_solid) continue;
-        if (!any_solid) { smin = smax = r.coord; any_solid = true; }
-        smin = I3{std::min(smin.x, r.coord.x), std::min(smin.y, r.coord.y), std::min(smin.z, r.coord.z)};
-        smax = I3{std::max(smax.x, r.coord.x), std::max(smax.y, r.coord.y), std::max(smax.z, r.coord.z)};
-        I3 page{floordiv(r.coord.x, 2), floordiv(r.coord.y, 2), floordiv(r.coord.z, 2)};
-        solid_pages[std::make_tuple(page.z, page.y, page.x)] = page;
-    }
+    PreviousRegions prev = previous_regions(s);
     std::map<std::tuple<int, int, int>, I3> edited_pages;
     std::vector<I3> edits;
     for (uint32_t i = 0; i < count; ++i) {
@@ -2753,6 +2778,57 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
             I3 c{kv.second.x * 2 + (o & 1), kv.second.y * 2 + ((o >> 1) & 1), kv.second.z * 2 + (o >> 2)};
             changed[std::make_tuple(c.z, c.y, c.x)] = c;
         }
+    return change_event(t, prev, changed, edits);
+}
+
+// The retention event of a page upload (VoxelGiInput page_loaded /
+// page_unloaded, input.rs:54-112): the scene loads the pages, then every
+// region of a touched page whose residency or content changed
+// (world.rs:987-1022) is a change -- a Load of the whole region, an Unload,
+// or an edit (geometry.rs:110-229).
+int64_t orc_load_event(void *tp, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels, const int32_t *removed_xyz,
+                       uint32_t removed) {
+    Tracer &t = *(Tracer *)tp;
+    t.last_event.clear();
+    Scene &s = *t.scene;
+    PreviousRegions prev = previous_regions(s);
+    std::map<std::tuple<int, int, int>, I3> touched;
+    auto touch = [&](I3 p) {
+        for (int o = 0; o < 8; ++o) {
+            I3 c{p.x * 2 + (o & 1), p.y * 2 + ((o >> 1) & 1), p.z * 2 + (o >> 2)};
+            touched[std::make_tuple(c.z, c.y, c.x)] = c;
+        }
+    };
+    for (uint32_t i = 0; i < pages; ++i) touch(I3{page_xyz[i * 3], page_xyz[i * 3 + 1], page_xyz[i * 3 + 2]});
+    for (uint32_t i = 0; i < removed; ++i) touch(I3{removed_xyz[i * 3], removed_xyz[i * 3 + 1], removed_xyz[i * 3 + 2]});
+    if (orc_scene_load_pages(&s, page_xyz, pages, voxels, removed_xyz, removed) != 0) return -1;
+    std::map<std::tuple<int, int, int>, I3> changed;
+    for (auto &kv : touched) {
+        auto pb = prev.regions.find(key_of(kv.second));
+        const Region *before = pb == prev.regions.end() ? nullptr : pb->second.get();
+        const Region *after = s.region(kv.second);
+        if (!before && !after) continue;
+        if (before && after) {
+            bool same = true;
+            for (int i = 0; i < 4096 && same; ++i) {
+                int x = i % 16, y = (i / 16) % 16, z = i / 256;
+                same = before->material(x, y, z) == after->material(x, y, z);
+            }
+            if (same) continue;
+        }
+        changed[kv.first] = kv.second;
+    }
+    return change_event(t, prev, changed, {});
+}
+
+static int64_t change_event(Tracer &t, const PreviousRegions &prev, const std::map<std::tuple<int, int, int>, I3> &changed,
+                            const std::vector<I3> &edits) {
+    Scene &s = *t.scene;
+    const auto &previous = prev.regions;
+    const bool any_solid = prev.any_solid;
+    const I3 smin = prev.smin, smax = prev.smax;
+    const auto &solid_pages = prev.solid_pages;
+    const float sky = sky_luminance(t);
     std::vector<OrcChange> bounds;
     for (auto &kv : changed) {
         I3 c = kv.second;
@@ -2793,8 +2869,24 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
         bool in_occupied = any_solid && c.x >= smin.x && c.y >= smin.y && c.z >= smin.z && c.x <= smax.x && c.y <= smax.y && c.z <= smax.z;
         bool transport = (before && before->any_solid) || (after && after->any_solid) || in_occupied;
         bool residency = before == nullptr || after == nullptr;
-        if (!transport || residency || tight_empty) continue;  // edits never change residency
-        bounds.push_back(OrcChange{tmin, tmax, 3, false, flux, changed_voxels});
+        if (!transport || (!residency && tight_empty)) continue;
+        // geometry.rs:181-215: a residency change is the whole region: a
+        // Load keeps its top solid row, an Unload predicts nothing.
+        if (!residency) {
+            bounds.push_back(OrcChange{tmin, tmax, 3, false, flux, changed_voxels, 0});
+            continue;
+        }
+        I3 emax{origin.x + 15, origin.y + 15, origin.z + 15};
+        if (!before) {
+            int32_t top = INT32_MIN;
+            for (int y = 15; y >= 0 && top == INT32_MIN; --y)
+                for (int z = 0; z < 16 && top == INT32_MIN; ++z)
+                    for (int x = 0; x < 16; ++x)
+                        if (after->material(x, y, z) != AIR) { top = origin.y + y; break; }
+            bounds.push_back(OrcChange{origin, emax, 2, false, sky * 16.0f, changed_voxels, top});
+        } else {
+            bounds.push_back(OrcChange{origin, emax, 1, false, 0.0f, changed_voxels, 0});
+        }
     }
     // from_resident_bounds: each solid page gathers the changes within reach.
     const int64_t r2 = 192 * 192;
@@ -2816,6 +2908,7 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
     if (nearby.empty()) return 0;
     uint64_t budget = 16384;
     for (OrcChange &c : nearby) {
+        if (c.kind != 3) continue;  // loads and unloads keep their weight
         int64_t sx = std::max<int64_t>((int64_t)c.mx.x - c.mn.x + 3, 1), sy = std::max<int64_t>((int64_t)c.mx.y - c.mn.y + 3, 1),
                 sz = std::max<int64_t>((int64_t)c.mx.z - c.mn.z + 3, 1);
         uint64_t positions = (uint64_t)(sx * sy * sz);
@@ -2828,8 +2921,10 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
     std::vector<size_t> footprint_of;
     for (int pass = 0; pass < (t.footprints ? 2 : 0); ++pass)
         for (size_t i = 0; i < nearby.size(); ++i) {
-            bool edit = nearby[i].kind != 0;
-            if ((pass == 0) == edit && footprint_of.size() < 64) footprint_of.push_back(i);
+            // Edits first, then changes of unknown flux; loads and unloads
+            // get none (resident.rs:1235-1247).
+            bool wanted = pass == 0 ? nearby[i].kind == 3 : nearby[i].kind == 0;
+            if (wanted && footprint_of.size() < 64) footprint_of.push_back(i);
         }
     const uint32_t n_events = 1u + (uint32_t)footprint_of.size();
     const int32_t far_lo = 1 << 29, far_hi = -(1 << 29);
@@ -2867,11 +2962,13 @@ int64_t orc_edit_event(void *tp, const int32_t *xyz, const uint16_t *values, uin
                 if (bounds_distance2(c.mn, c.mx, v, v) == 0) { urgent = true; break; }
         }
         uint32_t kind = c.kind == 0 ? 0u : (c.measure ? 6u : 3u);
+        if (c.kind == 1) kind = 1u;
+        if (c.kind == 2) kind = 2u | ((uint32_t)std::min(std::max(c.top, -(1 << 27)), (1 << 27) - 1) << 3);
         if (c.measure)
             measures.push_back(Measure{record, I3{c.mn.x - 1, c.mn.y - 1, c.mn.z - 1}, (uint32_t)(c.mx.x - c.mn.x + 3),
                                        (uint32_t)(c.mx.y - c.mn.y + 3), (uint32_t)(c.mx.z - c.mn.z + 3), c.voxels, c.flux});
         push((uint32_t)c.mn.x, (uint32_t)c.mn.y, (uint32_t)c.mn.z, fbits(c.flux));
-        push((uint32_t)c.mx.x, (uint32_t)c.mx.y, (uint32_t)c.mx.z, kind | (urgent ? 8u : 0u));
+        push((uint32_t)c.mx.x, (uint32_t)c.mx.y, (uint32_t)c.mx.z, kind | (urgent && c.kind != 1 && c.kind != 2 ? 8u : 0u));
         record += 2;
     }
     for (size_t k = 0; k < footprint_of.size(); ++k) {

@@ -165,6 +165,10 @@ int orc_cache_picks(void *tracer, const uint32_t *pick_slots, uint32_t count, fl
  * count, 0 when nothing within reach changed. orc_last_event copies the
  * words (at most `capacity`). */
 int64_t orc_edit_event(void *tracer, const int32_t *xyz, const uint16_t *values, uint32_t count);
+/* Load and unload pages (as orc_scene_load_pages) and build the upload's
+ * retention event (Load / Unload / edit changes), read with orc_last_event. */
+int64_t orc_load_event(void *tracer, const int32_t *page_xyz, uint32_t pages, const uint16_t *voxels, const int32_t *removed_xyz,
+                       uint32_t removed);
 int64_t orc_last_event(void *tracer, uint32_t *out, int64_t capacity);
 uint32_t orc_curve_limit(uint32_t d2);
 /* One traversal (voxel_traversal.wgsl:126-219): out_i = voxel xyz, material,

@@ -75,6 +75,7 @@ def lib():
         "orc_cache_apply_events": (ctypes.c_int, [P, P, u32, u32]),
         "orc_curve_limit": (u32, [u32]),
         "orc_edit_event": (i64, [P, P, P, u32]),
+        "orc_load_event": (i64, [P, P, u32, P, P, u32]),
         "orc_last_event": (i64, [P, P, i64]),
         "orc_cache_meta": (ctypes.c_int, [P, u32, u32, P]),
         "orc_cache_schedule": (ctypes.c_int, [P, P, u32, f32, u32, u32, u32]),
@@ -291,6 +292,19 @@ class Tracer:
         xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
         values = np.ascontiguousarray(values, np.uint16).reshape(-1)
         n = lib().orc_edit_event(self.h, ptr(xyz), ptr(values), len(values))
+        out = np.zeros(n, np.uint32)
+        if n:
+            lib().orc_last_event(self.h, ptr(out), n)
+        return out
+
+    def load_event(self, coords, pages, removed=None):
+        """Load (and remove) pages in the oracle scene; the upload's encoded
+        retention event, empty when none."""
+        coords = np.ascontiguousarray(coords, np.int32).reshape(-1, 3)
+        pages = np.ascontiguousarray(pages, np.uint16).reshape(-1, 32768)
+        removed = np.zeros((0, 3), np.int32) if removed is None else np.ascontiguousarray(removed, np.int32).reshape(-1, 3)
+        n = lib().orc_load_event(self.h, ptr(coords), len(coords), ptr(pages), ptr(removed), len(removed))
+        assert n >= 0
         out = np.zeros(n, np.uint32)
         if n:
             lib().orc_last_event(self.h, ptr(out), n)
