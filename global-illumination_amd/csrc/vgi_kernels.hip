@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include <type_traits>
 
+#include "vgi_blockscan.h"
 #include "vgi_kernels.h"
 #include "vgi_mlp.h"
 #include "vgi_trace.h"
@@ -684,24 +687,66 @@ __device__ __forceinline__ uint32_t curve_limit(uint32_t d2) {
 
 // apply_events (voxel_accumulate.wgsl:295-332) over every slot below the
 // high-water mark, with resolve_retention (voxel_retention.wgsl:29-96).
-__global__ void apply_events_kernel(const uint4 *__restrict__ events, uint32_t event_count, uint32_t high_water,
-                                    uint32_t *__restrict__ state, uint32_t *__restrict__ checkpoint, uint32_t *__restrict__ meta,
-                                    float physical, uint32_t reach, uint32_t urgent_measured, uint32_t urgent_curve) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= high_water) return;
-    uint32_t *id = meta + (size_t)slot * 12u;
-    if ((id[3] & 8u) == 0u) return;
-    int px = (int)id[0], py = (int)id[1], pz = (int)id[2];
+// A workgroup of 256 slots stages each event's change boxes through LDS, 256
+// at a time: each thread loads one box and keeps it only when it lies within
+// the reach of the workgroup's slots' bounding box (a box no slot can reach
+// changes no slot: resolve_retention skips it per face); the kept boxes are
+// compacted in index order (ballot ranks), and every slot then runs the
+// exact per-face rule over them from LDS -- the same boxes in the same order,
+// so the sums are the per-face loop's.
+__device__ __forceinline__ uint32_t box_box_distance2(int alx, int aly, int alz, int ahx, int ahy, int ahz, int blx, int bly, int blz, int bhx,
+                                                      int bhy, int bhz) {
+    uint32_t sx = (uint32_t)min(max(max(alx - bhx, blx - ahx), 0), 4096);
+    uint32_t sy = (uint32_t)min(max(max(aly - bhy, bly - ahy), 0), 4096);
+    uint32_t sz = (uint32_t)min(max(max(alz - bhz, blz - ahz), 0), 4096);
+    return sx * sx + sy * sy + sz * sz;
+}
+
+__global__ __launch_bounds__(256) void apply_events_kernel(const uint4 *__restrict__ events, uint32_t event_count, uint32_t high_water,
+                                                          uint32_t *__restrict__ state, uint32_t *__restrict__ checkpoint,
+                                                          uint32_t *__restrict__ meta, float physical, uint32_t reach,
+                                                          uint32_t urgent_measured, uint32_t urgent_curve) {
+    __shared__ uint4 s_box[256][2];
+    __shared__ uint32_t s_wave[1][4];
+    __shared__ int s_bounds[4][6];
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    bool live = false;
+    int px = 0, py = 0, pz = 0;
+    if (slot < high_water) {
+        const uint4 id = reinterpret_cast<const uint4 *>(meta)[(size_t)slot * 3u];
+        live = (id.w & 8u) != 0u;
+        px = (int)id.x; py = (int)id.y; pz = (int)id.z;
+    }
+    // The workgroup's bounding box of live slots (empty: every box culled).
+    int b[6] = {live ? px : INT_MAX, live ? py : INT_MAX, live ? pz : INT_MAX, live ? px : INT_MIN, live ? py : INT_MIN, live ? pz : INT_MIN};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int o = __shfl_xor(b[k], off, 64);
+            b[k] = k < 3 ? min(b[k], o) : max(b[k], o);
+        }
+    if (lane == 0u)
+        for (int k = 0; k < 6; ++k) s_bounds[wave][k] = b[k];
+    __syncthreads();
+    int g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        g[k] = s_bounds[0][k];
+        for (int w = 1; w < 4; ++w) g[k] = k < 3 ? min(g[k], s_bounds[w][k]) : max(g[k], s_bounds[w][k]);
+    }
+    const bool any_live = g[0] <= g[3];
     bool measured_mode = physical > 0.0f;
     uint32_t cap = NONE, source_cap = NONE;
     float change = 0.0f;
     bool known = true, woken = false, urgent_if_binding = false, urgent = false;
-    for (uint32_t e = 0u; e < event_count; ++e) {
-        uint4 h0 = events[e * 3u], h1 = events[e * 3u + 1u], h2 = events[e * 3u + 2u];
+    for (uint32_t e = 0u; e < event_count && any_live; ++e) {
+        const uint4 h0 = events[e * 3u], h1 = events[e * 3u + 1u], h2 = events[e * 3u + 2u];
         if ((h0.w & 2u) != 0u) {
             uint32_t limit = h2.x;
-            for (uint32_t b = 0u; b < h2.w; ++b) {
-                uint4 sp = events[event_count * 3u + (h2.z + b) * 2u];
+            for (uint32_t k = 0u; k < h2.w; ++k) {
+                uint4 sp = events[event_count * 3u + (h2.z + k) * 2u];
                 f3 d = mk((float)px, (float)py, (float)pz) - mk(__uint_as_float(sp.x), __uint_as_float(sp.y), __uint_as_float(sp.z));
                 if (dot3(d, d) <= __uint_as_float(sp.w)) limit = min(limit, 4u);
             }
@@ -709,34 +754,59 @@ __global__ void apply_events_kernel(const uint4 *__restrict__ events, uint32_t e
             woken = true;
             continue;
         }
-        uint32_t outside = box_distance2((int)h0.x, (int)h0.y, (int)h0.z, (int)h1.x, (int)h1.y, (int)h1.z, px, py, pz);
-        if (outside > reach) continue;
-        woken = true;
+        // No slot of the workgroup within reach of the event's union: no box
+        // of it can change any of them (uniform across the workgroup).
+        if (box_box_distance2((int)h0.x, (int)h0.y, (int)h0.z, (int)h1.x, (int)h1.y, (int)h1.z, g[0], g[1], g[2], g[3], g[4], g[5]) > reach)
+            continue;
+        const bool in_reach = live && box_distance2((int)h0.x, (int)h0.y, (int)h0.z, (int)h1.x, (int)h1.y, (int)h1.z, px, py, pz) <= reach;
+        if (in_reach) woken = true;
         bool exact = known;
         uint32_t nearest = NONE;
-        for (uint32_t b = 0u; b < h2.w; ++b) {
-            uint32_t record = event_count * 3u + (h2.z + b) * 2u;
-            uint4 b0 = events[record], b1 = events[record + 1u];
-            uint32_t d = box_distance2((int)b0.x, (int)b0.y, (int)b0.z, (int)b1.x, (int)b1.y, (int)b1.z, px, py, pz);
-            if (d > reach) continue;
-            nearest = min(nearest, d);
-            uint32_t edit_kind = b1.w & 7u;
-            if ((b1.w & 8u) != 0u && (edit_kind == 0u || edit_kind == 3u)) {
-                if (edit_kind == 3u && measured_mode) urgent_if_binding = urgent_if_binding || d <= urgent_measured;
-                else urgent = urgent || d <= urgent_curve;
+        for (uint32_t base = 0u; base < h2.w; base += 256u) {
+            uint32_t k = base + threadIdx.x;
+            uint4 b0 = make_uint4(0u, 0u, 0u, 0u), b1 = b0;
+            bool keep = false;
+            if (k < h2.w) {
+                uint32_t record = event_count * 3u + (h2.z + k) * 2u;
+                b0 = events[record];
+                b1 = events[record + 1u];
+                keep = box_box_distance2((int)b0.x, (int)b0.y, (int)b0.z, (int)b1.x, (int)b1.y, (int)b1.z, g[0], g[1], g[2], g[3], g[4],
+                                         g[5]) <= reach;
             }
-            if (exact) {
-                uint32_t kind = b1.w & 7u;
-                if (kind == 0u) { known = false; exact = false; }
-                else if (kind != 1u) {
-                    float area = 1.0f;
-                    if (kind == 2u) area = (float)min(max(((int)b1.w >> 3) + 1 - py, 0), 16);
-                    change = change + __uint_as_float(b0.w) * area / (3.14159265358979f * (float)max(d, 1u));
+            bool flag[1] = {keep};
+            uint32_t rank[1], total[1];
+            block_ranks<1>(flag, rank, total, s_wave);
+            if (keep) {
+                s_box[rank[0]][0] = b0;
+                s_box[rank[0]][1] = b1;
+            }
+            __syncthreads();
+            if (in_reach)
+                for (uint32_t j = 0u; j < total[0]; ++j) {
+                    const uint4 c0 = s_box[j][0], c1 = s_box[j][1];
+                    uint32_t d = box_distance2((int)c0.x, (int)c0.y, (int)c0.z, (int)c1.x, (int)c1.y, (int)c1.z, px, py, pz);
+                    if (d > reach) continue;
+                    nearest = min(nearest, d);
+                    uint32_t edit_kind = c1.w & 7u;
+                    if ((c1.w & 8u) != 0u && (edit_kind == 0u || edit_kind == 3u)) {
+                        if (edit_kind == 3u && measured_mode) urgent_if_binding = urgent_if_binding || d <= urgent_measured;
+                        else urgent = urgent || d <= urgent_curve;
+                    }
+                    if (exact) {
+                        uint32_t kind = c1.w & 7u;
+                        if (kind == 0u) { known = false; exact = false; }
+                        else if (kind != 1u) {
+                            float area = 1.0f;
+                            if (kind == 2u) area = (float)min(max(((int)c1.w >> 3) + 1 - py, 0), 16);
+                            change = change + __uint_as_float(c0.w) * area / (3.14159265358979f * (float)max(d, 1u));
+                        }
+                    }
                 }
-            }
+            __syncthreads();
         }
         if (nearest != NONE) cap = min(cap, curve_limit(nearest));
     }
+    if (!live) return;
     bool measured = measured_mode && known;
     uint32_t limit = min(source_cap, measured ? NONE : cap);
     uint32_t *st = state + (size_t)slot * 36u;
