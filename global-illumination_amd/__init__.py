@@ -150,6 +150,7 @@ def library():
         "vgi_mlp_table_lr": (ctypes.c_int, [P, ctypes.c_float]),
         "vgi_mlp_train_samples": (ctypes.c_int, [P, ctypes.c_uint32]),
         "vgi_mlp_last_loss": (f32, [P]),
+        "vgi_mlp_timings": (ctypes.c_int, [P, P]),
         "vgi_mlp_pretrain": (ctypes.c_int, [P, u32]),
         "vgi_scene_bounds": (ctypes.c_int, [P, P]),
         "vgi_scene_region_canonical": (ctypes.c_int, [P, i32, i32, i32, P]),
@@ -177,7 +178,7 @@ def exported_symbols():
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound", "vgi_read_bound_hits",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
-        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_timings", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
         "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
@@ -594,6 +595,13 @@ class GpuTracer:
 
     def mlp_pretrain(self, frames):
         _check(self._lib.vgi_mlp_pretrain(self._h, frames))
+
+    def mlp_timings(self):
+        """The last learned frame: {infer_ms, train_mfma_ms, train_ms, queries, train_samples, steps}."""
+        out = np.zeros(6, np.float32)
+        _check(self._lib.vgi_mlp_timings(self._h, _ptr(out)))
+        keys = ("infer_ms", "train_mfma_ms", "train_ms", "queries", "train_samples", "steps")
+        return {k: float(v) for k, v in zip(keys, out)}
 
     def mlp_last_loss(self):
         return float(self._lib.vgi_mlp_last_loss(self._h))

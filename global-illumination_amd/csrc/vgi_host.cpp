@@ -226,6 +226,13 @@ struct vgi_tracer {
     float mlp_lr = 1e-3f, last_loss = 0.0f;
     float table_lr = 1e-2f;  // the hash table's Adam rate
     uint32_t train_samples = 1;  // paths averaged into each training target
+    // GPU timestamps of the last learned frame: inference (0, 1), and per
+    // training step its MFMA passes (forward/backward, weight gradient) and
+    // the whole step with Adam.
+    hipEvent_t mlp_ev[2] = {};
+    std::vector<hipEvent_t> mlp_step_ev;  // 3 per step: start, after the weight gradient, after the update
+    uint32_t mlp_timed_steps = 0, mlp_timed_step_samples = 0, mlp_timed_queries = 0;
+    bool mlp_timed = false;
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
     // ---- wavefront path tracing of bound frames
@@ -1105,7 +1112,8 @@ vgi::MlpScene mlp_scene(const vgi_tracer &t) {
 // One optimizer step on `count` samples (a multiple of 16) of queries/targets
 // already on the device: fused forward/backward, weight-gradient GEMM, and
 // with `update` an Adam step and a re-pack of the BF16 fragments.
-int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets, uint32_t count, bool update) {
+int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets, uint32_t count, bool update,
+             hipEvent_t after_gradients = nullptr) {
     size_t n = count;
     HIP_OK(t.d_acts.reserve(n * 288 * 2));
     HIP_OK(t.d_grads.reserve(n * 272 * 2));
@@ -1115,6 +1123,7 @@ int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets,
                               t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), t.d_mlp_loss.as<float>(), t.d_train_pred.as<float>(),
                               t.d_table_g.as<float>(), t.stream);
     vgi::mlp_weight_grad(t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), count, t.d_mlp_g.as<float>(), t.stream);
+    if (after_gradients) HIP_OK(hipEventRecord(after_gradients, t.stream));
     if (update) {
         t.mlp_step += 1;
         vgi::mlp_adam(t.d_mlp_w.as<float>(), t.d_mlp_m.as<float>(), t.d_mlp_v.as<float>(), t.d_mlp_g.as<float>(), t.mlp_step,
@@ -1168,13 +1177,29 @@ int learned_targets(vgi_tracer &t, hipStream_t stream) {
     return 0;
 }
 
-// The frame's Adam steps on its training set (on the tracer's stream).
-int learned_steps(vgi_tracer &t) {
+// The frame's Adam steps on its training set (on the tracer's stream), each
+// bracketed by events (timed = true: the learned frame's MFMA fraction).
+int learned_steps(vgi_tracer &t, bool timed = false) {
     uint32_t total = (t.train_per_frame / 16u) * 16u;
     if (total == 0 || t.bound == 0) return 0;
     uint32_t per = std::max<uint32_t>(16u, (total / std::max<uint32_t>(t.train_steps, 1u) / 16u) * 16u);
-    for (uint32_t off = 0; off + per <= total; off += per)
-        if (mlp_step(t, t.d_train_q.as<vgi::MlpQuery>() + off, t.d_targets.as<float4>() + off, per, true) != 0) return -1;
+    uint32_t steps = total / per;
+    if (timed && t.mlp_step_ev.size() < (size_t)steps * 3) {
+        size_t have = t.mlp_step_ev.size();
+        t.mlp_step_ev.resize((size_t)steps * 3, nullptr);
+        for (size_t k = have; k < t.mlp_step_ev.size(); ++k) HIP_OK(hipEventCreate(&t.mlp_step_ev[k]));
+    }
+    uint32_t k = 0;
+    for (uint32_t off = 0; off + per <= total; off += per, ++k) {
+        if (timed) HIP_OK(hipEventRecord(t.mlp_step_ev[k * 3], t.stream));
+        if (mlp_step(t, t.d_train_q.as<vgi::MlpQuery>() + off, t.d_targets.as<float4>() + off, per, true, timed ? t.mlp_step_ev[k * 3 + 1] : nullptr) != 0)
+            return -1;
+        if (timed) HIP_OK(hipEventRecord(t.mlp_step_ev[k * 3 + 2], t.stream));
+    }
+    if (timed) {
+        t.mlp_timed_steps = steps;
+        t.mlp_timed_step_samples = per;
+    }
     t.train_cursor = (uint32_t)(((uint64_t)t.train_cursor + total) % t.bound);
     return 0;
 }
@@ -1595,7 +1620,7 @@ void vgi_destroy(vgi_tracer *t) {
     DeviceBuffer *bufs[] = {&t->d_grid, &t->d_headers, &t->d_cells, &t->d_info, &t->d_list, &t->d_top, &t->d_totals,
                             &t->d_scatter_idx, &t->d_scatter_val, &t->d_materials, &t->d_lights, &t->d_emitters,
                             &t->d_index, &t->d_state, &t->d_checkpoint, &t->d_meta, &t->d_tasks, &t->d_slots,
-                            &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters, &t->d_mlp_w, &t->d_mlp_m,
+                            &t->b_tasks, &t->b_slots, &t->b_results, &t->b_counters, &t->b_hits, &t->d_mlp_w, &t->d_mlp_m,
                             &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->d_table,
                             &t->d_table_m, &t->d_table_v, &t->d_table_g, &t->b_learned, &t->b_queries,
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
@@ -1613,6 +1638,8 @@ void vgi_destroy(vgi_tracer *t) {
     for (auto &e : t->side_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->sel_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->sun_ev) if (e) hipEventDestroy(e);
+    for (auto &e : t->mlp_ev) if (e) hipEventDestroy(e);
+    for (auto &e : t->mlp_step_ev) if (e) hipEventDestroy(e);
     if (t->stage_done) hipEventDestroy(t->stage_done);
     if (t->stage) hipHostFree(t->stage);
     if (t->side) hipStreamDestroy(t->side);
@@ -2011,8 +2038,13 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
                               t->b_results.as<float4>(), nullptr, counting ? t->b_counters.as<unsigned long long>() : nullptr, false,
                               t->stream, order_of(*t), t->b_learned.as<float4>(), t->b_queries.as<vgi::MlpQuery>()) != 0)
             return -1;
+        if (!t->mlp_ev[0])
+            for (auto &e : t->mlp_ev) HIP_OK(hipEventCreate(&e));
+        HIP_OK(hipEventRecord(t->mlp_ev[0], t->stream));
         vgi::mlp_infer(t->d_mlp_frag.as<uint16_t>(), mlp_scene(*t), t->b_queries.as<vgi::MlpQuery>(), (uint32_t)samples,
                        t->b_pred.as<float>(), t->stream);
+        HIP_OK(hipEventRecord(t->mlp_ev[1], t->stream));
+        t->mlp_timed_queries = (uint32_t)samples;
         vgi::launch_learned_combine(t->b_learned.as<float4>(), t->b_pred.as<float>(), count, t->transport.samples,
                                     lanes_for(t->transport), t->b_tasks.as<uint4>(), t->b_results.as<float4>(), t->stream);
     } else if (t->wavefront && !record_hits && (flags & VGI_COUNT_GATHERS) == 0 && o.light_count <= vgi::WF_MAX_LIGHTS &&
@@ -2037,7 +2069,8 @@ int vgi_run_frame(vgi_tracer *t, uint32_t flags) {
     HIP_OK(hipGetLastError());
     if (t->learned) {
         HIP_OK(hipStreamWaitEvent(t->stream, t->side_ev[1], 0));
-        if (learned_steps(*t) != 0) return -1;
+        if (learned_steps(*t, learned_frame) != 0) return -1;
+        t->mlp_timed = learned_frame;
     }
     HIP_OK(hipEventRecord(t->ev[9], t->stream));
     t->frame_index += 1;
@@ -2093,7 +2126,7 @@ int vgi_mlp_enable(vgi_tracer *t, uint32_t seed, uint32_t train_per_frame, uint3
     HIP_OK(t->d_mlp_m.reserve(vgi::MLP_PARAMS * 4));
     HIP_OK(t->d_mlp_v.reserve(vgi::MLP_PARAMS * 4));
     HIP_OK(t->d_mlp_g.reserve(vgi::MLP_PARAMS * 4));
-    HIP_OK(t->d_mlp_frag.reserve(vgi::MLP_FRAG * 256 * 2));
+    HIP_OK(t->d_mlp_frag.reserve((size_t)vgi::MLP_FRAG_VALUES * 2));
     HIP_OK(t->d_mlp_loss.reserve(16));
     HIP_OK(t->d_table.reserve((size_t)vgi::MLP_TABLE * 4));
     HIP_OK(t->d_table_m.reserve((size_t)vgi::MLP_TABLE * 4));
@@ -2245,6 +2278,29 @@ int vgi_mlp_pretrain(vgi_tracer *t, uint32_t frames) {
         if (learned_training(*t) != 0) return -1;
         t->frame_index += 1;
     }
+    return 0;
+}
+
+int vgi_mlp_timings(vgi_tracer *t, float out[6]) {
+    if (!t || !out) return fail("null argument");
+    if (!t->mlp_timed) return fail("no learned frame has run");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    float infer = 0.0f, mfma = 0.0f, steps = 0.0f;
+    HIP_OK(hipEventElapsedTime(&infer, t->mlp_ev[0], t->mlp_ev[1]));
+    for (uint32_t k = 0; k < t->mlp_timed_steps; ++k) {
+        float a = 0.0f, b = 0.0f;
+        HIP_OK(hipEventElapsedTime(&a, t->mlp_step_ev[k * 3], t->mlp_step_ev[k * 3 + 1]));
+        HIP_OK(hipEventElapsedTime(&b, t->mlp_step_ev[k * 3], t->mlp_step_ev[k * 3 + 2]));
+        mfma += a;
+        steps += b;
+    }
+    out[0] = infer;
+    out[1] = mfma;
+    out[2] = steps;
+    out[3] = (float)t->mlp_timed_queries;
+    out[4] = (float)t->mlp_timed_step_samples * (float)t->mlp_timed_steps;
+    out[5] = (float)t->mlp_timed_steps;
     return 0;
 }
 

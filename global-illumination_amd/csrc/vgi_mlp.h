@@ -30,13 +30,19 @@ constexpr int MLP_W3 = MLP_W2 + MLP_HIDDEN * MLP_HIDDEN;
 constexpr int MLP_W4 = MLP_W3 + MLP_HIDDEN * MLP_HIDDEN;
 constexpr int MLP_W5 = MLP_W4 + MLP_HIDDEN * MLP_HIDDEN;
 constexpr int MLP_PARAMS = MLP_W5 + MLP_OUT * MLP_HIDDEN;
-// BF16 MFMA fragments (64 lanes x 4 values per 16x16 tile): forward W tiles
-// (W1 4x2, W2..W4 4x4, W5 1x4 with rows padded to 16) then transposed tiles
-// for the backward data pass (W2..W4 4x4, W5^T 4x1, then W1^T's first row of
-// tiles, 1x4: the gradient of the hash features).
-constexpr int MLP_FWD_TILES = 8 + 16 * 3 + 4;
-constexpr int MLP_BWD_TILES = 16 * 3 + 4 + 4;
-constexpr int MLP_FRAG = MLP_FWD_TILES + MLP_BWD_TILES;
+// BF16 MFMA A-operand fragments, one slot per 16-row x 32-k tile (64 lanes x
+// 8 values, the v_mfma_f32_16x16x32_bf16 operand): forward W tiles (W1 4x1,
+// W2..W4 4x2, W5 1x2 with rows padded to 16), then transposed tiles for the
+// backward data pass (W2..W4 4x2, W1^T's first row 1x2: the gradient of the
+// hash features; W5^T 4 tiles of 16x16, values 0..3 of a slot, for the
+// 16x16x16 step over the 3 outputs).
+constexpr int MLP_FWD_SLOTS = 4 + 8 * 3 + 2;
+constexpr int MLP_BWD_SLOTS = 8 * 3 + 2 + 4;
+constexpr int MLP_SLOTS = MLP_FWD_SLOTS + MLP_BWD_SLOTS;
+constexpr int MLP_FRAG_VALUES = MLP_SLOTS * 64 * 8;  // bf16 values in the fragment buffer
+// The 16x16 tiles of the weight matrices (W1 4x2, W2..W4 4x4, W5 1x4): the
+// weight-gradient pass computes one per wavefront.
+constexpr int MLP_GRAD_TILES = 8 + 16 * 3 + 4;
 
 // One query of the cache: the face whose incident radiance is wanted.
 struct MlpQuery {

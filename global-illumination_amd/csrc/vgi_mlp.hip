@@ -1,12 +1,16 @@
-// Learned radiance cache kernels for gfx950: BF16 MFMA (16x16x16, FP32
-// accumulate), wave64.
+// Learned radiance cache kernels for gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulate), wave64.
 //
 // Formulation: each wavefront carries 16 samples and computes Y^T = W X^T, so
-// a 16x16 MFMA's A operand is a weight tile and B the activations. With the
-// 16x16x16 lane layout (lane l: column n = l % 16; rows/k = 4 (l / 16) + j for
-// j = 0..3) a layer's accumulator fragment for output tile t is exactly the
-// B fragment of k-step t of the next layer, so activations chain through the
-// five layers in registers. Weights sit in LDS as pre-packed fragments.
+// an MFMA's A operand is a weight tile and B the activations. The 16x16
+// accumulator layout (lane l: column n = l % 16, rows 4 (l / 16) + j, j =
+// 0..3) of two consecutive output tiles 2s and 2s + 1 is used, unchanged, as
+// the 8-value B operand of the next layer's 32-wide k-step s: lane group g =
+// l / 16 holds logical k = 32 s + 4 g + j (j < 4, tile 2s) and 32 s + 16 + 4 g
+// + j - 4 (j >= 4, tile 2s + 1). An MFMA multiplies the A and B values at
+// the same (lane group, j) position, so the weight fragments are packed with
+// the same k order and activations chain through the five layers in
+// registers. Weights sit in LDS as pre-packed fragments.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -18,6 +22,8 @@
 namespace vgi {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Round-to-nearest-even float -> bf16 bits (no NaNs arise here).
@@ -28,9 +34,17 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__device__ __forceinline__ f32x4 mfma(s16x4 a, s16x4 b, f32x4 c) {
+// 16x16x16 (k = 4 (l / 16) + j): only the 3-output W5^T step of the backward pass.
+__device__ __forceinline__ f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// 16x16x32: the CDNA4 double-k BF16 MFMA.
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+// Two 4-value fragments as the 8-value B operand of one 32-wide k-step.
+__device__ __forceinline__ s16x8 join(s16x4 lo, s16x4 hi) { return s16x8{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w}; }
+__device__ __forceinline__ s16x4 low4(s16x8 v) { return s16x4{v.s0, v.s1, v.s2, v.s3}; }
 __device__ __forceinline__ s16x4 pack4(float a, float b, float c, float d) {
     s16x4 r;
     r.x = (short)to_bf16(a);
@@ -40,40 +54,49 @@ __device__ __forceinline__ s16x4 pack4(float a, float b, float c, float d) {
     return r;
 }
 
-// Fragment tile indices (see vgi_mlp.h).
-__device__ __host__ constexpr int fwd_tile(int layer, int t, int s) {
-    return layer == 1 ? t * 2 + s : (layer <= 4 ? 8 + 16 * (layer - 2) + t * 4 + s : 56 + s);
+// Fragment slots (see vgi_mlp.h).
+__device__ __host__ constexpr int fwd_slot(int layer, int t, int s) {
+    return layer == 1 ? t : (layer <= 4 ? 4 + 8 * (layer - 2) + 2 * t + s : 28 + s);
 }
-__device__ __host__ constexpr int bwd_tile(int layer, int t, int s) {
-    return MLP_FWD_TILES + (layer == 1 ? 52 + s : layer <= 4 ? 16 * (layer - 2) + t * 4 + s : 48 + t);
+__device__ __host__ constexpr int bwd_slot(int layer, int t, int s) {
+    return MLP_FWD_SLOTS + (layer == 1 ? 24 + s : layer <= 4 ? 8 * (layer - 2) + 2 * t + s : 26 + t);
 }
+// The logical k (input index) of lane group g, value j of 32-wide k-step s.
+__device__ __host__ constexpr int chained_k(int g, int j, int s) { return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
 
-// Packs FP32 master weights into BF16 A-operand fragments: forward tile (t, s)
-// lane l value j = W[16t + l%16][16s + 4(l/16) + j]; the transposed (backward)
-// tile (t, s) = W[16s + 4(l/16) + j][16t + l%16]. Rows beyond the layer's
-// outputs are zero.
+// Packs FP32 master weights into BF16 A-operand slots: forward slot (layer,
+// t, s) lane l = m + 16 g value j = W[16 t + m][chained_k(g, j, s)];
+// transposed slot (layer, t, s) = W[chained_k(g, j, s)][16 t + m]; the W5^T
+// slots (16x16x16) value j < 4 = W5[4 g + j][16 t + m]. Entries beyond a
+// matrix are zero.
 __global__ void mlp_pack_kernel(const float *__restrict__ w, uint16_t *__restrict__ frags) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= MLP_FRAG * 256) return;
-    int tile = e >> 8, lane = (e >> 2) & 63, j = e & 3;
-    int col16 = lane & 15, k4 = 4 * (lane >> 4) + j;
+    if (e >= MLP_FRAG_VALUES) return;
+    int slot = e >> 9, lane = (e >> 3) & 63, j = e & 7;
+    int m = lane & 15, g = lane >> 4;
     int layer, t, s;
-    bool transposed = tile >= MLP_FWD_TILES;
-    int local = transposed ? tile - MLP_FWD_TILES : tile;
+    bool transposed = slot >= MLP_FWD_SLOTS;
+    int local = transposed ? slot - MLP_FWD_SLOTS : slot;
+    int row, col;
     if (!transposed) {
-        if (local < 8) { layer = 1; t = local / 2; s = local % 2; }
-        else if (local < 56) { layer = 2 + (local - 8) / 16; t = ((local - 8) % 16) / 4; s = (local - 8) % 4; }
-        else { layer = 5; t = 0; s = local - 56; }
+        if (local < 4) { layer = 1; t = local; s = 0; }
+        else if (local < 28) { layer = 2 + (local - 4) / 8; t = ((local - 4) % 8) / 2; s = (local - 4) % 2; }
+        else { layer = 5; t = 0; s = local - 28; }
+        row = 16 * t + m;
+        col = chained_k(g, j, s);
+    } else if (local < 26) {
+        if (local < 24) { layer = 2 + local / 8; t = (local % 8) / 2; s = local % 2; }
+        else { layer = 1; t = 0; s = local - 24; }
+        row = chained_k(g, j, s);
+        col = 16 * t + m;
     } else {
-        if (local < 48) { layer = 2 + local / 16; t = (local % 16) / 4; s = local % 4; }
-        else if (local < 52) { layer = 5; t = local - 48; s = 0; }
-        else { layer = 1; t = 0; s = local - 52; }
+        layer = 5; t = local - 26;
+        row = j < 4 ? 4 * g + j : 1 << 20;
+        col = 16 * t + m;
     }
     int in = layer == 1 ? MLP_IN : MLP_HIDDEN;
     int out = layer == 5 ? MLP_OUT : MLP_HIDDEN;
     int base = layer == 1 ? MLP_W1 : layer == 2 ? MLP_W2 : layer == 3 ? MLP_W3 : layer == 4 ? MLP_W4 : MLP_W5;
-    int row = transposed ? 16 * s + k4 : 16 * t + col16;
-    int col = transposed ? 16 * t + col16 : 16 * s + k4;
     float v = (row < out && col < in) ? w[base + row * in + col] : 0.0f;
     frags[e] = to_bf16(v);
 }
@@ -146,16 +169,13 @@ __device__ __forceinline__ float encode_tail(const MlpScene &sc, const MlpQuery 
 // Forward of one 16-sample tile held by a wavefront. x0: input fragments;
 // h: per-layer ReLU masks (bit (t*4+j) of masks[layer-1]); returns the output
 // accumulator (lanes 0..15 hold outputs 0..2 of their sample in j = 0..2).
-__device__ __forceinline__ f32x4 mlp_forward_tile(const s16x4 *__restrict__ lds, s16x4 x0[2], uint32_t masks[4], s16x4 acts[4][4]) {
+__device__ __forceinline__ f32x4 mlp_forward_tile(const s16x8 *__restrict__ lds, s16x4 x0[2], uint32_t masks[4], s16x4 acts[4][4]) {
     const int lane = threadIdx.x & 63;
     f32x4 acc[4];
-    // layer 1 (K = 32)
+    // layer 1 (K = 32: one k-step, features 4g..4g+3 and 16+4g..16+4g+3)
+    const s16x8 b0 = join(x0[0], x0[1]);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < 2; ++s) acc[t] = mfma(lds[fwd_tile(1, t, s) * 64 + lane], x0[s], acc[t]);
-    }
+    for (int t = 0; t < 4; ++t) acc[t] = mfma32(lds[fwd_slot(1, t, 0) * 64 + lane], b0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
     s16x4 h[4];
 #pragma unroll
     for (int layer = 2; layer <= 5; ++layer) {
@@ -169,17 +189,16 @@ __device__ __forceinline__ f32x4 mlp_forward_tile(const s16x4 *__restrict__ lds,
             acts[layer - 2][t] = h[t];
         }
         masks[layer - 2] = mask;
+        const s16x8 k0 = join(h[0], h[1]), k1 = join(h[2], h[3]);
         if (layer < 5) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc[t] = mfma(lds[fwd_tile(layer, t, s) * 64 + lane], h[s], acc[t]);
+                acc[t] = mfma32(lds[fwd_slot(layer, t, 0) * 64 + lane], k0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+                acc[t] = mfma32(lds[fwd_slot(layer, t, 1) * 64 + lane], k1, acc[t]);
             }
         } else {
-            acc[0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < 4; ++s) acc[0] = mfma(lds[fwd_tile(5, 0, s) * 64 + lane], h[s], acc[0]);
+            acc[0] = mfma32(lds[fwd_slot(5, 0, 0) * 64 + lane], k0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            acc[0] = mfma32(lds[fwd_slot(5, 0, 1) * 64 + lane], k1, acc[0]);
         }
     }
     return acc[0];
@@ -221,9 +240,9 @@ __device__ __forceinline__ void scatter_hash_grad(const MlpScene &sc, const MlpQ
     }
 }
 
-__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ frags, s16x4 *lds, int tiles) {
-    const s16x4 *src = reinterpret_cast<const s16x4 *>(frags);
-    for (int i = threadIdx.x; i < tiles * 64; i += blockDim.x) lds[i] = src[i];
+__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ frags, s16x8 *lds, int slots) {
+    const s16x8 *src = reinterpret_cast<const s16x8 *>(frags);
+    for (int i = threadIdx.x; i < slots * 64; i += blockDim.x) lds[i] = src[i];
     __syncthreads();
 }
 
@@ -232,8 +251,8 @@ __device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ frags
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const uint16_t *__restrict__ frags, MlpScene sc,
                                                         const MlpQuery *__restrict__ queries, uint32_t count,
                                                         float *__restrict__ pred) {
-    __shared__ s16x4 lds[MLP_FWD_TILES * 64];
-    stage_weights(frags, lds, MLP_FWD_TILES);
+    __shared__ s16x8 lds[MLP_FWD_SLOTS * 64];
+    stage_weights(frags, lds, MLP_FWD_SLOTS);
     const int lane = threadIdx.x & 63;
     const uint32_t tiles = (count + 15u) / 16u;
     const uint32_t wave = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
@@ -268,8 +287,8 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
                                                           uint32_t count, uint16_t *__restrict__ acts_out,
                                                           uint16_t *__restrict__ grads_out, float *__restrict__ loss,
                                                           float *__restrict__ pred, float *__restrict__ table_grad) {
-    __shared__ s16x4 lds[MLP_FRAG * 64];
-    stage_weights(frags, lds, MLP_FRAG);
+    __shared__ s16x8 lds[MLP_SLOTS * 64];
+    stage_weights(frags, lds, MLP_SLOTS);
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const uint32_t tile = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
     if (tile * 16u >= count) return;
@@ -309,11 +328,11 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
     if (lane == 0) atomicAdd(loss, l);
     s16x4 dy5 = pack4(d0, d1, d2, 0.0f);  // rows 4g+j: nonzero only for g = 0
     *reinterpret_cast<s16x4 *>(grads_out + 256 * N + (size_t)n * 16 + 4 * g) = dy5;
-    // dH4^T = W5^T dY5^T, masked by layer 4's ReLU.
+    // dH4^T = W5^T dY5^T (k = the 3 outputs: one 16x16x16 step), masked by layer 4's ReLU.
     s16x4 dy[4];
     f32x4 acc[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = mfma(lds[bwd_tile(5, t, 0) * 64 + lane], dy5, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    for (int t = 0; t < 4; ++t) acc[t] = mfma16(low4(lds[bwd_slot(5, t, 0) * 64 + lane]), dy5, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
     for (int layer = 4; layer >= 1; --layer) {
         uint32_t mask = masks[layer - 1];
 #pragma unroll
@@ -325,32 +344,32 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
             dy[t] = pack4(a, b, c, d);
             *reinterpret_cast<s16x4 *>(grads_out + 64 * N * (layer - 1) + (size_t)n * 64 + 16 * t + 4 * g) = dy[t];
         }
+        const s16x8 k0 = join(dy[0], dy[1]), k1 = join(dy[2], dy[3]);
         if (layer > 1) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc[t] = mfma(lds[bwd_tile(layer, t, s) * 64 + lane], dy[s], acc[t]);
+                acc[t] = mfma32(lds[bwd_slot(layer, t, 0) * 64 + lane], k0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+                acc[t] = mfma32(lds[bwd_slot(layer, t, 1) * 64 + lane], k1, acc[t]);
             }
         }
     }
     // dX0 rows 0..15 = (W1^T)[0..15] dY1: the hash features' gradient, lane
     // group g holding the rows of the levels it encoded.
-    f32x4 dx = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) dx = mfma(lds[bwd_tile(1, 0, s) * 64 + lane], dy[s], dx);
+    f32x4 dx = mfma32(lds[bwd_slot(1, 0, 0) * 64 + lane], join(dy[0], dy[1]), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    dx = mfma32(lds[bwd_slot(1, 0, 1) * 64 + lane], join(dy[2], dy[3]), dx);
     if (valid) scatter_hash_grad(sc, q, dx, table_grad);
 }
 
-// dW_i[o][in] = sum_n dY_i[n][o] X_i[n][in]: one wavefront per (weight tile,
-// chunk of 1024 samples); A = dY^T (o x samples), B = X (samples x in).
+// dW_i[o][in] = sum_n dY_i[n][o] X_i[n][in]: one wavefront per (16x16 weight
+// tile, chunk of 1024 samples); A = dY^T (o x samples), B = X (samples x
+// in), 32 samples per 16x16x32 MFMA (lane group g, value j: sample 8 g + j).
 __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const uint16_t *__restrict__ acts, const uint16_t *__restrict__ grads,
                                                         uint32_t count, float *__restrict__ grad) {
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const uint32_t wave = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
     const uint32_t chunks = (count + 1023u) / 1024u;
     const uint32_t tile = wave / chunks, chunk = wave % chunks;
-    if (tile >= (uint32_t)MLP_FWD_TILES) return;
+    if (tile >= (uint32_t)MLP_GRAD_TILES) return;
     int layer, t, s;
     if (tile < 8) { layer = 1; t = tile / 2; s = tile % 2; }
     else if (tile < 56) { layer = 2 + (tile - 8) / 16; t = ((tile - 8) % 16) / 4; s = (tile - 8) % 4; }
@@ -362,18 +381,16 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const uint16_t *__restri
     const uint16_t *DY = grads + (layer == 5 ? 256 * N : 64 * N * (layer - 1));
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t begin = chunk * 1024u, end = min(count, begin + 1024u);
-    for (uint32_t n0 = begin; n0 < end; n0 += 16u) {
-        s16x4 a, b;
-        short av[4], bv[4];
+    for (uint32_t n0 = begin; n0 < end; n0 += 32u) {
+        s16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            size_t n = n0 + 4 * g + j;
-            av[j] = (short)DY[n * gw + 16 * t + col];
-            bv[j] = (short)X[n * in + 16 * s + col];
+        for (int j = 0; j < 8; ++j) {
+            size_t n = n0 + 8 * g + j;
+            bool in_range = n < end;
+            a[j] = in_range ? (short)DY[n * gw + 16 * t + col] : (short)0;
+            b[j] = in_range ? (short)X[n * in + 16 * s + col] : (short)0;
         }
-        a.x = av[0]; a.y = av[1]; a.z = av[2]; a.w = av[3];
-        b.x = bv[0]; b.y = bv[1]; b.z = bv[2]; b.w = bv[3];
-        acc = mfma(a, b, acc);
+        acc = mfma32(a, b, acc);
     }
     // C layout: lane holds rows o = 16t + 4g + j, column in = 16s + col.
     const int out = layer == 5 ? MLP_OUT : MLP_HIDDEN;
@@ -421,7 +438,7 @@ void mlp_table_adam(float *table, float *m, float *v, float *grad, uint32_t step
 }
 
 void mlp_pack(const float *master, uint16_t *frags, hipStream_t stream) {
-    hipLaunchKernelGGL(mlp_pack_kernel, dim3((MLP_FRAG * 256 + 255) / 256), dim3(256), 0, stream, master, frags);
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((MLP_FRAG_VALUES + 255) / 256), dim3(256), 0, stream, master, frags);
 }
 void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, uint32_t count, float *pred,
                hipStream_t stream) {
@@ -441,7 +458,7 @@ void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const Ml
 void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream) {
     if (count == 0) return;
     uint32_t chunks = (count + 1023u) / 1024u;
-    uint32_t waves = (uint32_t)MLP_FWD_TILES * chunks;
+    uint32_t waves = (uint32_t)MLP_GRAD_TILES * chunks;
     hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((waves + 3u) / 4u), dim3(256), 0, stream, acts, grads, count, grad);
 }
 void mlp_adam(float *master, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream) {

@@ -322,6 +322,11 @@ int vgi_mlp_train_samples(vgi_tracer *tracer, uint32_t samples);
  * bound faces not traced): the network's warm-up. */
 int vgi_mlp_pretrain(vgi_tracer *tracer, uint32_t frames);
 float vgi_mlp_last_loss(vgi_tracer *tracer);
+/* GPU time of the last learned frame's network work (valid after it ran):
+ * [inference ms, training MFMA ms (forward/backward and weight-gradient
+ * passes of every step), training ms (the steps with their Adam updates),
+ * queries inferred, training samples (all steps), steps]. */
+int vgi_mlp_timings(vgi_tracer *tracer, float out[6]);
 
 /* ---- retention events (resident.rs:1326-1417 encode_events layout;
  * voxel_retention.wgsl:29-96, apply_events voxel_accumulate.wgsl:295-332) ---- */
