@@ -118,7 +118,6 @@ def library():
         "vgi_batch_last_ms": (f32, [P]),
         "vgi_batch_latency": (i32, [P, P, u32, P]),
         "vgi_batch_autotune": (i32, [P, u32, P]),
-        "vgi_scene_edits": (i64, [u32, u32, u32, u32, P, P, i64]),
         "vgi_bound_count": (u32, [P]),
         "vgi_read_bound_tasks": (i32, [P, P, u32]),
         "vgi_read_events": (i64, [P, P, u32]),
@@ -157,9 +156,6 @@ def library():
         "vgi_scene_stats": (i64, [P, P]),
         "vgi_emitter_words": (i64, [P, P, i64]),
         "vgi_last_error": (ctypes.c_char_p, []),
-        "vgi_scene_generate": (ctypes.c_int, [ctypes.c_int, u32, u32, P, P]),
-        "vgi_scene_optics": (ctypes.c_int, [ctypes.c_int, u32, P, P, P, P, P]),
-        "vgi_scene_faces": (i64, [P, P, ctypes.c_int, P, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -179,13 +175,52 @@ def exported_symbols():
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
         "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_timings", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
-        "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_scene_generate",
-        "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_batch_create", "vgi_batch_destroy",
+        "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
+
+
+WORKLOADS_PATH = os.path.join(HERE, "libvgi_workloads.so")
+_workloads = None
+
+
+def workloads():
+    """libvgi_workloads.so (include/vgi_workloads.h): the seeded scenes, optics
+    and edit streams of the tests and the benchmark -- not the product."""
+    global _workloads
+    if _workloads is not None:
+        return _workloads
+    if not os.path.exists(WORKLOADS_PATH):
+        raise VoxelGiError("libvgi_workloads.so is not built: run __graft_entry__.build() (or global-illumination_amd/build.py)")
+    lib = ctypes.CDLL(WORKLOADS_PATH)
+    P = ctypes.c_void_p
+    u32, i64 = ctypes.c_uint32, ctypes.c_int64
+    sig = {
+        "vgi_scene_generate": (ctypes.c_int, [ctypes.c_int, u32, u32, P, P]),
+        "vgi_scene_optics": (ctypes.c_int, [ctypes.c_int, u32, P, P, P, P, P]),
+        "vgi_scene_faces": (i64, [P, P, ctypes.c_int, P, i64]),
+        "vgi_scene_edits": (i64, [u32, u32, u32, u32, P, P, i64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _workloads = lib
+    return lib
+
+
+def workload_symbols():
+    """Names include/vgi_workloads.h declares (libvgi_workloads.so)."""
+    return ["vgi_scene_generate", "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits"]
+
+
+def _check_workload(rc, what):
+    if rc < 0:
+        raise VoxelGiError("%s failed (%d)" % (what, rc))
+    return rc
 
 
 def _ptr(a):
@@ -201,32 +236,32 @@ def _check(rc):
 # ---------------------------------------------------------------- scenes
 def generate(workload, seed, edge=0):
     """Dense u16 voxel grid (z, y, x) of a seeded workload; AIR = 256."""
-    lib = library()
+    lib = workloads()
     dims = np.zeros(3, np.int32)
-    _check(lib.vgi_scene_generate(workload, seed, edge, None, _ptr(dims)))
+    _check_workload(lib.vgi_scene_generate(workload, seed, edge, None, _ptr(dims)), "vgi_scene_generate")
     grid = np.empty((dims[2], dims[1], dims[0]), np.uint16)
-    _check(lib.vgi_scene_generate(workload, seed, edge, _ptr(grid), _ptr(dims)))
+    _check_workload(lib.vgi_scene_generate(workload, seed, edge, _ptr(grid), _ptr(dims)), "vgi_scene_generate")
     return grid
 
 
 def optics(workload, seed):
     """(diffuse[256,3], emission[256,3], lights (n,12) float32, environment[3])."""
-    lib = library()
+    lib = workloads()
     diffuse = np.zeros((256, 3), np.float32)
     emission = np.zeros((256, 3), np.float32)
     lights = np.zeros((16, 12), np.float32)
     count = np.zeros(1, np.uint32)
     env = np.zeros(3, np.float32)
-    _check(lib.vgi_scene_optics(workload, seed, _ptr(diffuse), _ptr(emission), _ptr(lights), _ptr(count), _ptr(env)))
+    _check_workload(lib.vgi_scene_optics(workload, seed, _ptr(diffuse), _ptr(emission), _ptr(lights), _ptr(count), _ptr(env)), "vgi_scene_optics")
     return diffuse, emission, lights[: int(count[0])].copy(), env
 
 
 def exposed_faces(grid, interior_only=False):
     """Exposed faces (solid voxel, air neighbour inside the grid) as FACE_TASK records."""
-    lib = library()
+    lib = workloads()
     grid = np.ascontiguousarray(grid, np.uint16)
     dims = np.array([grid.shape[2], grid.shape[1], grid.shape[0]], np.int32)
-    n = _check(lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), None, 0))
+    n = _check_workload(lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), None, 0), "vgi_scene_faces")
     out = np.zeros(n, FACE_TASK)
     lib.vgi_scene_faces(_ptr(grid), _ptr(dims), int(interior_only), _ptr(out), n)
     return out
@@ -234,8 +269,8 @@ def exposed_faces(grid, interior_only=False):
 
 def scene_edits(seed, frame, count, edge):
     """One frame of the seeded edit stream: (xyz int32 [n,3], values uint16 [n])."""
-    lib = library()
-    n = _check(lib.vgi_scene_edits(seed, frame, count, edge, None, None, 0))
+    lib = workloads()
+    n = _check_workload(lib.vgi_scene_edits(seed, frame, count, edge, None, None, 0), "vgi_scene_edits")
     xyz = np.zeros((n, 3), np.int32)
     values = np.zeros(n, np.uint16)
     lib.vgi_scene_edits(seed, frame, count, edge, _ptr(xyz), _ptr(values), n)

@@ -12,6 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvgi.so")
+WORKLOADS_LIB = os.path.join(HERE, "libvgi_workloads.so")
 BUILD = os.path.join(HERE, "_build")
 
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
@@ -31,7 +32,10 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-I" + INCLUDE, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 
 HIP_SOURCES = ["vgi_kernels.hip", "vgi_wavefront.hip", "vgi_select.hip", "vgi_sun.hip", "vgi_mlp.hip", "vgi_host.cpp"]
-HOST_SOURCES = ["vgi_host_scene.cpp", "vgi_scenes.cpp"]
+HOST_SOURCES = ["vgi_host_scene.cpp"]
+# The seeded workload generators of the tests and the benchmark: a library of
+# their own, outside the product's boundary.
+WORKLOAD_SOURCES = ["vgi_scenes.cpp"]
 
 
 def _run(cmd):
@@ -63,6 +67,9 @@ def build(verbose=False, jobs=None):
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objects + ["-lpthread"])
     os.replace(tmp, LIB)
+    tmp = WORKLOADS_LIB + ".tmp"
+    _run(["g++"] + HOST_FLAGS + ["-shared", "-o", tmp] + [os.path.join(CSRC, src) for src in WORKLOAD_SOURCES] + ["-lpthread"])
+    os.replace(tmp, WORKLOADS_LIB)
     return LIB
 
 

@@ -112,7 +112,8 @@ __device__ __forceinline__ float exp_own(float x) {
     return (p * __int_as_float((127 + e1) << 23)) * __int_as_float((127 + e2) << 23);
 }
 
-// voxel_path.wgsl:178-186 (hash-based Owen scramble, Vegdahl's sobol_burley)
+// voxel_path.wgsl:178-186 (hash-based Owen scramble, Vegdahl's sobol_burley,
+// MIT: NOTICE-sobol_burley.txt)
 __device__ __forceinline__ uint32_t scramble_reverse(uint32_t input, uint32_t seed) {
     uint32_t value = input;
     value ^= value * 0x3d20adeau;

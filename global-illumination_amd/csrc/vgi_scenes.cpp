@@ -7,7 +7,7 @@
 #include <thread>
 #include <vector>
 
-#include "vgi.h"
+#include "vgi_workloads.h"
 
 namespace {
 

@@ -187,6 +187,7 @@ static inline uint32_t reverse_bits(uint32_t v) {
     v = ((v >> 8) & 0x00ff00ffu) | ((v & 0x00ff00ffu) << 8);
     return (v >> 16) | (v << 16);
 }
+// Vegdahl's sobol_burley hash (MIT; global-illumination_amd/csrc/NOTICE-sobol_burley.txt).
 static inline uint32_t scramble_reverse(uint32_t input, uint32_t seed) {
     uint32_t value = input;
     value ^= value * 0x3d20adeau;

@@ -1,4 +1,5 @@
-"""The C ABI library loads and exports every symbol include/vgi.h declares
+"""The C ABI library loads and exports every symbol include/vgi.h declares,
+and only those; the workload library (include/vgi_workloads.h) likewise
 (no GPU calls)."""
 import ctypes
 import os
@@ -7,10 +8,18 @@ import re
 from conftest import ROOT, load_package
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "vgi.h")).read()
+def declared_symbols(header="vgi.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
     return sorted(set(re.findall(r"\b(vgi_[a-z0-9_]+)\s*\(", text)))
+
+
+def defined_symbols(path):
+    """Global text symbols of a built shared library (nm -D --defined-only)."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return sorted({ln.split()[-1] for ln in out.splitlines() if len(ln.split()) == 3 and ln.split()[1] == "T"})
 
 
 def test_library_exports_every_declared_symbol():
@@ -22,6 +31,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # The Python layer binds exactly the declared surface.
     assert sorted(vgi.exported_symbols()) == names
+    # The product exports no scene generator: those live in the workload library.
+    exported = [n for n in defined_symbols(vgi.LIB_PATH) if n.startswith("vgi_")]
+    assert exported == names, sorted(set(exported) ^ set(names))
+
+
+def test_workload_library_is_separate():
+    vgi = load_package()
+    names = declared_symbols("vgi_workloads.h")
+    assert names == sorted(vgi.workload_symbols())
+    lib = vgi.workloads()
+    assert all(hasattr(lib, n) for n in names)
+    assert [n for n in defined_symbols(vgi.WORKLOADS_PATH) if n.startswith("vgi_")] == names
 
 
 def test_struct_layouts_match_the_reference():
@@ -47,7 +68,7 @@ def test_every_entry_point_rejects_a_null_handle():
     lib = vgi.library()
     checked = 0
     for name in vgi.exported_symbols():
-        if name.startswith("vgi_scene_") or name in ("vgi_last_error", "vgi_destroy", "vgi_batch_destroy"):
+        if name in ("vgi_last_error", "vgi_destroy", "vgi_batch_destroy"):
             continue
         fn = getattr(lib, name)
         args = fn.argtypes or []
