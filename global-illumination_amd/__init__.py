@@ -148,6 +148,8 @@ def library():
         "vgi_mlp_table": (ctypes.c_int, [P, P, P, P]),
         "vgi_mlp_table_lr": (ctypes.c_int, [P, ctypes.c_float]),
         "vgi_mlp_train_samples": (ctypes.c_int, [P, ctypes.c_uint32]),
+        "vgi_mlp_training": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]),
+        "vgi_mlp_encoding": (ctypes.c_int, [P, ctypes.c_uint32]),
         "vgi_mlp_last_loss": (f32, [P]),
         "vgi_mlp_timings": (ctypes.c_int, [P, P]),
         "vgi_mlp_pretrain": (ctypes.c_int, [P, u32]),
@@ -174,7 +176,7 @@ def exported_symbols():
         "vgi_bind_faces", "vgi_run_frame", "vgi_sync", "vgi_last_trace_ms", "vgi_last_frame_ms", "vgi_set_wavefront", "vgi_read_bound", "vgi_read_bound_hits",
         "vgi_bind_exposed", "vgi_bound_count", "vgi_read_bound_tasks", "vgi_read_events",
         "vgi_cache_state", "vgi_apply_events", "vgi_probe_copy", "vgi_converged", "vgi_mlp_enable",
-        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_last_loss", "vgi_mlp_timings", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
+        "vgi_mlp_set_learned", "vgi_mlp_weights", "vgi_mlp_infer", "vgi_mlp_train", "vgi_mlp_table", "vgi_mlp_table_lr", "vgi_mlp_train_samples", "vgi_mlp_training", "vgi_mlp_encoding", "vgi_mlp_last_loss", "vgi_mlp_timings", "vgi_mlp_pretrain", "vgi_scene_bounds", "vgi_scene_region_canonical",
         "vgi_scene_stats", "vgi_emitter_words", "vgi_last_error", "vgi_batch_create", "vgi_batch_destroy",
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
@@ -182,6 +184,9 @@ def exported_symbols():
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
+
+MLP_ENCODING_HASH = 0
+MLP_ENCODING_FREQUENCY = 1
 
 WORKLOADS_PATH = os.path.join(HERE, "libvgi_workloads.so")
 _workloads = None
@@ -622,6 +627,14 @@ class GpuTracer:
 
     def mlp_train_samples(self, samples):
         _check(self._lib.vgi_mlp_train_samples(self._h, samples))
+
+    def mlp_training(self, train_per_frame, steps, lr):
+        """New training schedule, network kept; 0 targets freezes it."""
+        _check(self._lib.vgi_mlp_training(self._h, train_per_frame, steps, lr))
+
+    def mlp_encoding(self, mode):
+        """MLP_ENCODING_HASH (default) or MLP_ENCODING_FREQUENCY."""
+        _check(self._lib.vgi_mlp_encoding(self._h, mode))
 
     def mlp_table_grad(self):
         out = np.zeros(MLP_TABLE, np.float32)

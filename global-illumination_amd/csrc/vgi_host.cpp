@@ -219,6 +219,7 @@ struct vgi_tracer {
     std::vector<uint32_t> bound_material;  // material of each bound face's voxel
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
+    uint32_t mlp_encoding = 0u;  // vgi::MLP_ENCODING_*
     DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
     DeviceBuffer d_table, d_table_m, d_table_v, d_table_g;  // hash-grid encoding (FP32, Adam)
     std::vector<float> table_grad_last;                      // of the last vgi_mlp_train without update
@@ -234,7 +235,7 @@ struct vgi_tracer {
     uint32_t mlp_timed_steps = 0, mlp_timed_step_samples = 0, mlp_timed_queries = 0;
     bool mlp_timed = false;
     DeviceBuffer b_learned, b_queries, b_pred, d_perm, d_face_material;
-    DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io;
+    DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io, d_touched;
     // ---- wavefront path tracing of bound frames
     bool wavefront = false;
     int cus = 0;
@@ -1103,6 +1104,7 @@ vgi::MlpScene mlp_scene(const vgi_tracer &t) {
     sc.lo_z = (float)(t.bmin.z * 16);
     int extent = std::max(std::max(t.bmax.x - t.bmin.x + 1, t.bmax.y - t.bmin.y + 1), t.bmax.z - t.bmin.z + 1) * 16;
     sc.inv_extent = 1.0f / (float)std::max(extent, 1);
+    sc.encoding = t.mlp_encoding;
     sc.materials = t.d_materials.as<float4>();
     sc.table = t.d_table.as<float2>();
     for (int l = 0; l < vgi::MLP_HASH_LEVELS; ++l) sc.res[l] = vgi::mlp_hash_res(l, std::max(extent, 1));
@@ -1118,26 +1120,26 @@ int mlp_step(vgi_tracer &t, const vgi::MlpQuery *queries, const float4 *targets,
     HIP_OK(t.d_acts.reserve(n * 288 * 2));
     HIP_OK(t.d_grads.reserve(n * 272 * 2));
     HIP_OK(t.d_train_pred.reserve(n * 12));
+    HIP_OK(t.d_touched.reserve(n * vgi::MLP_TOUCHED * 4));
     HIP_OK(hipMemsetAsync(t.d_mlp_loss.ptr, 0, 4, t.stream));
     vgi::mlp_forward_backward(t.d_mlp_frag.as<uint16_t>(), mlp_scene(t), queries, reinterpret_cast<const float *>(targets), count,
                               t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), t.d_mlp_loss.as<float>(), t.d_train_pred.as<float>(),
-                              t.d_table_g.as<float>(), t.stream);
+                              t.d_table_g.as<float>(), t.d_touched.as<uint32_t>(), t.stream);
     vgi::mlp_weight_grad(t.d_acts.as<uint16_t>(), t.d_grads.as<uint16_t>(), count, t.d_mlp_g.as<float>(), t.stream);
     if (after_gradients) HIP_OK(hipEventRecord(after_gradients, t.stream));
     if (update) {
         t.mlp_step += 1;
         vgi::mlp_adam(t.d_mlp_w.as<float>(), t.d_mlp_m.as<float>(), t.d_mlp_v.as<float>(), t.d_mlp_g.as<float>(), t.mlp_step,
                       t.mlp_lr, t.stream);
-        vgi::mlp_table_adam(t.d_table.as<float>(), t.d_table_m.as<float>(), t.d_table_v.as<float>(), t.d_table_g.as<float>(),
-                            t.mlp_step, t.table_lr, t.stream);
+        if (t.mlp_encoding == vgi::MLP_ENCODING_HASH)
+            vgi::mlp_table_adam(t.d_table.as<float>(), t.d_table_m.as<float>(), t.d_table_v.as<float>(), t.d_table_g.as<float>(),
+                                t.d_touched.as<uint32_t>(), count, t.mlp_step, t.table_lr, t.stream);
         vgi::mlp_pack(t.d_mlp_w.as<float>(), t.d_mlp_frag.as<uint16_t>(), t.stream);
     }
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-// A learned frame's training: trace a window of the face permutation with
-// plain multi-bounce paths (one sample each) for targets, then Adam steps.
 // The training set of a frame, on `stream`: a window of the face permutation
 // traced with plain multi-bounce paths (one sample each) into targets. It
 // needs only the scene and the bound tasks, not the network, so a learned
@@ -1222,18 +1224,26 @@ uint64_t spread_bits(uint32_t v) {
     return x;
 }
 
-int compute_order(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
-    t.has_order = false;
-    if (count == 0) return 0;
+// Morton keys of the tasks' voxels (relative to their minimum corner).
+std::vector<uint64_t> morton_keys(const vgi_face_task *tasks, uint32_t count) {
     int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
     for (uint32_t i = 0; i < count; ++i)
         for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], tasks[i].voxel[a]);
-    std::vector<std::pair<uint64_t, uint32_t>> keys(count);
+    std::vector<uint64_t> keys(count);
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t c[3];
         for (int a = 0; a < 3; ++a) c[a] = (uint32_t)std::min<int64_t>((int64_t)tasks[i].voxel[a] - lo[a], 0x1fffff);
-        keys[i] = {spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2), i};
+        keys[i] = spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2);
     }
+    return keys;
+}
+
+int compute_order(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
+    t.has_order = false;
+    if (count == 0) return 0;
+    std::vector<uint64_t> morton = morton_keys(tasks, count);
+    std::vector<std::pair<uint64_t, uint32_t>> keys(count);
+    for (uint32_t i = 0; i < count; ++i) keys[i] = {morton[i], i};
     std::sort(keys.begin(), keys.end());
     std::vector<uint32_t> order(count);
     for (uint32_t i = 0; i < count; ++i) order[i] = keys[i].second;
@@ -1393,7 +1403,10 @@ int flush_events(vgi_tracer &t) {
 }
 
 // Each bound face's material (the learned cache encodes its albedo) and a
-// fixed random order for choosing training faces.
+// fixed random order for choosing training faces, sorted in Morton order
+// within runs of TRAIN_RUN faces: a frame's training window is a random
+// subset whose paths still start near their neighbours on a wavefront.
+constexpr uint32_t TRAIN_RUN = 8192;
 int bind_materials(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
     vgi_host::VoxelSource src{t.grid.data(), t.vmin, t.gx, t.gy, t.gz, t.loaded.data(), t.rmin, t.rsx, t.rsy, t.rsz};
     t.bound_material.assign(count, 0u);
@@ -1405,6 +1418,10 @@ int bind_materials(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count) {
     std::iota(perm.begin(), perm.end(), 0u);
     std::mt19937 rng(0x5eedu);
     std::shuffle(perm.begin(), perm.end(), rng);
+    std::vector<uint64_t> morton = morton_keys(tasks, count);
+    for (uint32_t b = 0; b < count; b += TRAIN_RUN)
+        std::sort(perm.begin() + b, perm.begin() + std::min(count, b + TRAIN_RUN),
+                  [&](uint32_t x, uint32_t y) { return morton[x] < morton[y]; });
     HIP_OK(t.d_face_material.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
     HIP_OK(t.d_perm.reserve((size_t)std::max<uint32_t>(count, 1) * 4));
     HIP_OK(hipMemcpyAsync(t.d_face_material.ptr, t.bound_material.data(), (size_t)count * 4, hipMemcpyHostToDevice, t.stream));
@@ -1624,7 +1641,7 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->d_mlp_v, &t->d_mlp_g, &t->d_mlp_frag, &t->d_mlp_loss, &t->d_table,
                             &t->d_table_m, &t->d_table_v, &t->d_table_g, &t->b_learned, &t->b_queries,
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
-                            &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io,
+                            &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io, &t->d_touched,
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
                             &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
@@ -2173,6 +2190,23 @@ int vgi_mlp_train_samples(vgi_tracer *t, uint32_t samples) {
     if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
     if (samples < 1 || samples > 64) return fail("training samples per target must be 1..64");
     t->train_samples = samples;
+    return 0;
+}
+
+int vgi_mlp_encoding(vgi_tracer *t, uint32_t mode) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (mode != vgi::MLP_ENCODING_HASH && mode != vgi::MLP_ENCODING_FREQUENCY) return fail("unknown encoding");
+    t->mlp_encoding = mode;
+    return 0;
+}
+
+int vgi_mlp_training(vgi_tracer *t, uint32_t train_per_frame, uint32_t steps, float lr) {
+    if (!t || !t->mlp_ready) return fail("the learned cache is not enabled");
+    if (!(lr >= 0.0f)) return fail("the learning rate is not a non-negative number");
+    t->train_per_frame = train_per_frame;
+    t->train_steps = std::max<uint32_t>(steps, 1u);
+    t->mlp_lr = lr;
+    t->table_lr = 10.0f * lr;
     return 0;
 }
 

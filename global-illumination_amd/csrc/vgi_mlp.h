@@ -13,6 +13,12 @@
 // hash x ^ 2654435761 y ^ 805459861 z (mod 2^18). Features 0..15: the grid;
 // 16..21: sin and cos of pi u per axis; 22..24: the normal; 25..27: the
 // albedo; 28: 1 (the bias); 29..31: 0.
+//
+// Frequency encoding (MLP_ENCODING_FREQUENCY, the network the 512^3 cave
+// configuration states: a 32-dim frequency encoding of position, normal and
+// albedo): sin and cos of 2^k pi u per axis for k = 0..3 in place of the
+// grid -- features 0..11: k = 1, 2 (axis-major pairs per band), 12..15: k = 3
+// for x and y, 29..30: k = 3 for z -- and no hash table.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -52,12 +58,21 @@ struct MlpQuery {
 
 constexpr int MLP_HASH_LEVELS = 8;
 constexpr int MLP_HASH_FEATURES = 2;
-constexpr uint32_t MLP_HASH_SIZE = 1u << 18;  // entries per level
+#ifndef VGI_MLP_HASH_LOG2
+#define VGI_MLP_HASH_LOG2 18
+#endif
+constexpr uint32_t MLP_HASH_SIZE = 1u << VGI_MLP_HASH_LOG2;  // entries per level
 constexpr uint32_t MLP_TABLE = (uint32_t)MLP_HASH_LEVELS * MLP_HASH_SIZE * MLP_HASH_FEATURES;
+// Table entries a training sample touches (8 corners per level).
+constexpr uint32_t MLP_TOUCHED = (uint32_t)MLP_HASH_LEVELS * 8u;
+
+constexpr uint32_t MLP_ENCODING_HASH = 0u;
+constexpr uint32_t MLP_ENCODING_FREQUENCY = 1u;
 
 struct MlpScene {
     float lo_x, lo_y, lo_z;  // scene minimum corner (voxels)
     float inv_extent;        // 1 / largest scene extent
+    uint32_t encoding;       // MLP_ENCODING_HASH or MLP_ENCODING_FREQUENCY
     const float4 *materials; // (diffuse rgb, emission power), (emission rgb, 0)
     const float2 *table;     // level-major, MLP_HASH_SIZE entries of 2 features per level
     uint32_t res[MLP_HASH_LEVELS];
@@ -77,15 +92,19 @@ void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *que
 // Fused forward + loss + backward-data over `count` samples (count a multiple
 // of 16): writes per-layer inputs X_i and output gradients dY_i (bf16) for the
 // weight-gradient pass, and the summed loss.
-// `table_grad` gets the hash table's gradient added (FP32 atomics).
+// `table_grad` gets the hash table's gradient added (FP32 atomics) and
+// `touched` (count x MLP_TOUCHED words) the entries each sample touched
+// (0xffffffff for a sample without a query); hash encoding only.
 void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, const float *targets,
                           uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, float *table_grad,
-                          hipStream_t stream);
+                          uint32_t *touched, hipStream_t stream);
 // dW_i = sum over samples of dY_i^T X_i on MFMA (split over sample chunks,
 // FP32 atomics into `grad`).
 void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream);
 void mlp_adam(float *master, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream);
-// Adam over the hash table (eps 1e-15); clears its gradient.
-void mlp_table_adam(float *table, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream);
+// Lazy Adam over the hash-table entries listed in `touched` (samples x
+// MLP_TOUCHED words; eps 1e-15); clears their gradient.
+void mlp_table_adam(float *table, float *m, float *v, float *grad, const uint32_t *touched, uint32_t samples, uint32_t step, float lr,
+                    hipStream_t stream);
 
 }  // namespace vgi

@@ -146,18 +146,29 @@ __device__ __forceinline__ float2 hash_level(const MlpScene &sc, int level, f3 u
     return make_float2(a, b);
 }
 
+// sin (cosine = false) or cos of 2^band pi u along `axis`.
+__device__ __forceinline__ float frequency_feature(f3 u, int axis, int band, bool cosine) {
+    float x = get(u, axis) * (float)(1 << band) * 3.14159265358979f;
+    float sv, cv;
+    sincos_own(x, sv, cv);
+    return cosine ? cv : sv;
+}
+
+// Feature f < 16 of the frequency encoding: bands 1 and 2 for every axis,
+// then band 3 for x and y.
+__device__ __forceinline__ float frequency_low(f3 u, int f) {
+    if (f < 12) return frequency_feature(u, (f % 6) >> 1, 1 + f / 6, (f & 1) != 0);
+    return frequency_feature(u, (f - 12) >> 1, 3, (f & 1) != 0);
+}
+
 // Features 16..31 of a query: sin and cos of pi u per axis, the normal, the
-// albedo, the constant.
+// albedo, the constant (and band 3 of z in 29..30 with the frequency encoding).
 __device__ __forceinline__ float encode_tail(const MlpScene &sc, const MlpQuery &q, int f) {
     uint32_t face = q.face_material & 0xffu, material = (q.face_material >> 8) & 0xffu;
     f3 n = face_normal(face);
-    if (f < 22) {
-        f3 u = query_position(sc, q, n);
-        float x = get(u, (f - 16) >> 1) * 3.14159265358979f;
-        float sv, cv;
-        sincos_own(x, sv, cv);
-        return (f & 1) ? cv : sv;
-    }
+    if (f < 22) return frequency_feature(query_position(sc, q, n), (f - 16) >> 1, 0, (f & 1) != 0);
+    if ((f == 29 || f == 30) && sc.encoding == MLP_ENCODING_FREQUENCY)
+        return frequency_feature(query_position(sc, q, n), 2, 3, f == 30);
     if (f < 25) return get(n, f - 22);
     if (f < 28) {
         float4 d = sc.materials[material * 2u];
@@ -211,8 +222,13 @@ __device__ __forceinline__ void load_input(const MlpScene &sc, const MlpQuery &q
     float2 a = make_float2(0.0f, 0.0f), b = make_float2(0.0f, 0.0f);
     if (valid) {
         f3 u = query_position(sc, q, face_normal(q.face_material & 0xffu));
-        a = hash_level(sc, 2 * g, u);
-        b = hash_level(sc, 2 * g + 1, u);
+        if (sc.encoding == MLP_ENCODING_FREQUENCY) {
+            a = make_float2(frequency_low(u, 4 * g), frequency_low(u, 4 * g + 1));
+            b = make_float2(frequency_low(u, 4 * g + 2), frequency_low(u, 4 * g + 3));
+        } else {
+            a = hash_level(sc, 2 * g, u);
+            b = hash_level(sc, 2 * g + 1, u);
+        }
     }
     x0[0] = pack4(a.x, a.y, b.x, b.y);
     float v[4];
@@ -222,8 +238,11 @@ __device__ __forceinline__ void load_input(const MlpScene &sc, const MlpQuery &q
 }
 
 // Adds dL/d(feature) of a lane's two hash levels (rows 4g..4g+3 of dX0) into
-// the table gradient, spread over each level's corners by their weights.
-__device__ __forceinline__ void scatter_hash_grad(const MlpScene &sc, const MlpQuery &q, f32x4 dx, float *__restrict__ table_grad) {
+// the table gradient, spread over each level's corners by their weights, and
+// records the entries it touched (level * MLP_HASH_SIZE + index, 8 per level)
+// in the sample's row of MLP_TOUCHED words for the table's Adam step.
+__device__ __forceinline__ void scatter_hash_grad(const MlpScene &sc, const MlpQuery &q, f32x4 dx, float *__restrict__ table_grad,
+                                                  uint32_t *__restrict__ touched) {
     const int g = (threadIdx.x & 63) >> 4;
     f3 u = query_position(sc, q, face_normal(q.face_material & 0xffu));
     float d[4] = {dx.x, dx.y, dx.z, dx.w};
@@ -236,6 +255,7 @@ __device__ __forceinline__ void scatter_hash_grad(const MlpScene &sc, const MlpQ
         for (int c = 0; c < 8; ++c) {
             atomicAdd(&t[h.index[c] * 2u], h.weight[c] * d[2 * k]);
             atomicAdd(&t[h.index[c] * 2u + 1u], h.weight[c] * d[2 * k + 1]);
+            touched[level * 8 + c] = (uint32_t)level * MLP_HASH_SIZE + h.index[c];
         }
     }
 }
@@ -286,7 +306,8 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
                                                           const MlpQuery *__restrict__ queries, const float *__restrict__ targets,
                                                           uint32_t count, uint16_t *__restrict__ acts_out,
                                                           uint16_t *__restrict__ grads_out, float *__restrict__ loss,
-                                                          float *__restrict__ pred, float *__restrict__ table_grad) {
+                                                          float *__restrict__ pred, float *__restrict__ table_grad,
+                                                          uint32_t *__restrict__ touched) {
     __shared__ s16x8 lds[MLP_SLOTS * 64];
     stage_weights(frags, lds, MLP_SLOTS);
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
@@ -357,7 +378,14 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const uint16_t *__rest
     // group g holding the rows of the levels it encoded.
     f32x4 dx = mfma32(lds[bwd_slot(1, 0, 0) * 64 + lane], join(dy[0], dy[1]), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
     dx = mfma32(lds[bwd_slot(1, 0, 1) * 64 + lane], join(dy[2], dy[3]), dx);
-    if (valid) scatter_hash_grad(sc, q, dx, table_grad);
+    if (sc.encoding == MLP_ENCODING_HASH) {
+        uint32_t *row = touched + (size_t)n * MLP_TOUCHED;
+        if (valid) {
+            scatter_hash_grad(sc, q, dx, table_grad, row);
+        } else {
+            for (int k = 0; k < 16; ++k) row[16 * g + k] = NONE;
+        }
+    }
 }
 
 // dW_i[o][in] = sum_n dY_i[n][o] X_i[n][in]: one wavefront per (16x16 weight
@@ -418,23 +446,35 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ m, fl
 }
 
 // Adam over the hash table (eps 1e-15, as the hash-grid encoding is trained
-// in its paper); clears the gradient.
+// in its paper), lazily: only entries the step's samples touched, each once
+// (the first of its duplicates in the touched list takes the gradient and
+// clears it; the rest read 0 and skip); untouched entries keep their moments.
+// Its cost follows the samples, not the table's size.
 __global__ void mlp_table_adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, float *__restrict__ g,
-                                      float lr, float c1, float c2) {
+                                      const uint32_t *__restrict__ touched, uint32_t n, float lr, float c1, float c2) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= MLP_TABLE) return;
-    float gi = g[i];
-    float mi = 0.9f * m[i] + 0.1f * gi;
-    float vi = 0.99f * v[i] + 0.01f * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-15f);
-    g[i] = 0.0f;
+    if (i >= n) return;
+    uint32_t entry = touched[i];
+    if (entry == NONE) return;
+#pragma unroll
+    for (uint32_t f = 0; f < 2u; ++f) {
+        size_t k = (size_t)entry * 2u + f;
+        float gi = atomicExch(&g[k], 0.0f);
+        if (gi == 0.0f) continue;
+        float mi = 0.9f * m[k] + 0.1f * gi;
+        float vi = 0.99f * v[k] + 0.01f * gi * gi;
+        m[k] = mi;
+        v[k] = vi;
+        w[k] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-15f);
+    }
 }
 
-void mlp_table_adam(float *table, float *m, float *v, float *grad, uint32_t step, float lr, hipStream_t stream) {
+void mlp_table_adam(float *table, float *m, float *v, float *grad, const uint32_t *touched, uint32_t samples, uint32_t step, float lr,
+                    hipStream_t stream) {
+    if (samples == 0) return;
     float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
-    hipLaunchKernelGGL(mlp_table_adam_kernel, dim3((MLP_TABLE + 255u) / 256u), dim3(256), 0, stream, table, m, v, grad, lr, c1, c2);
+    uint32_t n = samples * (uint32_t)MLP_TOUCHED;
+    hipLaunchKernelGGL(mlp_table_adam_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, table, m, v, grad, touched, n, lr, c1, c2);
 }
 
 void mlp_pack(const float *master, uint16_t *frags, hipStream_t stream) {
@@ -449,11 +489,11 @@ void mlp_infer(const uint16_t *frags, const MlpScene &scene, const MlpQuery *que
 }
 void mlp_forward_backward(const uint16_t *frags, const MlpScene &scene, const MlpQuery *queries, const float *targets,
                           uint32_t count, uint16_t *acts, uint16_t *grads, float *loss, float *pred, float *table_grad,
-                          hipStream_t stream) {
+                          uint32_t *touched, hipStream_t stream) {
     if (count == 0) return;
     uint32_t tiles = count / 16u;
     hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((tiles + 3u) / 4u), dim3(256), 0, stream, frags, scene, queries, targets, count,
-                       acts, grads, loss, pred, table_grad);
+                       acts, grads, loss, pred, table_grad, touched);
 }
 void mlp_weight_grad(const uint16_t *acts, const uint16_t *grads, uint32_t count, float *grad, hipStream_t stream) {
     if (count == 0) return;

@@ -561,6 +561,17 @@ impl GpuTracer {
         check(unsafe { sys::vgi_mlp_train_samples(self.raw(), samples) })
     }
 
+    /// A new training schedule for the enabled cache, network kept; 0 targets freezes it.
+    pub fn mlp_training(&mut self, train_per_frame: u32, steps: u32, lr: f32) -> Result<()> {
+        check(unsafe { sys::vgi_mlp_training(self.raw(), train_per_frame, steps, lr) })
+    }
+
+    /// The position encoding: sys::VGI_MLP_ENCODING_HASH (default) or
+    /// sys::VGI_MLP_ENCODING_FREQUENCY (no hash table).
+    pub fn mlp_encoding(&mut self, mode: u32) -> Result<()> {
+        check(unsafe { sys::vgi_mlp_encoding(self.raw(), mode) })
+    }
+
     pub fn mlp_pretrain(&mut self, frames: u32) -> Result<()> {
         check(unsafe { sys::vgi_mlp_pretrain(self.raw(), frames) })
     }

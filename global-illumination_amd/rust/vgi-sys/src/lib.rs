@@ -101,6 +101,10 @@ pub struct vgi_schedule {
     pub mirror_budget: u32,
 }
 
+pub const VGI_MLP_ENCODING_HASH: u32 = 0;
+
+pub const VGI_MLP_ENCODING_FREQUENCY: u32 = 1;
+
 #[link(name = "vgi")]
 extern "C" {
     /// lifecycle: GpuTracer::new / enable_resident (gpu.rs:309-366)
@@ -370,6 +374,19 @@ extern "C" {
 
     /// Paths averaged into each training target (1..64; 1 by default).
     pub fn vgi_mlp_train_samples(tracer: *mut vgi_tracer, samples: u32) -> c_int;
+
+    /// Change the training schedule of an enabled cache without touching the
+    /// network or its Adam moments: training targets and Adam steps per frame and
+    /// the network's rate (the table's becomes ten times it). 0 targets freezes
+    /// the network: learned frames then only infer.
+    pub fn vgi_mlp_training(tracer: *mut vgi_tracer, train_per_frame: u32, steps: u32, lr: f32) -> c_int;
+
+    /// The position encoding of an enabled cache: 0 (the default) the hash grid
+    /// described above; 1 frequencies only -- sin and cos of 2^k pi u per axis,
+    /// k = 0..3, beside the normal and albedo: the 32-dim frequency encoding of
+    /// the network stated for the 512^3 cave configuration; the hash table is
+    /// then neither read nor trained. Kept across vgi_mlp_enable.
+    pub fn vgi_mlp_encoding(tracer: *mut vgi_tracer, mode: u32) -> c_int;
 
     /// Training-only frames on the bound faces (targets traced, Adam steps taken,
     /// bound faces not traced): the network's warm-up.

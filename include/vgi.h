@@ -318,6 +318,19 @@ int vgi_mlp_table(vgi_tracer *tracer, float *out, const float *in, float *grad_o
 int vgi_mlp_table_lr(vgi_tracer *tracer, float lr);
 /* Paths averaged into each training target (1..64; 1 by default). */
 int vgi_mlp_train_samples(vgi_tracer *tracer, uint32_t samples);
+/* Change the training schedule of an enabled cache without touching the
+ * network or its Adam moments: training targets and Adam steps per frame and
+ * the network's rate (the table's becomes ten times it). 0 targets freezes
+ * the network: learned frames then only infer. */
+int vgi_mlp_training(vgi_tracer *tracer, uint32_t train_per_frame, uint32_t steps, float lr);
+/* The position encoding of an enabled cache: 0 (the default) the hash grid
+ * described above; 1 frequencies only -- sin and cos of 2^k pi u per axis,
+ * k = 0..3, beside the normal and albedo: the 32-dim frequency encoding of
+ * the network stated for the 512^3 cave configuration; the hash table is
+ * then neither read nor trained. Kept across vgi_mlp_enable. */
+#define VGI_MLP_ENCODING_HASH 0u
+#define VGI_MLP_ENCODING_FREQUENCY 1u
+int vgi_mlp_encoding(vgi_tracer *tracer, uint32_t mode);
 /* Training-only frames on the bound faces (targets traced, Adam steps taken,
  * bound faces not traced): the network's warm-up. */
 int vgi_mlp_pretrain(vgi_tracer *tracer, uint32_t frames);

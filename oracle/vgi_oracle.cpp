@@ -1509,9 +1509,18 @@ struct Estimator {
     }
 
     struct Hit { I3 voxel; uint32_t face, status, material, slot; };
+    // Learned frames (no counterpart in the reference): where a path stopped
+    // into the network, and the factor multiplying its prediction there.
+    struct Query { I3 voxel; uint32_t face_material; F3 factor; };
 
-    // voxel_path.wgsl:418-492 (begin_path + advance_path, terminal = true)
-    F3 trace_path(F3 start, F3 first_direction, F3 initial_normal, uint32_t stream, uint32_t sequence, bool &unknown, Hit *first) const {
+    // voxel_path.wgsl:418-492 (begin_path + advance_path, terminal = true).
+    // With `query`: the learned cache's path, which ends at its first explicit
+    // hit (the world-cache termination of voxel_path.wgsl:459-463 with the
+    // network's prediction as the cached value) after that hit's emission
+    // (MIS-weighted) and direct light; the prediction's factor (throughput x
+    // albedo) and the hit face go to `query`.
+    F3 trace_path(F3 start, F3 first_direction, F3 initial_normal, uint32_t stream, uint32_t sequence, bool &unknown, Hit *first,
+                  Query *query = nullptr) const {
         F3 energy = f3(0, 0, 0);
         unknown = false;
         F3 throughput = f3(1, 1, 1);
@@ -1549,6 +1558,13 @@ struct Estimator {
             float light_pdf = selection_pdf * dot3(delta, delta) / fmax_(dot3(hit.normal, -direction), 1e-8f);
             float mis = 1.0f;
             if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
+            if (query) {
+                energy = energy + throughput * (m.emission * mis + m.diffuse * dv);
+                query->voxel = hit.voxel;
+                query->face_material = face_index_of(hit.normal) | (hit.material << 8);
+                query->factor = throughput * m.diffuse;
+                break;
+            }
             F3 cached;
             uint32_t cslot = NONE;
             bool found = cached_energy(hit.voxel, face_index_of(hit.normal), cached, &cslot);
@@ -1581,14 +1597,15 @@ struct Estimator {
         return energy;
     }
     // voxel_path.wgsl:504-520
-    F3 trace_receiver(F3 surface, F3 normal, uint32_t stream, uint32_t sequence, bool &known, Hit *first) const {
+    F3 trace_receiver(F3 surface, F3 normal, uint32_t stream, uint32_t sequence, bool &known, Hit *first, Query *query = nullptr) const {
         if (cn) cn->paths++;
         F3 direction = hemisphere(normal, sample_1d(stream, sequence, 0u), sample_1d(stream, sequence, 1u));
         F3 value = sample_emitter(surface, normal, stream, sequence, 0u);
         bool unknown = false;
         if (first) { first->voxel = I3{0, 0, 0}; first->face = 6u; first->status = 4u; first->material = 0; first->slot = NONE; }
+        if (query) { query->voxel = I3{0, 0, 0}; query->face_material = NONE; query->factor = f3(0, 0, 0); }
         if (dot3(normal, direction) > 0.0f) {
-            F3 c = trace_path(surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first);
+            F3 c = trace_path(surface + normal * 0.001f, direction, normal, stream, sequence, unknown, first, query);
             value = value + c;
         }
         known = !unknown;
@@ -2250,10 +2267,34 @@ static double mlp_hash_feature(const MlpGrid &g, int level, int feature, F3 u) {
     return (double)a;
 }
 
+// The frequency encoding (the 512^3 cave configuration's stated network):
+// sin and cos of 2^k pi u per axis, k = 0..3; features 0..11 hold k = 1, 2
+// (axis-major pairs per band), 12..15 k = 3 for x and y, 29..30 k = 3 for z.
+static int mlp_encoding = 0;
+static double mlp_sincos(float v, int band, bool cosine) {
+    float x = v * (float)(1 << band) * 3.14159265358979f;
+    int k;
+    float r = reduce_pio2(x, &k);
+    float sp = sin_poly(r), cp = cos_poly(r);
+    float sv, cv;
+    switch (k & 3) {
+    case 0: sv = sp; cv = cp; break;
+    case 1: sv = cp; cv = -sp; break;
+    case 2: sv = -sp; cv = -cp; break;
+    default: sv = -cp; cv = sp; break;
+    }
+    return cosine ? cv : sv;
+}
+
 static double mlp_encode(const uint32_t *q, int f, const float *diffuse, const float scene[4], const MlpGrid &g) {
     uint32_t face = q[3] & 0xffu, material = (q[3] >> 8) & 0xffu;
     F3 n = face_normal(face);
     F3 u = mlp_position(q, scene);
+    if (mlp_encoding == 1) {
+        if (f < 12) return mlp_sincos(u[(f % 6) >> 1], 1 + f / 6, (f & 1) != 0);
+        if (f < 16) return mlp_sincos(u[(f - 12) >> 1], 3, (f & 1) != 0);
+        if (f == 29 || f == 30) return mlp_sincos(u[2], 3, f == 30);
+    }
     if (f < 16) return mlp_hash_feature(g, f >> 1, f & 1, u);
     if (f < 22) {
         float x = u[(f - 16) >> 1] * 3.14159265358979f;
@@ -2519,6 +2560,46 @@ int64_t orc_tracer_emitters(void *tp, uint32_t *out, int64_t capacity) {
 int orc_trace_faces(void *tp, const uint32_t *tasks, uint32_t count, float *results, uint32_t *hits, uint64_t *counters,
                     int threads) {
     return trace_faces(*(Tracer *)tp, tasks, count, results, hits, counters, threads);
+}
+
+// A learned frame's paths, per face-sample (samples i = 0.. of each task's
+// sequence): the partial value (xyz) and known flag (w), the prediction
+// factor, and the query (voxel xyz, face | material << 8; NONE when the
+// sample is unknown or its path stopped without a hit). No cache lookups.
+int orc_trace_learned(void *tp, const uint32_t *tasks, uint32_t count, float *partial, float *factor, uint32_t *queries, int threads) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    const uint32_t spp = t.transport.samples;
+    parallel_for(count, threads, [&](uint32_t b, uint32_t e, int) {
+        Estimator est(t, nullptr);
+        for (uint32_t f = b; f < e; ++f) {
+            const uint32_t *task = tasks + (size_t)f * 8;
+            I3 voxel{(int32_t)task[0], (int32_t)task[1], (int32_t)task[2]};
+            uint32_t face = task[3];
+            for (uint32_t i = 0; i < spp; ++i) {
+                size_t at = (size_t)f * spp + i;
+                float *pv = partial + at * 4;
+                float *fv = factor + at * 3;
+                uint32_t *qv = queries + at * 4;
+                pv[0] = pv[1] = pv[2] = pv[3] = 0.0f;
+                fv[0] = fv[1] = fv[2] = 0.0f;
+                qv[0] = qv[1] = qv[2] = 0u;
+                qv[3] = NONE;
+                if (face == NONE) continue;
+                F3 normal = face_normal(face);
+                F3 surface = to_f3(voxel) + f3(0.5f, 0.5f, 0.5f) + normal * 0.5f;
+                uint32_t stream = hash_voxel(voxel) ^ pcg(face + 1u);
+                bool k;
+                Estimator::Query q;
+                F3 value = est.trace_receiver(surface, normal, stream, task[4] + i, k, nullptr, &q);
+                pv[0] = value.x; pv[1] = value.y; pv[2] = value.z; pv[3] = k ? 1.0f : 0.0f;
+                fv[0] = q.factor.x; fv[1] = q.factor.y; fv[2] = q.factor.z;
+                qv[0] = (uint32_t)q.voxel.x; qv[1] = (uint32_t)q.voxel.y; qv[2] = (uint32_t)q.voxel.z;
+                qv[3] = k ? q.face_material : NONE;
+            }
+        }
+    });
+    return 0;
 }
 
 
@@ -3191,7 +3272,7 @@ int orc_mlp_gradient(const float *w, const float *table, const uint32_t *queries
         }
         // The hash features' gradient W1^T dY1, spread over each level's
         // corners by their weights.
-        if (queries[i * 4 + 3] == NONE) continue;
+        if (queries[i * 4 + 3] == NONE || mlp_encoding != 0) continue;
         F3 u = mlp_position(&queries[i * 4], scene);
         for (int level = 0; level < HASH_LEVELS; ++level) {
             uint32_t index[8];
@@ -3220,6 +3301,20 @@ int orc_mlp_hash(const float *table, const uint32_t *queries, uint32_t n, const 
             mlp_corners(g.res[level], u, &index[(i * HASH_LEVELS + level) * 8], &weight[(i * HASH_LEVELS + level) * 8]);
         }
     }
+    return 0;
+}
+
+// The 32 encoded features of each query (before BF16 rounding): a test hook
+// for the encodings' layout.
+int orc_mlp_encode(const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out) {
+    MlpGrid g = mlp_grid(table, scene);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int f = 0; f < 32; ++f) out[(size_t)i * 32 + f] = mlp_encode(&queries[i * 4], f, diffuse, scene, g);
+    return 0;
+}
+
+int orc_mlp_encoding(int mode) {
+    mlp_encoding = mode;
     return 0;
 }
 

@@ -106,6 +106,11 @@ int64_t orc_tracer_emitters(void *tracer, uint32_t *out_words, int64_t capacity)
  * counters (optional): [rays, steps, region loads, brick loads, cell loads, paths, 0, 0]. */
 int orc_trace_faces(void *tracer, const uint32_t *tasks, uint32_t count, float *results,
                     uint32_t *hits, uint64_t *counters, int threads);
+/* A learned frame's paths (ending at the first hit into the network): per
+ * face-sample the partial value + known flag (4 floats), the prediction
+ * factor (3 floats) and the query (4 u32). */
+int orc_trace_learned(void *tracer, const uint32_t *tasks, uint32_t count, float *partial, float *factor, uint32_t *queries,
+                      int threads);
 
 /* ---- resident world cache (voxel_accumulate.wgsl, face_index.rs, resident.rs) ---- */
 /* plain != 0: slots and accumulation as usual, but paths never end at cached faces. */
@@ -192,6 +197,11 @@ int orc_mlp_levels(const float scene[4], uint32_t *res);
 /* Exact mode on/off: no BF16 rounding and double hash features (the gradient
  * is then the exact derivative of the double-precision forward). */
 int orc_mlp_exact(int on);
+/* 0: the hash-grid encoding; 1: the frequency encoding (sin/cos of 2^k pi u,
+ * k = 0..3, no table). */
+int orc_mlp_encoding(int mode);
+/* The 32 encoded input features of each query (n x 32 doubles). */
+int orc_mlp_encode(const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out);
 
 /* Packed scene of a tracer: [cell words, regions, directory capacity,
  * interned mixed bricks, brick references, 0, 0, 0]; a voxel's material

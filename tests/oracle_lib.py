@@ -68,6 +68,7 @@ def lib():
         "orc_tracer_sync": (ctypes.c_int, [P]),
         "orc_tracer_emitters": (i64, [P, P, i64]),
         "orc_trace_faces": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
+        "orc_trace_learned": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
         "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32, u32]),
         "orc_cache_register": (ctypes.c_int, [P, P, u32, P]),
         "orc_cache_batch": (ctypes.c_int, [P, P, P, u32, P, P, P, ctypes.c_int, u32]),
@@ -98,6 +99,8 @@ def lib():
         "orc_mlp_hash": (ctypes.c_int, [P, P, u32, P, P, P, P]),
         "orc_mlp_levels": (ctypes.c_int, [P, P]),
         "orc_mlp_exact": (ctypes.c_int, [ctypes.c_int]),
+        "orc_mlp_encoding": (ctypes.c_int, [ctypes.c_int]),
+        "orc_mlp_encode": (ctypes.c_int, [P, P, u32, P, P, P]),
         "orc_packed_info": (ctypes.c_int, [P, P]),
         "orc_packed_material": (u32, [P, i32, i32, i32]),
         "orc_packed_words": (i64, [P, ctypes.c_int, P, i64]),
@@ -222,6 +225,16 @@ class Tracer:
         c = np.zeros(8, np.uint64) if counters else None
         lib().orc_trace_faces(self.h, ptr(tasks), n, ptr(results), ptr(h), ptr(c), threads)
         return results, h, c
+
+    def trace_learned(self, tasks, threads=8):
+        """A learned frame's paths: (partial [n*spp, 4], factor [n*spp, 3], queries [n*spp, 4])."""
+        tasks = np.ascontiguousarray(tasks).view(np.uint32).reshape(-1, 8).copy()
+        n = len(tasks) * self.samples
+        partial = np.zeros((n, 4), np.float32)
+        factor = np.zeros((n, 3), np.float32)
+        queries = np.zeros((n, 4), np.uint32)
+        lib().orc_trace_learned(self.h, ptr(tasks), len(tasks), ptr(partial), ptr(factor), ptr(queries), threads)
+        return partial, factor, queries
 
     def trace_ray(self, origin, direction, max_distance=768.0):
         o = np.asarray(origin, np.float32)
@@ -474,6 +487,22 @@ def mlp_hash(table, queries, scene):
 
 def mlp_exact(on):
     lib().orc_mlp_exact(1 if on else 0)
+
+
+def mlp_encoding(mode):
+    """0: hash grid (default); 1: frequency encoding."""
+    lib().orc_mlp_encoding(int(mode))
+
+
+def mlp_encode(table, queries, diffuse, scene):
+    """The 32 input features of each query (float64, before BF16 rounding)."""
+    t = np.ascontiguousarray(table, np.float32)
+    q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+    d = np.ascontiguousarray(diffuse, np.float32)
+    sc = np.ascontiguousarray(scene, np.float32)
+    out = np.zeros((len(q), 32), np.float64)
+    lib().orc_mlp_encode(ptr(t), ptr(q), len(q), ptr(d), ptr(sc), ptr(out))
+    return out
 
 
 def mlp_levels(scene):

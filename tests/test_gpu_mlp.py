@@ -35,10 +35,19 @@ def random_queries(vgi, setup, n, rng):
 LAYERS = [(0, 2048), (2048, 6144), (6144, 10240), (10240, 14336), (14336, 14528)]
 
 
-def test_mlp_forward_matches_fp64_oracle(vgi, orc):
+@pytest.fixture(params=[0, 1], ids=["hash", "frequency"])
+def encoding(request, orc):
+    """Both position encodings: the hash grid and the frequency-only one."""
+    orc.mlp_encoding(request.param)
+    yield request.param
+    orc.mlp_encoding(0)
+
+
+def test_mlp_forward_matches_fp64_oracle(vgi, orc, encoding):
     setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
     g = gpu_tracer(vgi, setup)
     g.mlp_enable(seed=7)
+    g.mlp_encoding(encoding)
     rng = np.random.default_rng(3)
     q = random_queries(vgi, setup, 4096, rng)
     q[::97, 3] = 0xFFFFFFFF  # invalid queries answer 0
@@ -58,10 +67,11 @@ def test_mlp_forward_matches_fp64_oracle(vgi, orc):
     g.close()
 
 
-def test_mlp_gradient_matches_fp64_oracle(vgi, orc):
+def test_mlp_gradient_matches_fp64_oracle(vgi, orc, encoding):
     setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
     g = gpu_tracer(vgi, setup)
     g.mlp_enable(seed=11)
+    g.mlp_encoding(encoding)
     rng = np.random.default_rng(4)
     q = random_queries(vgi, setup, 512, rng)
     targets = np.zeros((512, 4), np.float32)
@@ -80,9 +90,11 @@ def test_mlp_gradient_matches_fp64_oracle(vgi, orc):
         cos = np.dot(grad[a:b], ref[a:b]) / (np.linalg.norm(grad[a:b]) * np.linalg.norm(ref[a:b]) + 1e-30)
         print("layer [%d,%d): relative error %.3g cosine %.6f" % (a, b, e, cos), flush=True)
         assert e < 2e-2 and cos > 0.9995
-    # The hash table's gradient, per level.
+    # The hash table's gradient, per level (none with the frequency encoding).
     size = 1 << 19
-    for level in range(8):
+    if encoding == 1:
+        assert not table_grad.any() and not ref_table.any()
+    for level in range(8 if encoding == 0 else 0):
         a, b = table_grad[level * size:(level + 1) * size], ref_table[level * size:(level + 1) * size]
         e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         cos = np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
@@ -161,3 +173,127 @@ def test_learned_cache_gains_over_plain(vgi, orc):
         assert curves["learned"][f] < 0.92 * curves["plain"][f]
     # The learned frame stops at the first hit: fewer rays than plain paths.
     assert rays["learned"] < 0.8 * rays["plain"]
+
+
+def test_frozen_learned_frame_matches_oracle(vgi, orc):
+    """One learned frame with frozen weights against the oracle: the oracle's
+    learned paths (first hit, its MIS-weighted emission and direct light, the
+    receiver's emitter sample; bit-exact FP32 path arithmetic) plus its FP64
+    forward of the same weights and hash table at the same queries, combined
+    per face as value = partial + factor * max(prediction, 0).
+
+    Bar: known sample counts exact; faces none of whose samples query the
+    network equal the oracle to FP32 summation order (1e-5 relative); every
+    face within the forward's BF16 tolerance (2% of the mean |prediction|,
+    times the face's mean |factor|) of the oracle."""
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    env = (0.3, 0.5, 0.8)
+    faces = vgi.exposed_faces(setup["grid"])[::3][:8192]
+    g = gpu_tracer(vgi, setup)
+    g.set_transport(bounces=3, samples=2, environment=env, plain=True)
+    g.bind_faces(faces)
+    g.mlp_enable(seed=5, train_per_frame=4096, steps=4, lr=3e-3)
+    g.mlp_train_samples(4)
+    g.mlp_pretrain(30)
+    g.mlp_training(0, 1, 3e-3)  # frozen: the frame only infers
+    w = g.mlp_weights()
+    table = g.mlp_table()
+    g.run_frame()
+    g.sync()
+    rec, _ = g.read_bound()
+    # A frozen network trains nothing during the frame.
+    assert np.array_equal(g.mlp_weights(), w)
+    tasks = g.bound_tasks()
+    scene = scene_of(g)
+    g.close()
+
+    t = setup["tracer"]
+    t.set_transport(bounces=3, samples=2, environment=env)
+    partial, factor, queries = t.trace_learned(tasks)
+    pred = orc.mlp_forward(w, table, queries, setup["diffuse"], scene)
+    known = partial[:, 3] > 0
+    value = partial[:, 0:3].astype(np.float64) + factor.astype(np.float64) * np.maximum(pred, 0.0)
+    value[~known] = 0.0
+    n = len(faces)
+    spp = 2
+    k = known.reshape(n, spp).sum(axis=1)
+    mean = value.reshape(n, spp, 3).sum(axis=1) / np.maximum(k, 1)[:, None]
+    assert np.array_equal(rec[:, 5].astype(np.int64), k), "known sample counts"
+
+    queried = (queries[:, 3] != 0xFFFFFFFF).reshape(n, spp).any(axis=1)
+    term = np.abs(factor * np.maximum(pred, 0.0)).sum(axis=1).mean()
+    print("faces %d, queried %d, mean |factor * prediction| %.4f, mean |partial| %.4f" % (
+        n, int(queried.sum()), term, np.abs(partial[:, 0:3]).sum(axis=1).mean()), flush=True)
+    assert queried.sum() > 0.5 * n and (~queried).sum() > 0
+    # The prediction term matters: a wrong factor or query could not hide.
+    assert term > 0.1 * np.abs(partial[:, 0:3]).sum(axis=1).mean()
+    plain_faces = ~queried
+    assert np.allclose(rec[plain_faces, 0:3], mean[plain_faces], rtol=1e-5, atol=1e-6)
+    scale = np.abs(pred[queries[:, 3] != 0xFFFFFFFF]).mean()
+    fmag = np.abs(factor).reshape(n, spp, 3).mean(axis=1)
+    bound = 2e-2 * scale * fmag + 1e-5 * np.abs(mean) + 1e-6
+    err = np.abs(rec[:, 0:3] - mean)
+    worst = float((err / bound).max())
+    print("worst error / bound %.3f" % worst, flush=True)
+    assert worst <= 1.0
+
+
+def test_adam_steps_follow_the_update_rule(vgi, orc):
+    """Two Adam steps against the rule restated in numpy (Kingma & Ba 2015;
+    beta1 0.9, beta2 0.99; eps 1e-10 for the network, 1e-15 for the table;
+    bias correction by the global step): the network's weights everywhere,
+    the hash table lazily -- only entries whose gradient is non-zero in a step
+    move; an entry the second batch does not touch keeps its value exactly
+    (no momentum-only step), and entries neither batch touched keep their
+    initial values. Each step's gradient is read first with a gradient-only
+    call at the same weights.
+
+    Tolerance: the update divides the gradient by its own RMS, so values whose
+    gradient is within FP32 atomic-order noise of 0 may take either sign; the
+    check covers entries with |g| above 1e-4 of the largest and allows at most
+    0.1% of them to differ by more than 1e-5 relative."""
+    setup = make_scene(vgi, orc, 2, 0x5EED0002, edge=64)
+    g = gpu_tracer(vgi, setup)
+    lr = 1e-3
+    g.mlp_enable(seed=13, lr=lr)
+    rng = np.random.default_rng(6)
+    table = rng.uniform(-0.5, 0.5, vgi.MLP_TABLE).astype(np.float32)
+    g.mlp_table(table)
+    w = g.mlp_weights().astype(np.float64)
+    t = table.astype(np.float64)
+    mw, vw = np.zeros_like(w), np.zeros_like(w)
+    mt, vt = np.zeros_like(t), np.zeros_like(t)
+    ever = np.zeros(len(t), bool)
+    for step in range(2):
+        q = random_queries(vgi, setup, 1024 if step == 0 else 256, rng)
+        targets = np.zeros((len(q), 4), np.float32)
+        targets[:, :3] = rng.random((len(q), 3)).astype(np.float32)
+        targets[:, 3] = 1.0
+        gw, _ = g.mlp_train(q, targets, update=False)
+        gw = gw.astype(np.float64)
+        gt = g.mlp_table_grad().astype(np.float64)
+        before = g.mlp_table()
+        g.mlp_train(q, targets, update=True)
+        c1, c2 = 1.0 - 0.9 ** (step + 1), 1.0 - 0.99 ** (step + 1)
+        mw = 0.9 * mw + 0.1 * gw
+        vw = 0.99 * vw + 0.01 * gw * gw
+        w = w - lr * (mw / c1) / (np.sqrt(vw / c2) + 1e-10)
+        live = gt != 0.0
+        ever |= live
+        mt[live] = 0.9 * mt[live] + 0.1 * gt[live]
+        vt[live] = 0.99 * vt[live] + 0.01 * gt[live] ** 2
+        t[live] = t[live] - 10.0 * lr * (mt[live] / c1) / (np.sqrt(vt[live] / c2) + 1e-15)
+        got_w = g.mlp_weights().astype(np.float64)
+        got_t = g.mlp_table()
+        for name, got, want, grad in (("network", got_w, w, gw), ("table", got_t.astype(np.float64), t, gt)):
+            sel = np.abs(grad) > 1e-4 * np.abs(grad).max()
+            bad = np.abs(got[sel] - want[sel]) > 1e-5 * np.maximum(np.abs(want[sel]), 1e-2)
+            print("step %d %s: %d checked, %d differ" % (step + 1, name, int(sel.sum()), int(bad.sum())), flush=True)
+            assert sel.sum() > 100 and bad.sum() <= 1e-3 * sel.sum()
+        assert np.array_equal(got_t[~live], before[~live]), "entries the step did not touch keep their values"
+        if step == 1:
+            assert (ever & ~live).sum() > 1000, "some entries were touched by the first batch only"
+        assert np.array_equal(got_t[~ever], table[~ever])
+        # Continue from the library's FP32 state.
+        w, t = got_w, got_t.astype(np.float64)
+    g.close()

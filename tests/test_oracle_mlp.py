@@ -168,3 +168,103 @@ def test_gradients_match_finite_differences(exact):
             for c in range(8):
                 touched[(level * SIZE + index[i, level, c]) * 2 + np.arange(2)] = True
     assert np.all(table_grad[~touched] == 0.0)
+
+
+def test_learned_paths_stop_at_the_plain_first_hit():
+    """The oracle's learned path (the learned frame's: a path ends at its
+    first explicit hit into the network) against its plain path on the same
+    samples: the query is the plain path's first-hit voxel, face and material;
+    the factor is that material's albedo (throughput 1 at the first hit); a
+    sample that leaves to the sky is the plain sample exactly; a sample that
+    hits gathers no more than the plain path (every omitted term is
+    non-negative). A learned path that queried the wrong vertex, dropped the
+    albedo or kept tracing would fail one of these."""
+    from test_oracle_edits import pages_of
+    edge = 32
+    grid = np.full((edge, edge, edge), 256, np.uint16)
+    grid[:, 0:2, :] = 1
+    grid[10:13, 2:13, 10:13] = 2
+    grid[28:32, 2:24, :] = 1  # two walls
+    grid[:, 2:24, 28:32] = 2
+    grid[20, 2, 5] = 7
+    diffuse = np.full((256, 3), 0.5, np.float32)
+    diffuse[2] = (0.3, 0.6, 0.9)
+    emission = np.zeros((256, 3), np.float32)
+    emission[7] = (4.0, 2.0, 1.0)
+    scene = orc.Scene()
+    scene.load_pages(*pages_of(grid))
+    t = orc.Tracer(scene)
+    t.set_materials(diffuse, emission)
+    sun = np.zeros(12, np.float32)
+    sun[0:3] = np.array([0.3, 0.8, -0.5], np.float32) / np.linalg.norm([0.3, 0.8, -0.5])
+    sun[3] = -1.0
+    sun[4:7] = 3.0
+    t.set_lights(sun.reshape(1, 12))
+    t.set_transport(bounces=3, samples=1, lanes=1, environment=(0.2, 0.3, 0.4))
+    voxels = [(x, 1, z, 2) for x in range(1, 27, 2) for z in range(1, 27, 2) if not (9 <= x <= 13 and 9 <= z <= 13)]
+    voxels += [(12, y, 11, 0) for y in range(2, 13)] + [(11, y, 10, 5) for y in range(2, 13)]
+    tasks = np.zeros(len(voxels) * 4, [("voxel", "<i4", (3,)), ("face", "<u4"), ("sequence", "<u4"), ("padding", "<u4", (3,))])
+    for k, (x, y, z, f) in enumerate(voxels * 4):
+        tasks[k] = ((x, y, z), f, 17 + 5 * k, (0, 0, 0))
+    res, hits, _ = t.trace_faces(tasks, hits=True)
+    partial, factor, q = t.trace_learned(tasks)
+    status = hits[:, 3] >> 8
+    known = partial[:, 3] > 0
+    assert np.array_equal(known, hits[:, 7] == 1)
+    hit = (status == 1) & known
+    sky = (status == 0) & known
+    assert hit.sum() > 100 and sky.sum() > 100
+    assert np.array_equal(q[hit, 0:3], hits[hit, 0:3])
+    assert np.array_equal(q[hit, 3] & 0xFF, hits[hit, 3] & 0xFF)
+    assert np.array_equal(q[hit, 3] >> 8, hits[hit, 4])
+    assert np.array_equal(factor[hit], diffuse[hits[hit, 4]])
+    assert np.all(q[~hit, 3] == 0xFFFFFFFF) and np.all(factor[~hit] == 0.0)
+    assert np.array_equal(partial[sky, 0:3], res[sky, 0:3])
+    assert np.all(res[hit, 0:3] >= partial[hit, 0:3] - 1e-6)
+    assert np.any(res[hit, 0:3] > partial[hit, 0:3] + 1e-3)
+
+
+@pytest.fixture
+def frequency():
+    orc.mlp_encoding(1)
+    yield
+    orc.mlp_encoding(0)
+
+
+def test_encoding_layouts(frequency):
+    """Both encodings' 32 features against their definitions, computed here in
+    float64 numpy: hash mode -- the grid's 16 features (orc_mlp_hash), sin and
+    cos of pi u per axis, the normal, the albedo, 1, zeros; frequency mode --
+    sin and cos of 2^k pi u for k = 1, 2 (axis-major pairs per band) in 0..11,
+    k = 3 for x, y in 12..15 and for z in 29..30, the rest as in hash mode.
+    u is the face centre over the scene's largest extent."""
+    rng = np.random.default_rng(8)
+    extent = 256
+    sc = np.array([-32.0, 0.0, 16.0, 1.0 / extent], np.float32)
+    q = queries_in(rng, 300, 200)
+    q[:, 0:3] = (q[:, 0:3].astype(np.int64) + [-32, 0, 16]).astype(np.uint32)
+    diffuse = rng.random((256, 3)).astype(np.float32)
+    table = rng.uniform(-1, 1, SIZE * 16).astype(np.float32)
+    normals = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float64)
+    face = q[:, 3] & 0xFF
+    mat = (q[:, 3] >> 8) & 0xFF
+    centre = q[:, 0:3].view(np.int32).astype(np.float64) + 0.5 + 0.5 * normals[face]
+    u = (centre - sc[0:3].astype(np.float64)) / extent
+    tail = np.concatenate([np.stack([np.sin(np.pi * u[:, a]) if c == 0 else np.cos(np.pi * u[:, a])
+                                     for a in range(3) for c in range(2)], axis=1),
+                           normals[face], diffuse[mat].astype(np.float64), np.ones((len(q), 1))], axis=1)
+    freq = np.stack([np.sin(2.0 ** k * np.pi * u[:, a]) if c == 0 else np.cos(2.0 ** k * np.pi * u[:, a])
+                     for k in (1, 2) for a in range(3) for c in range(2)], axis=1)
+    band3 = lambda a, c: np.sin(8 * np.pi * u[:, a]) if c == 0 else np.cos(8 * np.pi * u[:, a])
+    got = orc.mlp_encode(table, q, diffuse, sc)
+    assert np.allclose(got[:, 0:12], freq, atol=2e-6)
+    assert np.allclose(got[:, 12:16], np.stack([band3(0, 0), band3(0, 1), band3(1, 0), band3(1, 1)], axis=1), atol=2e-6)
+    assert np.allclose(got[:, 16:29], tail, atol=2e-6)
+    assert np.allclose(got[:, 29:31], np.stack([band3(2, 0), band3(2, 1)], axis=1), atol=2e-6)
+    assert np.all(got[:, 31] == 0.0)
+    orc.mlp_encoding(0)
+    got = orc.mlp_encode(table, q, diffuse, sc)
+    feats, _, _ = orc.mlp_hash(table, q, sc)
+    assert np.array_equal(got[:, 0:16].astype(np.float32), feats)
+    assert np.allclose(got[:, 16:29], tail, atol=2e-6)
+    assert np.all(got[:, 29:32] == 0.0)
