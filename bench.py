@@ -34,6 +34,11 @@ LEARNED_TRAIN = (81920, 8, 16)
 LEARNED_SPP = 1
 # Adam rate of the network (the hash table's is ten times it).
 LEARNED_LR = 3e-3
+# History recency of the learned form's temporal merge (the transport's
+# history_recency, resident.rs:721-725): sample i of n weighs about
+# (i/n)^0.25, so the frames of the network's warm-up fade faster than in
+# the exact mean.
+LEARNED_RECENCY = 1.25
 # The network as the 512^3 cave configuration states it: a 32-dim frequency
 # encoding (no hash grid), 64k one-path samples per frame, Adam at 1e-3 (the
 # steps per frame are not stated: 8, as above).
@@ -166,13 +171,13 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     ordered = np.concatenate([faces[judged_idx], faces[rest]])
     judged = len(judged_idx)
 
-    def tracer(samples=spp):
+    def tracer(samples=spp, recency=1.0):
         g = vgi.GpuTracer(0)
         g.enable_resident()
         g.upload_pages(coords, pages)
         g.set_materials(diffuse, emission)
         g.set_lights(lights)
-        g.set_transport(bounces=bounces, samples=samples, environment=tuple(env), plain=True)
+        g.set_transport(bounces=bounces, samples=samples, environment=tuple(env), plain=True, history_recency=recency)
         return g
 
     t0 = time.time()
@@ -207,7 +212,7 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     # measures what the hash grid and the averaged targets are for.
     for name, encoding, train, lr in (("learned", vgi.MLP_ENCODING_HASH, LEARNED_TRAIN, LEARNED_LR),
                                       ("learned_stated_network", vgi.MLP_ENCODING_FREQUENCY, STATED_TRAIN, STATED_LR)):
-        g = tracer(LEARNED_SPP)
+        g = tracer(LEARNED_SPP, LEARNED_RECENCY)
         g.bind_faces(ordered)
         g.mlp_enable(seed=1, train_per_frame=train[0], steps=train[1], lr=lr)
         g.mlp_encoding(encoding)
@@ -242,6 +247,7 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
             encoding, train, lr = net
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
+            entry["history_recency"] = LEARNED_RECENCY
             entry["training"] = ("%d targets per frame of %d path%s each (3-bounce), %d Adam steps (lr %g%s), BF16 MFMA; encoding: %s" % (
                 train[0], train[2], "s" if train[2] > 1 else "", train[1], lr,
                 ", hash table %g, lazy" % (10 * lr) if encoding == vgi.MLP_ENCODING_HASH else "",
@@ -567,7 +573,8 @@ def main():
     if args.learned_only:
         # Learned frames alone (for profiling): the cache warmed up on
         # `pretrain_frames` training-only frames, then `steps` timed frames.
-        g.set_transport(bounces=bounces, samples=LEARNED_SPP, environment=tuple(optics[3]), plain=True)
+        g.set_transport(bounces=bounces, samples=LEARNED_SPP, environment=tuple(optics[3]), plain=True,
+                        history_recency=LEARNED_RECENCY)
         g.mlp_enable(seed=1, train_per_frame=LEARNED_TRAIN[0], steps=LEARNED_TRAIN[1], lr=LEARNED_LR)
         g.mlp_train_samples(LEARNED_TRAIN[2])
         g.mlp_pretrain(args.pretrain_frames)

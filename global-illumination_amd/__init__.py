@@ -25,7 +25,7 @@ NO_FACE = 0xFFFFFFFF
 RECORD_HITS = 1
 COUNT_RAYS = 2
 COUNT_GATHERS = 4
-MLP_TABLE = 8 * (1 << 18) * 2  # hash-grid entries (level-major, 2 features each)
+MLP_TABLE = 8 * (1 << 22) * 2  # hash-grid entries (level-major, 2 features each)
 MLP_PARAMS = 14528
 NOALLOC_MAX_FACES = 262144
 

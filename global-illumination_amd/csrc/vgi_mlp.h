@@ -9,8 +9,8 @@
 // level l a grid of res_l cells across the scene's largest extent (res_l =
 // floor(16 b^l), b = (extent / 16)^(1/7)), the features trilinearly
 // interpolated from the cell's 8 corners; a level whose (res_l + 2)^3 corners
-// fit the 2^18-entry table indexes them densely, finer levels by the spatial
-// hash x ^ 2654435761 y ^ 805459861 z (mod 2^18). Features 0..15: the grid;
+// fit the 2^22-entry table indexes them densely, finer levels by the spatial
+// hash x ^ 2654435761 y ^ 805459861 z (mod 2^22). Features 0..15: the grid;
 // 16..21: sin and cos of pi u per axis; 22..24: the normal; 25..27: the
 // albedo; 28: 1 (the bias); 29..31: 0.
 //
@@ -59,7 +59,7 @@ struct MlpQuery {
 constexpr int MLP_HASH_LEVELS = 8;
 constexpr int MLP_HASH_FEATURES = 2;
 #ifndef VGI_MLP_HASH_LOG2
-#define VGI_MLP_HASH_LOG2 18
+#define VGI_MLP_HASH_LOG2 22
 #endif
 constexpr uint32_t MLP_HASH_SIZE = 1u << VGI_MLP_HASH_LOG2;  // entries per level
 constexpr uint32_t MLP_TABLE = (uint32_t)MLP_HASH_LEVELS * MLP_HASH_SIZE * MLP_HASH_FEATURES;

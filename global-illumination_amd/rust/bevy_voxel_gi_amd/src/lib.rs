@@ -39,7 +39,7 @@ pub const META_WORDS: usize = 12;
 /// FP32 master weights of the learned cache's network (vgi_mlp_weights).
 pub const MLP_WEIGHTS: usize = 14_528;
 /// FP32 entries of the learned cache's hash grid (vgi_mlp_table).
-pub const MLP_TABLE: usize = 4_194_304;
+pub const MLP_TABLE: usize = 67_108_864;
 /// Words of the sun-flip region (vgi_read_sun_flips).
 pub const SUN_FLIP_WORDS: usize = 8_204;
 /// Words of one canonical region (vgi_scene_region_canonical): the 4 header

@@ -364,8 +364,8 @@ extern "C" {
     /// Adam step.
     pub fn vgi_mlp_train(tracer: *mut vgi_tracer, queries: *const u32, targets: *const f32, count: u32, update: c_int, grad_out: *mut f32, loss_out: *mut f32) -> c_int;
 
-    /// The hash grid: read (out) and/or write (in) its 4,194,304 FP32 entries
-    /// (level-major, 2^18 entries of 2 features per level), and copy the table
+    /// The hash grid: read (out) and/or write (in) its 67,108,864 FP32 entries
+    /// (level-major, 2^22 entries of 2 features per level), and copy the table
     /// gradient of the last vgi_mlp_train without update (grad_out).
     pub fn vgi_mlp_table(tracer: *mut vgi_tracer, out: *mut f32, in_: *const f32, grad_out: *mut f32) -> c_int;
 

@@ -310,8 +310,8 @@ int vgi_mlp_infer(vgi_tracer *tracer, const uint32_t *queries, uint32_t count, f
  * Adam step. */
 int vgi_mlp_train(vgi_tracer *tracer, const uint32_t *queries, const float *targets, uint32_t count, int update,
                   float *grad_out, float *loss_out);
-/* The hash grid: read (out) and/or write (in) its 4,194,304 FP32 entries
- * (level-major, 2^18 entries of 2 features per level), and copy the table
+/* The hash grid: read (out) and/or write (in) its 67,108,864 FP32 entries
+ * (level-major, 2^22 entries of 2 features per level), and copy the table
  * gradient of the last vgi_mlp_train without update (grad_out). */
 int vgi_mlp_table(vgi_tracer *tracer, float *out, const float *in, float *grad_out);
 /* The hash table's Adam learning rate (ten times the network's by default). */

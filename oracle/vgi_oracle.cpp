@@ -2207,10 +2207,10 @@ static const int MLP_OUTS[5] = {64, 64, 64, 64, 3};
 
 // The encoding's hash grid (Mueller et al. 2022): 8 levels of 2 features,
 // level l spanning the largest extent with floor(16 b^l) cells, b =
-// (extent / 16)^(1/7); a level whose (res + 2)^3 corners fit its 2^18 entries
-// indexes them densely, finer ones by x ^ 2654435761 y ^ 805459861 z mod 2^18.
+// (extent / 16)^(1/7); a level whose (res + 2)^3 corners fit its 2^22 entries
+// indexes them densely, finer ones by x ^ 2654435761 y ^ 805459861 z mod 2^22.
 static const int HASH_LEVELS = 8;
-static const uint32_t HASH_SIZE = 1u << 18;
+static const uint32_t HASH_SIZE = 1u << 22;
 static const size_t HASH_TABLE = (size_t)HASH_LEVELS * HASH_SIZE * 2;
 
 struct MlpGrid {

@@ -443,7 +443,7 @@ def select_threshold(histograms, picks):
     return thr, rem
 
 
-MLP_TABLE = 8 * (1 << 18) * 2
+MLP_TABLE = 8 * (1 << 22) * 2
 
 
 def mlp_forward(weights, table, queries, diffuse, scene):

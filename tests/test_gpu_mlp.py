@@ -91,7 +91,7 @@ def test_mlp_gradient_matches_fp64_oracle(vgi, orc, encoding):
         print("layer [%d,%d): relative error %.3g cosine %.6f" % (a, b, e, cos), flush=True)
         assert e < 2e-2 and cos > 0.9995
     # The hash table's gradient, per level (none with the frequency encoding).
-    size = 1 << 19
+    size = 1 << 23  # entries x features per level
     if encoding == 1:
         assert not table_grad.any() and not ref_table.any()
     for level in range(8 if encoding == 0 else 0):

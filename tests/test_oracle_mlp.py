@@ -17,7 +17,7 @@ import pytest
 
 import oracle_lib as orc
 
-SIZE = 1 << 18
+SIZE = 1 << 22
 
 
 def scene_for(extent):
@@ -83,7 +83,7 @@ def test_trilinear_interpolation_reproduces_linear_fields():
     table = np.zeros(orc.MLP_TABLE, np.float32)
     planes = [(0.25, 0.01, -0.02, 0.005), (-0.1, 0.003, 0.002, -0.004)]
     dense = [l for l in range(8) if (int(res[l]) + 2) ** 3 <= SIZE]
-    assert dense == [0, 1, 2]
+    assert dense == [0, 1, 2, 3, 4]
     for level in dense:
         side = int(res[level]) + 2
         z, y, x = np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij")
