@@ -68,6 +68,36 @@ class Schedule(ctypes.Structure):
     ]
 
 
+class ViewCamera(ctypes.Structure):
+    """vgi_view_camera: world_from_clip column-major (element (r, c) at 4c + r)."""
+    _fields_ = [
+        ("world_from_clip", ctypes.c_float * 16),
+        ("position", ctypes.c_float * 3),
+        ("inner_distance", ctypes.c_float),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("samples", ctypes.c_uint32),
+        ("measured_retention", ctypes.c_float),
+    ]
+
+
+def view_camera(world_from_clip, position, width, height, samples=1, measured_retention=1.0, inner_distance=0.0):
+    """A ViewCamera from a 4x4 row-major numpy matrix (clip -> world)."""
+    c = ViewCamera()
+    m = np.asarray(world_from_clip, np.float32)
+    for col in range(4):
+        for row in range(4):
+            c.world_from_clip[col * 4 + row] = float(m[row, col])
+    for i in range(3):
+        c.position[i] = float(position[i])
+    c.inner_distance = inner_distance
+    c.width, c.height, c.samples, c.measured_retention = width, height, samples, measured_retention
+    return c
+
+
+VIEW_FACE = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("surface", "<f4", (3,)), ("sequence", "<u4"), ("mean", "<f4", (4,)),
+                      ("moment", "<f4", (4,)), ("identity", "<u4", (4,)), ("direct", "<f4", (4,))])
+
 SEL_THRESHOLD, SEL_REMAINDER, SEL_TIES, SEL_TOTAL, SEL_MIRRORS, SEL_WORDS = 1024, 1028, 1032, 1036, 1037, 1038
 
 _lib = None
@@ -128,6 +158,10 @@ def library():
         "vgi_set_schedule": (ctypes.c_int, [P, ctypes.POINTER(Schedule)]),
         "vgi_set_footprints": (ctypes.c_int, [P, ctypes.c_uint32]),
         "vgi_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32]),
+        "vgi_view_frame": (ctypes.c_int, [P, P, P, P]),
+        "vgi_view_reset": (ctypes.c_int, [P]),
+        "vgi_view_read_owners": (i64, [P, P, ctypes.c_uint32, P]),
+        "vgi_view_read_faces": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_read_sun_flips": (ctypes.c_int64, [P, P, ctypes.c_uint32]),
         "vgi_read_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_sun_timings": (ctypes.c_int, [P, P, ctypes.c_uint32]),
@@ -181,6 +215,7 @@ def exported_symbols():
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
+        "vgi_view_frame", "vgi_view_reset", "vgi_view_read_owners", "vgi_view_read_faces",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -526,6 +561,30 @@ class GpuTracer:
 
     def set_footprints(self, on):
         _check(self._lib.vgi_set_footprints(self._h, 1 if on else 0))
+
+    # ---- view tier
+    def view_frame(self, camera, depth, normals):
+        """One view frame: camera (ViewCamera), depth [h, w] float32, normals [h, w, 3] float32."""
+        d = np.ascontiguousarray(depth, np.float32)
+        n = np.ascontiguousarray(normals, np.float32)
+        if d.size != camera.width * camera.height or n.size != 3 * d.size:
+            raise VoxelGiError("prepass buffers do not match the camera's size")
+        _check(self._lib.vgi_view_frame(self._h, ctypes.byref(camera), _ptr(d), _ptr(n)))
+
+    def view_reset(self):
+        _check(self._lib.vgi_view_reset(self._h))
+
+    def view_owners(self, pixels):
+        """(owner per pixel, owner count, hash overflow count) of the last view frame."""
+        out = np.zeros(pixels, np.uint32)
+        overflow = np.zeros(1, np.uint32)
+        n = _check(self._lib.vgi_view_read_owners(self._h, _ptr(out), pixels, _ptr(overflow)))
+        return out, int(n), int(overflow[0])
+
+    def view_faces(self, first, count):
+        out = np.zeros(count, VIEW_FACE)
+        _check(self._lib.vgi_view_read_faces(self._h, first, count, _ptr(out)))
+        return out
 
     # ---- sun corner words
     def sun_corners(self, frame=0, refresh_frames=1):

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-I" + INCLUDE, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"]
 
-HIP_SOURCES = ["vgi_kernels.hip", "vgi_wavefront.hip", "vgi_select.hip", "vgi_sun.hip", "vgi_mlp.hip", "vgi_host.cpp"]
+HIP_SOURCES = ["vgi_kernels.hip", "vgi_wavefront.hip", "vgi_select.hip", "vgi_sun.hip", "vgi_mlp.hip", "vgi_view.hip", "vgi_host.cpp"]
 HOST_SOURCES = ["vgi_host_scene.cpp"]
 # The seeded workload generators of the tests and the benchmark: a library of
 # their own, outside the product's boundary.

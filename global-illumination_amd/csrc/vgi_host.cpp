@@ -19,6 +19,7 @@
 #include "vgi_mlp.h"
 #include "vgi_wavefront.h"
 #include "vgi_trace.h"
+#include "vgi_view.h"
 
 #include <cmath>
 #include <numeric>
@@ -220,6 +221,14 @@ struct vgi_tracer {
     // ---- learned radiance cache
     bool mlp_ready = false, learned = false;
     uint32_t mlp_encoding = 0u;  // vgi::MLP_ENCODING_*
+    uint32_t incarnation = 0;    // the last slot allocation's incarnation (state word 28)
+    // ---- view tier: two generations of per-pixel receivers, estimates,
+    // owners and face hash; `view_next` is the generation the next frame
+    // writes, the other holds the last frame.
+    DeviceBuffer v_depth, v_normals, v_queue, v_receivers[2], v_estimates[2], v_owners[2], v_hash[2];
+    uint32_t view_next = 0;
+    uint32_t view_pixels[2] = {0u, 0u}, view_mask[2] = {0u, 0u};
+    bool view_history = false;   // the last frame's generation may be reprojected
     DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
     DeviceBuffer d_table, d_table_m, d_table_v, d_table_g;  // hash-grid encoding (FP32, Adam)
     std::vector<float> table_grad_last;                      // of the last vgi_mlp_train without update
@@ -255,7 +264,7 @@ struct vgi_tracer {
     std::vector<uint32_t> bound_slots;
     std::vector<uint32_t> bound_of_slot;  // slot -> bound index or NONE
     std::vector<uint16_t> slot_material;  // material in each slot's identity word
-    DeviceBuffer d_bound_index, d_bound_upd, d_bound_slot_upd, d_meta_writes;
+    DeviceBuffer d_bound_index, d_bound_upd, d_bound_slot_upd, d_meta_writes, d_seed_writes;
 };
 
 namespace {
@@ -1037,6 +1046,18 @@ int register_faces(vgi_tracer &t, const vgi_face_task *tasks, uint32_t count, st
         HIP_OK(hipMemsetAsync(t.d_checkpoint.as<uint32_t>() + (size_t)lo * 12, 0, n * 48, t.stream));
         HIP_OK(hipMemsetAsync(t.d_corners.as<uint32_t>() + lo, 0, n * 4, t.stream));
         if (staged_upload(t, t.d_meta.as<uint32_t>() + (size_t)lo * 12, meta.data(), n * 48) != 0) return -1;
+        // Record 7's x: each allocation's incarnation (resident.rs:457-462),
+        // in allocation order, which the view tier's exchange admits by.
+        std::vector<uint32_t> seeds;
+        seeds.reserve(n * 2);
+        for (size_t k = 0; k < identities.size(); k += 5) {
+            seeds.push_back(identities[k] * 36u + 28u);
+            seeds.push_back(++t.incarnation);
+        }
+        HIP_OK(t.d_seed_writes.reserve(seeds.size() * 4));
+        if (staged_upload(t, t.d_seed_writes.ptr, seeds.data(), seeds.size() * 4) != 0) return -1;
+        vgi::launch_scatter_words(t.d_seed_writes.as<uint2>(), (uint32_t)(seeds.size() / 2), t.d_state.as<uint32_t>(), t.stream);
+        HIP_OK(hipGetLastError());
     }
     (void)first_new;
     if (!meta_writes.empty()) {
@@ -1643,7 +1664,9 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->b_pred, &t->d_perm, &t->d_face_material, &t->d_train_tasks, &t->d_train_q,
                             &t->d_train_results, &t->d_targets, &t->d_acts, &t->d_grads, &t->d_train_pred, &t->d_io, &t->d_touched,
                             &t->d_events, &t->d_measures, &t->d_measure_max, &t->d_material_light, &t->d_bound_index,
-                            &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
+                            &t->d_bound_upd, &t->d_bound_slot_upd, &t->d_meta_writes, &t->d_seed_writes, &t->v_depth, &t->v_normals, &t->v_queue,
+                            &t->v_receivers[0], &t->v_receivers[1], &t->v_estimates[0], &t->v_estimates[1], &t->v_owners[0],
+                            &t->v_owners[1], &t->v_hash[0], &t->v_hash[1], &t->wf_queue, &t->wf_hit0, &t->wf_hit1,
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
                             &t->d_pick_results, &t->d_mirror, &t->d_select_ranges,
                             &t->d_footprint_tasks, &t->d_footprint_sums, &t->d_corners, &t->d_sun_flux, &t->d_sun_queue,
@@ -2460,6 +2483,117 @@ int vgi_read_sun_corners(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t
     HIP_OK(hipSetDevice(t->device));
     HIP_OK(hipStreamSynchronize(t->stream));
     if (count) HIP_OK(hipMemcpy(out, t->d_corners.as<uint32_t>() + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------- view tier
+// render_view's passes for one camera (gpu/view/runtime.rs:1254-1266):
+// discover, index, reproject, load_shared_faces, trace_visible_paths,
+// store_shared_faces, on the tracer's stream after its earlier work.
+int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *depth, const float *normals) {
+    if (!t || !camera || !depth || !normals) return fail("null argument");
+    if (camera->width == 0 || camera->height == 0) return fail("empty prepass");
+    const uint64_t pixels64 = (uint64_t)camera->width * camera->height;
+    if (pixels64 > VGI_VIEW_MAX_PIXELS) return fail("prepass larger than VGI_VIEW_MAX_PIXELS");
+    if (camera->samples > 1024u) return fail("more than 1024 paths per face per view frame");
+    if (!t->have_regions) return fail("no scene loaded");
+    HIP_OK(hipSetDevice(t->device));
+    if (sync_emitters(*t) != 0) return -1;
+    const uint32_t pixels = (uint32_t)pixels64;
+    uint32_t capacity = 64;
+    while (capacity < 2u * pixels) capacity <<= 1;
+    const uint32_t g = t->view_next, pg = g ^ 1u;
+    HIP_OK(t->v_depth.reserve((size_t)pixels * 4));
+    HIP_OK(t->v_normals.reserve((size_t)pixels * 12));
+    HIP_OK(t->v_queue.reserve(((size_t)pixels + 2) * 4));
+    HIP_OK(t->v_receivers[g].reserve((size_t)pixels * sizeof(vgi::ViewReceiver)));
+    HIP_OK(t->v_estimates[g].reserve((size_t)pixels * sizeof(vgi::ViewEstimate)));
+    HIP_OK(t->v_owners[g].reserve((size_t)pixels * 4));
+    HIP_OK(t->v_hash[g].reserve((size_t)capacity * 4));
+    if (staged_upload(*t, t->v_depth.ptr, depth, (size_t)pixels * 4) != 0) return -1;
+    if (staged_upload(*t, t->v_normals.ptr, normals, (size_t)pixels * 12) != 0) return -1;
+    vgi::launch_fill_words(t->v_hash[g].as<uint32_t>(), capacity, 0u, t->stream);
+    vgi::launch_fill_words(t->v_queue.as<uint32_t>(), 2, 0u, t->stream);
+    vgi::SceneView s;
+    vgi::Optics o;
+    views(*t, s, o);
+    vgi::ViewParams p;
+    std::memcpy(p.m, camera->world_from_clip, sizeof(p.m));
+    p.cam[0] = camera->position[0]; p.cam[1] = camera->position[1]; p.cam[2] = camera->position[2];
+    p.cam[3] = camera->inner_distance;
+    p.width = camera->width; p.height = camera->height; p.pixels = pixels;
+    p.mask = capacity - 1u;
+    const bool history = t->view_history && t->view_pixels[pg] > 0u;
+    p.prev_pixels = history ? t->view_pixels[pg] : 0u;
+    p.prev_mask = history ? t->view_mask[pg] : 0u;
+    p.samples = camera->samples;
+    p.retention = std::max(camera->measured_retention, 0.0f);
+    p.clock_ms = clock_ms(*t);
+    p.exchange = t->resident && t->high_water > 0 ? 1u : 0u;
+    vgi::SharedCache c;
+    c.index = t->d_index.as<uint4>();
+    c.state = t->d_state.as<uint32_t>();
+    c.meta = t->d_meta.as<uint32_t>();
+    c.mask = t->resident ? (uint32_t)(t->face_slots.entries.size() - 1) : 0u;
+    vgi::launch_view_frame(p, s, o, c, t->v_depth.as<float>(), t->v_normals.as<float>(), t->v_receivers[g].as<vgi::ViewReceiver>(),
+                           t->v_estimates[g].as<vgi::ViewEstimate>(), t->v_hash[g].as<uint32_t>(), t->v_owners[g].as<uint32_t>(),
+                           history ? t->v_receivers[pg].as<vgi::ViewReceiver>() : nullptr,
+                           history ? t->v_estimates[pg].as<vgi::ViewEstimate>() : nullptr,
+                           history ? t->v_hash[pg].as<uint32_t>() : nullptr, t->v_queue.as<uint32_t>(), t->stream);
+    HIP_OK(hipGetLastError());
+    t->view_pixels[g] = pixels;
+    t->view_mask[g] = capacity - 1u;
+    t->view_next = pg;
+    t->view_history = true;
+    return 0;
+}
+
+int vgi_view_reset(vgi_tracer *t) {
+    if (!t) return fail("null tracer");
+    t->view_history = false;
+    return 0;
+}
+
+// The last view frame's generation, or -1 when there is none.
+static int view_last(const vgi_tracer &t) {
+    uint32_t g = t.view_next ^ 1u;
+    return t.view_history && t.view_pixels[g] > 0u ? (int)g : -1;
+}
+
+int64_t vgi_view_read_owners(vgi_tracer *t, uint32_t *out, uint32_t pixels, uint32_t *overflow) {
+    if (!t) return fail("null tracer");
+    int g = view_last(*t);
+    if (g < 0) return fail("no view frame has run");
+    if (pixels && !out) return fail("null argument");
+    if (pixels > t->view_pixels[g]) return fail("more pixels than the last view frame's prepass");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    uint32_t counts[2];
+    HIP_OK(hipMemcpy(counts, t->v_queue.ptr, 8, hipMemcpyDeviceToHost));
+    if (pixels) HIP_OK(hipMemcpy(out, t->v_owners[g].ptr, (size_t)pixels * 4, hipMemcpyDeviceToHost));
+    if (overflow) *overflow = counts[1];
+    return (int64_t)counts[0];
+}
+
+int vgi_view_read_faces(vgi_tracer *t, uint32_t first, uint32_t count, vgi_view_face *out) {
+    if (!t || (count && !out)) return fail("null argument");
+    int g = view_last(*t);
+    if (g < 0) return fail("no view frame has run");
+    if ((uint64_t)first + count > t->view_pixels[g]) return fail("pixel range beyond the last view frame's prepass");
+    if (count == 0) return 0;
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    std::vector<vgi::ViewReceiver> r(count);
+    std::vector<vgi::ViewEstimate> e(count);
+    HIP_OK(hipMemcpy(r.data(), t->v_receivers[g].as<vgi::ViewReceiver>() + first, (size_t)count * sizeof(vgi::ViewReceiver),
+                     hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(e.data(), t->v_estimates[g].as<vgi::ViewEstimate>() + first, (size_t)count * sizeof(vgi::ViewEstimate),
+                     hipMemcpyDeviceToHost));
+    static_assert(sizeof(vgi_view_face) == sizeof(vgi::ViewReceiver) + sizeof(vgi::ViewEstimate), "view face record layout");
+    for (uint32_t i = 0; i < count; ++i) {
+        std::memcpy(&out[i], &r[i], sizeof(vgi::ViewReceiver));
+        std::memcpy(reinterpret_cast<char *>(&out[i]) + sizeof(vgi::ViewReceiver), &e[i], sizeof(vgi::ViewEstimate));
+    }
     return 0;
 }
 

@@ -108,6 +108,23 @@ __device__ __forceinline__ uint4 region_header(const SceneView &s, int rx, int r
     return s.headers[i];
 }
 
+// voxel_traversal.wgsl:48-70 (voxel_material / voxel_cell): a voxel's
+// material, AIR for air and for voxels outside the loaded regions.
+__device__ __forceinline__ uint32_t voxel_material(const SceneView &s, int vx, int vy, int vz) {
+    int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
+    if (!in_bounds(s, rx, ry, rz)) return AIR;
+    uint4 h = region_header(s, rx, ry, rz);
+    if (h.x == 0u || h.x == 1u) return AIR;
+    if (h.x != 3u) return h.y;
+    uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
+    uint32_t word = s.cells[h.y + ((lz >> 2) * 4u + (ly >> 2)) * 4u + (lx >> 2)];
+    if ((word & 3u) == 0u) return AIR;
+    if ((word & 3u) == 1u) return word >> 2u;
+    uint32_t idx = ((lz & 3u) * 4u + (ly & 3u)) * 4u + (lx & 3u);
+    uint32_t cell = (s.cells[(word >> 2u) + (idx >> 1)] >> ((idx & 1u) * 16u)) & 65535u;
+    return (cell & 1u) != 0u ? cell >> 1u : AIR;
+}
+
 // n / d rounded to binary32 as IEEE division rounds it, from r = RN64(1/d)
 // (binary64, correctly rounded, taken once per ray): RN32(RN64(n * r)).
 // Why it is exact: RN64(n * r) = (n/d)(1 + e) with |e| <= 2^-52 (1 + 2^-53).

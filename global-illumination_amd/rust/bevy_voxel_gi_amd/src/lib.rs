@@ -22,7 +22,7 @@ use std::ptr::{self, NonNull};
 pub use vgi_sys as sys;
 pub use vgi_sys::{
     vgi_face_result as FaceResult, vgi_face_task as FaceTask, vgi_light as Light, vgi_schedule as Schedule,
-    vgi_transport as Transport, VGI_AIR as AIR, VGI_COUNT_GATHERS as COUNT_GATHERS, VGI_COUNT_RAYS as COUNT_RAYS,
+    vgi_transport as Transport, vgi_view_camera as ViewCamera, vgi_view_face as ViewFace, VGI_AIR as AIR, VGI_COUNT_GATHERS as COUNT_GATHERS, VGI_COUNT_RAYS as COUNT_RAYS,
     VGI_NOALLOC_MAX_FACES as NOALLOC_MAX_FACES, VGI_NO_FACE as NO_FACE, VGI_PAGE_EDGE as PAGE_EDGE,
     VGI_RECORD_HITS as RECORD_HITS,
 };
@@ -413,6 +413,37 @@ impl GpuTracer {
     pub fn read_sun_corners(&self, first: u32, out: &mut [u32]) -> Result<()> {
         let n = len_u32(out.len(), "read_sun_corners")?;
         check(unsafe { sys::vgi_read_sun_corners(self.raw(), first, n, out.as_mut_ptr()) })
+    }
+
+    // ---- view tier (render_view, gpu/view/runtime.rs:749) ----
+
+    /// One camera's view frame over its depth and normal prepass (row-major,
+    /// width x height depths and 3 x width x height normal components).
+    pub fn view_frame(&mut self, camera: &ViewCamera, depth: &[f32], normals: &[f32]) -> Result<()> {
+        let pixels = camera.width as usize * camera.height as usize;
+        if depth.len() != pixels || normals.len() != 3 * pixels {
+            return Err(format!("view_frame: {} depths and {} normal components for {pixels} pixels", depth.len(), normals.len()));
+        }
+        check(unsafe { sys::vgi_view_frame(self.raw(), camera, depth.as_ptr(), normals.as_ptr()) })
+    }
+
+    /// Forget the view history.
+    pub fn view_reset(&mut self) -> Result<()> {
+        check(unsafe { sys::vgi_view_reset(self.raw()) })
+    }
+
+    /// The last view frame's owner per pixel; returns (faces, hash overflows).
+    pub fn view_owners(&self, out: &mut [u32]) -> Result<(usize, u32)> {
+        let n = len_u32(out.len(), "view_owners")?;
+        let mut overflow = 0u32;
+        let owners = check_count(unsafe { sys::vgi_view_read_owners(self.raw(), out.as_mut_ptr(), n, &mut overflow) })?;
+        Ok((owners, overflow))
+    }
+
+    /// Receivers and estimates of pixels first .. first + out.len() of the last view frame.
+    pub fn view_faces(&self, first: u32, out: &mut [ViewFace]) -> Result<()> {
+        let n = len_u32(out.len(), "view_faces")?;
+        check(unsafe { sys::vgi_view_read_faces(self.raw(), first, n, out.as_mut_ptr()) })
     }
 
     // ---- GPU work selection (voxel_accumulate.wgsl:448-695; gpu/resident.rs:1093-1164) ----

@@ -85,6 +85,54 @@ pub const VGI_COUNT_GATHERS: u32 = 4;
 
 pub const VGI_NOALLOC_MAX_FACES: u32 = 0x40000;
 
+/// view tier: camera-visible faces (voxel_view_discovery.wgsl:4-130,
+/// voxel_view_runtime.wgsl:94-324, voxel_view_cache.wgsl:1-258; render_view
+/// gpu/view/runtime.rs:749,1254-1266). A frame takes one camera's depth and
+/// normal prepass (host memory, row-major, x fastest), discovers each pixel's
+/// exposed voxel face (the decoded point with its normal axis snapped to the
+/// face plane; within one pixel's footprint of an in-plane voxel boundary the
+/// exposed neighbour), gives every face one owner -- the smallest pixel index
+/// among its pixels -- through a per-camera CAS face hash, carries each
+/// owner's estimate from the previous frame's owner of the same face (same
+/// material), exchanges it both ways with the resident world cache
+/// (load_shared_faces / store_shared_faces: an admitted face always loads the
+/// authority, a face entering the view loads it when the authority holds at
+/// least as many samples, and an admitted face stores its estimate back), and
+/// traces `samples` paths per owner with the world tier's estimator (running
+/// mean with weight 1 / min(count + 1, 65536), the measured-retention keep
+/// rule on a pending change) plus its direct term. Non-blocking, ordered after
+/// the tracer's earlier work. Not covered: coarse LOD blocks, the traced
+/// discovery fallback for foreign meshes (such pixels own no receiver),
+/// pending geometry, corner sun words and sun flips of view faces, topology
+/// priors and presentation (shown), the far per-pixel tier.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default, PartialEq)]
+pub struct vgi_view_camera {
+    pub world_from_clip: [f32; 16],
+    pub position: [f32; 3],
+    pub inner_distance: f32,
+    pub width: u32,
+    pub height: u32,
+    pub samples: u32,
+    pub measured_retention: f32,
+}
+
+/// One prepass pixel's receiver and, for an owner, its face's estimate.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default, PartialEq)]
+pub struct vgi_view_face {
+    pub voxel: [i32; 3],
+    pub face: u32,
+    pub surface: [f32; 3],
+    pub sequence: u32,
+    pub mean: [f32; 4],
+    pub moment: [f32; 4],
+    pub identity: [u32; 4],
+    pub direct: [f32; 4],
+}
+
+pub const VGI_VIEW_MAX_PIXELS: u32 = 0x1000000;
+
 /// Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
 /// stable_samples, audit_ms, mirror_budget}, work.rs:38-50; the plugin fills
 /// them from settings.rs:337-357). Every resident batch's accumulate ranks its
@@ -294,6 +342,19 @@ extern "C" {
     /// the view tier keeps the same word per receiver, estimate.identity.z,
     /// voxel_view_runtime.wgsl:175-198).
     pub fn vgi_read_sun_corners(tracer: *mut vgi_tracer, first: u32, count: u32, out: *mut u32) -> c_int;
+
+    pub fn vgi_view_frame(tracer: *mut vgi_tracer, camera: *const vgi_view_camera, depth: *const f32, normals: *const f32) -> c_int;
+
+    /// Forget the view history (the next frame reprojects nothing).
+    pub fn vgi_view_reset(tracer: *mut vgi_tracer) -> c_int;
+
+    /// The last view frame's owner per pixel (pixel index of the face's owner,
+    /// 0xffffffff: none) for the first `pixels` pixels; returns the number of
+    /// owners (faces); *overflow (optional) counts faces the hash could not place.
+    pub fn vgi_view_read_owners(tracer: *mut vgi_tracer, out: *mut u32, pixels: u32, overflow: *mut u32) -> i64;
+
+    /// Receivers and estimates of pixels [first, first + count) of the last view frame.
+    pub fn vgi_view_read_faces(tracer: *mut vgi_tracer, first: u32, count: u32, out: *mut vgi_view_face) -> c_int;
 
     pub fn vgi_set_schedule(tracer: *mut vgi_tracer, schedule: *const vgi_schedule) -> c_int;
 

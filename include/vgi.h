@@ -239,6 +239,57 @@ int vgi_sun_timings(vgi_tracer *tracer, float *out, uint32_t n);
  * the view tier keeps the same word per receiver, estimate.identity.z,
  * voxel_view_runtime.wgsl:175-198). */
 int vgi_read_sun_corners(vgi_tracer *tracer, uint32_t first, uint32_t count, uint32_t *out);
+/* ---- view tier: camera-visible faces (voxel_view_discovery.wgsl:4-130,
+ * voxel_view_runtime.wgsl:94-324, voxel_view_cache.wgsl:1-258; render_view
+ * gpu/view/runtime.rs:749,1254-1266). A frame takes one camera's depth and
+ * normal prepass (host memory, row-major, x fastest), discovers each pixel's
+ * exposed voxel face (the decoded point with its normal axis snapped to the
+ * face plane; within one pixel's footprint of an in-plane voxel boundary the
+ * exposed neighbour), gives every face one owner -- the smallest pixel index
+ * among its pixels -- through a per-camera CAS face hash, carries each
+ * owner's estimate from the previous frame's owner of the same face (same
+ * material), exchanges it both ways with the resident world cache
+ * (load_shared_faces / store_shared_faces: an admitted face always loads the
+ * authority, a face entering the view loads it when the authority holds at
+ * least as many samples, and an admitted face stores its estimate back), and
+ * traces `samples` paths per owner with the world tier's estimator (running
+ * mean with weight 1 / min(count + 1, 65536), the measured-retention keep
+ * rule on a pending change) plus its direct term. Non-blocking, ordered after
+ * the tracer's earlier work. Not covered: coarse LOD blocks, the traced
+ * discovery fallback for foreign meshes (such pixels own no receiver),
+ * pending geometry, corner sun words and sun flips of view faces, topology
+ * priors and presentation (shown), the far per-pixel tier. ---- */
+typedef struct vgi_view_camera {
+    float world_from_clip[16]; /* column-major: element (row r, column c) at [4 c + r]; clip y up, depth z > 0 valid */
+    float position[3];         /* camera position (world) */
+    float inner_distance;      /* points closer than this own no receiver (0: none) */
+    uint32_t width;            /* prepass pixels per row */
+    uint32_t height;           /* prepass rows */
+    uint32_t samples;          /* paths per visible face this frame (<= 1024) */
+    float measured_retention;  /* the keep rule's c (settings.rs:262) */
+} vgi_view_camera;
+/* One prepass pixel's receiver and, for an owner, its face's estimate. */
+typedef struct vgi_view_face {
+    int32_t voxel[3];
+    uint32_t face;             /* 0xffffffff: the pixel has no receiver */
+    float surface[3];          /* face centre */
+    uint32_t sequence;         /* next path sequence */
+    float mean[4];             /* raw incident rgb, effective sample weight */
+    float moment[4];           /* raw second moment rgb, pending predicted change */
+    uint32_t identity[4];      /* next sequence, material, corner word (0), resident incarnation (0: not admitted) */
+    float direct[4];           /* direct rgb at the face centre, 0 */
+} vgi_view_face;
+#define VGI_VIEW_MAX_PIXELS 16777216u
+int vgi_view_frame(vgi_tracer *tracer, const vgi_view_camera *camera, const float *depth, const float *normals);
+/* Forget the view history (the next frame reprojects nothing). */
+int vgi_view_reset(vgi_tracer *tracer);
+/* The last view frame's owner per pixel (pixel index of the face's owner,
+ * 0xffffffff: none) for the first `pixels` pixels; returns the number of
+ * owners (faces); *overflow (optional) counts faces the hash could not place. */
+int64_t vgi_view_read_owners(vgi_tracer *tracer, uint32_t *out, uint32_t pixels, uint32_t *overflow);
+/* Receivers and estimates of pixels [first, first + count) of the last view frame. */
+int vgi_view_read_faces(vgi_tracer *tracer, uint32_t first, uint32_t count, vgi_view_face *out);
+
 /* ---- GPU work selection (voxel_accumulate.wgsl:448-695; resident.rs:1093-1164) ---- */
 /* Scheduling settings (TraceWork.{focus, transport_revision, stable_error,
  * stable_samples, audit_ms, mirror_budget}, work.rs:38-50; the plugin fills

@@ -1048,6 +1048,17 @@ struct FaceIndex {  // gpu/face_index.rs:9-188
     }
 };
 
+// One prepass pixel of the view tier: the vgi_view_face record.
+struct ViewFace {
+    int32_t voxel[3];
+    uint32_t face;
+    float surface[3];
+    uint32_t sequence;
+    float mean[4], moment[4];
+    uint32_t identity[4];
+    float direct[4];
+};
+
 struct Tracer {
     Scene *scene;
     Packed packed;
@@ -1068,6 +1079,7 @@ struct Tracer {
     std::vector<uint32_t> meta;        // 12 words per slot
     std::vector<uint32_t> free_slots;
     uint32_t high_water = 0;
+    uint32_t incarnation = 0;  // the last slot allocation's incarnation
     uint32_t history_limit = 65536;
     float recency = 1.0f;
     float measured_retention = 1.0f;
@@ -1089,6 +1101,10 @@ struct Tracer {
     std::map<std::tuple<int, int, int>, std::pair<std::array<int, 6>, float>> sun_buckets;
     bool sun_published = false;
     uint32_t sun_published_ms = 0;
+    // View tier: the last frame's per-pixel records and each face's owner.
+    std::vector<ViewFace> view_faces;
+    std::map<std::tuple<int, int, int, uint32_t>, uint32_t> view_owner;
+    bool view_history = false;
 };
 
 // ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
@@ -1745,6 +1761,9 @@ static uint32_t allocate_slot(Tracer &t, I3 voxel, uint32_t face, uint32_t mater
     // statistics, sequence 0, no presentation yet.
     memset(&t.state[(size_t)index * 36], 0, 36 * 4);
     memset(&t.checkpoint[(size_t)index * 12], 0, 12 * 4);
+    // Record 7's x: the allocation's incarnation (resident.rs:457-462, a
+    // counter from 1), which the view tier's exchange checks its admission by.
+    t.state[(size_t)index * 36 + 28] = ++t.incarnation;
     return index;
 }
 
@@ -3567,6 +3586,246 @@ int orc_sun_take(void *tp, uint32_t now_ms, uint32_t *out, uint32_t capacity) {
 
 // One trace_domain_limited call (voxel_traversal.wgsl:126-219, visible_geometry
 // false, 2048 steps): out_i = voxel xyz, material, status, face; out_f = position, normal.
+// ---------------------------------------------------------------- view tier
+// The view tier's frame (voxel_view_discovery.wgsl:4-130; index_receivers,
+// reproject_paths, trace_visible_paths voxel_view_runtime.wgsl:128-324;
+// load_shared_faces / store_shared_faces voxel_view_cache.wgsl:1-99,188-258)
+// for unit voxels, every path completed in its visit, in the GPU build's
+// arithmetic order.
+static uint32_t view_material(const Tracer &t, I3 v) {
+    uint32_t m = t.scene->material_at(v);
+    return m == 0xffffu ? AIR : m;
+}
+
+static F3 view_unproject(const orc_view_camera &c, float px, float py, float z) {
+    float ux = (px + 0.5f) / (float)c.width, uy = (py + 0.5f) / (float)c.height;
+    float x = ux * 2.0f - 1.0f, y = 1.0f - uy * 2.0f;
+    const float *m = c.world_from_clip;
+    float h[4];
+    for (int i = 0; i < 4; ++i) h[i] = ((m[i] * x + m[4 + i] * y) + m[8 + i] * z) + m[12 + i];
+    return f3(h[0] / h[3], h[1] / h[3], h[2] / h[3]);
+}
+
+static float view_distance(F3 a, F3 b) {
+    F3 d = a - b;
+    return std::sqrt((d.x * d.x + d.y * d.y) + d.z * d.z);
+}
+
+// voxel_view_discovery.wgsl:4-11
+static uint32_t view_prepass_face(F3 n) {
+    if (fmax_(fmax_(std::fabs(n.x), std::fabs(n.y)), std::fabs(n.z)) < 0.99f) return NONE;
+    if (std::fabs(n.x) > 0.99f) return n.x < 0.0f ? 1u : 0u;
+    if (std::fabs(n.y) > 0.99f) return n.y < 0.0f ? 3u : 2u;
+    return n.z < 0.0f ? 5u : 4u;
+}
+
+// voxel_view_discovery.wgsl:214-226, unit voxels.
+static bool view_receiver_exists(const Tracer &t, I3 v, uint32_t face) {
+    if (face >= 6u || view_material(t, v) == AIR) return false;
+    I3 o = v;
+    int step = (face & 1u) == 0u ? 1 : -1;
+    if (face / 2u == 0u) o.x += step; else if (face / 2u == 1u) o.y += step; else o.z += step;
+    return view_material(t, o) == AIR;
+}
+
+// prepass_voxel_receiver (voxel_view_discovery.wgsl:40-104) without the
+// traced fallback and pending geometry: false when the pixel owns no face.
+static bool view_discover(const Tracer &t, const orc_view_camera &c, uint32_t i, float z, F3 normal_in, I3 &voxel, uint32_t &face) {
+    if (!(z > 0.0f)) return false;
+    float px = (float)(i % c.width), py = (float)(i / c.width);
+    F3 position = view_unproject(c, px, py, z);
+    if (c.inner_distance > 0.0f && view_distance(position, f3(c.position[0], c.position[1], c.position[2])) < c.inner_distance) return false;
+    face = view_prepass_face(normal_in);
+    if (face >= 6u) return false;
+    F3 n = face_normal(face);
+    uint32_t axis = face / 2u;
+    float sv[3] = {position.x, position.y, position.z};
+    sv[axis] = std::nearbyint(sv[axis]);
+    F3 surface = f3(sv[0], sv[1], sv[2]);
+    F3 inside = surface - n * 0.01f;
+    I3 v{(int32_t)std::floor(inside.x), (int32_t)std::floor(inside.y), (int32_t)std::floor(inside.z)};
+    if (view_receiver_exists(t, v, face)) { voxel = v; return true; }
+    float footprint = view_distance(view_unproject(c, px + 1.0f, py, z), position);
+    uint32_t u = (axis + 1u) % 3u, w = (axis + 2u) % 3u;
+    float lu = sv[u] - std::floor(sv[u]), lv = sv[w] - std::floor(sv[w]);
+    float gu = fmin_(lu, 1.0f - lu), gv = fmin_(lv, 1.0f - lv);
+    int su = lu < 0.5f ? -1 : 1, sw = lv < 0.5f ? -1 : 1;
+    float best_gap = 2.0f;
+    bool found = false;
+    for (uint32_t k = 1u; k < 4u; ++k) {
+        int off[3] = {0, 0, 0};
+        float needed = 0.0f;
+        if (k & 1u) { off[u] = su; needed = fmax_(needed, gu); }
+        if (k & 2u) { off[w] = sw; needed = fmax_(needed, gv); }
+        if (needed > footprint || needed >= best_gap) continue;
+        I3 nb{v.x + off[0], v.y + off[1], v.z + off[2]};
+        if (view_receiver_exists(t, nb, face)) { voxel = nb; best_gap = needed; found = true; }
+    }
+    return found;
+}
+
+// shared_slot (voxel_view_cache.wgsl:6-23)
+static uint32_t view_shared_slot(const Tracer &t, I3 v, uint32_t face, uint32_t material) {
+    long f = t.face_index.find(v, face);
+    if (f < 0) return NONE;
+    uint32_t slot = t.face_index.entries[f][3] & 0x1fffffffu;
+    uint32_t key = t.meta[(size_t)slot * 12 + 3];
+    if ((key & 8u) == 0u || (key >> 16) != material) return NONE;
+    return slot;
+}
+
+int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, const float *normals, uint32_t clock_ms, uint32_t *owners_out,
+                   void *faces_out, uint32_t *owner_count, int threads) {
+    Tracer &t = *(Tracer *)tp;
+    tracer_sync(t);
+    const orc_view_camera &c = *cam;
+    const uint32_t pixels = c.width * c.height;
+    std::vector<ViewFace> cur(pixels);
+    std::vector<uint32_t> owners(pixels, NONE);
+    std::map<std::tuple<int, int, int, uint32_t>, uint32_t> owner;
+    for (uint32_t i = 0; i < pixels; ++i) {
+        ViewFace &r = cur[i];
+        memset(&r, 0, sizeof(r));
+        r.face = NONE;
+        I3 v;
+        uint32_t face;
+        if (!view_discover(t, c, i, depth[i], f3(normals[i * 3], normals[i * 3 + 1], normals[i * 3 + 2]), v, face)) continue;
+        F3 n = face_normal(face);
+        F3 centre = to_f3(v) + f3(0.5f, 0.5f, 0.5f) + n * 0.5f;
+        r.voxel[0] = v.x; r.voxel[1] = v.y; r.voxel[2] = v.z;
+        r.face = face;
+        r.surface[0] = centre.x; r.surface[1] = centre.y; r.surface[2] = centre.z;
+        r.identity[1] = view_material(t, v);
+        // The owner is the smallest pixel index of the face (the GPU's hash
+        // lowers every duplicate's entry to it).
+        owner.emplace(std::make_tuple(v.x, v.y, v.z, face), i);
+    }
+    std::vector<uint32_t> list;
+    for (uint32_t i = 0; i < pixels; ++i) {
+        if (cur[i].face >= 6u) continue;
+        owners[i] = owner.at(std::make_tuple(cur[i].voxel[0], cur[i].voxel[1], cur[i].voxel[2], cur[i].face));
+        if (owners[i] == i) list.push_back(i);
+    }
+    const bool exchange = t.cache_on && t.high_water > 0;
+    for (uint32_t i : list) {
+        ViewFace &r = cur[i];
+        const uint32_t material = r.identity[1];
+        // reproject_paths: the previous owner's estimate, same material.
+        float mean[4] = {0, 0, 0, 0}, moment[4] = {0, 0, 0, 0}, direct[4] = {0, 0, 0, 0};
+        uint32_t identity[4] = {0, 0, 0, 0};
+        if (t.view_history) {
+            auto it = t.view_owner.find(std::make_tuple(r.voxel[0], r.voxel[1], r.voxel[2], r.face));
+            if (it != t.view_owner.end() && t.view_faces[it->second].identity[1] == material) {
+                const ViewFace &old = t.view_faces[it->second];
+                memcpy(mean, old.mean, 16); memcpy(moment, old.moment, 16); memcpy(identity, old.identity, 16); memcpy(direct, old.direct, 16);
+            }
+        }
+        identity[1] = material;
+        identity[2] = 0u;
+        memcpy(r.mean, mean, 16); memcpy(r.moment, moment, 16); memcpy(r.identity, identity, 16); memcpy(r.direct, direct, 16);
+        r.sequence = r.identity[0];
+        if (!exchange) continue;
+        // load_shared_face
+        I3 v{r.voxel[0], r.voxel[1], r.voxel[2]};
+        uint32_t slot = view_shared_slot(t, v, r.face, material);
+        if (slot == NONE) { r.identity[3] = 0u; continue; }
+        uint32_t *st = &t.state[(size_t)slot * 36];
+        const uint32_t incarnation = st[28], count = st[5] & 0x7fffffffu, next = st[6];
+        if (r.identity[3] == incarnation || r.mean[3] <= (float)count) {
+            r.mean[0] = bitsf(st[0]); r.mean[1] = bitsf(st[1]); r.mean[2] = bitsf(st[2]); r.mean[3] = (float)count;
+            r.moment[0] = bitsf(st[8]); r.moment[1] = bitsf(st[9]); r.moment[2] = bitsf(st[10]);
+            r.moment[3] = bitsf(t.meta[(size_t)slot * 12 + 4]);
+            r.identity[0] = next;
+        } else {
+            r.identity[0] = std::max(r.identity[0], next);
+        }
+        r.identity[3] = incarnation;
+        const uint32_t first = std::max(next, r.identity[0]);
+        st[6] = first + c.samples;
+        r.identity[0] = first;
+        r.sequence = first;
+    }
+    // trace_visible_paths, every owner independent.
+    parallel_for((uint32_t)list.size(), threads, [&](uint32_t b, uint32_t e, int) {
+        Estimator est(t, nullptr);
+        for (uint32_t k = b; k < e; ++k) {
+            ViewFace &r = cur[list[k]];
+            I3 v{r.voxel[0], r.voxel[1], r.voxel[2]};
+            F3 normal = face_normal(r.face), surface = f3(r.surface[0], r.surface[1], r.surface[2]);
+            uint32_t stream = hash_voxel(v) ^ pcg(r.face + 1u);
+            for (uint32_t s = 0; s < c.samples; ++s) {
+                bool known;
+                F3 value = est.trace_receiver(surface, normal, stream, r.identity[0], known, nullptr);
+                r.identity[0] += 1u;
+                if (!known) continue;
+                if (r.moment[3] > 0.0f) {
+                    float change = r.moment[3];
+                    float vx = fmax_(r.moment[0] - r.mean[0] * r.mean[0], 0.0f), vy = fmax_(r.moment[1] - r.mean[1] * r.mean[1], 0.0f),
+                          vz = fmax_(r.moment[2] - r.mean[2] * r.mean[2], 0.0f);
+                    float keep = c.measured_retention > 0.0f ? c.measured_retention : 0.0f;
+                    keep = keep * ((vx * 0.2126f + vy * 0.7152f) + vz * 0.0722f) / (change * change);
+                    r.mean[3] = fmin_(r.mean[3], std::floor(keep));
+                    r.moment[3] = 0.0f;
+                }
+                float count = fmin_(r.mean[3] + 1.0f, 65536.0f);
+                float tt = 1.0f / count, uu = 1.0f - tt;
+                r.mean[0] = r.mean[0] * uu + value.x * tt;
+                r.mean[1] = r.mean[1] * uu + value.y * tt;
+                r.mean[2] = r.mean[2] * uu + value.z * tt;
+                r.mean[3] = count;
+                r.moment[0] = r.moment[0] * uu + (value.x * value.x) * tt;
+                r.moment[1] = r.moment[1] * uu + (value.y * value.y) * tt;
+                r.moment[2] = r.moment[2] * uu + (value.z * value.z) * tt;
+            }
+            F3 dv, dc;
+            est.direct_at(surface, normal, dv, dc);
+            r.direct[0] = dv.x; r.direct[1] = dv.y; r.direct[2] = dv.z; r.direct[3] = 0.0f;
+            r.sequence = r.identity[0];
+        }
+    });
+    for (uint32_t i : list) {
+        if (!exchange) break;
+        const ViewFace &r = cur[i];
+        I3 v{r.voxel[0], r.voxel[1], r.voxel[2]};
+        uint32_t slot = view_shared_slot(t, v, r.face, r.identity[1]);
+        if (slot == NONE) continue;
+        uint32_t *st = &t.state[(size_t)slot * 36];
+        if (r.identity[3] != st[28]) continue;
+        const uint32_t count = (uint32_t)r.mean[3];
+        const bool changed_count = (st[5] & 0x7fffffffu) != count;
+        const bool changed_mean = st[0] != fbits(r.mean[0]) || st[1] != fbits(r.mean[1]) || st[2] != fbits(r.mean[2]);
+        st[0] = fbits(r.mean[0]); st[1] = fbits(r.mean[1]); st[2] = fbits(r.mean[2]); st[3] = 0u;
+        st[8] = fbits(r.moment[0]); st[9] = fbits(r.moment[1]); st[10] = fbits(r.moment[2]); st[11] = 0u;
+        st[4] = 0u;
+        st[12] = 0u;
+        st[5] = count | (r.moment[3] > 0.0f ? 0x80000000u : 0u);
+        st[6] = r.identity[0];
+        uint32_t *m = &t.meta[(size_t)slot * 12];
+        m[4] = fbits(r.moment[3]);
+        if (changed_mean || changed_count) {
+            float nx = std::sqrt(fmax_(r.moment[0] - r.mean[0] * r.mean[0], 0.0f)) / (0.002f + std::fabs(r.mean[0]));
+            float ny = std::sqrt(fmax_(r.moment[1] - r.mean[1] * r.mean[1], 0.0f)) / (0.002f + std::fabs(r.mean[1]));
+            float nz = std::sqrt(fmax_(r.moment[2] - r.mean[2] * r.mean[2], 0.0f)) / (0.002f + std::fabs(r.mean[2]));
+            m[8] = fbits(fmax_(nx, fmax_(ny, nz)));
+            m[9] = clock_ms;
+            m[10] = fbits(fmax_(r.mean[0], fmax_(r.mean[1], r.mean[2])));
+            m[11] = 0xa0000000u | count;
+        }
+    }
+    memcpy(owners_out, owners.data(), (size_t)pixels * 4);
+    memcpy(faces_out, cur.data(), (size_t)pixels * sizeof(ViewFace));
+    *owner_count = (uint32_t)list.size();
+    t.view_faces = std::move(cur);
+    t.view_owner = std::move(owner);
+    t.view_history = true;
+    return 0;
+}
+
+int orc_view_reset(void *tp) {
+    ((Tracer *)tp)->view_history = false;
+    return 0;
+}
+
 int orc_trace_ray(void *tp, const float origin[3], const float direction[3], float max_distance, int32_t *out_i, float *out_f) {
     Tracer &t = *(Tracer *)tp;
     tracer_sync(t);

@@ -200,6 +200,22 @@ int orc_mlp_exact(int on);
 /* 0: the hash-grid encoding; 1: the frequency encoding (sin/cos of 2^k pi u,
  * k = 0..3, no table). */
 int orc_mlp_encoding(int mode);
+
+/* ---- view tier (voxel_view_discovery.wgsl, voxel_view_runtime.wgsl,
+ * voxel_view_cache.wgsl): one camera frame over a depth/normal prepass; the
+ * camera record has vgi_view_camera's layout. Per pixel: the owner (pixel
+ * index, 0xffffffff none) and a 24-word face record (vgi_view_face). ---- */
+typedef struct orc_view_camera {
+    float world_from_clip[16];
+    float position[3];
+    float inner_distance;
+    uint32_t width, height;
+    uint32_t samples;
+    float measured_retention;
+} orc_view_camera;
+int orc_view_frame(void *tracer, const orc_view_camera *camera, const float *depth, const float *normals, uint32_t clock_ms,
+                   uint32_t *owners, void *faces, uint32_t *owner_count, int threads);
+int orc_view_reset(void *tracer);
 /* The 32 encoded input features of each query (n x 32 doubles). */
 int orc_mlp_encode(const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out);
 

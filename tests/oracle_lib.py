@@ -69,6 +69,8 @@ def lib():
         "orc_tracer_emitters": (i64, [P, P, i64]),
         "orc_trace_faces": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
         "orc_trace_learned": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
+        "orc_view_frame": (ctypes.c_int, [P, P, P, P, u32, P, P, P, ctypes.c_int]),
+        "orc_view_reset": (ctypes.c_int, [P]),
         "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32, u32]),
         "orc_cache_register": (ctypes.c_int, [P, P, u32, P]),
         "orc_cache_batch": (ctypes.c_int, [P, P, P, u32, P, P, P, ctypes.c_int, u32]),
@@ -235,6 +237,21 @@ class Tracer:
         queries = np.zeros((n, 4), np.uint32)
         lib().orc_trace_learned(self.h, ptr(tasks), len(tasks), ptr(partial), ptr(factor), ptr(queries), threads)
         return partial, factor, queries
+
+    def view_frame(self, camera, depth, normals, clock_ms=0, threads=8):
+        """The oracle's view frame: (owner per pixel, VIEW_FACE records, owner count).
+        `camera` is the package's ViewCamera (the same layout)."""
+        d = np.ascontiguousarray(depth, np.float32)
+        n = np.ascontiguousarray(normals, np.float32)
+        pixels = camera.width * camera.height
+        owners = np.zeros(pixels, np.uint32)
+        faces = np.zeros(pixels * 24, np.uint32)
+        count = np.zeros(1, np.uint32)
+        lib().orc_view_frame(self.h, ctypes.byref(camera), ptr(d), ptr(n), clock_ms, ptr(owners), ptr(faces), ptr(count), threads)
+        return owners, faces, int(count[0])
+
+    def view_reset(self):
+        lib().orc_view_reset(self.h)
 
     def trace_ray(self, origin, direction, max_distance=768.0):
         o = np.asarray(origin, np.float32)
