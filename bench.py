@@ -433,6 +433,73 @@ def sun_corner_words(vgi, g, faces, lights, args):
                  "faces": int(len(faces)), "calls_timed": args.sun_calls}, **out)
 
 
+def tunnel_camera(grid):
+    """An eye inside a worm tunnel near the scene centre (an air voxel with air
+    all around it within 2 voxels) and a target along its longest free line
+    of sight among the 26 lattice directions."""
+    from scipy import ndimage
+    c = grid.shape[0] // 2
+    lo = c - 64
+    air = grid[lo:lo + 128, lo:lo + 128, lo:lo + 128] == 256
+    inner = ndimage.binary_erosion(air, structure=np.ones((5, 5, 5), bool))
+    zyx = np.argwhere(inner)
+    if len(zyx) == 0:
+        return None
+    pick = zyx[np.argmin(((zyx - 64) ** 2).sum(axis=1))] + lo
+    eye = np.array([pick[2], pick[1], pick[0]], np.float64) + 0.5
+    best, target = -1, None
+    for d in [np.array(v, np.float64) for v in np.ndindex(3, 3, 3)]:
+        d = d - 1.0
+        if not d.any():
+            continue
+        d /= np.linalg.norm(d)
+        free = 0
+        while free < 256:
+            p = np.floor(eye + d * (free + 1)).astype(int)
+            if (p < 0).any() or (p >= grid.shape[0]).any() or grid[p[2], p[1], p[0]] != 256:
+                break
+            free += 1
+        # A line of sight near the vertical has no defined camera roll.
+        if free > best and abs(d[1]) < 0.9:
+            best, target = free, eye + d * 16.0
+    return eye, target
+
+
+def view_frames(vgi, g, grid, args):
+    """View frames inside the cave on the bound resident tracer: a 640x360
+    depth/normal prepass from a camera in a worm tunnel (CPU ray cast of the
+    voxel grid), `--view-frames` frames of discovery, face hash, reprojection,
+    world-cache exchange and 4 paths per visible face; the camera moves 0.1
+    voxel per frame along its line of sight."""
+    found = tunnel_camera(grid)
+    if found is None:
+        return None
+    eye, target = found
+    w, h = 640, 360
+    d = (target - eye) / np.linalg.norm(target - eye)
+    frame_ms, trace_ms, owners = [], [], []
+    for k in range(args.view_frames + 1):
+        e = eye + d * 0.1 * k
+        wfc, depth, normals, hits = vgi.scene_prepass(grid, e, e + d * 16.0, w, h)
+        cam = vgi.view_camera(wfc, e, w, h, samples=4)
+        g.view_frame(cam, depth, normals)
+        g.sync()
+        if k == 0:
+            continue  # first frame: nothing to reproject yet
+        ms = g.view_timings()
+        frame_ms.append(float(ms[0]))
+        trace_ms.append(float(ms[1]))
+        owners.append(g.view_owners(0)[1])
+    med = float(np.median(frame_ms))
+    faces = int(np.median(owners))
+    out = {"workload": "512^3 worm cave, 640x360 prepass from inside a tunnel, every bound face resident; 4 paths per visible face",
+           "pixels": w * h, "pixels_hit": int(hits), "visible_faces": faces, "frames_timed": args.view_frames,
+           "frame_gpu_ms_median": round(med, 3), "trace_gpu_ms_median": round(float(np.median(trace_ms)), 3),
+           "faces_per_s": round(faces / (med / 1000.0), 1)}
+    print("view frames: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
 def batched_scenes(vgi, args):
     """32 seeded noise scenes of log-uniform edges in [32, 512] (multiples of
     8), 2 rays per face, 2 bounces, traced in one persistent launch per frame
@@ -530,6 +597,8 @@ def main():
     ap.add_argument("--select-batches", type=int, default=8)
     ap.add_argument("--no-sun", action="store_true", help="skip the sun corner-word leg")
     ap.add_argument("--sun-calls", type=int, default=8)
+    ap.add_argument("--no-view", action="store_true", help="skip the view-tier leg")
+    ap.add_argument("--view-frames", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the hot path on a single MI355X "
@@ -672,6 +741,8 @@ def main():
         line["work_selection"] = work_selection(vgi, g, faces, args)
     if not args.no_sun and args.workload == "cave512":
         line["sun_corner_words"] = sun_corner_words(vgi, g, faces, lights, args)
+    if not args.no_view and args.workload == "cave512":
+        line["view_frames"] = view_frames(vgi, g, grid, args)
     g.close()
     if not args.no_edit_stream and args.workload == "cave512":
         line["edit_stream"] = edit_stream(vgi, args)

@@ -160,6 +160,7 @@ def library():
         "vgi_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32]),
         "vgi_view_frame": (ctypes.c_int, [P, P, P, P]),
         "vgi_view_reset": (ctypes.c_int, [P]),
+        "vgi_view_timings": (ctypes.c_int, [P, P]),
         "vgi_view_read_owners": (i64, [P, P, ctypes.c_uint32, P]),
         "vgi_view_read_faces": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_read_sun_flips": (ctypes.c_int64, [P, P, ctypes.c_uint32]),
@@ -215,7 +216,7 @@ def exported_symbols():
         "vgi_batch_add", "vgi_batch_configure", "vgi_batch_set_sequence", "vgi_batch_run_frame", "vgi_batch_sync",
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
-        "vgi_view_frame", "vgi_view_reset", "vgi_view_read_owners", "vgi_view_read_faces",
+        "vgi_view_frame", "vgi_view_reset", "vgi_view_timings", "vgi_view_read_owners", "vgi_view_read_faces",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -243,6 +244,7 @@ def workloads():
         "vgi_scene_optics": (ctypes.c_int, [ctypes.c_int, u32, P, P, P, P, P]),
         "vgi_scene_faces": (i64, [P, P, ctypes.c_int, P, i64]),
         "vgi_scene_edits": (i64, [u32, u32, u32, u32, P, P, i64]),
+        "vgi_scene_prepass": (i64, [P, P, P, P, ctypes.c_float, u32, u32, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -254,7 +256,7 @@ def workloads():
 
 def workload_symbols():
     """Names include/vgi_workloads.h declares (libvgi_workloads.so)."""
-    return ["vgi_scene_generate", "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits"]
+    return ["vgi_scene_generate", "vgi_scene_optics", "vgi_scene_faces", "vgi_scene_edits", "vgi_scene_prepass"]
 
 
 def _check_workload(rc, what):
@@ -315,6 +317,22 @@ def scene_edits(seed, frame, count, edge):
     values = np.zeros(n, np.uint16)
     lib.vgi_scene_edits(seed, frame, count, edge, _ptr(xyz), _ptr(values), n)
     return xyz, values
+
+
+def scene_prepass(grid, eye, target, width, height, fov_y=1.0471975511965976):
+    """(world_from_clip [4, 4] row-major, depth [h, w], normals [h, w, 3], pixels hit)
+    of a dense grid seen from `eye` towards `target` (workload library)."""
+    lib = workloads()
+    grid = np.ascontiguousarray(grid, np.uint16)
+    dims = np.array([grid.shape[2], grid.shape[1], grid.shape[0]], np.int32)
+    wfc = np.zeros(16, np.float32)
+    depth = np.zeros((height, width), np.float32)
+    normals = np.zeros((height, width, 3), np.float32)
+    e = np.asarray(eye, np.float32)
+    t = np.asarray(target, np.float32)
+    n = _check_workload(lib.vgi_scene_prepass(_ptr(grid), _ptr(dims), _ptr(e), _ptr(t), fov_y, width, height, _ptr(wfc), _ptr(depth),
+                                              _ptr(normals)), "vgi_scene_prepass")
+    return wfc.reshape(4, 4).T.copy(), depth, normals, n
 
 
 def pages_of(grid, origin=(0, 0, 0)):
@@ -573,6 +591,12 @@ class GpuTracer:
 
     def view_reset(self):
         _check(self._lib.vgi_view_reset(self._h))
+
+    def view_timings(self):
+        """GPU ms of the last view frame: [all passes, tracing]."""
+        out = np.zeros(2, np.float32)
+        _check(self._lib.vgi_view_timings(self._h, _ptr(out)))
+        return out
 
     def view_owners(self, pixels):
         """(owner per pixel, owner count, hash overflow count) of the last view frame."""

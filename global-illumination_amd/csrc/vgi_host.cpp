@@ -229,6 +229,7 @@ struct vgi_tracer {
     uint32_t view_next = 0;
     uint32_t view_pixels[2] = {0u, 0u}, view_mask[2] = {0u, 0u};
     bool view_history = false;   // the last frame's generation may be reprojected
+    hipEvent_t view_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // frame start, trace start, trace end, frame end
     DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
     DeviceBuffer d_table, d_table_m, d_table_v, d_table_g;  // hash-grid encoding (FP32, Adam)
     std::vector<float> table_grad_last;                      // of the last vgi_mlp_train without update
@@ -1680,6 +1681,7 @@ void vgi_destroy(vgi_tracer *t) {
     for (auto &e : t->sun_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->mlp_ev) if (e) hipEventDestroy(e);
     for (auto &e : t->mlp_step_ev) if (e) hipEventDestroy(e);
+    for (auto &e : t->view_ev) if (e) hipEventDestroy(e);
     if (t->stage_done) hipEventDestroy(t->stage_done);
     if (t->stage) hipHostFree(t->stage);
     if (t->side) hipStreamDestroy(t->side);
@@ -2535,11 +2537,16 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     c.state = t->d_state.as<uint32_t>();
     c.meta = t->d_meta.as<uint32_t>();
     c.mask = t->resident ? (uint32_t)(t->face_slots.entries.size() - 1) : 0u;
+    if (!t->view_ev[0])
+        for (auto &e : t->view_ev) HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(t->view_ev[0], t->stream));
     vgi::launch_view_frame(p, s, o, c, t->v_depth.as<float>(), t->v_normals.as<float>(), t->v_receivers[g].as<vgi::ViewReceiver>(),
                            t->v_estimates[g].as<vgi::ViewEstimate>(), t->v_hash[g].as<uint32_t>(), t->v_owners[g].as<uint32_t>(),
                            history ? t->v_receivers[pg].as<vgi::ViewReceiver>() : nullptr,
                            history ? t->v_estimates[pg].as<vgi::ViewEstimate>() : nullptr,
-                           history ? t->v_hash[pg].as<uint32_t>() : nullptr, t->v_queue.as<uint32_t>(), t->stream);
+                           history ? t->v_hash[pg].as<uint32_t>() : nullptr, t->v_queue.as<uint32_t>(), t->stream, t->view_ev[1],
+                           t->view_ev[2]);
+    HIP_OK(hipEventRecord(t->view_ev[3], t->stream));
     HIP_OK(hipGetLastError());
     t->view_pixels[g] = pixels;
     t->view_mask[g] = capacity - 1u;
@@ -2548,16 +2555,26 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     return 0;
 }
 
-int vgi_view_reset(vgi_tracer *t) {
-    if (!t) return fail("null tracer");
-    t->view_history = false;
-    return 0;
-}
-
 // The last view frame's generation, or -1 when there is none.
 static int view_last(const vgi_tracer &t) {
     uint32_t g = t.view_next ^ 1u;
     return t.view_history && t.view_pixels[g] > 0u ? (int)g : -1;
+}
+
+int vgi_view_timings(vgi_tracer *t, float out[2]) {
+    if (!t || !out) return fail("null argument");
+    if (view_last(*t) < 0 || !t->view_ev[0]) return fail("no view frame has run");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipEventSynchronize(t->view_ev[3]));
+    HIP_OK(hipEventElapsedTime(&out[0], t->view_ev[0], t->view_ev[3]));
+    HIP_OK(hipEventElapsedTime(&out[1], t->view_ev[1], t->view_ev[2]));
+    return 0;
+}
+
+int vgi_view_reset(vgi_tracer *t) {
+    if (!t) return fail("null tracer");
+    t->view_history = false;
+    return 0;
 }
 
 int64_t vgi_view_read_owners(vgi_tracer *t, uint32_t *out, uint32_t pixels, uint32_t *overflow) {

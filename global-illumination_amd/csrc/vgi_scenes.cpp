@@ -343,3 +343,136 @@ int64_t vgi_scene_edits(uint32_t seed, uint32_t frame, uint32_t count, uint32_t 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- prepass
+namespace {
+struct Mat4 {
+    double m[4][4];  // row-major
+};
+Mat4 mul(const Mat4 &a, const Mat4 &b) {
+    Mat4 r{};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < 4; ++k) r.m[i][j] += a.m[i][k] * b.m[k][j];
+    return r;
+}
+// Gauss-Jordan inverse with partial pivoting (the matrices here are well
+// conditioned perspective-times-rigid transforms).
+Mat4 inverse(Mat4 a) {
+    Mat4 r{};
+    for (int i = 0; i < 4; ++i) r.m[i][i] = 1.0;
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int i = c + 1; i < 4; ++i)
+            if (std::fabs(a.m[i][c]) > std::fabs(a.m[p][c])) p = i;
+        std::swap(a.m[p], a.m[c]);
+        std::swap(r.m[p], r.m[c]);
+        double d = a.m[c][c];
+        for (int j = 0; j < 4; ++j) { a.m[c][j] /= d; r.m[c][j] /= d; }
+        for (int i = 0; i < 4; ++i) {
+            if (i == c) continue;
+            double f = a.m[i][c];
+            for (int j = 0; j < 4; ++j) { a.m[i][j] -= f * a.m[c][j]; r.m[i][j] -= f * r.m[c][j]; }
+        }
+    }
+    return r;
+}
+}  // namespace
+
+int64_t vgi_scene_prepass(const uint16_t *voxels, const int32_t dims[3], const float eye_f[3], const float target_f[3], float fov_y,
+                          uint32_t width, uint32_t height, float *world_from_clip, float *depth, float *normals) {
+    if (!voxels || !dims || !eye_f || !target_f || !world_from_clip || !depth || !normals || width == 0 || height == 0) return -1;
+    const int sx = dims[0], sy = dims[1], sz = dims[2];
+    double eye[3] = {eye_f[0], eye_f[1], eye_f[2]};
+    double f[3] = {target_f[0] - eye[0], target_f[1] - eye[1], target_f[2] - eye[2]};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fl > 0.0)) return -1;
+    for (double &v : f) v /= fl;
+    double r[3] = {-f[2], 0.0, f[0]};  // f x (0, 1, 0)
+    double rl = std::sqrt(r[0] * r[0] + r[2] * r[2]);
+    if (!(rl > 1e-9)) return -1;
+    r[0] /= rl; r[2] /= rl;
+    double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+    Mat4 view{};
+    for (int j = 0; j < 3; ++j) { view.m[0][j] = r[j]; view.m[1][j] = u[j]; view.m[2][j] = -f[j]; }
+    for (int i = 0; i < 3; ++i) view.m[i][3] = -(view.m[i][0] * eye[0] + view.m[i][1] * eye[1] + view.m[i][2] * eye[2]);
+    view.m[3][3] = 1.0;
+    const double near = 0.1, t = 1.0 / std::tan(fov_y * 0.5), aspect = (double)width / (double)height;
+    Mat4 proj{};
+    proj.m[0][0] = t / aspect; proj.m[1][1] = t; proj.m[2][3] = near; proj.m[3][2] = -1.0;
+    const Mat4 cfw = mul(proj, view), wfc = inverse(cfw);
+    for (int c = 0; c < 4; ++c)
+        for (int rr = 0; rr < 4; ++rr) world_from_clip[c * 4 + rr] = (float)wfc.m[rr][c];
+    auto solid = [&](int x, int y, int z) { return voxels[at(x, y, z, sx, sy)] != AIR; };
+    int64_t hits = 0;
+    for (uint32_t py = 0; py < height; ++py) {
+        for (uint32_t px = 0; px < width; ++px) {
+            size_t i = (size_t)py * width + px;
+            depth[i] = 0.0f;
+            normals[i * 3] = normals[i * 3 + 1] = normals[i * 3 + 2] = 0.0f;
+            double clip[4] = {(px + 0.5) / width * 2.0 - 1.0, 1.0 - (py + 0.5) / height * 2.0, 1.0, 1.0}, w[4] = {0, 0, 0, 0};
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) w[a] += wfc.m[a][b] * clip[b];
+            double d[3] = {w[0] / w[3] - eye[0], w[1] / w[3] - eye[1], w[2] / w[3] - eye[2]};
+            double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            for (double &v : d) v /= dl;
+            // Clip the ray to the grid box, then walk voxels (Amanatides & Woo 1987).
+            double t0 = 0.0, t1 = 1e30;
+            const int size[3] = {sx, sy, sz};
+            bool miss = false;
+            for (int a = 0; a < 3; ++a) {
+                if (std::fabs(d[a]) < 1e-12) {
+                    if (eye[a] < 0.0 || eye[a] >= size[a]) miss = true;
+                    continue;
+                }
+                double ta = (0.0 - eye[a]) / d[a], tb = (size[a] - eye[a]) / d[a];
+                t0 = std::max(t0, std::min(ta, tb));
+                t1 = std::min(t1, std::max(ta, tb));
+            }
+            if (miss || t0 >= t1) continue;
+            double p[3];
+            int v[3], step[3], axis = -1;
+            double next[3], delta[3];
+            for (int a = 0; a < 3; ++a) {
+                p[a] = eye[a] + d[a] * t0;
+                v[a] = std::min(std::max((int)std::floor(p[a]), 0), size[a] - 1);
+                step[a] = d[a] > 0.0 ? 1 : -1;
+                delta[a] = std::fabs(d[a]) < 1e-12 ? 1e30 : std::fabs(1.0 / d[a]);
+                double boundary = d[a] > 0.0 ? v[a] + 1.0 : (double)v[a];
+                next[a] = std::fabs(d[a]) < 1e-12 ? 1e30 : t0 + (boundary - p[a]) / d[a];
+            }
+            if (t0 > 0.0) {  // entered through the box face with the largest entry distance
+                double best = -1.0;
+                for (int a = 0; a < 3; ++a) {
+                    if (std::fabs(d[a]) < 1e-12) continue;
+                    double ta = ((d[a] > 0.0 ? 0.0 : size[a]) - eye[a]) / d[a];
+                    if (ta > best) { best = ta; axis = a; }
+                }
+            }
+            double tc = t0;
+            while (true) {
+                if (solid(v[0], v[1], v[2])) {
+                    if (axis < 0) break;  // the camera is inside solid: no surface
+                    double hit[3] = {eye[0] + d[0] * tc, eye[1] + d[1] * tc, eye[2] + d[2] * tc};
+                    double c2 = 0.0, c3 = 0.0;
+                    for (int b = 0; b < 3; ++b) { c2 += cfw.m[2][b] * hit[b]; c3 += cfw.m[3][b] * hit[b]; }
+                    c2 += cfw.m[2][3];
+                    c3 += cfw.m[3][3];
+                    depth[i] = (float)(c2 / c3);
+                    normals[i * 3 + axis] = step[axis] > 0 ? -1.0f : 1.0f;
+                    ++hits;
+                    break;
+                }
+                int a = next[0] < next[1] ? (next[0] < next[2] ? 0 : 2) : (next[1] < next[2] ? 1 : 2);
+                tc = next[a];
+                if (tc > t1) break;
+                v[a] += step[a];
+                if (v[a] < 0 || v[a] >= size[a]) break;
+                next[a] += delta[a];
+                axis = a;
+            }
+        }
+    }
+    return hits;
+}
+

@@ -60,6 +60,6 @@ struct SharedCache {
 void launch_view_frame(const ViewParams &p, const SceneView &s, const Optics &o, const SharedCache &cache, const float *depth,
                        const float *normals, ViewReceiver *receivers, ViewEstimate *estimates, uint32_t *hash, uint32_t *owners,
                        const ViewReceiver *prev_receivers, const ViewEstimate *prev_estimates, const uint32_t *prev_hash,
-                       uint32_t *queue, hipStream_t stream);
+                       uint32_t *queue, hipStream_t stream, hipEvent_t before_trace = nullptr, hipEvent_t after_trace = nullptr);
 
 }  // namespace vgi

@@ -263,38 +263,63 @@ __global__ void view_load_kernel(ViewParams p, SharedCache c, ViewReceiver *__re
 // measured-retention keep rule) and then merges into the running mean with
 // weight 1 / min(count + 1, 65536). Unknown samples advance the sequence
 // only. The face's analytic direct term is traced once.
+// An owner's samples run on `lanes` consecutive threads (a power of two up
+// to 8, so a group never straddles a wavefront): lane l traces samples l,
+// l + lanes, ...; after each round the group's first lane merges the round's
+// values in sample order, so the running mean is the sequential one.
 __global__ __launch_bounds__(256) void view_trace_kernel(ViewParams p, SceneView s, Optics o, ViewReceiver *__restrict__ receivers,
-                                                         ViewEstimate *__restrict__ estimates, const uint32_t *__restrict__ queue) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= queue[0]) return;
-    const uint32_t i = queue[2u + k];
-    const ViewReceiver r = receivers[i];
-    ViewEstimate e = estimates[i];
+                                                         ViewEstimate *__restrict__ estimates, const uint32_t *__restrict__ queue,
+                                                         uint32_t lanes) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = tid / lanes, lane = tid % lanes;
+    const bool valid = k < queue[0];
+    const uint32_t i = valid ? queue[2u + k] : 0u;
+    ViewReceiver r = {0, 0, 0, 0u, 0.0f, 0.0f, 0.0f, 0u};
+    ViewEstimate e;
+    e.mean = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    e.moment = e.mean;
+    e.direct = e.mean;
+    e.identity = make_uint4(0u, 0u, 0u, 0u);
+    if (valid) {
+        r = receivers[i];
+        e = estimates[i];
+    }
     const f3 normal = face_normal(r.face);
     const f3 surface = mk(r.sx, r.sy, r.sz);
     const uint32_t stream = hash_voxel(r.vx, r.vy, r.vz) ^ pcg(r.face + 1u);
+    const uint32_t start = e.identity.x;
+    const int group = (int)((threadIdx.x & 63u) - lane);
     NoTally tally;
-    for (uint32_t v = 0u; v < p.samples; ++v) {
-        bool known;
-        FirstHit first;
-        f3 value = trace_receiver<NoTally>(s, o, surface, normal, stream, e.identity.x, known, first, tally);
-        e.identity.x += 1u;
-        if (!known) continue;
-        if (e.moment.w > 0.0f) {
-            float change = e.moment.w;
-            f3 variance = mk(fmx(e.moment.x - e.mean.x * e.mean.x, 0.0f), fmx(e.moment.y - e.mean.y * e.mean.y, 0.0f),
-                             fmx(e.moment.z - e.mean.z * e.mean.z, 0.0f));
-            float keep = p.retention * ((variance.x * 0.2126f + variance.y * 0.7152f) + variance.z * 0.0722f) / (change * change);
-            e.mean.w = fmn(e.mean.w, floorf(keep));
-            e.moment.w = 0.0f;
+    for (uint32_t base = 0u; base < p.samples; base += lanes) {
+        f3 value = mk(0.0f, 0.0f, 0.0f);
+        bool known = false;
+        if (valid && base + lane < p.samples) {
+            FirstHit first;
+            value = trace_receiver<NoTally>(s, o, surface, normal, stream, start + base + lane, known, first, tally);
         }
-        float count = fmn(e.mean.w + 1.0f, 65536.0f);
-        float t = 1.0f / count, u = 1.0f - t;
-        e.mean = make_float4(e.mean.x * u + value.x * t, e.mean.y * u + value.y * t, e.mean.z * u + value.z * t, count);
-        e.moment.x = e.moment.x * u + (value.x * value.x) * t;
-        e.moment.y = e.moment.y * u + (value.y * value.y) * t;
-        e.moment.z = e.moment.z * u + (value.z * value.z) * t;
+        for (uint32_t l = 0u; l < lanes; ++l) {
+            const int src = group + (int)l;
+            const f3 v = mk(__shfl(value.x, src, 64), __shfl(value.y, src, 64), __shfl(value.z, src, 64));
+            const bool kn = __shfl((int)known, src, 64) != 0;
+            if (lane != 0u || !valid || base + l >= p.samples || !kn) continue;
+            if (e.moment.w > 0.0f) {
+                float change = e.moment.w;
+                f3 variance = mk(fmx(e.moment.x - e.mean.x * e.mean.x, 0.0f), fmx(e.moment.y - e.mean.y * e.mean.y, 0.0f),
+                                 fmx(e.moment.z - e.mean.z * e.mean.z, 0.0f));
+                float keep = p.retention * ((variance.x * 0.2126f + variance.y * 0.7152f) + variance.z * 0.0722f) / (change * change);
+                e.mean.w = fmn(e.mean.w, floorf(keep));
+                e.moment.w = 0.0f;
+            }
+            float count = fmn(e.mean.w + 1.0f, 65536.0f);
+            float t = 1.0f / count, u = 1.0f - t;
+            e.mean = make_float4(e.mean.x * u + v.x * t, e.mean.y * u + v.y * t, e.mean.z * u + v.z * t, count);
+            e.moment.x = e.moment.x * u + (v.x * v.x) * t;
+            e.moment.y = e.moment.y * u + (v.y * v.y) * t;
+            e.moment.z = e.moment.z * u + (v.z * v.z) * t;
+        }
     }
+    if (!valid || lane != 0u) return;
+    e.identity.x = start + p.samples;
     f3 dv, dc;
     direct_at(s, o, surface, normal, dv, dc, tally);
     e.direct = make_float4(dv.x, dv.y, dv.z, 0.0f);
@@ -341,7 +366,7 @@ __global__ void view_store_kernel(ViewParams p, SharedCache c, const ViewReceive
 void launch_view_frame(const ViewParams &p, const SceneView &s, const Optics &o, const SharedCache &cache, const float *depth,
                        const float *normals, ViewReceiver *receivers, ViewEstimate *estimates, uint32_t *hash, uint32_t *owners,
                        const ViewReceiver *prev_receivers, const ViewEstimate *prev_estimates, const uint32_t *prev_hash,
-                       uint32_t *queue, hipStream_t stream) {
+                       uint32_t *queue, hipStream_t stream, hipEvent_t before_trace, hipEvent_t after_trace) {
     if (p.pixels == 0u) return;
     const dim3 block(256), grid((p.pixels + 255u) / 256u);
     hipLaunchKernelGGL(view_discover_kernel, grid, block, 0, stream, p, s, depth, normals, receivers, estimates);
@@ -350,7 +375,12 @@ void launch_view_frame(const ViewParams &p, const SceneView &s, const Optics &o,
     hipLaunchKernelGGL(view_reproject_kernel, grid, block, 0, stream, p, receivers, estimates, owners, prev_receivers, prev_estimates,
                        prev_hash, queue);
     if (p.exchange) hipLaunchKernelGGL(view_load_kernel, grid, block, 0, stream, p, cache, receivers, estimates, owners);
-    hipLaunchKernelGGL(view_trace_kernel, grid, block, 0, stream, p, s, o, receivers, estimates, queue);
+    if (before_trace) (void)hipEventRecord(before_trace, stream);
+    uint32_t lanes = 1u;
+    while (lanes < 8u && lanes < p.samples) lanes *= 2u;
+    const dim3 trace_grid((uint32_t)(((uint64_t)p.pixels * lanes + 255u) / 256u));
+    hipLaunchKernelGGL(view_trace_kernel, trace_grid, block, 0, stream, p, s, o, receivers, estimates, queue, lanes);
+    if (after_trace) (void)hipEventRecord(after_trace, stream);
     if (p.exchange) hipLaunchKernelGGL(view_store_kernel, grid, block, 0, stream, p, cache, receivers, estimates, owners);
 }
 

@@ -427,6 +427,13 @@ impl GpuTracer {
         check(unsafe { sys::vgi_view_frame(self.raw(), camera, depth.as_ptr(), normals.as_ptr()) })
     }
 
+    /// GPU ms of the last view frame: [all its passes, the owners' tracing].
+    pub fn view_timings(&self) -> Result<[f32; 2]> {
+        let mut out = [0f32; 2];
+        check(unsafe { sys::vgi_view_timings(self.raw(), out.as_mut_ptr()) })?;
+        Ok(out)
+    }
+
     /// Forget the view history.
     pub fn view_reset(&mut self) -> Result<()> {
         check(unsafe { sys::vgi_view_reset(self.raw()) })

@@ -345,6 +345,9 @@ extern "C" {
 
     pub fn vgi_view_frame(tracer: *mut vgi_tracer, camera: *const vgi_view_camera, depth: *const f32, normals: *const f32) -> c_int;
 
+    /// GPU ms of the last view frame: [all its passes, the owners' tracing].
+    pub fn vgi_view_timings(tracer: *mut vgi_tracer, out: *mut f32) -> c_int;
+
     /// Forget the view history (the next frame reprojects nothing).
     pub fn vgi_view_reset(tracer: *mut vgi_tracer) -> c_int;
 

@@ -281,6 +281,8 @@ typedef struct vgi_view_face {
 } vgi_view_face;
 #define VGI_VIEW_MAX_PIXELS 16777216u
 int vgi_view_frame(vgi_tracer *tracer, const vgi_view_camera *camera, const float *depth, const float *normals);
+/* GPU ms of the last view frame: [all its passes, the owners' tracing]. */
+int vgi_view_timings(vgi_tracer *tracer, float out[2]);
 /* Forget the view history (the next frame reprojects nothing). */
 int vgi_view_reset(vgi_tracer *tracer);
 /* The last view frame's owner per pixel (pixel index of the face's owner,

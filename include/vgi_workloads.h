@@ -30,6 +30,16 @@ int64_t vgi_scene_faces(const uint16_t *voxels, const int32_t dims[3], int inter
 int64_t vgi_scene_edits(uint32_t seed, uint32_t frame, uint32_t count, uint32_t edge, int32_t *xyz, uint16_t *values,
                         int64_t capacity);
 
+/* A depth/normal prepass of a dense grid seen from a look-at camera (up
+ * +y, vertical field of view fov_y radians, reverse-Z infinite perspective
+ * with near plane 0.1: depth = 0.1 / view distance along the axis, 0 where
+ * the ray leaves the grid): the first solid voxel along each pixel centre's
+ * ray (3D DDA), the hit point's depth and the hit face's normal. Writes
+ * world_from_clip (16 floats, column-major, vgi_view_camera's) and per pixel
+ * row-major depth and normal xyz. Returns the pixels that hit. */
+int64_t vgi_scene_prepass(const uint16_t *voxels, const int32_t dims[3], const float eye[3], const float target[3], float fov_y,
+                          uint32_t width, uint32_t height, float *world_from_clip, float *depth, float *normals);
+
 #ifdef __cplusplus
 }
 #endif

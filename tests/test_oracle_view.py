@@ -147,3 +147,28 @@ def test_estimates_carry_across_frames_and_restart_on_a_new_material():
     assert f["identity"][i, 1] == 2 and f["mean"][i, 3] == 2.0 and f["identity"][i, 0] == 2, "a new material starts over"
     others = own[1:]
     assert np.all(f["mean"][others, 3] == 6.0)
+
+
+def test_workload_prepass_agrees_with_the_oracle_traversal():
+    """The benchmark's CPU prepass renderer (a plain 3D DDA over the dense
+    grid, workload library) against rays through the oracle's NAADF
+    traversal: the same hit faces and depths, except where a ray grazes a
+    voxel edge (either side is a hit there)."""
+    vgi = package()
+    from view_helpers import prepass as oracle_prepass
+    grid = vgi.generate(2, 0x5EED0002, 64)
+    coords, pages = vgi.pages_of(grid)
+    scene = orc.Scene()
+    scene.load_pages(coords, pages)
+    t = orc.Tracer(scene)
+    eye, target = (-6.0, 40.0, -6.0), (32.0, 8.0, 32.0)
+    w, h = 40, 30
+    wfc, depth, normals, hits = vgi.scene_prepass(grid, eye, target, w, h)
+    cfw, wfc2 = camera(eye, target, w, h)
+    assert np.allclose(wfc, wfc2, rtol=1e-5, atol=1e-6)
+    d2, n2 = oracle_prepass(t, eye, cfw, wfc2, w, h)
+    assert hits == int((depth > 0).sum()) and hits > 0.5 * w * h
+    same = np.all(normals == n2, axis=2) & ((depth > 0) == (d2 > 0))
+    assert same.mean() > 0.97
+    both = same & (depth > 0)
+    assert np.allclose(depth[both], d2[both], rtol=1e-4)
