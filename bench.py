@@ -39,6 +39,10 @@ LEARNED_LR = 3e-3
 # (i/n)^0.25, so the frames of the network's warm-up fade faster than in
 # the exact mean.
 LEARNED_RECENCY = 1.25
+# The caller's training schedule (vgi_mlp_training): after 800 learned frames
+# the network has settled and half the targets in half the Adam steps keep
+# it there, at 1.1 ms less per frame.
+LEARNED_SCHEDULE = {800: (40960, 4, LEARNED_LR)}
 # The network as the 512^3 cave configuration states it: a 32-dim frequency
 # encoding (no hash grid), 64k one-path samples per frame, Adam at 1e-3 (the
 # steps per frame are not stated: 8, as above).
@@ -130,15 +134,18 @@ def mlp_roofline(timings):
     }
 
 
-def run_to_target(vgi, g, judged, reference, var, max_frames, target, budget_ms, every=8):
+def run_to_target(vgi, g, judged, reference, var, max_frames, target, budget_ms, every=8, schedule=None):
     """Frames on the bound faces (the judged ones bound first) until the
     judged faces' relative RMSE against the reference reaches `target` or the
-    GPU time (from `budget_ms`'s origin) runs out. Returns the (frame,
-    cumulative GPU ms, rmse with the reference's variance removed, raw rmse)
-    points evaluated."""
+    GPU time (from `budget_ms`'s origin) runs out; `schedule` maps a frame to
+    the learned cache's (targets, steps, rate) from then on. Returns the
+    (frame, cumulative GPU ms, rmse with the reference's variance removed,
+    raw rmse) points evaluated."""
     points = []
     gpu_ms = 0.0
     for f in range(1, max_frames + 1):
+        if schedule and f in schedule:
+            g.mlp_training(*schedule[f])
         g.run_frame()
         g.sync()
         gpu_ms += g.last_frame_ms()
@@ -210,8 +217,9 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
     # The learned form at 1 ray per face per frame; then the network as the
     # workload states it (frequency encoding, one-path targets), which
     # measures what the hash grid and the averaged targets are for.
-    for name, encoding, train, lr in (("learned", vgi.MLP_ENCODING_HASH, LEARNED_TRAIN, LEARNED_LR),
-                                      ("learned_stated_network", vgi.MLP_ENCODING_FREQUENCY, STATED_TRAIN, STATED_LR)):
+    for name, encoding, train, lr, schedule in (
+            ("learned", vgi.MLP_ENCODING_HASH, LEARNED_TRAIN, LEARNED_LR, LEARNED_SCHEDULE),
+            ("learned_stated_network", vgi.MLP_ENCODING_FREQUENCY, STATED_TRAIN, STATED_LR, None)):
         g = tracer(LEARNED_SPP, LEARNED_RECENCY)
         g.bind_faces(ordered)
         g.mlp_enable(seed=1, train_per_frame=train[0], steps=train[1], lr=lr)
@@ -222,9 +230,9 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
         g.sync()
         pretrain_ms = (time.perf_counter() - t1) * 1000.0
         pts = run_to_target(vgi, g, judged, reference, var, args.learned_frames, args.target_rmse,
-                            args.convergence_budget_ms - pretrain_ms)
+                            args.convergence_budget_ms - pretrain_ms, schedule=schedule)
         g.close()
-        legs[name] = (pts, pretrain_ms, LEARNED_SPP, (encoding, train, lr))
+        legs[name] = (pts, pretrain_ms, LEARNED_SPP, (encoding, train, lr, schedule))
         print("%s: %d frames, %.0f GPU ms, rmse %.4f (raw %.4f) (pretrain %.0f ms)" % ((name,) + pts[-1] + (pretrain_ms,)),
               file=sys.stderr, flush=True)
     budgets = [1000.0, 3000.0, 10000.0, 20000.0, 30000.0]
@@ -244,10 +252,12 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
             "rays_per_face_per_frame": rays,
         }
         if net is not None:
-            encoding, train, lr = net
+            encoding, train, lr, schedule = net
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
             entry["history_recency"] = LEARNED_RECENCY
+            if schedule:
+                entry["schedule"] = {str(k): {"targets": v[0], "steps": v[1], "lr": v[2]} for k, v in schedule.items()}
             entry["training"] = ("%d targets per frame of %d path%s each (3-bounce), %d Adam steps (lr %g%s), BF16 MFMA; encoding: %s" % (
                 train[0], train[2], "s" if train[2] > 1 else "", train[1], lr,
                 ", hash table %g, lazy" % (10 * lr) if encoding == vgi.MLP_ENCODING_HASH else "",
