@@ -69,7 +69,7 @@ class Schedule(ctypes.Structure):
 
 
 class ViewCamera(ctypes.Structure):
-    """vgi_view_camera: world_from_clip column-major (element (r, c) at 4c + r)."""
+    """vgi_view_camera: matrices column-major (element (r, c) at 4c + r)."""
     _fields_ = [
         ("world_from_clip", ctypes.c_float * 16),
         ("position", ctypes.c_float * 3),
@@ -78,25 +78,61 @@ class ViewCamera(ctypes.Structure):
         ("height", ctypes.c_uint32),
         ("samples", ctypes.c_uint32),
         ("measured_retention", ctypes.c_float),
+        ("clip_from_world", ctypes.c_float * 16),
+        ("forward", ctypes.c_float * 3),
+        ("pixels_per_unit", ctypes.c_float),
+        ("frame", ctypes.c_uint32),
     ]
 
 
-def view_camera(world_from_clip, position, width, height, samples=1, measured_retention=1.0, inner_distance=0.0):
-    """A ViewCamera from a 4x4 row-major numpy matrix (clip -> world)."""
+class FarSettings(ctypes.Structure):
+    """vgi_far_settings (FarTierSettings, settings.rs:95-138)."""
+    _fields_ = [
+        ("voxel_pixels", ctypes.c_float),
+        ("measure", ctypes.c_uint32),
+        ("cell", ctypes.c_uint32),
+        ("history", ctypes.c_float),
+        ("rate", ctypes.c_uint32),
+        ("steps", ctypes.c_uint32),
+    ]
+
+
+def view_camera(world_from_clip, position, width, height, samples=1, measured_retention=1.0, inner_distance=0.0,
+                clip_from_world=None, forward=(0.0, 0.0, -1.0), pixels_per_unit=0.0, frame=0):
+    """A ViewCamera from 4x4 row-major numpy matrices (clip -> world, and
+    world -> clip, by default the float64 inverse of the first)."""
     c = ViewCamera()
     m = np.asarray(world_from_clip, np.float32)
+    if clip_from_world is None:
+        try:
+            clip_from_world = np.linalg.inv(np.asarray(world_from_clip, np.float64))
+        except np.linalg.LinAlgError:  # a designed prepass with no camera behind it
+            clip_from_world = np.zeros((4, 4))
+    mi = np.asarray(clip_from_world, np.float32)
     for col in range(4):
         for row in range(4):
             c.world_from_clip[col * 4 + row] = float(m[row, col])
+            c.clip_from_world[col * 4 + row] = float(mi[row, col])
     for i in range(3):
         c.position[i] = float(position[i])
+        c.forward[i] = float(forward[i])
     c.inner_distance = inner_distance
     c.width, c.height, c.samples, c.measured_retention = width, height, samples, measured_retention
+    c.pixels_per_unit, c.frame = pixels_per_unit, frame
     return c
 
 
+def far_settings(voxel_pixels=8.0, measure=0, cell=0, history=32.0, rate=4, steps=128):
+    """FarSettings with the reference's defaults (settings.rs:127-138)."""
+    f = FarSettings()
+    f.voxel_pixels, f.measure, f.cell, f.history, f.rate, f.steps = voxel_pixels, measure, cell, history, rate, steps
+    return f
+
+
 VIEW_FACE = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("surface", "<f4", (3,)), ("sequence", "<u4"), ("mean", "<f4", (4,)),
-                      ("moment", "<f4", (4,)), ("identity", "<u4", (4,)), ("direct", "<f4", (4,))])
+                      ("moment", "<f4", (4,)), ("identity", "<u4", (4,)), ("direct", "<f4", (4,)), ("shown", "<f4", (4,))])
+VIEW_PIXEL = np.dtype([("surface", "<f4", (3,)), ("face", "<u4"), ("irradiance", "<f4", (3,)), ("evidence", "<f4"),
+                       ("sky", "<u4", (4,))])
 
 SEL_THRESHOLD, SEL_REMAINDER, SEL_TIES, SEL_TOTAL, SEL_MIRRORS, SEL_WORDS = 1024, 1028, 1032, 1036, 1037, 1038
 
@@ -163,6 +199,11 @@ def library():
         "vgi_view_timings": (ctypes.c_int, [P, P]),
         "vgi_view_read_owners": (i64, [P, P, ctypes.c_uint32, P]),
         "vgi_view_read_faces": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
+        "vgi_view_far": (ctypes.c_int, [P, P]),
+        "vgi_view_display": (ctypes.c_int, [P, f32, f32, f32]),
+        "vgi_view_read_pixels": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
+        "vgi_view_read_counters": (ctypes.c_int, [P, P]),
+        "vgi_view_read_far": (i64, [P, P, u64, P, u64]),
         "vgi_read_sun_flips": (ctypes.c_int64, [P, P, ctypes.c_uint32]),
         "vgi_read_sun_corners": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_sun_timings": (ctypes.c_int, [P, P, ctypes.c_uint32]),
@@ -217,6 +258,7 @@ def exported_symbols():
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
         "vgi_view_frame", "vgi_view_reset", "vgi_view_timings", "vgi_view_read_owners", "vgi_view_read_faces",
+        "vgi_view_far", "vgi_view_display", "vgi_view_read_pixels", "vgi_view_read_counters", "vgi_view_read_far",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -593,8 +635,8 @@ class GpuTracer:
         _check(self._lib.vgi_view_reset(self._h))
 
     def view_timings(self):
-        """GPU ms of the last view frame: [all passes, tracing]."""
-        out = np.zeros(2, np.float32)
+        """GPU ms of the last view frame: [all passes, owners' tracing, far tier]."""
+        out = np.zeros(3, np.float32)
         _check(self._lib.vgi_view_timings(self._h, _ptr(out)))
         return out
 
@@ -609,6 +651,33 @@ class GpuTracer:
         out = np.zeros(count, VIEW_FACE)
         _check(self._lib.vgi_view_read_faces(self._h, first, count, _ptr(out)))
         return out
+
+    def view_far(self, settings):
+        """The far tier's settings (FarSettings, or None: off)."""
+        _check(self._lib.vgi_view_far(self._h, ctypes.byref(settings) if settings is not None else None))
+
+    def view_display(self, noise_threshold=0.031, spatial=1.0, blend=0.19926):
+        _check(self._lib.vgi_view_display(self._h, noise_threshold, spatial, blend))
+
+    def view_pixels(self, first, count):
+        """Published records (VIEW_PIXEL) of pixels [first, first + count)."""
+        out = np.zeros(count, VIEW_PIXEL)
+        _check(self._lib.vgi_view_read_pixels(self._h, first, count, _ptr(out)))
+        return out
+
+    def view_counters(self):
+        out = np.zeros(16, np.uint32)
+        _check(self._lib.vgi_view_read_counters(self._h, _ptr(out)))
+        return out
+
+    def view_far_output(self, cells):
+        """(output words [16 + 4 cells], cell words [cells, 12]) of the last view frame."""
+        output = np.zeros(16 + 4 * cells, np.uint32)
+        history = np.zeros((cells, 12), np.uint32)
+        n = _check(self._lib.vgi_view_read_far(self._h, _ptr(output), output.size, _ptr(history), history.size))
+        if n != cells:
+            raise VoxelGiError("the last view frame had %d far cells, not %d" % (n, cells))
+        return output, history
 
     # ---- sun corner words
     def sun_corners(self, frame=0, refresh_frames=1):

@@ -226,10 +226,23 @@ struct vgi_tracer {
     // owners and face hash; `view_next` is the generation the next frame
     // writes, the other holds the last frame.
     DeviceBuffer v_depth, v_normals, v_queue, v_receivers[2], v_estimates[2], v_owners[2], v_hash[2];
+    DeviceBuffer v_pixels, v_counters;  // published records per pixel; far tier and presentation counters (16 words)
     uint32_t view_next = 0;
     uint32_t view_pixels[2] = {0u, 0u}, view_mask[2] = {0u, 0u};
     bool view_history = false;   // the last frame's generation may be reprojected
-    hipEvent_t view_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // frame start, trace start, trace end, frame end
+    // frame start, trace start, trace end, far tier start, frame end
+    hipEvent_t view_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float view_display[3] = {0.031f, 1.0f, 0.19926f};  // noise threshold, spatial strength, temporal blend
+    // The far tier: its settings, two generations of cells (one per view
+    // generation), the material's output, the trace queue; the last frame's
+    // camera (clip_from_world and frame number) for the next reprojection.
+    vgi_far_settings far = {0.0f, 0u, 0u, 32.0f, 4u, 128u};
+    DeviceBuffer f_cells[2], f_output, f_queue;
+    uint32_t far_cx = 0u, far_cy = 0u, far_size = 0u;  // shape of the allocated cells (0: none)
+    uint32_t far_last_cells = 0u;                      // cells of the last view frame (0: the tier was off)
+    float far_prev_clip[16] = {};
+    uint32_t far_prev_frame = 0u;
+    bool far_prev = false;
     DeviceBuffer d_mlp_w, d_mlp_m, d_mlp_v, d_mlp_g, d_mlp_frag, d_mlp_loss;
     DeviceBuffer d_table, d_table_m, d_table_v, d_table_g;  // hash-grid encoding (FP32, Adam)
     std::vector<float> table_grad_last;                      // of the last vgi_mlp_train without update
@@ -2491,7 +2504,9 @@ int vgi_read_sun_corners(vgi_tracer *t, uint32_t first, uint32_t count, uint32_t
 // ---------------------------------------------------------------- view tier
 // render_view's passes for one camera (gpu/view/runtime.rs:1254-1266):
 // discover, index, reproject, load_shared_faces, trace_visible_paths,
-// store_shared_faces, on the tracer's stream after its earlier work.
+// reconstruct_view, store_shared_faces, publish_view, then the far tier's
+// classify_pixels and trace_pixel_queue, on the tracer's stream after its
+// earlier work.
 int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *depth, const float *normals) {
     if (!t || !camera || !depth || !normals) return fail("null argument");
     if (camera->width == 0 || camera->height == 0) return fail("empty prepass");
@@ -2512,13 +2527,40 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     HIP_OK(t->v_estimates[g].reserve((size_t)pixels * sizeof(vgi::ViewEstimate)));
     HIP_OK(t->v_owners[g].reserve((size_t)pixels * 4));
     HIP_OK(t->v_hash[g].reserve((size_t)capacity * 4));
+    HIP_OK(t->v_pixels.reserve((size_t)pixels * sizeof(vgi::ViewPixel)));
+    HIP_OK(t->v_counters.reserve(16 * 4));
+    // The far tier's cell grid (settings.rs:141-147, gpu/view/runtime.rs:912-916).
+    const vgi_far_settings &fs = t->far;
+    const bool far_on = fs.voxel_pixels > 0.0f;
+    uint32_t size = fs.cell != 0u ? fs.cell : (uint32_t)std::lround((double)((float)camera->height / 480.0f));
+    size = std::min(std::max(size, 1u), 4u);
+    const uint32_t cells_x = (camera->width + size - 1u) / size, cells_y = (camera->height + size - 1u) / size;
+    if (far_on && (cells_x != t->far_cx || cells_y != t->far_cy || size != t->far_size)) {
+        // A new grid starts with empty cells (PixelState::new).
+        const size_t cells = (size_t)cells_x * cells_y;
+        for (auto &b : t->f_cells) {
+            HIP_OK(b.reserve(cells * sizeof(vgi::PixelHistory)));
+            vgi::launch_fill_words(b.as<uint32_t>(), cells * sizeof(vgi::PixelHistory) / 4, 0u, t->stream);
+        }
+        HIP_OK(t->f_output.reserve((16 + cells * 4) * 4));
+        vgi::launch_fill_words(t->f_output.as<uint32_t>(), 16 + cells * 4, 0u, t->stream);
+        HIP_OK(t->f_queue.reserve((1 + cells) * 4));
+        t->far_cx = cells_x; t->far_cy = cells_y; t->far_size = size;
+    }
     if (staged_upload(*t, t->v_depth.ptr, depth, (size_t)pixels * 4) != 0) return -1;
     if (staged_upload(*t, t->v_normals.ptr, normals, (size_t)pixels * 12) != 0) return -1;
     vgi::launch_fill_words(t->v_hash[g].as<uint32_t>(), capacity, 0u, t->stream);
     vgi::launch_fill_words(t->v_queue.as<uint32_t>(), 2, 0u, t->stream);
+    vgi::launch_fill_words(t->v_counters.as<uint32_t>(), 16, 0u, t->stream);
+    if (far_on) vgi::launch_fill_words(t->f_queue.as<uint32_t>(), 1, 0u, t->stream);
     vgi::SceneView s;
     vgi::Optics o;
     views(*t, s, o);
+    // params.transport.y of the view scene (gpu/view.rs:104-117): some light
+    // has custom visible rgb, which no host light draws.
+    uint32_t custom = 0u;
+    for (size_t l = 0; l + 12 <= t->lights.size(); l += 12)
+        if (t->lights[l + 9] != 0.0f || t->lights[l + 10] != 0.0f || t->lights[l + 11] != 0.0f) custom = 1u;
     vgi::ViewParams p;
     std::memcpy(p.m, camera->world_from_clip, sizeof(p.m));
     p.cam[0] = camera->position[0]; p.cam[1] = camera->position[1]; p.cam[2] = camera->position[2];
@@ -2532,6 +2574,29 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     p.retention = std::max(camera->measured_retention, 0.0f);
     p.clock_ms = clock_ms(*t);
     p.exchange = t->resident && t->high_water > 0 ? 1u : 0u;
+    std::memcpy(p.display, t->view_display, sizeof(p.display));
+    p.custom = custom;
+    vgi::FarParams f;
+    // The previous camera's clip_from_world (this frame's when there is none).
+    std::memcpy(f.clip, t->far_prev ? t->far_prev_clip : camera->clip_from_world, sizeof(f.clip));
+    f.axis[0] = camera->forward[0]; f.axis[1] = camera->forward[1]; f.axis[2] = camera->forward[2];
+    f.axis[3] = camera->pixels_per_unit;
+    f.cells_x = cells_x; f.cells_y = cells_y; f.size = size;
+    f.flags = (far_on ? 1u : 0u) | (fs.measure << 1u);
+    f.threshold = fs.voxel_pixels;
+    f.history = fs.history;
+    f.rate = (float)fs.rate;
+    f.previous = t->far_prev ? t->far_prev_frame : 0xffffffffu;
+    f.frame = camera->frame;
+    f.custom = custom;
+    f.steps = fs.steps;
+    f.cells = far_on ? cells_x * cells_y : 0u;
+    const bool have_cells = t->far_size != 0u;
+    vgi::FarBuffers fb;
+    fb.previous = have_cells ? t->f_cells[pg].as<vgi::PixelHistory>() : nullptr;
+    fb.cells = have_cells ? t->f_cells[g].as<vgi::PixelHistory>() : nullptr;
+    fb.output = t->f_output.as<uint32_t>();
+    fb.queue = t->f_queue.as<uint32_t>();
     vgi::SharedCache c;
     c.index = t->d_index.as<uint4>();
     c.state = t->d_state.as<uint32_t>();
@@ -2540,18 +2605,51 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     if (!t->view_ev[0])
         for (auto &e : t->view_ev) HIP_OK(hipEventCreate(&e));
     HIP_OK(hipEventRecord(t->view_ev[0], t->stream));
-    vgi::launch_view_frame(p, s, o, c, t->v_depth.as<float>(), t->v_normals.as<float>(), t->v_receivers[g].as<vgi::ViewReceiver>(),
+    vgi::launch_view_frame(p, f, s, o, c, t->v_depth.as<float>(), t->v_normals.as<float>(), t->v_receivers[g].as<vgi::ViewReceiver>(),
                            t->v_estimates[g].as<vgi::ViewEstimate>(), t->v_hash[g].as<uint32_t>(), t->v_owners[g].as<uint32_t>(),
                            history ? t->v_receivers[pg].as<vgi::ViewReceiver>() : nullptr,
                            history ? t->v_estimates[pg].as<vgi::ViewEstimate>() : nullptr,
-                           history ? t->v_hash[pg].as<uint32_t>() : nullptr, t->v_queue.as<uint32_t>(), t->stream, t->view_ev[1],
-                           t->view_ev[2]);
-    HIP_OK(hipEventRecord(t->view_ev[3], t->stream));
+                           history ? t->v_hash[pg].as<uint32_t>() : nullptr, t->v_queue.as<uint32_t>(),
+                           t->v_pixels.as<vgi::ViewPixel>(), t->v_counters.as<uint32_t>(), fb, t->stream, t->view_ev[1],
+                           t->view_ev[2], t->view_ev[3]);
+    HIP_OK(hipEventRecord(t->view_ev[4], t->stream));
     HIP_OK(hipGetLastError());
     t->view_pixels[g] = pixels;
     t->view_mask[g] = capacity - 1u;
     t->view_next = pg;
     t->view_history = true;
+    t->far_last_cells = f.cells;
+    std::memcpy(t->far_prev_clip, camera->clip_from_world, sizeof(t->far_prev_clip));
+    t->far_prev_frame = camera->frame;
+    t->far_prev = true;
+    return 0;
+}
+
+int vgi_view_far(vgi_tracer *t, const vgi_far_settings *settings) {
+    if (!t) return fail("null tracer");
+    if (!settings) {
+        t->far.voxel_pixels = 0.0f;
+        return 0;
+    }
+    vgi_far_settings f = *settings;
+    if (!(f.voxel_pixels >= 0.0f) || f.voxel_pixels > 4096.0f) return fail("voxel_pixels outside [0, 4096]");
+    if (f.measure > 2u) return fail("measure is 0, 1 or 2");
+    if (f.cell > 4u) return fail("cell is 0 (follow the height) or 1-4 pixels");
+    if (!(f.history >= 1.0f && f.history <= 1024.0f)) return fail("history outside [1, 1024]");
+    if (f.rate < 1u || f.rate > 8u) return fail("rate outside [1, 8]");
+    if (f.steps < 1u || f.steps > 2048u) return fail("steps outside [1, 2048]");
+    t->far = f;
+    return 0;
+}
+
+int vgi_view_display(vgi_tracer *t, float noise_threshold, float spatial, float blend) {
+    if (!t) return fail("null tracer");
+    if (!(noise_threshold >= 0.0f && noise_threshold <= 1.0f)) return fail("noise threshold outside [0, 1]");
+    if (!(spatial >= 0.0f && spatial <= 1.0f)) return fail("spatial strength outside [0, 1]");
+    if (!(blend > 0.0f && blend <= 1.0f)) return fail("temporal blend outside (0, 1]");
+    t->view_display[0] = noise_threshold;
+    t->view_display[1] = spatial;
+    t->view_display[2] = blend;
     return 0;
 }
 
@@ -2561,20 +2659,58 @@ static int view_last(const vgi_tracer &t) {
     return t.view_history && t.view_pixels[g] > 0u ? (int)g : -1;
 }
 
-int vgi_view_timings(vgi_tracer *t, float out[2]) {
+int vgi_view_timings(vgi_tracer *t, float out[3]) {
     if (!t || !out) return fail("null argument");
     if (view_last(*t) < 0 || !t->view_ev[0]) return fail("no view frame has run");
     HIP_OK(hipSetDevice(t->device));
-    HIP_OK(hipEventSynchronize(t->view_ev[3]));
-    HIP_OK(hipEventElapsedTime(&out[0], t->view_ev[0], t->view_ev[3]));
+    HIP_OK(hipEventSynchronize(t->view_ev[4]));
+    HIP_OK(hipEventElapsedTime(&out[0], t->view_ev[0], t->view_ev[4]));
     HIP_OK(hipEventElapsedTime(&out[1], t->view_ev[1], t->view_ev[2]));
+    HIP_OK(hipEventElapsedTime(&out[2], t->view_ev[3], t->view_ev[4]));
     return 0;
 }
 
 int vgi_view_reset(vgi_tracer *t) {
     if (!t) return fail("null tracer");
     t->view_history = false;
+    t->far_prev = false;
     return 0;
+}
+
+int vgi_view_read_pixels(vgi_tracer *t, uint32_t first, uint32_t count, vgi_view_pixel *out) {
+    if (!t || (count && !out)) return fail("null argument");
+    int g = view_last(*t);
+    if (g < 0) return fail("no view frame has run");
+    if ((uint64_t)first + count > t->view_pixels[g]) return fail("pixel range beyond the last view frame's prepass");
+    if (count == 0) return 0;
+    static_assert(sizeof(vgi_view_pixel) == sizeof(vgi::ViewPixel), "view pixel record layout");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, t->v_pixels.as<vgi::ViewPixel>() + first, (size_t)count * sizeof(vgi::ViewPixel), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vgi_view_read_counters(vgi_tracer *t, uint32_t out[16]) {
+    if (!t || !out) return fail("null argument");
+    if (view_last(*t) < 0) return fail("no view frame has run");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    HIP_OK(hipMemcpy(out, t->v_counters.ptr, 16 * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int64_t vgi_view_read_far(vgi_tracer *t, uint32_t *output, uint64_t output_words, uint32_t *cells, uint64_t cell_words) {
+    if (!t) return fail("null tracer");
+    if (view_last(*t) < 0) return fail("no view frame has run");
+    const uint64_t n = t->far_last_cells;
+    if (n == 0) return fail("the far tier did not run in the last view frame");
+    if (output && output_words < 16 + 4 * n) return fail("output holds fewer than 16 + 4 words per cell");
+    if (cells && cell_words < 12 * n) return fail("cells hold fewer than 12 words per cell");
+    HIP_OK(hipSetDevice(t->device));
+    HIP_OK(hipStreamSynchronize(t->stream));
+    if (output) HIP_OK(hipMemcpy(output, t->f_output.ptr, (16 + 4 * n) * 4, hipMemcpyDeviceToHost));
+    if (cells) HIP_OK(hipMemcpy(cells, t->f_cells[view_last(*t)].ptr, 12 * n * 4, hipMemcpyDeviceToHost));
+    return (int64_t)n;
 }
 
 int64_t vgi_view_read_owners(vgi_tracer *t, uint32_t *out, uint32_t pixels, uint32_t *overflow) {

@@ -76,7 +76,14 @@ struct RayResult {
 // timed frames count rays only (RayTally), and the parity/roofline launch
 // counts rays, traversal steps and the dependent gathers of the traversal --
 // region headers (16 B), brick-table words (4 B), mixed-brick cell words (4 B).
+// The tally also carries the ray budget of the path tracer's `trace`
+// (voxel_path.wgsl:6-14,239-243): 2048 traversal steps with a ray out of
+// steps unknown (kBudgeted false), or the far tier's budget (BudgetTally):
+// `limit` steps, unloaded regions crossed as empty space, and a ray out of
+// steps a miss.
 struct NoTally {
+    static constexpr bool kBudgeted = false;
+    __device__ uint32_t step_limit() const { return 2048u; }
     __device__ void ray() {}
     __device__ void step() {}
     __device__ void header() {}
@@ -90,6 +97,8 @@ struct RayTally : NoTally {
     __device__ uint32_t ray_count() const { return rays; }
 };
 struct FullTally {
+    static constexpr bool kBudgeted = false;
+    __device__ uint32_t step_limit() const { return 2048u; }
     uint32_t rays = 0u, steps = 0u, headers = 0u, bricks = 0u, cells = 0u;
     __device__ void ray() { rays += 1u; }
     __device__ void step() { steps += 1u; }
@@ -97,6 +106,11 @@ struct FullTally {
     __device__ void brick() { bricks += 1u; }
     __device__ void cell() { cells += 1u; }
     __device__ uint32_t ray_count() const { return rays; }
+};
+struct BudgetTally : NoTally {
+    static constexpr bool kBudgeted = true;
+    uint32_t limit = 128u;
+    __device__ uint32_t step_limit() const { return limit; }
 };
 
 __device__ __forceinline__ bool in_bounds(const SceneView &s, int rx, int ry, int rz) {
@@ -200,7 +214,8 @@ __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 
 }
 
 // voxel_traversal.wgsl:126-219 with visible_geometry = false, max_steps = 2048
-// (the path tracer's `trace`, voxel_path.wgsl:239-243), split into a start
+// (the path tracer's `trace`, voxel_path.wgsl:239-243; a budgeted tally
+// changes both, as the far tier's pipeline constants do), split into a start
 // and single steps so that a lane can take up a new ray whenever its current
 // one ends (the wavefront tracer); trace_ray runs one ray to its end.
 struct RayWalk {
@@ -271,7 +286,7 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
 // look for the real occluders beyond gaps in the loaded set).
 template <class Tally, bool kVisible = false>
 __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_t &status, uint32_t &hit_material, Tally &tally) {
-    if (w.step >= 2048u) {
+    if (w.step >= tally.step_limit()) {
         status = 2u;
         hit_material = NONE;
         return true;
@@ -289,7 +304,7 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
         uint4 h = region_header(s, rx, ry, rz);
         tally.header();
         if (h.x == 0u) {
-            if constexpr (kVisible) {
+            if constexpr (kVisible || Tally::kBudgeted) {
                 f3 n;
                 float skip;
                 advance_box(position, direction, w.inv, rx, ry, rz, 0, 0, 0, 0, 0, 0, 16.0f, n, skip);
@@ -396,6 +411,9 @@ __device__ __forceinline__ RayResult trace_ray(const SceneView &s, f3 origin, f3
     if (ray_start(s, origin, direction, max_distance, w, result, tally)) return result;
     while (!ray_advance<Tally, kVisible>(s, w, result, tally)) {
     }
+    // A budgeted `trace` counts a ray out of steps as a miss (voxel_path.wgsl:241).
+    if constexpr (Tally::kBudgeted && !kVisible)
+        if (result.status == 2u && result.material == NONE) result.status = 0u;
     return result;
 }
 

@@ -22,7 +22,8 @@ use std::ptr::{self, NonNull};
 pub use vgi_sys as sys;
 pub use vgi_sys::{
     vgi_face_result as FaceResult, vgi_face_task as FaceTask, vgi_light as Light, vgi_schedule as Schedule,
-    vgi_transport as Transport, vgi_view_camera as ViewCamera, vgi_view_face as ViewFace, VGI_AIR as AIR, VGI_COUNT_GATHERS as COUNT_GATHERS, VGI_COUNT_RAYS as COUNT_RAYS,
+    vgi_far_settings as FarTierSettings, vgi_transport as Transport, vgi_view_camera as ViewCamera, vgi_view_face as ViewFace,
+    vgi_view_pixel as FarPixel, VGI_AIR as AIR, VGI_COUNT_GATHERS as COUNT_GATHERS, VGI_COUNT_RAYS as COUNT_RAYS,
     VGI_NOALLOC_MAX_FACES as NOALLOC_MAX_FACES, VGI_NO_FACE as NO_FACE, VGI_PAGE_EDGE as PAGE_EDGE,
     VGI_RECORD_HITS as RECORD_HITS,
 };
@@ -427,11 +428,44 @@ impl GpuTracer {
         check(unsafe { sys::vgi_view_frame(self.raw(), camera, depth.as_ptr(), normals.as_ptr()) })
     }
 
-    /// GPU ms of the last view frame: [all its passes, the owners' tracing].
-    pub fn view_timings(&self) -> Result<[f32; 2]> {
-        let mut out = [0f32; 2];
+    /// GPU ms of the last view frame: [all its passes, the owners' tracing, the far tier].
+    pub fn view_timings(&self) -> Result<[f32; 3]> {
+        let mut out = [0f32; 3];
         check(unsafe { sys::vgi_view_timings(self.raw(), out.as_mut_ptr()) })?;
         Ok(out)
+    }
+
+    /// `settings.far_tier` (FarTierSettings, settings.rs:95-160); `None` turns the far tier off.
+    pub fn set_far_tier(&mut self, settings: Option<&FarTierSettings>) -> Result<()> {
+        let p = settings.map_or(ptr::null(), |s| s as *const FarTierSettings);
+        check(unsafe { sys::vgi_view_far(self.raw(), p) })
+    }
+
+    /// `view.display.xyz` (gpu/view/runtime.rs:962-967): the noise threshold,
+    /// the spatial strength and the temporal blend of the presentation.
+    pub fn set_display(&mut self, noise_threshold: f32, spatial: f32, blend: f32) -> Result<()> {
+        check(unsafe { sys::vgi_view_display(self.raw(), noise_threshold, spatial, blend) })
+    }
+
+    /// publish_view's records of pixels first .. first + out.len() of the last view frame.
+    pub fn view_pixels(&self, first: u32, out: &mut [FarPixel]) -> Result<()> {
+        let n = len_u32(out.len(), "view_pixels")?;
+        check(unsafe { sys::vgi_view_read_pixels(self.raw(), first, n, out.as_mut_ptr()) })
+    }
+
+    /// The last view frame's far tier and presentation counters.
+    pub fn view_counters(&self) -> Result<[u32; 16]> {
+        let mut out = [0u32; 16];
+        check(unsafe { sys::vgi_view_read_counters(self.raw(), out.as_mut_ptr()) })?;
+        Ok(out)
+    }
+
+    /// The far tier's output (the material's buffer) and cells of the last
+    /// view frame; returns the number of cells.
+    pub fn far_output(&self, output: &mut [u32], cells: &mut [u32]) -> Result<usize> {
+        check_count(unsafe {
+            sys::vgi_view_read_far(self.raw(), output.as_mut_ptr(), output.len() as u64, cells.as_mut_ptr(), cells.len() as u64)
+        })
     }
 
     /// Forget the view history.

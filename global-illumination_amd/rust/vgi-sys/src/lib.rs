@@ -104,7 +104,14 @@ pub const VGI_NOALLOC_MAX_FACES: u32 = 0x40000;
 /// the tracer's earlier work. Not covered: coarse LOD blocks, the traced
 /// discovery fallback for foreign meshes (such pixels own no receiver),
 /// pending geometry, corner sun words and sun flips of view faces, topology
-/// priors and presentation (shown), the far per-pixel tier.
+/// priors. After tracing, the presentation (reconstruct_view,
+/// voxel_view_runtime.wgsl:350-474) pools coplanar owners into each face's
+/// shown irradiance and the frame publishes one record per pixel
+/// (publish_view :476-521). With the far tier on (vgi_view_far), faces small on
+/// screen own no receiver and are lit per pixel instead
+/// (voxel_view_pixels.wgsl): cells of size x size pixels sample one pixel a
+/// frame, reproject their history by world position and trace one path when
+/// due.
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default, PartialEq)]
 pub struct vgi_view_camera {
@@ -115,6 +122,10 @@ pub struct vgi_view_camera {
     pub height: u32,
     pub samples: u32,
     pub measured_retention: f32,
+    pub clip_from_world: [f32; 16],
+    pub forward: [f32; 3],
+    pub pixels_per_unit: f32,
+    pub frame: u32,
 }
 
 /// One prepass pixel's receiver and, for an owner, its face's estimate.
@@ -129,6 +140,31 @@ pub struct vgi_view_face {
     pub moment: [f32; 4],
     pub identity: [u32; 4],
     pub direct: [f32; 4],
+    pub shown: [f32; 4],
+}
+
+/// publish_view's record per pixel (FarPixel, voxel_view_runtime.wgsl:40-46).
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default, PartialEq)]
+pub struct vgi_view_pixel {
+    pub surface: [f32; 3],
+    pub face: u32,
+    pub irradiance: [f32; 3],
+    pub evidence: f32,
+    pub sky: [u32; 4],
+}
+
+/// The far tier (FarTierSettings, settings.rs:95-160; gpu/view/runtime.rs:
+/// 912-1000). All zero (the default at vgi_create) is off.
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default, PartialEq)]
+pub struct vgi_far_settings {
+    pub voxel_pixels: f32,
+    pub measure: u32,
+    pub cell: u32,
+    pub history: f32,
+    pub rate: u32,
+    pub steps: u32,
 }
 
 pub const VGI_VIEW_MAX_PIXELS: u32 = 0x1000000;
@@ -345,8 +381,37 @@ extern "C" {
 
     pub fn vgi_view_frame(tracer: *mut vgi_tracer, camera: *const vgi_view_camera, depth: *const f32, normals: *const f32) -> c_int;
 
-    /// GPU ms of the last view frame: [all its passes, the owners' tracing].
+    /// GPU ms of the last view frame: [all its passes, the owners' tracing, the far tier].
     pub fn vgi_view_timings(tracer: *mut vgi_tracer, out: *mut f32) -> c_int;
+
+    /// The far tier's settings for the next view frames (the renderer's
+    /// settings.far_tier, normalized; a NULL pointer or voxel_pixels 0 turns it off).
+    pub fn vgi_view_far(tracer: *mut vgi_tracer, settings: *const vgi_far_settings) -> c_int;
+
+    /// The presentation's settings (view.display.xyz, gpu/view/runtime.rs:962-967):
+    /// the relative standard error under which a face is shown alone
+    /// (spatial_noise_threshold, default 0.031), the spatial reconstruction
+    /// strength (default 1), the temporal blend 1 - exp(-frame seconds / display
+    /// response) (default 0.1993: 60 frames a second, 75 ms).
+    pub fn vgi_view_display(tracer: *mut vgi_tracer, noise_threshold: f32, spatial: f32, blend: f32) -> c_int;
+
+    /// Published records of pixels [first, first + count) of the last view frame.
+    pub fn vgi_view_read_pixels(tracer: *mut vgi_tracer, first: u32, count: u32, out: *mut vgi_view_pixel) -> c_int;
+
+    /// The last view frame's counters: far tier (owned, traced, history, reprojected,
+    /// seeded from receivers, receivers seeded from cells, 0, 0; voxel_view_pixels.wgsl:34-39),
+    /// presentation (history, max history, uncovered, displayed, 0, 0, 0, 0;
+    /// voxel_view_runtime.wgsl:55-58).
+    pub fn vgi_view_read_counters(tracer: *mut vgi_tracer, out: *mut u32) -> c_int;
+
+    /// The far tier's output of the last view frame (the material's buffer: 16
+    /// header words -- camera, axis, grid (cells x, y, size, flags), origin
+    /// (0, 0, threshold bits, stamp) -- then per cell half rg, half (b, count),
+    /// face | stamp << 3, surface coordinate along the normal axis) and the
+    /// frame's cells (12 words each: mean rgb count, surface, face | stamp << 3,
+    /// sequence, 0, 0, 0). Cells not written this frame keep older contents (the
+    /// stamp tells). Either pointer may be NULL. Returns the number of cells.
+    pub fn vgi_view_read_far(tracer: *mut vgi_tracer, output: *mut u32, output_words: u64, cells: *mut u32, cell_words: u64) -> i64;
 
     /// Forget the view history (the next frame reprojects nothing).
     pub fn vgi_view_reset(tracer: *mut vgi_tracer) -> c_int;

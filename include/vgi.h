@@ -258,7 +258,14 @@ int vgi_read_sun_corners(vgi_tracer *tracer, uint32_t first, uint32_t count, uin
  * the tracer's earlier work. Not covered: coarse LOD blocks, the traced
  * discovery fallback for foreign meshes (such pixels own no receiver),
  * pending geometry, corner sun words and sun flips of view faces, topology
- * priors and presentation (shown), the far per-pixel tier. ---- */
+ * priors. After tracing, the presentation (reconstruct_view,
+ * voxel_view_runtime.wgsl:350-474) pools coplanar owners into each face's
+ * shown irradiance and the frame publishes one record per pixel
+ * (publish_view :476-521). With the far tier on (vgi_view_far), faces small on
+ * screen own no receiver and are lit per pixel instead
+ * (voxel_view_pixels.wgsl): cells of size x size pixels sample one pixel a
+ * frame, reproject their history by world position and trace one path when
+ * due. ---- */
 typedef struct vgi_view_camera {
     float world_from_clip[16]; /* column-major: element (row r, column c) at [4 c + r]; clip y up, depth z > 0 valid */
     float position[3];         /* camera position (world) */
@@ -267,6 +274,10 @@ typedef struct vgi_view_camera {
     uint32_t height;           /* prepass rows */
     uint32_t samples;          /* paths per visible face this frame (<= 1024) */
     float measured_retention;  /* the keep rule's c (settings.rs:262) */
+    float clip_from_world[16]; /* this frame's world -> clip (the next frame reprojects far cells with it), column-major */
+    float forward[3];          /* view forward (world, unit): the far tier's distance test */
+    float pixels_per_unit;     /* prepass pixels per world unit at view depth 1 (clip_from_view[1][1] * height / 2) */
+    uint32_t frame;            /* frame number: far cells are stamped with it (gpu/view/runtime.rs:994-1002) */
 } vgi_view_camera;
 /* One prepass pixel's receiver and, for an owner, its face's estimate. */
 typedef struct vgi_view_face {
@@ -278,11 +289,54 @@ typedef struct vgi_view_face {
     float moment[4];           /* raw second moment rgb, pending predicted change */
     uint32_t identity[4];      /* next sequence, material, corner word (0), resident incarnation (0: not admitted) */
     float direct[4];           /* direct rgb at the face centre, 0 */
+    float shown[4];            /* presented rgb (reconstructed), valid: 1 (0: never shown) */
 } vgi_view_face;
+/* publish_view's record per pixel (FarPixel, voxel_view_runtime.wgsl:40-46). */
+typedef struct vgi_view_pixel {
+    float surface[3];          /* owner's face centre */
+    uint32_t face;             /* 0xffffffff: the pixel is no face's owner */
+    float irradiance[3];       /* shown rgb plus custom direct rgb */
+    float evidence;            /* the sample weight behind it (>= 0.001), 0: nothing to show */
+    uint32_t sky[4];           /* corner sun words: lit lights (bits 0-3), known (bit 31) */
+} vgi_view_pixel;
+/* The far tier (FarTierSettings, settings.rs:95-160; gpu/view/runtime.rs:
+ * 912-1000). All zero (the default at vgi_create) is off. */
+typedef struct vgi_far_settings {
+    float voxel_pixels;        /* a face is far under this many pixels across (measure 0) or covered (1, 2); 0: off */
+    uint32_t measure;          /* 0 voxel side, 1 voxel area, 2 block area (unit voxels: as 1) */
+    uint32_t cell;             /* pixels per cell side 1-4; 0: round(height / 480) clamped to 1-4 */
+    float history;             /* history cap in samples (1-1024) */
+    uint32_t rate;             /* once a cell holds 4 samples, it traces one frame in `rate` (1-8) */
+    uint32_t steps;            /* traversal steps per far ray (1-2048); a ray out of steps is a miss */
+} vgi_far_settings;
 #define VGI_VIEW_MAX_PIXELS 16777216u
 int vgi_view_frame(vgi_tracer *tracer, const vgi_view_camera *camera, const float *depth, const float *normals);
-/* GPU ms of the last view frame: [all its passes, the owners' tracing]. */
-int vgi_view_timings(vgi_tracer *tracer, float out[2]);
+/* GPU ms of the last view frame: [all its passes, the owners' tracing, the far tier]. */
+int vgi_view_timings(vgi_tracer *tracer, float out[3]);
+/* The far tier's settings for the next view frames (the renderer's
+ * settings.far_tier, normalized; a NULL pointer or voxel_pixels 0 turns it off). */
+int vgi_view_far(vgi_tracer *tracer, const vgi_far_settings *settings);
+/* The presentation's settings (view.display.xyz, gpu/view/runtime.rs:962-967):
+ * the relative standard error under which a face is shown alone
+ * (spatial_noise_threshold, default 0.031), the spatial reconstruction
+ * strength (default 1), the temporal blend 1 - exp(-frame seconds / display
+ * response) (default 0.1993: 60 frames a second, 75 ms). */
+int vgi_view_display(vgi_tracer *tracer, float noise_threshold, float spatial, float blend);
+/* Published records of pixels [first, first + count) of the last view frame. */
+int vgi_view_read_pixels(vgi_tracer *tracer, uint32_t first, uint32_t count, vgi_view_pixel *out);
+/* The last view frame's counters: far tier (owned, traced, history, reprojected,
+ * seeded from receivers, receivers seeded from cells, 0, 0; voxel_view_pixels.wgsl:34-39),
+ * presentation (history, max history, uncovered, displayed, 0, 0, 0, 0;
+ * voxel_view_runtime.wgsl:55-58). */
+int vgi_view_read_counters(vgi_tracer *tracer, uint32_t out[16]);
+/* The far tier's output of the last view frame (the material's buffer: 16
+ * header words -- camera, axis, grid (cells x, y, size, flags), origin
+ * (0, 0, threshold bits, stamp) -- then per cell half rg, half (b, count),
+ * face | stamp << 3, surface coordinate along the normal axis) and the
+ * frame's cells (12 words each: mean rgb count, surface, face | stamp << 3,
+ * sequence, 0, 0, 0). Cells not written this frame keep older contents (the
+ * stamp tells). Either pointer may be NULL. Returns the number of cells. */
+int64_t vgi_view_read_far(vgi_tracer *tracer, uint32_t *output, uint64_t output_words, uint32_t *cells, uint64_t cell_words);
 /* Forget the view history (the next frame reprojects nothing). */
 int vgi_view_reset(vgi_tracer *tracer);
 /* The last view frame's owner per pixel (pixel index of the face's owner,

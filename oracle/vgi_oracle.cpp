@@ -17,7 +17,11 @@
 // traversal against a brute-force voxel walk; the estimator against the
 // furnace closed forms E*sum(r^k) and E/(1-r) (gpu/view_tests.rs:387-459,
 // gpu/cache_tests.rs:43-101); emitter tables against emission.rs:401-432;
-// retention against retention.rs:346-376.
+// retention against retention.rs:346-376; the far tier against the
+// reference's native far-tier tests (gpu/view/pixel_tests.rs:344-544) and
+// its half packing against IEEE binary16; the presentation against closed
+// forms (own mean, the temporal blend) and the convex-combination invariant
+// of its pooling.
 #include "vgi_oracle.h"
 
 #include <algorithm>
@@ -1057,6 +1061,23 @@ struct ViewFace {
     float mean[4], moment[4];
     uint32_t identity[4];
     float direct[4];
+    float shown[4];
+};
+// One far cell (voxel_view_pixels.wgsl:11-22, PixelHistory): 12 words.
+struct FarCell {
+    float mean[4];
+    float surface[3];
+    uint32_t face;
+    uint32_t sequence;
+    uint32_t pad[3];
+};
+// publish_view's record (FarPixel, voxel_view_runtime.wgsl:40-46): 12 words.
+struct ViewPixelRecord {
+    float surface[3];
+    uint32_t face;
+    float irradiance[3];
+    float evidence;
+    uint32_t sky[4];
 };
 
 struct Tracer {
@@ -1105,6 +1126,21 @@ struct Tracer {
     std::vector<ViewFace> view_faces;
     std::map<std::tuple<int, int, int, uint32_t>, uint32_t> view_owner;
     bool view_history = false;
+    // Presentation settings and the last frame's published records and counters.
+    float view_display[3] = {0.031f, 1.0f, 0.19926f};
+    std::vector<ViewPixelRecord> view_published;
+    uint32_t view_counters[16] = {};
+    // The far tier: settings, two generations of cells (alternating with the
+    // view frames), the material's output, the last camera's clip_from_world
+    // and frame number.
+    orc_far_settings far{0.0f, 0u, 0u, 32.0f, 4u, 128u};
+    std::vector<FarCell> far_cells[2];
+    std::vector<uint32_t> far_output;
+    uint32_t far_cx = 0, far_cy = 0, far_size = 0, far_last_cells = 0;
+    uint32_t view_generation = 0;
+    float far_prev_clip[16] = {};
+    uint32_t far_prev_frame = 0;
+    bool far_prev = false;
 };
 
 // ---------------------------------------------------------------- traversal (voxel_traversal.wgsl)
@@ -1300,9 +1336,16 @@ struct Estimator {
     }
     float extra_w() const { return (float)nfaces; }
 
-    // voxel_path.wgsl:239-243 (BUDGETED_RAYS false, PATH_STEPS 2048)
+    // voxel_path.wgsl:239-243: BUDGETED_RAYS false and PATH_STEPS 2048 for the
+    // world and view tiers; the far tier's pipeline constants set both
+    // (gpu/view/runtime.rs:564-576): `steps` traversal steps, unloaded
+    // regions crossed as empty space, a ray out of steps a miss.
+    bool budgeted = false;
+    uint32_t path_steps = 2048u;
     RayResult trace(F3 origin, F3 direction, float max_distance) const {
-        return trace_domain_limited(p, origin, direction, max_distance, false, 2048u, cn);
+        RayResult r = trace_domain_limited(p, origin, direction, max_distance, budgeted, path_steps, cn);
+        if (budgeted && r.status == 2u && r.material == NONE) r.status = 0u;
+        return r;
     }
     float connection_reach() const { return t.transport.limited_reach ? t.transport.reach : t.transport.max_distance; }
     // voxel_path.wgsl:271-282
@@ -3628,9 +3671,31 @@ static bool view_receiver_exists(const Tracer &t, I3 v, uint32_t face) {
     return view_material(t, o) == AIR;
 }
 
+// The far tier's per-frame constants (view.pixel_axis, pixel_grid,
+// pixel_tune, previous_clip_from_world; gpu/view/runtime.rs:982-1002).
+struct FarFrame {
+    float clip[16];
+    float axis[4];
+    uint32_t cells_x, cells_y, size, flags;
+    float threshold, history, rate;
+    uint32_t previous, frame, custom, steps, cells;
+};
+
+// pixel_owned (voxel_view_pixels.wgsl:54-64), unit voxels.
+static bool far_pixel_owned(const orc_view_camera &c, const FarFrame &f, I3 v, uint32_t face) {
+    if ((f.flags & 1u) == 0u || face >= 6u) return false;
+    const uint32_t mode = (f.flags >> 1u) & 3u;
+    const F3 ray = f3((float)v.x + 0.5f, (float)v.y + 0.5f, (float)v.z + 0.5f) - f3(c.position[0], c.position[1], c.position[2]);
+    if (dot3(ray, f3(f.axis[0], f.axis[1], f.axis[2])) <= 0.0f) return false;
+    const float side = f.axis[3] / length3(ray);
+    if (mode == 0u) return side < f.threshold;
+    return side * side * std::fabs(dot3(face_normal(face), normalize3(ray))) < f.threshold;
+}
+
 // prepass_voxel_receiver (voxel_view_discovery.wgsl:40-104) without the
 // traced fallback and pending geometry: false when the pixel owns no face.
-static bool view_discover(const Tracer &t, const orc_view_camera &c, uint32_t i, float z, F3 normal_in, I3 &voxel, uint32_t &face) {
+static bool view_discover(const Tracer &t, const orc_view_camera &c, uint32_t i, float z, F3 normal_in, I3 &voxel, uint32_t &face,
+                          const FarFrame &f) {
     if (!(z > 0.0f)) return false;
     float px = (float)(i % c.width), py = (float)(i / c.width);
     F3 position = view_unproject(c, px, py, z);
@@ -3644,6 +3709,8 @@ static bool view_discover(const Tracer &t, const orc_view_camera &c, uint32_t i,
     F3 surface = f3(sv[0], sv[1], sv[2]);
     F3 inside = surface - n * 0.01f;
     I3 v{(int32_t)std::floor(inside.x), (int32_t)std::floor(inside.y), (int32_t)std::floor(inside.z)};
+    // Faces small enough for the far tier own no receiver (voxel_view_discovery.wgsl:59-64).
+    if (far_pixel_owned(c, f, v, face)) return false;
     if (view_receiver_exists(t, v, face)) { voxel = v; return true; }
     float footprint = view_distance(view_unproject(c, px + 1.0f, py, z), position);
     uint32_t u = (axis + 1u) % 3u, w = (axis + 2u) % 3u;
@@ -3674,12 +3741,231 @@ static uint32_t view_shared_slot(const Tracer &t, I3 v, uint32_t face, uint32_t 
     return slot;
 }
 
+// pack2x16float (voxel_view_pixels.wgsl:79) as IEEE binary16 with
+// round-to-nearest-even (the WGSL rounding is the driver's; AMD's Vulkan
+// driver truncates, gpu/view/pixel_tests.rs:359-361).
+static uint16_t half_bits(float x) {
+    const uint32_t u = fbits(x);
+    const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    const uint32_t e = (u >> 23) & 255u, m = u & 0x7fffffu;
+    if (e == 255u) return (uint16_t)(sign | 0x7c00u | (m ? 0x200u : 0u));
+    if (e == 0u) return sign;  // binary32 zero or subnormal: below binary16's least half-ulp
+    const int ex = (int)e - 112;  // the binary16 exponent field of a normal result
+    const uint32_t full = m | 0x800000u;
+    const uint32_t shift = ex >= 1 ? 13u : (uint32_t)(14 - ex);  // significand bits dropped
+    if (shift >= 25u) return sign;  // under half the least subnormal
+    uint32_t kept = full >> shift;
+    const uint32_t rest = full & ((1u << shift) - 1u), halfway = 1u << (shift - 1u);
+    if (rest > halfway || (rest == halfway && (kept & 1u))) kept += 1u;
+    if (ex >= 1) {
+        uint32_t exponent = (uint32_t)ex;
+        if (kept == 0x800u) { kept = 0x400u; exponent += 1u; }
+        if (exponent >= 31u) return (uint16_t)(sign | 0x7c00u);
+        return (uint16_t)(sign | (exponent << 10) | (kept & 0x3ffu));
+    }
+    return (uint16_t)(sign | kept);  // a subnormal, or 0x400: rounded up to the least normal
+}
+
+uint32_t orc_pack_half2(float lo, float hi) { return (uint32_t)half_bits(lo) | ((uint32_t)half_bits(hi) << 16); }
+
+static inline uint32_t pixel_stamp(uint32_t frame) { return frame & 0x1fffffffu; }
+
+// previous_cell (voxel_view_pixels.wgsl:83-127): bilinear over the previous
+// frame's cells around the point's old screen position, same face and stamp,
+// within four voxels along the normal and two cells' footprint across.
+static void far_previous_cell(const orc_view_camera &c, const FarFrame &f, const std::vector<FarCell> *previous, F3 surface,
+                              uint32_t face, float found[4], uint32_t &sequence) {
+    found[0] = found[1] = found[2] = found[3] = 0.0f;
+    sequence = 0u;
+    if (f.previous == NONE || !previous || previous->empty()) return;
+    float h[4];
+    for (int i = 0; i < 4; ++i) h[i] = ((f.clip[i] * surface.x + f.clip[4 + i] * surface.y) + f.clip[8 + i] * surface.z) + f.clip[12 + i];
+    if (h[3] <= 0.0f) return;
+    const F3 normal = face_normal(face);
+    const float ux = h[0] / h[3] * 0.5f + 0.5f, uy = h[1] / h[3] * -0.5f + 0.5f;
+    const float qx = ux * (float)c.width / (float)f.size - 0.5f, qy = uy * (float)c.height / (float)f.size - 0.5f;
+    const float bx = std::floor(qx), by = std::floor(qy);
+    // Off screen or not a number: no cell.
+    if (!(bx >= -1.0f && by >= -1.0f && bx < (float)f.cells_x && by < (float)f.cells_y)) return;
+    const float fx = qx - bx, fy = qy - by;
+    const float reach = 2.0f * (float)f.size * length3(surface - f3(c.position[0], c.position[1], c.position[2])) / f.axis[3];
+    const float tolerance = 4.0f * 1.0f + 0.01f;
+    const uint32_t expected = face | (pixel_stamp(f.previous) << 3u);
+    float weight = 0.0f, samples = 0.0f, best = 0.0f;
+    F3 sum = f3(0, 0, 0);
+    for (int y = 0; y <= 1; ++y)
+        for (int x = 0; x <= 1; ++x) {
+            const int cx = (int)bx + x, cy = (int)by + y;
+            if (cx < 0 || cy < 0 || cx >= (int)f.cells_x || cy >= (int)f.cells_y) continue;
+            const FarCell &old = (*previous)[(size_t)cy * f.cells_x + cx];
+            if (old.face != expected || old.mean[3] == 0.0f) continue;
+            const F3 delta = f3(old.surface[0], old.surface[1], old.surface[2]) - surface;
+            if (std::fabs(dot3(delta, normal)) > tolerance || dot3(delta, delta) > reach * reach) continue;
+            const float w = (x == 1 ? fx : 1.0f - fx) * (y == 1 ? fy : 1.0f - fy) + 1e-4f;
+            sum = sum + f3(old.mean[0], old.mean[1], old.mean[2]) * w;
+            samples = samples + old.mean[3] * w;
+            weight = weight + w;
+            if (w > best) { best = w; sequence = old.sequence; }
+        }
+    if (weight > 0.0f) {
+        found[0] = sum.x / weight; found[1] = sum.y / weight; found[2] = sum.z / weight; found[3] = samples / weight;
+    }
+}
+
+// write_output (voxel_view_pixels.wgsl:71-81) with custom_direct (:71-75).
+static void far_write_output(const Tracer &t, const FarFrame &f, uint32_t index, const FarCell &cell) {
+    const uint32_t face = cell.face & 7u;
+    F3 shown = f3(cell.mean[0], cell.mean[1], cell.mean[2]);
+    if (f.custom) {
+        Estimator est(t, nullptr);
+        est.budgeted = true;
+        est.path_steps = f.steps;
+        F3 dv, dc;
+        est.direct_at(f3(cell.surface[0], cell.surface[1], cell.surface[2]), face_normal(face), dv, dc);
+        shown = shown + dc;
+    }
+    uint32_t *w = const_cast<uint32_t *>(&t.far_output[16 + (size_t)index * 4]);
+    w[0] = orc_pack_half2(shown.x, shown.y);
+    w[1] = orc_pack_half2(shown.z, cell.mean[3]);
+    w[2] = cell.face;
+    w[3] = fbits(cell.surface[face / 2u]);
+}
+
+// mean_variance (voxel_view_runtime.wgsl:380-394).
+static float luminance4(const float v[4]) { return (v[0] * 0.2126f + v[1] * 0.7152f) + v[2] * 0.0722f; }
+static float view_mean_variance(const float mean[4], const float moment[4], float reference) {
+    const float n = fmax_(mean[3], 1.0f);
+    const float l = luminance4(mean);
+    const float cx = fmax_(moment[0] - mean[0] * mean[0], 0.0f), cy = fmax_(moment[1] - mean[1] * mean[1], 0.0f),
+                cz = fmax_(moment[2] - mean[2] * mean[2], 0.0f);
+    const float deviation = (std::sqrt(cx) * 0.2126f + std::sqrt(cy) * 0.7152f) + std::sqrt(cz) * 0.0722f;
+    const float prior = l * l;
+    const float samples = fmax_(n - 1.0f, 0.0f);
+    const float shrunk = (deviation * deviation * samples + prior * 64.0f) / (samples + 64.0f);
+    const float unseen = reference / n;
+    return shrunk / n + unseen * unseen;
+}
+
+// coplanar_donor (voxel_view_runtime.wgsl:396-404), unit voxels.
+static bool view_coplanar_donor(const ViewFace &r, F3 normal, const std::vector<ViewFace> &cur, const std::vector<uint32_t> &owners,
+                                uint32_t candidate) {
+    const ViewFace &d = cur[candidate];
+    if (d.face != r.face) return false;
+    const F3 delta = f3(d.surface[0], d.surface[1], d.surface[2]) - f3(r.surface[0], r.surface[1], r.surface[2]);
+    const uint32_t u = (r.face / 2u + 1u) % 3u;
+    const float stride = 2.0f * (r.surface[u] - (float)r.voxel[u]);
+    const float reach = 16.0f * stride;
+    return std::fabs(dot3(delta, normal)) <= 0.05f && dot3(delta, delta) <= reach * reach && owners[candidate] == candidate;
+}
+
+// The members of 2x2-pixel tile (tx, ty) in pixel order.
+static void view_tile_members(const orc_view_camera &c, int tx, int ty, const std::function<void(uint32_t)> &visit) {
+    for (int b = 0; b < 2; ++b) {
+        const int y = ty * 2 + b;
+        if (y >= (int)c.height) break;
+        for (int a = 0; a < 2; ++a) {
+            const int x = tx * 2 + a;
+            if (x >= (int)c.width) break;
+            visit((uint32_t)y * c.width + (uint32_t)x);
+        }
+    }
+}
+
+// reconstruct_view (voxel_view_runtime.wgsl:406-474) for one owner.
+static void view_reconstruct(const Tracer &t, const orc_view_camera &c, std::vector<ViewFace> &cur, const std::vector<uint32_t> &owners,
+                             uint32_t i) {
+    const ViewFace &r = cur[i];
+    const ViewFace center = cur[i];
+    const F3 normal = face_normal(r.face);
+    const int tiles_x = (int)((c.width + 1u) / 2u), tiles_y = (int)((c.height + 1u) / 2u);
+    const int tx = (int)((i % c.width) / 2u), ty = (int)((i / c.width) / 2u);
+    const float center_luminance = luminance4(center.mean);
+    float reference = center_luminance;
+    for (int y = -1; y <= 1; ++y)
+        for (int x = -1; x <= 1; ++x) {
+            const int qx = tx + x, qy = ty + y;
+            if (qx < 0 || qy < 0 || qx >= tiles_x || qy >= tiles_y) continue;
+            view_tile_members(c, qx, qy, [&](uint32_t k) {
+                if (k == i || !view_coplanar_donor(r, normal, cur, owners, k)) return;
+                if (cur[k].mean[3] > 0.0f) reference = fmax_(reference, luminance4(cur[k].mean));
+            });
+        }
+    const float center_variance = view_mean_variance(center.mean, center.moment, reference);
+    const float tolerance = t.view_display[0] * center_luminance;
+    const float deficit = center_variance / fmax_(tolerance * tolerance, 1e-12f);
+    const float rf = std::ceil(std::sqrt(deficit / 3.14159f) * 0.5f);
+    const int radius = (int)fmin_(fmax_(rf, 1.0f), 4.0f);
+    const bool unsampled = center.mean[3] <= 0.0f;
+    F3 sum = f3(0, 0, 0);
+    float weight = 0.0f;
+    if (!unsampled) {
+        const float w = 1.0f / fmax_(center_variance, 1e-12f);
+        sum = f3(center.mean[0], center.mean[1], center.mean[2]) * w;
+        weight = w;
+    }
+    if ((unsampled || center_variance > tolerance * tolerance) && t.view_display[1] > 0.0f)
+        for (int y = -radius; y <= radius; ++y)
+            for (int x = -radius; x <= radius; ++x) {
+                const int qx = tx + x, qy = ty + y;
+                if (qx < 0 || qy < 0 || qx >= tiles_x || qy >= tiles_y) continue;
+                view_tile_members(c, qx, qy, [&](uint32_t k) {
+                    if (k == i || !view_coplanar_donor(r, normal, cur, owners, k)) return;
+                    const ViewFace &value = cur[k];
+                    if (value.mean[3] <= 0.0f) return;
+                    const float variance = view_mean_variance(value.mean, value.moment, reference);
+                    float bias2 = 0.0f;
+                    if (!unsampled) {
+                        const float difference = luminance4(value.mean) - center_luminance;
+                        bias2 = fmax_(difference * difference - variance - center_variance, 0.0f);
+                    }
+                    const float w = 1.0f / fmax_(variance + bias2, 1e-12f);
+                    sum = sum + f3(value.mean[0], value.mean[1], value.mean[2]) * w;
+                    weight = weight + w;
+                });
+            }
+    if (weight <= 0.0f) return;
+    const float tt = unsampled ? 0.0f : 1.0f - t.view_display[1];
+    const F3 pooled = f3(sum.x / weight, sum.y / weight, sum.z / weight);
+    const F3 desired = pooled * (1.0f - tt) + f3(center.mean[0], center.mean[1], center.mean[2]) * tt;
+    const float blend = center.shown[3] > 0.0f ? t.view_display[2] : 1.0f;
+    const F3 shown = f3(center.shown[0], center.shown[1], center.shown[2]) * (1.0f - blend) + desired * blend;
+    cur[i].shown[0] = shown.x; cur[i].shown[1] = shown.y; cur[i].shown[2] = shown.z; cur[i].shown[3] = 1.0f;
+}
+
 int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, const float *normals, uint32_t clock_ms, uint32_t *owners_out,
                    void *faces_out, uint32_t *owner_count, int threads) {
     Tracer &t = *(Tracer *)tp;
     tracer_sync(t);
     const orc_view_camera &c = *cam;
     const uint32_t pixels = c.width * c.height;
+    const uint32_t g = t.view_generation, pg = g ^ 1u;
+    // The far tier's frame (gpu/view/runtime.rs:912-1002, settings.rs:141-147).
+    const orc_far_settings &fs = t.far;
+    const bool far_on = fs.voxel_pixels > 0.0f;
+    uint32_t size = fs.cell != 0u ? fs.cell : (uint32_t)std::lround((double)((float)c.height / 480.0f));
+    size = std::min(std::max(size, 1u), 4u);
+    FarFrame f;
+    memcpy(f.clip, t.far_prev ? t.far_prev_clip : c.clip_from_world, sizeof(f.clip));
+    f.axis[0] = c.forward[0]; f.axis[1] = c.forward[1]; f.axis[2] = c.forward[2]; f.axis[3] = c.pixels_per_unit;
+    f.cells_x = (c.width + size - 1u) / size; f.cells_y = (c.height + size - 1u) / size; f.size = size;
+    f.flags = (far_on ? 1u : 0u) | (fs.measure << 1u);
+    f.threshold = fs.voxel_pixels; f.history = fs.history; f.rate = (float)fs.rate;
+    f.previous = t.far_prev ? t.far_prev_frame : NONE;
+    f.frame = c.frame;
+    f.custom = 0u;
+    for (const Light &l : t.lights)
+        if (l.extra[1] != 0.0f || l.extra[2] != 0.0f || l.extra[3] != 0.0f) f.custom = 1u;
+    f.steps = fs.steps;
+    f.cells = far_on ? f.cells_x * f.cells_y : 0u;
+    if (far_on && (f.cells_x != t.far_cx || f.cells_y != t.far_cy || size != t.far_size)) {
+        for (auto &v : t.far_cells) v.assign((size_t)f.cells, FarCell{});
+        t.far_output.assign(16 + (size_t)f.cells * 4, 0u);
+        t.far_cx = f.cells_x; t.far_cy = f.cells_y; t.far_size = size;
+    }
+    const std::vector<FarCell> *previous_cells = t.far_size != 0u ? &t.far_cells[pg] : nullptr;
+    memset(t.view_counters, 0, sizeof(t.view_counters));
+    uint32_t *counters = t.view_counters;
+
     std::vector<ViewFace> cur(pixels);
     std::vector<uint32_t> owners(pixels, NONE);
     std::map<std::tuple<int, int, int, uint32_t>, uint32_t> owner;
@@ -3689,7 +3975,7 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
         r.face = NONE;
         I3 v;
         uint32_t face;
-        if (!view_discover(t, c, i, depth[i], f3(normals[i * 3], normals[i * 3 + 1], normals[i * 3 + 2]), v, face)) continue;
+        if (!view_discover(t, c, i, depth[i], f3(normals[i * 3], normals[i * 3 + 1], normals[i * 3 + 2]), v, face, f)) continue;
         F3 n = face_normal(face);
         F3 centre = to_f3(v) + f3(0.5f, 0.5f, 0.5f) + n * 0.5f;
         r.voxel[0] = v.x; r.voxel[1] = v.y; r.voxel[2] = v.z;
@@ -3711,39 +3997,67 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
         ViewFace &r = cur[i];
         const uint32_t material = r.identity[1];
         // reproject_paths: the previous owner's estimate, same material.
-        float mean[4] = {0, 0, 0, 0}, moment[4] = {0, 0, 0, 0}, direct[4] = {0, 0, 0, 0};
+        float mean[4] = {0, 0, 0, 0}, moment[4] = {0, 0, 0, 0}, direct[4] = {0, 0, 0, 0}, shown[4] = {0, 0, 0, 0};
         uint32_t identity[4] = {0, 0, 0, 0};
         if (t.view_history) {
             auto it = t.view_owner.find(std::make_tuple(r.voxel[0], r.voxel[1], r.voxel[2], r.face));
             if (it != t.view_owner.end() && t.view_faces[it->second].identity[1] == material) {
                 const ViewFace &old = t.view_faces[it->second];
                 memcpy(mean, old.mean, 16); memcpy(moment, old.moment, 16); memcpy(identity, old.identity, 16); memcpy(direct, old.direct, 16);
+                memcpy(shown, old.shown, 16);
             }
         }
         identity[1] = material;
         identity[2] = 0u;
         memcpy(r.mean, mean, 16); memcpy(r.moment, moment, 16); memcpy(r.identity, identity, 16); memcpy(r.direct, direct, 16);
+        memcpy(r.shown, shown, 16);
         r.sequence = r.identity[0];
-        if (!exchange) continue;
-        // load_shared_face
-        I3 v{r.voxel[0], r.voxel[1], r.voxel[2]};
-        uint32_t slot = view_shared_slot(t, v, r.face, material);
-        if (slot == NONE) { r.identity[3] = 0u; continue; }
-        uint32_t *st = &t.state[(size_t)slot * 36];
-        const uint32_t incarnation = st[28], count = st[5] & 0x7fffffffu, next = st[6];
-        if (r.identity[3] == incarnation || r.mean[3] <= (float)count) {
-            r.mean[0] = bitsf(st[0]); r.mean[1] = bitsf(st[1]); r.mean[2] = bitsf(st[2]); r.mean[3] = (float)count;
-            r.moment[0] = bitsf(st[8]); r.moment[1] = bitsf(st[9]); r.moment[2] = bitsf(st[10]);
-            r.moment[3] = bitsf(t.meta[(size_t)slot * 12 + 4]);
-            r.identity[0] = next;
-        } else {
-            r.identity[0] = std::max(r.identity[0], next);
+        if (exchange) {
+            // load_shared_face
+            I3 v{r.voxel[0], r.voxel[1], r.voxel[2]};
+            uint32_t slot = view_shared_slot(t, v, r.face, material);
+            if (slot == NONE) {
+                r.identity[3] = 0u;
+            } else {
+                uint32_t *st = &t.state[(size_t)slot * 36];
+                const uint32_t incarnation = st[28], count = st[5] & 0x7fffffffu, next = st[6];
+                if (r.identity[3] == incarnation || r.mean[3] <= (float)count) {
+                    r.mean[0] = bitsf(st[0]); r.mean[1] = bitsf(st[1]); r.mean[2] = bitsf(st[2]); r.mean[3] = (float)count;
+                    r.moment[0] = bitsf(st[8]); r.moment[1] = bitsf(st[9]); r.moment[2] = bitsf(st[10]);
+                    r.moment[3] = bitsf(t.meta[(size_t)slot * 12 + 4]);
+                    r.identity[0] = next;
+                } else {
+                    r.identity[0] = std::max(r.identity[0], next);
+                }
+                if (r.shown[3] == 0.0f && count > 0u) {
+                    // The authority's presentation, or its mean.
+                    if (bitsf(st[27]) > 0.0f) {
+                        for (int k = 0; k < 4; ++k) r.shown[k] = bitsf(st[24 + k]);
+                    } else {
+                        r.shown[0] = r.mean[0]; r.shown[1] = r.mean[1]; r.shown[2] = r.mean[2]; r.shown[3] = 1.0f;
+                    }
+                }
+                r.identity[3] = incarnation;
+                const uint32_t first = std::max(next, r.identity[0]);
+                st[6] = first + c.samples;
+                r.identity[0] = first;
+                r.sequence = first;
+            }
         }
-        r.identity[3] = incarnation;
-        const uint32_t first = std::max(next, r.identity[0]);
-        st[6] = first + c.samples;
-        r.identity[0] = first;
-        r.sequence = first;
+        // load_shared_faces (voxel_view_cache.wgsl:201-218): a new face seeds
+        // from the far cell at its centre with one unit of weight.
+        if (r.mean[3] == 0.0f && r.identity[0] == 0u && f.flags != 0u) {
+            float cell[4];
+            uint32_t sequence;
+            far_previous_cell(c, f, previous_cells, f3(r.surface[0], r.surface[1], r.surface[2]), r.face, cell, sequence);
+            if (cell[3] > 0.0f) {
+                r.mean[0] = cell[0]; r.mean[1] = cell[1]; r.mean[2] = cell[2]; r.mean[3] = 1.0f;
+                r.moment[0] = cell[0] * cell[0]; r.moment[1] = cell[1] * cell[1]; r.moment[2] = cell[2] * cell[2]; r.moment[3] = 0.0f;
+                r.shown[0] = cell[0]; r.shown[1] = cell[1]; r.shown[2] = cell[2]; r.shown[3] = 1.0f;
+                counters[5] += 1u;
+            }
+        }
+        r.sequence = r.identity[0];
     }
     // trace_visible_paths, every owner independent.
     parallel_for((uint32_t)list.size(), threads, [&](uint32_t b, uint32_t e, int) {
@@ -3783,6 +4097,9 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
             r.sequence = r.identity[0];
         }
     });
+    // reconstruct_view: each owner reads its neighbours' mean and moment
+    // (never written here) and writes only its own presentation.
+    for (uint32_t i : list) view_reconstruct(t, c, cur, owners, i);
     for (uint32_t i : list) {
         if (!exchange) break;
         const ViewFace &r = cur[i];
@@ -3800,6 +4117,11 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
         st[12] = 0u;
         st[5] = count | (r.moment[3] > 0.0f ? 0x80000000u : 0u);
         st[6] = r.identity[0];
+        // voxel_view_cache.wgsl:239-249: the presentation into the display
+        // record and its copies in the result records.
+        for (int k = 0; k < 4; ++k) st[24 + k] = fbits(r.shown[k]);
+        st[7] = st[24];
+        st[13] = st[25]; st[14] = st[26]; st[15] = st[27];
         uint32_t *m = &t.meta[(size_t)slot * 12];
         m[4] = fbits(r.moment[3]);
         if (changed_mean || changed_count) {
@@ -3812,18 +4134,163 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
             m[11] = 0xa0000000u | count;
         }
     }
+    // publish_view (voxel_view_runtime.wgsl:476-521).
+    t.view_published.assign(pixels, ViewPixelRecord{});
+    {
+        Estimator est(t, nullptr);
+        for (uint32_t i = 0; i < pixels; ++i) {
+            ViewPixelRecord &px = t.view_published[i];
+            px.face = NONE;
+            const ViewFace &r = cur[i];
+            if (r.face >= 6u || owners[i] != i) continue;
+            memcpy(px.surface, r.surface, 12);
+            px.face = r.face;
+            F3 irradiance = f3(r.shown[0], r.shown[1], r.shown[2]);
+            if (f.custom) {
+                F3 dv, dc;
+                est.direct_at(f3(r.surface[0], r.surface[1], r.surface[2]), face_normal(r.face), dv, dc);
+                irradiance = irradiance + dc;
+            }
+            px.irradiance[0] = irradiance.x; px.irradiance[1] = irradiance.y; px.irradiance[2] = irradiance.z;
+            const bool lit = r.shown[3] > 0.0f || irradiance.x > 0.0f || irradiance.y > 0.0f || irradiance.z > 0.0f;
+            px.evidence = lit ? fmax_(r.mean[3], 0.001f) : 0.0f;
+            for (uint32_t corner = 0; corner < 4; ++corner) {
+                const uint32_t byte = (r.identity[2] >> (corner * 8u)) & 255u;
+                px.sky[corner] = (byte & 15u) | ((byte & 128u) << 24u);
+            }
+            const uint32_t history = (uint32_t)r.mean[3];
+            counters[8] += history;
+            counters[9] = std::max(counters[9], history);
+            if (r.mean[3] == 0.0f) counters[10] += 1u;
+            if (px.evidence > 0.0f) counters[11] += 1u;
+        }
+    }
+    // The far tier: classify_pixels then trace_pixel_queue
+    // (voxel_view_pixels.wgsl:129-216).
+    if (f.cells > 0u) {
+        std::vector<FarCell> &cells = t.far_cells[g];
+        const uint32_t stamp = pixel_stamp(f.frame);
+        uint32_t *h = t.far_output.data();
+        h[0] = fbits(c.position[0]); h[1] = fbits(c.position[1]); h[2] = fbits(c.position[2]); h[3] = 0u;
+        for (int k = 0; k < 4; ++k) h[4 + k] = fbits(f.axis[k]);
+        h[8] = f.cells_x; h[9] = f.cells_y; h[10] = f.size; h[11] = f.flags;
+        h[12] = 0u; h[13] = 0u; h[14] = fbits(f.threshold); h[15] = stamp;
+        std::vector<uint32_t> queue;
+        const F3 camera = f3(c.position[0], c.position[1], c.position[2]);
+        for (uint32_t id = 0; id < f.cells; ++id) {
+            const uint32_t cx = id % f.cells_x, cy = id / f.cells_x;
+            const uint32_t k = (f.frame * 5u + (cx ^ cy)) % (size * size);
+            const uint32_t px = cx * size + k % size, py = cy * size + k / size;
+            if (px >= c.width || py >= c.height) continue;
+            const uint32_t pixel = py * c.width + px;
+            const float z = depth[pixel];
+            if (z <= 0.0f) continue;
+            const F3 position = view_unproject(c, (float)px, (float)py, z);
+            if (((f.flags >> 1u) & 3u) == 0u && f.axis[3] > 1.05f * f.threshold * length3(position - camera)) continue;
+            const uint32_t face = view_prepass_face(f3(normals[pixel * 3], normals[pixel * 3 + 1], normals[pixel * 3 + 2]));
+            if (face >= 6u) continue;
+            const F3 n = face_normal(face);
+            float sv[3] = {position.x, position.y, position.z};
+            sv[face / 2u] = std::nearbyint(sv[face / 2u]);
+            const F3 surface = f3(sv[0], sv[1], sv[2]);
+            const F3 inside = surface - n * 0.01f;
+            const I3 v{(int32_t)std::floor(inside.x), (int32_t)std::floor(inside.y), (int32_t)std::floor(inside.z)};
+            if (!far_pixel_owned(c, f, v, face)) continue;
+            counters[0] += 1u;
+            float history[4];
+            uint32_t sequence;
+            far_previous_cell(c, f, previous_cells, surface, face, history, sequence);
+            if (history[3] > 0.0f) {
+                counters[3] += 1u;
+            } else if (t.view_history) {
+                // Handoff from the face's view receiver of the previous frame.
+                auto it = t.view_owner.find(std::make_tuple(v.x, v.y, v.z, face));
+                if (it != t.view_owner.end() && t.view_faces[it->second].mean[3] > 0.0f) {
+                    memcpy(history, t.view_faces[it->second].mean, 16);
+                    sequence = t.view_faces[it->second].identity[0];
+                    counters[4] += 1u;
+                }
+            }
+            const float count = fmin_(history[3], f.history);
+            counters[2] += (uint32_t)count;
+            FarCell cell{};
+            cell.mean[0] = history[0]; cell.mean[1] = history[1]; cell.mean[2] = history[2]; cell.mean[3] = count;
+            cell.surface[0] = surface.x; cell.surface[1] = surface.y; cell.surface[2] = surface.z;
+            cell.face = face | (stamp << 3u);
+            cell.sequence = sequence;
+            cells[id] = cell;
+            const uint32_t rate = std::max((uint32_t)f.rate, 1u);
+            if (count < 4.0f || (id + f.frame) % rate == 0u) queue.push_back(id);
+            else far_write_output(t, f, id, cell);
+        }
+        counters[1] += (uint32_t)queue.size();
+        parallel_for((uint32_t)queue.size(), threads, [&](uint32_t b, uint32_t e, int) {
+            Estimator est(t, nullptr);
+            est.budgeted = true;
+            est.path_steps = f.steps;
+            for (uint32_t q = b; q < e; ++q) {
+                const uint32_t index = queue[q];
+                FarCell &cell = cells[index];
+                bool known;
+                const F3 value = est.trace_receiver(f3(cell.surface[0], cell.surface[1], cell.surface[2]), face_normal(cell.face & 7u),
+                                                    pcg(index * 9781u + 0x9e3779b9u), cell.sequence, known, nullptr);
+                cell.sequence += 1u;
+                if (known) {
+                    cell.mean[3] = fmin_(cell.mean[3] + 1.0f, f.history);
+                    const float tt = 1.0f / cell.mean[3], uu = 1.0f - tt;
+                    cell.mean[0] = cell.mean[0] * uu + value.x * tt;
+                    cell.mean[1] = cell.mean[1] * uu + value.y * tt;
+                    cell.mean[2] = cell.mean[2] * uu + value.z * tt;
+                }
+                far_write_output(t, f, index, cell);
+            }
+        });
+    }
     memcpy(owners_out, owners.data(), (size_t)pixels * 4);
     memcpy(faces_out, cur.data(), (size_t)pixels * sizeof(ViewFace));
     *owner_count = (uint32_t)list.size();
     t.view_faces = std::move(cur);
     t.view_owner = std::move(owner);
     t.view_history = true;
+    t.far_last_cells = f.cells;
+    memcpy(t.far_prev_clip, c.clip_from_world, sizeof(t.far_prev_clip));
+    t.far_prev_frame = c.frame;
+    t.far_prev = true;
+    t.view_generation = pg;
     return 0;
 }
 
 int orc_view_reset(void *tp) {
     ((Tracer *)tp)->view_history = false;
+    ((Tracer *)tp)->far_prev = false;
     return 0;
+}
+
+int orc_view_far(void *tp, const orc_far_settings *settings) {
+    Tracer &t = *(Tracer *)tp;
+    if (!settings) t.far.voxel_pixels = 0.0f;
+    else t.far = *settings;
+    return 0;
+}
+
+int orc_view_display(void *tp, float noise_threshold, float spatial, float blend) {
+    Tracer &t = *(Tracer *)tp;
+    t.view_display[0] = noise_threshold;
+    t.view_display[1] = spatial;
+    t.view_display[2] = blend;
+    return 0;
+}
+
+int64_t orc_view_read(void *tp, void *pixels, uint32_t *counters, uint32_t *far_output, uint32_t *far_cells) {
+    Tracer &t = *(Tracer *)tp;
+    if (pixels && !t.view_published.empty()) memcpy(pixels, t.view_published.data(), t.view_published.size() * sizeof(ViewPixelRecord));
+    if (counters) memcpy(counters, t.view_counters, sizeof(t.view_counters));
+    const uint32_t g = t.view_generation ^ 1u;  // the last frame's generation
+    if (t.far_last_cells) {
+        if (far_output) memcpy(far_output, t.far_output.data(), t.far_output.size() * 4);
+        if (far_cells) memcpy(far_cells, t.far_cells[g].data(), (size_t)t.far_last_cells * sizeof(FarCell));
+    }
+    return (int64_t)t.far_last_cells;
 }
 
 int orc_trace_ray(void *tp, const float origin[3], const float direction[3], float max_distance, int32_t *out_i, float *out_f) {

@@ -202,9 +202,10 @@ int orc_mlp_exact(int on);
 int orc_mlp_encoding(int mode);
 
 /* ---- view tier (voxel_view_discovery.wgsl, voxel_view_runtime.wgsl,
- * voxel_view_cache.wgsl): one camera frame over a depth/normal prepass; the
- * camera record has vgi_view_camera's layout. Per pixel: the owner (pixel
- * index, 0xffffffff none) and a 24-word face record (vgi_view_face). ---- */
+ * voxel_view_cache.wgsl, voxel_view_pixels.wgsl): one camera frame over a
+ * depth/normal prepass; the camera record has vgi_view_camera's layout. Per
+ * pixel: the owner (pixel index, 0xffffffff none) and a 28-word face record
+ * (vgi_view_face). ---- */
 typedef struct orc_view_camera {
     float world_from_clip[16];
     float position[3];
@@ -212,10 +213,30 @@ typedef struct orc_view_camera {
     uint32_t width, height;
     uint32_t samples;
     float measured_retention;
+    float clip_from_world[16];
+    float forward[3];
+    float pixels_per_unit;
+    uint32_t frame;
 } orc_view_camera;
+/* vgi_far_settings' layout. */
+typedef struct orc_far_settings {
+    float voxel_pixels;
+    uint32_t measure, cell;
+    float history;
+    uint32_t rate, steps;
+} orc_far_settings;
 int orc_view_frame(void *tracer, const orc_view_camera *camera, const float *depth, const float *normals, uint32_t clock_ms,
                    uint32_t *owners, void *faces, uint32_t *owner_count, int threads);
 int orc_view_reset(void *tracer);
+int orc_view_far(void *tracer, const orc_far_settings *settings);
+int orc_view_display(void *tracer, float noise_threshold, float spatial, float blend);
+/* The last view frame's published records (12 words per pixel, vgi_view_pixel),
+ * counters (16 words) and far tier (output: 16 + 4 words per cell; cells: 12
+ * words each); any pointer may be NULL. Returns the number of far cells (0:
+ * the tier did not run). */
+int64_t orc_view_read(void *tracer, void *pixels, uint32_t *counters, uint32_t *far_output, uint32_t *far_cells);
+/* pack2x16float with round-to-nearest-even: lo in bits 0-15, hi in 16-31. */
+uint32_t orc_pack_half2(float lo, float hi);
 /* The 32 encoded input features of each query (n x 32 doubles). */
 int orc_mlp_encode(const float *table, const uint32_t *queries, uint32_t n, const float *diffuse, const float scene[4], double *out);
 

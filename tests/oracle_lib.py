@@ -71,6 +71,10 @@ def lib():
         "orc_trace_learned": (ctypes.c_int, [P, P, u32, P, P, P, ctypes.c_int]),
         "orc_view_frame": (ctypes.c_int, [P, P, P, P, u32, P, P, P, ctypes.c_int]),
         "orc_view_reset": (ctypes.c_int, [P]),
+        "orc_view_far": (ctypes.c_int, [P, P]),
+        "orc_view_display": (ctypes.c_int, [P, f32, f32, f32]),
+        "orc_view_read": (ctypes.c_int64, [P, P, P, P, P]),
+        "orc_pack_half2": (u32, [f32, f32]),
         "orc_cache_enable": (ctypes.c_int, [P, u32, f32, f32, u32]),
         "orc_cache_register": (ctypes.c_int, [P, P, u32, P]),
         "orc_cache_batch": (ctypes.c_int, [P, P, P, u32, P, P, P, ctypes.c_int, u32]),
@@ -245,13 +249,30 @@ class Tracer:
         n = np.ascontiguousarray(normals, np.float32)
         pixels = camera.width * camera.height
         owners = np.zeros(pixels, np.uint32)
-        faces = np.zeros(pixels * 24, np.uint32)
+        faces = np.zeros(pixels * 28, np.uint32)
         count = np.zeros(1, np.uint32)
         lib().orc_view_frame(self.h, ctypes.byref(camera), ptr(d), ptr(n), clock_ms, ptr(owners), ptr(faces), ptr(count), threads)
         return owners, faces, int(count[0])
 
     def view_reset(self):
         lib().orc_view_reset(self.h)
+
+    def view_far(self, settings):
+        """The far tier's settings (the package's FarSettings, the same layout; None: off)."""
+        lib().orc_view_far(self.h, ctypes.byref(settings) if settings is not None else None)
+
+    def view_display(self, noise_threshold=0.031, spatial=1.0, blend=0.19926):
+        lib().orc_view_display(self.h, noise_threshold, spatial, blend)
+
+    def view_read(self, pixels):
+        """(published records [pixels, 12] words, counters [16], far output words, far cells [n, 12]) of the last frame."""
+        records = np.zeros((pixels, 12), np.uint32)
+        counters = np.zeros(16, np.uint32)
+        n = lib().orc_view_read(self.h, None, None, None, None)
+        output = np.zeros(16 + 4 * n, np.uint32)
+        cells = np.zeros((max(n, 1), 12), np.uint32)
+        lib().orc_view_read(self.h, ptr(records), ptr(counters), ptr(output), ptr(cells))
+        return records, counters, output, cells[:n]
 
     def trace_ray(self, origin, direction, max_distance=768.0):
         o = np.asarray(origin, np.float32)

@@ -33,7 +33,7 @@ def compare_frame(vgi, g, t, cam, depth, normals, clock, label):
     assert np.array_equal(gown, cown), "%s: owners differ" % label
     assert gcount == ccount
     cfaces = cwords.view(vgi.VIEW_FACE)
-    gw, cw = gfaces.view(np.uint32).reshape(pixels, 24), cwords.reshape(pixels, 24)
+    gw, cw = gfaces.view(np.uint32).reshape(pixels, 28), cwords.reshape(pixels, 28)
     bad = np.flatnonzero(np.any(gw != cw, axis=1))
     assert len(bad) == 0, "%s: %d pixels differ, first %d: gpu %s cpu %s" % (label, len(bad), bad[0], gfaces[bad[0]], cfaces[bad[0]])
     return gown, gfaces

@@ -14,7 +14,7 @@ import oracle_lib as orc
 from test_oracle_edits import floor_scene
 from view_helpers import camera
 
-VIEW_FACE_WORDS = 24
+VIEW_FACE_WORDS = 28
 
 
 def package():
