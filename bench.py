@@ -479,19 +479,26 @@ def view_frames(vgi, g, grid, args):
     """View frames inside the cave on the bound resident tracer: a 640x360
     depth/normal prepass from a camera in a worm tunnel (CPU ray cast of the
     voxel grid), `--view-frames` frames of discovery, face hash, reprojection,
-    world-cache exchange and 4 paths per visible face; the camera moves 0.1
-    voxel per frame along its line of sight."""
+    world-cache exchange, 4 paths per visible face, presentation
+    (reconstruction and publication) and the far per-pixel tier at the
+    reference's default settings (faces under 8 pixels across, 1-pixel cells
+    at this height, one path per cell every fourth frame after four, 128
+    steps per ray); the camera moves 0.1 voxel per frame along its line of
+    sight."""
     found = tunnel_camera(grid)
     if found is None:
         return None
     eye, target = found
     w, h = 640, 360
+    fov_y = np.radians(60.0)
     d = (target - eye) / np.linalg.norm(target - eye)
-    frame_ms, trace_ms, owners = [], [], []
+    g.view_far(vgi.far_settings())
+    g.view_display()
+    frame_ms, trace_ms, far_ms, owners, cells, traced, displayed = [], [], [], [], [], [], []
     for k in range(args.view_frames + 1):
         e = eye + d * 0.1 * k
-        wfc, depth, normals, hits = vgi.scene_prepass(grid, e, e + d * 16.0, w, h)
-        cam = vgi.view_camera(wfc, e, w, h, samples=4)
+        wfc, depth, normals, hits = vgi.scene_prepass(grid, e, e + d * 16.0, w, h, fov_y)
+        cam = vgi.view_camera(wfc, e, w, h, samples=4, forward=d, pixels_per_unit=h * 0.5 / np.tan(fov_y * 0.5), frame=k)
         g.view_frame(cam, depth, normals)
         g.sync()
         if k == 0:
@@ -499,12 +506,21 @@ def view_frames(vgi, g, grid, args):
         ms = g.view_timings()
         frame_ms.append(float(ms[0]))
         trace_ms.append(float(ms[1]))
+        far_ms.append(float(ms[2]))
         owners.append(g.view_owners(0)[1])
+        counters = g.view_counters()
+        cells.append(int(counters[0]))
+        traced.append(int(counters[1]))
+        displayed.append(int(counters[11]))
+    g.view_far(None)
     med = float(np.median(frame_ms))
     faces = int(np.median(owners))
-    out = {"workload": "512^3 worm cave, 640x360 prepass from inside a tunnel, every bound face resident; 4 paths per visible face",
-           "pixels": w * h, "pixels_hit": int(hits), "visible_faces": faces, "frames_timed": args.view_frames,
+    out = {"workload": "512^3 worm cave, 640x360 prepass from inside a tunnel, every bound face resident; 4 paths per visible face; "
+                       "far tier at its defaults (8 px, 1-px cells, rate 4, 128 steps)",
+           "pixels": w * h, "pixels_hit": int(hits), "visible_faces": faces, "far_cells": int(np.median(cells)),
+           "far_cells_traced": int(np.median(traced)), "faces_displayed": int(np.median(displayed)), "frames_timed": args.view_frames,
            "frame_gpu_ms_median": round(med, 3), "trace_gpu_ms_median": round(float(np.median(trace_ms)), 3),
+           "far_gpu_ms_median": round(float(np.median(far_ms)), 3),
            "faces_per_s": round(faces / (med / 1000.0), 1)}
     print("view frames: %s" % out, file=sys.stderr, flush=True)
     return out

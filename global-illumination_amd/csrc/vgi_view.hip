@@ -661,43 +661,50 @@ __device__ __forceinline__ void far_output(const FarParams &f, const SceneView &
     *reinterpret_cast<uint4 *>(output + 16u + (size_t)index * 4u) = word;
 }
 
+// Wavefront-aggregated counting: one atomic per wavefront instead of one per
+// lane (every lane of the wavefront must reach the call). wave_count adds the
+// number of lanes whose flag is set; wave_sum adds the lanes' values.
+__device__ __forceinline__ void wave_count(uint32_t *counter, bool flag) {
+    const uint64_t mask = __ballot(flag);
+    if (mask != 0ull && __lane_id() == (uint32_t)__ffsll((long long)mask) - 1u) atomicAdd(counter, (uint32_t)__popcll(mask));
+}
+__device__ __forceinline__ void wave_sum(uint32_t *counter, uint32_t value) {
+    uint32_t sum = value;
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) sum += __shfl_xor(sum, offset, 64);
+    if (__lane_id() == 0u && sum != 0u) atomicAdd(counter, sum);
+}
+
 // classify_pixels (voxel_view_pixels.wgsl:129-194): each cell samples one of
 // its pixels this frame (cycling through all of them), keeps it if the
 // pixel's face is a far face, takes its history from the previous cells
 // (reprojected by world position) or, failing that, from the face's view
 // receiver of the previous frame, and queues itself for a path when its
 // history is short or one frame in `rate`; otherwise it writes its output.
-__global__ void far_classify_kernel(ViewParams p, FarParams f, SceneView s, Optics o, const float *__restrict__ depth,
-                                    const float *__restrict__ normals, const ViewReceiver *__restrict__ prev_receivers,
-                                    const ViewEstimate *__restrict__ prev_estimates, const uint32_t *__restrict__ prev_hash,
-                                    FarBuffers b, uint32_t *__restrict__ counters) {
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stamp = pixel_stamp(f.frame);
-    if (id == 0u) {
-        uint32_t *h = b.output;
-        h[0] = __float_as_uint(p.cam[0]); h[1] = __float_as_uint(p.cam[1]); h[2] = __float_as_uint(p.cam[2]); h[3] = 0u;
-        h[4] = __float_as_uint(f.axis[0]); h[5] = __float_as_uint(f.axis[1]); h[6] = __float_as_uint(f.axis[2]);
-        h[7] = __float_as_uint(f.axis[3]);
-        h[8] = f.cells_x; h[9] = f.cells_y; h[10] = f.size; h[11] = f.flags;
-        h[12] = 0u; h[13] = 0u; h[14] = __float_as_uint(f.threshold); h[15] = stamp;
-    }
-    if (id >= f.cells) return;
+// The queue is filled by wavefront-compacted reservations (ballot, prefix,
+// one atomic per wavefront), so its order is the arrival order of
+// wavefronts; each queued cell's path depends only on the cell.
+__device__ __forceinline__ bool far_classify(const ViewParams &p, const FarParams &f, const float *__restrict__ depth,
+                                             const float *__restrict__ normals, const ViewReceiver *__restrict__ prev_receivers,
+                                             const ViewEstimate *__restrict__ prev_estimates, const uint32_t *__restrict__ prev_hash,
+                                             const PixelHistory *__restrict__ previous, uint32_t id, PixelHistory &cell,
+                                             bool &reprojected, bool &seeded) {
     const uint32_t size = f.size;
     const uint32_t cx = id % f.cells_x, cy = id / f.cells_x;
     // One pixel of the cell per frame, cycling through all of them (5 is
     // coprime with every cell area 1, 4, 9, 16).
     const uint32_t k = (f.frame * 5u + (cx ^ cy)) % (size * size);
     const uint32_t px = cx * size + k % size, py = cy * size + k / size;
-    if (px >= p.width || py >= p.height) return;
+    if (px >= p.width || py >= p.height) return false;
     const uint32_t pixel = py * p.width + px;
     const float z = depth[pixel];
-    if (z <= 0.0f) return;
+    if (z <= 0.0f) return false;
     const f3 position = view_unproject(p, (float)px, (float)py, z);
     const f3 camera = mk(p.cam[0], p.cam[1], p.cam[2]);
     // The side test needs only the distance: most near cells stop here.
-    if (((f.flags >> 1u) & 3u) == 0u && f.axis[3] > 1.05f * f.threshold * len3(position - camera)) return;
+    if (((f.flags >> 1u) & 3u) == 0u && f.axis[3] > 1.05f * f.threshold * len3(position - camera)) return false;
     const uint32_t face = prepass_face(mk(normals[pixel * 3u], normals[pixel * 3u + 1u], normals[pixel * 3u + 2u]));
-    if (face >= 6u) return;
+    if (face >= 6u) return false;
     const f3 n = face_normal(face);
     const uint32_t axis = face / 2u;
     float sv[3] = {position.x, position.y, position.z};
@@ -705,12 +712,11 @@ __global__ void far_classify_kernel(ViewParams p, FarParams f, SceneView s, Opti
     const f3 surface = mk(sv[0], sv[1], sv[2]);
     const f3 inside = surface - n * 0.01f;
     const int vx = (int)floorf(inside.x), vy = (int)floorf(inside.y), vz = (int)floorf(inside.z);
-    if (!pixel_owned(p, f, vx, vy, vz, face)) return;
-    atomicAdd(&counters[PIXEL_OWNED], 1u);
+    if (!pixel_owned(p, f, vx, vy, vz, face)) return false;
     uint32_t sequence;
-    float4 history = previous_cell(p, f, b.previous, surface, face, sequence);
+    float4 history = previous_cell(p, f, previous, surface, face, sequence);
     if (history.w > 0.0f) {
-        atomicAdd(&counters[PIXEL_REPROJECTED], 1u);
+        reprojected = true;
     } else {
         // Handoff: a face crossing into the far band starts from its view
         // receiver's history (the same estimator).
@@ -724,25 +730,60 @@ __global__ void far_classify_kernel(ViewParams p, FarParams f, SceneView s, Opti
         if (owner != NONE && prev_estimates[owner].mean.w > 0.0f) {
             history = prev_estimates[owner].mean;
             sequence = prev_estimates[owner].identity.x;
-            atomicAdd(&counters[PIXEL_SEEDED], 1u);
+            seeded = true;
         }
     }
     const float count = fmn(history.w, f.history);
-    atomicAdd(&counters[PIXEL_HISTORY], (uint32_t)count);
-    PixelHistory cell;
     cell.mean = make_float4(history.x, history.y, history.z, count);
     cell.sx = surface.x; cell.sy = surface.y; cell.sz = surface.z;
-    cell.face = face | (stamp << 3u);
+    cell.face = face | (pixel_stamp(f.frame) << 3u);
     cell.sequence = sequence;
     cell.pad[0] = cell.pad[1] = cell.pad[2] = 0u;
-    b.cells[id] = cell;
-    const uint32_t rate = max((uint32_t)f.rate, 1u);
-    if (count < 4.0f || (id + f.frame) % rate == 0u) {
-        const uint32_t slot = atomicAdd(&b.queue[0], 1u);
-        b.queue[1u + slot] = id;  // the trace writes this cell's output
-    } else {
-        far_output(f, s, o, b.output, id, cell);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void far_classify_kernel(ViewParams p, FarParams f, SceneView s, Optics o, const float *__restrict__ depth,
+                                                           const float *__restrict__ normals,
+                                                           const ViewReceiver *__restrict__ prev_receivers,
+                                                           const ViewEstimate *__restrict__ prev_estimates,
+                                                           const uint32_t *__restrict__ prev_hash, FarBuffers b,
+                                                           uint32_t *__restrict__ counters) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id == 0u) {
+        uint32_t *h = b.output;
+        h[0] = __float_as_uint(p.cam[0]); h[1] = __float_as_uint(p.cam[1]); h[2] = __float_as_uint(p.cam[2]); h[3] = 0u;
+        h[4] = __float_as_uint(f.axis[0]); h[5] = __float_as_uint(f.axis[1]); h[6] = __float_as_uint(f.axis[2]);
+        h[7] = __float_as_uint(f.axis[3]);
+        h[8] = f.cells_x; h[9] = f.cells_y; h[10] = f.size; h[11] = f.flags;
+        h[12] = 0u; h[13] = 0u; h[14] = __float_as_uint(f.threshold); h[15] = pixel_stamp(f.frame);
     }
+    PixelHistory cell;
+    bool reprojected = false, seeded = false;
+    const bool owned = id < f.cells &&
+                       far_classify(p, f, depth, normals, prev_receivers, prev_estimates, prev_hash, b.previous, id, cell, reprojected, seeded);
+    bool queued = false;
+    if (owned) {
+        b.cells[id] = cell;
+        const uint32_t rate = max((uint32_t)f.rate, 1u);
+        queued = cell.mean.w < 4.0f || (id + f.frame) % rate == 0u;
+        if (!queued) far_output(f, s, o, b.output, id, cell);  // the trace writes a queued cell's output
+    }
+    // Every lane of the wavefront reaches the counting and the queue's
+    // reservation: ballot, the lane's rank among the queued lanes, one
+    // atomic per wavefront for the queue and for each counter.
+    const uint64_t mask = __ballot(queued);
+    if (mask != 0ull) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+        uint32_t base = 0u;
+        if (lane == leader) base = atomicAdd(&b.queue[0], (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader, 64);
+        if (queued) b.queue[1u + base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = id;
+    }
+    wave_count(&counters[PIXEL_OWNED], owned);
+    wave_count(&counters[PIXEL_REPROJECTED], reprojected);
+    wave_count(&counters[PIXEL_SEEDED], seeded);
+    wave_sum(&counters[PIXEL_HISTORY], owned ? (uint32_t)cell.mean.w : 0u);
 }
 
 // trace_pixel_queue (voxel_view_pixels.wgsl:201-216): one path per queued
@@ -750,10 +791,11 @@ __global__ void far_classify_kernel(ViewParams p, FarParams f, SceneView s, Opti
 // the cell's own sequence; a known sample enters the capped running mean.
 __global__ void far_trace_kernel(FarParams f, SceneView s, Optics o, FarBuffers b, uint32_t *__restrict__ counters) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= b.queue[0]) return;
+    const uint32_t queued = b.queue[0];
+    wave_count(&counters[PIXEL_TRACED], id < queued);
+    if (id >= queued) return;
     const uint32_t index = b.queue[1u + id];
     PixelHistory cell = b.cells[index];
-    atomicAdd(&counters[PIXEL_TRACED], 1u);
     BudgetTally tally;
     tally.limit = f.steps;
     bool known;
