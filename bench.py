@@ -746,7 +746,7 @@ def main():
             "rays_per_frame": int(rays / args.steps),
             "faces_per_s": round(len(faces) * args.steps / elapsed, 1),
             "trace_ms_per_frame": round(float(np.mean(trace_ms)), 3),
-            "method": "plain: every path runs its 4 bounces (no cache termination); temporal merge on",
+            "method": "plain: paths of up to 4 bounces (Russian roulette after the second hit, survival clamp(max throughput, 0.1, 0.95); no cache termination); temporal merge on",
             "parallelism": "single GPU",
         },
         "roofline": {
