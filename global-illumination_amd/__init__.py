@@ -200,6 +200,7 @@ def library():
         "vgi_view_read_owners": (i64, [P, P, ctypes.c_uint32, P]),
         "vgi_view_read_faces": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_view_far": (ctypes.c_int, [P, P]),
+        "vgi_set_dedup": (ctypes.c_int, [P, ctypes.c_int]),
         "vgi_view_display": (ctypes.c_int, [P, f32, f32, f32]),
         "vgi_view_read_pixels": (ctypes.c_int, [P, ctypes.c_uint32, ctypes.c_uint32, P]),
         "vgi_view_read_counters": (ctypes.c_int, [P, P]),
@@ -258,7 +259,7 @@ def exported_symbols():
         "vgi_batch_last_ms", "vgi_batch_latency", "vgi_batch_autotune", "vgi_cache_meta", "vgi_set_schedule", "vgi_set_footprints",
         "vgi_sun_corners", "vgi_read_sun_flips", "vgi_read_sun_corners", "vgi_sun_timings",
         "vgi_view_frame", "vgi_view_reset", "vgi_view_timings", "vgi_view_read_owners", "vgi_view_read_faces",
-        "vgi_view_far", "vgi_view_display", "vgi_view_read_pixels", "vgi_view_read_counters", "vgi_view_read_far",
+        "vgi_set_dedup", "vgi_view_far", "vgi_view_display", "vgi_view_read_pixels", "vgi_view_read_counters", "vgi_view_read_far",
         "vgi_select_frame", "vgi_select_timings", "vgi_read_picks", "vgi_read_selection", "vgi_read_keys", "vgi_read_mirror",
     ]
 
@@ -554,6 +555,10 @@ class GpuTracer:
     def set_wavefront(self, on):
         _check(self._lib.vgi_set_wavefront(self._h, int(on)))
 
+    def set_dedup(self, on):
+        """The wavefront tracer's per-wavefront hit-face deduplication."""
+        _check(self._lib.vgi_set_dedup(self._h, int(on)))
+
     def last_frame_ms(self):
         return float(self._lib.vgi_last_frame_ms(self._h))
 
@@ -584,7 +589,7 @@ class GpuTracer:
         """Records of the first `count` bound faces (all by default)."""
         count = self._bound if count is None else int(count)
         out = np.zeros((count, 24), np.float32)
-        counters = np.zeros(5, np.uint64)
+        counters = np.zeros(7, np.uint64)
         _check(self._lib.vgi_read_bound(self._h, _ptr(out), count, _ptr(counters)))
         return out, counters
 

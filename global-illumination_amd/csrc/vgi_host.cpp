@@ -261,6 +261,7 @@ struct vgi_tracer {
     DeviceBuffer d_train_tasks, d_train_q, d_train_results, d_targets, d_acts, d_grads, d_train_pred, d_io, d_touched;
     // ---- wavefront path tracing of bound frames
     bool wavefront = false;
+    bool dedup = false;  // the wavefront tracer's per-wavefront hit-face deduplication
     int cus = 0;
     DeviceBuffer d_order;      // bound faces in Morton order of their voxels
     bool has_order = false;
@@ -1364,7 +1365,7 @@ int wavefront_trace(vgi_tracer &t, const vgi::SceneView &s, const vgi::Optics &o
     b.control = t.wf_control.as<uint32_t>();
     b.order = order_of(t);
     uint32_t grid = (uint32_t)std::max(t.cus, 1) * (4u * VGI_TRACE_WAVES) / 4u;
-    vgi::launch_wavefront(s, o, tasks, count, samples, lanes_for(t.transport), sequence_base, b, grid, results, rays, t.stream);
+    vgi::launch_wavefront(s, o, tasks, count, samples, lanes_for(t.transport), sequence_base, b, grid, results, rays, t.stream, t.dedup);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -2405,6 +2406,12 @@ int vgi_set_wavefront(vgi_tracer *t, int on) {
     return 0;
 }
 
+int vgi_set_dedup(vgi_tracer *t, int on) {
+    if (!t) return fail("null tracer");
+    t->dedup = on != 0;
+    return 0;
+}
+
 
 int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t *counters) {
     if (!t || (count && !out)) return fail("null argument");
@@ -2412,7 +2419,7 @@ int vgi_read_bound(vgi_tracer *t, vgi_face_result *out, uint32_t count, uint64_t
     HIP_OK(hipSetDevice(t->device));
     HIP_OK(hipStreamSynchronize(t->stream));
     if (count && read_bound_rows(*t, t->b_results.ptr, 96, count, out) != 0) return -1;
-    if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 40, hipMemcpyDeviceToHost));
+    if (counters) HIP_OK(hipMemcpy(counters, t->b_counters.ptr, 56, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -57,6 +57,6 @@ struct WfBuffers {
 // `rays` (optional) counts traversals.
 void launch_wavefront(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples, uint32_t lanes,
                       uint32_t sequence_base, const WfBuffers &buffers, uint32_t trace_grid, float4 *results, unsigned long long *rays,
-                      hipStream_t stream);
+                      hipStream_t stream, bool dedup = false);
 
 }  // namespace vgi

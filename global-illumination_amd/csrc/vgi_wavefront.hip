@@ -11,7 +11,9 @@
 //   shade   one thread per path: resolves the previous hit's deferred terms
 //           (sun and emitter shadows, terminal connection) into the path's
 //           energy, then shades the new hit and queues its shadow rays and
-//           the next path ray (compacted: wave prefix sum + one atomic).
+//           the next path ray (compacted: wave prefix sum + one atomic);
+//           optionally the new hits' world-cache lookups are deduplicated
+//           per wavefront first (ballot match, one probe per distinct face).
 //   final   one thread per face: the samples' values summed in the
 //           megakernel's lane order, the analytic direct term, the record.
 //
@@ -90,6 +92,47 @@ __device__ __forceinline__ void face_of_path(const uint4 *__restrict__ tasks, ui
     normal = valid ? face_normal(t0.w) : mk(0.0f, 0.0f, 0.0f);
     stream = hash_voxel(vx, vy, vz) ^ pcg(t0.w + 1u);
     sequence = t1.x + sequence_base + sample;
+}
+
+// Hit-face deduplication of the world-cache lookups (voxel_path.wgsl:63-87)
+// of one shading launch. Every lane of the wavefront calls it; `want` lanes
+// hold a (voxel, face) hit. A match loop over the wavefront's distinct keys
+// (ballot of the lanes equal to the lowest pending lane's key) gives every
+// lane the lowest lane of its key; only those lanes probe the cache (in
+// parallel), and the others take their answer across the wavefront. The
+// lookup is a pure function of the key and the cache state, which no lane
+// writes while shading, so every lane's answer is the one it would have
+// probed itself. `stats` (optional) gains the lookups and the probes made.
+__device__ __forceinline__ bool wave_cached_energy(const Optics &o, bool want, int vx, int vy, int vz, uint32_t face, f3 &value,
+                                                   unsigned long long *__restrict__ stats) {
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(want);
+    uint32_t leader = lane;
+    const uint64_t wanted = pending;
+    while (pending != 0ull) {
+        const int first = __ffsll((long long)pending) - 1;
+        const int lx = __shfl(vx, first, 64), ly = __shfl(vy, first, 64), lz = __shfl(vz, first, 64);
+        const uint32_t lf = (uint32_t)__shfl((int)face, first, 64);
+        const bool same = want && vx == lx && vy == ly && vz == lz && face == lf;
+        const uint64_t group = __ballot(same) & pending;
+        if ((group >> lane) & 1ull) leader = (uint32_t)first;
+        pending &= ~group;
+    }
+    const bool probe = want && leader == lane;
+    bool found = false;
+    f3 cached = mk(0.0f, 0.0f, 0.0f);
+    if (probe) {
+        uint32_t slot;
+        found = cached_energy(o, vx, vy, vz, face, cached, slot);
+    }
+    const bool shared = __shfl((int)found, (int)leader, 64) != 0;
+    value = mk(__shfl(cached.x, (int)leader, 64), __shfl(cached.y, (int)leader, 64), __shfl(cached.z, (int)leader, 64));
+    const uint64_t probing = __ballot(probe);
+    if (stats && lane == 0u && wanted != 0ull) {
+        atomicAdd(&stats[5], (unsigned long long)__popcll(wanted));
+        atomicAdd(&stats[6], (unsigned long long)__popcll(probing));
+    }
+    return want && shared;
 }
 
 }  // namespace
@@ -217,11 +260,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VGI_TRACE_W
 }
 
 // ------------------------------------------------------------------ shade
+// kDedup: the new hits' world-cache lookups are deduplicated per wavefront
+// (wave_cached_energy) before shading; the records are the same either way.
+template <bool kDedup>
 __global__ __launch_bounds__(256) void wf_shade_kernel(SceneView scene, Optics optics, const uint4 *__restrict__ tasks, uint32_t count,
                                                        uint32_t samples, uint32_t sequence_base, WfPaths paths,
                                                        const WfHit *__restrict__ hit0, const WfHit *__restrict__ hit1,
                                                        const uint8_t *__restrict__ vis, WfRay *__restrict__ queue,
-                                                       uint32_t *__restrict__ qcount, const uint32_t *__restrict__ order) {
+                                                       uint32_t *__restrict__ qcount, const uint32_t *__restrict__ order,
+                                                       unsigned long long *__restrict__ stats) {
     (void)scene;
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     Emit e;
@@ -232,6 +279,22 @@ __global__ __launch_bounds__(256) void wf_shade_kernel(SceneView scene, Optics o
         flags = __float_as_uint(en4.w);
     }
     const uint32_t work = flags & (F_PATH | F_RESOLVE | F_NEE0);
+    f3 deduped_value = mk(0.0f, 0.0f, 0.0f);
+    bool deduped_found = false;
+    if constexpr (kDedup) {
+        bool want = false;
+        int kx = 0, ky = 0, kz = 0;
+        uint32_t kface = 6u;
+        if ((flags & F_PATH) && optics.cache_on) {
+            const WfHit h = hit0[p];
+            if ((h.status_normal & 255u) == 1u) {
+                want = true;
+                kx = h.vx; ky = h.vy; kz = h.vz;
+                kface = face_index_of(normal_of(h.status_normal >> 8));
+            }
+        }
+        deduped_found = wave_cached_energy(optics, want, kx, ky, kz, kface, deduped_value, stats);
+    }
     if (work) {
         uint32_t face_index = p / samples, sample = p % samples;
         if (order) face_index = order[face_index];
@@ -340,9 +403,12 @@ __global__ __launch_bounds__(256) void wf_shade_kernel(SceneView scene, Optics o
                 if (light_pdf > 0.0f) mis = previous_pdf / (previous_pdf + light_pdf);
                 uint32_t kind = K_ZERO;
                 f3 c = mk(0.0f, 0.0f, 0.0f);
-                f3 cached;
-                uint32_t cslot;
-                bool found = cached_energy(optics, h.vx, h.vy, h.vz, face_index_of(hn), cached, cslot);
+                f3 cached = deduped_value;
+                bool found = deduped_found;
+                if constexpr (!kDedup) {
+                    uint32_t cslot;
+                    found = cached_energy(optics, h.vx, h.vy, h.vz, face_index_of(hn), cached, cslot);
+                }
                 bool last = bounce + 1u == optics.bounces;
                 bool stop = found || last;
                 if (found) {
@@ -477,7 +543,7 @@ static inline uint32_t wf_blocks(uint64_t n, uint32_t per) { return (uint32_t)((
 
 void launch_wavefront(const SceneView &scene, const Optics &optics, const uint4 *tasks, uint32_t count, uint32_t samples, uint32_t lanes,
                       uint32_t sequence_base, const WfBuffers &b, uint32_t trace_grid, float4 *results, unsigned long long *rays,
-                      hipStream_t stream) {
+                      hipStream_t stream, bool dedup) {
     uint64_t n = (uint64_t)count * samples;
     if (n == 0) return;
     // control words: [0] queue count, [1] queue head
@@ -488,8 +554,12 @@ void launch_wavefront(const SceneView &scene, const Optics &optics, const uint4 
         hipLaunchKernelGGL(wf_trace_kernel, dim3(trace_grid), dim3(256), 0, stream, scene, optics, b.queue, b.control, b.control + 1,
                            b.hit0, b.hit1, b.vis, rays);
         hipLaunchKernelGGL(wf_reset_kernel, dim3(1), dim3(64), 0, stream, b.control, 2u);
-        hipLaunchKernelGGL(wf_shade_kernel, dim3(wf_blocks(n, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples,
-                           sequence_base, b.paths, b.hit0, b.hit1, b.vis, b.queue, b.control, b.order);
+        if (dedup)
+            hipLaunchKernelGGL(wf_shade_kernel<true>, dim3(wf_blocks(n, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples,
+                               sequence_base, b.paths, b.hit0, b.hit1, b.vis, b.queue, b.control, b.order, rays);
+        else
+            hipLaunchKernelGGL(wf_shade_kernel<false>, dim3(wf_blocks(n, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples,
+                               sequence_base, b.paths, b.hit0, b.hit1, b.vis, b.queue, b.control, b.order, rays);
     }
     hipLaunchKernelGGL(wf_final_kernel, dim3(wf_blocks(count, 256)), dim3(256), 0, stream, scene, optics, tasks, count, samples, lanes,
                        b.paths, results, rays, b.order);

@@ -353,9 +353,16 @@ impl GpuTracer {
         check(unsafe { sys::vgi_set_wavefront(self.raw(), on as c_int) })
     }
 
+    /// The wavefront tracer's per-wavefront hit-face deduplication of
+    /// world-cache lookups (the same records either way).
+    pub fn set_dedup(&mut self, on: bool) -> Result<()> {
+        check(unsafe { sys::vgi_set_dedup(self.raw(), on as c_int) })
+    }
+
     /// The bound faces' records after the last frame; `counters` receives
-    /// [rays, steps, header, brick-word, cell-word loads] of a counted frame.
-    pub fn read_bound(&self, out: &mut [FaceResult], counters: Option<&mut [u64; 5]>) -> Result<()> {
+    /// [rays, steps, header, brick-word, cell-word loads, deduplicated
+    /// world-cache lookups, their probes] of a counted frame.
+    pub fn read_bound(&self, out: &mut [FaceResult], counters: Option<&mut [u64; 7]>) -> Result<()> {
         let n = len_u32(out.len(), "read_bound")?;
         let records = if out.is_empty() { ptr::null_mut() } else { out.as_mut_ptr() };
         let c = counters.map_or(ptr::null_mut(), |c| c.as_mut_ptr());

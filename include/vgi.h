@@ -174,10 +174,19 @@ float vgi_last_frame_ms(vgi_tracer *tracer);
  * (VGI_COUNT_GATHERS), learned frames and more than 5 lights use the
  * one-kernel estimator either way. */
 int vgi_set_wavefront(vgi_tracer *tracer, int on);
+/* The wavefront tracer's hit-face deduplication (on) or not (off, the
+ * default): before a shading pass, the world-cache lookups of its new hits
+ * (cached_energy, voxel_path.wgsl:63-87) are matched per wavefront (ballot
+ * over the distinct (voxel, face) keys) and each distinct face is probed
+ * once, its answer shared across the wavefront. The same records bit for
+ * bit; with VGI_COUNT_RAYS the frame's counters 5 and 6 receive the lookups
+ * and the probes made. */
+int vgi_set_dedup(vgi_tracer *tracer, int on);
 /* The bound faces' 96-byte records after the last frame (out may be NULL with
  * count 0 to read only the counters); counters (optional)
- * receive [rays, steps, header loads, brick-word loads, cell-word loads] when
- * the frame ran with VGI_COUNT_RAYS. Records follow the faces bound now: after
+ * receive [rays, steps, header loads, brick-word loads, cell-word loads,
+ * deduplicated world-cache lookups, their probes] when the frame ran with
+ * VGI_COUNT_RAYS. Records follow the faces bound now: after
  * an edit that moved bound faces (vgi_bind_exposed), including one issued
  * while the frame still ran, each face reads its record of the last frame,
  * and a face bound since reads zeros -- the same records in either call
