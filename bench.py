@@ -704,7 +704,7 @@ def main():
     g.run_frame(vgi.COUNT_GATHERS)
     g.sync()
     gather_counts = g.read_counters_bound().astype(np.float64)
-    counters = np.zeros(5, np.float64)
+    counters = np.zeros(vgi.BOUND_COUNTERS, np.float64)
     trace_ms = []
     t_start = time.perf_counter()
     for _ in range(args.steps):

@@ -129,6 +129,11 @@ def far_settings(voxel_pixels=8.0, measure=0, cell=0, history=32.0, rate=4, step
     return f
 
 
+# vgi_read_bound's counters: rays, steps, header, brick-word and cell-word
+# loads, deduplicated world-cache lookups, their probes (the library writes
+# all seven).
+BOUND_COUNTERS = 7
+
 VIEW_FACE = np.dtype([("voxel", "<i4", (3,)), ("face", "<u4"), ("surface", "<f4", (3,)), ("sequence", "<u4"), ("mean", "<f4", (4,)),
                       ("moment", "<f4", (4,)), ("identity", "<u4", (4,)), ("direct", "<f4", (4,)), ("shown", "<f4", (4,))])
 VIEW_PIXEL = np.dtype([("surface", "<f4", (3,)), ("face", "<u4"), ("irradiance", "<f4", (3,)), ("evidence", "<f4"),
@@ -589,7 +594,7 @@ class GpuTracer:
         """Records of the first `count` bound faces (all by default)."""
         count = self._bound if count is None else int(count)
         out = np.zeros((count, 24), np.float32)
-        counters = np.zeros(7, np.uint64)
+        counters = np.zeros(BOUND_COUNTERS, np.uint64)
         _check(self._lib.vgi_read_bound(self._h, _ptr(out), count, _ptr(counters)))
         return out, counters
 
@@ -600,7 +605,7 @@ class GpuTracer:
         return out
 
     def read_counters_bound(self):
-        counters = np.zeros(5, np.uint64)
+        counters = np.zeros(BOUND_COUNTERS, np.uint64)
         _check(self._lib.vgi_read_bound(self._h, None, 0, _ptr(counters)))
         return counters
 

@@ -1685,7 +1685,8 @@ void vgi_destroy(vgi_tracer *t) {
                             &t->wf_vis, &t->wf_control, &t->d_order, &t->d_keys, &t->d_selection, &t->d_pick_tasks, &t->d_pick_slots,
                             &t->d_pick_results, &t->d_mirror, &t->d_select_ranges,
                             &t->d_footprint_tasks, &t->d_footprint_sums, &t->d_corners, &t->d_sun_flux, &t->d_sun_queue,
-                            &t->d_sun_ranges, &t->d_sun_region};
+                            &t->d_sun_ranges, &t->d_sun_region, &t->v_pixels, &t->v_counters, &t->f_cells[0], &t->f_cells[1],
+                            &t->f_output, &t->f_queue};
     for (auto &f : t->wf_fields) f.release();
     if (t->side) hipStreamSynchronize(t->side);
     for (DeviceBuffer *b : bufs) b->release();
