@@ -121,6 +121,23 @@ __device__ __forceinline__ uint4 region_header(const SceneView &s, int rx, int r
     size_t i = (size_t)(rx - s.minx) + (size_t)s.sx * ((size_t)(ry - s.miny) + (size_t)s.sy * (size_t)(rz - s.minz));
     return s.headers[i];
 }
+// (int)floorf(x) as one instruction, the floor and the conversion together
+// (v_cvt_flr_i32_f32; the same value for every finite x in int range, which
+// every traversal position is).
+__device__ __forceinline__ int floor_to_int(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// The loaded box's index of region (rx, ry, rz), NONE outside the box: one
+// unsigned compare per axis and 32-bit arithmetic (the box holds fewer than
+// 2^32 regions: its headers alone would fill 64 GiB).
+__device__ __forceinline__ uint32_t region_index(const SceneView &s, int rx, int ry, int rz) {
+    const uint32_t dx = (uint32_t)(rx - s.minx), dy = (uint32_t)(ry - s.miny), dz = (uint32_t)(rz - s.minz);
+    if (dx >= (uint32_t)s.sx || dy >= (uint32_t)s.sy || dz > (uint32_t)(s.maxz - s.minz)) return NONE;
+    return (dz * (uint32_t)s.sy + dy) * (uint32_t)s.sx + dx;
+}
 
 // voxel_traversal.wgsl:48-70 (voxel_material / voxel_cell): a voxel's
 // material, AIR for air and for voxels outside the loaded regions.
@@ -194,6 +211,39 @@ __device__ __forceinline__ void advance_box(f3 position, f3 direction, const Ray
     dist_out = fmx(distance + 0.001f, 0.001f);
 }
 
+// advance_box for a box given by its cell and packed extents (`field` bits
+// per axis, `shift` apart; the negative extents in `neg`, the positive ones
+// in `pos`). Per axis only the face the ray leaves through matters -- the
+// positive side when it moves up the axis, else the negative side -- so the
+// extent is taken from that word before it is unpacked. The planes, the
+// divisions and the choice are advance_box's, operation for operation.
+__device__ __forceinline__ void advance_packed(f3 position, f3 direction, const RayInverse &inv, int cx, int cy, int cz, uint32_t neg,
+                                               uint32_t pos, uint32_t shift, uint32_t field, float size, f3 &normal_out,
+                                               float &dist_out) {
+    const float dv[3] = {direction.x, direction.y, direction.z};
+    const float pv[3] = {position.x, position.y, position.z};
+    const double iv[3] = {inv.x, inv.y, inv.z};
+    const int cv[3] = {cx, cy, cz};
+    float tr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool up = dv[a] > 0.0f;
+        const int e = (int)(((up ? pos : neg) >> (shift * (uint32_t)a)) & field);
+        const float plane = (float)(up ? cv[a] + e + 1 : cv[a] - e) * size;
+        const bool moving = fabsf(dv[a]) > 1e-8f;
+        const float raw = div_by(plane - pv[a], iv[a]);  // (plane - p) / d
+        tr[a] = (moving && raw >= -0.001f && raw < 1e30f) ? raw : 1e30f;
+    }
+    bool choose_y = tr[1] < fmx(tr[0], 0.0f);
+    float distance_xy = fmx(choose_y ? tr[1] : tr[0], 0.0f);
+    bool choose_z = tr[2] < distance_xy;
+    float distance = fmx(choose_z ? tr[2] : distance_xy, 0.0f);
+    f3 axis = choose_z ? mk(0.0f, 0.0f, 1.0f) : (choose_y ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f));
+    f3 sgn = mk(direction.x > 0.0f ? -1.0f : 1.0f, direction.y > 0.0f ? -1.0f : 1.0f, direction.z > 0.0f ? -1.0f : 1.0f);
+    normal_out = distance < 1e30f ? axis * sgn : mk(0.0f, 0.0f, 0.0f);
+    dist_out = fmx(distance + 0.001f, 0.001f);
+}
+
 // voxel_traversal.wgsl:108-121
 __device__ __forceinline__ bool geometry_ahead(const SceneView &s, f3 start, f3 direction, float max_distance) {
     f3 position = start;
@@ -225,7 +275,9 @@ struct RayWalk {
     f3 position;
     float distance;
     f3 normal;
-    int lrx, lry, lrz;
+    uint32_t last_region;  // loaded-box index of the region `meta` belongs to (NONE: none)
+    uint32_t last_brick;   // cells index of the brick word `brick` holds (NONE: none)
+    uint32_t brick;
     uint4 meta;
     uint32_t step;
 };
@@ -271,7 +323,9 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
     w.distance = fmx(near_, 0.0f) + 0.001f;
     w.position = origin + direction * w.distance;
     w.normal = normal;
-    w.lrx = w.lry = w.lrz = 0x7fffffff;
+    w.last_region = NONE;
+    w.last_brick = NONE;
+    w.brick = 0u;
     w.meta = make_uint4(0u, 0u, 0u, 0u);
     w.step = 0u;
     return false;
@@ -297,11 +351,16 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
     tally.step();
     const f3 position = w.position, direction = w.direction;
     if (w.distance > w.max_distance) return true;
-    int vx = (int)floorf(position.x), vy = (int)floorf(position.y), vz = (int)floorf(position.z);
+    int vx = floor_to_int(position.x), vy = floor_to_int(position.y), vz = floor_to_int(position.z);
     int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
-    if (!in_bounds(s, rx, ry, rz)) return true;
-    if (rx != w.lrx || ry != w.lry || rz != w.lrz) {
-        uint4 h = region_header(s, rx, ry, rz);
+    const uint32_t region = region_index(s, rx, ry, rz);
+    if (region == NONE) return true;
+    if (region != w.last_region) {
+        uint4 h = s.headers[region];
+        // One 16-byte load: the kind decides the branch, the rest is used on
+        // most paths; without this the compiler loads the kind first and
+        // the other words after the branch, a second round trip.
+        asm volatile("" : "+v"(h.y), "+v"(h.z), "+v"(h.w));
         tally.header();
         if (h.x == 0u) {
             if constexpr (kVisible || Tally::kBudgeted) {
@@ -317,7 +376,7 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
             return true;
         }
         w.meta = h;
-        w.lrx = rx; w.lry = ry; w.lrz = rz;
+        w.last_region = region;
     }
     const uint4 meta = w.meta;
     // The empty box to skip -- the region's top-level box (kind 1), an air
@@ -333,7 +392,14 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
         if (meta.x == 3u) {
             uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
             uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
-            uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
+            // A step in the brick of the last step reads the same word again:
+            // it is kept (cells indices are unique across regions).
+            const uint32_t brick_at = meta.y + (bz * 4u + by) * 4u + bx;
+            if (brick_at != w.last_brick) {
+                w.brick = s.cells[brick_at];
+                w.last_brick = brick_at;
+            }
+            const uint32_t brick = w.brick;
             tally.brick();
             uint32_t st = brick & 3u;
             if (st == 0u) {
@@ -370,10 +436,7 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
     }
     f3 adv_n = mk(0.0f, 0.0f, 0.0f);
     float adv_w = 0.0f;
-    if (skip)
-        advance_box(position, direction, w.inv, cx, cy, cz, (int)(neg & field), (int)((neg >> shift) & field),
-                    (int)((neg >> (2u * shift)) & field), (int)(pos & field), (int)((pos >> shift) & field),
-                    (int)((pos >> (2u * shift)) & field), size, adv_n, adv_w);
+    if (skip) advance_packed(position, direction, w.inv, cx, cy, cz, neg, pos, shift, field, size, adv_n, adv_w);
     w.distance = w.distance + adv_w;
     w.position = position + direction * adv_w;
     w.normal = adv_n;
