@@ -276,8 +276,6 @@ struct RayWalk {
     float distance;
     f3 normal;
     uint32_t last_region;  // loaded-box index of the region `meta` belongs to (NONE: none)
-    uint32_t last_brick;   // cells index of the brick word `brick` holds (NONE: none)
-    uint32_t brick;
     uint4 meta;
     uint32_t step;
 };
@@ -324,8 +322,6 @@ __device__ __forceinline__ bool ray_start(const SceneView &s, f3 origin, f3 dire
     w.position = origin + direction * w.distance;
     w.normal = normal;
     w.last_region = NONE;
-    w.last_brick = NONE;
-    w.brick = 0u;
     w.meta = make_uint4(0u, 0u, 0u, 0u);
     w.step = 0u;
     return false;
@@ -392,14 +388,7 @@ __device__ __forceinline__ bool ray_step(const SceneView &s, RayWalk &w, uint32_
         if (meta.x == 3u) {
             uint32_t lx = (uint32_t)(vx & 15), ly = (uint32_t)(vy & 15), lz = (uint32_t)(vz & 15);
             uint32_t bx = lx >> 2, by = ly >> 2, bz = lz >> 2;
-            // A step in the brick of the last step reads the same word again:
-            // it is kept (cells indices are unique across regions).
-            const uint32_t brick_at = meta.y + (bz * 4u + by) * 4u + bx;
-            if (brick_at != w.last_brick) {
-                w.brick = s.cells[brick_at];
-                w.last_brick = brick_at;
-            }
-            const uint32_t brick = w.brick;
+            const uint32_t brick = s.cells[meta.y + (bz * 4u + by) * 4u + bx];
             tally.brick();
             uint32_t st = brick & 3u;
             if (st == 0u) {
