@@ -58,6 +58,15 @@ __device__ __forceinline__ bool same_receiver(const ViewReceiver &a, const ViewR
 
 __device__ __forceinline__ uint32_t pixel_stamp(uint32_t frame) { return frame & 0x1fffffffu; }
 
+// in_interest (voxel_view_runtime.wgsl:26-30): the voxel's region inside the
+// host's geometry interest -- the regions of the loaded pages (pages.rs:
+// 186-240), here the scene's loaded box. Outside it a rendered surface never
+// gets GI geometry: no receiver waits for it and no far cell lights it.
+__device__ __forceinline__ bool in_interest(const SceneView &s, int vx, int vy, int vz) {
+    const int rx = vx >> 4, ry = vy >> 4, rz = vz >> 4;
+    return rx >= s.minx && ry >= s.miny && rz >= s.minz && rx <= s.maxx && ry <= s.maxy && rz <= s.maxz;
+}
+
 // pixel_owned (voxel_view_pixels.wgsl:54-64): the far tier takes a face whose
 // voxel, at its distance from the camera (not its view depth, so turning
 // does not move the boundary), is under `threshold` pixels across (measure
@@ -102,8 +111,8 @@ __global__ void view_discover_kernel(ViewParams p, FarParams f, SceneView s, con
             // Faces small enough for the far tier are lit per pixel and own
             // no receiver (voxel_view_discovery.wgsl:59-64).
             const bool per_pixel = pixel_owned(p, f, vx, vy, vz, face);
-            if (per_pixel) {
-                // lit by the far tier
+            if (!in_interest(s, vx, vy, vz) || per_pixel) {
+                // outside the interest, or lit by the far tier
             } else if (receiver_exists(s, vx, vy, vz, face)) {
                 r = ViewReceiver{vx, vy, vz, face, 0.0f, 0.0f, 0.0f, 0u};
                 found = true;
@@ -684,7 +693,7 @@ __device__ __forceinline__ void wave_sum(uint32_t *counter, uint32_t value) {
 // The queue is filled by wavefront-compacted reservations (ballot, prefix,
 // one atomic per wavefront), so its order is the arrival order of
 // wavefronts; each queued cell's path depends only on the cell.
-__device__ __forceinline__ bool far_classify(const ViewParams &p, const FarParams &f, const float *__restrict__ depth,
+__device__ __forceinline__ bool far_classify(const ViewParams &p, const FarParams &f, const SceneView &s, const float *__restrict__ depth,
                                              const float *__restrict__ normals, const ViewReceiver *__restrict__ prev_receivers,
                                              const ViewEstimate *__restrict__ prev_estimates, const uint32_t *__restrict__ prev_hash,
                                              const PixelHistory *__restrict__ previous, uint32_t id, PixelHistory &cell,
@@ -712,7 +721,7 @@ __device__ __forceinline__ bool far_classify(const ViewParams &p, const FarParam
     const f3 surface = mk(sv[0], sv[1], sv[2]);
     const f3 inside = surface - n * 0.01f;
     const int vx = (int)floorf(inside.x), vy = (int)floorf(inside.y), vz = (int)floorf(inside.z);
-    if (!pixel_owned(p, f, vx, vy, vz, face)) return false;
+    if (!in_interest(s, vx, vy, vz) || !pixel_owned(p, f, vx, vy, vz, face)) return false;
     uint32_t sequence;
     float4 history = previous_cell(p, f, previous, surface, face, sequence);
     if (history.w > 0.0f) {
@@ -760,7 +769,8 @@ __global__ __launch_bounds__(256) void far_classify_kernel(ViewParams p, FarPara
     PixelHistory cell;
     bool reprojected = false, seeded = false;
     const bool owned = id < f.cells &&
-                       far_classify(p, f, depth, normals, prev_receivers, prev_estimates, prev_hash, b.previous, id, cell, reprojected, seeded);
+                       far_classify(p, f, s, depth, normals, prev_receivers, prev_estimates, prev_hash, b.previous, id, cell, reprojected,
+                                    seeded);
     bool queued = false;
     if (owned) {
         b.cells[id] = cell;

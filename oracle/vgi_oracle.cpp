@@ -3681,6 +3681,14 @@ struct FarFrame {
     uint32_t previous, frame, custom, steps, cells;
 };
 
+// in_interest (voxel_view_runtime.wgsl:26-30): the loaded scene's region box.
+static bool view_in_interest(const Tracer &t, I3 v) {
+    const I3 r{v.x >> 4, v.y >> 4, v.z >> 4};
+    const Packed &p = t.packed;
+    return r.x >= p.minimum.x && r.y >= p.minimum.y && r.z >= p.minimum.z && r.x <= p.maximum.x && r.y <= p.maximum.y &&
+           r.z <= p.maximum.z;
+}
+
 // pixel_owned (voxel_view_pixels.wgsl:54-64), unit voxels.
 static bool far_pixel_owned(const orc_view_camera &c, const FarFrame &f, I3 v, uint32_t face) {
     if ((f.flags & 1u) == 0u || face >= 6u) return false;
@@ -3709,8 +3717,9 @@ static bool view_discover(const Tracer &t, const orc_view_camera &c, uint32_t i,
     F3 surface = f3(sv[0], sv[1], sv[2]);
     F3 inside = surface - n * 0.01f;
     I3 v{(int32_t)std::floor(inside.x), (int32_t)std::floor(inside.y), (int32_t)std::floor(inside.z)};
-    // Faces small enough for the far tier own no receiver (voxel_view_discovery.wgsl:59-64).
-    if (far_pixel_owned(c, f, v, face)) return false;
+    // Outside the interest no receiver waits; faces small enough for the far
+    // tier own none (voxel_view_discovery.wgsl:53-64).
+    if (!view_in_interest(t, v) || far_pixel_owned(c, f, v, face)) return false;
     if (view_receiver_exists(t, v, face)) { voxel = v; return true; }
     float footprint = view_distance(view_unproject(c, px + 1.0f, py, z), position);
     uint32_t u = (axis + 1u) % 3u, w = (axis + 2u) % 3u;
@@ -4195,7 +4204,7 @@ int orc_view_frame(void *tp, const orc_view_camera *cam, const float *depth, con
             const F3 surface = f3(sv[0], sv[1], sv[2]);
             const F3 inside = surface - n * 0.01f;
             const I3 v{(int32_t)std::floor(inside.x), (int32_t)std::floor(inside.y), (int32_t)std::floor(inside.z)};
-            if (!far_pixel_owned(c, f, v, face)) continue;
+            if (!view_in_interest(t, v) || !far_pixel_owned(c, f, v, face)) continue;
             counters[0] += 1u;
             float history[4];
             uint32_t sequence;

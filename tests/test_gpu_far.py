@@ -5,7 +5,7 @@ The reference's far-tier rig (gpu/view/pixel_tests.rs:24-330) in the
 configurations its native tests use -- 2-pixel cells at rate 1 with a
 one-cell pan, the 1440p defaults (3-pixel cells, rate 4) for 36 frames, a
 custom light, a one-step ray budget under an emissive slab, a shortened
-reach -- and a 64^3 terrain whose distant faces fall to the far tier while
+reach, half the view beyond the loaded page (the geometry interest) -- and a 64^3 terrain whose distant faces fall to the far tier while
 near ones own view receivers, with the resident world cache exchanging
 estimates and presentation: owners, receivers, estimates (with the shown
 presentation), published records, counters, the material's output and the
@@ -43,17 +43,19 @@ def compare(vgi, g, t, cam, depth, normals, label, clock=0):
     return ccounters, cout, ccells
 
 
-@pytest.mark.parametrize("case", ["pan", "defaults", "custom", "one_step", "reach"])
+@pytest.mark.parametrize("case", ["pan", "defaults", "custom", "one_step", "reach", "interest"])
 def test_far_rig_bit_exact(vgi, orc, case):
     kw = dict(pan=dict(cell=2, rate=1), defaults=dict(cell=3, rate=4), custom=dict(cell=3, rate=4, custom_light=(16.0, 10.0, 16.0)),
               one_step=dict(cell=3, rate=4, ceiling=5, custom_light=(16.0, 31.0, 16.0), environment=(0.0, 0.0, 0.0), steps=1),
-              reach=dict(cell=3, rate=1, ceiling=5, environment=(0.0, 0.0, 0.0), reach=16.0))[case]
+              reach=dict(cell=3, rate=1, ceiling=5, environment=(0.0, 0.0, 0.0), reach=16.0),
+              interest=dict(cell=2, rate=1))[case]
     t = rig.oracle(vgi, **kw)
     g = rig.gpu(vgi, **kw)
     depth, normals = rig.prepass()
     frames = 36 if case == "defaults" else 8
     for n in range(frames):
-        shift = 4.0 if case == "pan" and n == frames - 1 else 0.0
+        # "interest": panned 16 voxels, half the view lies beyond the loaded page.
+        shift = 4.0 if case == "pan" and n == frames - 1 else (16.0 if case == "interest" else 0.0)
         counters, output, _ = compare(vgi, g, t, rig.camera(vgi, n, shift), depth, normals, "%s %d" % (case, n))
         assert counters[0] > 0 and output[15] == n
     g.close()

@@ -62,10 +62,15 @@ def test_small_faces_are_lit_per_pixel_without_receivers():
         assert rig.f16(output[16] & 0xFFFF) > 0.0, "the floor sees the sky"
         assert counters[0] == 64 and counters[1] == 64, "every cell owned and traced (rate 1)"
         assert counters[3] == (64 if n > 0 else 0), "reprojected from the previous frame"
-    # A one-cell pan (four voxels): history follows the world point.
-    (_, _, output, _), _ = frame(t, vgi, 6, shift=4.0)
-    kept = int((rig.counts(output, 64) > 1.0).sum())
-    assert kept >= 48, "reprojected history: %d of 64" % kept
+    # A one-cell pan (four voxels): history follows the world point. (The
+    # last column of cells now samples x = 33..35, beyond the loaded page:
+    # outside the interest, those cells are not written.)
+    (_, counters, output, _), _ = frame(t, vgi, 6, shift=4.0)
+    words = output[16:].reshape(64, 4)
+    stamped = np.flatnonzero(words[:, 2] >> 3 == 6)
+    assert len(stamped) == counters[0] >= 56
+    kept = int((rig.counts(output, 64, stamped) > 1.0).sum())
+    assert kept >= 48, "reprojected history: %d of %d" % (kept, len(stamped))
 
 
 def test_cells_at_the_default_settings_sample_every_pixel_and_index():
@@ -173,3 +178,21 @@ def test_cells_hand_off_to_and_from_view_receivers():
     seeded = f[(f["face"] == 2) & (f["mean"][:, 3] == 1.0)]
     assert counters[5] > 0 and len(seeded) == counters[5]
     assert np.all(seeded["shown"][:, 3] == 1.0) and np.all(seeded["moment"][:, 3] == 0.0)
+
+
+def test_far_cells_keep_to_the_interest():
+    # voxel_view_runtime.wgsl:26-30, voxel_view_pixels.wgsl:160: a pixel whose
+    # surface lies outside the loaded regions (the host's geometry interest)
+    # owns no far cell. Panned 16 voxels, the prepass's right half shows
+    # x = 33..47, beyond the 32-voxel page.
+    vgi = package()
+    t = rig.oracle(vgi, cell=2, rate=1)
+    (_, counters, output, history), _ = frame(t, vgi, 0, shift=16.0)
+    stamped = (history[:, 7] >> 3) == 0
+    stamped &= (history[:, 7] & 7) == 2
+    x = history[:, 4].view(np.float32)
+    assert counters[0] == int(stamped.sum()) and 16 <= counters[0] < 64
+    assert np.all(x[stamped] < 32.0), "no cell outside the loaded page"
+    columns = {int(c) % 8 for c in np.flatnonzero(stamped)}
+    assert columns <= set(range(8)) and max(columns) <= 7
+
