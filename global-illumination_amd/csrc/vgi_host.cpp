@@ -2540,6 +2540,15 @@ int vgi_view_frame(vgi_tracer *t, const vgi_view_camera *camera, const float *de
     // The far tier's cell grid (settings.rs:141-147, gpu/view/runtime.rs:912-916).
     const vgi_far_settings &fs = t->far;
     const bool far_on = fs.voxel_pixels > 0.0f;
+    if (far_on) {
+        // The far tier measures faces against the camera's forward axis and
+        // its pixels per unit (view.pixel_axis, gpu/view/runtime.rs:982-987).
+        const float *fw = camera->forward;
+        const float len2 = fw[0] * fw[0] + fw[1] * fw[1] + fw[2] * fw[2];
+        if (!(camera->pixels_per_unit > 0.0f) || !std::isfinite(camera->pixels_per_unit))
+            return fail("the far tier needs the camera's pixels per unit at view depth 1 (> 0)");
+        if (!(len2 > 0.5f && len2 < 2.0f)) return fail("the far tier needs the camera's unit forward vector");
+    }
     uint32_t size = fs.cell != 0u ? fs.cell : (uint32_t)std::lround((double)((float)camera->height / 480.0f));
     size = std::min(std::max(size, 1u), 4u);
     const uint32_t cells_x = (camera->width + size - 1u) / size, cells_y = (camera->height + size - 1u) / size;

@@ -41,6 +41,21 @@ def test_invalid_arguments_are_errors_and_the_tracer_stays_usable(vgi, orc):
     raises(vgi, lambda: g.mlp_train_samples(0), "1..64")
     raises(vgi, lambda: g.mlp_table_lr(-1.0), "learning rate")
     raises(vgi, lambda: g.mlp_table_grad(), "no gradient")
+    print("case: far tier settings and camera", flush=True)
+    raises(vgi, lambda: g.view_far(vgi.far_settings(rate=9)), "rate")
+    raises(vgi, lambda: g.view_far(vgi.far_settings(steps=0)), "steps")
+    raises(vgi, lambda: g.view_far(vgi.far_settings(cell=5)), "cell")
+    raises(vgi, lambda: g.view_display(0.031, 2.0, 0.2), "spatial")
+    g.view_far(vgi.far_settings())
+    depth = np.full((4, 4), 0.5, np.float32)
+    normals = np.zeros((4, 4, 3), np.float32)
+    normals[..., 1] = 1.0
+    wfc = np.eye(4)
+    raises(vgi, lambda: g.view_frame(vgi.view_camera(wfc, (0.0, 9.0, 0.0), 4, 4, forward=(0.0, -1.0, 0.0)), depth, normals),
+           "pixels per unit")
+    raises(vgi, lambda: g.view_frame(vgi.view_camera(wfc, (0.0, 9.0, 0.0), 4, 4, forward=(0.0, 0.0, 0.0), pixels_per_unit=2.0),
+                                     depth, normals), "forward")
+    g.view_far(None)
     # Still usable: a valid frame runs and its records are finite.
     print("case: a valid frame after the errors", flush=True)
     g.set_transport(bounces=2, samples=2)
