@@ -120,8 +120,12 @@ def mlp_roofline(timings):
     learned frames (medians): inference at 29,056 FLOP per query; a training
     sample's forward, backward-data and weight-gradient GEMMs at 3 x 29,056."""
     med = {k: float(np.median([t[k] for t in timings])) for k in timings[0]}
-    infer_tflops = med["queries"] * MLP_FLOP_PER_QUERY / (med["infer_ms"] / 1000.0) / 1e12
-    train_tflops = med["train_samples"] * 3 * MLP_FLOP_PER_QUERY / (med["train_mfma_ms"] / 1000.0) / 1e12
+    # Rates frame by frame, then their median: a schedule that changes the
+    # training load part-way keeps each frame's samples with its own time.
+    infer_tflops = float(np.median([t["queries"] * MLP_FLOP_PER_QUERY / (t["infer_ms"] / 1000.0) / 1e12
+                                    for t in timings]))
+    train_tflops = float(np.median([t["train_samples"] * 3 * MLP_FLOP_PER_QUERY / (t["train_mfma_ms"] / 1000.0) / 1e12
+                                    for t in timings if t["train_mfma_ms"] > 0.0] or [0.0]))
     return {
         "bound": "mfma", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
         "inference": {"achieved": round(infer_tflops, 2), "frac": round(infer_tflops / MFMA_BF16_PEAK_TFLOPS, 5),
@@ -134,13 +138,14 @@ def mlp_roofline(timings):
     }
 
 
-def run_to_target(vgi, g, judged, reference, var, max_frames, target, budget_ms, every=8, schedule=None):
+def run_to_target(vgi, g, judged, reference, var, max_frames, target, budget_ms, every=8, schedule=None, mlp=None):
     """Frames on the bound faces (the judged ones bound first) until the
     judged faces' relative RMSE against the reference reaches `target` or the
     GPU time (from `budget_ms`'s origin) runs out; `schedule` maps a frame to
-    the learned cache's (targets, steps, rate) from then on. Returns the
-    (frame, cumulative GPU ms, rmse with the reference's variance removed,
-    raw rmse) points evaluated."""
+    the learned cache's (targets, steps, rate) from then on; a list `mlp`
+    receives the learned cache's kernel timings of every evaluated frame.
+    Returns the (frame, cumulative GPU ms, rmse with the reference's variance
+    removed, raw rmse) points evaluated."""
     points = []
     gpu_ms = 0.0
     for f in range(1, max_frames + 1):
@@ -150,6 +155,8 @@ def run_to_target(vgi, g, judged, reference, var, max_frames, target, budget_ms,
         g.sync()
         gpu_ms += g.last_frame_ms()
         if f <= 8 or f % every == 0 or f == max_frames or gpu_ms >= budget_ms:
+            if mlp is not None:
+                mlp.append(g.mlp_timings())
             rec, _ = g.read_bound(judged)
             est = rec[:, 0:3] + rec[:, 16:19]
             points.append((f, gpu_ms, vgi.relative_rmse(est, reference, var), vgi.relative_rmse(est, reference)))
@@ -229,10 +236,11 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
         g.mlp_pretrain(args.pretrain_frames)
         g.sync()
         pretrain_ms = (time.perf_counter() - t1) * 1000.0
+        timings = []
         pts = run_to_target(vgi, g, judged, reference, var, args.learned_frames, args.target_rmse,
-                            args.convergence_budget_ms - pretrain_ms, schedule=schedule)
+                            args.convergence_budget_ms - pretrain_ms, schedule=schedule, mlp=timings)
         g.close()
-        legs[name] = (pts, pretrain_ms, LEARNED_SPP, (encoding, train, lr, schedule))
+        legs[name] = (pts, pretrain_ms, LEARNED_SPP, (encoding, train, lr, schedule, timings))
         print("%s: %d frames, %.0f GPU ms, rmse %.4f (raw %.4f) (pretrain %.0f ms)" % ((name,) + pts[-1] + (pretrain_ms,)),
               file=sys.stderr, flush=True)
     budgets = [1000.0, 3000.0, 10000.0, 20000.0, 30000.0]
@@ -252,7 +260,8 @@ def convergence(vgi, coords, pages, optics, faces, bounces, spp, args):
             "rays_per_face_per_frame": rays,
         }
         if net is not None:
-            encoding, train, lr, schedule = net
+            encoding, train, lr, schedule, timings = net
+            entry["mlp"] = mlp_roofline(timings)
             entry["pretrain_frames"] = args.pretrain_frames
             entry["pretrain_ms"] = round(pre_ms, 1)
             entry["history_recency"] = LEARNED_RECENCY
@@ -776,6 +785,9 @@ def main():
         line["batched_scenes"] = batched_scenes(vgi, args)
     if not args.no_convergence and args.workload == "cave512":
         line["convergence"] = convergence(vgi, coords, pages, optics, faces, bounces, spp, args)
+        # The learned cache's BF16 MFMA kernels (inference, training GEMMs)
+        # against the dense BF16 peak, from the learned leg's frames.
+        line["roofline_mlp"] = line["convergence"]["learned"]["mlp"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(vgi, grid, coords, pages, optics, faces, bounces, spp, args.cpu_budget)
     print(json.dumps(line))
