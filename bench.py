@@ -753,6 +753,10 @@ def main():
             "workload": desc,
             "faces": int(len(faces)),
             "rays_per_frame": int(rays / args.steps),
+            "per_ray": {"steps": round(gather_counts[1] / gather_counts[0], 3),
+                        "header_loads": round(gather_counts[2] / gather_counts[0], 3),
+                        "brick_word_loads": round(gather_counts[3] / gather_counts[0], 3),
+                        "cell_word_loads": round(gather_counts[4] / gather_counts[0], 3)},
             "faces_per_s": round(len(faces) * args.steps / elapsed, 1),
             "trace_ms_per_frame": round(float(np.mean(trace_ms)), 3),
             "method": "plain: paths of up to 4 bounces (Russian roulette after the second hit, survival clamp(max throughput, 0.1, 0.95); no cache termination); temporal merge on",

@@ -878,11 +878,18 @@ int sync_emitters(vgi_tracer &t, bool no_alloc = false) {
                         regions.push_back(Int3{t.rmin.x + x, t.rmin.y + y, t.rmin.z + z});
                 }
     vgi_host::build_emitters(src, regions, t.emission, t.emitters);
-    if (no_alloc && (t.emitters.words.size() * 4 > t.d_emitters.bytes || t.d_emitters.ptr == nullptr))
+    // The walk words hold a child's index in 26 bits.
+    if (t.emitters.nodes >= (1u << 26)) return fail("more than 2^25 emissive faces: the light tree's node indices exceed 26 bits");
+    size_t words = t.emitters.words.size(), walk = t.emitters.walk.size();
+    if (no_alloc && ((words + walk) * 4 > t.d_emitters.bytes || t.d_emitters.ptr == nullptr))
         return fail("the emitter table outgrew its device buffer: one allocating call (vgi_submit_faces or vgi_run_frame) "
                     "must follow this scene or material change");
-    HIP_OK(t.d_emitters.reserve(t.emitters.words.size() * 4));
-    HIP_OK(hipMemcpyAsync(t.d_emitters.ptr, t.emitters.words.data(), t.emitters.words.size() * 4, hipMemcpyHostToDevice, t.stream));
+    // The emitter words, then the tree's walk words (Optics::tree).
+    HIP_OK(t.d_emitters.reserve((words + walk) * 4));
+    HIP_OK(hipMemcpyAsync(t.d_emitters.ptr, t.emitters.words.data(), words * 4, hipMemcpyHostToDevice, t.stream));
+    if (walk)
+        HIP_OK(hipMemcpyAsync(t.d_emitters.as<uint32_t>() + words, t.emitters.walk.data(), walk * 4, hipMemcpyHostToDevice,
+                              t.stream));
     HIP_OK(hipStreamSynchronize(t.stream));
     t.emitters_dirty = false;
     return 0;
@@ -907,6 +914,7 @@ void views(vgi_tracer &t, vgi::SceneView &s, vgi::Optics &o) {
     o.materials = t.d_materials.as<float4>();
     o.lights = t.d_lights.as<float4>();
     o.emitters = t.d_emitters.as<uint4>();
+    o.tree = t.d_emitters.as<uint4>() + t.emitters.words.size() / 4;
     o.light_count = t.light_count;
     o.directional = std::min(t.directional, t.light_count);
     o.emitter_faces = t.emitters.faces;

@@ -355,6 +355,27 @@ void build_emitters(const VoxelSource &scene, const std::vector<Int3> &regions, 
         w[8] = (uint32_t)n.pivot[0]; w[9] = (uint32_t)n.pivot[1]; w[10] = (uint32_t)n.pivot[2]; w[11] = n.face;
         w[12] = n.left; w[13] = n.right; w[14] = n.axis; w[15] = 0u;
     }
+    // The walk's view of each node (voxel_path.wgsl:108-131's per-node
+    // terms): (mn + mx) * 0.5, len(mx - mn) * 0.5 with the dot product summed
+    // x, y, z, and left.power / (left.power + right.power), each rounded as
+    // binary32 exactly as the walk computes them.
+    out.walk.assign(nodes.size() * 8, 0u);
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        const TreeNode &n = nodes[i];
+        uint32_t *w = &out.walk[i * 8];
+        float e[3], c[3];
+        for (int a = 0; a < 3; ++a) {
+            c[a] = (n.mn[a] + n.mx[a]) * 0.5f;
+            e[a] = n.mx[a] - n.mn[a];
+        }
+        float sq = e[0] * e[0] + e[1] * e[1];
+        sq = sq + e[2] * e[2];
+        float radius = std::sqrt(sq) * 0.5f;
+        float fraction = 0.0f;
+        if (n.left != NONE) fraction = nodes[n.left].power / (nodes[n.left].power + nodes[n.right].power);
+        w[0] = fbits(c[0]); w[1] = fbits(c[1]); w[2] = fbits(c[2]); w[3] = fbits(radius);
+        w[4] = fbits(n.power); w[5] = fbits(fraction); w[6] = n.left; w[7] = n.right | (n.normals << 26);
+    }
 }
 
 // ---------------------------------------------------------------- face index

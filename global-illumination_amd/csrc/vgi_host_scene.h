@@ -55,6 +55,12 @@ struct TopLevelField {
 // kernels read (gpu.rs:686-713, tree.rs:170-192).
 struct EmitterTable {
     std::vector<uint32_t> words;
+    // The light tree as the device walks it, 8 words per node: centre xyz and
+    // radius of its box; power, its left child's share of its power, left
+    // child (NONE for a leaf), right child (a leaf's emitter face) with the
+    // node's normal bits above bit 26. Derived from `words` by the same
+    // binary32 operations the walk would otherwise repeat at every visit.
+    std::vector<uint32_t> walk;
     uint32_t faces = 0;
     uint32_t nodes = 0;
     float total_weight = 0.0f;

@@ -30,6 +30,7 @@ struct Optics {
     const float4 *__restrict__ materials;   // 2 per material: (diffuse rgb, emission power), (emission rgb, 0)
     const float4 *__restrict__ lights;      // 3 per light (gpu.rs:136-141)
     const uint4 *__restrict__ emitters;     // 2 uint4 per emissive face, then 4 per tree node
+    const uint4 *__restrict__ tree;         // 2 uint4 per tree node: the walk's terms (centre, radius, power, share, children)
     uint32_t light_count;
     uint32_t directional;                   // params.maximum.w: the first `directional` lights are suns
     uint32_t emitter_faces;                 // params.extra.w
@@ -481,35 +482,38 @@ __device__ __forceinline__ Emitter load_emitter(const Optics &o, uint32_t index)
     return Emitter{(int)a.x, (int)a.y, (int)a.z, a.w, __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z),
                    __uint_as_float(b.w)};
 }
-struct Node {
-    f3 minimum;
-    float power;
-    f3 maximum;
-    uint32_t normals;
-    int px, py, pz;
-    uint32_t face;
-    uint4 children;
+// A tree node as the walk reads it: the per-node terms of
+// voxel_path.wgsl:108-131 that depend on the node alone, rounded once when
+// the table is built ((min + max) * 0.5, len(max - min) * 0.5 and the left
+// child's share of the power), so each visit computes only what depends on
+// the shading point.
+struct WalkNode {
+    f3 centre;
+    float radius, power, share;
+    uint32_t left, right, normals;
 };
-__device__ __forceinline__ Node load_node(const Optics &o, uint32_t index) {
-    uint32_t off = o.emitter_faces * 2u + index * 4u;
-    uint4 a = o.emitters[off], b = o.emitters[off + 1u], c = o.emitters[off + 2u], d = o.emitters[off + 3u];
-    Node n;
-    n.minimum = mk(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
-    n.power = __uint_as_float(a.w);
-    n.maximum = mk(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
-    n.normals = b.w;
-    n.px = (int)c.x; n.py = (int)c.y; n.pz = (int)c.z;
-    n.face = c.w;
-    n.children = d;
+__device__ __forceinline__ WalkNode load_walk(const Optics &o, uint32_t index) {
+    uint4 a = o.tree[index * 2u], b = o.tree[index * 2u + 1u];
+    WalkNode n;
+    n.centre = mk(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+    n.radius = __uint_as_float(a.w);
+    n.power = __uint_as_float(b.x);
+    n.share = __uint_as_float(b.y);
+    n.left = b.z;
+    n.right = b.w & 0x3ffffffu;
+    n.normals = b.w >> 26u;
     return n;
 }
-// voxel_path.wgsl:108-122
-__device__ __forceinline__ float node_importance(const Node &node, f3 position, f3 normal) {
-    f3 delta = (node.minimum + node.maximum) * 0.5f - position;
-    float radius = len3(node.maximum - node.minimum) * 0.5f;
+// voxel_path.wgsl:108-122. The four quotients by the distance go through
+// one binary64 reciprocal, RN32(RN64(n * RN64(1/d))), which equals binary32
+// division RN32(n/d) for every binary32 n and d whose quotient is normal
+// (the division check in tests/host covers it).
+__device__ __forceinline__ float node_importance(const WalkNode &node, f3 position, f3 normal) {
+    f3 delta = node.centre - position;
     float distance = fmx(len3(delta), 1e-5f);
-    f3 direction = delta / distance;
-    float spread = fmn(radius / distance, 1.0f);
+    double inv = recip64(distance);
+    f3 direction = mk(div_by(delta.x, inv), div_by(delta.y, inv), div_by(delta.z, inv));
+    float spread = fmn(div_by(node.radius, inv), 1.0f);
     float receiver = fclamp(dot3(normal, direction) + spread, 0.0f, 1.0f);
     float ppx = (node.normals & 1u) ? -direction.x : -1.0f, ppy = (node.normals & 4u) ? -direction.y : -1.0f,
           ppz = (node.normals & 16u) ? -direction.z : -1.0f;
@@ -517,37 +521,49 @@ __device__ __forceinline__ float node_importance(const Node &node, f3 position, 
           pnz = (node.normals & 32u) ? direction.z : -1.0f;
     float qx = fmx(ppx, pnx), qy = fmx(ppy, pny), qz = fmx(ppz, pnz);
     float emitter = fclamp(fmx(fmx(qx, qy), qz) + spread, 0.0f, 1.0f);
-    return node.power * receiver * emitter / (distance * distance + radius * radius + 0.25f);
+    return node.power * receiver * emitter / (distance * distance + node.radius * node.radius + 0.25f);
 }
 // voxel_path.wgsl:123-131
-__device__ __forceinline__ float node_left_probability(const Optics &o, const Node &node, f3 position, f3 normal) {
-    Node left = load_node(o, node.children.x);
-    Node right = load_node(o, node.children.y);
-    float power = left.power / (left.power + right.power);
+__device__ __forceinline__ float node_left_probability(const WalkNode &node, const WalkNode &left, const WalkNode &right, f3 position,
+                                                       f3 normal) {
     float a = node_importance(left, position, normal);
     float b = node_importance(right, position, normal);
-    if (a + b <= 1e-20f) return fclamp(power, 1e-6f, 1.0f - 1e-6f);
-    return fclamp(0.05f * power + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
+    if (a + b <= 1e-20f) return fclamp(node.share, 1e-6f, 1.0f - 1e-6f);
+    return fclamp(0.05f * node.share + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
+}
+__device__ __forceinline__ WalkNode pick(bool first, const WalkNode &a, const WalkNode &b) {
+    WalkNode n;
+    n.centre = mk(first ? a.centre.x : b.centre.x, first ? a.centre.y : b.centre.y, first ? a.centre.z : b.centre.z);
+    n.radius = first ? a.radius : b.radius;
+    n.power = first ? a.power : b.power;
+    n.share = first ? a.share : b.share;
+    n.left = first ? a.left : b.left;
+    n.right = first ? a.right : b.right;
+    n.normals = first ? a.normals : b.normals;
+    return n;
 }
 // voxel_path.wgsl:132-149
 __device__ __forceinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 normal, float choice) {
-    uint32_t index = 0u;
     float uniform = choice;
     float probability = 1.0f;
+    WalkNode node = load_walk(o, 0u);
     for (;;) {
-        Node node = load_node(o, index);
-        if (node.children.x == NONE) {
-            Emitter e = load_emitter(o, node.children.y);
+        if (node.left == NONE) {
+            Emitter e = load_emitter(o, node.right);
             e.probability = probability;
             return e;
         }
-        float p = node_left_probability(o, node, position, normal);
-        if (uniform < p) { probability = probability * p; uniform = uniform / p; index = node.children.x; }
-        else { probability = probability * (1.0f - p); uniform = (uniform - p) / (1.0f - p); index = node.children.y; }
+        WalkNode left = load_walk(o, node.left), right = load_walk(o, node.right);
+        float p = node_left_probability(node, left, right, position, normal);
+        bool go_left = uniform < p;
+        if (go_left) { probability = probability * p; uniform = uniform / p; }
+        else { probability = probability * (1.0f - p); uniform = (uniform - p) / (1.0f - p); }
         uniform = fmn(uniform, __uint_as_float(0x3f7fffffu));
+        node = pick(go_left, left, right);
     }
 }
-// voxel_path.wgsl:158-174 (key_less :150-157)
+// voxel_path.wgsl:158-174 (key_less :150-157); the split key (pivot voxel,
+// face, axis) comes from the node's emitter-table words.
 __device__ __forceinline__ float tree_probability(const Optics &o, f3 position, f3 normal, int vx, int vy, int vz, f3 hit_normal) {
     uint32_t face = 0u;
     for (uint32_t i = 0u; i < 6u; ++i) {
@@ -556,23 +572,27 @@ __device__ __forceinline__ float tree_probability(const Optics &o, f3 position, 
     }
     uint32_t index = 0u;
     float probability = 1.0f;
+    WalkNode node = load_walk(o, 0u);
     for (;;) {
-        Node node = load_node(o, index);
-        if (node.children.x == NONE) {
-            Emitter f = load_emitter(o, node.children.y);
+        if (node.left == NONE) {
+            Emitter f = load_emitter(o, node.right);
             return (f.vx == vx && f.vy == vy && f.vz == vz && f.face == face) ? probability : 0.0f;
         }
-        float p = node_left_probability(o, node, position, normal);
-        uint32_t axis = node.children.z;
+        WalkNode left = load_walk(o, node.left), right = load_walk(o, node.right);
+        float p = node_left_probability(node, left, right, position, normal);
+        uint32_t off = o.emitter_faces * 2u + index * 4u;
+        uint4 key = o.emitters[off + 2u];
+        uint32_t axis = o.emitters[off + 3u].z;
         const int v[3] = {vx, vy, vz};
-        const int pv[3] = {node.px, node.py, node.pz};
-        bool less = face < node.face;
+        const int pv[3] = {(int)key.x, (int)key.y, (int)key.z};
+        bool less = face < key.w;
         for (uint32_t j = 0u; j < 3u; ++j) {
             uint32_t i = (axis + j) % 3u;
             if (v[i] != pv[i]) { less = v[i] < pv[i]; break; }
         }
-        if (less) { probability = probability * p; index = node.children.x; }
-        else { probability = probability * (1.0f - p); index = node.children.y; }
+        if (less) { probability = probability * p; index = node.left; }
+        else { probability = probability * (1.0f - p); index = node.right; }
+        node = pick(less, left, right);
     }
 }
 // voxel_path.wgsl:309-324

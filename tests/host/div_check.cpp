@@ -48,6 +48,14 @@ int main(int argc, char **argv) {
         float t = std::ldexp(unit(rng), -26);
         if (std::fabs(t) > 1e-8f && !check(n, t)) return 1;
     }
+    // The light tree's quotients: a node's offset and radius over the
+    // distance to it (at least 1e-5, at most a few thousand voxels).
+    std::uniform_real_distribution<float> logd(-16.7f, 11.5f);
+    for (long i = 0; i < count / 4; ++i) {
+        float d = std::exp2(logd(rng));
+        float n = (i & 1) ? num(rng) : std::ldexp(unit(rng), expo(rng));
+        if (!check(n, d)) return 1;
+    }
     // Quotients next to binary32 midpoints: n = RN32(m * d) for a midpoint m
     // of two neighbouring floats, and its neighbours.
     std::uniform_int_distribution<uint32_t> mant(0, (1u << 23) - 1);

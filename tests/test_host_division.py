@@ -1,9 +1,11 @@
 """The traversal divides by a ray's direction components through a binary64
 reciprocal taken once per ray: RN32(RN64(n * RN64(1/d))) (vgi_trace.h
-div_by). Built alone with g++ (IEEE SSE arithmetic, no contraction) and
-checked bit for bit against binary32 division on random operands over the
-traversal's ranges, quotients next to binary32 midpoints, neighbouring
-operands, powers of two, signed zeros and the still axis's divisor 1."""
+div_by), and the light tree's walk divides a node's offset and radius by
+its distance the same way. Built alone with g++ (IEEE SSE arithmetic, no
+contraction) and checked bit for bit against binary32 division on random
+operands over the traversal's and the walk's ranges, quotients next to
+binary32 midpoints, neighbouring operands, powers of two, signed zeros and
+the still axis's divisor 1."""
 import os
 import shutil
 import subprocess
