@@ -171,8 +171,27 @@ __device__ __forceinline__ uint32_t voxel_material(const SceneView &s, int vx, i
 // quotients here stay far from binary32 overflow and subnormals (|n| <
 // 2^20, 1e-8 < |d| <= 1). It costs two conversions and one binary64
 // multiply instead of the ~10-instruction correctly rounded binary32 divide.
-__device__ __forceinline__ double recip64(float d) { return 1.0 / (double)d; }
+// The reciprocal need not be the correctly rounded one: the argument holds
+// for any r within a few binary64 ulps of 1/d (the error budget is 2^-49
+// against the product's ~2^-52), so r is the hardware's reciprocal estimate
+// refined by two Newton steps (each squares the relative error; the last
+// rounding leaves r within half an ulp plus 2^-100 of 1/d) instead of the
+// full binary64 division sequence with its scaling and fix-up.
+__device__ __forceinline__ double recip64(float d) {
+    const double x = (double)d;
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
 __device__ __forceinline__ float div_by(float n, double r) { return (float)((double)n * r); }
+// a / s for the three components through one reciprocal (s a normal binary32
+// number, the quotients normal or zero).
+__device__ __forceinline__ f3 div3(f3 a, float s) {
+    const double r = recip64(s);
+    return mk(div_by(a.x, r), div_by(a.y, r), div_by(a.z, r));
+}
 
 // The reciprocals a ray's box steps divide by: 1/d for an axis the ray moves
 // along (|d| > 1e-8), 1 otherwise (voxel_traversal.wgsl:94-99 divides by 1).
@@ -524,42 +543,40 @@ __device__ __forceinline__ float node_importance(const WalkNode &node, f3 positi
     return node.power * receiver * emitter / (distance * distance + node.radius * node.radius + 0.25f);
 }
 // voxel_path.wgsl:123-131
-__device__ __forceinline__ float node_left_probability(const WalkNode &node, const WalkNode &left, const WalkNode &right, f3 position,
+__device__ __forceinline__ float node_left_probability(float share, const WalkNode &left, const WalkNode &right, f3 position,
                                                        f3 normal) {
     float a = node_importance(left, position, normal);
     float b = node_importance(right, position, normal);
-    if (a + b <= 1e-20f) return fclamp(node.share, 1e-6f, 1.0f - 1e-6f);
-    return fclamp(0.05f * node.share + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
+    if (a + b <= 1e-20f) return fclamp(share, 1e-6f, 1.0f - 1e-6f);
+    return fclamp(0.05f * share + 0.95f * a / (a + b), 1e-6f, 1.0f - 1e-6f);
 }
-__device__ __forceinline__ WalkNode pick(bool first, const WalkNode &a, const WalkNode &b) {
-    WalkNode n;
-    n.centre = mk(first ? a.centre.x : b.centre.x, first ? a.centre.y : b.centre.y, first ? a.centre.z : b.centre.z);
-    n.radius = first ? a.radius : b.radius;
-    n.power = first ? a.power : b.power;
-    n.share = first ? a.share : b.share;
-    n.left = first ? a.left : b.left;
-    n.right = first ? a.right : b.right;
-    n.normals = first ? a.normals : b.normals;
-    return n;
-}
-// voxel_path.wgsl:132-149
+// voxel_path.wgsl:132-149. Between levels the walk keeps only what the
+// next level needs of the child it took: its children and its power share.
 __device__ __forceinline__ Emitter tree_emitter(const Optics &o, f3 position, f3 normal, float choice) {
     float uniform = choice;
     float probability = 1.0f;
-    WalkNode node = load_walk(o, 0u);
+    WalkNode root = load_walk(o, 0u);
+    uint32_t left_index = root.left, right_index = root.right;
+    float node_share = root.share;
     for (;;) {
-        if (node.left == NONE) {
-            Emitter e = load_emitter(o, node.right);
+        if (left_index == NONE) {
+            Emitter e = load_emitter(o, right_index);
             e.probability = probability;
             return e;
         }
-        WalkNode left = load_walk(o, node.left), right = load_walk(o, node.right);
-        float p = node_left_probability(node, left, right, position, normal);
+        WalkNode left = load_walk(o, left_index), right = load_walk(o, right_index);
+        float p = node_left_probability(node_share, left, right, position, normal);
+        // The branch taken rescales the choice: uniform / p on the left,
+        // (uniform - p) / (1 - p) on the right, as one division of the
+        // selected operands (the same binary32 operations, no divergence).
         bool go_left = uniform < p;
-        if (go_left) { probability = probability * p; uniform = uniform / p; }
-        else { probability = probability * (1.0f - p); uniform = (uniform - p) / (1.0f - p); }
-        uniform = fmn(uniform, __uint_as_float(0x3f7fffffu));
-        node = pick(go_left, left, right);
+        float share = go_left ? p : 1.0f - p;
+        float offset = go_left ? uniform : uniform - p;
+        probability = probability * share;
+        uniform = fmn(offset / share, __uint_as_float(0x3f7fffffu));
+        left_index = go_left ? left.left : right.left;
+        right_index = go_left ? left.right : right.right;
+        node_share = go_left ? left.share : right.share;
     }
 }
 // voxel_path.wgsl:158-174 (key_less :150-157); the split key (pivot voxel,
@@ -572,14 +589,16 @@ __device__ __forceinline__ float tree_probability(const Optics &o, f3 position, 
     }
     uint32_t index = 0u;
     float probability = 1.0f;
-    WalkNode node = load_walk(o, 0u);
+    WalkNode root = load_walk(o, 0u);
+    uint32_t left_index = root.left, right_index = root.right;
+    float node_share = root.share;
     for (;;) {
-        if (node.left == NONE) {
-            Emitter f = load_emitter(o, node.right);
+        if (left_index == NONE) {
+            Emitter f = load_emitter(o, right_index);
             return (f.vx == vx && f.vy == vy && f.vz == vz && f.face == face) ? probability : 0.0f;
         }
-        WalkNode left = load_walk(o, node.left), right = load_walk(o, node.right);
-        float p = node_left_probability(node, left, right, position, normal);
+        WalkNode left = load_walk(o, left_index), right = load_walk(o, right_index);
+        float p = node_left_probability(node_share, left, right, position, normal);
         uint32_t off = o.emitter_faces * 2u + index * 4u;
         uint4 key = o.emitters[off + 2u];
         uint32_t axis = o.emitters[off + 3u].z;
@@ -590,9 +609,12 @@ __device__ __forceinline__ float tree_probability(const Optics &o, f3 position, 
             uint32_t i = (axis + j) % 3u;
             if (v[i] != pv[i]) { less = v[i] < pv[i]; break; }
         }
-        if (less) { probability = probability * p; index = node.left; }
-        else { probability = probability * (1.0f - p); index = node.right; }
-        node = pick(less, left, right);
+        probability = probability * (less ? p : 1.0f - p);
+        index = less ? left_index : right_index;
+        node_share = less ? left.share : right.share;
+        uint32_t next_left = less ? left.left : right.left;
+        right_index = less ? left.right : right.right;
+        left_index = next_left;
     }
 }
 // voxel_path.wgsl:309-324
@@ -629,7 +651,7 @@ __device__ __forceinline__ f3 sample_emitter(const SceneView &s, const Optics &o
     float reach = connection_reach(o);
     if (d2 < 0.0001f || d2 > reach * reach) return zero;
     float distance = fsqrt(d2);
-    f3 direction = delta / distance;
+    f3 direction = div3(delta, distance);
     float receiver_cosine = fmx(dot3(normal, direction), 0.0f);
     float emitter_cosine = fmx(dot3(en, -direction), 0.0f);
     if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || e.probability <= 0.0f) return zero;
@@ -638,7 +660,7 @@ __device__ __forceinline__ f3 sample_emitter(const SceneView &s, const Optics &o
     f3 shadow_origin = position + normal * 0.002f;
     f3 shadow_delta = light_position + en * 0.002f - shadow_origin;
     float shadow_distance = len3(shadow_delta);
-    if (trace_ray(s, shadow_origin, shadow_delta / shadow_distance, shadow_distance, tally).status != 0u) return zero;
+    if (trace_ray(s, shadow_origin, div3(shadow_delta, shadow_distance), shadow_distance, tally).status != 0u) return zero;
     float4 em = o.materials[(uint32_t)e.material * 2u + 1u];
     float factor = directional_pdf / (light_pdf + directional_pdf);
     return mk(em.x, em.y, em.z) * factor;
@@ -662,7 +684,7 @@ __device__ __forceinline__ void direct_at(const SceneView &s, const Optics &o, f
             float d2 = dot3(delta, delta);
             if (d2 <= 0.0000001f || d2 > pr.w * pr.w) continue;
             distance = fsqrt(d2);
-            direction = delta / distance;
+            direction = div3(delta, distance);
             distance = distance - 0.004f;
             attenuation = 1.0f / fmx(d2, 1.0f);
         }
@@ -776,7 +798,7 @@ __device__ __forceinline__ bool emitter_candidate(const Optics &o, f3 position, 
     float reach = connection_reach(o);
     if (d2 < 0.0001f || d2 > reach * reach) return false;
     float dist = fsqrt(d2);
-    f3 dir = delta / dist;
+    f3 dir = div3(delta, dist);
     float receiver_cosine = fmx(dot3(normal, dir), 0.0f);
     float emitter_cosine = fmx(dot3(en, -dir), 0.0f);
     if (receiver_cosine <= 0.0f || emitter_cosine <= 0.0f || e.probability <= 0.0f) return false;
@@ -789,7 +811,7 @@ __device__ __forceinline__ bool emitter_candidate(const Optics &o, f3 position, 
     float factor = directional_pdf / (light_pdf + directional_pdf);
     candidate = mk(em.x, em.y, em.z) * factor;
     origin = shadow_origin;
-    direction = shadow_delta / shadow_distance;
+    direction = div3(shadow_delta, shadow_distance);
     distance = shadow_distance;
     return true;
 }
@@ -808,7 +830,7 @@ __device__ __forceinline__ bool light_ray(const Optics &o, uint32_t i, f3 positi
         float d2 = dot3(delta, delta);
         if (d2 <= 0.0000001f || d2 > pr.w * pr.w) return false;
         distance = fsqrt(d2);
-        direction = delta / distance;
+        direction = div3(delta, distance);
         distance = distance - 0.004f;
         attenuation = 1.0f / fmx(d2, 1.0f);
     }

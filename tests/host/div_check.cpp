@@ -1,5 +1,6 @@
 // The traversal's binary32 quotient from a binary64 reciprocal,
-// RN32(RN64(n * RN64(1/d))), against IEEE binary32 division RN32(n/d), on
+// RN32(RN64(n * r)) with r = RN64(1/d) or up to 3 binary64 ulps off it,
+// against IEEE binary32 division RN32(n/d), on
 // the host (SSE arithmetic is IEEE binary32/binary64, no contraction):
 // random operands over the traversal's ranges, operands next to each other
 // (quotients near 1), quotients next to binary32 midpoints, powers of two,
@@ -11,8 +12,11 @@
 #include <cstring>
 #include <random>
 
-static float via_reciprocal(float n, float d) {
+static float via_reciprocal(float n, float d, int ulps = 0) {
     volatile double r = 1.0 / (double)d;
+    // The device's reciprocal is an estimate refined by Newton steps, within
+    // an ulp or so of RN64(1/d): the quotient must not depend on which.
+    for (int k = 0; k < (ulps < 0 ? -ulps : ulps); ++k) r = std::nextafter((double)r, ulps < 0 ? -1e300 : 1e300);
     volatile double p = (double)n * r;
     return (float)p;
 }
@@ -22,12 +26,14 @@ static float val(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 static long checked = 0;
 static bool check(float n, float d) {
     volatile float q = n / d;
-    float r = via_reciprocal(n, d);
     ++checked;
-    if (bits(q) != bits(r)) {
-        std::printf("mismatch n=%.9g (%08x) d=%.9g (%08x): div %.9g (%08x) reciprocal %.9g (%08x)\n", n, bits(n), d, bits(d), (float)q,
-                    bits(q), r, bits(r));
-        return false;
+    for (int ulps = -3; ulps <= 3; ++ulps) {
+        float r = via_reciprocal(n, d, ulps);
+        if (bits(q) != bits(r)) {
+            std::printf("mismatch n=%.9g (%08x) d=%.9g (%08x): div %.9g (%08x) reciprocal %+d ulps %.9g (%08x)\n", n, bits(n), d,
+                        bits(d), (float)q, bits(q), ulps, r, bits(r));
+            return false;
+        }
     }
     return true;
 }

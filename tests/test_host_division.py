@@ -2,8 +2,9 @@
 reciprocal taken once per ray: RN32(RN64(n * RN64(1/d))) (vgi_trace.h
 div_by), and the light tree's walk divides a node's offset and radius by
 its distance the same way. Built alone with g++ (IEEE SSE arithmetic, no
-contraction) and checked bit for bit against binary32 division on random
-operands over the traversal's and the walk's ranges, quotients next to
+contraction) and checked bit for bit against binary32 division, with the
+reciprocal exact and up to 3 binary64 ulps off it (the device refines a
+reciprocal estimate by Newton steps), on random operands over the traversal's and the walk's ranges, quotients next to
 binary32 midpoints, neighbouring operands, powers of two, signed zeros and
 the still axis's divisor 1."""
 import os
