@@ -1114,7 +1114,10 @@ void launch_trace(const SceneView &scene, const Optics &optics, const uint4 *tas
                   bool full_counters, hipStream_t stream, const uint32_t *order) {
     uint64_t threads = (uint64_t)count * lanes;
     if (threads == 0) return;
-    dim3 grid(blocks(threads, 256)), block(256);
+    // One wavefront per workgroup: a workgroup's slots free as soon as its
+    // wavefront's longest path ends, not its slowest of four (512^3 cave,
+    // plain frame: 256 threads 53.2 ms, 128: 49.0, 64: 47.4).
+    dim3 grid(blocks(threads, 64)), block(64);
     float4 *no_l = nullptr;
     MlpQuery *no_q = nullptr;
     if (hits)
