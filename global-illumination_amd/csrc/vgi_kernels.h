@@ -5,7 +5,8 @@
 
 // Waves per SIMD the trace kernels are compiled for (their register budget:
 // 7 waves leave 72 VGPRs a lane, 8 leave 64). Measured on the 512^3 cave's
-// plain frame: 6 waves 58.4 ms, 7 waves 53.0 ms, 8 waves 56.9 ms.
+// plain frame with one-wavefront workgroups: 6 waves 51.1 ms, 7 waves 47.4
+// ms, 8 waves 51.7 ms (256-thread workgroups: 58.4, 53.0 and 56.9 ms).
 #ifndef VGI_TRACE_WAVES
 #define VGI_TRACE_WAVES 7
 #endif
